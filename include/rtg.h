@@ -1,0 +1,186 @@
+/*
+ * rtg.h - C ABI of the MI355X-native MIS path tracer (librtg.so).
+ *
+ * The drop-in boundary replaces the reference renderer's per-job CPU loop
+ *   Scene::compute -> JobManager::addJobs -> SampleJob::run -> PathTracingMIS::li
+ *   (RaytracerNew2/src/core/Scene.cpp:83-101, core/JobManager.cpp:103-232,
+ *    core/SampleJob.cpp:28-72, integrator/PathTracingMIS.cpp:27-263)
+ * at image/job granularity: the host describes a scene once (meshes, spheres,
+ * materials, lights), then asks for whole renders or submits render requests.
+ *
+ * Conventions: plain C types only; every call returns RTG_OK (0) or a negative
+ * error code and never throws; rtg_last_error() gives the message of the last
+ * failing call on a context. Host pointers unless a name says _device.
+ * Primitive ids: triangles of mesh 0, mesh 1, ... in index order, then spheres.
+ */
+#ifndef RTG_H
+#define RTG_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  RTG_OK = 0,
+  RTG_ERR_INVALID = -1,    /* bad argument or unsupported feature */
+  RTG_ERR_HIP = -2,        /* HIP runtime failure */
+  RTG_ERR_OOM = -3,        /* device memory exhausted / over budget */
+  RTG_ERR_PARSE = -4,      /* scene file could not be parsed */
+  RTG_ERR_NOT_READY = -5,  /* job still running (rtg_poll) */
+  RTG_ERR_CANCELLED = -6   /* job was cancelled */
+};
+
+/* Material types: bsdf/Diffuse.cpp, bsdf/RoughConductor.cpp, bsdf/RoughDielectric.cpp */
+enum { RTG_MAT_DIFFUSE = 0, RTG_MAT_ROUGH_CONDUCTOR = 1, RTG_MAT_ROUGH_DIELECTRIC = 2 };
+/* Microfacet distributions: bsdf/MicrofacetDistribution.h:8-12 */
+enum { RTG_DIST_BECKMANN = 0, RTG_DIST_GGX = 2 };
+/* Light selection strategies: integrator/Integrator.h:18-22 */
+enum { RTG_LIGHTS_ALL = 0, RTG_LIGHTS_UNIFORM = 1, RTG_LIGHTS_WEIGHTED = 2 };
+/* Samplers: sampler/RandomSampler.cpp, sampler/StratifiedSampler.cpp */
+enum { RTG_SAMPLER_RANDOM = 0, RTG_SAMPLER_STRATIFIED = 1 };
+
+/* Constant-texture material parameters (texture/ConstantTexture.h:6-37).
+ * albedo: Diffuse albedo, or reflectanceTexture of the rough BSDFs.
+ * alpha: roughness as written in the scene (the BSDF uses Color(alpha).average()).
+ * eta/k: conductor IOR and absorption (RoughConductor.cpp:24).
+ * eta_int/eta_ext: dielectric IORs (RoughDielectric.cpp:24-25).
+ * transmitted: RoughDielectric transmittedTexture. */
+typedef struct rtg_material {
+  int32_t type;
+  int32_t distribution;
+  float albedo[3];
+  float alpha;
+  float eta[3];
+  float k[3];
+  float eta_int;
+  float eta_ext;
+  float transmitted[3];
+} rtg_material;
+
+/* A triangle mesh (geometricShape/Mesh.h:19-58). emission != NULL makes the
+ * mesh a MeshLight with that radiance (light/MeshLight.cpp:32-64). */
+typedef struct rtg_mesh_desc {
+  const float* vertices;       /* num_vertices x 3 */
+  const float* normals;        /* num_vertices x 3 or NULL */
+  const float* uvs;            /* num_vertices x 2 or NULL */
+  int32_t num_vertices;
+  const int32_t* indices;      /* num_triangles x 3 */
+  int32_t num_triangles;
+  const int32_t* tri_material; /* per-triangle material ids or NULL */
+  int32_t material;            /* material when tri_material is NULL */
+  const float* emission;       /* 3 floats or NULL */
+} rtg_mesh_desc;
+
+/* geometricShape/Sphere.cpp:10-17: radius and toWorld (row-major 4x4). */
+typedef struct rtg_sphere_desc {
+  float to_world[16];
+  float radius;
+  int32_t material;
+} rtg_sphere_desc;
+
+/* Whole scene. Light order (for the selection CDF): emissive meshes in mesh
+ * order, then the environment light if any (light/EnvironmentLight.cpp). */
+typedef struct rtg_scene_desc {
+  const rtg_material* materials;
+  int32_t num_materials;
+  const rtg_mesh_desc* meshes;
+  int32_t num_meshes;
+  const rtg_sphere_desc* spheres;
+  int32_t num_spheres;
+  const float* env_rgb;        /* env_height x env_width x 3 (lat-long) or NULL */
+  int32_t env_width;
+  int32_t env_height;
+  float env_scale;
+  int32_t light_strategy;      /* RTG_LIGHTS_UNIFORM or RTG_LIGHTS_WEIGHTED */
+} rtg_scene_desc;
+
+/* Camera (camera/NoriPerspective.cpp:10-37 + core/Transform.cpp:199-227 lookAt),
+ * sampler and integrator parameters (integrator/PathTracingMIS.cpp:12-20). */
+typedef struct rtg_render_desc {
+  float origin[3];
+  float target[3];
+  float up[3];
+  float fov;
+  float near_clip;
+  float far_clip;
+  int32_t width;
+  int32_t height;
+  int32_t spp;
+  int32_t sampler;             /* RTG_SAMPLER_* */
+  int32_t dimension;           /* stratified dimensions (StratifiedSampler.cpp:24) */
+  int32_t min_depth;
+  int32_t max_depth;
+  uint64_t seed;
+  int32_t guiding;             /* 0 = plain MIS path tracing */
+} rtg_render_desc;
+
+typedef struct rtg_stats {
+  uint64_t samples;            /* camera paths finished */
+  uint64_t extension_rays;
+  uint64_t shadow_rays;
+  uint64_t nan_samples;        /* dropped as in core/Screen.cpp:46-50 */
+  uint64_t iterations;         /* wavefront iterations */
+  double milliseconds;         /* device time of the render */
+} rtg_stats;
+
+typedef struct rtg_ctx rtg_ctx;
+typedef struct rtg_scene rtg_scene;
+
+/* Context: one per GPU. Replaces JobManager's thread pool (core/JobManager.cpp:140-232). */
+int rtg_ctx_create(int device, rtg_ctx** out);
+int rtg_ctx_destroy(rtg_ctx* ctx);
+const char* rtg_last_error(rtg_ctx* ctx);
+/* Wavefront pool size (path slots). 0 keeps the autotuned default. */
+int rtg_ctx_set_pool(rtg_ctx* ctx, int32_t slots);
+
+/* Scene upload: builds the BVH on the host and makes the scene resident.
+ * Replaces Scene::initialize (core/Scene.cpp:115-138) + Bvh::build (accelStruct/Bvh.cpp:51-79). */
+int rtg_scene_upload(rtg_ctx* ctx, const rtg_scene_desc* desc, rtg_scene** out);
+/* Scene file in the reference's XML schema (core/XMLLoader.cpp:33-904): the
+ * hot-path types only. Fills *render from <camera>, <sampler>, <integrator>. */
+int rtg_scene_load_xml(rtg_ctx* ctx, const char* path, rtg_scene** out, rtg_render_desc* render);
+int rtg_scene_release(rtg_ctx* ctx, rtg_scene* scene);
+/* Device bytes held by a scene. */
+int rtg_scene_bytes(rtg_scene* scene, uint64_t* bytes);
+
+/* Blocking render. out_rgbw (width*height*4 floats, host) receives per-pixel
+ * sums of radiance and the filter weight (box filter r=0.5); divide rgb by w
+ * to get the image (core/Screen.cpp:113-122). */
+int rtg_render(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, float* out_rgbw, rtg_stats* stats);
+/* Same with a caller-owned device buffer (width*height*4 floats), not cleared. */
+int rtg_render_device(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, float* out_rgbw_device,
+                      rtg_stats* stats);
+
+/* Parity tap for Scene::computeIntersection (core/Scene.cpp:106-113, accelStruct/Bvh.cpp:392-481).
+ * rays: n x 8 floats (origin, direction, minT, maxT). prim: id or -1. t/uv/n may be NULL.
+ * uv: barycentrics (b1, b2) for triangles. p: hit point, nrm: shading normal (closest hit only). */
+int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int32_t any_hit, int32_t* prim,
+              float* t, float* uv, float* p, float* nrm);
+
+/* Parity tap for BSDF::eval/pdf/sample (bsdf/BSDF.h:70-74) on local directions.
+ * op 0: eval+pdf of (wi, wo) -> f (n x 3), pdf (n).
+ * op 1: sample from wi with u (n x 2) -> wo (n x 3), f = weight (n x 3), pdf (n), rel_eta (n). */
+int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const float* wi, const float* wo_or_u,
+                   int32_t n, float* out_dir, float* f, float* pdf, float* rel_eta);
+
+/* Continuous-batching serving (no reference counterpart). Requests of any
+ * resident scene are admitted as pool slots and HBM budget free up; their
+ * paths share the wavefront queues. */
+typedef struct rtg_server rtg_server;
+int rtg_server_create(rtg_ctx* ctx, uint64_t hbm_budget_bytes, rtg_server** out);
+int rtg_server_destroy(rtg_server* srv);
+/* Enqueue a request; returns its id in *job_id. Does not block. */
+int rtg_submit(rtg_server* srv, rtg_scene* scene, const rtg_render_desc* desc, int64_t* job_id);
+/* Run the device loop until no request is pending or max_ms elapsed (0 = until idle). */
+int rtg_server_step(rtg_server* srv, double max_ms);
+/* RTG_OK when finished (copies the image if out_rgbw != NULL), RTG_ERR_NOT_READY otherwise. */
+int rtg_poll(rtg_server* srv, int64_t job_id, float* out_rgbw, double* latency_ms);
+int rtg_cancel(rtg_server* srv, int64_t job_id);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RTG_H */

@@ -1,0 +1,205 @@
+// Host BVH builder: binned SAH (16 bins per axis over centroid bounds, all three axes),
+// leaves of up to 16 primitives, flattened into child-pair nodes of two 32-byte slots.
+// The tree topology is this library's own; the reference builds a different tree
+// (accelStruct/Bvh.cpp:51-79,611-771). First hits agree up to exact/near ties because
+// both return the closest primitive under the same FP32 intersection arithmetic.
+#include "rtg_bvh.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <stdexcept>
+
+namespace rtg {
+
+namespace {
+
+struct Bounds {
+  float mn[3], mx[3];
+  void reset() {
+    for (int i = 0; i < 3; i++) {
+      mn[i] = INFINITY;
+      mx[i] = -INFINITY;
+    }
+  }
+  void grow(const float* a, const float* b) {
+    for (int i = 0; i < 3; i++) {
+      mn[i] = std::min(mn[i], a[i]);
+      mx[i] = std::max(mx[i], b[i]);
+    }
+  }
+  void growp(const float* p) { grow(p, p); }
+  float area() const {
+    float d0 = mx[0] - mn[0], d1 = mx[1] - mn[1], d2 = mx[2] - mn[2];
+    if (d0 < 0.f || d1 < 0.f || d2 < 0.f) return 0.f;
+    return 2.f * (d0 * d1 + d1 * d2 + d0 * d2);
+  }
+};
+
+struct Builder {
+  const BuildPrim* prims;
+  std::vector<uint32_t> idx;
+  std::vector<float> nodes;  // 16 floats per interior node
+  uint32_t base;             // leaf-order offset of this primitive set
+  int maxDepth = 0;
+
+  uint32_t make_leaf(int begin, int end) { return make_leaf_ref(base + (uint32_t)begin, (uint32_t)(end - begin)); }
+
+  void bounds_of(int begin, int end, Bounds& b, Bounds& cb) const {
+    b.reset();
+    cb.reset();
+    for (int i = begin; i < end; i++) {
+      const BuildPrim& p = prims[idx[i]];
+      b.grow(p.bmin, p.bmax);
+      cb.growp(p.c);
+    }
+  }
+
+  // returns the ref of the subtree over idx[begin, end)
+  uint32_t build(int begin, int end, int depth) {
+    int n = end - begin;
+    Bounds b, cb;
+    bounds_of(begin, end, b, cb);
+    if (n <= 1) return make_leaf(begin, end);
+    const int NB = 16;
+    float bestCost = INFINITY;
+    int bestAxis = -1, bestBin = -1;
+    float parentArea = b.area();
+    for (int axis = 0; axis < 3; axis++) {
+      float lo = cb.mn[axis], hi = cb.mx[axis];
+      if (!(hi > lo)) continue;
+      float scale = NB / (hi - lo);
+      Bounds bb[NB];
+      int cnt[NB];
+      for (int k = 0; k < NB; k++) {
+        bb[k].reset();
+        cnt[k] = 0;
+      }
+      for (int i = begin; i < end; i++) {
+        const BuildPrim& p = prims[idx[i]];
+        int k = std::min(NB - 1, std::max(0, (int)((p.c[axis] - lo) * scale)));
+        bb[k].grow(p.bmin, p.bmax);
+        cnt[k]++;
+      }
+      float rightArea[NB];
+      int rightCnt[NB];
+      Bounds acc;
+      acc.reset();
+      int c = 0;
+      for (int k = NB - 1; k > 0; k--) {
+        if (cnt[k]) acc.grow(bb[k].mn, bb[k].mx);
+        c += cnt[k];
+        rightArea[k] = acc.area();
+        rightCnt[k] = c;
+      }
+      acc.reset();
+      c = 0;
+      for (int k = 0; k < NB - 1; k++) {
+        if (cnt[k]) acc.grow(bb[k].mn, bb[k].mx);
+        c += cnt[k];
+        if (c == 0 || rightCnt[k + 1] == 0) continue;
+        float cost = 1.0f + (c * acc.area() + rightCnt[k + 1] * rightArea[k + 1]) / std::max(parentArea, 1e-30f);
+        if (cost < bestCost) {
+          bestCost = cost;
+          bestAxis = axis;
+          bestBin = k;
+        }
+      }
+    }
+    float leafCost = (float)n;
+    if (n <= kMaxLeafPrims && (bestAxis < 0 || leafCost <= bestCost)) return make_leaf(begin, end);
+    int mid;
+    if (bestAxis < 0) {
+      // all centroids coincide: split by index
+      mid = begin + n / 2;
+    } else {
+      float lo = cb.mn[bestAxis], hi = cb.mx[bestAxis];
+      float scale = NB / (hi - lo);
+      auto it = std::partition(idx.begin() + begin, idx.begin() + end, [&](uint32_t id) {
+        const BuildPrim& p = prims[id];
+        int k = std::min(NB - 1, std::max(0, (int)((p.c[bestAxis] - lo) * scale)));
+        return k <= bestBin;
+      });
+      mid = (int)(it - idx.begin());
+      if (mid == begin || mid == end) mid = begin + n / 2;
+    }
+    uint32_t me = (uint32_t)(nodes.size() / 16);
+    nodes.resize(nodes.size() + 16, 0.f);
+    maxDepth = std::max(maxDepth, depth + 1);
+    uint32_t l = build(begin, mid, depth + 1);
+    uint32_t r = build(mid, end, depth + 1);
+    Bounds lb, rb, tmp;
+    bounds_of(begin, mid, lb, tmp);
+    bounds_of(mid, end, rb, tmp);
+    float* nd = &nodes[(size_t)me * 16];
+    write_slot(nd, lb, l);
+    write_slot(nd + 8, rb, r);
+    return me;
+  }
+  static void write_slot(float* s, const Bounds& b, uint32_t ref) {
+    s[0] = b.mn[0]; s[1] = b.mn[1]; s[2] = b.mn[2];
+    s[3] = b.mx[0]; s[4] = b.mx[1]; s[5] = b.mx[2];
+    std::memcpy(&s[6], &ref, 4);
+    s[7] = 0.f;
+  }
+};
+
+}  // namespace
+
+BvhResult build_bvh(const std::vector<BuildPrim>& tris, const std::vector<BuildPrim>& spheres) {
+  BvhResult res;
+  size_t total = tris.size() + spheres.size();
+  if (total >= (1u << 27)) throw std::runtime_error("too many primitives for the BVH reference encoding");
+  res.empty = total == 0;
+  Bounds all;
+  all.reset();
+  for (auto& p : tris) all.grow(p.bmin, p.bmax);
+  for (auto& p : spheres) all.grow(p.bmin, p.bmax);
+  for (int i = 0; i < 3; i++) {
+    res.root_min[i] = all.mn[i];
+    res.root_max[i] = all.mx[i];
+  }
+  if (res.empty) return res;
+  // one builder over a shared node array: triangles, then spheres
+  Builder bt;
+  bt.prims = tris.data();
+  bt.idx.resize(tris.size());
+  for (size_t i = 0; i < tris.size(); i++) bt.idx[i] = (uint32_t)i;
+  bt.base = 0;
+  bool both = !tris.empty() && !spheres.empty();
+  if (both) bt.nodes.resize(16, 0.f);  // node 0 joins the two subtrees
+  uint32_t triRef = 0, sphRef = 0;
+  if (!tris.empty()) triRef = bt.build(0, (int)tris.size(), both ? 1 : 0);
+  int depth = bt.maxDepth;
+  res.order_tri = bt.idx;
+  if (!spheres.empty()) {
+    Builder bs;
+    bs.prims = spheres.data();
+    bs.idx.resize(spheres.size());
+    for (size_t i = 0; i < spheres.size(); i++) bs.idx[i] = (uint32_t)i;
+    bs.base = (uint32_t)tris.size();
+    bs.nodes.swap(bt.nodes);
+    sphRef = bs.build(0, (int)spheres.size(), both ? 1 : 0);
+    bt.nodes.swap(bs.nodes);
+    depth = std::max(depth, bs.maxDepth);
+    res.order_sph = bs.idx;
+  }
+  if (both) {
+    Bounds tb, sb;
+    tb.reset();
+    sb.reset();
+    for (auto& p : tris) tb.grow(p.bmin, p.bmax);
+    for (auto& p : spheres) sb.grow(p.bmin, p.bmax);
+    Builder::write_slot(&bt.nodes[0], tb, triRef);
+    Builder::write_slot(&bt.nodes[8], sb, sphRef);
+    res.root_ref = 0;
+    depth = std::max(depth, 1);
+  } else {
+    res.root_ref = tris.empty() ? sphRef : triRef;
+  }
+  res.nodes.swap(bt.nodes);
+  res.max_depth = depth;
+  return res;
+}
+
+}  // namespace rtg

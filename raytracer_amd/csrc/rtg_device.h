@@ -1,0 +1,835 @@
+// Device-side building blocks of the wavefront path tracer (gfx950, wave64).
+// Arithmetic follows the reference's FP32 operation order, without FMA contraction
+// (the library is compiled with -ffp-contract=off), so BVH first hits and BSDF
+// eval/pdf match a CPU restatement bit for bit; sampling uses the device's
+// sin/cos/atan/acos/log, which differ from glibc by a few ulp.
+#pragma once
+#include "rtg_internal.h"
+
+namespace rtg {
+namespace dev {
+
+// core/BasicMath.h:16-21 (float branch)
+constexpr float PI = 3.14159265358979323846f;
+constexpr float INV_PI = 0.3183098861837906715377f;
+constexpr float MAX_REAL = 10e7f;
+constexpr float EPS = 1e-4f;
+constexpr float ONE_MINUS_EPS = 0.99999994f;
+// |det| < 1e-8 in Triangle.cpp:66 is a double comparison; for a float |det| it is
+// equivalent to |det| < the smallest float >= 1e-8.
+constexpr float DET_EPS = 0x1.5798f0p-27f;
+
+struct f3 {
+  float x, y, z;
+};
+__device__ __forceinline__ f3 mk(float x, float y, float z) { return f3{x, y, z}; }
+__device__ __forceinline__ f3 operator+(f3 a, f3 b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 operator-(f3 a, f3 b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 operator-(f3 a) { return mk(-a.x, -a.y, -a.z); }
+__device__ __forceinline__ f3 operator*(f3 a, float s) { return mk(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ f3 operator*(float s, f3 a) { return mk(s * a.x, s * a.y, s * a.z); }
+__device__ __forceinline__ f3 operator/(f3 a, float s) { return mk(a.x / s, a.y / s, a.z / s); }
+__device__ __forceinline__ f3 cmul(f3 a, f3 b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+__device__ __forceinline__ f3 cdiv(f3 a, f3 b) { return mk(a.x / b.x, a.y / b.y, a.z / b.z); }
+__device__ __forceinline__ f3 cadd(f3 a, float s) { return mk(a.x + s, a.y + s, a.z + s); }
+// 3-term reductions in Eigen's unrolled order a0 + (a1 + a2)
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + (a.y * b.y + a.z * b.z); }
+__device__ __forceinline__ f3 cross(f3 a, f3 b) {
+  return mk(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float sqnorm(f3 a) { return dot(a, a); }
+__device__ __forceinline__ float norm(f3 a) { return sqrtf(sqnorm(a)); }
+__device__ __forceinline__ f3 normalized(f3 a) {
+  float z = sqnorm(a);
+  if (z > 0.f) return a / sqrtf(z);
+  return a;
+}
+__device__ __forceinline__ f3 ld3(const float* p) { return mk(p[0], p[1], p[2]); }
+__device__ __forceinline__ f3 xyz(float4 v) { return mk(v.x, v.y, v.z); }
+__device__ __forceinline__ float luminance(f3 c) { return 0.2126f * c.x + 0.7152f * c.y + 0.0722f * c.z; }
+__device__ __forceinline__ bool is_zero(f3 c) {
+  return fabsf(c.x) <= 1e-5f && fabsf(c.y) <= 1e-5f && fabsf(c.z) <= 1e-5f;
+}
+__device__ __forceinline__ float average(float a) {  // Color(a).average(), Color.h:151-158
+  float s = 0.f;
+  s += a; s += a; s += a;
+  return s / 3.f;
+}
+// std::min/std::max argument semantics (NaN handling follows the C++ definitions)
+__device__ __forceinline__ float smin(float a, float b) { return (b < a) ? b : a; }
+__device__ __forceinline__ float smax(float a, float b) { return (a < b) ? b : a; }
+
+// ---------------------------------------------------------------- frames (DifferentialGeometry.h:36-66)
+struct Frame {
+  f3 s, t, n;
+};
+__device__ __forceinline__ Frame make_frame(f3 n) {
+  Frame f;
+  f.n = n;
+  f3 c;
+  if (fabsf(n.x) > fabsf(n.y)) {
+    float inv = 1.0f / sqrtf(n.x * n.x + n.z * n.z);
+    c = mk(n.z * inv, 0.0f, -n.x * inv);
+  } else {
+    float inv = 1.0f / sqrtf(n.y * n.y + n.z * n.z);
+    c = mk(0.0f, n.z * inv, -n.y * inv);
+  }
+  f.t = c;
+  f.s = cross(c, n);
+  return f;
+}
+__device__ __forceinline__ f3 to_local(const Frame& f, f3 v) { return mk(dot(v, f.s), dot(v, f.t), dot(v, f.n)); }
+__device__ __forceinline__ f3 to_world(const Frame& f, f3 v) { return f.s * v.x + f.t * v.y + f.n * v.z; }
+
+// ---------------------------------------------------------------- transforms (Transform.h:87-116,176-184)
+__device__ __forceinline__ f3 xform_point(const float4* M, f3 p) {
+  float4 r0 = M[0], r1 = M[1], r2 = M[2], r3 = M[3];
+  f3 o;
+  o.x = r0.x * p.x + r0.y * p.y + r0.z * p.z + r0.w;
+  o.y = r1.x * p.x + r1.y * p.y + r1.z * p.z + r1.w;
+  o.z = r2.x * p.x + r2.y * p.y + r2.z * p.z + r2.w;
+  float w = r3.x * p.x + r3.y * p.y + r3.z * p.z + r3.w;
+  if (w != 1.f && w != 0.f) o = o / w;
+  return o;
+}
+__device__ __forceinline__ f3 xform_vector(const float4* M, f3 v) {
+  float4 r0 = M[0], r1 = M[1], r2 = M[2];
+  return mk(r0.x * v.x + (r0.y * v.y + r0.z * v.z), r1.x * v.x + (r1.y * v.y + r1.z * v.z),
+            r2.x * v.x + (r2.y * v.y + r2.z * v.z));
+}
+__device__ __forceinline__ f3 xform_normal(const float4* Inv, f3 n) {
+  float4 r0 = Inv[0], r1 = Inv[1], r2 = Inv[2];
+  return mk(r0.x * n.x + (r1.x * n.y + r2.x * n.z), r0.y * n.x + (r1.y * n.y + r2.y * n.z),
+            r0.z * n.x + (r1.z * n.y + r2.z * n.z));
+}
+
+// ---------------------------------------------------------------- sampler
+// Counter-based streams: every (seed, pixel, sample, dimension) has its own value, so a path's
+// result does not depend on which slot, iteration or concurrent request runs it.
+__device__ __forceinline__ uint64_t mix64(uint64_t x) {  // SplitMix64 finaliser (Steele et al. 2014)
+  x += 0x9E3779B97F4A7C15ull;
+  x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+  x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+  return x ^ (x >> 31);
+}
+__device__ __forceinline__ uint32_t draw32(uint64_t key, uint32_t counter) {
+  return (uint32_t)(mix64(key + (uint64_t)counter * 0xD1B54A32D192ED03ull) >> 32);
+}
+// tools/Rng.h:132-141 bit trick: uniform float in [0,1) from the top 23 bits
+__device__ __forceinline__ float to_unit(uint32_t u) { return __uint_as_float((u >> 9) | 0x3f800000u) - 1.0f; }
+// Hash-based permutation of [0, l) (Kensler, "Correlated Multi-Jittered Sampling", 2013)
+__device__ __forceinline__ uint32_t permute(uint32_t i, uint32_t l, uint32_t p) {
+  uint32_t w = l - 1;
+  w |= w >> 1; w |= w >> 2; w |= w >> 4; w |= w >> 8; w |= w >> 16;
+  do {
+    i ^= p; i *= 0xe170893du;
+    i ^= p >> 16;
+    i ^= (i & w) >> 4;
+    i ^= p >> 8; i *= 0x0929eb3fu;
+    i ^= p >> 23;
+    i ^= (i & w) >> 1; i *= 1u | p >> 27;
+    i *= 0x6935fa69u;
+    i ^= (i & w) >> 11; i *= 0x74dcb303u;
+    i ^= (i & w) >> 2; i *= 0x9e501cc3u;
+    i ^= (i & w) >> 2; i *= 0xc860a3dfu;
+    i &= w;
+    i ^= i >> 5;
+  } while (i >= l);
+  return (i + p) % l;
+}
+struct PathRng {
+  uint64_t kpix;   // per (seed, pixel): stratum permutations shared by the pixel's samples
+  uint64_t kpath;  // per (seed, pixel, sample)
+  uint32_t sample, spp, sqrt_spp;
+  int32_t dimension;  // stratified dimensions (0 for the random sampler)
+};
+__device__ __forceinline__ PathRng make_rng(uint64_t seed, uint32_t pixel, uint32_t sample, const DevJob& J) {
+  PathRng r;
+  r.kpix = mix64(mix64(seed) ^ ((uint64_t)pixel * 0x9E3779B97F4A7C15ull));
+  r.kpath = mix64(r.kpix ^ ((uint64_t)(sample + 1u) * 0xC2B2AE3D27D4EB4Full));
+  r.sample = sample;
+  r.spp = (uint32_t)J.spp;
+  r.sqrt_spp = (uint32_t)J.sqrt_spp;
+  r.dimension = J.sampler == 1 ? J.dimension : 0;
+  return r;
+}
+// 2D draw number k: stratified over the pixel's samples for k < dimension
+// (sampler/StratifiedSampler.cpp:145-151,174-215), independent uniform beyond.
+__device__ __forceinline__ void sample2d(const PathRng& r, uint32_t k, float& x, float& y) {
+  if ((int32_t)k < r.dimension) {
+    uint32_t j = permute(r.sample, r.spp, draw32(r.kpix, 2u * k));
+    uint32_t sx = j / r.sqrt_spp, sy = j % r.sqrt_spp;
+    float jx = to_unit(draw32(r.kpath, 0x40000000u + 2u * k));
+    float jy = to_unit(draw32(r.kpath, 0x40000001u + 2u * k));
+    float dx = 1.f / (float)r.sqrt_spp;
+    x = smin(((float)sx + jx) * dx, ONE_MINUS_EPS);
+    y = smin(((float)sy + jy) * dx, ONE_MINUS_EPS);
+  } else {
+    x = to_unit(draw32(r.kpath, 2u * k));
+    y = to_unit(draw32(r.kpath, 2u * k + 1u));
+  }
+}
+__device__ __forceinline__ float sample1d(const PathRng& r, uint32_t k) {
+  if ((int32_t)k < r.dimension) {
+    uint32_t j = permute(r.sample, r.spp, draw32(r.kpix, 0x20000001u + 2u * k));
+    float u = to_unit(draw32(r.kpath, 0x60000000u + k));
+    float dx = 1.f / (float)r.spp;
+    return smin(((float)j + u) * dx, ONE_MINUS_EPS);
+  }
+  return to_unit(draw32(r.kpath, 0x80000000u + k));
+}
+
+// ---------------------------------------------------------------- wave primitives
+// Wave-aggregated append: one atomic per wave, lanes get consecutive positions.
+__device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
+  unsigned long long mask = __ballot(pred);
+  if (mask == 0ull) return 0xffffffffu;
+  uint32_t lane = __lane_id();
+  uint32_t leader = (uint32_t)__builtin_ctzll(mask);
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+  base = __shfl(base, (int)leader);
+  return pred ? base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)) : 0xffffffffu;
+}
+__device__ __forceinline__ void wave_count64(unsigned long long* counter, bool pred) {
+  unsigned long long mask = __ballot(pred);
+  if (mask == 0ull) return;
+  uint32_t lane = __lane_id();
+  if (lane == (uint32_t)__builtin_ctzll(mask)) atomicAdd(counter, (unsigned long long)__popcll(mask));
+}
+
+// ---------------------------------------------------------------- ray / box / primitives
+struct Ray {
+  f3 o, d, inv;
+  float tmin, tmax;
+};
+// accelStruct/Bvh.cpp:411-413 adaptive epsilon
+__device__ __forceinline__ void adapt_eps(Ray& r) {
+  if (r.tmin == EPS) r.tmin = smax(r.tmin, r.tmin * smax(smax(fabsf(r.o.x), fabsf(r.o.y)), fabsf(r.o.z)));
+}
+// core/BoundingBox.h:154-183
+__device__ __forceinline__ bool slab(const Ray& r, float bx0, float by0, float bz0, float bx1, float by1,
+                                     float bz1, float& nearT) {
+  float nT = -MAX_REAL, fT = MAX_REAL;
+  const float o[3] = {r.o.x, r.o.y, r.o.z};
+  const float d[3] = {r.d.x, r.d.y, r.d.z};
+  const float iv[3] = {r.inv.x, r.inv.y, r.inv.z};
+  const float mn[3] = {bx0, by0, bz0};
+  const float mx[3] = {bx1, by1, bz1};
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    if (d[i] == 0.f) {
+      if (o[i] < mn[i] || o[i] > mx[i]) return false;
+    } else {
+      float t1 = (mn[i] - o[i]) * iv[i];
+      float t2 = (mx[i] - o[i]) * iv[i];
+      if (t1 > t2) {
+        float tmp = t1; t1 = t2; t2 = tmp;
+      }
+      nT = smax(t1, nT);
+      fT = smin(t2, fT);
+      if (!(nT <= fT)) return false;
+    }
+  }
+  nearT = nT;
+  return (r.tmin <= fT && r.tmax >= nT);
+}
+// geometricShape/Triangle.cpp:46-89
+__device__ __forceinline__ bool tri_hit(const Ray& r, const float* v, float& t, float& b1, float& b2) {
+  f3 p0 = mk(v[0], v[1], v[2]);
+  f3 p1 = mk(v[3], v[4], v[5]);
+  f3 p2 = mk(v[6], v[7], v[8]);
+  f3 e1 = p1 - p0;
+  f3 e2 = p2 - p0;
+  f3 s = r.o - p0;
+  f3 s1 = cross(r.d, e2);
+  f3 s2 = cross(s, e1);
+  float det = dot(s1, e1);
+  if (fabsf(det) < DET_EPS) return false;
+  float inv = 1.f / det;
+  float u = inv * dot(s1, s);
+  if (u < 0.f || u > 1.f) return false;
+  float w = inv * dot(s2, r.d);
+  if (w < 0.f || u + w > 1.f) return false;
+  float tt = inv * dot(s2, e2);
+  if (tt < r.tmin || tt > r.tmax) return false;
+  t = tt; b1 = u; b2 = w;
+  return true;
+}
+// geometricShape/Sphere.cpp:38-111 (sph: inverse rows 0..3, then radius)
+__device__ __forceinline__ bool sphere_hit(const Ray& r, const float4* sph, float& t) {
+  f3 lo = xform_point(sph, r.o);
+  f3 ld = xform_vector(sph, r.d);
+  float radius = sph[4].x;
+  float a = 1.f;
+  float b = 2.f * dot(ld, lo);
+  float c = dot(lo, lo) - radius * radius;
+  float delta = b * b - 4.f * a * c;
+  if (delta < 0.f) return false;
+  float t1 = (-b + sqrtf(delta)) / (2.f * a);
+  float t2 = (-b - sqrtf(delta)) / (2.f * a);
+  float tn = t1, tf = t2;
+  if (tn > tf) { float x = tn; tn = tf; tf = x; }
+  if (tn <= r.tmin) {
+    if (tf > r.tmin && tf < r.tmax) { t = tf; return true; }
+  } else if (tn > r.tmin && tn < r.tmax) {
+    t = tn; return true;
+  }
+  return false;
+}
+
+struct HitRec {
+  float t, b1, b2;
+  int32_t prim;  // leaf-order primitive index, -1 on miss
+};
+
+// BVH traversal over child-pair nodes; stack in LDS ([depth][blockDim] layout, no bank conflicts).
+// Closest hit keeps the reference's acceptance rules (triangles accept t == maxT, the last
+// accepted primitive wins); ANY returns at the first accepted primitive.
+template <bool ANY>
+__device__ __forceinline__ bool traverse(const DevScene& S, Ray r, uint32_t* stk_ref, float* stk_t, int stride,
+                                         int cap, HitRec& h, uint32_t& nodes_visited) {
+  h.prim = -1;
+  adapt_eps(r);
+  if (S.empty || r.tmax < r.tmin) return false;
+  float tn;
+  if (!slab(r, S.root_min[0], S.root_min[1], S.root_min[2], S.root_max[0], S.root_max[1], S.root_max[2], tn))
+    return false;
+  uint32_t ref = S.root_ref;
+  int sp = 0;
+  bool hit = false;
+  while (true) {
+    if (!(ref & kLeafBit)) {
+      const float4* nd = S.nodes + 4u * ref;
+      float4 a0 = nd[0], a1 = nd[1], c0 = nd[2], c1 = nd[3];
+      nodes_visited++;
+      float t0, t1;
+      bool h0 = slab(r, a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, t0);
+      bool h1 = slab(r, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, t1);
+      uint32_t r0 = __float_as_uint(a1.z), r1 = __float_as_uint(c1.z);
+      if (h0 && h1) {
+        uint32_t nearR = r0, farR = r1;
+        float farT = t1;
+        if (t1 < t0) { nearR = r1; farR = r0; farT = t0; }
+        if (sp < cap) {  // the host sizes cap above the tree height, so this always holds
+          stk_ref[sp * stride] = farR;
+          stk_t[sp * stride] = farT;
+          sp++;
+        }
+        ref = nearR;
+        continue;
+      } else if (h0) {
+        ref = r0;
+        continue;
+      } else if (h1) {
+        ref = r1;
+        continue;
+      }
+    } else {
+      uint32_t first = leaf_first(ref), cnt = leaf_count(ref);
+      if ((int32_t)first < S.ntri) {
+        for (uint32_t i = 0; i < cnt; i++) {
+          float t, b1, b2;
+          if (tri_hit(r, S.tri_v + 9u * (first + i), t, b1, b2)) {
+            h.t = t; h.b1 = b1; h.b2 = b2; h.prim = (int32_t)(first + i);
+            if (ANY) return true;
+            r.tmax = t;
+            hit = true;
+          }
+        }
+      } else {
+        for (uint32_t i = 0; i < cnt; i++) {
+          float t;
+          uint32_t si = first + i - (uint32_t)S.ntri;
+          if (sphere_hit(r, S.spheres + 5u * si, t)) {
+            h.t = t; h.b1 = 0.f; h.b2 = 0.f; h.prim = (int32_t)(first + i);
+            if (ANY) return true;
+            r.tmax = t;
+            hit = true;
+          }
+        }
+      }
+    }
+    // pop; a popped subtree whose box starts beyond the current maxT fails its box test
+    while (true) {
+      if (sp == 0) return hit;
+      sp--;
+      float pt = stk_t[sp * stride];
+      if (r.tmax >= pt) {
+        ref = stk_ref[sp * stride];
+        break;
+      }
+    }
+  }
+}
+
+// Surface data at a hit (Triangle.cpp:130-215, Sphere.cpp:115-139, Primitive.cpp:28)
+struct Surface {
+  f3 p;
+  Frame sh;
+  uint32_t material;
+  uint32_t light;
+};
+__device__ __forceinline__ Surface surface_at(const DevScene& S, const HitRec& h, f3 ro, f3 rd) {
+  Surface s;
+  uint4 info = S.prim_info[h.prim];
+  s.material = info.y;
+  s.light = info.z;
+  if (h.prim < S.ntri) {
+    const float* v = S.tri_v + 9u * (uint32_t)h.prim;
+    f3 p0 = mk(v[0], v[1], v[2]), p1 = mk(v[3], v[4], v[5]), p2 = mk(v[6], v[7], v[8]);
+    float b0 = 1.f - (h.b1 + h.b2);
+    s.p = b0 * p0 + h.b1 * p1 + h.b2 * p2;
+    f3 n;
+    if (S.tri_has_n && S.tri_has_n[h.prim]) {
+      const int32_t* vi = S.tri_vidx + 3u * (uint32_t)h.prim;
+      f3 n0 = ld3(S.vnormals + 3 * vi[0]), n1 = ld3(S.vnormals + 3 * vi[1]), n2 = ld3(S.vnormals + 3 * vi[2]);
+      n = normalized(b0 * n0 + h.b1 * n1 + h.b2 * n2);
+    } else {
+      n = normalized(cross(p1 - p0, p2 - p0));
+    }
+    s.sh = make_frame(n);
+  } else {
+    const float4* sph = S.spheres + 5u * (uint32_t)(h.prim - S.ntri);
+    s.p = ro + h.t * rd;
+    f3 local = xform_point(sph, s.p);
+    f3 nl = normalized(local);
+    s.sh = make_frame(normalized(xform_normal(sph, nl)));
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------- microfacets (MicrofacetDistribution.cpp)
+__device__ __forceinline__ float tan2(f3 v) {
+  float s2 = 1.f - v.z * v.z;
+  if (s2 <= 0.f) return 0.f;
+  return s2 / (v.z * v.z);
+}
+__device__ __forceinline__ f3 mf_sample(int dist, float ux, float uy, float alpha) {
+  float theta;
+  if (dist == 0)
+    theta = atanf(sqrtf(-alpha * alpha * logf(1.f - ux)));
+  else
+    theta = atanf(alpha * sqrtf(ux) / sqrtf(1.f - ux));
+  float phi = 2.f * PI * uy;
+  float st = sinf(theta);
+  return mk(st * cosf(phi), st * sinf(phi), cosf(theta));
+}
+__device__ __forceinline__ float mf_D(int dist, f3 wh, float alpha) {
+  if (wh.z <= 0.f) return 0.f;
+  if (dist == 0) {
+    float c2 = wh.z * wh.z;
+    float t2 = (1.f - wh.z * wh.z) / c2;
+    float a2 = alpha * alpha;
+    return expf(-t2 / a2) / (PI * a2 * c2 * c2);
+  }
+  float a2 = alpha * alpha;
+  float c2 = wh.z * wh.z;
+  float t2 = tan2(wh);
+  float ss = a2 + t2;
+  return a2 / (PI * c2 * c2 * ss * ss);
+}
+__device__ __forceinline__ float mf_G1(int dist, f3 v, f3 m, float alpha) {
+  if (dot(v, m) / v.z <= 0.f) return 0.f;
+  if (dist == 0) {
+    float s2 = 1.f - v.z * v.z;
+    float st = s2 < 0.f ? 0.f : sqrtf(s2);
+    float tv = st / v.z;
+    if (tv == 0.f) return 1.f;
+    float b = fabsf(1.f / (alpha * tv));
+    if (b < 1.6f) return (3.535f * b + 2.181f * b * b) / (1.f + 2.276f * b + 2.577f * b * b);
+    return 1.f;
+  }
+  float a2 = alpha * alpha;
+  float t2 = tan2(v);
+  return 2.f / (1.f + sqrtf(1.f + a2 * t2));
+}
+__device__ __forceinline__ float mf_G(int dist, f3 wi, f3 wo, f3 wh, float alpha) {
+  return mf_G1(dist, wi, wh, alpha) * mf_G1(dist, wo, wh, alpha);
+}
+__device__ __forceinline__ float adj_alpha(float alpha, float cosI) {
+  return (1.2f - 0.2f * sqrtf(fabsf(cosI))) * alpha;
+}
+
+// ---------------------------------------------------------------- Fresnel (core/Fresnel.h)
+__device__ __forceinline__ f3 fresnel_conductor(f3 eta, f3 k, float cosI) {
+  f3 epa = cmul(eta, eta) + cmul(k, k);
+  float c2 = cosI * cosI;
+  f3 epac2 = epa * c2;
+  f3 e2c = (2.f * eta) * cosI;
+  f3 rpar = cdiv(cadd(epac2 - e2c, 1.f), cadd(epac2 + e2c, 1.f));
+  f3 rperp = cdiv(cadd(epa - e2c, c2), cadd(epa + e2c, c2));
+  return (cmul(rpar, rpar) + cmul(rperp, rperp)) * 0.5f;
+}
+__device__ __forceinline__ float fresnel_dielectric(float etaI, float etaT, float cosI, float cosT) {
+  float a = etaT * cosI, b = etaI * cosT, c = etaI * cosI, d = etaT * cosT;
+  float rpar = (a - b) / (a + b);
+  float rperp = (c - d) / (c + d);
+  return (rpar * rpar + rperp * rperp) * 0.5f;
+}
+__device__ __forceinline__ float est_dielectric(float etaExt, float etaInt, float cosI, float& etaI, float& etaT,
+                                                float& rel, float& cosT) {
+  etaI = etaExt;
+  etaT = etaInt;
+  bool entering = true;
+  float fr;
+  if (cosI <= 0.f) {
+    entering = false;
+    float x = etaI; etaI = etaT; etaT = x;
+  }
+  rel = etaI / etaT;
+  float sinT = rel * sqrtf(smax(0.f, 1.f - cosI * cosI));
+  cosT = sqrtf(smax(0.f, 1.f - sinT * sinT));
+  if (sinT >= 1.f) {
+    cosT = 0.f;
+    fr = 1.f;
+  } else {
+    fr = fresnel_dielectric(etaI, etaT, fabsf(cosI), cosT);
+    if (entering) cosT = -cosT;
+  }
+  return fr;
+}
+__device__ __forceinline__ float est_dielectric(float etaExt, float etaInt, float cosI) {
+  float a, b, c, d;
+  return est_dielectric(etaExt, etaInt, cosI, a, b, c, d);
+}
+
+// ---------------------------------------------------------------- BSDFs
+struct BsdfSample {
+  f3 wo;
+  f3 f;        // f * |cos| / pdf
+  float pdf;
+  float rel_eta;
+};
+__device__ __forceinline__ f3 mat_col(const float* c) { return mk(c[0], c[1], c[2]); }
+__device__ __forceinline__ f3 reflect_n(f3 v, f3 n) { return (2.f * dot(v, n)) * n - v; }
+__device__ __forceinline__ float sgn(float v) { return v >= 0.f ? 1.f : -1.f; }
+
+// bsdf/Diffuse.cpp:28-106
+__device__ __forceinline__ f3 diffuse_eval(const DevMaterial& m, f3 wi, f3 wo) {
+  if (wi.z <= 0.f || wo.z <= 0.f) return mk(0.f, 0.f, 0.f);
+  return mat_col(m.albedo) * INV_PI;
+}
+__device__ __forceinline__ float diffuse_pdf(f3 wi, f3 wo) {
+  if (wi.z <= 0.f || wo.z <= 0.f) return 0.f;
+  return INV_PI * wo.z;
+}
+__device__ __forceinline__ f3 concentric_disk_cosine(float ux, float uy) {  // Mapping.h:62-67,110-140
+  const float r1 = 2.f * ux - 1.f;
+  const float r2 = 2.f * uy - 1.f;
+  float rx, ry;
+  if (r1 > -r2 && r1 > r2) {
+    rx = r1; ry = (PI * r2) / (4.f * r1);
+  } else if (r1 < r2 && r1 > -r2) {
+    rx = r2; ry = (PI / 4.f) * (2.f - r1 / r2);
+  } else if (r1 < -r2 && r1 < r2) {
+    rx = -r1; ry = (PI / 4.f) * (4.f + r2 / r1);
+  } else {
+    rx = -r2; ry = (PI / 4.f) * (6.f - r1 / r2);
+  }
+  float dx = rx * cosf(ry), dy = rx * sinf(ry);
+  float z = sqrtf(smax(0.f, 1.f - dx * dx - dy * dy));
+  return mk(dx, dy, z);
+}
+__device__ __forceinline__ bool diffuse_sample(const DevMaterial& m, f3 wi, float ux, float uy, BsdfSample& bs) {
+  if (wi.z <= 0.f) return false;
+  bs.wo = concentric_disk_cosine(ux, uy);
+  bs.pdf = diffuse_pdf(wi, bs.wo);
+  bs.rel_eta = 1.f;
+  bs.f = mat_col(m.albedo);
+  return true;
+}
+
+// bsdf/RoughConductor.cpp:77-145
+__device__ __forceinline__ f3 conductor_eval(const DevMaterial& m, f3 wi, f3 wo) {
+  float ci = wi.z, co = wo.z;
+  if (ci <= 0.f || co <= 0.f) return mk(0.f, 0.f, 0.f);
+  f3 wh = normalized(wi + wo);
+  float D = mf_D(m.dist, wh, m.alpha);
+  float G = mf_G(m.dist, wi, wo, wh, m.alpha);
+  f3 F = fresnel_conductor(mat_col(m.eta), mat_col(m.k), ci);
+  return cmul(mat_col(m.albedo) * D, F) * G / (4.f * ci * co);
+}
+__device__ __forceinline__ float conductor_pdf(const DevMaterial& m, f3 wi, f3 wo) {
+  float ci = wi.z, co = wo.z;
+  if (ci <= 0.f || co <= 0.f) return 0.f;
+  f3 wh = normalized(wi + wo);
+  float Jh = 1.f / (4.f * fabsf(dot(wh, wo)));
+  float D = mf_D(m.dist, wh, m.alpha);
+  return D * wh.z * Jh;
+}
+__device__ __forceinline__ bool conductor_sample(const DevMaterial& m, f3 wi, float ux, float uy, BsdfSample& bs) {
+  if (wi.z <= 0.f) return false;
+  float a = adj_alpha(m.alpha, wi.z);
+  f3 n = mf_sample(m.dist, ux, uy, a);
+  bs.wo = reflect_n(wi, n);
+  bs.rel_eta = 1.f;
+  float co = bs.wo.z;
+  if (co <= 0.f) return false;
+  bs.pdf = conductor_pdf(m, wi, bs.wo);
+  if (bs.pdf <= 0.f) return false;
+  bs.f = conductor_eval(m, wi, bs.wo) * fabsf(co) / bs.pdf;
+  return true;
+}
+
+// bsdf/RoughDielectric.cpp:83-329
+__device__ __forceinline__ f3 half_t(f3 wi, f3 wo, float etaI, float etaT) {
+  return normalized(-1.f * (etaI * wi + etaT * wo));
+}
+__device__ __forceinline__ f3 diel_eval_refl(const DevMaterial& m, f3 wi, f3 wo) {
+  float ci = wi.z, co = wo.z;
+  f3 wh = normalized(wi + wo);
+  wh = sgn(ci) * wh;
+  float F = est_dielectric(m.eta_ext, m.eta_int, dot(wi, wh));
+  f3 r = mat_col(m.albedo) * mf_D(m.dist, wh, m.alpha);
+  r = r * F;
+  return r * mf_G(m.dist, wi, wo, wh, m.alpha) / (4.f * ci * co);
+}
+__device__ __forceinline__ f3 diel_eval_refr(const DevMaterial& m, f3 wi, f3 wo) {
+  float ci = wi.z, co = wo.z;
+  float etaI, etaT, rel, cosT;
+  float fr = est_dielectric(m.eta_ext, m.eta_int, ci, etaI, etaT, rel, cosT);
+  f3 wt = half_t(wi, wo, etaI, etaT);
+  float left = dot(wi, wt) * dot(wo, wt) / (ci * co);
+  f3 term = mat_col(m.trans) * left * etaT * etaT;
+  term = term * ((-1.f * fr) + 1.f);
+  term = term * mf_G(m.dist, wi, wo, wt, m.alpha) * mf_D(m.dist, wt, m.alpha) * rel * rel;
+  float denom = (etaI * (dot(wi, wt)) + etaT * dot(wo, wt));
+  return term / (denom * denom);
+}
+__device__ __forceinline__ f3 dielectric_eval(const DevMaterial& m, f3 wi, f3 wo) {
+  return (wi.z * wo.z > 0.f) ? diel_eval_refl(m, wi, wo) : diel_eval_refr(m, wi, wo);
+}
+__device__ __forceinline__ float diel_pdf_refr(const DevMaterial& m, f3 wi, f3 wo) {
+  float ci = wi.z;
+  float etaI, etaT, rel, cosT;
+  est_dielectric(m.eta_ext, m.eta_int, ci, etaI, etaT, rel, cosT);
+  f3 wt = half_t(wi, wo, etaI, etaT);
+  float a = (1.2f - 0.2f * sqrtf(fabsf(ci))) * m.alpha;
+  float hd = (etaI * dot(wi, wt) + etaT * dot(wo, wt));
+  float Jh = (etaT * etaT * dot(wo, wt)) / (hd * hd);
+  return fabsf(mf_D(m.dist, wt, a) * wt.z * Jh);
+}
+__device__ __forceinline__ float diel_pdf_refl(const DevMaterial& m, f3 wi, f3 wo) {
+  float ci = wi.z;
+  float a = (1.2f - 0.2f * sqrtf(fabsf(ci))) * m.alpha;
+  f3 wh = normalized(wi + wo);
+  wh = sgn(ci) * wh;
+  float Jh = 1.f / (4.f * dot(wh, wo));
+  return fabsf(mf_D(m.dist, wh, a) * wh.z * Jh);
+}
+__device__ __forceinline__ float dielectric_pdf(const DevMaterial& m, f3 wi, f3 wo) {
+  float ci = wi.z, co = wo.z;
+  if (ci * co > 0.f) {
+    f3 wh = normalized(wi + wo);
+    wh = sgn(ci) * wh;
+    float fr = average(est_dielectric(m.eta_ext, m.eta_int, dot(wi, wh)));
+    return diel_pdf_refl(m, wi, wo) * fr;
+  }
+  float etaI = m.eta_ext, etaT = m.eta_int;
+  if (ci < 0.f) { float x = etaI; etaI = etaT; etaT = x; }
+  f3 wt = half_t(wi, wo, etaI, etaT);
+  float fr = average(est_dielectric(m.eta_ext, m.eta_int, dot(wi, wt)));
+  return diel_pdf_refr(m, wi, wo) * (1.f - fr);
+}
+__device__ __forceinline__ bool dielectric_sample(const DevMaterial& m, f3 wi, float ux, float uy, BsdfSample& bs) {
+  float ci = wi.z;
+  float a = adj_alpha(m.alpha, ci);
+  f3 n = mf_sample(m.dist, ux, uy, a);
+  float etaI, etaT, rel, cosT;
+  float fr = average(est_dielectric(m.eta_ext, m.eta_int, dot(wi, n), etaI, etaT, rel, cosT));
+  bs.rel_eta = 1.f;
+  if (ux <= fr) {
+    bs.wo = reflect_n(wi, n);
+    if (wi.z * bs.wo.z <= 0.f) return false;
+    bs.pdf = diel_pdf_refl(m, wi, bs.wo) * fr;
+    if (bs.pdf <= 0.f) return false;
+    f3 wh = normalized(wi + bs.wo);
+    wh = sgn(ci) * wh;
+    float w = 1.f * dot(wi, n);
+    w = w * mf_G(m.dist, wi, bs.wo, wh, a);
+    w = w / (ci * n.z);
+    bs.f = mk(w, w, w);
+    return true;
+  }
+  bs.wo = n * (dot(wi, n) * rel + cosT) - wi * rel;
+  bs.wo = normalized(bs.wo);
+  bs.rel_eta = rel;
+  float co = dot(bs.wo, n);
+  if (wi.z * bs.wo.z >= 0.f) return false;
+  bs.pdf = diel_pdf_refr(m, wi, bs.wo) * (1.f - fr);
+  if (bs.pdf <= 0.f) return false;
+  bs.f = diel_eval_refr(m, wi, bs.wo) * fabsf(co) / bs.pdf;
+  return true;
+}
+
+template <int MT>
+__device__ __forceinline__ f3 bsdf_eval(const DevMaterial& m, f3 wi, f3 wo) {
+  if (MT == 0) return diffuse_eval(m, wi, wo);
+  if (MT == 1) return conductor_eval(m, wi, wo);
+  return dielectric_eval(m, wi, wo);
+}
+template <int MT>
+__device__ __forceinline__ float bsdf_pdf(const DevMaterial& m, f3 wi, f3 wo) {
+  if (MT == 0) return diffuse_pdf(wi, wo);
+  if (MT == 1) return conductor_pdf(m, wi, wo);
+  return dielectric_pdf(m, wi, wo);
+}
+template <int MT>
+__device__ __forceinline__ bool bsdf_sample(const DevMaterial& m, f3 wi, float ux, float uy, BsdfSample& bs) {
+  if (MT == 0) return diffuse_sample(m, wi, ux, uy, bs);
+  if (MT == 1) return conductor_sample(m, wi, ux, uy, bs);
+  return dielectric_sample(m, wi, ux, uy, bs);
+}
+
+// ---------------------------------------------------------------- CDF (core/CDF.h:44-81)
+__device__ __forceinline__ int cdf_sample(const float* c, int n /* entries incl. leading 0 */, float s) {
+  int lo = 0, hi = n;  // lower_bound
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (c[mid] < s) lo = mid + 1; else hi = mid;
+  }
+  int index = lo - 1;
+  if (index < 0) index = 0;
+  if (index > n - 2) index = n - 2;
+  return index;
+}
+__device__ __forceinline__ int cdf_sample_reuse(const float* c, int n, float& s, float& pdf) {
+  int i = cdf_sample(c, n, s);
+  s = (s - c[i]) / (c[i + 1] - c[i]);
+  pdf = c[i + 1] - c[i];
+  return i;
+}
+
+// ---------------------------------------------------------------- lights
+struct LightSample {
+  f3 dir;
+  float dist;
+  f3 intensity;
+  float pdf;
+};
+// light/MeshLight.cpp:32-64 with geometricShape/Mesh.cpp:38-68, Mapping.h:94-105
+__device__ __forceinline__ LightSample mesh_light_sample(const DevScene& S, const DevLight& L, f3 pFrom, float ux,
+                                                         float uy) {
+  LightSample ls;
+  float dummy;
+  float sx = ux;
+  int idx = cdf_sample_reuse(S.emit_cdf + L.cdf_offset, L.emit_count + 1, sx, dummy);
+  const float* v = S.emit_tri + 9u * (uint32_t)(L.emit_offset + idx);
+  f3 p0 = mk(v[0], v[1], v[2]), p1 = mk(v[3], v[4], v[5]), p2 = mk(v[6], v[7], v[8]);
+  float sq = sqrtf(sx);
+  float al = 1.f - sq, be = (1.f - uy) * sq, ga = uy * sq;
+  f3 pt = p0 * al + p1 * be + p2 * ga;
+  f3 n;
+  if (L.has_normals) {
+    const float* nn = S.emit_nrm + 9u * (uint32_t)(L.emit_offset + idx);
+    n = normalized(al * mk(nn[0], nn[1], nn[2]) + be * mk(nn[3], nn[4], nn[5]) + ga * mk(nn[6], nn[7], nn[8]));
+  } else {
+    n = normalized(cross(p1 - p0, p2 - p0));
+    n = normalized(n);
+  }
+  ls.dir = pt - pFrom;
+  ls.dist = norm(ls.dir);
+  ls.dir = ls.dir / ls.dist;
+  float cosine = dot(n, -ls.dir);
+  if (cosine <= 0.f) {
+    ls.intensity = mk(0.f, 0.f, 0.f);
+    ls.pdf = 0.f;
+  } else {
+    ls.intensity = mat_col(L.radiance);
+    ls.pdf = ls.dist * ls.dist / (L.area * cosine);
+  }
+  return ls;
+}
+// geometricShape/Mesh.cpp:70-93
+__device__ __forceinline__ float mesh_light_pdf(const DevLight& L, f3 pFrom, f3 pt, f3 n) {
+  f3 v = pFrom - pt;
+  float d = norm(v);
+  v = normalized(v);
+  float c = dot(n, v);
+  if (c > 0.f) return d * d / (L.area * c);
+  return 0.f;
+}
+// geometricShape/Geometry.h:163-190
+__device__ __forceinline__ float sph_phi(f3 v) {
+  float r = atan2f(v.x, -v.z);
+  return r < 0.f ? r + 2.f * 3.14159265358979323846f : r;
+}
+__device__ __forceinline__ float sph_theta(f3 v) { return acosf(v.y); }
+// light/EnvironmentLight.cpp:176-186
+__device__ __forceinline__ void check_xy(const DevScene& S, float& x, float& y) {
+  int W = S.env_w, H = S.env_h;
+  if (x >= (float)W) x = (float)((int)x % W) + (x - (float)(int)x);
+  else if (x < 0.f) x = (float)(W - abs((int)x) % W) + (x - (float)(int)x);
+  if (y >= (float)H) y = (float)((int)y % H) + (y - (float)(int)y);
+  else if (y < 0.f) y = (float)(H - abs((int)y) % H) + (y - (float)(int)y);
+}
+// core/BasicMath.h:74-90
+__device__ __forceinline__ f3 interp2(const DevScene& S, float x, float y) {
+  int W = S.env_w, H = S.env_h;
+  float ax = x - (float)(int)x;
+  float ay = y - (float)(int)y;
+  int x0 = min(max((int)x, 0), W - 2);
+  int y0 = min(max((int)y, 0), H - 2);
+  int x1 = x0 + 1 < W ? x0 + 1 : x0;
+  x1 = min(max(x1, 0), W - 1);
+  int y1 = y0 + 1 < H ? y0 + 1 : y0;
+  y1 = min(max(y1, 0), H - 1);
+  const float* m = S.env_map;
+  f3 A = ld3(m + 3 * (y0 * W + x0)), B = ld3(m + 3 * (y1 * W + x0));
+  f3 C = ld3(m + 3 * (y0 * W + x1)), D = ld3(m + 3 * (y1 * W + x1));
+  return ((1.f - ax) * (1.f - ay)) * A + ((1.f - ax) * ay) * B + (ax * (1.f - ay)) * C + (ax * ay) * D;
+}
+__device__ __forceinline__ void env_coords(const DevScene& S, f3 d, float& fx, float& fy) {
+  f3 dir = normalized(d);
+  fx = (sph_phi(dir) / (2.f * PI)) * (float)S.env_w;
+  fy = (sph_theta(dir) / PI) * (float)S.env_h;
+  check_xy(S, fx, fy);
+}
+// light/EnvironmentLight.cpp:137-166
+__device__ __forceinline__ f3 env_le(const DevScene& S, f3 d) {
+  float fx, fy;
+  env_coords(S, d, fx, fy);
+  return interp2(S, fx, fy) * S.env_factor;
+}
+// light/EnvironmentLight.cpp:105-135 (sin of the texel coordinate, as in the reference)
+__device__ __forceinline__ float env_pdf(const DevScene& S, f3 d) {
+  float fx, fy;
+  env_coords(S, d, fx, fy);
+  float sinT = sinf(fy);
+  int row = min(max((int)fy, 0), S.env_h - 1);
+  float nf = S.env_weight[row] / (S.env_marg_sum * (2.f * PI / (float)S.env_w) * (PI / (float)S.env_h));
+  float lum = luminance(interp2(S, fx, fy) * S.env_factor);
+  float pdf = lum * nf;
+  pdf /= smax(fabsf(sinT), EPS);
+  return pdf;
+}
+// light/EnvironmentLight.cpp:56-103
+__device__ __forceinline__ LightSample env_sample(const DevScene& S, f3 pFrom, float ux, float uy) {
+  LightSample ls;
+  int W = S.env_w, H = S.env_h;
+  int row = cdf_sample(S.env_marg_cdf, H + 1, ux);
+  int col = cdf_sample(S.env_row_cdf + (size_t)row * (W + 1), W + 1, uy);
+  float rcol = (float)col + 0.f * (float)W;
+  float rrow = (float)row + 0.f * (float)H;
+  check_xy(S, rcol, rrow);
+  row = (int)rrow;
+  col = (int)rcol;
+  float theta = PI * (float)row / (float)H;
+  float phi = 2.f * PI * (float)col / (float)W;
+  float sinTheta = sinf(theta);
+  float cosTheta = cosf(theta);
+  float cosPhi = cosf(phi), sinPhi = sinf(phi);
+  ls.dir = mk(sinPhi * sinTheta, cosTheta, -cosPhi * sinTheta);
+  ls.dist = S.env_radius;
+  ls.dir = normalized(ls.dir);
+  int cc = min(max(col, 0), W - 1), rr = min(max(row, 0), H - 1);
+  ls.intensity = ld3(S.env_map + 3 * (rr * W + cc)) * S.env_factor;
+  ls.pdf = luminance(ls.intensity) * S.env_weight[row] /
+           (S.env_marg_sum * (2.f * PI / (float)W) * (PI / (float)H));
+  ls.pdf /= smax(fabsf(sinTheta), EPS);
+  return ls;
+}
+
+}  // namespace dev
+}  // namespace rtg

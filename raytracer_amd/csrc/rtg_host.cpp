@@ -1,0 +1,835 @@
+// Host side of librtg: device context, scene upload (BVH, primitive tables, lights),
+// the wavefront render loop and the C ABI declared in include/rtg.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/rtg.h"
+#include "rtg_bvh.h"
+#include "rtg_host.h"
+#include "rtg_internal.h"
+
+using namespace rtg;
+
+namespace rtg {
+
+[[noreturn]] void fail(int code, const std::string& msg) { throw RtgError(code, msg); }
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) {
+    int code = (e == hipErrorOutOfMemory) ? RTG_ERR_OOM : RTG_ERR_HIP;
+    fail(code, std::string(what) + ": " + hipGetErrorString(e));
+  }
+}
+
+void* dev_alloc(rtg_ctx* ctx, size_t bytes, std::vector<void*>& owned) {
+  void* p = nullptr;
+  if (bytes == 0) bytes = 16;
+  hip_check(hipMalloc(&p, bytes), "hipMalloc");
+  owned.push_back(p);
+  (void)ctx;
+  return p;
+}
+
+template <class T>
+T* upload(rtg_ctx* ctx, const std::vector<T>& v, std::vector<void*>& owned, uint64_t& bytes) {
+  size_t n = v.size() * sizeof(T);
+  T* p = (T*)dev_alloc(ctx, n, owned);
+  if (n) hip_check(hipMemcpyAsync(p, v.data(), n, hipMemcpyHostToDevice, ctx->stream), "upload");
+  bytes += n ? n : 16;
+  return p;
+}
+
+// ---------------------------------------------------------------- CDF (core/CDF.h:14-34)
+static void cdf_build(const std::vector<float>& w, std::vector<float>& out, float& sum) {
+  out.clear();
+  out.push_back(0.f);
+  sum = 0.f;
+  for (float x : w) {
+    sum += x;
+    out.push_back(sum);
+  }
+  if (sum != 0.f) {
+    float f = 1.f / sum;
+    for (size_t i = 1; i < out.size(); i++) out[i] *= f;
+    out.back() = 1.f;
+  }
+}
+
+static inline float lum(const float* c) { return 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2]; }
+static inline float avg3(float a) {  // Color(a).average()
+  float s = 0.f;
+  s += a; s += a; s += a;
+  return s / 3.f;
+}
+
+// 4x4 inverse in double, rounded to float (row-major)
+static void inverse44(const float* a, float* out) {
+  double m[4][8];
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+      m[i][j] = a[4 * i + j];
+      m[i][j + 4] = (i == j) ? 1.0 : 0.0;
+    }
+  for (int c = 0; c < 4; c++) {
+    int p = c;
+    for (int r = c + 1; r < 4; r++)
+      if (std::fabs(m[r][c]) > std::fabs(m[p][c])) p = r;
+    if (m[p][c] == 0.0) fail(RTG_ERR_INVALID, "singular sphere transform");
+    if (p != c)
+      for (int j = 0; j < 8; j++) std::swap(m[p][j], m[c][j]);
+    double inv = 1.0 / m[c][c];
+    for (int j = 0; j < 8; j++) m[c][j] *= inv;
+    for (int r = 0; r < 4; r++)
+      if (r != c) {
+        double f = m[r][c];
+        for (int j = 0; j < 8; j++) m[r][j] -= f * m[c][j];
+      }
+  }
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) out[4 * i + j] = (float)m[i][j + 4];
+}
+
+static inline float xpt(const float* M, int row, float x, float y, float z) {
+  return M[4 * row] * x + M[4 * row + 1] * y + M[4 * row + 2] * z + M[4 * row + 3];
+}
+
+// Camera matrices: core/Transform.cpp:199-227 (lookAt) and camera/NoriPerspective.cpp:10-37,
+// evaluated in double and rounded to float.
+void camera_matrices(const rtg_render_desc& d, float* s2c, float* c2w) {
+  auto nrm = [](double v[3]) {
+    double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (!(l > 0.0)) fail(RTG_ERR_INVALID, "degenerate camera frame");
+    v[0] /= l; v[1] /= l; v[2] /= l;
+  };
+  auto crs = [](const double a[3], const double b[3], double o[3]) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  double dir[3] = {(double)d.target[0] - d.origin[0], (double)d.target[1] - d.origin[1],
+                   (double)d.target[2] - d.origin[2]};
+  nrm(dir);
+  double up[3] = {d.up[0], d.up[1], d.up[2]};
+  nrm(up);
+  double left[3], nu[3];
+  crs(up, dir, left);
+  nrm(left);
+  crs(dir, left, nu);
+  nrm(nu);
+  std::memset(c2w, 0, 16 * sizeof(float));
+  for (int i = 0; i < 3; i++) {
+    c2w[4 * i + 0] = (float)left[i];
+    c2w[4 * i + 1] = (float)nu[i];
+    c2w[4 * i + 2] = (float)dir[i];
+    c2w[4 * i + 3] = d.origin[i];
+  }
+  c2w[15] = 1.f;
+  double aspect = d.width / (double)d.height;
+  double recip = 1.0 / ((double)d.far_clip - d.near_clip);
+  double cot = 1.0 / std::tan((d.fov * 3.14159265358979323846 / 180.0) / 2.0);
+  double P[4][4] = {{cot, 0, 0, 0},
+                    {0, cot, 0, 0},
+                    {0, 0, d.far_clip * recip, -(double)d.near_clip * d.far_clip * recip},
+                    {0, 0, 1, 0}};
+  double S[4][4] = {{-0.5, 0, 0, 0}, {0, -0.5 * aspect, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+  double T[4][4] = {{1, 0, 0, -1.0}, {0, 1, 0, -1.0 / aspect}, {0, 0, 1, 0}, {0, 0, 0, 1}};
+  double ST[4][4] = {}, M[4][4] = {};
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      for (int k = 0; k < 4; k++) ST[i][j] += S[i][k] * T[k][j];
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++)
+      for (int k = 0; k < 4; k++) M[i][j] += ST[i][k] * P[k][j];
+  double m[4][8];
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+      m[i][j] = M[i][j];
+      m[i][j + 4] = (i == j);
+    }
+  for (int c = 0; c < 4; c++) {
+    int p = c;
+    for (int r = c + 1; r < 4; r++)
+      if (std::fabs(m[r][c]) > std::fabs(m[p][c])) p = r;
+    for (int j = 0; j < 8; j++) std::swap(m[p][j], m[c][j]);
+    double iv = 1.0 / m[c][c];
+    for (int j = 0; j < 8; j++) m[c][j] *= iv;
+    for (int r = 0; r < 4; r++)
+      if (r != c) {
+        double f = m[r][c];
+        for (int j = 0; j < 8; j++) m[r][j] -= f * m[c][j];
+      }
+  }
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) s2c[4 * i + j] = (float)m[i][j + 4];
+}
+
+// ---------------------------------------------------------------- scene upload
+rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
+  if (!desc) fail(RTG_ERR_INVALID, "null scene description");
+  if (desc->num_meshes < 0 || desc->num_spheres < 0 || desc->num_materials <= 0)
+    fail(RTG_ERR_INVALID, "scene needs at least one material and non-negative counts");
+  if (desc->light_strategy != RTG_LIGHTS_UNIFORM && desc->light_strategy != RTG_LIGHTS_WEIGHTED)
+    fail(RTG_ERR_INVALID, "light_strategy must be one_light_uniform or one_light_weighted on the GPU");
+  int slot = -1;
+  for (int i = 0; i < kMaxScenes; i++)
+    if (!ctx->scene_slots[i]) {
+      slot = i;
+      break;
+    }
+  if (slot < 0) fail(RTG_ERR_OOM, "too many resident scenes");
+  const int nmat = desc->num_materials;
+  // materials (texture/ConstantTexture.h; bsdf constructors)
+  std::vector<DevMaterial> mats(nmat);
+  for (int i = 0; i < nmat; i++) {
+    const rtg_material& m = desc->materials[i];
+    if (m.type < 0 || m.type > 2) fail(RTG_ERR_INVALID, "unknown material type");
+    if (m.distribution != RTG_DIST_GGX && m.distribution != RTG_DIST_BECKMANN)
+      fail(RTG_ERR_INVALID, "distribution must be ggx or beckmann");
+    DevMaterial d;
+    std::memset(&d, 0, sizeof(d));
+    d.type = m.type;
+    d.dist = m.distribution;
+    d.alpha = avg3(m.alpha);
+    d.eta_int = m.eta_int;
+    d.eta_ext = m.eta_ext;
+    for (int c = 0; c < 3; c++) {
+      d.albedo[c] = m.albedo[c];
+      d.eta[c] = m.eta[c];
+      d.k[c] = m.k[c];
+      d.trans[c] = m.transmitted[c];
+    }
+    mats[i] = d;
+  }
+  // triangles in global id order
+  std::vector<BuildPrim> tprims;
+  std::vector<float> tv;           // 9 per triangle, global order
+  std::vector<uint32_t> tmat, tlight, tmesh;
+  std::vector<int32_t> tvidx;      // global vertex ids
+  std::vector<uint8_t> thasn;
+  std::vector<float> vnormals;
+  std::vector<DevLight> lights;
+  std::vector<float> emit_tri, emit_nrm, emit_cdf;
+  int vbase = 0;
+  bool anyNormals = false;
+  for (int mi = 0; mi < desc->num_meshes; mi++) {
+    const rtg_mesh_desc& M = desc->meshes[mi];
+    if (M.num_triangles < 0 || M.num_vertices < 0 || (M.num_triangles > 0 && (!M.indices || !M.vertices)))
+      fail(RTG_ERR_INVALID, "mesh " + std::to_string(mi) + ": missing arrays");
+    uint32_t lightId = 0xffffffffu;
+    if (M.emission) {
+      DevLight L;
+      std::memset(&L, 0, sizeof(L));
+      L.type = LIGHT_MESH;
+      L.emit_offset = (int)(emit_tri.size() / 9);
+      L.emit_count = M.num_triangles;
+      L.cdf_offset = (int)emit_cdf.size();
+      for (int c = 0; c < 3; c++) L.radiance[c] = M.emission[c];
+      L.has_normals = M.normals ? 1 : 0;
+      std::vector<float> areas(M.num_triangles);
+      float area = 0.f;
+      for (int t = 0; t < M.num_triangles; t++) {
+        const float* p[3];
+        for (int k = 0; k < 3; k++) {
+          int vi = M.indices[3 * t + k];
+          p[k] = M.vertices + 3 * vi;
+          for (int c = 0; c < 3; c++) emit_tri.push_back(p[k][c]);
+          for (int c = 0; c < 3; c++) emit_nrm.push_back(M.normals ? M.normals[3 * vi + c] : 0.f);
+        }
+        // Triangle.h:128-135 (Eigen reduction order a0 + (a1 + a2) for the norm)
+        float e1[3], e2[3];
+        for (int c = 0; c < 3; c++) {
+          e1[c] = p[1][c] - p[0][c];
+          e2[c] = p[2][c] - p[0][c];
+        }
+        float cx = e1[1] * e2[2] - e1[2] * e2[1];
+        float cy = e1[2] * e2[0] - e1[0] * e2[2];
+        float cz = e1[0] * e2[1] - e1[1] * e2[0];
+        float s = 0.5f * std::sqrt(cx * cx + (cy * cy + cz * cz));
+        areas[t] = s;
+        area += s;
+      }
+      std::vector<float> c;
+      float sum;
+      cdf_build(areas, c, sum);
+      emit_cdf.insert(emit_cdf.end(), c.begin(), c.end());
+      L.area = area;
+      lights.push_back(L);
+      lightId = (uint32_t)(lights.size() - 1);
+    }
+    if (M.normals) anyNormals = true;
+    for (int t = 0; t < M.num_triangles; t++) {
+      BuildPrim bp;
+      float v[9];
+      for (int k = 0; k < 3; k++) {
+        int vi = M.indices[3 * t + k];
+        if (vi < 0 || vi >= M.num_vertices) fail(RTG_ERR_INVALID, "mesh index out of range");
+        for (int c = 0; c < 3; c++) v[3 * k + c] = M.vertices[3 * vi + c];
+        tvidx.push_back(vbase + vi);
+      }
+      for (int c = 0; c < 3; c++) {
+        bp.bmin[c] = std::min(std::min(v[c], v[3 + c]), v[6 + c]);
+        bp.bmax[c] = std::max(std::max(v[c], v[3 + c]), v[6 + c]);
+        bp.c[c] = (v[c] + v[3 + c] + v[6 + c]) * (1.f / 3.f);
+      }
+      tprims.push_back(bp);
+      tv.insert(tv.end(), v, v + 9);
+      int mat = M.tri_material ? M.tri_material[t] : M.material;
+      if (mat < -1 || mat >= nmat) fail(RTG_ERR_INVALID, "material id out of range");
+      tmat.push_back(mat < 0 ? 0xffffffffu : (uint32_t)mat);
+      tlight.push_back(lightId);
+      tmesh.push_back((uint32_t)mi);
+      thasn.push_back(M.normals ? 1 : 0);
+    }
+    for (int i = 0; i < M.num_vertices; i++)
+      for (int c = 0; c < 3; c++) vnormals.push_back(M.normals ? M.normals[3 * i + c] : 0.f);
+    vbase += M.num_vertices;
+  }
+  // spheres
+  std::vector<BuildPrim> sprims;
+  std::vector<float> sdata;  // 20 floats per sphere, global order
+  for (int si = 0; si < desc->num_spheres; si++) {
+    const rtg_sphere_desc& s = desc->spheres[si];
+    if (s.material < -1 || s.material >= nmat) fail(RTG_ERR_INVALID, "sphere material out of range");
+    float inv[16];
+    inverse44(s.to_world, inv);
+    float r = s.radius;
+    // GeometricShape.h:87 + Transform.h:141-153: 8 transformed corners of the local box
+    float corners[8][3] = {{-r, -r, -r}, {-r, -r, r}, {-r, r, r}, {r, r, r},
+                           {r, -r, -r}, {r, r, -r}, {r, -r, r}, {-r, r, -r}};
+    BuildPrim bp;
+    for (int c = 0; c < 3; c++) {
+      bp.bmin[c] = 1e8f;
+      bp.bmax[c] = -1e8f;
+    }
+    const float* W = s.to_world;
+    for (auto& cn : corners) {
+      float p[3];
+      float w = W[12] * cn[0] + W[13] * cn[1] + W[14] * cn[2] + W[15];
+      for (int c = 0; c < 3; c++) p[c] = xpt(W, c, cn[0], cn[1], cn[2]);
+      if (w != 1.f && w != 0.f)
+        for (int c = 0; c < 3; c++) p[c] = p[c] / w;
+      for (int c = 0; c < 3; c++) {
+        bp.bmin[c] = std::min(bp.bmin[c], p[c]);
+        bp.bmax[c] = std::max(bp.bmax[c], p[c]);
+      }
+    }
+    for (int c = 0; c < 3; c++) bp.c[c] = xpt(W, c, 0.f, 0.f, 0.f);
+    sprims.push_back(bp);
+    for (int k = 0; k < 16; k++) sdata.push_back(inv[k]);
+    sdata.push_back(r);
+    sdata.push_back(0.f);
+    sdata.push_back(0.f);
+    sdata.push_back(0.f);
+  }
+  const int ntri = (int)tprims.size(), nsph = (int)sprims.size();
+  BvhResult bvh = build_bvh(tprims, sprims);
+  if (bvh.max_depth > 62) fail(RTG_ERR_INVALID, "BVH deeper than the traversal stack");
+  // leaf-order tables
+  std::vector<float> lv((size_t)ntri * 9);
+  std::vector<uint4> info((size_t)ntri + nsph);
+  std::vector<int32_t> lvidx;
+  std::vector<uint8_t> lhasn;
+  if (anyNormals) {
+    lvidx.resize((size_t)ntri * 3);
+    lhasn.resize(ntri);
+  }
+  uint32_t sphBase = (uint32_t)ntri;
+  for (int i = 0; i < ntri; i++) {
+    uint32_t g = bvh.order_tri[i];
+    std::memcpy(&lv[(size_t)i * 9], &tv[(size_t)g * 9], 9 * sizeof(float));
+    info[i] = make_uint4(g, tmat[g], tlight[g], tmesh[g]);
+    if (anyNormals) {
+      for (int k = 0; k < 3; k++) lvidx[(size_t)i * 3 + k] = tvidx[(size_t)g * 3 + k];
+      lhasn[i] = thasn[g];
+    }
+  }
+  std::vector<float4> lsph((size_t)nsph * 5);
+  for (int i = 0; i < nsph; i++) {
+    uint32_t g = bvh.order_sph[i];
+    const float* sd = &sdata[(size_t)g * 20];
+    for (int k = 0; k < 5; k++) lsph[(size_t)i * 5 + k] = make_float4(sd[4 * k], sd[4 * k + 1], sd[4 * k + 2], sd[4 * k + 3]);
+    const rtg_sphere_desc& s = desc->spheres[g];
+    info[sphBase + i] = make_uint4(sphBase + g, s.material < 0 ? 0xffffffffu : (uint32_t)s.material, 0xffffffffu,
+                                   0xffffffffu);
+  }
+  // environment light (light/EnvironmentLight.cpp:8-44,169-174)
+  std::unique_ptr<rtg_scene> sc(new rtg_scene());
+  sc->ctx = ctx;
+  sc->slot = slot;
+  DevScene& D = sc->dev;
+  std::memset(&D, 0, sizeof(D));
+  D.env_light = -1;
+  std::vector<float> env_map, row_cdf, marg_cdf, env_w;
+  if (desc->env_rgb) {
+    int W = desc->env_width, H = desc->env_height;
+    if (W < 2 || H < 2) fail(RTG_ERR_INVALID, "environment map must be at least 2x2");
+    env_map.assign(desc->env_rgb, desc->env_rgb + (size_t)W * H * 3);
+    std::vector<float> margW;
+    float scale = desc->env_scale;
+    const float PI = 3.14159265358979323846f;
+    for (int i = 0; i < H; i++) {
+      float weight = std::sin(PI * (i + 0.5f) / H);
+      std::vector<float> rw(W);
+      for (int j = 0; j < W; j++) {
+        const float* px = &env_map[((size_t)i * W + j) * 3];
+        float c[3] = {px[0] * scale, px[1] * scale, px[2] * scale};
+        rw[j] = lum(c);
+      }
+      std::vector<float> rc;
+      float rsum;
+      cdf_build(rw, rc, rsum);
+      row_cdf.insert(row_cdf.end(), rc.begin(), rc.end());
+      env_w.push_back(weight);
+      margW.push_back(rsum * weight);
+    }
+    float msum;
+    cdf_build(margW, marg_cdf, msum);
+    D.env_w = W;
+    D.env_h = H;
+    D.env_factor = scale;
+    D.env_marg_sum = msum;
+    // bounding sphere of the scene box (EnvironmentLight.cpp:169-174)
+    float c[3], r1 = 0.f, r2 = 0.f;
+    for (int k = 0; k < 3; k++) c[k] = 0.5f * (bvh.root_max[k] + bvh.root_min[k]);
+    {
+      float a0 = bvh.root_min[0] - c[0], a1 = bvh.root_min[1] - c[1], a2 = bvh.root_min[2] - c[2];
+      float b0 = bvh.root_max[0] - c[0], b1 = bvh.root_max[1] - c[1], b2 = bvh.root_max[2] - c[2];
+      r1 = std::sqrt(a0 * a0 + (a1 * a1 + a2 * a2));
+      r2 = std::sqrt(b0 * b0 + (b1 * b1 + b2 * b2));
+    }
+    D.env_radius = 2 * std::max(r1, r2);
+    DevLight L;
+    std::memset(&L, 0, sizeof(L));
+    L.type = LIGHT_ENV;
+    lights.push_back(L);
+    D.env_light = (int)lights.size() - 1;
+  }
+  // light selection CDF (integrator/Integrator.cpp:24-38)
+  std::vector<float> lw, lcdf;
+  float lsum;
+  for (auto& L : lights) {
+    if (desc->light_strategy == RTG_LIGHTS_WEIGHTED) {
+      const float PI = 3.14159265358979323846f;
+      float pw[3];
+      if (L.type == LIGHT_MESH) {
+        for (int c = 0; c < 3; c++) pw[c] = PI * L.radiance[c] * L.area;
+      } else {
+        for (int c = 0; c < 3; c++) pw[c] = 4 * PI * D.env_radius * D.env_radius * 1.f;
+      }
+      lw.push_back(lum(pw));
+    } else {
+      lw.push_back(1.f / lights.size());
+    }
+  }
+  cdf_build(lw, lcdf, lsum);
+  // upload
+  uint64_t bytes = 0;
+  auto& own = sc->owned;
+  D.nodes = (const float4*)upload(ctx, bvh.nodes, own, bytes);
+  D.root_ref = bvh.root_ref;
+  D.empty = bvh.empty ? 1 : 0;
+  for (int k = 0; k < 3; k++) {
+    D.root_min[k] = bvh.root_min[k];
+    D.root_max[k] = bvh.root_max[k];
+  }
+  D.tri_v = upload(ctx, lv, own, bytes);
+  D.prim_info = upload(ctx, info, own, bytes);
+  if (anyNormals) {
+    D.tri_vidx = upload(ctx, lvidx, own, bytes);
+    D.tri_has_n = upload(ctx, lhasn, own, bytes);
+    D.vnormals = upload(ctx, vnormals, own, bytes);
+  }
+  D.spheres = upload(ctx, lsph, own, bytes);
+  D.ntri = ntri;
+  D.nsph = nsph;
+  D.mats = upload(ctx, mats, own, bytes);
+  D.nmat = nmat;
+  D.nlights = (int)lights.size();
+  D.lights = upload(ctx, lights, own, bytes);
+  D.light_cdf = upload(ctx, lcdf, own, bytes);
+  D.strategy = desc->light_strategy;
+  D.emit_tri = upload(ctx, emit_tri, own, bytes);
+  D.emit_nrm = upload(ctx, emit_nrm, own, bytes);
+  D.emit_cdf = upload(ctx, emit_cdf, own, bytes);
+  D.env_map = upload(ctx, env_map, own, bytes);
+  D.env_row_cdf = upload(ctx, row_cdf, own, bytes);
+  D.env_marg_cdf = upload(ctx, marg_cdf, own, bytes);
+  D.env_weight = upload(ctx, env_w, own, bytes);
+  hip_check(hipMemcpyAsync(ctx->scenes_dev + slot, &D, sizeof(DevScene), hipMemcpyHostToDevice, ctx->stream),
+            "scene table");
+  hip_check(hipStreamSynchronize(ctx->stream), "scene upload");
+  sc->bytes = bytes;
+  sc->stack = std::max(16, ((bvh.max_depth + 2 + 7) / 8) * 8);
+  sc->ntri = ntri;
+  sc->nsph = nsph;
+  ctx->scene_slots[slot] = sc.get();
+  return sc.release();
+}
+
+void release_scene(rtg_ctx* ctx, rtg_scene* s) {
+  (void)hipStreamSynchronize(ctx->stream);
+  for (void* p : s->owned) (void)hipFree(p);
+  s->owned.clear();
+  if (s->slot >= 0 && s->slot < kMaxScenes && ctx->scene_slots[s->slot] == s) ctx->scene_slots[s->slot] = nullptr;
+  delete s;
+}
+
+// ---------------------------------------------------------------- pool
+void ensure_pool(rtg_ctx* ctx, int capacity) {
+  if (ctx->pool.capacity == capacity && ctx->pool_owned.size()) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (void* p : ctx->pool_owned) (void)hipFree(p);
+  ctx->pool_owned.clear();
+  DevPool P;
+  std::memset(&P, 0, sizeof(P));
+  P.capacity = capacity;
+  auto A = [&](size_t bytes) { return dev_alloc(ctx, bytes, ctx->pool_owned); };
+  size_t n = (size_t)capacity;
+  P.state = (uint32_t*)A(n * 4);
+  P.flags = (uint32_t*)A(n * 4);
+  P.pixel = (uint32_t*)A(n * 4);
+  P.sample = (uint32_t*)A(n * 4);
+  P.ray_o = (float4*)A(n * 16);
+  P.ray_d = (float4*)A(n * 16);
+  P.hit = (float4*)A(n * 16);
+  P.T = (float4*)A(n * 16);
+  P.F = (float4*)A(n * 16);
+  P.L = (float4*)A(n * 16);
+  P.sh_d = (float4*)A(n * 16);
+  P.sh_c = (float4*)A(n * 16);
+  P.q_ext = (uint32_t*)A(n * 4);
+  P.q_shade = (uint32_t*)A(n * 4 * 4);
+  P.q_shadow = (uint32_t*)A(n * 4);
+  P.counters = (uint32_t*)A(C_COUNT * 4);
+  ctx->pool = P;
+}
+
+int stack_for_resident(rtg_ctx* ctx) {
+  int s = 16;
+  for (auto* sc : ctx->scene_slots)
+    if (sc) s = std::max(s, sc->stack);
+  return s;
+}
+
+DevJob make_job(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc& d, float4* accum) {
+  if (d.width <= 0 || d.height <= 0 || d.spp <= 0) fail(RTG_ERR_INVALID, "width, height and spp must be positive");
+  if (d.sampler != RTG_SAMPLER_RANDOM && d.sampler != RTG_SAMPLER_STRATIFIED) fail(RTG_ERR_INVALID, "unknown sampler");
+  if (d.max_depth < 0 || d.max_depth > 4000 || d.min_depth < 0) fail(RTG_ERR_INVALID, "depth out of range");
+  if (d.guiding) fail(RTG_ERR_INVALID, "guiding is not available in this entry point");
+  if (!(d.fov > 0.f && d.fov < 180.f)) fail(RTG_ERR_INVALID, "fov out of range");
+  DevJob J;
+  std::memset(&J, 0, sizeof(J));
+  camera_matrices(d, J.s2c, J.c2w);
+  J.inv_w = 1.f / d.width;
+  J.inv_h = 1.f / d.height;
+  J.near_clip = d.near_clip;
+  J.far_clip = d.far_clip;
+  J.width = d.width;
+  J.height = d.height;
+  int spp = d.spp;
+  int sq = 1;
+  if (d.sampler == RTG_SAMPLER_STRATIFIED) {
+    // StratifiedSampler.cpp:23-35: a non-square count rounds up to a square
+    int i = 0;
+    for (i = 0; i < spp; i++)
+      if (i * i >= spp) break;
+    spp = i * i;
+    sq = i;
+  }
+  J.spp = spp;
+  J.sqrt_spp = sq;
+  J.sampler = d.sampler;
+  J.dimension = d.dimension > 0 ? d.dimension : 4;
+  J.min_depth = d.min_depth;
+  J.max_depth = d.max_depth;
+  J.seed = d.seed;
+  J.total = (uint64_t)d.width * d.height * (uint64_t)spp;
+  J.next = 0;
+  J.done = 0;
+  J.accum = accum;
+  J.scene = scene->slot;
+  J.active = 1;
+  J.tiles_x = (d.width + 7) / 8;
+  (void)ctx;
+  return J;
+}
+
+}  // namespace rtg
+
+// ==================================================================== C ABI
+#define RTG_GUARD(ctx, body)                                   \
+  try {                                                        \
+    body;                                                      \
+    return RTG_OK;                                             \
+  } catch (const RtgError& e) {                                \
+    if (ctx) (ctx)->err = e.what();                            \
+    return e.code;                                             \
+  } catch (const std::exception& e) {                          \
+    if (ctx) (ctx)->err = e.what();                            \
+    return RTG_ERR_INVALID;                                    \
+  }
+
+extern "C" {
+
+int rtg_ctx_create(int device, rtg_ctx** out) {
+  if (!out) return RTG_ERR_INVALID;
+  *out = nullptr;
+  rtg_ctx* ctx = new rtg_ctx();
+  try {
+    int n = 0;
+    hip_check(hipGetDeviceCount(&n), "hipGetDeviceCount");
+    if (device < 0 || device >= n) fail(RTG_ERR_INVALID, "no such device");
+    ctx->device = device;
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    ctx->cu_count = prop.multiProcessorCount;
+    ctx->arch = prop.gcnArchName;
+    if (ctx->arch.rfind("gfx950", 0) != 0) fail(RTG_ERR_INVALID, "librtg is built for gfx950, device is " + ctx->arch);
+    hip_check(hipMalloc(&ctx->scenes_dev, sizeof(DevScene) * kMaxScenes), "hipMalloc scenes");
+    hip_check(hipMalloc(&ctx->jobs_dev, sizeof(DevJob) * kMaxJobs), "hipMalloc jobs");
+    hip_check(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault), "hipHostMalloc");
+    ctx->scene_slots.assign(kMaxScenes, nullptr);
+    ctx->pool_slots = 1 << 21;
+    ctx->grid = std::max(64, ctx->cu_count * 4);
+  } catch (const RtgError& e) {
+    int code = e.code;
+    delete ctx;
+    return code;
+  }
+  *out = ctx;
+  return RTG_OK;
+}
+
+int rtg_ctx_destroy(rtg_ctx* ctx) {
+  if (!ctx) return RTG_ERR_INVALID;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  for (auto* s : ctx->scene_slots)
+    if (s) release_scene(ctx, s);
+  for (void* p : ctx->pool_owned) (void)hipFree(p);
+  if (ctx->scenes_dev) (void)hipFree(ctx->scenes_dev);
+  if (ctx->jobs_dev) (void)hipFree(ctx->jobs_dev);
+  if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+  for (void* p : ctx->scratch_owned) (void)hipFree(p);
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return RTG_OK;
+}
+
+const char* rtg_last_error(rtg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int rtg_ctx_set_pool(rtg_ctx* ctx, int32_t slots) {
+  if (!ctx) return RTG_ERR_INVALID;
+  RTG_GUARD(ctx, {
+    if (slots < 0 || slots > (1 << 26)) fail(RTG_ERR_INVALID, "pool size out of range");
+    ctx->pool_slots = slots == 0 ? (1 << 21) : ((slots + 255) / 256) * 256;
+  })
+}
+
+int rtg_scene_upload(rtg_ctx* ctx, const rtg_scene_desc* desc, rtg_scene** out) {
+  if (!ctx || !out) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  RTG_GUARD(ctx, {
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    *out = build_scene(ctx, desc);
+  })
+}
+
+int rtg_scene_release(rtg_ctx* ctx, rtg_scene* scene) {
+  if (!ctx || !scene) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  RTG_GUARD(ctx, { release_scene(ctx, scene); })
+}
+
+int rtg_scene_bytes(rtg_scene* scene, uint64_t* bytes) {
+  if (!scene || !bytes) return RTG_ERR_INVALID;
+  *bytes = scene->bytes;
+  return RTG_OK;
+}
+
+static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, float* out_dev, rtg_stats* st) {
+  if (!scene || !desc) fail(RTG_ERR_INVALID, "null scene or render description");
+  hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+  DevJob J = make_job(ctx, scene, *desc, (float4*)out_dev);
+  ensure_pool(ctx, ctx->pool_slots);
+  LaunchCfg cfg;
+  cfg.grid = ctx->grid;
+  cfg.block = 256;
+  cfg.stack = stack_for_resident(ctx);
+  launch_reset_pool(ctx->pool, ctx->stream);
+  hip_check(hipMemcpyAsync(ctx->jobs_dev, &J, sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream), "job upload");
+  hipEvent_t e0, e1;
+  hip_check(hipEventCreate(&e0), "event");
+  hip_check(hipEventCreate(&e1), "event");
+  hip_check(hipEventRecord(e0, ctx->stream), "event");
+  uint64_t iters = 0;
+  uint64_t* done = (uint64_t*)ctx->pinned;
+  const int batch = 4;
+  auto wall0 = std::chrono::steady_clock::now();
+  while (true) {
+    for (int b = 0; b < batch; b++) launch_iteration(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, 1, cfg, ctx->stream);
+    iters += batch;
+    hip_check(hipGetLastError(), "kernel launch");
+    hip_check(hipMemcpyAsync(done, &ctx->jobs_dev[0].done, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream),
+              "done readback");
+    hip_check(hipStreamSynchronize(ctx->stream), "render iteration");
+    if (*done >= J.total) break;
+    double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
+    if (iters > 200000 || wall > 600.0) fail(RTG_ERR_HIP, "render did not finish (iteration or time limit)");
+  }
+  hip_check(hipEventRecord(e1, ctx->stream), "event");
+  hip_check(hipEventSynchronize(e1), "event");
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st) {
+    uint32_t C[C_COUNT];
+    hip_check(hipMemcpy(C, ctx->pool.counters, sizeof(C), hipMemcpyDeviceToHost), "counters");
+    uint64_t ext, sh;
+    std::memcpy(&ext, &C[C_EXTRAYS], 8);
+    std::memcpy(&sh, &C[C_SHRAYS], 8);
+    st->samples = *done;
+    st->extension_rays = ext;
+    st->shadow_rays = sh;
+    st->nan_samples = C[C_NAN];
+    st->iterations = iters;
+    st->milliseconds = ms;
+  }
+}
+
+int rtg_render_device(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, float* out_dev, rtg_stats* st) {
+  if (!ctx) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  RTG_GUARD(ctx, {
+    if (!out_dev) fail(RTG_ERR_INVALID, "null output");
+    render_impl(ctx, scene, desc, out_dev, st);
+  })
+}
+
+int rtg_render(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, float* out_rgbw, rtg_stats* st) {
+  if (!ctx) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  RTG_GUARD(ctx, {
+    if (!out_rgbw || !desc) fail(RTG_ERR_INVALID, "null output");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    size_t bytes = (size_t)desc->width * desc->height * 16;
+    if (desc->width <= 0 || desc->height <= 0) fail(RTG_ERR_INVALID, "bad size");
+    void* acc = nullptr;
+    hip_check(hipMalloc(&acc, bytes), "hipMalloc film");
+    try {
+      launch_fill_zero(acc, bytes, ctx->stream);
+      render_impl(ctx, scene, desc, (float*)acc, st);
+      hip_check(hipMemcpy(out_rgbw, acc, bytes, hipMemcpyDeviceToHost), "film readback");
+    } catch (...) {
+      (void)hipFree(acc);
+      throw;
+    }
+    (void)hipFree(acc);
+  })
+}
+
+int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int32_t any_hit, int32_t* prim, float* t,
+              float* uv, float* p, float* nrm) {
+  if (!ctx) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  RTG_GUARD(ctx, {
+    if (!scene || !rays || !prim || n < 0) fail(RTG_ERR_INVALID, "bad trace arguments");
+    if (n == 0) return RTG_OK;
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    std::vector<void*> own;
+    float* dr = (float*)dev_alloc(ctx, (size_t)n * 32, own);
+    int32_t* dp = (int32_t*)dev_alloc(ctx, (size_t)n * 4, own);
+    float* dt = (float*)dev_alloc(ctx, (size_t)n * 4, own);
+    float* duv = (float*)dev_alloc(ctx, (size_t)n * 8, own);
+    float* dpp = (float*)dev_alloc(ctx, (size_t)n * 12, own);
+    float* dn = (float*)dev_alloc(ctx, (size_t)n * 12, own);
+    try {
+      hip_check(hipMemcpyAsync(dr, rays, (size_t)n * 32, hipMemcpyHostToDevice, ctx->stream), "rays");
+      LaunchCfg cfg;
+      cfg.grid = ctx->grid;
+      cfg.block = 128;
+      cfg.stack = scene->stack;
+      launch_trace_tap(ctx->scenes_dev + scene->slot, dr, n, any_hit, dp, dt, duv, dpp, dn, cfg, ctx->stream);
+      hip_check(hipGetLastError(), "trace launch");
+      hip_check(hipMemcpyAsync(prim, dp, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "prim");
+      if (t) hip_check(hipMemcpyAsync(t, dt, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "t");
+      if (uv) hip_check(hipMemcpyAsync(uv, duv, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream), "uv");
+      if (p) hip_check(hipMemcpyAsync(p, dpp, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "p");
+      if (nrm) hip_check(hipMemcpyAsync(nrm, dn, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "n");
+      hip_check(hipStreamSynchronize(ctx->stream), "trace");
+    } catch (...) {
+      for (void* q : own) (void)hipFree(q);
+      throw;
+    }
+    for (void* q : own) (void)hipFree(q);
+  })
+}
+
+int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const float* wi, const float* wo_or_u,
+                   int32_t n, float* out_dir, float* f, float* pdf, float* rel_eta) {
+  if (!ctx) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  RTG_GUARD(ctx, {
+    if (!material || !wi || !wo_or_u || !f || !pdf || n < 0 || (op != 0 && op != 1))
+      fail(RTG_ERR_INVALID, "bad bsdf arguments");
+    if (op == 1 && (!out_dir || !rel_eta)) fail(RTG_ERR_INVALID, "sample needs out_dir and rel_eta");
+    if (n == 0) return RTG_OK;
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const rtg_material& m = *material;
+    if (m.type < 0 || m.type > 2) fail(RTG_ERR_INVALID, "unknown material type");
+    DevMaterial d;
+    std::memset(&d, 0, sizeof(d));
+    d.type = m.type;
+    d.dist = m.distribution;
+    d.alpha = avg3(m.alpha);
+    d.eta_int = m.eta_int;
+    d.eta_ext = m.eta_ext;
+    for (int c = 0; c < 3; c++) {
+      d.albedo[c] = m.albedo[c];
+      d.eta[c] = m.eta[c];
+      d.k[c] = m.k[c];
+      d.trans[c] = m.transmitted[c];
+    }
+    std::vector<void*> own;
+    DevMaterial* dm = (DevMaterial*)dev_alloc(ctx, sizeof(DevMaterial), own);
+    float* dwi = (float*)dev_alloc(ctx, (size_t)n * 12, own);
+    size_t w2 = (size_t)n * (op == 0 ? 12 : 8);
+    float* dwo = (float*)dev_alloc(ctx, w2, own);
+    float* ddir = (float*)dev_alloc(ctx, (size_t)n * 12, own);
+    float* df = (float*)dev_alloc(ctx, (size_t)n * 12, own);
+    float* dp = (float*)dev_alloc(ctx, (size_t)n * 4, own);
+    float* de = (float*)dev_alloc(ctx, (size_t)n * 4, own);
+    try {
+      hip_check(hipMemcpyAsync(dm, &d, sizeof(d), hipMemcpyHostToDevice, ctx->stream), "mat");
+      hip_check(hipMemcpyAsync(dwi, wi, (size_t)n * 12, hipMemcpyHostToDevice, ctx->stream), "wi");
+      hip_check(hipMemcpyAsync(dwo, wo_or_u, w2, hipMemcpyHostToDevice, ctx->stream), "wo");
+      launch_bsdf_tap(dm, op, dwi, dwo, n, ddir, df, dp, de, ctx->stream);
+      hip_check(hipGetLastError(), "bsdf launch");
+      hip_check(hipMemcpyAsync(f, df, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "f");
+      hip_check(hipMemcpyAsync(pdf, dp, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "pdf");
+      if (op == 1) {
+        hip_check(hipMemcpyAsync(out_dir, ddir, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "dir");
+        hip_check(hipMemcpyAsync(rel_eta, de, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "eta");
+      }
+      hip_check(hipStreamSynchronize(ctx->stream), "bsdf");
+    } catch (...) {
+      for (void* q : own) (void)hipFree(q);
+      throw;
+    }
+    for (void* q : own) (void)hipFree(q);
+  })
+}
+
+}  // extern "C"

@@ -1,0 +1,64 @@
+// Host-side objects behind the opaque handles of include/rtg.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/rtg.h"
+#include "rtg_internal.h"
+
+namespace rtg {
+
+static const int kMaxScenes = 64;
+static const int kMaxJobs = 256;
+
+struct RtgError : std::runtime_error {
+  int code;
+  RtgError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+[[noreturn]] void fail(int code, const std::string& msg);
+void hip_check(hipError_t e, const char* what);
+
+}  // namespace rtg
+
+struct rtg_scene {
+  rtg_ctx* ctx = nullptr;
+  int slot = -1;
+  rtg::DevScene dev;
+  std::vector<void*> owned;
+  uint64_t bytes = 0;
+  int stack = 16;
+  int ntri = 0, nsph = 0;
+};
+
+struct rtg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::string arch;
+  int cu_count = 0;
+  int grid = 1024;
+  int pool_slots = 1 << 21;
+  rtg::DevPool pool{};
+  std::vector<void*> pool_owned;
+  std::vector<void*> scratch_owned;
+  rtg::DevScene* scenes_dev = nullptr;
+  rtg::DevJob* jobs_dev = nullptr;
+  std::vector<rtg_scene*> scene_slots;
+  void* pinned = nullptr;
+  std::recursive_mutex mtx_r;
+  std::mutex mtx;
+};
+
+namespace rtg {
+rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc);
+void release_scene(rtg_ctx* ctx, rtg_scene* s);
+void ensure_pool(rtg_ctx* ctx, int capacity);
+int stack_for_resident(rtg_ctx* ctx);
+DevJob make_job(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc& d, float4* accum);
+void camera_matrices(const rtg_render_desc& d, float* s2c, float* c2w);
+void* dev_alloc(rtg_ctx* ctx, size_t bytes, std::vector<void*>& owned);
+}  // namespace rtg

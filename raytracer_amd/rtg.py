@@ -1,0 +1,295 @@
+"""ctypes binding of librtg.so (include/rtg.h).
+
+Loading fails loudly when the HIP library is missing: there is no CPU fallback.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librtg.so")
+
+RTG_OK = 0
+ERRORS = {-1: "INVALID", -2: "HIP", -3: "OOM", -4: "PARSE", -5: "NOT_READY", -6: "CANCELLED"}
+
+MAT_DIFFUSE, MAT_ROUGH_CONDUCTOR, MAT_ROUGH_DIELECTRIC = 0, 1, 2
+DIST_BECKMANN, DIST_GGX = 0, 2
+LIGHTS_ALL, LIGHTS_UNIFORM, LIGHTS_WEIGHTED = 0, 1, 2
+SAMPLER_RANDOM, SAMPLER_STRATIFIED = 0, 1
+
+
+class rtg_material(C.Structure):
+    _fields_ = [
+        ("type", C.c_int32), ("distribution", C.c_int32), ("albedo", C.c_float * 3), ("alpha", C.c_float),
+        ("eta", C.c_float * 3), ("k", C.c_float * 3), ("eta_int", C.c_float), ("eta_ext", C.c_float),
+        ("transmitted", C.c_float * 3),
+    ]
+
+
+class rtg_mesh_desc(C.Structure):
+    _fields_ = [
+        ("vertices", C.c_void_p), ("normals", C.c_void_p), ("uvs", C.c_void_p), ("num_vertices", C.c_int32),
+        ("indices", C.c_void_p), ("num_triangles", C.c_int32), ("tri_material", C.c_void_p),
+        ("material", C.c_int32), ("emission", C.c_void_p),
+    ]
+
+
+class rtg_sphere_desc(C.Structure):
+    _fields_ = [("to_world", C.c_float * 16), ("radius", C.c_float), ("material", C.c_int32)]
+
+
+class rtg_scene_desc(C.Structure):
+    _fields_ = [
+        ("materials", C.c_void_p), ("num_materials", C.c_int32), ("meshes", C.c_void_p), ("num_meshes", C.c_int32),
+        ("spheres", C.c_void_p), ("num_spheres", C.c_int32), ("env_rgb", C.c_void_p), ("env_width", C.c_int32),
+        ("env_height", C.c_int32), ("env_scale", C.c_float), ("light_strategy", C.c_int32),
+    ]
+
+
+class rtg_render_desc(C.Structure):
+    _fields_ = [
+        ("origin", C.c_float * 3), ("target", C.c_float * 3), ("up", C.c_float * 3), ("fov", C.c_float),
+        ("near_clip", C.c_float), ("far_clip", C.c_float), ("width", C.c_int32), ("height", C.c_int32),
+        ("spp", C.c_int32), ("sampler", C.c_int32), ("dimension", C.c_int32), ("min_depth", C.c_int32),
+        ("max_depth", C.c_int32), ("seed", C.c_uint64), ("guiding", C.c_int32),
+    ]
+
+
+class rtg_stats(C.Structure):
+    _fields_ = [
+        ("samples", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+        ("nan_samples", C.c_uint64), ("iterations", C.c_uint64), ("milliseconds", C.c_double),
+    ]
+
+
+# exported entry points: name -> (restype, argtypes)
+P = C.c_void_p
+SIGNATURES = {
+    "rtg_ctx_create": (C.c_int, [C.c_int, C.POINTER(P)]),
+    "rtg_ctx_destroy": (C.c_int, [P]),
+    "rtg_last_error": (C.c_char_p, [P]),
+    "rtg_ctx_set_pool": (C.c_int, [P, C.c_int32]),
+    "rtg_scene_upload": (C.c_int, [P, C.POINTER(rtg_scene_desc), C.POINTER(P)]),
+    "rtg_scene_load_xml": (C.c_int, [P, C.c_char_p, C.POINTER(P), C.POINTER(rtg_render_desc)]),
+    "rtg_scene_release": (C.c_int, [P, P]),
+    "rtg_scene_bytes": (C.c_int, [P, C.POINTER(C.c_uint64)]),
+    "rtg_render": (C.c_int, [P, P, C.POINTER(rtg_render_desc), P, C.POINTER(rtg_stats)]),
+    "rtg_render_device": (C.c_int, [P, P, C.POINTER(rtg_render_desc), P, C.POINTER(rtg_stats)]),
+    "rtg_trace": (C.c_int, [P, P, P, C.c_int32, C.c_int32, P, P, P, P, P]),
+    "rtg_bsdf_batch": (C.c_int, [P, C.POINTER(rtg_material), C.c_int32, P, P, C.c_int32, P, P, P, P]),
+    "rtg_server_create": (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
+    "rtg_server_destroy": (C.c_int, [P]),
+    "rtg_submit": (C.c_int, [P, P, C.POINTER(rtg_render_desc), C.POINTER(C.c_int64)]),
+    "rtg_server_step": (C.c_int, [P, C.c_double]),
+    "rtg_poll": (C.c_int, [P, C.c_int64, P, C.POINTER(C.c_double)]),
+    "rtg_cancel": (C.c_int, [P, C.c_int64]),
+}
+
+_lib = None
+
+
+def load():
+    """Load librtg.so; raises if it is absent (no fallback path exists)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"librtg.so not found at {LIB_PATH}; run __graft_entry__.build() first")
+        lib = C.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class RtgError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data
+
+
+def _f32(a, shape=None):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    if shape is not None:
+        a = a.reshape(shape)
+    return a
+
+
+def material_struct(m):
+    s = rtg_material()
+    s.type = int(m["type"])
+    s.distribution = int(m.get("distribution", DIST_BECKMANN))
+    s.albedo[:] = [float(x) for x in m.get("albedo", (0.5, 0.5, 0.5))]
+    s.alpha = float(m.get("alpha", 0.2))
+    s.eta[:] = [float(x) for x in m.get("eta", (0.0, 0.0, 0.0))]
+    s.k[:] = [float(x) for x in m.get("k", (0.0, 0.0, 0.0))]
+    s.eta_int = float(m.get("eta_int", 1.5046))
+    s.eta_ext = float(m.get("eta_ext", 1.000277))
+    s.transmitted[:] = [float(x) for x in m.get("transmitted", (1.0, 1.0, 1.0))]
+    return s
+
+
+def render_struct(cam, rp):
+    d = rtg_render_desc()
+    d.origin[:] = [float(x) for x in cam["origin"]]
+    d.target[:] = [float(x) for x in cam["target"]]
+    d.up[:] = [float(x) for x in cam["up"]]
+    d.fov = float(cam["fov"])
+    d.near_clip = float(cam.get("near", 1e-4))
+    d.far_clip = float(cam.get("far", 1e4))
+    d.width = int(cam["width"])
+    d.height = int(cam["height"])
+    d.spp = int(rp["spp"])
+    d.sampler = int(rp.get("sampler", SAMPLER_STRATIFIED))
+    d.dimension = int(rp.get("dimension", 4))
+    d.min_depth = int(rp.get("min_depth", 3))
+    d.max_depth = int(rp.get("max_depth", 1000))
+    d.seed = int(rp.get("seed", 1))
+    d.guiding = int(rp.get("guiding", 0))
+    return d
+
+
+class Context:
+    """One GPU. Owns the wavefront path pool, the scene table and the job table."""
+
+    def __init__(self, device=0):
+        self.lib = load()
+        h = P()
+        rc = self.lib.rtg_ctx_create(device, C.byref(h))
+        if rc != RTG_OK:
+            raise RtgError(rc, "rtg_ctx_create failed (no gfx950 device?)")
+        self.h = h
+
+    def check(self, rc):
+        if rc != RTG_OK:
+            raise RtgError(rc, self.lib.rtg_last_error(self.h).decode())
+
+    def close(self):
+        if self.h:
+            self.lib.rtg_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_pool(self, slots):
+        self.check(self.lib.rtg_ctx_set_pool(self.h, int(slots)))
+
+    def upload(self, scene):
+        return Scene(self, scene)
+
+    def bsdf_eval(self, material, wi, wo):
+        wi = _f32(wi, (-1, 3)); wo = _f32(wo, (-1, 3))
+        n = wi.shape[0]
+        f = np.zeros((n, 3), np.float32); pdf = np.zeros(n, np.float32)
+        m = material_struct(material)
+        self.check(self.lib.rtg_bsdf_batch(self.h, C.byref(m), 0, _ptr(wi), _ptr(wo), n, None, _ptr(f), _ptr(pdf), None))
+        return f, pdf
+
+    def bsdf_sample(self, material, wi, u):
+        wi = _f32(wi, (-1, 3)); u = _f32(u, (-1, 2))
+        n = wi.shape[0]
+        wo = np.zeros((n, 3), np.float32); f = np.zeros((n, 3), np.float32)
+        pdf = np.zeros(n, np.float32); eta = np.zeros(n, np.float32)
+        m = material_struct(material)
+        self.check(self.lib.rtg_bsdf_batch(self.h, C.byref(m), 1, _ptr(wi), _ptr(u), n, _ptr(wo), _ptr(f),
+                                           _ptr(pdf), _ptr(eta)))
+        return wo, f, pdf, eta
+
+
+class Scene:
+    """A resident scene (BVH + tables in HBM)."""
+
+    def __init__(self, ctx, sd):
+        self.ctx = ctx
+        self._keep = []
+        keep = self._keep
+        mats = (rtg_material * len(sd.materials))(*[material_struct(m) for m in sd.materials])
+        keep.append(mats)
+        meshes = (rtg_mesh_desc * max(1, len(sd.meshes)))()
+        for i, m in enumerate(sd.meshes):
+            v = _f32(m.vertices, (-1, 3)); idx = np.ascontiguousarray(m.indices, dtype=np.int32).reshape(-1, 3)
+            keep += [v, idx]
+            md = meshes[i]
+            md.vertices = _ptr(v); md.num_vertices = v.shape[0]
+            md.indices = _ptr(idx); md.num_triangles = idx.shape[0]
+            md.normals = None; md.uvs = None; md.tri_material = None; md.emission = None
+            if m.normals is not None:
+                n = _f32(m.normals, (-1, 3)); keep.append(n); md.normals = _ptr(n)
+            if m.uvs is not None:
+                uv = _f32(m.uvs, (-1, 2)); keep.append(uv); md.uvs = _ptr(uv)
+            if m.tri_material is not None:
+                tm = np.ascontiguousarray(m.tri_material, dtype=np.int32); keep.append(tm); md.tri_material = _ptr(tm)
+            md.material = int(m.material)
+            if m.emission is not None:
+                e = _f32(m.emission, (3,)); keep.append(e); md.emission = _ptr(e)
+        keep.append(meshes)
+        sph = (rtg_sphere_desc * max(1, len(sd.spheres)))()
+        for i, s in enumerate(sd.spheres):
+            sph[i].to_world[:] = [float(x) for x in np.asarray(s.to_world, np.float32).reshape(16)]
+            sph[i].radius = float(s.radius)
+            sph[i].material = int(s.material)
+        keep.append(sph)
+        desc = rtg_scene_desc()
+        desc.materials = C.cast(mats, C.c_void_p); desc.num_materials = len(sd.materials)
+        desc.meshes = C.cast(meshes, C.c_void_p); desc.num_meshes = len(sd.meshes)
+        desc.spheres = C.cast(sph, C.c_void_p); desc.num_spheres = len(sd.spheres)
+        desc.env_rgb = None; desc.env_width = 0; desc.env_height = 0; desc.env_scale = 1.0
+        if sd.env is not None:
+            env = _f32(sd.env); keep.append(env)
+            desc.env_rgb = _ptr(env); desc.env_height, desc.env_width = env.shape[0], env.shape[1]
+            desc.env_scale = float(sd.env_scale)
+        desc.light_strategy = int(sd.light_strategy)
+        h = P()
+        ctx.check(ctx.lib.rtg_scene_upload(ctx.h, C.byref(desc), C.byref(h)))
+        self.h = h
+        self._keep = []  # the library copied everything
+
+    def release(self):
+        if self.h:
+            self.ctx.check(self.ctx.lib.rtg_scene_release(self.ctx.h, self.h))
+            self.h = None
+
+    def bytes(self):
+        b = C.c_uint64()
+        self.ctx.check(self.ctx.lib.rtg_scene_bytes(self.h, C.byref(b)))
+        return b.value
+
+    def trace(self, rays, any_hit=False):
+        rays = _f32(rays, (-1, 8))
+        n = rays.shape[0]
+        prim = np.zeros(n, np.int32); t = np.zeros(n, np.float32); uv = np.zeros((n, 2), np.float32)
+        p = np.zeros((n, 3), np.float32); nrm = np.zeros((n, 3), np.float32)
+        self.ctx.check(self.ctx.lib.rtg_trace(self.ctx.h, self.h, _ptr(rays), n, int(any_hit), _ptr(prim), _ptr(t),
+                                              _ptr(uv), _ptr(p), _ptr(nrm)))
+        return prim, t, uv, p, nrm
+
+    def render(self, camera, params):
+        d = render_struct(camera, params)
+        W, H = d.width, d.height
+        out = np.zeros((H, W, 4), np.float32)
+        st = rtg_stats()
+        self.ctx.check(self.ctx.lib.rtg_render(self.ctx.h, self.h, C.byref(d), _ptr(out), C.byref(st)))
+        return out, st
+
+    def render_device(self, camera, params, out_ptr):
+        d = render_struct(camera, params)
+        st = rtg_stats()
+        self.ctx.check(self.ctx.lib.rtg_render_device(self.ctx.h, self.h, C.byref(d), C.c_void_p(out_ptr),
+                                                      C.byref(st)))
+        return st
+
+
+def resolve(rgbw):
+    """Divide accumulated radiance by the filter weight (core/Screen.cpp:113-122)."""
+    w = rgbw[..., 3:4]
+    return np.where(w > 0, rgbw[..., :3] / np.maximum(w, 1e-30), 0.0).astype(np.float32)

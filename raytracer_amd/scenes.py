@@ -1,0 +1,319 @@
+"""Host-side scene descriptions and the procedural benchmark scenes.
+
+A SceneDesc is what the reference builds from its XML + OBJ inputs: triangle
+meshes (geometricShape/Mesh.h), spheres (geometricShape/Sphere.h), constant-texture
+materials (Diffuse / RoughConductor / RoughDielectric), mesh emitters
+(light/MeshLight.cpp) and a lat-long environment map (light/EnvironmentLight.cpp).
+Light order for the selection CDF: emissive meshes in mesh order, then the
+environment light.
+
+Procedural scenes:
+  cornell()        Cornell-box, 256x256, 16 spp stratified, depth 8, RR after 3, seed 1.
+  displaced_sphere() level-8 icosphere (1,310,720 tris) displaced by 4-octave Perlin noise,
+                   rough GGX conductor patches, Lambertian ground, sky env map, 1024^2, 64 spp,
+                   depth 6.
+  glossy_cornell() Cornell variant: rough dielectric spheres, rough conductor walls, a small
+                   light hidden behind a baffle; 512^2.
+"""
+from dataclasses import dataclass, field
+from typing import List, Optional
+
+import numpy as np
+
+from . import rtg as R
+
+# RoughConductor.cpp:27-28 (gold, as given in the reference's comment)
+GOLD_ETA = (0.22273, 0.42791, 1.2597)
+GOLD_K = (3.0325, 2.3345, 1.7531)
+
+
+@dataclass
+class MeshDesc:
+    vertices: np.ndarray
+    indices: np.ndarray
+    material: int = 0
+    normals: Optional[np.ndarray] = None
+    uvs: Optional[np.ndarray] = None
+    tri_material: Optional[np.ndarray] = None
+    emission: Optional[tuple] = None
+
+
+@dataclass
+class SphereDesc:
+    to_world: np.ndarray
+    radius: float
+    material: int
+
+
+@dataclass
+class SceneDesc:
+    materials: List[dict] = field(default_factory=list)
+    meshes: List[MeshDesc] = field(default_factory=list)
+    spheres: List[SphereDesc] = field(default_factory=list)
+    env: Optional[np.ndarray] = None  # H x W x 3
+    env_scale: float = 1.0
+    light_strategy: int = R.LIGHTS_UNIFORM
+    camera: dict = field(default_factory=dict)
+    render: dict = field(default_factory=dict)
+
+    def add_material(self, **m):
+        self.materials.append(m)
+        return len(self.materials) - 1
+
+    def num_triangles(self):
+        return int(sum(m.indices.shape[0] for m in self.meshes))
+
+
+def diffuse(albedo):
+    return dict(type=R.MAT_DIFFUSE, albedo=tuple(albedo))
+
+
+def conductor(alpha, eta=GOLD_ETA, k=GOLD_K, dist=R.DIST_GGX, reflectance=(1.0, 1.0, 1.0)):
+    return dict(type=R.MAT_ROUGH_CONDUCTOR, alpha=alpha, eta=tuple(eta), k=tuple(k), distribution=dist,
+                albedo=tuple(reflectance))
+
+
+def dielectric(alpha, eta_int=1.5, eta_ext=1.000277, dist=R.DIST_GGX):
+    return dict(type=R.MAT_ROUGH_DIELECTRIC, alpha=alpha, eta_int=eta_int, eta_ext=eta_ext, distribution=dist,
+                albedo=(1.0, 1.0, 1.0), transmitted=(1.0, 1.0, 1.0))
+
+
+def quad(corner, e1, e2, normal):
+    """Two triangles spanning corner + [0,1]e1 + [0,1]e2, wound so (p1-p0)x(p2-p0) follows `normal`."""
+    c = np.asarray(corner, np.float64); a = np.asarray(e1, np.float64); b = np.asarray(e2, np.float64)
+    v = np.array([c, c + a, c + a + b, c + b], np.float64)
+    idx = np.array([[0, 1, 2], [0, 2, 3]], np.int32)
+    if np.dot(np.cross(a, b), np.asarray(normal, np.float64)) < 0:
+        idx = idx[:, ::-1].copy()
+    return v.astype(np.float32), idx
+
+
+def box_mesh(center, size, angle_deg):
+    """Closed box (12 triangles, outward normals) rotated about +y (geometricShape/Cube.cpp:18-66)."""
+    sx, sy, sz = [s * 0.5 for s in size]
+    a = np.radians(angle_deg)
+    Rm = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]])
+    corners = np.array([[x, y, z] for x in (-sx, sx) for y in (-sy, sy) for z in (-sz, sz)])
+    faces = [  # (indices into corners of a face in CCW order seen from outside)
+        (0, 1, 3, 2), (4, 6, 7, 5), (0, 4, 5, 1), (2, 3, 7, 6), (0, 2, 6, 4), (1, 5, 7, 3)]
+    verts = (corners @ Rm.T) + np.asarray(center)
+    idx = []
+    for f in faces:
+        p = verts[list(f)]
+        n = np.cross(p[1] - p[0], p[2] - p[0])
+        outward = np.dot(n, p.mean(0) - np.asarray(center)) > 0
+        tri = [(f[0], f[1], f[2]), (f[0], f[2], f[3])]
+        if not outward:
+            tri = [(t[0], t[2], t[1]) for t in tri]
+        idx += tri
+    return verts.astype(np.float32), np.array(idx, np.int32)
+
+
+def _cornell_walls(sd, white, red, green, left_m=None, right_m=None, back_m=None):
+    L = -0.5
+    walls = [
+        (quad((L, 0, L), (1, 0, 0), (0, 0, 1), (0, 1, 0)), white),            # floor
+        (quad((L, 1, L), (1, 0, 0), (0, 0, 1), (0, -1, 0)), white),           # ceiling
+        (quad((L, 0, L), (1, 0, 0), (0, 1, 0), (0, 0, 1)), back_m if back_m is not None else white),  # back
+        (quad((L, 0, L), (0, 1, 0), (0, 0, 1), (1, 0, 0)), left_m if left_m is not None else red),    # left
+        (quad((0.5, 0, L), (0, 1, 0), (0, 0, 1), (-1, 0, 0)), right_m if right_m is not None else green),  # right
+    ]
+    v = []; idx = []; tm = []; off = 0
+    for (vv, ii), m in walls:
+        v.append(vv); idx.append(ii + off); tm += [m] * len(ii); off += len(vv)
+    sd.meshes.append(MeshDesc(np.concatenate(v), np.concatenate(idx), material=white,
+                              tri_material=np.array(tm, np.int32)))
+
+
+def cornell(width=256, height=256, spp=16, max_depth=8, min_depth=3, seed=1):
+    """Cornell box: 5 Lambertian quads (10 tris), two boxes rotated +-18 deg (24 tris), a 0.25x0.25
+    ceiling light of radiance (17,12,4) facing down, NoriPerspective fov 40."""
+    sd = SceneDesc()
+    white = sd.add_material(**diffuse((0.73, 0.73, 0.73)))
+    red = sd.add_material(**diffuse((0.65, 0.05, 0.05)))
+    green = sd.add_material(**diffuse((0.12, 0.45, 0.15)))
+    black = sd.add_material(**diffuse((0.0, 0.0, 0.0)))
+    _cornell_walls(sd, white, red, green)
+    v, i = box_mesh((0.17, 0.15, 0.12), (0.3, 0.3, 0.3), -18.0)
+    sd.meshes.append(MeshDesc(v, i, material=white))
+    v, i = box_mesh((-0.16, 0.3, -0.15), (0.3, 0.6, 0.3), 18.0)
+    sd.meshes.append(MeshDesc(v, i, material=white))
+    lv, li = quad((-0.125, 0.999, -0.125), (0.25, 0, 0), (0, 0, 0.25), (0, -1, 0))
+    sd.meshes.append(MeshDesc(lv, li, material=black, emission=(17.0, 12.0, 4.0)))
+    sd.camera = dict(origin=(0.0, 0.5, 1.95), target=(0.0, 0.5, 0.0), up=(0.0, 1.0, 0.0), fov=40.0,
+                     width=width, height=height, near=1e-4, far=1e4)
+    sd.render = dict(spp=spp, sampler=R.SAMPLER_STRATIFIED, dimension=4, min_depth=min_depth, max_depth=max_depth,
+                     seed=seed)
+    return sd
+
+
+# ---------------------------------------------------------------- Perlin noise (seeded)
+class Pcg32:
+    """PCG32 (tools/Rng.h:68-141) for procedural parameters."""
+    MULT = 0x5851f42d4c957f2d
+    MASK = (1 << 64) - 1
+
+    def __init__(self, initstate, initseq=1):
+        self.state = 0
+        self.inc = ((initseq << 1) | 1) & self.MASK
+        self.next_uint()
+        self.state = (self.state + initstate) & self.MASK
+        self.next_uint()
+
+    def next_uint(self):
+        old = self.state
+        self.state = (old * self.MULT + self.inc) & self.MASK
+        xs = (((old >> 18) ^ old) >> 27) & 0xFFFFFFFF
+        rot = old >> 59
+        return ((xs >> rot) | (xs << ((-rot) & 31))) & 0xFFFFFFFF
+
+    def next_bounded(self, bound):
+        threshold = ((~bound + 1) & 0xFFFFFFFF) % bound
+        while True:
+            r = self.next_uint()
+            if r >= threshold:
+                return r % bound
+
+    def next_real(self):
+        u = (self.next_uint() >> 9) | 0x3F800000
+        return float(np.frombuffer(np.uint32(u).tobytes(), np.float32)[0]) - 1.0
+
+
+def perlin_perm(seed):
+    rng = Pcg32(seed)
+    p = list(range(256))
+    for i in range(255, 0, -1):  # Fisher-Yates
+        j = rng.next_bounded(i + 1)
+        p[i], p[j] = p[j], p[i]
+    return np.array(p + p, np.int64)
+
+
+def perlin3(x, y, z, perm):
+    """Improved Perlin noise (Perlin 2002) on arrays; the permutation is seeded instead of fixed."""
+    xi = np.floor(x).astype(np.int64) & 255; yi = np.floor(y).astype(np.int64) & 255
+    zi = np.floor(z).astype(np.int64) & 255
+    xf = x - np.floor(x); yf = y - np.floor(y); zf = z - np.floor(z)
+    fade = lambda t: t * t * t * (t * (t * 6 - 15) + 10)
+    u, v, w = fade(xf), fade(yf), fade(zf)
+
+    def grad(h, x, y, z):
+        h = h & 15
+        a = np.where(h < 8, x, y)
+        b = np.where(h < 4, y, np.where((h == 12) | (h == 14), x, z))
+        return np.where(h & 1, -a, a) + np.where(h & 2, -b, b)
+
+    A = perm[xi] + yi; AA = perm[A] + zi; AB = perm[A + 1] + zi
+    B = perm[xi + 1] + yi; BA = perm[B] + zi; BB = perm[B + 1] + zi
+    lerp = lambda t, a, b: a + t * (b - a)
+    return lerp(w,
+                lerp(v, lerp(u, grad(perm[AA], xf, yf, zf), grad(perm[BA], xf - 1, yf, zf)),
+                     lerp(u, grad(perm[AB], xf, yf - 1, zf), grad(perm[BB], xf - 1, yf - 1, zf))),
+                lerp(v, lerp(u, grad(perm[AA + 1], xf, yf, zf - 1), grad(perm[BA + 1], xf - 1, yf, zf - 1)),
+                     lerp(u, grad(perm[AB + 1], xf, yf - 1, zf - 1), grad(perm[BB + 1], xf - 1, yf - 1, zf - 1))))
+
+
+def icosphere(level):
+    """Unit icosphere: 20 * 4^level triangles; children of face f are faces 4f..4f+3."""
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t], [0, 1, -t],
+                  [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    v /= np.linalg.norm(v, axis=1, keepdims=True)
+    f = np.array([[0, 11, 5], [0, 5, 1], [0, 1, 7], [0, 7, 10], [0, 10, 11], [1, 5, 9], [5, 11, 4], [11, 10, 2],
+                  [10, 7, 6], [7, 1, 8], [3, 9, 4], [3, 4, 2], [3, 2, 6], [3, 6, 8], [3, 8, 9], [4, 9, 5],
+                  [2, 4, 11], [6, 2, 10], [8, 6, 7], [9, 8, 1]], np.int64)
+    for _ in range(level):
+        e = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+        e.sort(axis=1)
+        key = e[:, 0] * (len(v) + 1) + e[:, 1]
+        uk, inv = np.unique(key, return_inverse=True)
+        a = uk // (len(v) + 1); b = uk % (len(v) + 1)
+        mid = v[a] + v[b]
+        mid /= np.linalg.norm(mid, axis=1, keepdims=True)
+        m = inv.reshape(3, -1) + len(v)  # m[0]=mid(0,1), m[1]=mid(1,2), m[2]=mid(2,0)
+        v = np.concatenate([v, mid])
+        n = len(f)
+        nf = np.empty((n, 4, 3), np.int64)
+        nf[:, 0] = np.stack([f[:, 0], m[0], m[2]], 1)
+        nf[:, 1] = np.stack([m[0], f[:, 1], m[1]], 1)
+        nf[:, 2] = np.stack([m[2], m[1], f[:, 2]], 1)
+        nf[:, 3] = np.stack([m[0], m[1], m[2]], 1)
+        f = nf.reshape(-1, 3)
+    return v, f
+
+
+def sky_env(width=512, height=256):
+    """Procedural sky gradient, lat-long (row 0 = +y): zenith blue to bright horizon, dim ground."""
+    th = (np.arange(height) + 0.5) / height * np.pi
+    cy = np.cos(th)[:, None]
+    zen = np.array([0.25, 0.45, 1.0]); hor = np.array([1.0, 0.95, 0.85]); gnd = np.array([0.25, 0.22, 0.2])
+    up = np.clip(cy, 0, 1) ** 0.5
+    col = np.where(cy > 0, hor * (1 - up[..., None]) + zen * up[..., None], gnd * (0.5 + 0.5 * (1 + cy[..., None])))
+    col = col * 1.5
+    env = np.broadcast_to(col, (height, 1, 3)) * np.ones((1, width, 1))
+    # a soft warm sun lobe to give NEE some structure
+    ph = (np.arange(width) + 0.5) / width * 2 * np.pi
+    d = np.stack([np.sin(ph)[None, :] * np.sin(th)[:, None], np.cos(th)[:, None] * np.ones((1, width)),
+                  -np.cos(ph)[None, :] * np.sin(th)[:, None]], -1)
+    sun = np.array([0.4, 0.8, 0.45]); sun /= np.linalg.norm(sun)
+    lobe = np.exp(40.0 * (d @ sun - 1.0))[..., None] * np.array([40.0, 36.0, 30.0])
+    return np.ascontiguousarray((env + lobe).astype(np.float32))
+
+
+def displaced_sphere(width=1024, height=1024, spp=64, max_depth=6, min_depth=3, level=8, seed=2, patch=1024):
+    """Large-mesh BVH stress scene: icosphere level 8 (1,310,720 triangles) radially displaced by
+    4-octave Perlin noise (amplitude 0.05, seeded permutation), GGX conductor with alpha ~ U[0.05, 0.4]
+    per `patch` consecutive triangles, Lambertian ground plane, sky env map 512x256."""
+    sd = SceneDesc()
+    ground = sd.add_material(**diffuse((0.5, 0.5, 0.5)))  # Diffuse.cpp:17 default albedo
+    v, f = icosphere(level)
+    perm = perlin_perm(seed)
+    p = v * 4.0
+    disp = np.zeros(len(v))
+    amp = 1.0; freq = 1.0
+    for _ in range(4):
+        disp += amp * perlin3(p[:, 0] * freq, p[:, 1] * freq, p[:, 2] * freq, perm)
+        amp *= 0.5; freq *= 2.0
+    center = np.array([0.0, 1.1, 0.0])
+    verts = (v * (1.0 + 0.05 * disp)[:, None] + center).astype(np.float32)
+    ntri = len(f)
+    npatch = (ntri + patch - 1) // patch
+    rng = Pcg32(seed)
+    base = len(sd.materials)
+    for _ in range(npatch):
+        a = 0.05 + 0.35 * rng.next_real()
+        sd.add_material(**conductor(a))
+    tri_mat = (base + np.arange(ntri) // patch).astype(np.int32)
+    sd.meshes.append(MeshDesc(verts, f.astype(np.int32), material=base, tri_material=tri_mat))
+    gv, gi = quad((-10, 0, -10), (20, 0, 0), (0, 0, 20), (0, 1, 0))
+    sd.meshes.append(MeshDesc(gv, gi, material=ground))
+    sd.env = sky_env(512, 256)
+    sd.camera = dict(origin=(0.0, 1.9, 4.3), target=(0.0, 1.05, 0.0), up=(0.0, 1.0, 0.0), fov=40.0,
+                     width=width, height=height, near=1e-4, far=1e4)
+    sd.render = dict(spp=spp, sampler=R.SAMPLER_STRATIFIED, dimension=4, min_depth=min_depth, max_depth=max_depth,
+                     seed=seed)
+    return sd
+
+
+def glossy_cornell(width=512, height=512, spp=64, max_depth=1000, min_depth=3, seed=5):
+    """Glossy-indirect scene: rough dielectric spheres (IOR 1.5, alpha 0.1), rough GGX conductor walls
+    (alpha 0.15), a 0.05x0.05 light hidden above a baffle so illumination is mostly indirect."""
+    sd = SceneDesc()
+    white = sd.add_material(**diffuse((0.73, 0.73, 0.73)))
+    wall_l = sd.add_material(**conductor(0.15, eta=(0.2, 0.92, 1.1), k=(3.9, 2.45, 2.14)))   # copper-like
+    wall_r = sd.add_material(**conductor(0.15, eta=(1.66, 0.88, 0.52), k=(9.2, 6.3, 4.8)))   # aluminium-like
+    wall_b = sd.add_material(**conductor(0.15))                                              # gold
+    glass = sd.add_material(**dielectric(0.1, eta_int=1.5, eta_ext=1.000277))
+    black = sd.add_material(**diffuse((0.0, 0.0, 0.0)))
+    _cornell_walls(sd, white, wall_l, wall_r, left_m=wall_l, right_m=wall_r, back_m=wall_b)
+    bv, bi = quad((-0.1, 0.95, -0.1), (0.2, 0, 0), (0, 0, 0.2), (0, 1, 0))  # baffle, lit face up
+    sd.meshes.append(MeshDesc(bv, bi, material=white))
+    lv, li = quad((-0.025, 0.999, -0.025), (0.05, 0, 0), (0, 0, 0.05), (0, -1, 0))
+    sd.meshes.append(MeshDesc(lv, li, material=black, emission=(425.0, 300.0, 100.0)))
+    for c, r in (((-0.2, 0.18, -0.1), 0.18), ((0.22, 0.15, 0.15), 0.15)):
+        M = np.eye(4, dtype=np.float32); M[:3, 3] = c
+        sd.spheres.append(SphereDesc(M, r, glass))
+    sd.camera = dict(origin=(0.0, 0.5, 1.95), target=(0.0, 0.5, 0.0), up=(0.0, 1.0, 0.0), fov=40.0,
+                     width=width, height=height, near=1e-4, far=1e4)
+    sd.render = dict(spp=spp, sampler=R.SAMPLER_STRATIFIED, dimension=4, min_depth=min_depth, max_depth=max_depth,
+                     seed=seed)
+    return sd
