@@ -123,7 +123,18 @@ typedef struct rtg_stats {
   uint64_t nan_samples;        /* dropped as in core/Screen.cpp:46-50 */
   uint64_t iterations;         /* wavefront iterations */
   double milliseconds;         /* device time of the render */
+  /* traversal work, filled when RTG_PROFILE_COUNT is set (rtg_ctx_set_profiling) */
+  uint64_t nodes_visited;      /* interior nodes (64-byte child pairs) fetched by extension rays */
+  uint64_t prims_tested;       /* primitives intersected by extension rays */
+  uint64_t shadow_nodes_visited;
+  uint64_t shadow_prims_tested;
+  /* device time per stage and launches, filled when RTG_PROFILE_EVENTS is set:
+   * 0 regen, 1 extend, 2 shade diffuse, 3 shade conductor, 4 shade dielectric, 5 miss, 6 shadow */
+  double stage_ms[7];
+  uint64_t stage_launches[7];
 } rtg_stats;
+
+enum { RTG_PROFILE_EVENTS = 1, RTG_PROFILE_COUNT = 2 };
 
 typedef struct rtg_ctx rtg_ctx;
 typedef struct rtg_scene rtg_scene;
@@ -134,6 +145,8 @@ int rtg_ctx_destroy(rtg_ctx* ctx);
 const char* rtg_last_error(rtg_ctx* ctx);
 /* Wavefront pool size (path slots). 0 keeps the autotuned default. */
 int rtg_ctx_set_pool(rtg_ctx* ctx, int32_t slots);
+/* Profiling flags (RTG_PROFILE_*): per-stage HIP events and traversal work counters. */
+int rtg_ctx_set_profiling(rtg_ctx* ctx, int32_t flags);
 
 /* Scene upload: builds the BVH on the host and makes the scene resident.
  * Replaces Scene::initialize (core/Scene.cpp:115-138) + Bvh::build (accelStruct/Bvh.cpp:51-79). */

@@ -1689,7 +1689,8 @@ struct RenderParams {
 
 // SampleJob.cpp:28-72 for one 32x32 block, merged into the film (Screen.cpp:136-158)
 static void runBlock(const Scene& scene, const Camera& cam, const RenderParams& rp, int bx, int by,
-                     Film& film, std::atomic<uint64_t>& rayCount, std::atomic<uint64_t>& nanCount) {
+                     Film& film, std::atomic<uint64_t>& rayCount, std::atomic<uint64_t>& nanCount,
+                     uint64_t seedOffset) {
     std::unique_ptr<Sampler> sampler;
     if (rp.samplerType == 1)
         sampler.reset(new StratifiedSampler(rp.spp, rp.dimension));
@@ -1697,7 +1698,8 @@ static void runBlock(const Scene& scene, const Camera& cam, const RenderParams& 
         sampler.reset(new RandomSampler());
         sampler->number = rp.spp;
     }
-    sampler->seed((uint64_t)bx, (uint64_t)by);  // Scene.cpp:89-90
+    // Scene.cpp:89-90 (seedOffset 0 is the reference schedule; other values give independent renders)
+    sampler->seed((uint64_t)bx + (seedOffset << 32), (uint64_t)by);
     PathTracer pt;
     pt.scene = &scene;
     pt.minDepth = rp.minDepth;
@@ -2120,7 +2122,8 @@ void orc_li(void* sp, const float* ray, const float* u2, int n2, const float* u1
 // cam: origin3 target3 up3 fov near far. stats: [rays, nan samples]. Optional block range restricts work
 // to blocks [b0, b1) in creation order (for bounded CPU baselines).
 int orc_render(void* sp, const float* cam, int W, int H, int spp, int samplerType, int minDepth, int maxDepth,
-               int threads, int b0, int b1, double* rgb, double* wgt, uint64_t* stats) {
+               int threads, int b0, int b1, int bstride, uint64_t seedOffset, double* rgb, double* wgt,
+               uint64_t* stats) {
     Scene* s = (Scene*)sp;
     Camera c;
     buildCamera(c, cam, cam + 3, cam + 6, cam[9], W, H, cam[10], cam[11]);
@@ -2140,7 +2143,10 @@ int orc_render(void* sp, const float* cam, int W, int H, int spp, int samplerTyp
         for (int x = 0; x < W; x += rp.block) jobs.push_back({x, y});
     if (b1 <= 0 || b1 > (int)jobs.size()) b1 = (int)jobs.size();
     if (b0 < 0) b0 = 0;
-    std::vector<std::pair<int, int>> todo(jobs.begin() + b0, jobs.begin() + b1);
+    if (bstride < 1) bstride = 1;
+    std::vector<std::pair<int, int>> todo;
+    for (int b = b0; b < b1; b += bstride) todo.push_back(jobs[b]);
+    int ntodo = (int)todo.size();
     std::mutex jm;
     std::atomic<uint64_t> rays{0}, nans{0};
     auto worker = [&]() {
@@ -2152,7 +2158,7 @@ int orc_render(void* sp, const float* cam, int W, int H, int spp, int samplerTyp
                 j = todo.back();
                 todo.pop_back();
             }
-            runBlock(*s, c, rp, j.first, j.second, film, rays, nans);
+            runBlock(*s, c, rp, j.first, j.second, film, rays, nans, seedOffset);
         }
     };
     std::vector<std::thread> th;
@@ -2164,7 +2170,7 @@ int orc_render(void* sp, const float* cam, int W, int H, int spp, int samplerTyp
         stats[0] = rays.load();
         stats[1] = nans.load();
     }
-    return (int)jobs.size();
+    return ntodo;
 }
 
 // BVH statistics (for tests of structure invariants): returns node count of scene BVH; depth via out

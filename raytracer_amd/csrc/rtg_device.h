@@ -191,6 +191,10 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
   base = __shfl(base, (int)leader);
   return pred ? base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)) : 0xffffffffu;
 }
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
 __device__ __forceinline__ void wave_count64(unsigned long long* counter, bool pred) {
   unsigned long long mask = __ballot(pred);
   if (mask == 0ull) return;
@@ -288,7 +292,7 @@ struct HitRec {
 // accepted primitive wins); ANY returns at the first accepted primitive.
 template <bool ANY>
 __device__ __forceinline__ bool traverse(const DevScene& S, Ray r, uint32_t* stk_ref, float* stk_t, int stride,
-                                         int cap, HitRec& h, uint32_t& nodes_visited) {
+                                         int cap, HitRec& h, uint32_t& nodes_visited, uint32_t& prims_tested) {
   h.prim = -1;
   adapt_eps(r);
   if (S.empty || r.tmax < r.tmin) return false;
@@ -327,6 +331,7 @@ __device__ __forceinline__ bool traverse(const DevScene& S, Ray r, uint32_t* stk
       }
     } else {
       uint32_t first = leaf_first(ref), cnt = leaf_count(ref);
+      prims_tested += cnt;
       if ((int32_t)first < S.ntri) {
         for (uint32_t i = 0; i < cnt; i++) {
           float t, b1, b2;

@@ -622,6 +622,7 @@ int rtg_ctx_destroy(rtg_ctx* ctx) {
   if (ctx->jobs_dev) (void)hipFree(ctx->jobs_dev);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   for (void* p : ctx->scratch_owned) (void)hipFree(p);
+  for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return RTG_OK;
@@ -635,6 +636,12 @@ int rtg_ctx_set_pool(rtg_ctx* ctx, int32_t slots) {
     if (slots < 0 || slots > (1 << 26)) fail(RTG_ERR_INVALID, "pool size out of range");
     ctx->pool_slots = slots == 0 ? (1 << 21) : ((slots + 255) / 256) * 256;
   })
+}
+
+int rtg_ctx_set_profiling(rtg_ctx* ctx, int32_t flags) {
+  if (!ctx) return RTG_ERR_INVALID;
+  ctx->profiling = flags;
+  return RTG_OK;
 }
 
 int rtg_scene_upload(rtg_ctx* ctx, const rtg_scene_desc* desc, rtg_scene** out) {
@@ -663,6 +670,7 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   hip_check(hipSetDevice(ctx->device), "hipSetDevice");
   DevJob J = make_job(ctx, scene, *desc, (float4*)out_dev);
   ensure_pool(ctx, ctx->pool_slots);
+  ctx->pool.count_work = (ctx->profiling & RTG_PROFILE_COUNT) ? 1 : 0;
   LaunchCfg cfg;
   cfg.grid = ctx->grid;
   cfg.block = 256;
@@ -676,14 +684,32 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   uint64_t iters = 0;
   uint64_t* done = (uint64_t*)ctx->pinned;
   const int batch = 4;
+  const bool prof = (ctx->profiling & RTG_PROFILE_EVENTS) != 0;
+  double stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};
   auto wall0 = std::chrono::steady_clock::now();
   while (true) {
-    for (int b = 0; b < batch; b++) launch_iteration(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, 1, cfg, ctx->stream);
+    if (prof && ctx->events.size() < (size_t)batch * 8) {
+      size_t have = ctx->events.size();
+      ctx->events.resize((size_t)batch * 8);
+      for (size_t i = have; i < ctx->events.size(); i++) hip_check(hipEventCreate(&ctx->events[i]), "event");
+    }
+    for (int b = 0; b < batch; b++)
+      launch_iteration(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, 1, cfg, ctx->stream,
+                       prof ? &ctx->events[(size_t)b * 8] : nullptr);
     iters += batch;
     hip_check(hipGetLastError(), "kernel launch");
     hip_check(hipMemcpyAsync(done, &ctx->jobs_dev[0].done, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream),
               "done readback");
     hip_check(hipStreamSynchronize(ctx->stream), "render iteration");
+    if (prof) {
+      for (int b = 0; b < batch; b++)
+        for (int k = 0; k < 7; k++) {
+          float ms = 0.f;
+          hip_check(hipEventElapsedTime(&ms, ctx->events[(size_t)b * 8 + k], ctx->events[(size_t)b * 8 + k + 1]),
+                    "event time");
+          stage_ms[k] += ms;
+        }
+    }
     if (*done >= J.total) break;
     double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
     if (iters > 200000 || wall > 600.0) fail(RTG_ERR_HIP, "render did not finish (iteration or time limit)");
@@ -697,15 +723,26 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   if (st) {
     uint32_t C[C_COUNT];
     hip_check(hipMemcpy(C, ctx->pool.counters, sizeof(C), hipMemcpyDeviceToHost), "counters");
-    uint64_t ext, sh;
-    std::memcpy(&ext, &C[C_EXTRAYS], 8);
-    std::memcpy(&sh, &C[C_SHRAYS], 8);
+    auto u64 = [&](int i) {
+      uint64_t v;
+      std::memcpy(&v, &C[i], 8);
+      return v;
+    };
+    std::memset(st, 0, sizeof(*st));
     st->samples = *done;
-    st->extension_rays = ext;
-    st->shadow_rays = sh;
+    st->extension_rays = u64(C_EXTRAYS);
+    st->shadow_rays = u64(C_SHRAYS);
     st->nan_samples = C[C_NAN];
     st->iterations = iters;
     st->milliseconds = ms;
+    st->nodes_visited = u64(C_NODES);
+    st->prims_tested = u64(C_PRIMS);
+    st->shadow_nodes_visited = u64(C_SHNODES);
+    st->shadow_prims_tested = u64(C_SHPRIMS);
+    for (int k = 0; k < 7; k++) {
+      st->stage_ms[k] = stage_ms[k];
+      st->stage_launches[k] = prof ? iters : 0;
+    }
   }
 }
 

@@ -42,6 +42,8 @@ struct rtg_ctx {
   int cu_count = 0;
   int grid = 1024;
   int pool_slots = 1 << 21;
+  int profiling = 0;
+  std::vector<hipEvent_t> events;
   rtg::DevPool pool{};
   std::vector<void*> pool_owned;
   std::vector<void*> scratch_owned;

@@ -94,17 +94,15 @@ struct DevJob {
   int32_t pad;
 };
 
-// Path-pool state (structure of arrays). Slot flags word:
-//   bits 0..7 depth, bits 8..15 job slot, bit 16 LAST (emission-only segment),
-//   bit 17 EMISSION (depth-0 emission enabled), bit 18 DELTA (prev sample was delta)
+// Path-pool state (structure of arrays). Slot flags word (rtg_kernels.hip):
+//   bits 0..11 depth of the pending hit, bits 12..19 job slot, bit 20 LAST (only the emission
+//   of the pending segment remains), bit 21 EMISSION (depth-0 emission enabled)
 enum : uint32_t {
   ST_FREE = 0u, ST_ACTIVE = 1u, ST_DONE = 2u
 };
-static const uint32_t F_LAST = 1u << 16;
-static const uint32_t F_EMISSION = 1u << 17;
 struct DevPool {
   int32_t capacity;
-  int32_t pad;
+  int32_t count_work;   // 1: accumulate traversal work counters (nodes, primitives)
   uint32_t* state;      // ST_*
   uint32_t* flags;
   uint32_t* pixel;
@@ -121,13 +119,12 @@ struct DevPool {
   uint32_t* q_ext;      // extension queue
   uint32_t* q_shade;    // 4 queues of capacity each: diffuse, conductor, dielectric, miss
   uint32_t* q_shadow;
-  uint32_t* counters;   // [0] ext, [1..4] shade queues, [5] shadow, [6] active slots,
-                        // [7] nan samples, [8..9] ext rays (u64), [10..11] shadow rays (u64),
-                        // [12..13] samples done (u64)
+  uint32_t* counters;   // C_* below; u64 counters occupy two words
 };
 enum {
   C_EXT = 0, C_SHADE = 1, C_SHADOW = 5, C_ACTIVE = 6, C_NAN = 7, C_EXTRAYS = 8, C_SHRAYS = 10, C_DONE = 12,
-  C_COUNT = 16
+  C_NODES = 14, C_PRIMS = 16, C_SHNODES = 18, C_SHPRIMS = 20,  // u64 work counters (counting mode)
+  C_COUNT = 24
 };
 
 // Launch wrappers implemented in rtg_kernels.hip
@@ -136,8 +133,9 @@ struct LaunchCfg {
   int block;     // threads per block
   int stack;     // traversal stack depth (entries)
 };
+// ev (optional): 8 events recorded before regen and after each of the 7 stage launches
 void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, int njobs,
-                      const LaunchCfg& cfg, hipStream_t stream);
+                      const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev = nullptr);
 void launch_reset_pool(const DevPool& pool, hipStream_t stream);
 void launch_trace_tap(const DevScene* scene_dev, const float* rays, int n, int any_hit, int32_t* prim,
                       float* t, float* uv, float* p, float* nrm, const LaunchCfg& cfg, hipStream_t stream);

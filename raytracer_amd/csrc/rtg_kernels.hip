@@ -162,6 +162,7 @@ __global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scene
     bool valid = k < n;
     int cls = -1;
     uint32_t slot = 0;
+    uint32_t nv = 0, np = 0;
     if (valid) {
       slot = P.q_ext[k];
       uint32_t f = P.flags[slot];
@@ -174,8 +175,7 @@ __global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scene
       r.tmin = o.w;
       r.tmax = d.w;
       HitRec h;
-      uint32_t nv = 0;
-      traverse<false>(S, r, stk_ref, stk_t, blockDim.x, stack_depth, h, nv);
+      traverse<false>(S, r, stk_ref, stk_t, blockDim.x, stack_depth, h, nv, np);
       P.hit[slot] = make_float4(h.t, h.b1, h.b2, __int_as_float(h.prim));
       if (h.prim < 0) {
         cls = 3;
@@ -186,6 +186,13 @@ __global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scene
       }
     }
     wave_count64((unsigned long long*)&C[C_EXTRAYS], valid);
+    if (P.count_work) {
+      uint32_t a = wave_sum(nv), b = wave_sum(np);
+      if (__lane_id() == 0) {
+        atomicAdd((unsigned long long*)&C[C_NODES], (unsigned long long)a);
+        atomicAdd((unsigned long long*)&C[C_PRIMS], (unsigned long long)b);
+      }
+    }
     // classify into the four shade queues (no-BSDF hits go with misses)
     for (int q = 0; q < 4; q++) {
       bool mine = valid && (cls == q || (q == 3 && cls == 4));
@@ -390,6 +397,7 @@ __global__ void __launch_bounds__(128) k_shadow(DevPool P, const DevScene* scene
   for (uint32_t it = 0; it < nIter; it++) {
     uint32_t k = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = k < n;
+    uint32_t nv = 0, np = 0;
     if (valid) {
       uint32_t slot = P.q_shadow[k];
       uint32_t f = P.flags[slot];
@@ -402,8 +410,7 @@ __global__ void __launch_bounds__(128) k_shadow(DevPool P, const DevScene* scene
       r.tmin = EPS;
       r.tmax = d.w;
       HitRec h;
-      uint32_t nv = 0;
-      bool occ = traverse<true>(S, r, stk_ref, stk_t, blockDim.x, stack_depth, h, nv);
+      bool occ = traverse<true>(S, r, stk_ref, stk_t, blockDim.x, stack_depth, h, nv, np);
       if (!occ) {
         float4 c = P.sh_c[slot];
         float4 L = P.L[slot];
@@ -412,6 +419,13 @@ __global__ void __launch_bounds__(128) k_shadow(DevPool P, const DevScene* scene
       }
     }
     wave_count64((unsigned long long*)&C[C_SHRAYS], valid);
+    if (P.count_work) {
+      uint32_t a = wave_sum(nv), b = wave_sum(np);
+      if (__lane_id() == 0) {
+        atomicAdd((unsigned long long*)&C[C_SHNODES], (unsigned long long)a);
+        atomicAdd((unsigned long long*)&C[C_SHPRIMS], (unsigned long long)b);
+      }
+    }
   }
 }
 
@@ -444,9 +458,9 @@ __global__ void __launch_bounds__(128) k_trace_tap(const DevScene* scene, const 
   r.tmin = q[6];
   r.tmax = q[7];
   HitRec h;
-  uint32_t nv = 0;
-  bool hit = any_hit ? traverse<true>(S, r, stk_ref, stk_t, blockDim.x, stack_depth, h, nv)
-                     : traverse<false>(S, r, stk_ref, stk_t, blockDim.x, stack_depth, h, nv);
+  uint32_t nv = 0, np = 0;
+  bool hit = any_hit ? traverse<true>(S, r, stk_ref, stk_t, blockDim.x, stack_depth, h, nv, np)
+                     : traverse<false>(S, r, stk_ref, stk_t, blockDim.x, stack_depth, h, nv, np);
   prim[i] = hit ? (int32_t)S.prim_info[h.prim].x : -1;
   if (t) t[i] = hit ? h.t : 0.f;
   if (uv) {
@@ -503,17 +517,26 @@ __global__ void k_bsdf_tap(const DevMaterial* mat, int op, const float* wi, cons
 static size_t stack_bytes(int block, int depth) { return (size_t)block * depth * 8u; }
 
 void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, int njobs,
-                      const LaunchCfg& cfg, hipStream_t stream) {
+                      const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev) {
   int g = cfg.grid;
+  const DevJob* cj = jobs_dev;
+  if (ev) (void)hipEventRecord(ev[0], stream);
   hipLaunchKernelGGL(k_regen, dim3(g), dim3(256), 0, stream, pool, jobs_dev, njobs);
-  hipLaunchKernelGGL(k_extend, dim3(g * 2), dim3(128), stack_bytes(128, cfg.stack), stream, pool, scenes_dev,
-                     (const DevJob*)jobs_dev, cfg.stack);
-  hipLaunchKernelGGL(k_shade<0>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, (const DevJob*)jobs_dev);
-  hipLaunchKernelGGL(k_shade<1>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, (const DevJob*)jobs_dev);
-  hipLaunchKernelGGL(k_shade<2>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, (const DevJob*)jobs_dev);
-  hipLaunchKernelGGL(k_shade_miss, dim3(g), dim3(256), 0, stream, pool, scenes_dev, (const DevJob*)jobs_dev);
-  hipLaunchKernelGGL(k_shadow, dim3(g * 2), dim3(128), stack_bytes(128, cfg.stack), stream, pool, scenes_dev,
-                     (const DevJob*)jobs_dev, cfg.stack);
+  if (ev) (void)hipEventRecord(ev[1], stream);
+  hipLaunchKernelGGL(k_extend, dim3(g * 2), dim3(128), stack_bytes(128, cfg.stack), stream, pool, scenes_dev, cj,
+                     cfg.stack);
+  if (ev) (void)hipEventRecord(ev[2], stream);
+  hipLaunchKernelGGL(k_shade<0>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj);
+  if (ev) (void)hipEventRecord(ev[3], stream);
+  hipLaunchKernelGGL(k_shade<1>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj);
+  if (ev) (void)hipEventRecord(ev[4], stream);
+  hipLaunchKernelGGL(k_shade<2>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj);
+  if (ev) (void)hipEventRecord(ev[5], stream);
+  hipLaunchKernelGGL(k_shade_miss, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj);
+  if (ev) (void)hipEventRecord(ev[6], stream);
+  hipLaunchKernelGGL(k_shadow, dim3(g * 2), dim3(128), stack_bytes(128, cfg.stack), stream, pool, scenes_dev, cj,
+                     cfg.stack);
+  if (ev) (void)hipEventRecord(ev[7], stream);
 }
 
 void launch_reset_pool(const DevPool& pool, hipStream_t stream) {

@@ -60,7 +60,13 @@ class rtg_stats(C.Structure):
     _fields_ = [
         ("samples", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
         ("nan_samples", C.c_uint64), ("iterations", C.c_uint64), ("milliseconds", C.c_double),
+        ("nodes_visited", C.c_uint64), ("prims_tested", C.c_uint64), ("shadow_nodes_visited", C.c_uint64),
+        ("shadow_prims_tested", C.c_uint64), ("stage_ms", C.c_double * 7), ("stage_launches", C.c_uint64 * 7),
     ]
+
+
+STAGES = ("regen", "extend", "shade_diffuse", "shade_conductor", "shade_dielectric", "shade_miss", "shadow")
+PROFILE_EVENTS, PROFILE_COUNT = 1, 2
 
 
 # exported entry points: name -> (restype, argtypes)
@@ -70,6 +76,7 @@ SIGNATURES = {
     "rtg_ctx_destroy": (C.c_int, [P]),
     "rtg_last_error": (C.c_char_p, [P]),
     "rtg_ctx_set_pool": (C.c_int, [P, C.c_int32]),
+    "rtg_ctx_set_profiling": (C.c_int, [P, C.c_int32]),
     "rtg_scene_upload": (C.c_int, [P, C.POINTER(rtg_scene_desc), C.POINTER(P)]),
     "rtg_scene_load_xml": (C.c_int, [P, C.c_char_p, C.POINTER(P), C.POINTER(rtg_render_desc)]),
     "rtg_scene_release": (C.c_int, [P, P]),
@@ -183,6 +190,9 @@ class Context:
 
     def set_pool(self, slots):
         self.check(self.lib.rtg_ctx_set_pool(self.h, int(slots)))
+
+    def set_profiling(self, flags):
+        self.check(self.lib.rtg_ctx_set_profiling(self.h, int(flags)))
 
     def upload(self, scene):
         return Scene(self, scene)

@@ -242,16 +242,17 @@ def icosphere(level):
 
 
 def sky_env(width=512, height=256):
-    """Procedural sky gradient, lat-long (row 0 = +y): zenith blue to bright horizon, dim ground."""
+    """Procedural sky gradient, lat-long (row 0 = +y): zenith blue to bright horizon, dim ground,
+    plus a soft warm sun lobe so the env-light sampler has structure to follow."""
     th = (np.arange(height) + 0.5) / height * np.pi
-    cy = np.cos(th)[:, None]
-    zen = np.array([0.25, 0.45, 1.0]); hor = np.array([1.0, 0.95, 0.85]); gnd = np.array([0.25, 0.22, 0.2])
-    up = np.clip(cy, 0, 1) ** 0.5
-    col = np.where(cy > 0, hor * (1 - up[..., None]) + zen * up[..., None], gnd * (0.5 + 0.5 * (1 + cy[..., None])))
-    col = col * 1.5
-    env = np.broadcast_to(col, (height, 1, 3)) * np.ones((1, width, 1))
-    # a soft warm sun lobe to give NEE some structure
     ph = (np.arange(width) + 0.5) / width * 2 * np.pi
+    cy = np.cos(th)[:, None, None]                       # H x 1 x 1
+    zen = np.array([0.25, 0.45, 1.0]); hor = np.array([1.0, 0.95, 0.85]); gnd = np.array([0.25, 0.22, 0.2])
+    up = np.sqrt(np.clip(cy, 0, 1))
+    sky = hor * (1 - up) + zen * up                      # H x 1 x 3
+    ground = gnd * (0.5 + 0.5 * (1 + cy))
+    col = np.where(cy > 0, sky, ground) * 1.5            # H x 1 x 3
+    env = np.repeat(col, width, axis=1)                  # H x W x 3
     d = np.stack([np.sin(ph)[None, :] * np.sin(th)[:, None], np.cos(th)[:, None] * np.ones((1, width)),
                   -np.cos(ph)[None, :] * np.sin(th)[:, None]], -1)
     sun = np.array([0.4, 0.8, 0.45]); sun /= np.linalg.norm(sun)

@@ -53,7 +53,7 @@ def load():
             "orc_light_cdf": (None, [P, P]),
             "orc_li": (None, [P, P, P, C.c_int, P, C.c_int, C.c_int, C.c_int, P]),
             "orc_render": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                     C.c_int, P, P, P]),
+                                     C.c_int, C.c_int, C.c_uint64, P, P, P]),
             "orc_bvh_info": (C.c_int, [P, P]),
         }
         for k, (r, a) in sig.items():
@@ -94,7 +94,8 @@ class OracleScene:
         self.lib = load()
         self.h = self.lib.orc_scene_new()
         for m in sd.materials:
-            self.lib.orc_add_material(self.h, _p(material_params(m)))
+            mp = material_params(m)
+            self.lib.orc_add_material(self.h, _p(mp))
         for m in sd.meshes:
             v = f32(m.vertices, (-1, 3)); idx = np.ascontiguousarray(m.indices, np.int32).reshape(-1, 3)
             n = None if m.normals is None else f32(m.normals, (-1, 3))
@@ -167,7 +168,10 @@ class OracleScene:
         self.lib.orc_li(self.h, _p(ray), _p(u2), u2.shape[0], _p(u1), u1.shape[0], min_depth, max_depth, _p(L))
         return L
 
-    def render(self, camera=None, spp=None, threads=1, blocks=(0, 0), sampler=1, min_depth=None, max_depth=None):
+    def render(self, camera=None, spp=None, threads=1, blocks=(0, 0, 1), sampler=1, min_depth=None, max_depth=None,
+               seed_offset=0):
+        """Reference schedule (32x32 blocks seeded (blockX, blockY), LIFO job pop). blocks = (first, end,
+        stride) over the blocks in creation order; returns (rgb sums, weights, [rays, nans], blocks run)."""
         sd = self.sd
         cam = dict(sd.camera if camera is None else camera)
         rp = sd.render
@@ -179,7 +183,8 @@ class OracleScene:
         nblocks = self.lib.orc_render(self.h, _p(cv), W, H, int(spp or rp["spp"]), int(sampler),
                                       int(rp["min_depth"] if min_depth is None else min_depth),
                                       int(rp["max_depth"] if max_depth is None else max_depth), int(threads),
-                                      int(blocks[0]), int(blocks[1]), _p(rgb), _p(wgt), _p(stats))
+                                      int(blocks[0]), int(blocks[1]), int(blocks[2] if len(blocks) > 2 else 1),
+                                      int(seed_offset), _p(rgb), _p(wgt), _p(stats))
         return rgb, wgt, stats, nblocks
 
 
@@ -242,7 +247,8 @@ def mf_sample(dist, u, alpha):
 def fresnel_conductor(eta, k, cosi):
     cosi = f32(cosi, (-1,)); n = cosi.shape[0]
     out = np.zeros((n, 3), np.float32)
-    load().orc_fresnel_conductor(_p(f32(eta, (3,))), _p(f32(k, (3,))), _p(cosi), n, _p(out))
+    e = f32(eta, (3,)); kk = f32(k, (3,))
+    load().orc_fresnel_conductor(_p(e), _p(kk), _p(cosi), n, _p(out))
     return out
 
 
