@@ -287,12 +287,46 @@ struct HitRec {
   int32_t prim;  // leaf-order primitive index, -1 on miss
 };
 
-// BVH traversal over child-pair nodes; stack in LDS ([depth][blockDim] layout, no bank conflicts).
-// Closest hit keeps the reference's acceptance rules (triangles accept t == maxT, the last
-// accepted primitive wins); ANY returns at the first accepted primitive.
+// Traversal stack: the top kLdsStack entries of each lane live in LDS ([entry][block thread] layout,
+// conflict-free), deeper entries spill to a global buffer ([entry][grid thread] layout, coalesced).
+// An entry is a subtree ref plus the entry distance of its box, so a popped subtree that starts
+// beyond the current closest hit is skipped without fetching its node.
+constexpr int kLdsStack = 16;
+struct TravStack {
+  uint32_t* lref;  // LDS refs, this lane's column
+  float* lt;       // LDS distances
+  int lstride;     // threads per block
+  uint2* g;        // global spill, this lane's column
+  int gstride;     // threads in the grid
+  int cap;         // total entries (>= tree height, set by the host)
+};
+__device__ __forceinline__ void stk_push(const TravStack& s, int& sp, uint32_t ref, float t) {
+  if (sp < kLdsStack) {
+    s.lref[sp * s.lstride] = ref;
+    s.lt[sp * s.lstride] = t;
+    sp++;
+  } else if (sp < s.cap) {
+    s.g[(size_t)(sp - kLdsStack) * s.gstride] = make_uint2(ref, __float_as_uint(t));
+    sp++;
+  }
+}
+__device__ __forceinline__ void stk_top(const TravStack& s, int sp, uint32_t& ref, float& t) {
+  if (sp < kLdsStack) {
+    ref = s.lref[sp * s.lstride];
+    t = s.lt[sp * s.lstride];
+  } else {
+    uint2 e = s.g[(size_t)(sp - kLdsStack) * s.gstride];
+    ref = e.x;
+    t = __uint_as_float(e.y);
+  }
+}
+
+// BVH traversal over child-pair nodes, nearer child first. Closest hit keeps the reference's
+// acceptance rules (triangles accept t == maxT, the last accepted primitive wins); ANY returns at
+// the first accepted primitive.
 template <bool ANY>
-__device__ __forceinline__ bool traverse(const DevScene& S, Ray r, uint32_t* stk_ref, float* stk_t, int stride,
-                                         int cap, HitRec& h, uint32_t& nodes_visited, uint32_t& prims_tested) {
+__device__ __forceinline__ bool traverse(const DevScene& S, Ray r, const TravStack& stk, HitRec& h,
+                                         uint32_t& nodes_visited, uint32_t& prims_tested) {
   h.prim = -1;
   adapt_eps(r);
   if (S.empty || r.tmax < r.tmin) return false;
@@ -315,11 +349,7 @@ __device__ __forceinline__ bool traverse(const DevScene& S, Ray r, uint32_t* stk
         uint32_t nearR = r0, farR = r1;
         float farT = t1;
         if (t1 < t0) { nearR = r1; farR = r0; farT = t0; }
-        if (sp < cap) {  // the host sizes cap above the tree height, so this always holds
-          stk_ref[sp * stride] = farR;
-          stk_t[sp * stride] = farT;
-          sp++;
-        }
+        stk_push(stk, sp, farR, farT);
         ref = nearR;
         continue;
       } else if (h0) {
@@ -359,9 +389,11 @@ __device__ __forceinline__ bool traverse(const DevScene& S, Ray r, uint32_t* stk
     while (true) {
       if (sp == 0) return hit;
       sp--;
-      float pt = stk_t[sp * stride];
+      float pt;
+      uint32_t pr;
+      stk_top(stk, sp, pr, pt);
       if (r.tmax >= pt) {
-        ref = stk_ref[sp * stride];
+        ref = pr;
         break;
       }
     }

@@ -468,7 +468,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
             "scene table");
   hip_check(hipStreamSynchronize(ctx->stream), "scene upload");
   sc->bytes = bytes;
-  sc->stack = std::max(16, ((bvh.max_depth + 2 + 7) / 8) * 8);
+  sc->stack = std::min(kMaxStack, std::max(16, bvh.max_depth + 2));
   sc->ntri = ntri;
   sc->nsph = nsph;
   ctx->scene_slots[slot] = sc.get();
@@ -601,7 +601,10 @@ int rtg_ctx_create(int device, rtg_ctx** out) {
     hip_check(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault), "hipHostMalloc");
     ctx->scene_slots.assign(kMaxScenes, nullptr);
     ctx->pool_slots = 1 << 21;
-    ctx->grid = std::max(64, ctx->cu_count * 4);
+    ctx->grid = std::max(64, ctx->cu_count * 5);
+    // traversal stack spill area for every lane of a traversal launch (2 * grid blocks of 128)
+    size_t lanes = (size_t)ctx->grid * 2 * 128;
+    hip_check(hipMalloc(&ctx->gstack, lanes * (kMaxStack - 16) * sizeof(uint2)), "hipMalloc stack spill");
   } catch (const RtgError& e) {
     int code = e.code;
     delete ctx;
@@ -620,6 +623,7 @@ int rtg_ctx_destroy(rtg_ctx* ctx) {
   for (void* p : ctx->pool_owned) (void)hipFree(p);
   if (ctx->scenes_dev) (void)hipFree(ctx->scenes_dev);
   if (ctx->jobs_dev) (void)hipFree(ctx->jobs_dev);
+  if (ctx->gstack) (void)hipFree(ctx->gstack);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   for (void* p : ctx->scratch_owned) (void)hipFree(p);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
@@ -675,6 +679,7 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   cfg.grid = ctx->grid;
   cfg.block = 256;
   cfg.stack = stack_for_resident(ctx);
+  cfg.gstack = ctx->gstack;
   launch_reset_pool(ctx->pool, ctx->stream);
   hip_check(hipMemcpyAsync(ctx->jobs_dev, &J, sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream), "job upload");
   hipEvent_t e0, e1;
@@ -798,6 +803,7 @@ int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int3
       cfg.grid = ctx->grid;
       cfg.block = 128;
       cfg.stack = scene->stack;
+      cfg.gstack = ctx->gstack;
       launch_trace_tap(ctx->scenes_dev + scene->slot, dr, n, any_hit, dp, dt, duv, dpp, dn, cfg, ctx->stream);
       hip_check(hipGetLastError(), "trace launch");
       hip_check(hipMemcpyAsync(prim, dp, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "prim");

@@ -49,6 +49,7 @@ struct rtg_ctx {
   std::vector<void*> scratch_owned;
   rtg::DevScene* scenes_dev = nullptr;
   rtg::DevJob* jobs_dev = nullptr;
+  uint2* gstack = nullptr;
   std::vector<rtg_scene*> scene_slots;
   void* pinned = nullptr;
   std::recursive_mutex mtx_r;

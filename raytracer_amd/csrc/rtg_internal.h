@@ -129,10 +129,12 @@ enum {
 
 // Launch wrappers implemented in rtg_kernels.hip
 struct LaunchCfg {
-  int grid;      // blocks for grid-stride kernels
+  int grid;      // blocks for grid-stride kernels (traversal kernels use 2x grid blocks of 128)
   int block;     // threads per block
-  int stack;     // traversal stack depth (entries)
+  int stack;     // traversal stack capacity (entries, >= tree height)
+  uint2* gstack; // spill area: (stack - 16) entries x (2 * grid * 128) lanes
 };
+static const int kMaxStack = 64;
 // ev (optional): 8 events recorded before regen and after each of the 7 stage launches
 void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, int njobs,
                       const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev = nullptr);

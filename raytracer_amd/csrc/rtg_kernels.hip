@@ -66,9 +66,12 @@ __device__ __forceinline__ void splat(const DevJob& J, uint32_t pixel, float sx,
 }
 
 // ---------------------------------------------------------------- regen
-// Finalises finished paths into their job's film, claims new samples for free slots
-// (wave-aggregated 64-bit claims, jobs in admission order) and builds the extension queue.
+// Finalises finished paths into their job's film, claims new samples for free slots and builds the
+// extension queue. Claims are aggregated per block (one 64-bit atomic per block and job), the
+// finished-sample count per wave.
 __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, int njobs) {
+  __shared__ uint32_t s_cnt[4];
+  __shared__ unsigned long long s_base;
   uint32_t* C = P.counters;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     // shade queues are filled by the extend stage of this iteration
@@ -76,15 +79,20 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, int njob
   }
   const int cap = P.capacity;
   const int stride = gridDim.x * blockDim.x;
-  // iterate whole waves so ballots see converged lanes
+  const uint32_t lane = __lane_id();
+  const uint32_t wave = threadIdx.x >> 6;
+  // iterate whole blocks so ballots and barriers see every lane
   int nIter = (cap + stride - 1) / stride;
   for (int it = 0; it < nIter; it++) {
     int i = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
     bool valid = i < cap;
     uint32_t st = valid ? P.state[i] : ST_ACTIVE;
-    if (valid && st == ST_DONE) {
+    bool fin = valid && st == ST_DONE;
+    uint32_t fjob = 0;
+    if (fin) {
       uint32_t f = P.flags[i];
-      const DevJob& J = jobs[fl_job(f)];
+      fjob = fl_job(f);
+      const DevJob& J = jobs[fjob];
       float4 L = P.L[i];
       f3 l = mk(L.x, L.y, L.z);
       bool nan = isnan(l.x) || isnan(l.y) || isnan(l.z);
@@ -97,47 +105,59 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, int njob
         sample2d(rng, 0, sx, sy);
         splat(J, pix, sx, sy, l);
       }
-      atomicAdd((unsigned long long*)&jobs[fl_job(f)].done, 1ull);
       st = ST_FREE;
       P.state[i] = ST_FREE;
     }
+    // finished-sample counts: one atomic per (wave, job) present in the wave
+    {
+      unsigned long long m = __ballot(fin);
+      while (m) {
+        uint32_t leader = (uint32_t)__builtin_ctzll(m);
+        uint32_t lj = (uint32_t)__shfl((int)fjob, (int)leader);
+        unsigned long long same = __ballot(fin && fjob == lj);
+        if (lane == leader) atomicAdd((unsigned long long*)&jobs[lj].done, (unsigned long long)__popcll(same));
+        m &= ~same;
+      }
+    }
     bool need = valid && st == ST_FREE;
-    // claim from jobs in order
     for (int j = 0; j < njobs; j++) {
-      if (__ballot(need) == 0ull) break;
+      if (!__syncthreads_or(need)) break;
       DevJob& J = jobs[j];
       if (!J.active) continue;
       unsigned long long mask = __ballot(need);
-      uint32_t lane = __lane_id();
-      uint32_t leader = (uint32_t)__builtin_ctzll(mask);
-      unsigned long long base = 0;
-      if (lane == leader) base = atomicAdd((unsigned long long*)&J.next, (unsigned long long)__popcll(mask));
-      base = __shfl(base, (int)leader);
-      if (need) {
-        uint64_t s = base + (uint64_t)__popcll(mask & ((1ull << lane) - 1ull));
-        if (s < J.total) {
-          uint32_t pix, si;
-          int x, y;
-          decode_sample(J, s, pix, si, x, y);
-          PathRng rng = make_rng(J.seed, pix, si, J);
-          float sx, sy;
-          sample2d(rng, 0, sx, sy);  // SampleJob.cpp:49-53 pixel jitter
-          f3 o, d;
-          float tmin, tmax;
-          camera_ray(J, (float)x + sx, (float)y + sy, o, d, tmin, tmax);
-          P.ray_o[i] = make_float4(o.x, o.y, o.z, tmin);
-          P.ray_d[i] = make_float4(d.x, d.y, d.z, tmax);
-          P.T[i] = make_float4(1.f, 1.f, 1.f, 1.f);
-          P.F[i] = make_float4(1.f, 1.f, 1.f, 0.f);
-          P.L[i] = make_float4(0.f, 0.f, 0.f, 1.f);
-          P.pixel[i] = pix;
-          P.sample[i] = si;
-          P.flags[i] = ((uint32_t)j << JOB_SHIFT) | FL_EMIT;
-          P.state[i] = ST_ACTIVE;
-          st = ST_ACTIVE;
-          need = false;
-        }
+      if (lane == 0) s_cnt[wave] = (uint32_t)__popcll(mask);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        uint32_t tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+        s_base = tot ? atomicAdd((unsigned long long*)&J.next, (unsigned long long)tot) : 0ull;
       }
+      __syncthreads();
+      uint32_t woff = 0;
+      for (uint32_t w = 0; w < wave; w++) woff += s_cnt[w];
+      uint64_t s = s_base + woff + (uint64_t)__popcll(mask & ((1ull << lane) - 1ull));
+      if (need && s < J.total) {
+        uint32_t pix, si;
+        int x, y;
+        decode_sample(J, s, pix, si, x, y);
+        PathRng rng = make_rng(J.seed, pix, si, J);
+        float sx, sy;
+        sample2d(rng, 0, sx, sy);  // SampleJob.cpp:49-53 pixel jitter
+        f3 o, d;
+        float tmin, tmax;
+        camera_ray(J, (float)x + sx, (float)y + sy, o, d, tmin, tmax);
+        P.ray_o[i] = make_float4(o.x, o.y, o.z, tmin);
+        P.ray_d[i] = make_float4(d.x, d.y, d.z, tmax);
+        P.T[i] = make_float4(1.f, 1.f, 1.f, 1.f);
+        P.F[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+        P.L[i] = make_float4(0.f, 0.f, 0.f, 1.f);
+        P.pixel[i] = pix;
+        P.sample[i] = si;
+        P.flags[i] = ((uint32_t)j << JOB_SHIFT) | FL_EMIT;
+        P.state[i] = ST_ACTIVE;
+        st = ST_ACTIVE;
+        need = false;
+      }
+      __syncthreads();  // s_cnt / s_base are rewritten by the next job's round
     }
     bool act = valid && st == ST_ACTIVE;
     uint32_t pos = wave_append(&C[C_EXT], act);
@@ -147,11 +167,21 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, int njob
 }
 
 // ---------------------------------------------------------------- extend: closest hit + classify
+__device__ __forceinline__ TravStack make_stack(uint32_t* smem, uint2* gstack, int cap) {
+  TravStack s;
+  s.lref = smem + threadIdx.x;
+  s.lt = reinterpret_cast<float*>(smem + (size_t)kLdsStack * blockDim.x) + threadIdx.x;
+  s.lstride = blockDim.x;
+  s.gstride = gridDim.x * blockDim.x;
+  s.g = gstack + blockIdx.x * blockDim.x + threadIdx.x;
+  s.cap = cap;
+  return s;
+}
+
 __global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scenes, const DevJob* jobs,
-                                                int stack_depth) {
+                                                uint2* gstack, int stack_depth) {
   extern __shared__ uint32_t smem[];
-  uint32_t* stk_ref = smem + threadIdx.x;
-  float* stk_t = reinterpret_cast<float*>(smem + (size_t)stack_depth * blockDim.x) + threadIdx.x;
+  const TravStack stk = make_stack(smem, gstack, stack_depth);
   uint32_t* C = P.counters;
   if (blockIdx.x == 0 && threadIdx.x == 0) C[C_SHADOW] = 0;  // shade stage of this iteration fills it
   const uint32_t n = C[C_EXT];
@@ -175,7 +205,7 @@ __global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scene
       r.tmin = o.w;
       r.tmax = d.w;
       HitRec h;
-      traverse<false>(S, r, stk_ref, stk_t, blockDim.x, stack_depth, h, nv, np);
+      traverse<false>(S, r, stk, h, nv, np);
       P.hit[slot] = make_float4(h.t, h.b1, h.b2, __int_as_float(h.prim));
       if (h.prim < 0) {
         cls = 3;
@@ -382,10 +412,9 @@ __global__ void __launch_bounds__(256) k_shade_miss(DevPool P, const DevScene* s
 
 // ---------------------------------------------------------------- shadow rays (Integrator.cpp:228-235)
 __global__ void __launch_bounds__(128) k_shadow(DevPool P, const DevScene* scenes, const DevJob* jobs,
-                                                int stack_depth) {
+                                                uint2* gstack, int stack_depth) {
   extern __shared__ uint32_t smem[];
-  uint32_t* stk_ref = smem + threadIdx.x;
-  float* stk_t = reinterpret_cast<float*>(smem + (size_t)stack_depth * blockDim.x) + threadIdx.x;
+  const TravStack stk = make_stack(smem, gstack, stack_depth);
   uint32_t* C = P.counters;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     C[C_EXT] = 0;  // regen of the next iteration fills it
@@ -410,7 +439,7 @@ __global__ void __launch_bounds__(128) k_shadow(DevPool P, const DevScene* scene
       r.tmin = EPS;
       r.tmax = d.w;
       HitRec h;
-      bool occ = traverse<true>(S, r, stk_ref, stk_t, blockDim.x, stack_depth, h, nv, np);
+      bool occ = traverse<true>(S, r, stk, h, nv, np);
       if (!occ) {
         float4 c = P.sh_c[slot];
         float4 L = P.L[slot];
@@ -443,39 +472,37 @@ __global__ void k_fill_zero(uint32_t* p, size_t n) {
 // ---------------------------------------------------------------- parity taps
 __global__ void __launch_bounds__(128) k_trace_tap(const DevScene* scene, const float* rays, int n, int any_hit,
                                                    int32_t* prim, float* t, float* uv, float* p, float* nrm,
-                                                   int stack_depth) {
+                                                   uint2* gstack, int stack_depth) {
   extern __shared__ uint32_t smem[];
-  uint32_t* stk_ref = smem + threadIdx.x;
-  float* stk_t = reinterpret_cast<float*>(smem + (size_t)stack_depth * blockDim.x) + threadIdx.x;
+  const TravStack stk = make_stack(smem, gstack, stack_depth);
   const DevScene& S = *scene;
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const float* q = rays + 8 * (size_t)i;
-  Ray r;
-  r.o = mk(q[0], q[1], q[2]);
-  r.d = mk(q[3], q[4], q[5]);
-  r.inv = mk(1.f / q[3], 1.f / q[4], 1.f / q[5]);
-  r.tmin = q[6];
-  r.tmax = q[7];
-  HitRec h;
-  uint32_t nv = 0, np = 0;
-  bool hit = any_hit ? traverse<true>(S, r, stk_ref, stk_t, blockDim.x, stack_depth, h, nv, np)
-                     : traverse<false>(S, r, stk_ref, stk_t, blockDim.x, stack_depth, h, nv, np);
-  prim[i] = hit ? (int32_t)S.prim_info[h.prim].x : -1;
-  if (t) t[i] = hit ? h.t : 0.f;
-  if (uv) {
-    uv[2 * i] = hit ? h.b1 : 0.f;
-    uv[2 * i + 1] = hit ? h.b2 : 0.f;
-  }
-  if (p || nrm) {
-    f3 pp = mk(0.f, 0.f, 0.f), nn = mk(0.f, 0.f, 0.f);
-    if (hit && !any_hit) {
-      Surface sf = surface_at(S, h, r.o, r.d);
-      pp = sf.p;
-      nn = sf.sh.n;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const float* q = rays + 8 * (size_t)i;
+    Ray r;
+    r.o = mk(q[0], q[1], q[2]);
+    r.d = mk(q[3], q[4], q[5]);
+    r.inv = mk(1.f / q[3], 1.f / q[4], 1.f / q[5]);
+    r.tmin = q[6];
+    r.tmax = q[7];
+    HitRec h;
+    uint32_t nv = 0, np = 0;
+    bool hit = any_hit ? traverse<true>(S, r, stk, h, nv, np) : traverse<false>(S, r, stk, h, nv, np);
+    prim[i] = hit ? (int32_t)S.prim_info[h.prim].x : -1;
+    if (t) t[i] = hit ? h.t : 0.f;
+    if (uv) {
+      uv[2 * i] = hit ? h.b1 : 0.f;
+      uv[2 * i + 1] = hit ? h.b2 : 0.f;
     }
-    if (p) { p[3 * i] = pp.x; p[3 * i + 1] = pp.y; p[3 * i + 2] = pp.z; }
-    if (nrm) { nrm[3 * i] = nn.x; nrm[3 * i + 1] = nn.y; nrm[3 * i + 2] = nn.z; }
+    if (p || nrm) {
+      f3 pp = mk(0.f, 0.f, 0.f), nn = mk(0.f, 0.f, 0.f);
+      if (hit && !any_hit) {
+        Surface sf = surface_at(S, h, r.o, r.d);
+        pp = sf.p;
+        nn = sf.sh.n;
+      }
+      if (p) { p[3 * i] = pp.x; p[3 * i + 1] = pp.y; p[3 * i + 2] = pp.z; }
+      if (nrm) { nrm[3 * i] = nn.x; nrm[3 * i + 1] = nn.y; nrm[3 * i + 2] = nn.z; }
+    }
   }
 }
 
@@ -514,7 +541,7 @@ __global__ void k_bsdf_tap(const DevMaterial* mat, int op, const float* wi, cons
 }
 
 // ---------------------------------------------------------------- launchers
-static size_t stack_bytes(int block, int depth) { return (size_t)block * depth * 8u; }
+static size_t stack_bytes(int block) { return (size_t)block * kLdsStack * 8u; }
 
 void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, int njobs,
                       const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev) {
@@ -523,8 +550,8 @@ void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* j
   if (ev) (void)hipEventRecord(ev[0], stream);
   hipLaunchKernelGGL(k_regen, dim3(g), dim3(256), 0, stream, pool, jobs_dev, njobs);
   if (ev) (void)hipEventRecord(ev[1], stream);
-  hipLaunchKernelGGL(k_extend, dim3(g * 2), dim3(128), stack_bytes(128, cfg.stack), stream, pool, scenes_dev, cj,
-                     cfg.stack);
+  hipLaunchKernelGGL(k_extend, dim3(g * 2), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
+                     cfg.gstack, cfg.stack);
   if (ev) (void)hipEventRecord(ev[2], stream);
   hipLaunchKernelGGL(k_shade<0>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj);
   if (ev) (void)hipEventRecord(ev[3], stream);
@@ -534,8 +561,8 @@ void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* j
   if (ev) (void)hipEventRecord(ev[5], stream);
   hipLaunchKernelGGL(k_shade_miss, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj);
   if (ev) (void)hipEventRecord(ev[6], stream);
-  hipLaunchKernelGGL(k_shadow, dim3(g * 2), dim3(128), stack_bytes(128, cfg.stack), stream, pool, scenes_dev, cj,
-                     cfg.stack);
+  hipLaunchKernelGGL(k_shadow, dim3(g * 2), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
+                     cfg.gstack, cfg.stack);
   if (ev) (void)hipEventRecord(ev[7], stream);
 }
 
@@ -545,10 +572,10 @@ void launch_reset_pool(const DevPool& pool, hipStream_t stream) {
 
 void launch_trace_tap(const DevScene* scene_dev, const float* rays, int n, int any_hit, int32_t* prim, float* t,
                       float* uv, float* p, float* nrm, const LaunchCfg& cfg, hipStream_t stream) {
-  int blocks = (n + 127) / 128;
-  if (blocks == 0) return;
-  hipLaunchKernelGGL(k_trace_tap, dim3(blocks), dim3(128), stack_bytes(128, cfg.stack), stream, scene_dev, rays,
-                     n, any_hit, prim, t, uv, p, nrm, cfg.stack);
+  if (n == 0) return;
+  int blocks = std::min((n + 127) / 128, cfg.grid * 2);
+  hipLaunchKernelGGL(k_trace_tap, dim3(blocks), dim3(128), stack_bytes(128), stream, scene_dev, rays, n, any_hit,
+                     prim, t, uv, p, nrm, cfg.gstack, cfg.stack);
 }
 
 void launch_bsdf_tap(const DevMaterial* mat, int op, const float* wi, const float* wo_or_u, int n, float* out_dir,

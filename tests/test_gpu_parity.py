@@ -95,11 +95,23 @@ def test_bsdf_eval_pdf(ctx, mi):
     wi = _dirs(rng, n); wo = _dirs(rng, n)
     fg, pg = ctx.bsdf_eval(m, wi, wo)
     fo, po = o.bsdf_eval(0, wi, wo)
-    # eval/pdf use only + - * / sqrt: bit-exact; the 1e-5 relative bound is the stated tolerance
-    assert np.allclose(fg, fo, rtol=1e-5, atol=0, equal_nan=True)
-    assert np.allclose(pg, po, rtol=1e-5, atol=0, equal_nan=True)
-    exact = np.mean(fg.view(np.uint32) == fo.view(np.uint32))
-    print(f"  bit-exact fraction f={exact:.6f} pdf={np.mean(pg.view(np.uint32) == po.view(np.uint32)):.6f}")
+    # GGX eval/pdf use only + - * / sqrt (bit-exact); Beckmann's D adds exp(), a few ulp apart
+    # between the device and glibc and flushed to zero by the device below the normal range.
+    # Stated tolerance: 1e-5 relative wherever the CPU value is a normal float.
+    tiny = np.finfo(np.float32).tiny
+    nf = np.abs(fo) >= tiny
+    npd = np.abs(po) >= tiny
+    rel_f = np.abs(fg - fo)[nf] / np.abs(fo)[nf]
+    rel_p = np.abs(pg - po)[npd] / np.abs(po)[npd]
+    print(f"  max rel f {rel_f.max() if rel_f.size else 0:.3e} pdf {rel_p.max() if rel_p.size else 0:.3e}; "
+          f"bit-exact f={np.mean(fg.view(np.uint32) == fo.view(np.uint32)):.6f} "
+          f"pdf={np.mean(pg.view(np.uint32) == po.view(np.uint32)):.6f}; subnormal refs {np.sum(~nf)}")
+    assert np.all(rel_f <= 1e-5) and np.all(rel_p <= 1e-5)
+    assert np.all(np.abs(fg[~nf]) < 1e-30) and np.all(np.abs(pg[~npd]) < 1e-30)
+    assert np.array_equal(np.isnan(fg), np.isnan(fo)) and np.array_equal(np.isnan(pg), np.isnan(po))
+    if m.get("distribution", 0) == rtg.DIST_GGX or m["type"] == rtg.MAT_DIFFUSE:
+        assert np.array_equal(fg.view(np.uint32), fo.view(np.uint32))
+        assert np.array_equal(pg.view(np.uint32), po.view(np.uint32))
 
 
 @pytest.mark.parametrize("mi", range(len(MATS)))
