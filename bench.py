@@ -34,7 +34,10 @@ def parse():
     ap.add_argument("--spp", type=int, default=64)
     ap.add_argument("--size", type=int, default=1024)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-stride", type=int, default=8, help="CPU sample: every k-th 32x32 block")
+    ap.add_argument("--copy-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (scale reference for PMC byte counters)")
+    ap.add_argument("--cpu-stride", type=int, default=7,
+                    help="CPU sample: every k-th 32x32 block in creation order (7 is coprime with 32 blocks per row)")
     return ap.parse_args()
 
 
@@ -59,8 +62,23 @@ def cpu_baseline(sd, spp, stride):
     }
 
 
+def copy_reference(mb):
+    import torch
+
+    n = mb * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_reference_mib": mb, "bytes_read_per_copy": n * 4, "bytes_written_per_copy": n * 4}))
+
+
 def main():
     a = parse()
+    if a.copy_mb:
+        copy_reference(a.copy_mb)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU design is described, not built.",
               file=sys.stderr)

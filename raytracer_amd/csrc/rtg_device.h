@@ -191,6 +191,38 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t* counter, bool pred) {
   base = __shfl(base, (int)leader);
   return pred ? base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull)) : 0xffffffffu;
 }
+// Block-aggregated append into NQ queues at once: lane class c in [0, NQ) or -1. One atomic per
+// (block, queue). Every thread of the block must call it (it synchronises the block).
+// s: LDS scratch of NWAVES * NQ + NQ words.
+template <int NWAVES, int NQ>
+__device__ __forceinline__ uint32_t block_append(uint32_t* const* counters, int c, uint32_t* s) {
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  const unsigned long long lt = (1ull << lane) - 1ull;
+  unsigned long long mine = 0ull;
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    unsigned long long m = __ballot(c == q);
+    if (c == q) mine = m;
+    if (lane == 0) s[wave * NQ + q] = (uint32_t)__popcll(m);
+  }
+  __syncthreads();
+  if (threadIdx.x < NQ) {
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w = 0; w < NWAVES; w++) tot += s[w * NQ + threadIdx.x];
+    s[NWAVES * NQ + threadIdx.x] = tot ? atomicAdd(counters[threadIdx.x], tot) : 0u;
+  }
+  __syncthreads();
+  uint32_t pos = 0xffffffffu;
+  if (c >= 0) {
+    pos = s[NWAVES * NQ + c];
+    for (uint32_t w = 0; w < wave; w++) pos += s[w * NQ + c];
+    pos += (uint32_t)__popcll(mine & lt);
+  }
+  __syncthreads();
+  return pos;
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
   return v;
@@ -207,6 +239,7 @@ struct Ray {
   f3 o, d, inv;
   float tmin, tmax;
 };
+__device__ __forceinline__ bool has_zero_dir(const Ray& r) { return r.d.x == 0.f || r.d.y == 0.f || r.d.z == 0.f; }
 // accelStruct/Bvh.cpp:411-413 adaptive epsilon
 __device__ __forceinline__ void adapt_eps(Ray& r) {
   if (r.tmin == EPS) r.tmin = smax(r.tmin, r.tmin * smax(smax(fabsf(r.o.x), fabsf(r.o.y)), fabsf(r.o.z)));
@@ -238,6 +271,30 @@ __device__ __forceinline__ bool slab(const Ray& r, float bx0, float by0, float b
   nearT = nT;
   return (r.tmin <= fT && r.tmax >= nT);
 }
+// Same result as slab() when no direction component is 0: the per-axis early exits are folded
+// into one flag (nearT never decreases and farT never increases, so a failed axis stays failed;
+// the flag is taken per axis so NaN distances fail exactly where slab() fails).
+__device__ __forceinline__ bool slab_fast(const Ray& r, float bx0, float by0, float bz0, float bx1, float by1,
+                                          float bz1, float& nearT) {
+  float a1 = (bx0 - r.o.x) * r.inv.x, a2 = (bx1 - r.o.x) * r.inv.x;
+  float b1 = (by0 - r.o.y) * r.inv.y, b2 = (by1 - r.o.y) * r.inv.y;
+  float c1 = (bz0 - r.o.z) * r.inv.z, c2 = (bz1 - r.o.z) * r.inv.z;
+  bool sa = a1 > a2, sb = b1 > b2, sc = c1 > c2;
+  float alo = sa ? a2 : a1, ahi = sa ? a1 : a2;
+  float blo = sb ? b2 : b1, bhi = sb ? b1 : b2;
+  float clo = sc ? c2 : c1, chi = sc ? c1 : c2;
+  float nT = smax(alo, -MAX_REAL), fT = smin(ahi, MAX_REAL);
+  bool ok = nT <= fT;
+  nT = smax(blo, nT);
+  fT = smin(bhi, fT);
+  ok = ok && (nT <= fT);
+  nT = smax(clo, nT);
+  fT = smin(chi, fT);
+  ok = ok && (nT <= fT);
+  nearT = nT;
+  return ok && (r.tmin <= fT && r.tmax >= nT);
+}
+
 // geometricShape/Triangle.cpp:46-89
 __device__ __forceinline__ bool tri_hit(const Ray& r, const float* v, float& t, float& b1, float& b2) {
   f3 p0 = mk(v[0], v[1], v[2]);
@@ -336,14 +393,22 @@ __device__ __forceinline__ bool traverse(const DevScene& S, Ray r, const TravSta
   uint32_t ref = S.root_ref;
   int sp = 0;
   bool hit = false;
+  // rays with a zero direction component take the reference's per-axis branch (rare)
+  const bool zdir = has_zero_dir(r);
   while (true) {
     if (!(ref & kLeafBit)) {
       const float4* nd = S.nodes + 4u * ref;
       float4 a0 = nd[0], a1 = nd[1], c0 = nd[2], c1 = nd[3];
       nodes_visited++;
       float t0, t1;
-      bool h0 = slab(r, a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, t0);
-      bool h1 = slab(r, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, t1);
+      bool h0, h1;
+      if (!zdir) {
+        h0 = slab_fast(r, a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, t0);
+        h1 = slab_fast(r, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, t1);
+      } else {
+        h0 = slab(r, a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, t0);
+        h1 = slab(r, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, t1);
+      }
       uint32_t r0 = __float_as_uint(a1.z), r1 = __float_as_uint(c1.z);
       if (h0 && h1) {
         uint32_t nearR = r0, farR = r1;

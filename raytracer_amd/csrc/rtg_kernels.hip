@@ -58,10 +58,11 @@ __device__ __forceinline__ void splat(const DevJob& J, uint32_t pixel, float sx,
   for (int yy = y0; yy <= y; yy++)
     for (int xx = x0; xx <= x; xx++) {
       float4* a = J.accum + (size_t)yy * J.width + xx;
-      atomicAdd(&a->x, L.x);
-      atomicAdd(&a->y, L.y);
-      atomicAdd(&a->z, L.z);
-      atomicAdd(&a->w, 1.0f);
+      // hardware float atomics (the film is ordinary coarse-grained device memory)
+      unsafeAtomicAdd(&a->x, L.x);
+      unsafeAtomicAdd(&a->y, L.y);
+      unsafeAtomicAdd(&a->z, L.z);
+      unsafeAtomicAdd(&a->w, 1.0f);
     }
 }
 
@@ -70,6 +71,7 @@ __device__ __forceinline__ void splat(const DevJob& J, uint32_t pixel, float sx,
 // extension queue. Claims are aggregated per block (one 64-bit atomic per block and job), the
 // finished-sample count per wave.
 __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, int njobs) {
+  __shared__ uint32_t s_app[8];
   __shared__ uint32_t s_cnt[4];
   __shared__ unsigned long long s_base;
   uint32_t* C = P.counters;
@@ -160,9 +162,9 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, int njob
       __syncthreads();  // s_cnt / s_base are rewritten by the next job's round
     }
     bool act = valid && st == ST_ACTIVE;
-    uint32_t pos = wave_append(&C[C_EXT], act);
+    uint32_t* qc[1] = {&C[C_EXT]};
+    uint32_t pos = block_append<4, 1>(qc, act ? 0 : -1, s_app);
     if (act) P.q_ext[pos] = (uint32_t)i;
-    wave_append(&C[C_ACTIVE], act);
   }
 }
 
@@ -181,6 +183,7 @@ __device__ __forceinline__ TravStack make_stack(uint32_t* smem, uint2* gstack, i
 __global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scenes, const DevJob* jobs,
                                                 uint2* gstack, int stack_depth) {
   extern __shared__ uint32_t smem[];
+  __shared__ uint32_t s_app[12];
   const TravStack stk = make_stack(smem, gstack, stack_depth);
   uint32_t* C = P.counters;
   if (blockIdx.x == 0 && threadIdx.x == 0) C[C_SHADOW] = 0;  // shade stage of this iteration fills it
@@ -224,11 +227,10 @@ __global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scene
       }
     }
     // classify into the four shade queues (no-BSDF hits go with misses)
-    for (int q = 0; q < 4; q++) {
-      bool mine = valid && (cls == q || (q == 3 && cls == 4));
-      uint32_t pos = wave_append(&C[C_SHADE + q], mine);
-      if (mine) P.q_shade[(size_t)q * P.capacity + pos] = slot;
-    }
+    int q = valid ? (cls == 4 ? 3 : cls) : -1;
+    uint32_t* qc[4] = {&C[C_SHADE + 0], &C[C_SHADE + 1], &C[C_SHADE + 2], &C[C_SHADE + 3]};
+    uint32_t pos = block_append<2, 4>(qc, q, s_app);
+    if (q >= 0) P.q_shade[(size_t)q * P.capacity + pos] = slot;
   }
 }
 
@@ -254,6 +256,7 @@ __device__ __forceinline__ void prev_segment(const DevScene& S, f3 Le, float pdf
 
 template <int MT>
 __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes, const DevJob* jobs) {
+  __shared__ uint32_t s_app[8];
   uint32_t* C = P.counters;
   const uint32_t n = C[C_SHADE + MT];
   const uint32_t* q = P.q_shade + (size_t)MT * P.capacity;
@@ -359,7 +362,8 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
       P.flags[slot] = nf;
       P.state[slot] = done ? ST_DONE : ST_ACTIVE;
     }
-    uint32_t pos = wave_append(&C[C_SHADOW], shadow);
+    uint32_t* qc[1] = {&C[C_SHADOW]};
+    uint32_t pos = block_append<4, 1>(qc, shadow ? 0 : -1, s_app);
     if (shadow) P.q_shadow[pos] = slot;
   }
 }
@@ -416,10 +420,7 @@ __global__ void __launch_bounds__(128) k_shadow(DevPool P, const DevScene* scene
   extern __shared__ uint32_t smem[];
   const TravStack stk = make_stack(smem, gstack, stack_depth);
   uint32_t* C = P.counters;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    C[C_EXT] = 0;  // regen of the next iteration fills it
-    C[C_ACTIVE] = 0;
-  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) C[C_EXT] = 0;  // regen of the next iteration fills it
   const uint32_t n = C[C_SHADOW];
   const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t nIter = (n + stride - 1) / stride;

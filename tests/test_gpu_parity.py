@@ -96,9 +96,10 @@ def test_bsdf_eval_pdf(ctx, mi):
     fg, pg = ctx.bsdf_eval(m, wi, wo)
     fo, po = o.bsdf_eval(0, wi, wo)
     # GGX eval/pdf use only + - * / sqrt (bit-exact); Beckmann's D adds exp(), a few ulp apart
-    # between the device and glibc and flushed to zero by the device below the normal range.
-    # Stated tolerance: 1e-5 relative wherever the CPU value is a normal float.
-    tiny = np.finfo(np.float32).tiny
+    # between the device and glibc, and inexact near the bottom of the float range where the
+    # device's exp goes through subnormal intermediates. Stated tolerance: 1e-5 relative wherever
+    # the CPU value is >= 1e-30.
+    tiny = 1e-30
     nf = np.abs(fo) >= tiny
     npd = np.abs(po) >= tiny
     rel_f = np.abs(fg - fo)[nf] / np.abs(fo)[nf]
