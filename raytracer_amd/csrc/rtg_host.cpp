@@ -509,6 +509,8 @@ void ensure_pool(rtg_ctx* ctx, int capacity) {
   P.q_ext = (uint32_t*)A(n * 4);
   P.q_shade = (uint32_t*)A(n * 4 * 4);
   P.q_shadow = (uint32_t*)A(n * 4);
+  P.cls_q = (uint8_t*)A(n);
+  P.shflag = (uint8_t*)A(n);
   P.counters = (uint32_t*)A(C_COUNT * 4);
   ctx->pool = P;
 }

@@ -12,6 +12,7 @@ namespace rtg {
 // A child ref with bit 31 set is a leaf: bits 0..26 first primitive (leaf order),
 // bits 27..30 count-1 (1..16 primitives). Otherwise the ref is an interior node index.
 static const uint32_t kLeafBit = 0x80000000u;
+static const int kMaxJobsDev = 256;  // job table entries (the flags word holds 8 bits of job slot)
 static const int kMaxLeafPrims = 16;
 __host__ __device__ inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
   return kLeafBit | ((count - 1u) << 27) | first;
@@ -119,6 +120,8 @@ struct DevPool {
   uint32_t* q_ext;      // extension queue
   uint32_t* q_shade;    // 4 queues of capacity each: diffuse, conductor, dielectric, miss
   uint32_t* q_shadow;
+  uint8_t* cls_q;       // material class per extension-queue entry (0..2 BSDF type, 3 miss)
+  uint8_t* shflag;      // per slot: the shade stage produced a shadow ray
   uint32_t* counters;   // C_* below; u64 counters occupy two words
 };
 enum {

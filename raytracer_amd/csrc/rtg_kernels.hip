@@ -1,9 +1,12 @@
 // Wavefront MIS path tracer kernels for gfx950 (wave64).
 //
-// One iteration of the pool = regen -> extend(+classify) -> shade per material type -> shadow.
-// Slots keep per-path state in structure-of-arrays buffers; every stage consumes a compacted
-// queue built with wave ballots + one atomic per wave, so lanes of a wave run the same BSDF
-// code. The control flow restates integrator/PathTracingMIS.cpp:27-263 and
+// One iteration of the pool:
+//   regen -> extend -> classify -> shade (diffuse, conductor, dielectric, miss) -> compact -> shadow.
+// Slots keep per-path state in structure-of-arrays buffers. Every stage consumes a compacted
+// queue, so lanes of a wave run the same BSDF code. Queues are built by block-wide prefix sums
+// over 2048-entry chunks with ONE global atomic per chunk: same-address atomics are resolved
+// outside the eight XCDs' L2 caches and serialise, so they are kept to ~n/2048 per launch.
+// The control flow restates integrator/PathTracingMIS.cpp:27-263 and
 // integrator/Integrator.cpp:176-238 split at the two ray casts of each bounce.
 #include "rtg_device.h"
 
@@ -66,109 +69,159 @@ __device__ __forceinline__ void splat(const DevJob& J, uint32_t pixel, float sx,
     }
 }
 
-// ---------------------------------------------------------------- regen
-// Finalises finished paths into their job's film, claims new samples for free slots and builds the
-// extension queue. Claims are aggregated per block (one 64-bit atomic per block and job), the
-// finished-sample count per wave.
-__global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, int njobs) {
-  __shared__ uint32_t s_app[8];
-  __shared__ uint32_t s_cnt[4];
-  __shared__ unsigned long long s_base;
-  uint32_t* C = P.counters;
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
-    // shade queues are filled by the extend stage of this iteration
-    C[C_SHADE + 0] = 0; C[C_SHADE + 1] = 0; C[C_SHADE + 2] = 0; C[C_SHADE + 3] = 0;
+// ---------------------------------------------------------------- block primitives
+// Exclusive prefix sum over the block (NW waves); total over the block in `total`.
+template <int NW, typename T>
+__device__ __forceinline__ T block_excl_scan(T v, T* s_w, T& total) {
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  T x = v;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    T y = __shfl_up(x, off);
+    if (lane >= (uint32_t)off) x += y;
   }
-  const int cap = P.capacity;
-  const int stride = gridDim.x * blockDim.x;
-  const uint32_t lane = __lane_id();
-  const uint32_t wave = threadIdx.x >> 6;
-  // iterate whole blocks so ballots and barriers see every lane
-  int nIter = (cap + stride - 1) / stride;
-  for (int it = 0; it < nIter; it++) {
-    int i = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = i < cap;
-    uint32_t st = valid ? P.state[i] : ST_ACTIVE;
-    bool fin = valid && st == ST_DONE;
-    uint32_t fjob = 0;
-    if (fin) {
-      uint32_t f = P.flags[i];
-      fjob = fl_job(f);
-      const DevJob& J = jobs[fjob];
-      float4 L = P.L[i];
-      f3 l = mk(L.x, L.y, L.z);
-      bool nan = isnan(l.x) || isnan(l.y) || isnan(l.z);
-      if (nan) {
-        atomicAdd(&C[C_NAN], 1u);
-      } else {
-        uint32_t pix = P.pixel[i], si = P.sample[i];
-        PathRng rng = make_rng(J.seed, pix, si, J);
-        float sx, sy;
-        sample2d(rng, 0, sx, sy);
-        splat(J, pix, sx, sy, l);
-      }
-      st = ST_FREE;
-      P.state[i] = ST_FREE;
-    }
-    // finished-sample counts: one atomic per (wave, job) present in the wave
-    {
-      unsigned long long m = __ballot(fin);
-      while (m) {
-        uint32_t leader = (uint32_t)__builtin_ctzll(m);
-        uint32_t lj = (uint32_t)__shfl((int)fjob, (int)leader);
-        unsigned long long same = __ballot(fin && fjob == lj);
-        if (lane == leader) atomicAdd((unsigned long long*)&jobs[lj].done, (unsigned long long)__popcll(same));
-        m &= ~same;
-      }
-    }
-    bool need = valid && st == ST_FREE;
-    for (int j = 0; j < njobs; j++) {
-      if (!__syncthreads_or(need)) break;
-      DevJob& J = jobs[j];
-      if (!J.active) continue;
-      unsigned long long mask = __ballot(need);
-      if (lane == 0) s_cnt[wave] = (uint32_t)__popcll(mask);
-      __syncthreads();
-      if (threadIdx.x == 0) {
-        uint32_t tot = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-        s_base = tot ? atomicAdd((unsigned long long*)&J.next, (unsigned long long)tot) : 0ull;
-      }
-      __syncthreads();
-      uint32_t woff = 0;
-      for (uint32_t w = 0; w < wave; w++) woff += s_cnt[w];
-      uint64_t s = s_base + woff + (uint64_t)__popcll(mask & ((1ull << lane) - 1ull));
-      if (need && s < J.total) {
-        uint32_t pix, si;
-        int x, y;
-        decode_sample(J, s, pix, si, x, y);
-        PathRng rng = make_rng(J.seed, pix, si, J);
-        float sx, sy;
-        sample2d(rng, 0, sx, sy);  // SampleJob.cpp:49-53 pixel jitter
-        f3 o, d;
-        float tmin, tmax;
-        camera_ray(J, (float)x + sx, (float)y + sy, o, d, tmin, tmax);
-        P.ray_o[i] = make_float4(o.x, o.y, o.z, tmin);
-        P.ray_d[i] = make_float4(d.x, d.y, d.z, tmax);
-        P.T[i] = make_float4(1.f, 1.f, 1.f, 1.f);
-        P.F[i] = make_float4(1.f, 1.f, 1.f, 0.f);
-        P.L[i] = make_float4(0.f, 0.f, 0.f, 1.f);
-        P.pixel[i] = pix;
-        P.sample[i] = si;
-        P.flags[i] = ((uint32_t)j << JOB_SHIFT) | FL_EMIT;
-        P.state[i] = ST_ACTIVE;
-        st = ST_ACTIVE;
-        need = false;
-      }
-      __syncthreads();  // s_cnt / s_base are rewritten by the next job's round
-    }
-    bool act = valid && st == ST_ACTIVE;
-    uint32_t* qc[1] = {&C[C_EXT]};
-    uint32_t pos = block_append<4, 1>(qc, act ? 0 : -1, s_app);
-    if (act) P.q_ext[pos] = (uint32_t)i;
+  if (lane == 63) s_w[wave] = x;
+  __syncthreads();
+  T woff = 0, tot = 0;
+#pragma unroll
+  for (int w = 0; w < NW; w++) {
+    T c = s_w[w];
+    if ((uint32_t)w < wave) woff += c;
+    tot += c;
   }
+  __syncthreads();
+  total = tot;
+  return woff + x - v;
+}
+// One value per block: thread 0 reserves `total` entries of a queue and every thread learns the base.
+__device__ __forceinline__ uint32_t block_reserve(uint32_t* counter, uint32_t total, uint32_t* s_base) {
+  if (threadIdx.x == 0) *s_base = total ? atomicAdd(counter, total) : 0u;
+  __syncthreads();
+  uint32_t b = *s_base;
+  __syncthreads();
+  return b;
 }
 
-// ---------------------------------------------------------------- extend: closest hit + classify
+// ---------------------------------------------------------------- regen
+// Each block owns chunks of kRegenSlots consecutive slots (K per thread, strided by the block size).
+// It finalises finished paths into their job's film, claims new samples for its free slots (one
+// 64-bit atomic per chunk and job) and appends its active slots to the extension queue (one atomic
+// per chunk).
+constexpr int kRegenK = 8;
+constexpr int kRegenSlots = 256 * kRegenK;
+
+__global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, int njobs) {
+  __shared__ uint32_t s_w[4];
+  __shared__ uint32_t s_base;
+  __shared__ unsigned long long s_base64;
+  __shared__ uint32_t s_done[kMaxJobsDev];
+  __shared__ uint32_t s_nan;
+  uint32_t* C = P.counters;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // shade queues are filled by the classify stage of this iteration
+    C[C_SHADE + 0] = 0; C[C_SHADE + 1] = 0; C[C_SHADE + 2] = 0; C[C_SHADE + 3] = 0;
+  }
+  for (int j = threadIdx.x; j < njobs; j += blockDim.x) s_done[j] = 0;
+  if (threadIdx.x == 0) s_nan = 0;
+  __syncthreads();
+  const int cap = P.capacity;
+  const int nchunks = (cap + kRegenSlots - 1) / kRegenSlots;
+  for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const int base = chunk * kRegenSlots + threadIdx.x;
+    uint32_t need = 0;  // bit k: slot base + k*256 is free
+    uint32_t act = 0;   // bit k: slot is active after this kernel
+#pragma unroll 1
+    for (int k = 0; k < kRegenK; k++) {
+      int i = base + k * 256;
+      if (i >= cap) continue;
+      uint32_t st = P.state[i];
+      if (st == ST_DONE) {
+        uint32_t f = P.flags[i];
+        uint32_t jb = fl_job(f);
+        const DevJob& J = jobs[jb];
+        float4 L = P.L[i];
+        f3 l = mk(L.x, L.y, L.z);
+        if (isnan(l.x) || isnan(l.y) || isnan(l.z)) {
+          atomicAdd(&s_nan, 1u);
+        } else {
+          uint32_t pix = P.pixel[i], si = P.sample[i];
+          PathRng rng = make_rng(J.seed, pix, si, J);
+          float sx, sy;
+          sample2d(rng, 0, sx, sy);
+          splat(J, pix, sx, sy, l);
+        }
+        atomicAdd(&s_done[jb], 1u);
+        P.state[i] = ST_FREE;
+        st = ST_FREE;
+      }
+      if (st == ST_FREE) need |= 1u << k;
+      if (st == ST_ACTIVE) act |= 1u << k;
+    }
+    // claims, jobs in admission order
+    for (int j = 0; j < njobs; j++) {
+      uint32_t total;
+      uint32_t my = (uint32_t)__popc(need);
+      uint32_t off = block_excl_scan<4, uint32_t>(my, s_w, total);
+      if (total == 0) break;
+      DevJob& J = jobs[j];
+      if (!J.active) continue;
+      if (threadIdx.x == 0) {
+        unsigned long long nx = *(volatile unsigned long long*)&J.next;
+        s_base64 = (nx >= J.total) ? ~0ull : atomicAdd((unsigned long long*)&J.next, (unsigned long long)total);
+      }
+      __syncthreads();
+      unsigned long long b = s_base64;
+      __syncthreads();
+      if (b == ~0ull || b >= J.total) continue;
+      uint64_t got = J.total - b;
+      if (got > total) got = total;
+      uint32_t idx = off;
+#pragma unroll 1
+      for (int k = 0; k < kRegenK; k++) {
+        if (!(need & (1u << k))) continue;
+        if ((uint64_t)idx < got) {
+          int i = base + k * 256;
+          uint64_t s = b + idx;
+          uint32_t pix, si;
+          int x, y;
+          decode_sample(J, s, pix, si, x, y);
+          PathRng rng = make_rng(J.seed, pix, si, J);
+          float sx, sy;
+          sample2d(rng, 0, sx, sy);  // SampleJob.cpp:49-53 pixel jitter
+          f3 o, d;
+          float tmin, tmax;
+          camera_ray(J, (float)x + sx, (float)y + sy, o, d, tmin, tmax);
+          P.ray_o[i] = make_float4(o.x, o.y, o.z, tmin);
+          P.ray_d[i] = make_float4(d.x, d.y, d.z, tmax);
+          P.T[i] = make_float4(1.f, 1.f, 1.f, 1.f);
+          P.F[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+          P.L[i] = make_float4(0.f, 0.f, 0.f, 1.f);
+          P.pixel[i] = pix;
+          P.sample[i] = si;
+          P.flags[i] = ((uint32_t)j << JOB_SHIFT) | FL_EMIT;
+          P.state[i] = ST_ACTIVE;
+          need &= ~(1u << k);
+          act |= 1u << k;
+        }
+        idx++;
+      }
+    }
+    // extension queue
+    uint32_t total;
+    uint32_t off = block_excl_scan<4, uint32_t>((uint32_t)__popc(act), s_w, total);
+    uint32_t qb = block_reserve(&C[C_EXT], total, &s_base);
+    uint32_t pos = qb + off;
+#pragma unroll
+    for (int k = 0; k < kRegenK; k++)
+      if (act & (1u << k)) P.q_ext[pos++] = (uint32_t)(base + k * 256);
+  }
+  __syncthreads();
+  for (int j = threadIdx.x; j < njobs; j += blockDim.x)
+    if (s_done[j]) atomicAdd((unsigned long long*)&jobs[j].done, (unsigned long long)s_done[j]);
+  if (threadIdx.x == 0 && s_nan) atomicAdd(&C[C_NAN], s_nan);
+}
+
+// ---------------------------------------------------------------- extend: closest hit
 __device__ __forceinline__ TravStack make_stack(uint32_t* smem, uint2* gstack, int cap) {
   TravStack s;
   s.lref = smem + threadIdx.x;
@@ -180,57 +233,98 @@ __device__ __forceinline__ TravStack make_stack(uint32_t* smem, uint2* gstack, i
   return s;
 }
 
+// Work counters (counting mode): block sums, one atomic per block.
+__device__ __forceinline__ void add_work(uint32_t* C, int idx, uint32_t nv, uint32_t np, uint32_t* s_w) {
+  uint32_t a = wave_sum(nv), b = wave_sum(np);
+  uint32_t wave = threadIdx.x >> 6;
+  if (__lane_id() == 0) { s_w[2 * wave] = a; s_w[2 * wave + 1] = b; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long ta = 0, tb = 0;
+    for (uint32_t w = 0; w < blockDim.x / 64; w++) { ta += s_w[2 * w]; tb += s_w[2 * w + 1]; }
+    atomicAdd((unsigned long long*)&C[idx], ta);
+    atomicAdd((unsigned long long*)&C[idx + 2], tb);
+  }
+  __syncthreads();
+}
+
 __global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scenes, const DevJob* jobs,
                                                 uint2* gstack, int stack_depth) {
   extern __shared__ uint32_t smem[];
-  __shared__ uint32_t s_app[12];
+  __shared__ uint32_t s_w[4];
   const TravStack stk = make_stack(smem, gstack, stack_depth);
   uint32_t* C = P.counters;
-  if (blockIdx.x == 0 && threadIdx.x == 0) C[C_SHADOW] = 0;  // shade stage of this iteration fills it
   const uint32_t n = C[C_EXT];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    C[C_SHADOW] = 0;  // compact stage of this iteration fills it
+    atomicAdd((unsigned long long*)&C[C_EXTRAYS], (unsigned long long)n);
+  }
   const uint32_t stride = gridDim.x * blockDim.x;
-  uint32_t nIter = (n + stride - 1) / stride;
-  for (uint32_t it = 0; it < nIter; it++) {
-    uint32_t k = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = k < n;
-    int cls = -1;
-    uint32_t slot = 0;
-    uint32_t nv = 0, np = 0;
-    if (valid) {
-      slot = P.q_ext[k];
-      uint32_t f = P.flags[slot];
-      const DevScene& S = scenes[jobs[fl_job(f)].scene];
-      float4 o = P.ray_o[slot], d = P.ray_d[slot];
-      Ray r;
-      r.o = mk(o.x, o.y, o.z);
-      r.d = mk(d.x, d.y, d.z);
-      r.inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
-      r.tmin = o.w;
-      r.tmax = d.w;
-      HitRec h;
-      traverse<false>(S, r, stk, h, nv, np);
-      P.hit[slot] = make_float4(h.t, h.b1, h.b2, __int_as_float(h.prim));
-      if (h.prim < 0) {
-        cls = 3;
-      } else {
-        uint32_t m = S.prim_info[h.prim].y;
-        cls = (m == 0xffffffffu) ? 3 : S.mats[m].type;
-        if (m == 0xffffffffu) cls = 4;  // no BSDF: handled by the miss kernel as a terminator
+  uint32_t nv = 0, np = 0;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    uint32_t slot = P.q_ext[k];
+    uint32_t f = P.flags[slot];
+    const DevScene& S = scenes[jobs[fl_job(f)].scene];
+    float4 o = P.ray_o[slot], d = P.ray_d[slot];
+    Ray r;
+    r.o = mk(o.x, o.y, o.z);
+    r.d = mk(d.x, d.y, d.z);
+    r.inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    r.tmin = o.w;
+    r.tmax = d.w;
+    HitRec h;
+    traverse<false>(S, r, stk, h, nv, np);
+    P.hit[slot] = make_float4(h.t, h.b1, h.b2, __int_as_float(h.prim));
+    // material class for the shade queues: 0..2 BSDF type, 3 miss or no BSDF
+    uint8_t cls = 3;
+    if (h.prim >= 0) {
+      uint32_t m = S.prim_info[h.prim].y;
+      if (m != 0xffffffffu) cls = (uint8_t)S.mats[m].type;
+    }
+    P.cls_q[k] = cls;
+  }
+  if (P.count_work) add_work(C, C_NODES, nv, np, s_w);
+}
+
+// ---------------------------------------------------------------- classify: shade queues
+// Chunks of 2048 extension-queue entries per block; the four per-class counts are packed into
+// 16-bit fields of one 64-bit word so a single block scan serves all four queues.
+__global__ void __launch_bounds__(256) k_classify(DevPool P) {
+  __shared__ unsigned long long s_w[4];
+  __shared__ uint32_t s_base[4];
+  uint32_t* C = P.counters;
+  const uint32_t n = C[C_EXT];
+  const uint32_t nchunks = (n + kRegenSlots - 1) / kRegenSlots;
+  for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint32_t base = chunk * kRegenSlots + threadIdx.x;
+    uint32_t cls[kRegenK];
+    unsigned long long cnt = 0;
+#pragma unroll
+    for (int k = 0; k < kRegenK; k++) {
+      uint32_t e = base + k * 256;
+      cls[k] = e < n ? (uint32_t)P.cls_q[e] : 4u;
+      if (cls[k] < 4) cnt += 1ull << (16 * cls[k]);
+    }
+    unsigned long long total;
+    unsigned long long off = block_excl_scan<4, unsigned long long>(cnt, s_w, total);
+    if (threadIdx.x < 4) {
+      uint32_t t = (uint32_t)((total >> (16 * threadIdx.x)) & 0xFFFFull);
+      s_base[threadIdx.x] = t ? atomicAdd(&C[C_SHADE + threadIdx.x], t) : 0u;
+    }
+    __syncthreads();
+    uint32_t pos[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) pos[q] = s_base[q] + (uint32_t)((off >> (16 * q)) & 0xFFFFull);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kRegenK; k++) {
+      uint32_t c = cls[k];
+      if (c < 4) {
+        uint32_t e = base + k * 256;
+        P.q_shade[(size_t)c * P.capacity + pos[c]] = P.q_ext[e];
+        pos[c]++;
       }
     }
-    wave_count64((unsigned long long*)&C[C_EXTRAYS], valid);
-    if (P.count_work) {
-      uint32_t a = wave_sum(nv), b = wave_sum(np);
-      if (__lane_id() == 0) {
-        atomicAdd((unsigned long long*)&C[C_NODES], (unsigned long long)a);
-        atomicAdd((unsigned long long*)&C[C_PRIMS], (unsigned long long)b);
-      }
-    }
-    // classify into the four shade queues (no-BSDF hits go with misses)
-    int q = valid ? (cls == 4 ? 3 : cls) : -1;
-    uint32_t* qc[4] = {&C[C_SHADE + 0], &C[C_SHADE + 1], &C[C_SHADE + 2], &C[C_SHADE + 3]};
-    uint32_t pos = block_append<2, 4>(qc, q, s_app);
-    if (q >= 0) P.q_shade[(size_t)q * P.capacity + pos] = slot;
   }
 }
 
@@ -256,19 +350,14 @@ __device__ __forceinline__ void prev_segment(const DevScene& S, f3 Le, float pdf
 
 template <int MT>
 __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes, const DevJob* jobs) {
-  __shared__ uint32_t s_app[8];
   uint32_t* C = P.counters;
   const uint32_t n = C[C_SHADE + MT];
   const uint32_t* q = P.q_shade + (size_t)MT * P.capacity;
   const uint32_t stride = gridDim.x * blockDim.x;
-  uint32_t nIter = (n + stride - 1) / stride;
-  for (uint32_t it = 0; it < nIter; it++) {
-    uint32_t k = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = k < n;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
     bool shadow = false;
-    uint32_t slot = 0;
-    if (valid) {
-      slot = q[k];
+    uint32_t slot = q[k];
+    {
       uint32_t f = P.flags[slot];
       uint32_t depth = fl_depth(f);
       const DevJob& J = jobs[fl_job(f)];
@@ -362,9 +451,7 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
       P.flags[slot] = nf;
       P.state[slot] = done ? ST_DONE : ST_ACTIVE;
     }
-    uint32_t* qc[1] = {&C[C_SHADOW]};
-    uint32_t pos = block_append<4, 1>(qc, shadow ? 0 : -1, s_app);
-    if (shadow) P.q_shadow[pos] = slot;
+    P.shflag[slot] = shadow ? 1 : 0;
   }
 }
 
@@ -414,49 +501,70 @@ __global__ void __launch_bounds__(256) k_shade_miss(DevPool P, const DevScene* s
   }
 }
 
+// ---------------------------------------------------------------- compact: shadow queue
+__global__ void __launch_bounds__(256) k_compact_shadow(DevPool P) {
+  __shared__ uint32_t s_w[4];
+  __shared__ uint32_t s_base;
+  uint32_t* C = P.counters;
+  const uint32_t n = C[C_EXT];
+  const uint32_t nchunks = (n + kRegenSlots - 1) / kRegenSlots;
+  for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint32_t base = chunk * kRegenSlots + threadIdx.x;
+    uint32_t bits = 0;
+    uint32_t slots[kRegenK];
+#pragma unroll
+    for (int k = 0; k < kRegenK; k++) {
+      uint32_t e = base + k * 256;
+      slots[k] = 0;
+      if (e < n && P.cls_q[e] < 3) {
+        slots[k] = P.q_ext[e];
+        if (P.shflag[slots[k]]) bits |= 1u << k;
+      }
+    }
+    uint32_t total;
+    uint32_t off = block_excl_scan<4, uint32_t>((uint32_t)__popc(bits), s_w, total);
+    uint32_t pos = block_reserve(&C[C_SHADOW], total, &s_base) + off;
+#pragma unroll
+    for (int k = 0; k < kRegenK; k++)
+      if (bits & (1u << k)) P.q_shadow[pos++] = slots[k];
+  }
+}
+
 // ---------------------------------------------------------------- shadow rays (Integrator.cpp:228-235)
 __global__ void __launch_bounds__(128) k_shadow(DevPool P, const DevScene* scenes, const DevJob* jobs,
                                                 uint2* gstack, int stack_depth) {
   extern __shared__ uint32_t smem[];
+  __shared__ uint32_t s_w[4];
   const TravStack stk = make_stack(smem, gstack, stack_depth);
   uint32_t* C = P.counters;
-  if (blockIdx.x == 0 && threadIdx.x == 0) C[C_EXT] = 0;  // regen of the next iteration fills it
   const uint32_t n = C[C_SHADOW];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    C[C_EXT] = 0;  // regen of the next iteration fills it
+    atomicAdd((unsigned long long*)&C[C_SHRAYS], (unsigned long long)n);
+  }
   const uint32_t stride = gridDim.x * blockDim.x;
-  uint32_t nIter = (n + stride - 1) / stride;
-  for (uint32_t it = 0; it < nIter; it++) {
-    uint32_t k = it * stride + blockIdx.x * blockDim.x + threadIdx.x;
-    bool valid = k < n;
-    uint32_t nv = 0, np = 0;
-    if (valid) {
-      uint32_t slot = P.q_shadow[k];
-      uint32_t f = P.flags[slot];
-      const DevScene& S = scenes[jobs[fl_job(f)].scene];
-      float4 o = P.ray_o[slot], d = P.sh_d[slot];
-      Ray r;
-      r.o = mk(o.x, o.y, o.z);
-      r.d = mk(d.x, d.y, d.z);
-      r.inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
-      r.tmin = EPS;
-      r.tmax = d.w;
-      HitRec h;
-      bool occ = traverse<true>(S, r, stk, h, nv, np);
-      if (!occ) {
-        float4 c = P.sh_c[slot];
-        float4 L = P.L[slot];
-        L.x += c.x; L.y += c.y; L.z += c.z;
-        P.L[slot] = L;
-      }
-    }
-    wave_count64((unsigned long long*)&C[C_SHRAYS], valid);
-    if (P.count_work) {
-      uint32_t a = wave_sum(nv), b = wave_sum(np);
-      if (__lane_id() == 0) {
-        atomicAdd((unsigned long long*)&C[C_SHNODES], (unsigned long long)a);
-        atomicAdd((unsigned long long*)&C[C_SHPRIMS], (unsigned long long)b);
-      }
+  uint32_t nv = 0, np = 0;
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
+    uint32_t slot = P.q_shadow[k];
+    uint32_t f = P.flags[slot];
+    const DevScene& S = scenes[jobs[fl_job(f)].scene];
+    float4 o = P.ray_o[slot], d = P.sh_d[slot];
+    Ray r;
+    r.o = mk(o.x, o.y, o.z);
+    r.d = mk(d.x, d.y, d.z);
+    r.inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    r.tmin = EPS;
+    r.tmax = d.w;
+    HitRec h;
+    bool occ = traverse<true>(S, r, stk, h, nv, np);
+    if (!occ) {
+      float4 c = P.sh_c[slot];
+      float4 L = P.L[slot];
+      L.x += c.x; L.y += c.y; L.z += c.z;
+      P.L[slot] = L;
     }
   }
+  if (P.count_work) add_work(C, C_SHNODES, nv, np, s_w);
 }
 
 __global__ void k_reset_pool(DevPool P) {
@@ -546,13 +654,15 @@ static size_t stack_bytes(int block) { return (size_t)block * kLdsStack * 8u; }
 
 void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, int njobs,
                       const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev) {
-  int g = cfg.grid;
+  const int g = cfg.grid;
   const DevJob* cj = jobs_dev;
+  const int chunks = (pool.capacity + kRegenSlots - 1) / kRegenSlots;
   if (ev) (void)hipEventRecord(ev[0], stream);
-  hipLaunchKernelGGL(k_regen, dim3(g), dim3(256), 0, stream, pool, jobs_dev, njobs);
+  hipLaunchKernelGGL(k_regen, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, njobs);
   if (ev) (void)hipEventRecord(ev[1], stream);
   hipLaunchKernelGGL(k_extend, dim3(g * 2), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
                      cfg.gstack, cfg.stack);
+  hipLaunchKernelGGL(k_classify, dim3(chunks), dim3(256), 0, stream, pool);
   if (ev) (void)hipEventRecord(ev[2], stream);
   hipLaunchKernelGGL(k_shade<0>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj);
   if (ev) (void)hipEventRecord(ev[3], stream);
@@ -562,6 +672,7 @@ void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* j
   if (ev) (void)hipEventRecord(ev[5], stream);
   hipLaunchKernelGGL(k_shade_miss, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj);
   if (ev) (void)hipEventRecord(ev[6], stream);
+  hipLaunchKernelGGL(k_compact_shadow, dim3(chunks), dim3(256), 0, stream, pool);
   hipLaunchKernelGGL(k_shadow, dim3(g * 2), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
                      cfg.gstack, cfg.stack);
   if (ev) (void)hipEventRecord(ev[7], stream);
