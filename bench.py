@@ -36,6 +36,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (scale reference for PMC byte counters)")
+    ap.add_argument("--serving", action="store_true",
+                    help="also run the serving-mix leg (off by default until its GPU run is re-validated)")
+    ap.add_argument("--serve-requests", type=int, default=48,
+                    help="serving leg: requests of the Poisson stream (lambda 40/s, seed 4)")
     ap.add_argument("--cpu-stride", type=int, default=7,
                     help="CPU sample: every k-th 32x32 block in creation order (7 is coprime with 32 blocks per row)")
     return ap.parse_args()
@@ -59,6 +63,69 @@ def cpu_baseline(sd, spp, stride):
         "sample": f"every {stride}th 32x32 block of the same 1024x1024 render ({nblocks} blocks, {samples} samples, "
                   f"{dt:.1f} s), reference schedule (blocks seeded (x, y), LIFO pop)",
         "rays_per_s": float(st[0]) / dt,
+    }
+
+
+def serving(ctx, big, n_req, lam=40.0, seed=4):
+    """Serving mix: a Poisson stream (lambda req/s) of requests, each drawing a scene uniformly from
+    {Cornell box, large displaced sphere, glossy Cornell}, a resolution from {128,256,512,1024}^2
+    and spp from {4,16,64,256}. A producer thread submits at the arrival times while this thread
+    steps the loop; latency = submit to finish. The heterogeneous-volume scene of the same mix is not
+    part of this renderer, so the glossy Cornell variant takes its place."""
+    import threading
+
+    import numpy as np
+    from raytracer_amd import rtg, scenes
+
+    sds = [scenes.cornell(), None, scenes.glossy_cornell()]
+    resident = [ctx.upload(sds[0]), big[1], ctx.upload(sds[2])]
+    sds[1] = big[0]
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0 / lam, n_req)
+    kinds = rng.integers(0, 3, n_req)
+    sizes = rng.choice([128, 256, 512, 1024], n_req)
+    spps = rng.choice([4, 16, 64, 256], n_req)
+    reqs = []
+    for g, k, sz, spp in zip(gaps, kinds, sizes, spps):
+        sd = sds[int(k)]
+        reqs.append((float(g), int(k), dict(sd.camera, width=int(sz), height=int(sz)),
+                     dict(sd.render, spp=int(spp), seed=int(rng.integers(1, 1 << 30)))))
+    offered = sum(c["width"] * c["height"] * r["spp"] for _, _, c, r in reqs)
+    with rtg.Server(ctx) as srv:
+        ids = []
+
+        def producer():
+            t = time.perf_counter()
+            for g, k, cam, rp in reqs:
+                t += g
+                dt = t - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                ids.append(srv.submit(resident[k], cam, rp))
+
+        t0 = time.perf_counter()
+        th = threading.Thread(target=producer)
+        th.start()
+        while True:
+            alive = th.is_alive()
+            srv.step(max_ms=5.0)
+            st = srv.stats()
+            if not alive and st["running"] == 0 and st["pending"] == 0:
+                break
+        th.join()
+        makespan = time.perf_counter() - t0
+        lat = np.array([srv.poll(j, want_image=False)[2] for j in ids])
+        st = srv.stats()
+    for r in (resident[0], resident[2]):
+        r.release()
+    return {
+        "requests": n_req, "lambda_per_s": lam, "seed": seed,
+        "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+        "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
+        "samples_per_s": st["samples"] / makespan, "makespan_s": makespan,
+        "offered_samples_per_s": offered * lam / n_req,
+        "max_running": st["max_running"], "film_bytes_peak": st["film_bytes_peak"],
+        "completed": st["completed"],
     }
 
 
@@ -87,6 +154,10 @@ def main():
     import torch
     from raytracer_amd import rtg, scenes
 
+    def progress(msg):
+        print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    progress("building the displaced-sphere scene")
     sd = scenes.displaced_sphere(width=a.size, height=a.size, spp=a.spp)
     ctx = rtg.Context(0)
     scene = ctx.upload(sd)
@@ -95,6 +166,7 @@ def main():
     film = torch.zeros(H * W * 4, dtype=torch.float32, device="cuda")
 
     # one counted render: traversal work per launch (not timed)
+    progress("counted render")
     ctx.set_profiling(rtg.PROFILE_COUNT)
     film.zero_()
     st = scene.render_device(cam, rp, film.data_ptr())
@@ -107,6 +179,7 @@ def main():
         film.zero_()
         scene.render_device(cam, rp, film.data_ptr())
     torch.cuda.synchronize()
+    progress(f"timing {a.steps} steps")
     stage_ms = np.zeros(7)
     launches = np.zeros(7)
     dev_ms = 0.0
@@ -167,7 +240,11 @@ def main():
                         "prims": counted["prims"] / max(1, counted["ext_rays"])},
         },
     }
+    if a.serving:
+        progress(f"serving mix ({a.serve_requests} requests)")
+        out["serving"] = serving(ctx, (sd, scene), a.serve_requests)
     if not a.no_cpu:
+        progress("cpu baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(sd, rp["spp"], a.cpu_stride)
         out["speedup_vs_cpu"] = value / out["cpu_baseline"]["value"]
     print(json.dumps(out), flush=True)

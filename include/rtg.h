@@ -10,7 +10,7 @@
  *
  * Conventions: plain C types only; every call returns RTG_OK (0) or a negative
  * error code and never throws; rtg_last_error() gives the message of the last
- * failing call on a context. Host pointers unless a name says _device.
+ * failing call made by the calling thread. Host pointers unless a name says _device.
  * Primitive ids: triangles of mesh 0, mesh 1, ... in index order, then spheres.
  */
 #ifndef RTG_H
@@ -178,19 +178,42 @@ int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int3
 int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const float* wi, const float* wo_or_u,
                    int32_t n, float* out_dir, float* f, float* pdf, float* rel_eta);
 
-/* Continuous-batching serving (no reference counterpart). Requests of any
- * resident scene are admitted as pool slots and HBM budget free up; their
- * paths share the wavefront queues. */
+/* Continuous-batching serving (no reference counterpart; for a host that renders many requests
+ * on one GPU, e.g. an interactive preview service). Requests of any resident scene are admitted
+ * in arrival order while a job slot (256) and the HBM budget for their FP32 films allow; every
+ * wavefront iteration hands free path slots to the admitted requests, shortest remaining work
+ * first, so their paths share the queues. A request's image equals its rtg_render image for the
+ * same seed up to the order of the film's float additions. While a server exists on a context,
+ * rtg_render / rtg_render_device on that context fail with RTG_ERR_INVALID.
+ * hbm_budget_bytes: film bytes that may be resident at once (0 = 8 GiB); a request whose film
+ * alone exceeds it is cancelled at admission. */
 typedef struct rtg_server rtg_server;
 int rtg_server_create(rtg_ctx* ctx, uint64_t hbm_budget_bytes, rtg_server** out);
 int rtg_server_destroy(rtg_server* srv);
-/* Enqueue a request; returns its id in *job_id. Does not block. */
+/* Enqueue a request (validated now); returns its id in *job_id. Does not block. */
 int rtg_submit(rtg_server* srv, rtg_scene* scene, const rtg_render_desc* desc, int64_t* job_id);
-/* Run the device loop until no request is pending or max_ms elapsed (0 = until idle). */
+/* Run the device loop until no request is pending or running, or max_ms elapsed (0 = until idle).
+ * Requests submitted from another thread between calls join at the next iteration batch. */
 int rtg_server_step(rtg_server* srv, double max_ms);
-/* RTG_OK when finished (copies the image if out_rgbw != NULL), RTG_ERR_NOT_READY otherwise. */
+/* RTG_OK when finished (copies width*height*4 floats of (rgb sum, weight) if out_rgbw != NULL),
+ * RTG_ERR_NOT_READY while queued or running, RTG_ERR_CANCELLED after a cancel. *latency_ms =
+ * submit-to-finish wall time. */
 int rtg_poll(rtg_server* srv, int64_t job_id, float* out_rgbw, double* latency_ms);
+/* Stop claiming new samples for the request; its paths in flight drain, then it ends cancelled. */
 int rtg_cancel(rtg_server* srv, int64_t job_id);
+
+typedef struct rtg_server_stats {
+  uint64_t submitted, admitted, completed, cancelled;
+  uint64_t iterations;          /* wavefront iterations run by the loop */
+  uint64_t samples;             /* camera samples of finished requests */
+  uint64_t film_bytes;          /* film bytes resident now */
+  uint64_t film_bytes_peak;     /* most film bytes resident at once */
+  int32_t running;              /* requests admitted and not finished */
+  int32_t pending;              /* requests waiting for admission */
+  int32_t max_running;          /* most requests sharing one iteration */
+  int32_t reserved;
+} rtg_server_stats;
+int rtg_server_query(rtg_server* srv, rtg_server_stats* out);
 
 #ifdef __cplusplus
 }

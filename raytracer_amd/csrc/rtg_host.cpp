@@ -476,6 +476,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
 }
 
 void release_scene(rtg_ctx* ctx, rtg_scene* s) {
+  if (s->users > 0) fail(RTG_ERR_INVALID, "scene is used by queued or running requests");
   (void)hipStreamSynchronize(ctx->stream);
   for (void* p : s->owned) (void)hipFree(p);
   s->owned.clear();
@@ -520,6 +521,15 @@ int stack_for_resident(rtg_ctx* ctx) {
   for (auto* sc : ctx->scene_slots)
     if (sc) s = std::max(s, sc->stack);
   return s;
+}
+
+LaunchCfg launch_cfg(rtg_ctx* ctx) {
+  LaunchCfg cfg;
+  cfg.grid = ctx->grid;
+  cfg.block = 256;
+  cfg.stack = stack_for_resident(ctx);
+  cfg.gstack = ctx->gstack;
+  return cfg;
 }
 
 DevJob make_job(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc& d, float4* accum) {
@@ -568,15 +578,22 @@ DevJob make_job(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc& d, float4
 }  // namespace rtg
 
 // ==================================================================== C ABI
+namespace rtg {
+// Error messages are per calling thread: a failing call on one thread never races another
+// thread's message (the serving entry points run concurrently with a step).
+static thread_local std::string t_err;
+void set_error(const char* msg) { t_err = msg; }
+}  // namespace rtg
+
 #define RTG_GUARD(ctx, body)                                   \
   try {                                                        \
     body;                                                      \
     return RTG_OK;                                             \
   } catch (const RtgError& e) {                                \
-    if (ctx) (ctx)->err = e.what();                            \
+    set_error(e.what());                                       \
     return e.code;                                             \
   } catch (const std::exception& e) {                          \
-    if (ctx) (ctx)->err = e.what();                            \
+    set_error(e.what());                                       \
     return RTG_ERR_INVALID;                                    \
   }
 
@@ -600,6 +617,7 @@ int rtg_ctx_create(int device, rtg_ctx** out) {
     if (ctx->arch.rfind("gfx950", 0) != 0) fail(RTG_ERR_INVALID, "librtg is built for gfx950, device is " + ctx->arch);
     hip_check(hipMalloc(&ctx->scenes_dev, sizeof(DevScene) * kMaxScenes), "hipMalloc scenes");
     hip_check(hipMalloc(&ctx->jobs_dev, sizeof(DevJob) * kMaxJobs), "hipMalloc jobs");
+    hip_check(hipMalloc(&ctx->order_dev, sizeof(int32_t) * kMaxJobs), "hipMalloc job order");
     hip_check(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault), "hipHostMalloc");
     ctx->scene_slots.assign(kMaxScenes, nullptr);
     ctx->pool_slots = 1 << 21;
@@ -618,14 +636,21 @@ int rtg_ctx_create(int device, rtg_ctx** out) {
 
 int rtg_ctx_destroy(rtg_ctx* ctx) {
   if (!ctx) return RTG_ERR_INVALID;
+  if (ctx->server) (void)rtg_server_destroy(ctx->server);  // drops its requests' scene references
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto* s : ctx->scene_slots)
-    if (s) release_scene(ctx, s);
+    if (s) {
+      try {
+        release_scene(ctx, s);
+      } catch (const std::exception&) {
+      }
+    }
   for (void* p : ctx->pool_owned) (void)hipFree(p);
   if (ctx->scenes_dev) (void)hipFree(ctx->scenes_dev);
   if (ctx->jobs_dev) (void)hipFree(ctx->jobs_dev);
   if (ctx->gstack) (void)hipFree(ctx->gstack);
+  if (ctx->order_dev) (void)hipFree(ctx->order_dev);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   for (void* p : ctx->scratch_owned) (void)hipFree(p);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
@@ -634,7 +659,10 @@ int rtg_ctx_destroy(rtg_ctx* ctx) {
   return RTG_OK;
 }
 
-const char* rtg_last_error(rtg_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char* rtg_last_error(rtg_ctx* ctx) {
+  (void)ctx;
+  return t_err.c_str();
+}
 
 int rtg_ctx_set_pool(rtg_ctx* ctx, int32_t slots) {
   if (!ctx) return RTG_ERR_INVALID;
@@ -673,17 +701,21 @@ int rtg_scene_bytes(rtg_scene* scene, uint64_t* bytes) {
 
 static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, float* out_dev, rtg_stats* st) {
   if (!scene || !desc) fail(RTG_ERR_INVALID, "null scene or render description");
+  if (ctx->server) fail(RTG_ERR_INVALID, "the context is serving requests; submit the render instead");
   hip_check(hipSetDevice(ctx->device), "hipSetDevice");
   DevJob J = make_job(ctx, scene, *desc, (float4*)out_dev);
+  // sources of the asynchronous uploads sit in the context's pinned page (offsets 64 and 256)
+  int32_t* up_order = (int32_t*)((char*)ctx->pinned + 64);
+  DevJob* up_job = (DevJob*)((char*)ctx->pinned + 256);
+  static_assert(256 + sizeof(DevJob) <= 4096, "pinned staging page too small");
+  *up_order = 0;
+  *up_job = J;
+  hip_check(hipMemcpyAsync(ctx->order_dev, up_order, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream), "order");
   ensure_pool(ctx, ctx->pool_slots);
   ctx->pool.count_work = (ctx->profiling & RTG_PROFILE_COUNT) ? 1 : 0;
-  LaunchCfg cfg;
-  cfg.grid = ctx->grid;
-  cfg.block = 256;
-  cfg.stack = stack_for_resident(ctx);
-  cfg.gstack = ctx->gstack;
+  LaunchCfg cfg = launch_cfg(ctx);
   launch_reset_pool(ctx->pool, ctx->stream);
-  hip_check(hipMemcpyAsync(ctx->jobs_dev, &J, sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream), "job upload");
+  hip_check(hipMemcpyAsync(ctx->jobs_dev, up_job, sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream), "job upload");
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "event");
   hip_check(hipEventCreate(&e1), "event");
@@ -701,7 +733,7 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
       for (size_t i = have; i < ctx->events.size(); i++) hip_check(hipEventCreate(&ctx->events[i]), "event");
     }
     for (int b = 0; b < batch; b++)
-      launch_iteration(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, 1, cfg, ctx->stream,
+      launch_iteration(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, 1, cfg, ctx->stream,
                        prof ? &ctx->events[(size_t)b * 8] : nullptr);
     iters += batch;
     hip_check(hipGetLastError(), "kernel launch");

@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -21,6 +22,7 @@ struct RtgError : std::runtime_error {
 };
 [[noreturn]] void fail(int code, const std::string& msg);
 void hip_check(hipError_t e, const char* what);
+void set_error(const char* msg);
 
 }  // namespace rtg
 
@@ -32,12 +34,12 @@ struct rtg_scene {
   uint64_t bytes = 0;
   int stack = 16;
   int ntri = 0, nsph = 0;
+  std::atomic<int> users{0};  // requests of the serving loop that reference this scene
 };
 
 struct rtg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
-  std::string err;
   std::string arch;
   int cu_count = 0;
   int grid = 1024;
@@ -50,6 +52,8 @@ struct rtg_ctx {
   rtg::DevScene* scenes_dev = nullptr;
   rtg::DevJob* jobs_dev = nullptr;
   uint2* gstack = nullptr;
+  int32_t* order_dev = nullptr;    // claim order of job-table slots
+  rtg_server* server = nullptr;    // the serving loop owning the pool, if any
   std::vector<rtg_scene*> scene_slots;
   void* pinned = nullptr;
   std::recursive_mutex mtx_r;
@@ -62,6 +66,7 @@ void release_scene(rtg_ctx* ctx, rtg_scene* s);
 void ensure_pool(rtg_ctx* ctx, int capacity);
 int stack_for_resident(rtg_ctx* ctx);
 DevJob make_job(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc& d, float4* accum);
+LaunchCfg launch_cfg(rtg_ctx* ctx);
 void camera_matrices(const rtg_render_desc& d, float* s2c, float* c2w);
 void* dev_alloc(rtg_ctx* ctx, size_t bytes, std::vector<void*>& owned);
 }  // namespace rtg

@@ -130,6 +130,39 @@ enum {
   C_COUNT = 24
 };
 
+// ---------------------------------------------------------------- path guiding (rtg_guide.hip)
+// Network: 12-frequency sin/cos encoding of (position in the scene box, shading normal) -> 144
+// inputs, three hidden layers of 64 (ReLU), 40 outputs = 8 vMF lobes x (logit, mean xyz, log kappa).
+// FP32 master parameters, row-major W[out][in] then bias, per layer:
+static const int kGuideIn = 144, kGuideHid = 64, kGuideOut = 40, kGuideLobes = 8;
+static const int kGOffW1 = 0, kGOffB1 = 9216, kGOffW2 = 9280, kGOffB2 = 13376, kGOffW3 = 13440, kGOffB3 = 17536,
+                 kGOffW4 = 17600, kGOffB4 = 20160, kGuideParams = 20200;
+// bf16 copies for the MFMA kernels: forward layout wf = W[out][in] per layer (L4 padded to 64 rows),
+// backward layout wb = W^T[in][out] for layers 2..4 (L4 padded to 64 columns); FP32 biases padded.
+static const int kGWf1 = 0, kGWf2 = 9216, kGWf3 = 13312, kGWf4 = 17408, kGWfSize = 21504;
+static const int kGWb2 = 0, kGWb3 = 4096, kGWb4 = 8192, kGWbSize = 12288;
+static const int kGBias1 = 0, kGBias2 = 64, kGBias3 = 128, kGBias4 = 192, kGBiasSize = 256;
+static const int kGuideBatch = 65536;        // training samples per step
+static const float kGuideLogKappaMin = -3.f, kGuideLogKappaMax = 8.f;
+static const uint32_t kGuideSelDim = 100u;   // 1D sampler dimension of the strategy choice (+ depth)
+
+struct DevGuide {
+  int32_t enabled;
+  int32_t job;               // job-table slot whose depth-0 vertices are guided
+  float bmin[3], inv_ext[3]; // position normalisation to [0,1]^3 (scene box)
+  const uint16_t* wf;        // bf16 forward weights
+  const uint16_t* wb;        // bf16 backward weights
+  const float* bias;         // FP32 biases
+  float* mix;                // per pool slot: 8 lobes x (pi, mu xyz, kappa)
+  float4* rec;               // per pool slot: 3 float4 = pos(3) n(3) dir(3) L snapshot(3)
+  uint8_t* rstate;           // per pool slot: 0 none, 1 vertex recorded, 2 radiance snapshot taken
+  uint32_t* gq;              // guide queue: pool slots of guided depth-0 vertices
+  uint32_t* gq_count;
+  float* ring;               // training samples: pos(3) n(3) dir(3) weight
+  uint32_t ring_mask;        // capacity - 1 (power of two)
+  unsigned long long* ring_count;
+};
+
 // Launch wrappers implemented in rtg_kernels.hip
 struct LaunchCfg {
   int grid;      // blocks for grid-stride kernels (traversal kernels use 2x grid blocks of 128)
@@ -139,13 +172,34 @@ struct LaunchCfg {
 };
 static const int kMaxStack = 64;
 // ev (optional): 8 events recorded before regen and after each of the 7 stage launches
-void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, int njobs,
-                      const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev = nullptr);
+// order_dev: job-table slots in claim order (free path slots go to the first job with samples left)
+void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, const int32_t* order_dev,
+                      int norder, const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev = nullptr,
+                      const DevGuide* guide = nullptr);
 void launch_reset_pool(const DevPool& pool, hipStream_t stream);
 void launch_trace_tap(const DevScene* scene_dev, const float* rays, int n, int any_hit, int32_t* prim,
                       float* t, float* uv, float* p, float* nrm, const LaunchCfg& cfg, hipStream_t stream);
 void launch_bsdf_tap(const DevMaterial* mat, int op, const float* wi, const float* wo_or_u, int n,
                      float* out_dir, float* f, float* pdf, float* rel_eta, hipStream_t stream);
 void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream);
+
+// Guiding kernels (rtg_guide.hip). Scratch for one training step (column-blocked bf16 tensors).
+struct GuideTrainScratch {
+  uint16_t* x0;   // B/8 x 160 x 8
+  uint16_t* h[3]; // B/8 x 64 x 8 each (H1, H2, H3)
+  uint16_t* g[4]; // B/8 x 64 x 8 each (dH1, dH2, dH3, dOut)
+  float* grad;    // kGuideParams
+  float* scal;    // [0] weight sum, [1] loss sum
+};
+void launch_guide_refresh(const float* master, uint16_t* wf, uint16_t* wb, float* bias, hipStream_t stream);
+void launch_guide_infer_pool(const DevGuide& G, const DevPool& P, const DevScene* scenes, const DevJob* jobs,
+                             int grid, hipStream_t stream);
+void launch_guide_infer_array(const DevGuide& G, const float* in6, int n, float* mix_out, hipStream_t stream);
+// One gradient evaluation over n samples (pos, n, dir, weight; 10 floats each) read from
+// samples[(base + i) & mask]; leaves the gradient in S.grad and (weight sum, loss) in S.scal.
+void launch_guide_grad(const DevGuide& G, const float* samples, uint64_t base, uint32_t mask, int n,
+                       const GuideTrainScratch& S, hipStream_t stream);
+void launch_guide_adam(float* master, float* m, float* v, const float* grad, int step, float lr, uint16_t* wf,
+                       uint16_t* wb, float* bias, hipStream_t stream);
 
 }  // namespace rtg

@@ -110,7 +110,7 @@ __device__ __forceinline__ uint32_t block_reserve(uint32_t* counter, uint32_t to
 constexpr int kRegenK = 8;
 constexpr int kRegenSlots = 256 * kRegenK;
 
-__global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, int njobs) {
+__global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, const int32_t* order, int norder) {
   __shared__ uint32_t s_w[4];
   __shared__ uint32_t s_base;
   __shared__ unsigned long long s_base64;
@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, int njob
     // shade queues are filled by the classify stage of this iteration
     C[C_SHADE + 0] = 0; C[C_SHADE + 1] = 0; C[C_SHADE + 2] = 0; C[C_SHADE + 3] = 0;
   }
-  for (int j = threadIdx.x; j < njobs; j += blockDim.x) s_done[j] = 0;
+  for (int j = threadIdx.x; j < kMaxJobsDev; j += blockDim.x) s_done[j] = 0;
   if (threadIdx.x == 0) s_nan = 0;
   __syncthreads();
   const int cap = P.capacity;
@@ -157,8 +157,9 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, int njob
       if (st == ST_FREE) need |= 1u << k;
       if (st == ST_ACTIVE) act |= 1u << k;
     }
-    // claims, jobs in admission order
-    for (int j = 0; j < njobs; j++) {
+    // claims, jobs in the host's claim order
+    for (int jj = 0; jj < norder; jj++) {
+      const int j = order[jj];
       uint32_t total;
       uint32_t my = (uint32_t)__popc(need);
       uint32_t off = block_excl_scan<4, uint32_t>(my, s_w, total);
@@ -216,7 +217,7 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, int njob
       if (act & (1u << k)) P.q_ext[pos++] = (uint32_t)(base + k * 256);
   }
   __syncthreads();
-  for (int j = threadIdx.x; j < njobs; j += blockDim.x)
+  for (int j = threadIdx.x; j < kMaxJobsDev; j += blockDim.x)
     if (s_done[j]) atomicAdd((unsigned long long*)&jobs[j].done, (unsigned long long)s_done[j]);
   if (threadIdx.x == 0 && s_nan) atomicAdd(&C[C_NAN], s_nan);
 }
@@ -652,13 +653,14 @@ __global__ void k_bsdf_tap(const DevMaterial* mat, int op, const float* wi, cons
 // ---------------------------------------------------------------- launchers
 static size_t stack_bytes(int block) { return (size_t)block * kLdsStack * 8u; }
 
-void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, int njobs,
-                      const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev) {
+void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, const int32_t* order_dev,
+                      int norder, const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev, const DevGuide* guide) {
+  (void)guide;
   const int g = cfg.grid;
   const DevJob* cj = jobs_dev;
   const int chunks = (pool.capacity + kRegenSlots - 1) / kRegenSlots;
   if (ev) (void)hipEventRecord(ev[0], stream);
-  hipLaunchKernelGGL(k_regen, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, njobs);
+  hipLaunchKernelGGL(k_regen, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, norder);
   if (ev) (void)hipEventRecord(ev[1], stream);
   hipLaunchKernelGGL(k_extend, dim3(g * 2), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
                      cfg.gstack, cfg.stack);

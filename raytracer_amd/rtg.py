@@ -65,6 +65,15 @@ class rtg_stats(C.Structure):
     ]
 
 
+class rtg_server_stats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64), ("cancelled", C.c_uint64),
+        ("iterations", C.c_uint64), ("samples", C.c_uint64), ("film_bytes", C.c_uint64),
+        ("film_bytes_peak", C.c_uint64), ("running", C.c_int32), ("pending", C.c_int32),
+        ("max_running", C.c_int32), ("reserved", C.c_int32),
+    ]
+
+
 STAGES = ("regen", "extend", "shade_diffuse", "shade_conductor", "shade_dielectric", "shade_miss", "shadow")
 PROFILE_EVENTS, PROFILE_COUNT = 1, 2
 
@@ -91,6 +100,7 @@ SIGNATURES = {
     "rtg_server_step": (C.c_int, [P, C.c_double]),
     "rtg_poll": (C.c_int, [P, C.c_int64, P, C.POINTER(C.c_double)]),
     "rtg_cancel": (C.c_int, [P, C.c_int64]),
+    "rtg_server_query": (C.c_int, [P, C.POINTER(rtg_server_stats)]),
 }
 
 _lib = None
@@ -172,6 +182,7 @@ class Context:
         if rc != RTG_OK:
             raise RtgError(rc, "rtg_ctx_create failed (no gfx950 device?)")
         self.h = h
+        self._servers = []
 
     def check(self, rc):
         if rc != RTG_OK:
@@ -179,6 +190,8 @@ class Context:
 
     def close(self):
         if self.h:
+            for srv in self._servers:  # rtg_ctx_destroy destroys a live server
+                srv.h = None
             self.lib.rtg_ctx_destroy(self.h)
             self.h = None
 
@@ -297,6 +310,72 @@ class Scene:
         self.ctx.check(self.ctx.lib.rtg_render_device(self.ctx.h, self.h, C.byref(d), C.c_void_p(out_ptr),
                                                       C.byref(st)))
         return st
+
+
+class Server:
+    """Continuous-batching serving loop on one context (rtg_server_* in include/rtg.h).
+
+    submit/poll/cancel may be called from other threads while step() runs: ctypes releases the
+    GIL during the call and the library guards its request table separately from device work."""
+
+    def __init__(self, ctx, hbm_budget_bytes=0):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = P()
+        ctx.check(self.lib.rtg_server_create(ctx.h, C.c_uint64(int(hbm_budget_bytes)), C.byref(h)))
+        self.h = h
+        self.shapes = {}
+        ctx._servers.append(self)
+
+    def close(self):
+        if self.h:
+            self.lib.rtg_server_destroy(self.h)
+            self.h = None
+        if self in self.ctx._servers:
+            self.ctx._servers.remove(self)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, scene, camera, params):
+        d = render_struct(camera, params)
+        jid = C.c_int64()
+        self.ctx.check(self.lib.rtg_submit(self.h, scene.h, C.byref(d), C.byref(jid)))
+        self.shapes[jid.value] = (d.height, d.width)
+        return jid.value
+
+    def step(self, max_ms=0.0):
+        self.ctx.check(self.lib.rtg_server_step(self.h, C.c_double(float(max_ms))))
+
+    def poll(self, jid, want_image=True):
+        """-> (status, image or None, latency_ms); status is "done", "running" or "cancelled"."""
+        H, W = self.shapes[jid]
+        out = np.zeros((H, W, 4), np.float32) if want_image else None
+        lat = C.c_double(0.0)
+        rc = self.lib.rtg_poll(self.h, C.c_int64(jid), _ptr(out), C.byref(lat))
+        if rc == -5:
+            return "running", None, None
+        if rc == -6:
+            return "cancelled", None, lat.value
+        self.ctx.check(rc)
+        return "done", out, lat.value
+
+    def cancel(self, jid):
+        self.ctx.check(self.lib.rtg_cancel(self.h, C.c_int64(jid)))
+
+    def stats(self):
+        st = rtg_server_stats()
+        self.ctx.check(self.lib.rtg_server_query(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in st._fields_ if k != "reserved"}
 
 
 def resolve(rgbw):

@@ -1,0 +1,37 @@
+// CPU stand-ins for librtg's kernel launchers (tests/test_serving_host.py): an iteration finishes
+// every sample claimed before it, splats a deterministic value per (job, sample) into the job's film,
+// then hands free path slots to the jobs in claim order, like the device regeneration stage.
+#include <algorithm>
+#include <cstdio>
+#include "rtg_internal.h"
+namespace rtg {
+void launch_iteration(const DevPool& P, const DevScene*, DevJob* jobs, const int32_t* order, int norder,
+                      const LaunchCfg&, hipStream_t, hipEvent_t*, const DevGuide*) {
+  uint64_t free_slots = (uint64_t)P.capacity;
+  for (int j = 0; j < kMaxJobsDev; j++) {  // finish everything claimed earlier
+    DevJob& J = jobs[j];
+    uint64_t fin = std::min(J.next, J.total);
+    for (uint64_t s = J.done; s < fin; s++) {
+      uint64_t npix = (uint64_t)J.width * J.height;
+      float4& a = J.accum[s % npix];
+      a.x += 1.f; a.y += (float)(s / npix); a.z += 0.5f; a.w += 1.f;
+    }
+    J.done = fin;
+  }
+  for (int jj = 0; jj < norder; jj++) {
+    int j = order[jj];
+    if (j < 0 || j >= kMaxJobsDev) { fprintf(stderr, "bad order entry\n"); abort(); }
+    DevJob& J = jobs[j];
+    if (!J.active || J.next >= J.total || free_slots == 0) continue;
+    uint64_t take = std::min<uint64_t>(free_slots, J.total - J.next);
+    J.next += take;
+    free_slots -= take;
+  }
+}
+void launch_reset_pool(const DevPool&, hipStream_t) {}
+void launch_trace_tap(const DevScene*, const float*, int, int, int32_t*, float*, float*, float*, float*,
+                      const LaunchCfg&, hipStream_t) {}
+void launch_bsdf_tap(const DevMaterial*, int, const float*, const float*, int, float*, float*, float*, float*,
+                     hipStream_t) {}
+void launch_fill_zero(void* p, size_t b, hipStream_t) { memset(p, 0, b); }
+}  // namespace rtg

@@ -55,6 +55,12 @@ def load():
             "orc_render": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                      C.c_int, C.c_int, C.c_uint64, P, P, P]),
             "orc_bvh_info": (C.c_int, [P, P]),
+            "gor_num_params": (C.c_int, []),
+            "gor_bf16": (C.c_double, [C.c_double]),
+            "gor_encode": (None, [P, C.c_int, P]),
+            "gor_forward": (None, [P, C.c_int, P, C.c_int, P, P]),
+            "gor_mix_logpdf": (None, [P, P, C.c_int, P]),
+            "gor_loss_grad": (C.c_double, [P, C.c_int, P, C.c_int, P]),
         }
         for k, (r, a) in sig.items():
             f = getattr(lib, k)
@@ -271,3 +277,57 @@ def cdf(w, u):
     idx = np.zeros(n, np.int32); ur = np.zeros(n, np.float32); pdf = np.zeros(n, np.float32)
     load().orc_cdf(_p(w), w.shape[0], _p(u), n, _p(idx), _p(ur), _p(pdf))
     return idx, ur, pdf
+
+
+# ---------------------------------------------------------------- guiding network (guide_oracle.cpp)
+GUIDE_PARAMS = 20200
+GUIDE_SLICES = {  # name -> (offset, shape) in the FP32 parameter vector
+    "W1": (0, (64, 144)), "b1": (9216, (64,)), "W2": (9280, (64, 64)), "b2": (13376, (64,)),
+    "W3": (13440, (64, 64)), "b3": (17536, (64,)), "W4": (17600, (40, 64)), "b4": (20160, (40,)),
+}
+
+
+def guide_init(seed, scale=1.0):
+    """Random parameters (Xavier-uniform weights, small biases) for tests."""
+    rng = np.random.default_rng(seed)
+    p = np.zeros(GUIDE_PARAMS, np.float32)
+    for name, (off, shp) in GUIDE_SLICES.items():
+        n = int(np.prod(shp))
+        if name[0] == "W":
+            lim = scale * np.sqrt(6.0 / (shp[0] + shp[1]))
+            p[off:off + n] = rng.uniform(-lim, lim, n)
+        else:
+            p[off:off + n] = rng.uniform(-0.05, 0.05, n)
+    return p
+
+
+def guide_encode(in6):
+    lib = load()
+    in6 = f32(in6, (-1, 6)); n = in6.shape[0]
+    out = np.zeros((n, 144), np.float64)
+    lib.gor_encode(_p(in6), n, _p(out))
+    return out
+
+
+def guide_forward(params, in6, emulate_bf16=True):
+    lib = load()
+    params = f32(params); in6 = f32(in6, (-1, 6)); n = in6.shape[0]
+    out = np.zeros((n, 40)); mix = np.zeros((n, 40))
+    lib.gor_forward(_p(params), int(emulate_bf16), _p(in6), n, _p(out), _p(mix))
+    return out, mix
+
+
+def guide_mix_logpdf(mix, w):
+    lib = load()
+    mix = np.ascontiguousarray(mix, np.float64).reshape(-1, 40); w = np.ascontiguousarray(w, np.float64).reshape(-1, 3)
+    out = np.zeros(w.shape[0])
+    lib.gor_mix_logpdf(_p(mix), _p(w), w.shape[0], _p(out))
+    return out
+
+
+def guide_loss_grad(params, samples, emulate_bf16=True):
+    lib = load()
+    params = f32(params); samples = f32(samples, (-1, 10))
+    g = np.zeros(GUIDE_PARAMS)
+    loss = lib.gor_loss_grad(_p(params), int(emulate_bf16), _p(samples), samples.shape[0], _p(g))
+    return loss, g

@@ -1,0 +1,59 @@
+"""Serving-loop host logic on the CPU: librtg's host sources (serving loop, job table, scene
+upload) built with g++ against a HIP stand-in (tests/mock_hip) and driven through the same
+ctypes binding and scenarios as tests/test_gpu_serving.py, under glibc's heap checking
+(MALLOC_CHECK_=3). Regression test for a use-after-free in rtg_server_destroy (the request-table
+mutex was unlocked after the server object holding it had been deleted), which glibc reports as
+"free(): corrupted unsorted chunks"."""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+CSRC = os.path.join(ROOT, "raytracer_amd", "csrc")
+MOCK = os.path.join(ROOT, "tests", "mock_hip")
+BUILD = os.path.join(ROOT, "tests", "_build")
+
+RUNNER = r"""
+import sys
+sys.path.insert(0, {root!r}); sys.path.insert(0, {tests!r})
+from raytracer_amd import rtg, scenes
+rtg.LIB_PATH = {lib!r}
+import test_gpu_serving as T
+ctx = rtg.Context(0)
+ctx.set_pool(1 << 16)
+res = {{}}
+for name, sd in (("cornell", scenes.cornell()), ("glossy", scenes.glossy_cornell()),
+                 ("displaced", scenes.displaced_sphere(level=3))):
+    res[name] = (sd, ctx.upload(sd))
+for fn in [T.test_same_alone_and_in_mix, T.test_render_refused_while_server_alive_and_restored_after,
+           T.test_budget_admission_serialises_films, T.test_cancel_drains_and_others_finish,
+           T.test_submit_from_another_thread_while_stepping]:
+    fn(ctx, res)
+    print("PASS", fn.__name__, flush=True)
+ctx.close()
+print("ALL OK", flush=True)
+"""
+
+
+def build_mock_lib():
+    os.makedirs(BUILD, exist_ok=True)
+    out = os.path.join(BUILD, "librtg_hostmock.so")
+    srcs = [os.path.join(CSRC, f) for f in ("rtg_serve.cpp", "rtg_host.cpp", "rtg_bvh.cpp", "rtg_stubs.cpp")]
+    srcs.append(os.path.join(MOCK, "mock_kernels.cpp"))
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-pthread", "-I" + MOCK, "-I" + CSRC] + srcs + \
+          ["-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    return out
+
+
+def test_serving_host_logic_under_heap_checks():
+    lib = build_mock_lib()
+    code = RUNNER.format(root=ROOT, tests=os.path.join(ROOT, "tests"), lib=lib)
+    env = dict(os.environ, MALLOC_CHECK_="3", MALLOC_PERTURB_="165")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=600)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-3000:])
+    assert "ALL OK" in r.stdout
