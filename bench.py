@@ -36,6 +36,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (scale reference for PMC byte counters)")
+    ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain leg")
+    ap.add_argument("--guide-size", type=int, default=512, help="guided-vs-plain leg: image size")
+    ap.add_argument("--guide-ref-spp", type=int, default=65536,
+                    help="guided-vs-plain leg: reference spp (two independent halves)")
     ap.add_argument("--serving", action="store_true",
                     help="also run the serving-mix leg (off by default until its GPU run is re-validated)")
     ap.add_argument("--serve-requests", type=int, default=48,
@@ -129,6 +133,83 @@ def serving(ctx, big, n_req, lam=40.0, seed=4):
     }
 
 
+def relmse(x, r, noise=None):
+    """mean (x - r)^2 / (r^2 + 1e-2); `noise` (per-pixel variance of the reference, estimated from
+    two independent halves) is subtracted so the reference's own noise does not count."""
+    import numpy as np
+    d = (x - r) ** 2
+    if noise is not None:
+        d = d - noise
+    return float(np.mean(d / (r * r + 1e-2)))
+
+
+def guiding_leg(ctx, size, ref_spp, target=1e-3):
+    """Time to relMSE <= target on the glossy-indirect scene, plain vs guided, both on the GPU.
+
+    The judge of the error is the plain tracer's converged image (ref_spp, rendered as two halves
+    whose difference estimates the reference's own per-pixel noise, which is subtracted from every
+    squared error so that it does not count against either method). Each render starts from
+    scratch (a guided render initialises and trains its network online inside the timed call), at
+    a ladder of square spp counts; the time to reach the target is interpolated log-log between
+    the ladder's points. Also reports the guide network's training and inference device time."""
+    import numpy as np
+    from raytracer_amd import rtg, scenes
+
+    sd = scenes.glossy_cornell(width=size, height=size)
+    g = ctx.upload(sd)
+    cam = sd.camera
+    t0 = time.perf_counter()
+    halves = []
+    for s in (1, 2):
+        h, _ = g.render(cam, dict(sd.render, spp=ref_spp // 2, seed=s))
+        halves.append(rtg.resolve(h).astype(np.float64))
+    t_ref = time.perf_counter() - t0
+    ref = 0.5 * (halves[0] + halves[1])
+    noise = 0.25 * (halves[0] - halves[1]) ** 2  # variance of the mean of the two halves
+    ladder = [64, 144, 256, 576, 1024, 2304, 4096, 9216, 16384]
+    curves = {}
+    extra = {}
+    for name, gd in (("plain", 0), ("guided", 1)):
+        pts = []
+        ctx.set_profiling(rtg.PROFILE_EVENTS)
+        for i, spp in enumerate(ladder):
+            t0 = time.perf_counter()
+            out, st = g.render(cam, dict(sd.render, spp=spp, seed=1000 + i, guiding=gd))
+            dt = time.perf_counter() - t0
+            e = relmse(rtg.resolve(out).astype(np.float64), ref, noise)
+            pts.append((spp, dt, e, st.samples))
+            if gd and spp == ladder[-1]:
+                extra = {"train_steps": int(st.guide_train_steps), "train_ms": st.guide_train_ms,
+                         "infer_ms": st.stage_ms[2], "render_ms": st.milliseconds,
+                         "training_samples": int(st.guide_samples), "final_loss": st.guide_loss}
+            if e <= target * 0.5:
+                break
+        ctx.set_profiling(0)
+        curves[name] = pts
+
+    def time_to(pts):
+        for (s0, t0_, e0, _), (s1, t1, e1, _) in zip(pts, pts[1:]):
+            if e0 > target >= e1:
+                f = (np.log(target) - np.log(e0)) / (np.log(e1) - np.log(e0))
+                return float(np.exp(np.log(t0_) + f * (np.log(t1) - np.log(t0_)))), \
+                    float(np.exp(np.log(s0) + f * (np.log(s1) - np.log(s0))))
+        if pts and pts[0][2] <= target:
+            return pts[0][1], float(pts[0][0])
+        return None, None
+
+    res = {"scene": f"glossy_cornell {size}x{size}", "target_relmse": target, "reference_spp": ref_spp,
+           "reference_s": t_ref}
+    for name, pts in curves.items():
+        t, spp = time_to(pts)
+        res[name] = {"ladder": [{"spp": p[0], "s": round(p[1], 4), "relmse": p[2]} for p in pts],
+                     "s_to_target": t, "spp_to_target": spp,
+                     "samples_per_s_to_target": (spp * size * size / t) if t else None}
+    if res["plain"]["s_to_target"] and res["guided"]["s_to_target"]:
+        res["guided_over_plain_time"] = res["guided"]["s_to_target"] / res["plain"]["s_to_target"]
+    res["guide_network"] = extra
+    return res
+
+
 def copy_reference(mb):
     import torch
 
@@ -180,8 +261,8 @@ def main():
         scene.render_device(cam, rp, film.data_ptr())
     torch.cuda.synchronize()
     progress(f"timing {a.steps} steps")
-    stage_ms = np.zeros(7)
-    launches = np.zeros(7)
+    stage_ms = np.zeros(len(rtg.STAGES))
+    launches = np.zeros(len(rtg.STAGES))
     dev_ms = 0.0
     samples = 0
     rays = 0
@@ -240,6 +321,9 @@ def main():
                         "prims": counted["prims"] / max(1, counted["ext_rays"])},
         },
     }
+    if not a.no_guiding:
+        progress("guided vs plain (glossy-indirect scene)")
+        out["guiding"] = guiding_leg(ctx, a.guide_size, a.guide_ref_spp)
     if a.serving:
         progress(f"serving mix ({a.serve_requests} requests)")
         out["serving"] = serving(ctx, (sd, scene), a.serve_requests)

@@ -113,7 +113,10 @@ typedef struct rtg_render_desc {
   int32_t min_depth;
   int32_t max_depth;
   uint64_t seed;
-  int32_t guiding;             /* 0 = plain MIS path tracing */
+  int32_t guiding;             /* 0 = plain MIS path tracing; 1 = learned path guiding at the camera
+                                  vertices (rtg_render / rtg_render_device only; no reference
+                                  counterpart): a vMF-mixture network trained online on this
+                                  render's own paths, one-sample MIS with BSDF sampling */
 } rtg_render_desc;
 
 typedef struct rtg_stats {
@@ -129,9 +132,16 @@ typedef struct rtg_stats {
   uint64_t shadow_nodes_visited;
   uint64_t shadow_prims_tested;
   /* device time per stage and launches, filled when RTG_PROFILE_EVENTS is set:
-   * 0 regen, 1 extend, 2 shade diffuse, 3 shade conductor, 4 shade dielectric, 5 miss, 6 shadow */
-  double stage_ms[7];
-  uint64_t stage_launches[7];
+   * 0 regen, 1 extend (+ classify), 2 guide inference, 3 shade diffuse, 4 shade conductor,
+   * 5 shade dielectric, 6 miss, 7 shadow (+ compaction) */
+  double stage_ms[8];
+  uint64_t stage_launches[8];
+  /* guided renders */
+  uint64_t guide_train_steps;  /* training steps run during the render */
+  uint64_t guide_samples;      /* training samples the render produced */
+  double guide_loss;           /* loss of the last training step */
+  double guide_train_ms;       /* device time of the training steps (RTG_PROFILE_EVENTS) */
+  uint64_t guide_queries;      /* network inferences (guided camera vertices) */
 } rtg_stats;
 
 enum { RTG_PROFILE_EVENTS = 1, RTG_PROFILE_COUNT = 2 };
@@ -174,9 +184,31 @@ int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int3
 
 /* Parity tap for BSDF::eval/pdf/sample (bsdf/BSDF.h:70-74) on local directions.
  * op 0: eval+pdf of (wi, wo) -> f (n x 3), pdf (n).
- * op 1: sample from wi with u (n x 2) -> wo (n x 3), f = weight (n x 3), pdf (n), rel_eta (n). */
+ * op 1: sample from wi with u (n x 2) -> wo (n x 3), f = weight (n x 3), pdf (n), rel_eta (n).
+ * op 2 (diffuse and rough conductor; used by guided sampling, no reference counterpart): for
+ *   (wi, wo) the density with which op 1 produces wo -> pdf, the weight op 1 returns for wo -> f,
+ *   the pdf op 1 reports for wo -> rel_eta. */
 int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const float* wi, const float* wo_or_u,
                    int32_t n, float* out_dir, float* f, float* pdf, float* rel_eta);
+
+/* Path-guiding network parity taps (no reference counterpart; the network is this library's own).
+ * params: RTG_GUIDE_PARAMS FP32 values, per layer W[out][in] row-major then bias:
+ *   W1 64x144, b1 64, W2 64x64, b2 64, W3 64x64, b3 64, W4 40x64, b4 40.
+ * The device computes in bf16 with FP32 accumulation (weights and activations rounded to bf16).
+ * rtg_guide_eval: inputs in6 (n x 6: position in [0,1]^3, unit normal) -> mixture (n x 40: per lobe
+ *   weight, mean direction xyz, concentration).
+ * rtg_guide_grad: samples (n x 10: position, normal, direction, weight) -> loss
+ *   -(1/sum w) sum w log q(direction) and its gradient (RTG_GUIDE_PARAMS floats); n <= 65536.
+ * rtg_guide_train: `steps` Adam steps (lr) over the same samples starting from params;
+ *   params_out gets the result, losses (steps entries, may be NULL) the loss before each step. */
+enum { RTG_GUIDE_PARAMS = 20200 };
+int rtg_guide_eval(rtg_ctx* ctx, const float* params, const float* in6, int32_t n, float* mix_out);
+int rtg_guide_grad(rtg_ctx* ctx, const float* params, const float* samples, int32_t n, float* loss, float* grad);
+int rtg_guide_train(rtg_ctx* ctx, const float* params, const float* samples, int32_t n, int32_t steps, float lr,
+                    float* params_out, float* losses);
+/* Directions from mixtures (n x 40, as rtg_guide_eval returns) with u (n x 3: lobe choice, two for
+ * the lobe) -> dirs (n x 3) and the mixture density at each (n). */
+int rtg_guide_sample(rtg_ctx* ctx, const float* mix, const float* u, int32_t n, float* dirs, float* pdf);
 
 /* Continuous-batching serving (no reference counterpart; for a host that renders many requests
  * on one GPU, e.g. an interactive preview service). Requests of any resident scene are admitted

@@ -5,8 +5,8 @@
 // restates the network the library defines (encoding, layers, vMF-mixture head, loss) in double
 // precision, optionally rounding to bfloat16 at the same points as the GPU kernels:
 //   * the 144 input features, the weights, and each hidden activation after its ReLU;
-//   * the output gradient and each hidden-layer gradient before they feed a product
-//     (the weight gradients are sums of bf16 x bf16 products; bias gradients sum the unrounded
+//   * the output gradient and each hidden-layer gradient before they feed a product (the weight
+//     gradients are sums of bf16 x bf16 products, the bias gradients sums of the rounded
 //     gradients).
 // Pinned by: central finite differences of the FP64 loss (no rounding), the vMF density's
 // normalisation over the sphere, and its closed-form sampling (tests/test_oracle_guide.py).
@@ -233,8 +233,8 @@ double gor_loss_grad(const float* params, int emu, const float* samples, int n, 
     // layer 4
     double dq[kOut];
     for (int o = 0; o < kOut; o++) {
-      g[kOffB4 + o] += dout[o];
       dq[o] = rnd(dout[o], N.emu);
+      g[kOffB4 + o] += dq[o];
       for (int h = 0; h < kHid; h++) g[kOffW4 + o * kHid + h] += dq[o] * A.h3[h];
     }
     // back through layers 3, 2, 1
@@ -252,8 +252,8 @@ double gor_loss_grad(const float* params, int emu, const float* samples, int n, 
     for (int L = 0; L < 3; L++) {
       double dhq[kHid];
       for (int o = 0; o < kHid; o++) {
-        g[offB[L] + o] += dh[o];
         dhq[o] = rnd(dh[o], N.emu);
+        g[offB[L] + o] += dhq[o];
         for (int j = 0; j < nin[L]; j++) g[offW[L] + o * nin[L] + j] += dhq[o] * xs[L][j];
       }
       if (L == 2) break;

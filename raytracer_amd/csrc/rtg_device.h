@@ -9,6 +9,16 @@
 namespace rtg {
 namespace dev {
 
+// Path-slot flags word: bits 0..11 depth of the pending hit, 12..19 job slot, bit 20 LAST (only the
+// emission of the pending segment remains), bit 21 EMISSION (depth-0 emission enabled).
+static const uint32_t DEPTH_MASK = 0xFFFu;
+static const uint32_t JOB_SHIFT = 12;
+static const uint32_t JOB_MASK = 0xFFu;
+static const uint32_t FL_LAST = 1u << 20;
+static const uint32_t FL_EMIT = 1u << 21;
+__device__ __forceinline__ uint32_t fl_depth(uint32_t f) { return f & DEPTH_MASK; }
+__device__ __forceinline__ uint32_t fl_job(uint32_t f) { return (f >> JOB_SHIFT) & JOB_MASK; }
+
 // core/BasicMath.h:16-21 (float branch)
 constexpr float PI = 3.14159265358979323846f;
 constexpr float INV_PI = 0.3183098861837906715377f;
@@ -782,6 +792,64 @@ __device__ __forceinline__ bool bsdf_sample(const DevMaterial& m, f3 wi, float u
   if (MT == 0) return diffuse_sample(m, wi, ux, uy, bs);
   if (MT == 1) return conductor_sample(m, wi, ux, uy, bs);
   return dielectric_sample(m, wi, ux, uy, bs);
+}
+
+// ---------------------------------------------------------------- guiding: one-sample MIS support
+// Density with which bsdf_sample<MT>(m, wi, u) produces the direction wo (solid angle), together
+// with the weight the sampler returns for it (f*cos/pdf), the pdf it reports (the value the MIS
+// weights use) and the relative IOR. Defined for the diffuse and rough-conductor BSDFs; the
+// conductor samples microfacet normals with density D(alpha')(m) cos(theta_m) (mf_sample) and
+// reflects, so p = D(alpha')(wh) wh.z / (4 |wo.wh|), while its reported pdf uses alpha.
+template <int MT>
+__device__ __forceinline__ bool bsdf_density(const DevMaterial& m, f3 wi, f3 wo, f3& weight, float& ps, float& pref,
+                                             float& rel) {
+  rel = 1.f;
+  if (wi.z <= 0.f || wo.z <= 0.f) return false;
+  if (MT == 0) {
+    ps = wo.z * INV_PI;
+    pref = diffuse_pdf(wi, wo);
+    weight = mat_col(m.albedo);
+    return ps > 0.f;
+  }
+  f3 wh = normalized(wi + wo);
+  float a = adj_alpha(m.alpha, wi.z);
+  ps = mf_D(m.dist, wh, a) * wh.z / (4.f * fabsf(dot(wh, wo)));
+  pref = conductor_pdf(m, wi, wo);
+  if (!(ps > 0.f) || !(pref > 0.f)) return false;
+  weight = conductor_eval(m, wi, wo) * fabsf(wo.z) / pref;
+  return true;
+}
+
+// von Mises-Fisher lobe on the unit sphere: kappa / (2 pi (1 - e^{-2 kappa})) e^{kappa (mu.w - 1)}
+__device__ __forceinline__ float vmf_pdf(f3 mu, float kappa, f3 w) {
+  return kappa / (2.f * PI * (-expm1f(-2.f * kappa))) * expf(kappa * (dot(mu, w) - 1.f));
+}
+// mixture given as 8 x (pi, mu xyz, kappa)
+__device__ __forceinline__ float mix_pdf(const float* mx, f3 w) {
+  float p = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; k++) p += mx[5 * k] * vmf_pdf(mk(mx[5 * k + 1], mx[5 * k + 2], mx[5 * k + 3]), mx[5 * k + 4], w);
+  return p;
+}
+// Sample a lobe with ul, then its vMF with (u1, u2) by inverting the CDF of cos(theta) (Wenzel
+// Jakob, "Numerically stable sampling of the von Mises Fisher distribution on S2", 2012).
+__device__ __forceinline__ f3 mix_sample(const float* mx, float ul, float u1, float u2) {
+  int sel = 7;  // first lobe whose cumulative weight exceeds ul (the last one absorbs rounding)
+  float acc = 0.f;
+#pragma unroll
+  for (int i = 0; i < 7; i++) {
+    acc += mx[5 * i];
+    if (sel == 7 && ul < acc) sel = i;
+  }
+  const f3 mu = mk(mx[5 * sel + 1], mx[5 * sel + 2], mx[5 * sel + 3]);
+  const float kappa = mx[5 * sel + 4];
+  float w = 1.f + logf(u1 + (1.f - u1) * expf(-2.f * kappa)) / kappa;
+  w = fminf(fmaxf(w, -1.f), 1.f);
+  const float sr = sqrtf(fmaxf(0.f, 1.f - w * w));
+  float sp, cp;
+  sincosf(2.f * PI * u2, &sp, &cp);
+  Frame F = make_frame(mu);
+  return to_world(F, mk(sr * cp, sr * sp, w));
 }
 
 // ---------------------------------------------------------------- CDF (core/CDF.h:44-81)

@@ -536,7 +536,6 @@ DevJob make_job(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc& d, float4
   if (d.width <= 0 || d.height <= 0 || d.spp <= 0) fail(RTG_ERR_INVALID, "width, height and spp must be positive");
   if (d.sampler != RTG_SAMPLER_RANDOM && d.sampler != RTG_SAMPLER_STRATIFIED) fail(RTG_ERR_INVALID, "unknown sampler");
   if (d.max_depth < 0 || d.max_depth > 4000 || d.min_depth < 0) fail(RTG_ERR_INVALID, "depth out of range");
-  if (d.guiding) fail(RTG_ERR_INVALID, "guiding is not available in this entry point");
   if (!(d.fov > 0.f && d.fov < 180.f)) fail(RTG_ERR_INVALID, "fov out of range");
   DevJob J;
   std::memset(&J, 0, sizeof(J));
@@ -637,6 +636,7 @@ int rtg_ctx_create(int device, rtg_ctx** out) {
 int rtg_ctx_destroy(rtg_ctx* ctx) {
   if (!ctx) return RTG_ERR_INVALID;
   if (ctx->server) (void)rtg_server_destroy(ctx->server);  // drops its requests' scene references
+  guide_destroy(ctx);
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   for (auto* s : ctx->scene_slots)
@@ -704,7 +704,11 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   if (ctx->server) fail(RTG_ERR_INVALID, "the context is serving requests; submit the render instead");
   hip_check(hipSetDevice(ctx->device), "hipSetDevice");
   DevJob J = make_job(ctx, scene, *desc, (float4*)out_dev);
-  // sources of the asynchronous uploads sit in the context's pinned page (offsets 64 and 256)
+  // pinned page: [0] done count, [16] guide ring and inference counts, [40] guide loss, [64] claim
+  // order, [256] job
+  uint64_t* done = (uint64_t*)ctx->pinned;
+  unsigned long long* ring_cnt = (unsigned long long*)((char*)ctx->pinned + 16);
+  float* loss_h = (float*)((char*)ctx->pinned + 40);
   int32_t* up_order = (int32_t*)((char*)ctx->pinned + 64);
   DevJob* up_job = (DevJob*)((char*)ctx->pinned + 256);
   static_assert(256 + sizeof(DevJob) <= 4096, "pinned staging page too small");
@@ -716,40 +720,101 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   LaunchCfg cfg = launch_cfg(ctx);
   launch_reset_pool(ctx->pool, ctx->stream);
   hip_check(hipMemcpyAsync(ctx->jobs_dev, up_job, sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream), "job upload");
+  const bool guided = desc->guiding != 0;
+  GuideState* gs = nullptr;
+  DevGuide G;
+  std::memset(&G, 0, sizeof(G));
+  // Guided renders sample plainly (recording training vertices) until the network has had
+  // kWarmSteps training steps, then switch to one-sample MIS with the guide; training (and the
+  // recording that feeds it) stops after kMaxSteps steps.
+  const uint64_t kWarmSteps = 32, kMaxSteps = 512;
+  const int kMaxStepsPerBatch = 8;
+  if (guided) {
+    gs = guide_state(ctx);
+    guide_prepare_render(ctx, gs, scene, desc->seed, G);
+    G.sample = 0;
+  }
   hipEvent_t e0, e1;
   hip_check(hipEventCreate(&e0), "event");
   hip_check(hipEventCreate(&e1), "event");
   hip_check(hipEventRecord(e0, ctx->stream), "event");
   uint64_t iters = 0;
-  uint64_t* done = (uint64_t*)ctx->pinned;
   const int batch = 4;
+  const int kEv = 9;             // events per iteration (8 stages)
+  const int kTrainEv = 2 * kMaxStepsPerBatch;
   const bool prof = (ctx->profiling & RTG_PROFILE_EVENTS) != 0;
-  double stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+  double stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  double train_ms = 0.0;
+  uint64_t train_steps = 0, trained_upto = 0, ring_total = 0;
+  double last_loss = 0.0;
   auto wall0 = std::chrono::steady_clock::now();
+  const size_t nev = (size_t)batch * kEv + kTrainEv;
+  if (prof && ctx->events.size() < nev) {
+    size_t have = ctx->events.size();
+    ctx->events.resize(nev);
+    for (size_t i = have; i < ctx->events.size(); i++) hip_check(hipEventCreate(&ctx->events[i]), "event");
+  }
+  int pending_train = 0;  // training steps enqueued before the batch now in flight
   while (true) {
-    if (prof && ctx->events.size() < (size_t)batch * 8) {
-      size_t have = ctx->events.size();
-      ctx->events.resize((size_t)batch * 8);
-      for (size_t i = have; i < ctx->events.size(); i++) hip_check(hipEventCreate(&ctx->events[i]), "event");
-    }
     for (int b = 0; b < batch; b++)
       launch_iteration(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, 1, cfg, ctx->stream,
-                       prof ? &ctx->events[(size_t)b * 8] : nullptr);
+                       prof ? &ctx->events[(size_t)b * kEv] : nullptr, guided ? &G : nullptr);
     iters += batch;
     hip_check(hipGetLastError(), "kernel launch");
     hip_check(hipMemcpyAsync(done, &ctx->jobs_dev[0].done, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream),
               "done readback");
+    if (guided) {
+      hip_check(hipMemcpyAsync(ring_cnt, gs->ring_count, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                               ctx->stream),
+                "ring readback");
+      hip_check(hipMemcpyAsync(loss_h, gs->S.scal + 1, sizeof(float), hipMemcpyDeviceToHost, ctx->stream),
+                "loss readback");
+    }
     hip_check(hipStreamSynchronize(ctx->stream), "render iteration");
     if (prof) {
       for (int b = 0; b < batch; b++)
-        for (int k = 0; k < 7; k++) {
+        for (int k = 0; k < 8; k++) {
           float ms = 0.f;
-          hip_check(hipEventElapsedTime(&ms, ctx->events[(size_t)b * 8 + k], ctx->events[(size_t)b * 8 + k + 1]),
+          hip_check(hipEventElapsedTime(&ms, ctx->events[(size_t)b * kEv + k], ctx->events[(size_t)b * kEv + k + 1]),
                     "event time");
           stage_ms[k] += ms;
         }
+      for (int t = 0; t < pending_train; t++) {
+        float ms = 0.f;
+        hip_check(hipEventElapsedTime(&ms, ctx->events[(size_t)batch * kEv + 2 * t],
+                                      ctx->events[(size_t)batch * kEv + 2 * t + 1]),
+                  "event time");
+        train_ms += ms;
+      }
     }
+    if (pending_train) last_loss = *loss_h;
+    pending_train = 0;
     if (*done >= J.total) break;
+    if (guided) {
+      // train on the newest samples: one step per full batch of new samples, at most
+      // kMaxStepsPerBatch per iteration batch
+      ring_total = *ring_cnt;
+      const uint64_t B = kGuideBatch;
+      int steps = 0;
+      if (G.record && ring_total >= B && ring_total > trained_upto) {
+        uint64_t fresh = ring_total - trained_upto;
+        steps = (int)std::min<uint64_t>(kMaxStepsPerBatch, std::max<uint64_t>(1, fresh / B));
+        steps = (int)std::min<uint64_t>((uint64_t)steps, kMaxSteps - train_steps);
+      }
+      for (int t = 0; t < steps; t++) {
+        const uint64_t base = ring_total - (uint64_t)(t + 1) * B;
+        if (prof) (void)hipEventRecord(ctx->events[(size_t)batch * kEv + 2 * t], ctx->stream);
+        guide_train_step(ctx, gs, G, gs->ring, base, G.ring_mask, (int)B, 1e-3f);
+        if (prof) (void)hipEventRecord(ctx->events[(size_t)batch * kEv + 2 * t + 1], ctx->stream);
+      }
+      if (steps) {
+        trained_upto = ring_total;
+        train_steps += steps;
+        pending_train = steps;
+        if (train_steps >= kWarmSteps) G.sample = 1;
+        if (train_steps >= kMaxSteps) G.record = 0;
+      }
+    }
     double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
     if (iters > 200000 || wall > 600.0) fail(RTG_ERR_HIP, "render did not finish (iteration or time limit)");
   }
@@ -778,9 +843,16 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
     st->prims_tested = u64(C_PRIMS);
     st->shadow_nodes_visited = u64(C_SHNODES);
     st->shadow_prims_tested = u64(C_SHPRIMS);
-    for (int k = 0; k < 7; k++) {
+    for (int k = 0; k < 8; k++) {
       st->stage_ms[k] = stage_ms[k];
       st->stage_launches[k] = prof ? iters : 0;
+    }
+    if (guided) {
+      st->guide_train_steps = train_steps;
+      st->guide_samples = *ring_cnt;
+      st->guide_loss = last_loss;
+      st->guide_train_ms = train_ms;
+      st->guide_queries = ring_cnt[1];
     }
   }
 }
@@ -859,9 +931,11 @@ int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const
   if (!ctx) return RTG_ERR_INVALID;
   std::lock_guard<std::mutex> lk(ctx->mtx);
   RTG_GUARD(ctx, {
-    if (!material || !wi || !wo_or_u || !f || !pdf || n < 0 || (op != 0 && op != 1))
+    if (!material || !wi || !wo_or_u || !f || !pdf || n < 0 || op < 0 || op > 2)
       fail(RTG_ERR_INVALID, "bad bsdf arguments");
     if (op == 1 && (!out_dir || !rel_eta)) fail(RTG_ERR_INVALID, "sample needs out_dir and rel_eta");
+    if (op == 2 && (!rel_eta || material->type == RTG_MAT_ROUGH_DIELECTRIC))
+      fail(RTG_ERR_INVALID, "sampling density needs rel_eta and a diffuse or conductor material");
     if (n == 0) return RTG_OK;
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     const rtg_material& m = *material;
@@ -882,7 +956,7 @@ int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const
     std::vector<void*> own;
     DevMaterial* dm = (DevMaterial*)dev_alloc(ctx, sizeof(DevMaterial), own);
     float* dwi = (float*)dev_alloc(ctx, (size_t)n * 12, own);
-    size_t w2 = (size_t)n * (op == 0 ? 12 : 8);
+    size_t w2 = (size_t)n * (op == 1 ? 8 : 12);
     float* dwo = (float*)dev_alloc(ctx, w2, own);
     float* ddir = (float*)dev_alloc(ctx, (size_t)n * 12, own);
     float* df = (float*)dev_alloc(ctx, (size_t)n * 12, own);
@@ -896,10 +970,10 @@ int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const
       hip_check(hipGetLastError(), "bsdf launch");
       hip_check(hipMemcpyAsync(f, df, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "f");
       hip_check(hipMemcpyAsync(pdf, dp, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "pdf");
-      if (op == 1) {
+      if (op == 1)
         hip_check(hipMemcpyAsync(out_dir, ddir, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "dir");
+      if (op >= 1)
         hip_check(hipMemcpyAsync(rel_eta, de, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "eta");
-      }
       hip_check(hipStreamSynchronize(ctx->stream), "bsdf");
     } catch (...) {
       for (void* q : own) (void)hipFree(q);

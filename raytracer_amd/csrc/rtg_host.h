@@ -24,6 +24,26 @@ struct RtgError : std::runtime_error {
 void hip_check(hipError_t e, const char* what);
 void set_error(const char* msg);
 
+// Path-guiding state of a context (rtg_guide_host.cpp).
+struct GuideState {
+  std::vector<void*> owned, pool_owned;
+  float *master = nullptr, *adam_m = nullptr, *adam_v = nullptr;
+  uint16_t *wf = nullptr, *wb = nullptr;
+  float* bias = nullptr;
+  GuideTrainScratch S{};
+  float* samples = nullptr;  // tap input (kGuideBatch x 10)
+  int step = 0;              // Adam step count
+  int capacity = 0;          // pool slots the per-slot buffers cover
+  float* mix = nullptr;
+  float4* rec = nullptr;
+  uint8_t* rstate = nullptr;
+  uint32_t *gq = nullptr, *gq_count = nullptr;
+  float* ring = nullptr;
+  uint32_t ring_cap = 0;
+  unsigned long long* ring_count = nullptr;  // [0] ring count, [1] inference count
+  ~GuideState();
+};
+
 }  // namespace rtg
 
 struct rtg_scene {
@@ -54,6 +74,7 @@ struct rtg_ctx {
   uint2* gstack = nullptr;
   int32_t* order_dev = nullptr;    // claim order of job-table slots
   rtg_server* server = nullptr;    // the serving loop owning the pool, if any
+  rtg::GuideState* guide = nullptr;  // path-guiding network and buffers, created on first use
   std::vector<rtg_scene*> scene_slots;
   void* pinned = nullptr;
   std::recursive_mutex mtx_r;
@@ -69,4 +90,12 @@ DevJob make_job(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc& d, float4
 LaunchCfg launch_cfg(rtg_ctx* ctx);
 void camera_matrices(const rtg_render_desc& d, float* s2c, float* c2w);
 void* dev_alloc(rtg_ctx* ctx, size_t bytes, std::vector<void*>& owned);
+GuideState* guide_state(rtg_ctx* ctx);
+void guide_destroy(rtg_ctx* ctx);
+void guide_init_params(rtg_ctx* ctx, GuideState* g, uint64_t seed);
+void guide_set_params(rtg_ctx* ctx, GuideState* g, const float* params);
+DevGuide guide_dev(GuideState* g);
+void guide_prepare_render(rtg_ctx* ctx, GuideState* g, rtg_scene* scene, uint64_t seed, DevGuide& G);
+void guide_train_step(rtg_ctx* ctx, GuideState* g, const DevGuide& G, const float* samples, uint64_t base,
+                      uint32_t mask, int n, float lr);
 }  // namespace rtg

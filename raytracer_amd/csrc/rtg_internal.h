@@ -144,10 +144,12 @@ static const int kGWb2 = 0, kGWb3 = 4096, kGWb4 = 8192, kGWbSize = 12288;
 static const int kGBias1 = 0, kGBias2 = 64, kGBias3 = 128, kGBias4 = 192, kGBiasSize = 256;
 static const int kGuideBatch = 65536;        // training samples per step
 static const float kGuideLogKappaMin = -3.f, kGuideLogKappaMax = 8.f;
-static const uint32_t kGuideSelDim = 100u;   // 1D sampler dimension of the strategy choice (+ depth)
+static const uint32_t kGuideSelDim = 1u << 20; // 1D sampler dimension of the strategy choice (+ depth)
 
 struct DevGuide {
   int32_t enabled;
+  int32_t sample;            // 1: guided sampling; 0: plain sampling, vertices still recorded (warm-up)
+  int32_t record;            // 1: record guided vertices and emit training samples
   int32_t job;               // job-table slot whose depth-0 vertices are guided
   float bmin[3], inv_ext[3]; // position normalisation to [0,1]^3 (scene box)
   const uint16_t* wf;        // bf16 forward weights
@@ -161,6 +163,7 @@ struct DevGuide {
   float* ring;               // training samples: pos(3) n(3) dir(3) weight
   uint32_t ring_mask;        // capacity - 1 (power of two)
   unsigned long long* ring_count;
+  unsigned long long* infer_count;  // inferences run (statistics)
 };
 
 // Launch wrappers implemented in rtg_kernels.hip
@@ -182,6 +185,7 @@ void launch_trace_tap(const DevScene* scene_dev, const float* rays, int n, int a
 void launch_bsdf_tap(const DevMaterial* mat, int op, const float* wi, const float* wo_or_u, int n,
                      float* out_dir, float* f, float* pdf, float* rel_eta, hipStream_t stream);
 void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream);
+void launch_guide_sample_tap(const float* mix, const float* u, int n, float* dirs, float* pdf, hipStream_t stream);
 
 // Guiding kernels (rtg_guide.hip). Scratch for one training step (column-blocked bf16 tensors).
 struct GuideTrainScratch {

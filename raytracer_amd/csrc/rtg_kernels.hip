@@ -8,19 +8,13 @@
 // outside the eight XCDs' L2 caches and serialise, so they are kept to ~n/2048 per launch.
 // The control flow restates integrator/PathTracingMIS.cpp:27-263 and
 // integrator/Integrator.cpp:176-238 split at the two ray casts of each bounce.
+#include <cstring>
+
 #include "rtg_device.h"
 
 namespace rtg {
 using namespace dev;
 
-static const uint32_t DEPTH_MASK = 0xFFFu;
-static const uint32_t JOB_SHIFT = 12;
-static const uint32_t JOB_MASK = 0xFFu;
-static const uint32_t FL_LAST = 1u << 20;
-static const uint32_t FL_EMIT = 1u << 21;
-
-__device__ __forceinline__ uint32_t fl_depth(uint32_t f) { return f & DEPTH_MASK; }
-__device__ __forceinline__ uint32_t fl_job(uint32_t f) { return (f >> JOB_SHIFT) & JOB_MASK; }
 
 // ---------------------------------------------------------------- camera (NoriPerspective.cpp:80-118)
 __device__ __forceinline__ void camera_ray(const DevJob& J, float px, float py, f3& o, f3& d, float& tmin,
@@ -110,7 +104,12 @@ __device__ __forceinline__ uint32_t block_reserve(uint32_t* counter, uint32_t to
 constexpr int kRegenK = 8;
 constexpr int kRegenSlots = 256 * kRegenK;
 
-__global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, const int32_t* order, int norder) {
+// GUIDED: finished paths whose depth-0 vertex was guided and whose radiance snapshot was taken
+// append one training sample (position, normal, direction, luminance of the radiance gathered
+// after that vertex) to the guide's ring; one 64-bit atomic per chunk.
+template <bool GUIDED>
+__global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, const int32_t* order, int norder,
+                                               DevGuide G) {
   __shared__ uint32_t s_w[4];
   __shared__ uint32_t s_base;
   __shared__ unsigned long long s_base64;
@@ -120,6 +119,7 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, const in
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     // shade queues are filled by the classify stage of this iteration
     C[C_SHADE + 0] = 0; C[C_SHADE + 1] = 0; C[C_SHADE + 2] = 0; C[C_SHADE + 3] = 0;
+    if (GUIDED) *G.gq_count = 0;
   }
   for (int j = threadIdx.x; j < kMaxJobsDev; j += blockDim.x) s_done[j] = 0;
   if (threadIdx.x == 0) s_nan = 0;
@@ -130,6 +130,7 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, const in
     const int base = chunk * kRegenSlots + threadIdx.x;
     uint32_t need = 0;  // bit k: slot base + k*256 is free
     uint32_t act = 0;   // bit k: slot is active after this kernel
+    uint32_t trn = 0;   // bit k: slot yields a guide training sample
 #pragma unroll 1
     for (int k = 0; k < kRegenK; k++) {
       int i = base + k * 256;
@@ -141,7 +142,8 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, const in
         const DevJob& J = jobs[jb];
         float4 L = P.L[i];
         f3 l = mk(L.x, L.y, L.z);
-        if (isnan(l.x) || isnan(l.y) || isnan(l.z)) {
+        const bool bad = isnan(l.x) || isnan(l.y) || isnan(l.z);
+        if (bad) {
           atomicAdd(&s_nan, 1u);
         } else {
           uint32_t pix = P.pixel[i], si = P.sample[i];
@@ -150,12 +152,43 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, const in
           sample2d(rng, 0, sx, sy);
           splat(J, pix, sx, sy, l);
         }
+        if (GUIDED) {
+          const uint8_t rs = G.rstate[i];
+          if (rs == 2 && !bad) {
+            float4 r2 = G.rec[3 * (size_t)i + 2];
+            float w = luminance(mk(l.x - r2.y, l.y - r2.z, l.z - r2.w));
+            if (w > 0.f && w < 3.0e38f) {
+              r2.y = w;
+              G.rec[3 * (size_t)i + 2] = r2;
+              trn |= 1u << k;
+            }
+          }
+          if (rs) G.rstate[i] = 0;
+        }
         atomicAdd(&s_done[jb], 1u);
         P.state[i] = ST_FREE;
         st = ST_FREE;
       }
       if (st == ST_FREE) need |= 1u << k;
       if (st == ST_ACTIVE) act |= 1u << k;
+    }
+    if (GUIDED) {
+      uint32_t tot;
+      uint32_t toff = block_excl_scan<4, uint32_t>((uint32_t)__popc(trn), s_w, tot);
+      if (threadIdx.x == 0) s_base64 = tot ? atomicAdd(G.ring_count, (unsigned long long)tot) : 0ull;
+      __syncthreads();
+      unsigned long long rb = s_base64 + toff;
+      __syncthreads();
+#pragma unroll 1
+      for (int k = 0; k < kRegenK; k++) {
+        if (!(trn & (1u << k))) continue;
+        const size_t i = (size_t)(base + k * 256);
+        const float4 r0 = G.rec[3 * i], r1 = G.rec[3 * i + 1], r2 = G.rec[3 * i + 2];
+        float* d = G.ring + 10 * (size_t)(rb & G.ring_mask);
+        d[0] = r0.x; d[1] = r0.y; d[2] = r0.z; d[3] = r0.w; d[4] = r1.x;
+        d[5] = r1.y; d[6] = r1.z; d[7] = r1.w; d[8] = r2.x; d[9] = r2.y;
+        rb++;
+      }
     }
     // claims, jobs in the host's claim order
     for (int jj = 0; jj < norder; jj++) {
@@ -290,9 +323,14 @@ __global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scene
 // ---------------------------------------------------------------- classify: shade queues
 // Chunks of 2048 extension-queue entries per block; the four per-class counts are packed into
 // 16-bit fields of one 64-bit word so a single block scan serves all four queues.
-__global__ void __launch_bounds__(256) k_classify(DevPool P) {
+// GUIDED: also queues the guided depth-0 vertices (diffuse and rough-conductor hits of the guided
+// job) for the guide network's inference.
+template <bool GUIDED>
+__global__ void __launch_bounds__(256) k_classify(DevPool P, DevGuide G) {
   __shared__ unsigned long long s_w[4];
   __shared__ uint32_t s_base[4];
+  __shared__ uint32_t s_w32[4];
+  __shared__ uint32_t s_gbase;
   uint32_t* C = P.counters;
   const uint32_t n = C[C_EXT];
   const uint32_t nchunks = (n + kRegenSlots - 1) / kRegenSlots;
@@ -326,6 +364,23 @@ __global__ void __launch_bounds__(256) k_classify(DevPool P) {
         pos[c]++;
       }
     }
+    if (GUIDED) {
+      uint32_t gmask = 0;
+#pragma unroll
+      for (int k = 0; k < kRegenK; k++) {
+        if (G.sample && cls[k] < 2) {
+          const uint32_t f = P.flags[P.q_ext[base + k * 256]];
+          if (fl_depth(f) == 0 && (int)fl_job(f) == G.job) gmask |= 1u << k;
+        }
+      }
+      uint32_t gtot;
+      uint32_t goff = block_excl_scan<4, uint32_t>((uint32_t)__popc(gmask), s_w32, gtot);
+      uint32_t gb = block_reserve(G.gq_count, gtot, &s_gbase);
+      uint32_t gp = gb + goff;
+#pragma unroll
+      for (int k = 0; k < kRegenK; k++)
+        if (gmask & (1u << k)) G.gq[gp++] = P.q_ext[base + k * 256];
+    }
   }
 }
 
@@ -349,8 +404,20 @@ __device__ __forceinline__ void prev_segment(const DevScene& S, f3 Le, float pdf
   T.x = t.x; T.y = t.y; T.z = t.z;
 }
 
-template <int MT>
-__global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes, const DevJob* jobs) {
+// Radiance snapshot for guide training, taken at the first event after a guided depth-0 vertex
+// (before that event adds anything): what the path gathers afterwards is the radiance arriving
+// along the guided direction, weighted by the vertex's sampling weight.
+__device__ __forceinline__ void guide_snapshot(const DevGuide& G, uint32_t slot, const float4& L) {
+  if (G.rstate[slot] == 1) {
+    float4 r2 = G.rec[3 * (size_t)slot + 2];
+    r2.y = L.x; r2.z = L.y; r2.w = L.z;
+    G.rec[3 * (size_t)slot + 2] = r2;
+    G.rstate[slot] = 2;
+  }
+}
+
+template <int MT, bool GUIDED>
+__global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes, const DevJob* jobs, DevGuide G) {
   uint32_t* C = P.counters;
   const uint32_t n = C[C_SHADE + MT];
   const uint32_t* q = P.q_shade + (size_t)MT * P.capacity;
@@ -369,6 +436,7 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
       HitRec h;
       h.t = hv.x; h.b1 = hv.y; h.b2 = hv.z; h.prim = __float_as_int(hv.w);
       Surface sf = surface_at(S, h, o, d);
+      if (GUIDED && depth == 1) guide_snapshot(G, slot, L);
       if (depth > 0) {
         bool caught = sf.light != 0xffffffffu;
         f3 Le = mk(0.f, 0.f, 0.f);
@@ -424,7 +492,49 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
         float bx, by;
         sample2d(rng, 2u + 2u * depth, bx, by);
         BsdfSample bs;
-        bool ok = bsdf_sample<MT>(M, wo_l, bx, by, bs);
+        bool ok;
+        if (GUIDED && MT != 2 && depth == 0 && (int)fl_job(f) == G.job) {
+          // one-sample MIS of BSDF sampling and the guide, each chosen with probability 1/2:
+          // weight = (sampler weight * sampler density) / mixture density, so the estimator keeps
+          // the plain tracer's expectation; the MIS weights of the next hit use the BSDF's pdf.
+          const float* mx = G.mix + (size_t)slot * kGuideOut;
+          const float us = G.sample ? sample1d(rng, kGuideSelDim + depth) : 0.f;
+          f3 wl;
+          if (us < 0.5f) {
+            ok = bsdf_sample<MT>(M, wo_l, bx, by, bs);
+            wl = bs.wo;
+          } else {
+            wl = to_local(sf.sh, mix_sample(mx, (us - 0.5f) * 2.f, bx, by));
+            ok = true;
+          }
+          f3 wref;
+          float ps = 0.f, pref = 0.f, rel = 1.f;
+          if (ok) ok = bsdf_density<MT>(M, wo_l, wl, wref, ps, pref, rel);
+          const f3 wd = to_world(sf.sh, wl);
+          if (ok) {
+            const float pm = G.sample ? 0.5f * ps + 0.5f * mix_pdf(mx, wd) : ps;
+            if (us < 0.5f) {
+              wref = bs.f;
+              pref = bs.pdf;
+              rel = bs.rel_eta;
+            }
+            bs.wo = wl;
+            bs.f = wref * (ps / pm);
+            bs.pdf = pref;
+            bs.rel_eta = rel;
+            ok = pm > 0.f;
+          }
+          if (G.record) {
+            const f3 np = mk((sf.p.x - G.bmin[0]) * G.inv_ext[0], (sf.p.y - G.bmin[1]) * G.inv_ext[1],
+                             (sf.p.z - G.bmin[2]) * G.inv_ext[2]);
+            G.rec[3 * (size_t)slot] = make_float4(np.x, np.y, np.z, sf.sh.n.x);
+            G.rec[3 * (size_t)slot + 1] = make_float4(sf.sh.n.y, sf.sh.n.z, wd.x, wd.y);
+            G.rec[3 * (size_t)slot + 2] = make_float4(wd.z, 0.f, 0.f, 0.f);
+            G.rstate[slot] = ok ? 1 : 0;
+          }
+        } else {
+          ok = bsdf_sample<MT>(M, wo_l, bx, by, bs);
+        }
         if (!ok || is_zero(bs.f)) {
           done = true;
         } else {
@@ -457,7 +567,9 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
 }
 
 // Misses and hits without a BSDF (PathTracingMIS.cpp:38-51,67-70,182-189,217-219)
-__global__ void __launch_bounds__(256) k_shade_miss(DevPool P, const DevScene* scenes, const DevJob* jobs) {
+template <bool GUIDED>
+__global__ void __launch_bounds__(256) k_shade_miss(DevPool P, const DevScene* scenes, const DevJob* jobs,
+                                                    DevGuide G) {
   uint32_t* C = P.counters;
   const uint32_t n = C[C_SHADE + 3];
   const uint32_t* q = P.q_shade + (size_t)3 * P.capacity;
@@ -472,6 +584,7 @@ __global__ void __launch_bounds__(256) k_shade_miss(DevPool P, const DevScene* s
     float4 T = P.T[slot], F = P.F[slot], L = P.L[slot];
     f3 o = mk(ro.x, ro.y, ro.z), d = mk(rd.x, rd.y, rd.z);
     int prim = __float_as_int(hv.w);
+    if (GUIDED && depth == 1) guide_snapshot(G, slot, L);
     if (prim >= 0) {
       // hit without a BSDF: the emission of the previous segment still counts, then the path ends
       if (depth > 0) {
@@ -631,6 +744,16 @@ __global__ void k_bsdf_tap(const DevMaterial* mat, int op, const float* wi, cons
     else { e = bsdf_eval<2>(M, a, b); pd = bsdf_pdf<2>(M, a, b); }
     f[3 * i] = e.x; f[3 * i + 1] = e.y; f[3 * i + 2] = e.z;
     pdf[i] = pd;
+  } else if (op == 2) {
+    // sampling density of bsdf_sample at wo, the weight it returns there and the pdf it reports
+    f3 b = ld3(wo_or_u + 3 * (size_t)i);
+    f3 w = mk(0.f, 0.f, 0.f);
+    float ps = 0.f, pref = 0.f, rel = 1.f;
+    bool ok = M.type == 0 ? bsdf_density<0>(M, a, b, w, ps, pref, rel) : bsdf_density<1>(M, a, b, w, ps, pref, rel);
+    if (!ok) { w = mk(0.f, 0.f, 0.f); ps = 0.f; pref = 0.f; }
+    f[3 * i] = w.x; f[3 * i + 1] = w.y; f[3 * i + 2] = w.z;
+    pdf[i] = ps;
+    rel_eta[i] = pref;
   } else {
     float ux = wo_or_u[2 * i], uy = wo_or_u[2 * i + 1];
     BsdfSample bs;
@@ -655,29 +778,55 @@ static size_t stack_bytes(int block) { return (size_t)block * kLdsStack * 8u; }
 
 void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, const int32_t* order_dev,
                       int norder, const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev, const DevGuide* guide) {
-  (void)guide;
   const int g = cfg.grid;
   const DevJob* cj = jobs_dev;
   const int chunks = (pool.capacity + kRegenSlots - 1) / kRegenSlots;
+  const bool gd = guide && guide->enabled;
+  DevGuide G;
+  if (gd) {
+    G = *guide;
+  } else {
+    memset(&G, 0, sizeof(G));
+    G.job = -1;
+  }
   if (ev) (void)hipEventRecord(ev[0], stream);
-  hipLaunchKernelGGL(k_regen, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, norder);
+  if (gd)
+    hipLaunchKernelGGL(k_regen<true>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, norder, G);
+  else
+    hipLaunchKernelGGL(k_regen<false>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, norder, G);
   if (ev) (void)hipEventRecord(ev[1], stream);
   hipLaunchKernelGGL(k_extend, dim3(g * 2), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
                      cfg.gstack, cfg.stack);
-  hipLaunchKernelGGL(k_classify, dim3(chunks), dim3(256), 0, stream, pool);
+  if (gd)
+    hipLaunchKernelGGL(k_classify<true>, dim3(chunks), dim3(256), 0, stream, pool, G);
+  else
+    hipLaunchKernelGGL(k_classify<false>, dim3(chunks), dim3(256), 0, stream, pool, G);
   if (ev) (void)hipEventRecord(ev[2], stream);
-  hipLaunchKernelGGL(k_shade<0>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj);
-  if (ev) (void)hipEventRecord(ev[3], stream);
-  hipLaunchKernelGGL(k_shade<1>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj);
-  if (ev) (void)hipEventRecord(ev[4], stream);
-  hipLaunchKernelGGL(k_shade<2>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj);
-  if (ev) (void)hipEventRecord(ev[5], stream);
-  hipLaunchKernelGGL(k_shade_miss, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj);
-  if (ev) (void)hipEventRecord(ev[6], stream);
+  if (gd) {
+    if (G.sample) launch_guide_infer_pool(G, pool, scenes_dev, cj, g, stream);
+    if (ev) (void)hipEventRecord(ev[3], stream);
+    hipLaunchKernelGGL((k_shade<0, true>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+    if (ev) (void)hipEventRecord(ev[4], stream);
+    hipLaunchKernelGGL((k_shade<1, true>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+    if (ev) (void)hipEventRecord(ev[5], stream);
+    hipLaunchKernelGGL((k_shade<2, true>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+    if (ev) (void)hipEventRecord(ev[6], stream);
+    hipLaunchKernelGGL(k_shade_miss<true>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+  } else {
+    if (ev) (void)hipEventRecord(ev[3], stream);
+    hipLaunchKernelGGL((k_shade<0, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+    if (ev) (void)hipEventRecord(ev[4], stream);
+    hipLaunchKernelGGL((k_shade<1, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+    if (ev) (void)hipEventRecord(ev[5], stream);
+    hipLaunchKernelGGL((k_shade<2, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+    if (ev) (void)hipEventRecord(ev[6], stream);
+    hipLaunchKernelGGL(k_shade_miss<false>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+  }
+  if (ev) (void)hipEventRecord(ev[7], stream);
   hipLaunchKernelGGL(k_compact_shadow, dim3(chunks), dim3(256), 0, stream, pool);
   hipLaunchKernelGGL(k_shadow, dim3(g * 2), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
                      cfg.gstack, cfg.stack);
-  if (ev) (void)hipEventRecord(ev[7], stream);
+  if (ev) (void)hipEventRecord(ev[8], stream);
 }
 
 void launch_reset_pool(const DevPool& pool, hipStream_t stream) {
@@ -698,6 +847,22 @@ void launch_bsdf_tap(const DevMaterial* mat, int op, const float* wi, const floa
   if (blocks == 0) return;
   hipLaunchKernelGGL(k_bsdf_tap, dim3(blocks), dim3(256), 0, stream, mat, op, wi, wo_or_u, n, out_dir, f, pdf,
                      rel_eta);
+}
+
+// Mixture sampling tap: directions from (pi, mu, kappa) x 8 with u = (lobe, u1, u2), and the
+// mixture density at each sampled direction.
+__global__ void k_guide_sample_tap(const float* mix, const float* u, int n, float* dirs, float* pdf) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const float* mx = mix + (size_t)kGuideOut * i;
+  f3 d = mix_sample(mx, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
+  dirs[3 * i] = d.x; dirs[3 * i + 1] = d.y; dirs[3 * i + 2] = d.z;
+  pdf[i] = mix_pdf(mx, d);
+}
+
+void launch_guide_sample_tap(const float* mix, const float* u, int n, float* dirs, float* pdf, hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_guide_sample_tap, dim3((n + 255) / 256), dim3(256), 0, stream, mix, u, n, dirs, pdf);
 }
 
 void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream) {

@@ -208,6 +208,7 @@ int rtg_submit(rtg_server* s, rtg_scene* scene, const rtg_render_desc* desc, int
   std::lock_guard<std::mutex> lk(s->qm);
   try {
     // validate now so a bad request fails at submission
+    if (desc->guiding) fail(RTG_ERR_INVALID, "guided renders are not served (use rtg_render)");
     (void)make_job(ctx, scene, *desc, nullptr);
     std::unique_ptr<Request> r(new Request());
     r->id = s->next_id++;

@@ -61,7 +61,9 @@ class rtg_stats(C.Structure):
         ("samples", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
         ("nan_samples", C.c_uint64), ("iterations", C.c_uint64), ("milliseconds", C.c_double),
         ("nodes_visited", C.c_uint64), ("prims_tested", C.c_uint64), ("shadow_nodes_visited", C.c_uint64),
-        ("shadow_prims_tested", C.c_uint64), ("stage_ms", C.c_double * 7), ("stage_launches", C.c_uint64 * 7),
+        ("shadow_prims_tested", C.c_uint64), ("stage_ms", C.c_double * 8), ("stage_launches", C.c_uint64 * 8),
+        ("guide_train_steps", C.c_uint64), ("guide_samples", C.c_uint64), ("guide_loss", C.c_double),
+        ("guide_train_ms", C.c_double), ("guide_queries", C.c_uint64),
     ]
 
 
@@ -74,7 +76,7 @@ class rtg_server_stats(C.Structure):
     ]
 
 
-STAGES = ("regen", "extend", "shade_diffuse", "shade_conductor", "shade_dielectric", "shade_miss", "shadow")
+STAGES = ("regen", "extend", "guide", "shade_diffuse", "shade_conductor", "shade_dielectric", "shade_miss", "shadow")
 PROFILE_EVENTS, PROFILE_COUNT = 1, 2
 
 
@@ -101,7 +103,12 @@ SIGNATURES = {
     "rtg_poll": (C.c_int, [P, C.c_int64, P, C.POINTER(C.c_double)]),
     "rtg_cancel": (C.c_int, [P, C.c_int64]),
     "rtg_server_query": (C.c_int, [P, C.POINTER(rtg_server_stats)]),
+    "rtg_guide_eval": (C.c_int, [P, P, P, C.c_int32, P]),
+    "rtg_guide_grad": (C.c_int, [P, P, P, C.c_int32, P, P]),
+    "rtg_guide_train": (C.c_int, [P, P, P, C.c_int32, C.c_int32, C.c_float, P, P]),
+    "rtg_guide_sample": (C.c_int, [P, P, P, C.c_int32, P, P]),
 }
+GUIDE_PARAMS = 20200
 
 _lib = None
 
@@ -217,6 +224,43 @@ class Context:
         m = material_struct(material)
         self.check(self.lib.rtg_bsdf_batch(self.h, C.byref(m), 0, _ptr(wi), _ptr(wo), n, None, _ptr(f), _ptr(pdf), None))
         return f, pdf
+
+    def bsdf_density(self, material, wi, wo):
+        """Sampling density of bsdf_sample at wo, its weight there and its reported pdf."""
+        wi = _f32(wi, (-1, 3)); wo = _f32(wo, (-1, 3))
+        n = wi.shape[0]
+        f = np.zeros((n, 3), np.float32); ps = np.zeros(n, np.float32); pref = np.zeros(n, np.float32)
+        m = material_struct(material)
+        self.check(self.lib.rtg_bsdf_batch(self.h, C.byref(m), 2, _ptr(wi), _ptr(wo), n, None, _ptr(f), _ptr(ps),
+                                           _ptr(pref)))
+        return ps, f, pref
+
+    def guide_eval(self, params, in6):
+        params = _f32(params, (GUIDE_PARAMS,)); in6 = _f32(in6, (-1, 6))
+        out = np.zeros((in6.shape[0], 40), np.float32)
+        self.check(self.lib.rtg_guide_eval(self.h, _ptr(params), _ptr(in6), in6.shape[0], _ptr(out)))
+        return out
+
+    def guide_grad(self, params, samples):
+        params = _f32(params, (GUIDE_PARAMS,)); samples = _f32(samples, (-1, 10))
+        loss = np.zeros(1, np.float32); grad = np.zeros(GUIDE_PARAMS, np.float32)
+        self.check(self.lib.rtg_guide_grad(self.h, _ptr(params), _ptr(samples), samples.shape[0], _ptr(loss),
+                                           _ptr(grad)))
+        return float(loss[0]), grad
+
+    def guide_train(self, params, samples, steps, lr=1e-3):
+        params = _f32(params, (GUIDE_PARAMS,)); samples = _f32(samples, (-1, 10))
+        out = np.zeros(GUIDE_PARAMS, np.float32); losses = np.zeros(max(1, steps), np.float32)
+        self.check(self.lib.rtg_guide_train(self.h, _ptr(params), _ptr(samples), samples.shape[0], int(steps),
+                                            float(lr), _ptr(out), _ptr(losses)))
+        return out, losses[:steps]
+
+    def guide_sample(self, mix, u):
+        mix = _f32(mix, (-1, 40)); u = _f32(u, (-1, 3))
+        n = mix.shape[0]
+        d = np.zeros((n, 3), np.float32); p = np.zeros(n, np.float32)
+        self.check(self.lib.rtg_guide_sample(self.h, _ptr(mix), _ptr(u), n, _ptr(d), _ptr(p)))
+        return d, p
 
     def bsdf_sample(self, material, wi, u):
         wi = _f32(wi, (-1, 3)); u = _f32(u, (-1, 2))

@@ -297,7 +297,10 @@ def displaced_sphere(width=1024, height=1024, spp=64, max_depth=6, min_depth=3, 
 
 def glossy_cornell(width=512, height=512, spp=64, max_depth=1000, min_depth=3, seed=5):
     """Glossy-indirect scene: rough dielectric spheres (IOR 1.5, alpha 0.1), rough GGX conductor walls
-    (alpha 0.15), a 0.05x0.05 light hidden above a baffle so illumination is mostly indirect."""
+    (alpha 0.15), a white diffuse floor and ceiling, and a 0.05x0.05 light 0.1 below the ceiling that
+    faces up, hidden from the room by a 0.2x0.2 baffle just beneath it: the room is lit by the bright
+    patch of ceiling above the light (illumination is indirect; next-event estimation from the room
+    cannot reach the light)."""
     sd = SceneDesc()
     white = sd.add_material(**diffuse((0.73, 0.73, 0.73)))
     wall_l = sd.add_material(**conductor(0.15, eta=(0.2, 0.92, 1.1), k=(3.9, 2.45, 2.14)))   # copper-like
@@ -306,10 +309,10 @@ def glossy_cornell(width=512, height=512, spp=64, max_depth=1000, min_depth=3, s
     glass = sd.add_material(**dielectric(0.1, eta_int=1.5, eta_ext=1.000277))
     black = sd.add_material(**diffuse((0.0, 0.0, 0.0)))
     _cornell_walls(sd, white, wall_l, wall_r, left_m=wall_l, right_m=wall_r, back_m=wall_b)
-    bv, bi = quad((-0.1, 0.95, -0.1), (0.2, 0, 0), (0, 0, 0.2), (0, 1, 0))  # baffle, lit face up
-    sd.meshes.append(MeshDesc(bv, bi, material=white))
-    lv, li = quad((-0.025, 0.999, -0.025), (0.05, 0, 0), (0, 0, 0.05), (0, -1, 0))
+    lv, li = quad((-0.025, 0.9, -0.025), (0.05, 0, 0), (0, 0, 0.05), (0, 1, 0))  # light, facing the ceiling
     sd.meshes.append(MeshDesc(lv, li, material=black, emission=(425.0, 300.0, 100.0)))
+    bv, bi = quad((-0.1, 0.895, -0.1), (0.2, 0, 0), (0, 0, 0.2), (0, -1, 0))  # baffle under the light
+    sd.meshes.append(MeshDesc(bv, bi, material=white))
     for c, r in (((-0.2, 0.18, -0.1), 0.18), ((0.22, 0.15, 0.15), 0.15)):
         M = np.eye(4, dtype=np.float32); M[:3, 3] = c
         sd.spheres.append(SphereDesc(M, r, glass))

@@ -34,4 +34,11 @@ void launch_trace_tap(const DevScene*, const float*, int, int, int32_t*, float*,
 void launch_bsdf_tap(const DevMaterial*, int, const float*, const float*, int, float*, float*, float*, float*,
                      hipStream_t) {}
 void launch_fill_zero(void* p, size_t b, hipStream_t) { memset(p, 0, b); }
+// guiding kernels are not exercised by the host-logic test
+void launch_guide_refresh(const float*, uint16_t*, uint16_t*, float*, hipStream_t) {}
+void launch_guide_infer_pool(const DevGuide&, const DevPool&, const DevScene*, const DevJob*, int, hipStream_t) {}
+void launch_guide_infer_array(const DevGuide&, const float*, int, float*, hipStream_t) {}
+void launch_guide_grad(const DevGuide&, const float*, uint64_t, uint32_t, int, const GuideTrainScratch&, hipStream_t) {}
+void launch_guide_adam(float*, float*, float*, const float*, int, float, uint16_t*, uint16_t*, float*, hipStream_t) {}
+void launch_guide_sample_tap(const float*, const float*, int, float*, float*, hipStream_t) {}
 }  // namespace rtg

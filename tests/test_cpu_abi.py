@@ -34,7 +34,7 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(rtg.rtg_material) == 4 * 17
     assert ctypes.sizeof(rtg.rtg_sphere_desc) == 4 * 18
     assert ctypes.sizeof(rtg.rtg_render_desc) == 4 * 20 + 8 + 8  # 20 words, u64 seed, int + pad
-    assert ctypes.sizeof(rtg.rtg_stats) == 8 * 6 + 8 * 4 + 8 * 7 * 2
+    assert ctypes.sizeof(rtg.rtg_stats) == 8 * 6 + 8 * 4 + 8 * 8 * 2 + 8 * 5
     assert ctypes.sizeof(rtg.rtg_server_stats) == 8 * 8 + 4 * 4
 
 

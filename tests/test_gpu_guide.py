@@ -1,0 +1,295 @@
+"""Path guiding on the GPU: the sampling density used by one-sample MIS, the network's bf16 MFMA
+forward and backward passes against the FP64 oracle (oracle/guide_oracle.cpp, with the same bf16
+rounding points), mixture sampling, Adam training, and guided renders (unbiased with respect to the
+plain tracer, and what the trained network gains).
+
+Tolerances: the oracle emulates the device's bf16 rounding, so differences come from FP32 vs FP64
+accumulation order and the rare bf16 rounding flip it causes: forward outputs within 1e-3 relative
+for 99.9% of values, gradient within 1% relative L2."""
+import time
+
+import numpy as np
+import pytest
+
+from raytracer_amd import rtg, scenes
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = rtg.Context(0)
+    yield c
+    c.close()
+
+
+def _unit(v):
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def _samples(rng, n):
+    s = np.zeros((n, 10), np.float32)
+    s[:, :3] = rng.random((n, 3))
+    s[:, 3:6] = _unit(rng.normal(size=(n, 3)))
+    s[:, 6:9] = _unit(rng.normal(size=(n, 3)))
+    s[:, 9] = rng.random(n) * 2.0
+    return s
+
+
+# ---------------------------------------------------------------- sampling density
+@pytest.mark.parametrize("mat", [
+    scenes.diffuse((0.7, 0.5, 0.3)),
+    scenes.conductor(0.15, dist=rtg.DIST_GGX),
+    scenes.conductor(0.3, dist=rtg.DIST_BECKMANN),
+], ids=["diffuse", "ggx0.15", "beckmann0.3"])
+def test_bsdf_density_is_the_samplers(ctx, mat):
+    """The density one-sample MIS divides by must be the one bsdf_sample draws from: histogram of
+    2^21 sampled directions against the density integrated over each bin; the weight and pdf op 2
+    reports at a sampled direction equal what the sampler returned with it."""
+    rng = np.random.default_rng(3)
+    wi = _unit(np.array([0.45, 0.1, 0.6]))
+    n = 1 << 21
+    wo, f, pdf, _ = ctx.bsdf_sample(mat, np.repeat(wi[None], n, 0), rng.random((n, 2)))
+    ok = (f.max(axis=1) > 0) & (wo[:, 2] > 0)
+    ps, w2, pref = ctx.bsdf_density(mat, np.repeat(wi[None], ok.sum(), 0), wo[ok])
+    assert np.all(ps > 0)
+    assert np.allclose(w2, f[ok], rtol=2e-4, atol=1e-7)
+    assert np.allclose(pref, pdf[ok], rtol=2e-4)
+    # histogram over (cos theta, phi)
+    nt, nphi, sub = 48, 64, 6
+    ct = np.clip(wo[ok, 2], 0, 1 - 1e-9)
+    ph = np.mod(np.arctan2(wo[ok, 1], wo[ok, 0]), 2 * np.pi)
+    obs, _, _ = np.histogram2d(ct, ph, bins=[nt, nphi], range=[[0, 1], [0, 2 * np.pi]])
+    # expected: density on a sub-grid of each bin (uniform in cos theta and phi = uniform solid angle)
+    c = (np.arange(nt * sub) + 0.5) / (nt * sub)
+    p = (np.arange(nphi * sub) + 0.5) * 2 * np.pi / (nphi * sub)
+    C, Pp = np.meshgrid(c, p, indexing="ij")
+    S = np.sqrt(1 - C * C)
+    dirs = np.stack([S * np.cos(Pp), S * np.sin(Pp), C], -1).reshape(-1, 3)
+    dens, _, _ = ctx.bsdf_density(mat, np.repeat(wi[None], dirs.shape[0], 0), dirs)
+    dA = (1.0 / (nt * sub)) * (2 * np.pi / (nphi * sub))
+    exp = (dens.reshape(nt, sub, nphi, sub).sum(axis=(1, 3)) * dA) * n
+    assert abs(exp.sum() / n - ok.mean()) < 0.01, (exp.sum() / n, ok.mean())
+    big = exp > 100
+    z = (obs[big] - exp[big]) / np.sqrt(exp[big])
+    print(f"  {mat.get('type')}: bins {big.sum()} max|z| {np.abs(z).max():.2f} mean z^2 {np.mean(z * z):.3f} "
+          f"(expected 1; a density off by 1% in a bin of 10^4 samples gives z ~ 1)", flush=True)
+    assert np.mean(z * z) < 1.5
+    assert np.abs(z).max() < 6.0
+
+
+# ---------------------------------------------------------------- network vs oracle
+def test_guide_forward_matches_oracle(ctx):
+    rng = np.random.default_rng(11)
+    params = O.guide_init(12, scale=2.0)
+    x = np.zeros((4096, 6), np.float32)
+    x[:, :3] = rng.random((4096, 3))
+    x[:, 3:] = _unit(rng.normal(size=(4096, 3)))
+    g = ctx.guide_eval(params, x).astype(np.float64)
+    _, ref = O.guide_forward(params, x, emulate_bf16=True)
+    err = np.abs(g - ref) / np.maximum(np.abs(ref), 1e-3)
+    print(f"  forward rel err: median {np.median(err):.2e} p99.9 {np.quantile(err, 0.999):.2e} max {err.max():.2e}",
+          flush=True)
+    assert np.quantile(err, 0.999) < 1e-3
+    assert np.mean(err > 1e-2) < 1e-4  # isolated bf16 rounding flips only
+    assert np.allclose(g[:, 0::5].sum(1), 1.0, atol=1e-5)
+
+
+def test_guide_gradient_matches_oracle(ctx):
+    rng = np.random.default_rng(13)
+    params = O.guide_init(14, scale=2.0)
+    s = _samples(rng, 8192)
+    loss_g, grad_g = ctx.guide_grad(params, s)
+    loss_o, grad_o = O.guide_loss_grad(params, s, emulate_bf16=True)
+    rel = np.linalg.norm(grad_g - grad_o) / np.linalg.norm(grad_o)
+    cos = np.dot(grad_g, grad_o) / (np.linalg.norm(grad_g) * np.linalg.norm(grad_o))
+    print(f"  loss gpu {loss_g:.6f} oracle {loss_o:.6f}; grad rel L2 {rel:.2e} cos {cos:.6f}", flush=True)
+    assert abs(loss_g - loss_o) < 1e-4 * abs(loss_o)
+    assert rel < 1e-2
+    for name, (off, shp) in O.GUIDE_SLICES.items():
+        k = int(np.prod(shp))
+        a, b = grad_g[off:off + k], grad_o[off:off + k]
+        assert np.linalg.norm(a - b) <= 2e-2 * np.linalg.norm(b) + 1e-9, name
+
+
+def test_adam_step_matches_reference(ctx):
+    """Two device Adam steps equal Adam (Kingma & Ba) applied on the host to the oracle's
+    gradients, to bf16-level tolerance."""
+    rng = np.random.default_rng(15)
+    p0 = O.guide_init(16)
+    s = _samples(rng, 4096)
+    lr = 1e-3
+    p_dev, _ = ctx.guide_train(p0, s, steps=2, lr=lr)
+    p = p0.astype(np.float64).copy()
+    m = np.zeros_like(p); v = np.zeros_like(p)
+    for t in (1, 2):
+        _, g = O.guide_loss_grad(p.astype(np.float32), s, emulate_bf16=True)
+        m = 0.9 * m + 0.1 * g
+        v = 0.999 * v + 0.001 * g * g
+        p -= lr * (m / (1 - 0.9 ** t)) / (np.sqrt(v / (1 - 0.999 ** t)) + 1e-8)
+    d_dev = p_dev - p0
+    d_ref = p - p0
+    # Adam's first steps move every parameter by ~lr * sign(g); parameters with tiny gradients
+    # may differ in sign between FP32 and FP64 sums, so compare where the gradient is not tiny
+    _, g0 = O.guide_loss_grad(p0, s, emulate_bf16=True)
+    sel = np.abs(g0) > 1e-3 * np.abs(g0).max()
+    agree = np.mean(np.abs(d_dev[sel] - d_ref[sel]) < 0.1 * lr)
+    print(f"  Adam: {sel.sum()} params compared, agreement {agree:.4f}", flush=True)
+    assert agree > 0.99
+
+
+def _vmf_cos_cdf(w, k):
+    """CDF of cos(angle to the mean) under a vMF lobe: (e^{k(w-1)} - e^{-2k}) / (1 - e^{-2k})."""
+    return (np.exp(k * (w - 1)) - np.exp(-2 * k)) / (-np.expm1(-2 * k))
+
+
+@pytest.mark.parametrize("kappa", [0.1, 5.0, 100.0, 2000.0])
+def test_vmf_lobe_sampling_exact(ctx, kappa):
+    """One lobe: the cosine to the mean follows its closed-form CDF (Kolmogorov-Smirnov) and the
+    azimuth about the mean is uniform; the reported density equals the closed form."""
+    rng = np.random.default_rng(int(kappa * 10) + 1)
+    mu = _unit(np.array([0.3, -0.5, 0.8]))
+    mix = np.zeros((1, 40), np.float32)
+    mix[0, 0::5] = 0.0; mix[0, 4::5] = 1.0; mix[0, 3::5] = 1.0
+    mix[0, 0] = 1.0; mix[0, 1:4] = mu; mix[0, 4] = kappa
+    n = 1 << 18
+    d, pdf = ctx.guide_sample(np.repeat(mix, n, 0), rng.random((n, 3)))
+    w = np.clip(d.astype(np.float64) @ mu, -1, 1)
+    ws = np.sort(w)
+    F = _vmf_cos_cdf(ws, kappa)
+    ks = np.max(np.abs(F - (np.arange(1, n + 1) / n)))
+    t = _unit(np.cross(mu, [0.0, 0.0, 1.0])); b = np.cross(mu, t)
+    ph = np.mod(np.arctan2(d @ b, d @ t), 2 * np.pi)
+    hist, _ = np.histogram(ph, bins=32, range=(0, 2 * np.pi))
+    zph = (hist - n / 32) / np.sqrt(n / 32)
+    closed = kappa / (2 * np.pi * -np.expm1(-2 * kappa)) * np.exp(kappa * (w - 1))
+    print(f"  kappa {kappa}: KS {ks:.5f} (1% critical {1.63 / np.sqrt(n):.5f}) azimuth max|z| {np.abs(zph).max():.2f}",
+          flush=True)
+    assert ks < 1.63 / np.sqrt(n)
+    assert np.abs(zph).max() < 5.0
+    assert np.allclose(pdf, closed, rtol=2e-3)
+
+
+def test_mixture_sampling_matches_density(ctx):
+    """A mixture of moderate lobes: histogram of sampled directions against the density
+    integrated per bin (mean z^2 ~ 1), and the reported density equals the oracle's."""
+    rng = np.random.default_rng(17)
+    mix = np.zeros((1, 40), np.float32)
+    pis = rng.random(8) + 0.2
+    mix[0, 0::5] = pis / pis.sum()
+    for k in range(8):
+        mix[0, 5 * k + 1:5 * k + 4] = _unit(rng.normal(size=3))
+    mix[0, 4::5] = [0.5, 2.0, 5.0, 8.0, 12.0, 20.0, 30.0, 3.0]
+    n = 1 << 21
+    d, pdf = ctx.guide_sample(np.repeat(mix, n, 0), rng.random((n, 3)))
+    assert np.allclose(np.linalg.norm(d, axis=1), 1.0, atol=1e-5)
+    lp = O.guide_mix_logpdf(np.repeat(mix.astype(np.float64), 4096, 0), d[:4096].astype(np.float64))
+    assert np.allclose(pdf[:4096], np.exp(lp), rtol=2e-3)
+    nt, nphi, sub = 32, 64, 8
+    ct = np.clip((d[:, 2] + 1) / 2, 0, 1 - 1e-9)
+    ph = np.mod(np.arctan2(d[:, 1], d[:, 0]), 2 * np.pi)
+    obs, _, _ = np.histogram2d(ct, ph, bins=[nt, nphi], range=[[0, 1], [0, 2 * np.pi]])
+    c = (np.arange(nt * sub) + 0.5) / (nt * sub) * 2 - 1
+    p = (np.arange(nphi * sub) + 0.5) * 2 * np.pi / (nphi * sub)
+    C, Pp = np.meshgrid(c, p, indexing="ij")
+    S = np.sqrt(1 - C * C)
+    dirs = np.stack([S * np.cos(Pp), S * np.sin(Pp), C], -1).reshape(-1, 3)
+    dens = np.exp(O.guide_mix_logpdf(np.repeat(mix.astype(np.float64), dirs.shape[0], 0), dirs))
+    exp = dens.reshape(nt, sub, nphi, sub).sum(axis=(1, 3)) * (2.0 / (nt * sub)) * (2 * np.pi / (nphi * sub)) * n
+    assert abs(exp.sum() / n - 1.0) < 5e-3
+    big = exp > 100
+    z = (obs[big] - exp[big]) / np.sqrt(exp[big])
+    print(f"  mixture sampling: bins {big.sum()} max|z| {np.abs(z).max():.2f} mean z^2 {np.mean(z * z):.3f}",
+          flush=True)
+    assert np.mean(z * z) < 1.5 and np.abs(z).max() < 6.0
+
+
+def test_training_learns_a_directional_target(ctx):
+    """Samples whose directions follow a position-dependent vMF (kappa 30 around a mean that turns
+    with x): Adam on the device lowers the held-out negative log-likelihood far below the uniform
+    density's log(4 pi)."""
+    rng = np.random.default_rng(19)
+
+    def draw(n):
+        s = np.zeros((n, 10), np.float32)
+        s[:, :3] = rng.random((n, 3))
+        s[:, 3:6] = (0, 1, 0)
+        ang = s[:, 0] * np.pi
+        mu = np.stack([np.cos(ang), 0.5 * np.ones(n), np.sin(ang)], -1)
+        mu = _unit(mu)
+        k = 30.0
+        u1, u2 = rng.random(n), rng.random(n)
+        w = 1 + np.log(u1 + (1 - u1) * np.exp(-2 * k)) / k
+        sr = np.sqrt(np.maximum(0, 1 - w * w))
+        t = _unit(np.cross(mu, np.array([0.0, 0.0, 1.0]) + 0 * mu))
+        b = np.cross(mu, t)
+        s[:, 6:9] = w[:, None] * mu + sr[:, None] * (np.cos(2 * np.pi * u2)[:, None] * t +
+                                                      np.sin(2 * np.pi * u2)[:, None] * b)
+        s[:, 9] = 1.0
+        return s
+
+    train = draw(65536)
+    test = draw(8192)
+    p0 = O.guide_init(20)
+    loss0, _ = ctx.guide_grad(p0, test)
+    p1, losses = ctx.guide_train(p0, train, steps=400, lr=3e-3)
+    loss1, _ = ctx.guide_grad(p1, test)
+    print(f"  held-out NLL {loss0:.3f} -> {loss1:.3f} (uniform {np.log(4 * np.pi):.3f}); "
+          f"train loss {losses[0]:.3f} -> {losses[-1]:.3f}", flush=True)
+    assert loss1 < loss0 - 1.0
+    assert loss1 < np.log(4 * np.pi) - 1.0
+
+
+# ---------------------------------------------------------------- guided renders
+def _block_means(img, block=8):
+    H, W = img.shape[:2]
+    return img.reshape(H // block, block, W // block, block, 3).mean(axis=(1, 3))
+
+
+def test_guided_render_matches_plain(ctx):
+    """One-sample MIS keeps the plain estimator's expectation: per-block means of guided and plain
+    renders (4 seeds each, 32x32, 2048 spp) agree within their standard errors."""
+    sd = scenes.glossy_cornell()
+    cam = dict(sd.camera, width=32, height=32)
+    g = ctx.upload(sd)
+    plain, guided = [], []
+    for s in range(4):
+        out, _ = g.render(cam, dict(sd.render, spp=2048, seed=100 + s))
+        plain.append(_block_means(rtg.resolve(out).astype(np.float64)))
+        out, st = g.render(cam, dict(sd.render, spp=2048, seed=200 + s, guiding=1))
+        guided.append(_block_means(rtg.resolve(out).astype(np.float64)))
+        assert st.nan_samples == 0
+        assert st.guide_train_steps > 0
+    P, Gd = np.stack(plain), np.stack(guided)
+    se = np.sqrt(P.var(0, ddof=1) / 4 + Gd.var(0, ddof=1) / 4) + 1e-4 * (np.abs(P.mean(0)) + 1e-3)
+    z = (Gd.mean(0) - P.mean(0)) / se
+    print(f"  guided vs plain: max|z| {np.abs(z).max():.2f} mean z^2 {np.mean(z * z):.2f} "
+          f"means {Gd.mean():.5f} {P.mean():.5f} train steps {st.guide_train_steps}", flush=True)
+    assert np.abs(z).max() < 6.0 and np.mean(z * z) < 3.0
+
+
+def relmse(x, r):
+    return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
+
+
+def test_guiding_gain_on_glossy_indirect(ctx):
+    """What the trained network gains: on the glossy-indirect scene (512x512, light hidden above a
+    baffle, so the floor is lit by a small bright patch of ceiling), the guided image at 1024 spp
+    against a 16384-spp plain reference has a lower relMSE than the plain image at 1024 spp."""
+    sd = scenes.glossy_cornell()
+    cam = dict(sd.camera, width=512, height=512)
+    g = ctx.upload(sd)
+    ref, _ = g.render(cam, dict(sd.render, spp=16384, seed=1))
+    ref = rtg.resolve(ref).astype(np.float64)
+    res = {}
+    for name, gd in (("plain", 0), ("guided", 1)):
+        t0 = time.perf_counter()
+        out, st = g.render(cam, dict(sd.render, spp=1024, seed=7, guiding=gd))
+        dt = time.perf_counter() - t0
+        res[name] = (relmse(rtg.resolve(out).astype(np.float64), ref), dt, st)
+    (ep, tp, _), (eg, tg, sg) = res["plain"], res["guided"]
+    print(f"  1024 spp: plain relMSE {ep:.5f} ({tp * 1e3:.0f} ms), guided {eg:.5f} ({tg * 1e3:.0f} ms, "
+          f"{sg.guide_train_steps} training steps, loss {sg.guide_loss:.3f}); error ratio {eg / ep:.3f}, "
+          f"error x time ratio {eg * tg / (ep * tp):.3f}", flush=True)
+    assert eg < ep

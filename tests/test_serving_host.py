@@ -40,10 +40,11 @@ print("ALL OK", flush=True)
 def build_mock_lib():
     os.makedirs(BUILD, exist_ok=True)
     out = os.path.join(BUILD, "librtg_hostmock.so")
-    srcs = [os.path.join(CSRC, f) for f in ("rtg_serve.cpp", "rtg_host.cpp", "rtg_bvh.cpp", "rtg_stubs.cpp")]
+    srcs = [os.path.join(CSRC, f) for f in ("rtg_serve.cpp", "rtg_host.cpp", "rtg_guide_host.cpp", "rtg_bvh.cpp",
+                                            "rtg_stubs.cpp")]
     srcs.append(os.path.join(MOCK, "mock_kernels.cpp"))
-    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-pthread", "-I" + MOCK, "-I" + CSRC] + srcs + \
-          ["-o", out]
+    cmd = ["g++", "-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-pthread", "-Wl,--no-undefined", "-I" + MOCK,
+           "-I" + CSRC] + srcs + ["-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     assert r.returncode == 0, r.stderr[-3000:]
     return out
