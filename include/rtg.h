@@ -117,6 +117,9 @@ typedef struct rtg_render_desc {
                                   vertices (rtg_render / rtg_render_device only; no reference
                                   counterpart): a vMF-mixture network trained online on this
                                   render's own paths, one-sample MIS with BSDF sampling */
+  int32_t use_camera_matrix;   /* 1: camera_to_world (row-major, the XML camera's toWorld, e.g. a
+                                  lookAt with a mirroring scale) replaces origin/target/up */
+  float camera_to_world[16];
 } rtg_render_desc;
 
 typedef struct rtg_stats {
@@ -167,6 +170,12 @@ int rtg_scene_load_xml(rtg_ctx* ctx, const char* path, rtg_scene** out, rtg_rend
 int rtg_scene_release(rtg_ctx* ctx, rtg_scene* scene);
 /* Device bytes held by a scene. */
 int rtg_scene_bytes(rtg_scene* scene, uint64_t* bytes);
+/* What a resident scene holds (for loaders and tests). */
+typedef struct rtg_scene_info {
+  int32_t triangles, spheres, materials, lights, env_width, env_height;
+  float bbox_min[3], bbox_max[3];
+} rtg_scene_info;
+int rtg_scene_query(rtg_scene* scene, rtg_scene_info* info);
 
 /* Blocking render. out_rgbw (width*height*4 floats, host) receives per-pixel
  * sums of radiance and the filter weight (box filter r=0.5); divide rgb by w
@@ -206,6 +215,8 @@ int rtg_guide_eval(rtg_ctx* ctx, const float* params, const float* in6, int32_t 
 int rtg_guide_grad(rtg_ctx* ctx, const float* params, const float* samples, int32_t n, float* loss, float* grad);
 int rtg_guide_train(rtg_ctx* ctx, const float* params, const float* samples, int32_t n, int32_t steps, float lr,
                     float* params_out, float* losses);
+/* The network of the context's last guided render (RTG_GUIDE_PARAMS floats, layout as above). */
+int rtg_guide_params(rtg_ctx* ctx, float* params_out);
 /* Directions from mixtures (n x 40, as rtg_guide_eval returns) with u (n x 3: lobe choice, two for
  * the lobe) -> dirs (n x 3) and the mixture density at each (n). */
 int rtg_guide_sample(rtg_ctx* ctx, const float* mix, const float* u, int32_t n, float* dirs, float* pdf);

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "librtg.so")
 
-SOURCES = ["rtg_kernels.hip", "rtg_guide.hip", "rtg_bvh.cpp", "rtg_host.cpp", "rtg_guide_host.cpp", "rtg_serve.cpp", "rtg_stubs.cpp"]
+SOURCES = ["rtg_kernels.hip", "rtg_guide.hip", "rtg_bvh.cpp", "rtg_host.cpp", "rtg_guide_host.cpp", "rtg_serve.cpp", "rtg_xml.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -60,8 +60,8 @@ static void guide_alloc_pool(rtg_ctx* ctx, GuideState* g, int capacity) {
   };
   const size_t n = (size_t)capacity;
   g->mix = (float*)A(n * kGuideOut * sizeof(float));
-  g->rec = (float4*)A(n * 3 * sizeof(float4));
-  g->rstate = (uint8_t*)A(n);
+  g->rec = (float4*)A(n * kGuideRec * 4 * sizeof(float4));
+  g->rstate = (uint8_t*)A((n * kGuideRec + 3) & ~(size_t)3);
   g->gq = (uint32_t*)A(n * sizeof(uint32_t));
   g->gq_count = (uint32_t*)A(64);
   g->ring_cap = 1u << 21;
@@ -101,7 +101,13 @@ void guide_init_params(rtg_ctx* ctx, GuideState* g, uint64_t seed) {
                        {kGOffW4, kGuideOut, 64, 0.1}};
   for (const L& l : layers) {
     const double lim = l.scale * std::sqrt(6.0 / (l.nout + l.nin));
-    for (int i = 0; i < l.nout * l.nin; i++) p[l.off + i] = (float)((2.0 * next() - 1.0) * lim);
+    for (int i = 0; i < l.nout * l.nin; i++) {
+      double w = (2.0 * next() - 1.0) * lim;
+      // first layer: features of frequency 2^f start with weight scale 0.7^f, so the initial
+      // network is a smooth function of position and normal
+      if (l.off == kGOffW1) w *= std::pow(0.7, (double)(((i % kGuideIn) % 24) / 2));
+      p[l.off + i] = (float)w;
+    }
   }
   guide_set_params(ctx, g, p.data());
 }
@@ -144,7 +150,7 @@ DevGuide guide_dev(GuideState* g) {
 void guide_prepare_render(rtg_ctx* ctx, GuideState* g, rtg_scene* scene, uint64_t seed, DevGuide& G) {
   guide_alloc_pool(ctx, g, ctx->pool.capacity);
   guide_init_params(ctx, g, seed);
-  launch_fill_zero(g->rstate, (size_t)g->capacity, ctx->stream);
+  launch_fill_zero(g->rstate, ((size_t)g->capacity * kGuideRec + 3) & ~(size_t)3, ctx->stream);
   launch_fill_zero(g->ring_count, 16, ctx->stream);
   G = guide_dev(g);
   for (int c = 0; c < 3; c++) {
@@ -248,6 +254,22 @@ int rtg_guide_train(rtg_ctx* ctx, const float* params, const float* samples, int
     hip_check(hipGetLastError(), "guide train");
     hip_check(hipStreamSynchronize(ctx->stream), "guide train");
     hip_check(hipMemcpy(params_out, g->master, sizeof(float) * kGuideParams, hipMemcpyDeviceToHost), "params");
+    return RTG_OK;
+  } catch (const RtgError& e) {
+    set_error(e.what());
+    return e.code;
+  }
+}
+
+int rtg_guide_params(rtg_ctx* ctx, float* params_out) {
+  if (!ctx || !params_out) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  try {
+    if (!ctx->guide || !ctx->guide->master) fail(RTG_ERR_INVALID, "no guided render has run on this context");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    hip_check(hipStreamSynchronize(ctx->stream), "guide params");
+    hip_check(hipMemcpy(params_out, ctx->guide->master, sizeof(float) * kGuideParams, hipMemcpyDeviceToHost),
+              "guide params");
     return RTG_OK;
   } catch (const RtgError& e) {
     set_error(e.what());

@@ -115,24 +115,28 @@ void camera_matrices(const rtg_render_desc& d, float* s2c, float* c2w) {
     o[1] = a[2] * b[0] - a[0] * b[2];
     o[2] = a[0] * b[1] - a[1] * b[0];
   };
-  double dir[3] = {(double)d.target[0] - d.origin[0], (double)d.target[1] - d.origin[1],
-                   (double)d.target[2] - d.origin[2]};
-  nrm(dir);
-  double up[3] = {d.up[0], d.up[1], d.up[2]};
-  nrm(up);
-  double left[3], nu[3];
-  crs(up, dir, left);
-  nrm(left);
-  crs(dir, left, nu);
-  nrm(nu);
-  std::memset(c2w, 0, 16 * sizeof(float));
-  for (int i = 0; i < 3; i++) {
-    c2w[4 * i + 0] = (float)left[i];
-    c2w[4 * i + 1] = (float)nu[i];
-    c2w[4 * i + 2] = (float)dir[i];
-    c2w[4 * i + 3] = d.origin[i];
+  if (d.use_camera_matrix) {
+    for (int i = 0; i < 16; i++) c2w[i] = d.camera_to_world[i];
+  } else {
+    double dir[3] = {(double)d.target[0] - d.origin[0], (double)d.target[1] - d.origin[1],
+                     (double)d.target[2] - d.origin[2]};
+    nrm(dir);
+    double up[3] = {d.up[0], d.up[1], d.up[2]};
+    nrm(up);
+    double left[3], nu[3];
+    crs(up, dir, left);
+    nrm(left);
+    crs(dir, left, nu);
+    nrm(nu);
+    std::memset(c2w, 0, 16 * sizeof(float));
+    for (int i = 0; i < 3; i++) {
+      c2w[4 * i + 0] = (float)left[i];
+      c2w[4 * i + 1] = (float)nu[i];
+      c2w[4 * i + 2] = (float)dir[i];
+      c2w[4 * i + 3] = d.origin[i];
+    }
+    c2w[15] = 1.f;
   }
-  c2w[15] = 1.f;
   double aspect = d.width / (double)d.height;
   double recip = 1.0 / ((double)d.far_clip - d.near_clip);
   double cot = 1.0 / std::tan((d.fov * 3.14159265358979323846 / 180.0) / 2.0);
@@ -693,6 +697,22 @@ int rtg_scene_release(rtg_ctx* ctx, rtg_scene* scene) {
   RTG_GUARD(ctx, { release_scene(ctx, scene); })
 }
 
+int rtg_scene_query(rtg_scene* scene, rtg_scene_info* info) {
+  if (!scene || !info) return RTG_ERR_INVALID;
+  std::memset(info, 0, sizeof(*info));
+  info->triangles = scene->ntri;
+  info->spheres = scene->nsph;
+  info->materials = scene->dev.nmat;
+  info->lights = scene->dev.nlights;
+  info->env_width = scene->dev.env_w;
+  info->env_height = scene->dev.env_h;
+  for (int k = 0; k < 3; k++) {
+    info->bbox_min[k] = scene->dev.root_min[k];
+    info->bbox_max[k] = scene->dev.root_max[k];
+  }
+  return RTG_OK;
+}
+
 int rtg_scene_bytes(rtg_scene* scene, uint64_t* bytes) {
   if (!scene || !bytes) return RTG_ERR_INVALID;
   *bytes = scene->bytes;
@@ -724,11 +744,13 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   GuideState* gs = nullptr;
   DevGuide G;
   std::memset(&G, 0, sizeof(G));
-  // Guided renders sample plainly (recording training vertices) until the network has had
-  // kWarmSteps training steps, then switch to one-sample MIS with the guide; training (and the
-  // recording that feeds it) stops after kMaxSteps steps.
-  const uint64_t kWarmSteps = 32, kMaxSteps = 512;
-  const int kMaxStepsPerBatch = 8;
+  // Guided renders sample plainly (recording training vertices) for the first kWarmSteps training
+  // steps, which run kWarmPerBatch at a time on the newest samples (a sample may serve several
+  // steps); then one-sample MIS with the guide, with kPerBatch steps per iteration batch until
+  // kMaxSteps, after which training and the recording that feeds it stop.
+  const uint64_t kWarmSteps = 256, kMaxSteps = 2048;
+  const int kWarmPerBatch = 32, kPerBatch = 4;
+  const int kMaxStepsPerBatch = kWarmPerBatch;
   if (guided) {
     gs = guide_state(ctx);
     guide_prepare_render(ctx, gs, scene, desc->seed, G);
@@ -797,12 +819,14 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
       const uint64_t B = kGuideBatch;
       int steps = 0;
       if (G.record && ring_total >= B && ring_total > trained_upto) {
-        uint64_t fresh = ring_total - trained_upto;
-        steps = (int)std::min<uint64_t>(kMaxStepsPerBatch, std::max<uint64_t>(1, fresh / B));
+        steps = train_steps < kWarmSteps ? kWarmPerBatch : kPerBatch;
         steps = (int)std::min<uint64_t>((uint64_t)steps, kMaxSteps - train_steps);
       }
+      // step t trains on the newest samples, shifted back by t half-batches (within the ring)
+      const uint64_t span = std::min<uint64_t>(ring_total, (uint64_t)G.ring_mask + 1);
       for (int t = 0; t < steps; t++) {
-        const uint64_t base = ring_total - (uint64_t)(t + 1) * B;
+        const uint64_t back = ((uint64_t)t * (B / 2)) % (span - B + 1);
+        const uint64_t base = ring_total - B - back;
         if (prof) (void)hipEventRecord(ctx->events[(size_t)batch * kEv + 2 * t], ctx->stream);
         guide_train_step(ctx, gs, G, gs->ring, base, G.ring_mask, (int)B, 1e-3f);
         if (prof) (void)hipEventRecord(ctx->events[(size_t)batch * kEv + 2 * t + 1], ctx->stream);

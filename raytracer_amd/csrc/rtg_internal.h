@@ -145,6 +145,10 @@ static const int kGBias1 = 0, kGBias2 = 64, kGBias3 = 128, kGBias4 = 192, kGBias
 static const int kGuideBatch = 65536;        // training samples per step
 static const float kGuideLogKappaMin = -3.f, kGuideLogKappaMax = 8.f;
 static const uint32_t kGuideSelDim = 1u << 20; // 1D sampler dimension of the strategy choice (+ depth)
+// Guided vertices: diffuse ones up to depth kGuideRec-1 (their product f*cos*L_in does not depend
+// on the incoming direction, so a network of position and normal can represent it), rough
+// conductors at depth 0 only (there the incoming direction is fixed by the pixel).
+static const int kGuideRec = 3;
 
 struct DevGuide {
   int32_t enabled;
@@ -156,8 +160,9 @@ struct DevGuide {
   const uint16_t* wb;        // bf16 backward weights
   const float* bias;         // FP32 biases
   float* mix;                // per pool slot: 8 lobes x (pi, mu xyz, kappa)
-  float4* rec;               // per pool slot: 3 float4 = pos(3) n(3) dir(3) L snapshot(3)
-  uint8_t* rstate;           // per pool slot: 0 none, 1 vertex recorded, 2 radiance snapshot taken
+  float4* rec;               // per pool slot, kGuideRec records of 4 float4 (guided vertex at depth r):
+                             //   pos(3) n(3) dir(3) L snapshot(3) throughput before the vertex(3) weight
+  uint8_t* rstate;           // per slot and record: 0 none, 1 vertex recorded, 2 radiance snapshot taken
   uint32_t* gq;              // guide queue: pool slots of guided depth-0 vertices
   uint32_t* gq_count;
   float* ring;               // training samples: pos(3) n(3) dir(3) weight

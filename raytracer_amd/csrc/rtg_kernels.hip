@@ -104,9 +104,13 @@ __device__ __forceinline__ uint32_t block_reserve(uint32_t* counter, uint32_t to
 constexpr int kRegenK = 8;
 constexpr int kRegenSlots = 256 * kRegenK;
 
-// GUIDED: finished paths whose depth-0 vertex was guided and whose radiance snapshot was taken
-// append one training sample (position, normal, direction, luminance of the radiance gathered
-// after that vertex) to the guide's ring; one 64-bit atomic per chunk.
+__device__ __forceinline__ bool guided_vertex(int mt, uint32_t depth) {
+  return depth < (uint32_t)kGuideRec && (mt == 0 || (mt == 1 && depth == 0));
+}
+
+// GUIDED: every guided vertex of a finished path whose radiance snapshot was taken appends one
+// training sample (position, normal, direction, luminance of the radiance gathered after the
+// vertex divided by the throughput before it) to the guide's ring; one 64-bit atomic per chunk.
 template <bool GUIDED>
 __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, const int32_t* order, int norder,
                                                DevGuide G) {
@@ -153,17 +157,26 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, const in
           splat(J, pix, sx, sy, l);
         }
         if (GUIDED) {
-          const uint8_t rs = G.rstate[i];
-          if (rs == 2 && !bad) {
-            float4 r2 = G.rec[3 * (size_t)i + 2];
-            float w = luminance(mk(l.x - r2.y, l.y - r2.z, l.z - r2.w));
-            if (w > 0.f && w < 3.0e38f) {
-              r2.y = w;
-              G.rec[3 * (size_t)i + 2] = r2;
-              trn |= 1u << k;
+#pragma unroll
+          for (int r = 0; r < kGuideRec; r++) {
+            const uint8_t rs = G.rstate[(size_t)i * kGuideRec + r];
+            if (rs == 2 && !bad) {
+              // f*cos*L_in / p of the vertex = radiance gathered after it / throughput before it
+              float4* q = G.rec + ((size_t)i * kGuideRec + r) * 4;
+              const float4 q2 = q[2];
+              float4 q3 = q[3];
+              const f3 d = mk(l.x - q2.y, l.y - q2.z, l.z - q2.w);
+              const f3 t = mk(q3.x > 0.f ? d.x / q3.x : 0.f, q3.y > 0.f ? d.y / q3.y : 0.f,
+                              q3.z > 0.f ? d.z / q3.z : 0.f);
+              const float w = luminance(t);
+              if (w > 0.f && w < 3.0e38f) {
+                q3.w = w;
+                q[3] = q3;
+                trn |= 1u << (k * kGuideRec + r);
+              }
             }
+            if (rs) G.rstate[(size_t)i * kGuideRec + r] = 0;
           }
-          if (rs) G.rstate[i] = 0;
         }
         atomicAdd(&s_done[jb], 1u);
         P.state[i] = ST_FREE;
@@ -180,13 +193,14 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, const in
       unsigned long long rb = s_base64 + toff;
       __syncthreads();
 #pragma unroll 1
-      for (int k = 0; k < kRegenK; k++) {
-        if (!(trn & (1u << k))) continue;
-        const size_t i = (size_t)(base + k * 256);
-        const float4 r0 = G.rec[3 * i], r1 = G.rec[3 * i + 1], r2 = G.rec[3 * i + 2];
+      for (int b = 0; b < kRegenK * kGuideRec; b++) {
+        if (!(trn & (1u << b))) continue;
+        const int k = b / kGuideRec, r = b - k * kGuideRec;
+        const float4* q = G.rec + ((size_t)(base + k * 256) * kGuideRec + r) * 4;
+        const float4 r0 = q[0], r1 = q[1], r2 = q[2], r3 = q[3];
         float* d = G.ring + 10 * (size_t)(rb & G.ring_mask);
         d[0] = r0.x; d[1] = r0.y; d[2] = r0.z; d[3] = r0.w; d[4] = r1.x;
-        d[5] = r1.y; d[6] = r1.z; d[7] = r1.w; d[8] = r2.x; d[9] = r2.y;
+        d[5] = r1.y; d[6] = r1.z; d[7] = r1.w; d[8] = r2.x; d[9] = r3.w;
         rb++;
       }
     }
@@ -323,8 +337,8 @@ __global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scene
 // ---------------------------------------------------------------- classify: shade queues
 // Chunks of 2048 extension-queue entries per block; the four per-class counts are packed into
 // 16-bit fields of one 64-bit word so a single block scan serves all four queues.
-// GUIDED: also queues the guided depth-0 vertices (diffuse and rough-conductor hits of the guided
-// job) for the guide network's inference.
+// GUIDED: also queues the guided vertices of the guided job (guided_vertex) for the guide
+// network's inference.
 template <bool GUIDED>
 __global__ void __launch_bounds__(256) k_classify(DevPool P, DevGuide G) {
   __shared__ unsigned long long s_w[4];
@@ -370,7 +384,7 @@ __global__ void __launch_bounds__(256) k_classify(DevPool P, DevGuide G) {
       for (int k = 0; k < kRegenK; k++) {
         if (G.sample && cls[k] < 2) {
           const uint32_t f = P.flags[P.q_ext[base + k * 256]];
-          if (fl_depth(f) == 0 && (int)fl_job(f) == G.job) gmask |= 1u << k;
+          if (guided_vertex((int)cls[k], fl_depth(f)) && (int)fl_job(f) == G.job) gmask |= 1u << k;
         }
       }
       uint32_t gtot;
@@ -407,12 +421,16 @@ __device__ __forceinline__ void prev_segment(const DevScene& S, f3 Le, float pdf
 // Radiance snapshot for guide training, taken at the first event after a guided depth-0 vertex
 // (before that event adds anything): what the path gathers afterwards is the radiance arriving
 // along the guided direction, weighted by the vertex's sampling weight.
-__device__ __forceinline__ void guide_snapshot(const DevGuide& G, uint32_t slot, const float4& L) {
-  if (G.rstate[slot] == 1) {
-    float4 r2 = G.rec[3 * (size_t)slot + 2];
-    r2.y = L.x; r2.z = L.y; r2.w = L.z;
-    G.rec[3 * (size_t)slot + 2] = r2;
-    G.rstate[slot] = 2;
+// (depth: the event's depth; the vertex recorded at depth - 1 is the one awaiting it.)
+__device__ __forceinline__ void guide_snapshot(const DevGuide& G, uint32_t slot, uint32_t depth, const float4& L) {
+  if (depth < 1 || depth > (uint32_t)kGuideRec) return;
+  const size_t r = (size_t)slot * kGuideRec + (depth - 1);
+  if (G.rstate[r] == 1) {
+    float4* q = G.rec + r * 4;
+    float4 q2 = q[2];
+    q2.y = L.x; q2.z = L.y; q2.w = L.z;
+    q[2] = q2;
+    G.rstate[r] = 2;
   }
 }
 
@@ -436,7 +454,7 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
       HitRec h;
       h.t = hv.x; h.b1 = hv.y; h.b2 = hv.z; h.prim = __float_as_int(hv.w);
       Surface sf = surface_at(S, h, o, d);
-      if (GUIDED && depth == 1) guide_snapshot(G, slot, L);
+      if (GUIDED) guide_snapshot(G, slot, depth, L);
       if (depth > 0) {
         bool caught = sf.light != 0xffffffffu;
         f3 Le = mk(0.f, 0.f, 0.f);
@@ -493,7 +511,7 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
         sample2d(rng, 2u + 2u * depth, bx, by);
         BsdfSample bs;
         bool ok;
-        if (GUIDED && MT != 2 && depth == 0 && (int)fl_job(f) == G.job) {
+        if (GUIDED && guided_vertex(MT, depth) && (int)fl_job(f) == G.job) {
           // one-sample MIS of BSDF sampling and the guide, each chosen with probability 1/2:
           // weight = (sampler weight * sampler density) / mixture density, so the estimator keeps
           // the plain tracer's expectation; the MIS weights of the next hit use the BSDF's pdf.
@@ -527,10 +545,12 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
           if (G.record) {
             const f3 np = mk((sf.p.x - G.bmin[0]) * G.inv_ext[0], (sf.p.y - G.bmin[1]) * G.inv_ext[1],
                              (sf.p.z - G.bmin[2]) * G.inv_ext[2]);
-            G.rec[3 * (size_t)slot] = make_float4(np.x, np.y, np.z, sf.sh.n.x);
-            G.rec[3 * (size_t)slot + 1] = make_float4(sf.sh.n.y, sf.sh.n.z, wd.x, wd.y);
-            G.rec[3 * (size_t)slot + 2] = make_float4(wd.z, 0.f, 0.f, 0.f);
-            G.rstate[slot] = ok ? 1 : 0;
+            float4* q = G.rec + ((size_t)slot * kGuideRec + depth) * 4;
+            q[0] = make_float4(np.x, np.y, np.z, sf.sh.n.x);
+            q[1] = make_float4(sf.sh.n.y, sf.sh.n.z, wd.x, wd.y);
+            q[2] = make_float4(wd.z, 0.f, 0.f, 0.f);
+            q[3] = make_float4(T.x, T.y, T.z, 0.f);
+            G.rstate[(size_t)slot * kGuideRec + depth] = ok ? 1 : 0;
           }
         } else {
           ok = bsdf_sample<MT>(M, wo_l, bx, by, bs);
@@ -584,7 +604,7 @@ __global__ void __launch_bounds__(256) k_shade_miss(DevPool P, const DevScene* s
     float4 T = P.T[slot], F = P.F[slot], L = P.L[slot];
     f3 o = mk(ro.x, ro.y, ro.z), d = mk(rd.x, rd.y, rd.z);
     int prim = __float_as_int(hv.w);
-    if (GUIDED && depth == 1) guide_snapshot(G, slot, L);
+    if (GUIDED) guide_snapshot(G, slot, depth, L);
     if (prim >= 0) {
       // hit without a BSDF: the emission of the previous segment still counts, then the path ends
       if (depth > 0) {

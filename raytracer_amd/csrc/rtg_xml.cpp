@@ -1,0 +1,916 @@
+// Scene files in the reference's XML schema (core/XMLLoader.cpp:33-904), for the types the GPU
+// path renders: PathTracingMIS, RandomSampler / StratifiedSampler, NoriPerspective with the box
+// filter, Mesh (OBJ, io/ObjLoader.cpp:97-289) / Sphere / Cube shapes, Diffuse / RoughConductor /
+// RoughDielectric BSDFs with constant parameters, area lights on meshes, and an environment light
+// from a Radiance RGBE (.hdr) image. Anything else fails with RTG_ERR_PARSE and names the element.
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <memory>
+#include <sstream>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "rtg_host.h"
+
+using namespace rtg;
+
+namespace {
+
+[[noreturn]] void perr(const std::string& m) { fail(RTG_ERR_PARSE, m); }
+
+// object types are looked up case-insensitively (factory/ObjectFactoryManager.h:82-89)
+std::string lower(const std::string& s) {
+  std::string r = s;
+  for (char& c : r) c = (char)std::tolower((unsigned char)c);
+  return r;
+}
+
+// ---------------------------------------------------------------- minimal XML
+struct XNode {
+  std::string name;
+  std::map<std::string, std::string> attr;
+  std::string text;
+  std::vector<XNode> kids;
+  const std::string* get(const char* k) const {
+    auto it = attr.find(k);
+    return it == attr.end() ? nullptr : &it->second;
+  }
+};
+
+class XmlParser {
+ public:
+  explicit XmlParser(const std::string& s) : s_(s) {}
+  XNode parse() {
+    if (s_.compare(0, 3, "\xEF\xBB\xBF") == 0) p_ = 3;  // UTF-8 byte-order mark
+    skip_misc();
+    if (p_ >= s_.size() || s_[p_] != '<') perr("xml: no root element");
+    XNode root = element();
+    return root;
+  }
+
+ private:
+  const std::string& s_;
+  size_t p_ = 0;
+  void ws() {
+    while (p_ < s_.size() && std::isspace((unsigned char)s_[p_])) p_++;
+  }
+  bool starts(const char* t) const { return s_.compare(p_, std::strlen(t), t) == 0; }
+  void skip_until(const char* t) {
+    size_t q = s_.find(t, p_);
+    if (q == std::string::npos) perr(std::string("xml: unterminated construct, missing ") + t);
+    p_ = q + std::strlen(t);
+  }
+  // declarations, comments, processing instructions and whitespace between elements
+  void skip_misc() {
+    while (true) {
+      ws();
+      if (starts("<?")) skip_until("?>");
+      else if (starts("<!--")) skip_until("-->");
+      else if (starts("<!")) skip_until(">");
+      else return;
+    }
+  }
+  static std::string unescape(const std::string& v) {
+    std::string o;
+    for (size_t i = 0; i < v.size(); i++) {
+      if (v[i] == '&') {
+        static const char* ents[][2] = {{"&lt;", "<"}, {"&gt;", ">"}, {"&amp;", "&"}, {"&quot;", "\""}, {"&apos;", "'"}};
+        bool done = false;
+        for (auto& e : ents)
+          if (v.compare(i, std::strlen(e[0]), e[0]) == 0) {
+            o += e[1];
+            i += std::strlen(e[0]) - 1;
+            done = true;
+            break;
+          }
+        if (!done) o += v[i];
+      } else {
+        o += v[i];
+      }
+    }
+    return o;
+  }
+  std::string name() {
+    size_t b = p_;
+    while (p_ < s_.size() && (std::isalnum((unsigned char)s_[p_]) || s_[p_] == '_' || s_[p_] == '-' ||
+                              s_[p_] == ':' || s_[p_] == '.'))
+      p_++;
+    if (b == p_) perr("xml: expected a name at offset " + std::to_string(p_));
+    return s_.substr(b, p_ - b);
+  }
+  XNode element() {
+    XNode n;
+    p_++;  // '<'
+    n.name = name();
+    while (true) {
+      ws();
+      if (p_ >= s_.size()) perr("xml: unexpected end in <" + n.name + ">");
+      if (starts("/>")) {
+        p_ += 2;
+        return n;
+      }
+      if (s_[p_] == '>') {
+        p_++;
+        break;
+      }
+      std::string k = name();
+      ws();
+      if (p_ >= s_.size() || s_[p_] != '=') perr("xml: expected '=' after attribute " + k);
+      p_++;
+      ws();
+      char q = p_ < s_.size() ? s_[p_] : 0;
+      if (q != '"' && q != '\'') perr("xml: unquoted attribute " + k);
+      size_t e = s_.find(q, p_ + 1);
+      if (e == std::string::npos) perr("xml: unterminated attribute " + k);
+      n.attr[k] = unescape(s_.substr(p_ + 1, e - p_ - 1));
+      p_ = e + 1;
+    }
+    // content
+    while (true) {
+      if (p_ >= s_.size()) perr("xml: missing </" + n.name + ">");
+      if (starts("<!--")) {
+        skip_until("-->");
+      } else if (starts("<![CDATA[")) {
+        size_t e = s_.find("]]>", p_);
+        if (e == std::string::npos) perr("xml: unterminated CDATA");
+        n.text += s_.substr(p_ + 9, e - p_ - 9);
+        p_ = e + 3;
+      } else if (starts("</")) {
+        p_ += 2;
+        std::string c = name();
+        if (c != n.name) perr("xml: </" + c + "> closes <" + n.name + ">");
+        ws();
+        if (p_ >= s_.size() || s_[p_] != '>') perr("xml: bad closing tag " + c);
+        p_++;
+        return n;
+      } else if (starts("<?")) {
+        skip_until("?>");
+      } else if (s_[p_] == '<') {
+        n.kids.push_back(element());
+      } else {
+        size_t e = s_.find('<', p_);
+        if (e == std::string::npos) e = s_.size();
+        n.text += unescape(s_.substr(p_, e - p_));
+        p_ = e;
+      }
+    }
+  }
+};
+
+std::string type_of(const XNode& n) {
+  const std::string* t = n.get("type");
+  return t ? lower(*t) : std::string();
+}
+
+// ---------------------------------------------------------------- values
+std::vector<double> numbers(const std::string& s) {
+  std::vector<double> v;
+  std::string t;
+  for (char c : s) t += (c == ',') ? ' ' : c;
+  std::istringstream is(t);
+  std::string tok;
+  while (is >> tok) {
+    char* end = nullptr;
+    double x = std::strtod(tok.c_str(), &end);
+    if (!end || *end) perr("bad number '" + tok + "'");
+    v.push_back(x);
+  }
+  return v;
+}
+double number(const std::string& s) {
+  auto v = numbers(s);
+  if (v.size() != 1) perr("expected one number, got '" + s + "'");
+  return v[0];
+}
+
+// row-major 4x4 in double
+struct M4 {
+  double m[16];
+  static M4 id() {
+    M4 r;
+    for (int i = 0; i < 16; i++) r.m[i] = (i % 5 == 0) ? 1.0 : 0.0;
+    return r;
+  }
+};
+M4 mul(const M4& a, const M4& b) {
+  M4 r;
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+      double s = 0;
+      for (int k = 0; k < 4; k++) s += a.m[4 * i + k] * b.m[4 * k + j];
+      r.m[4 * i + j] = s;
+    }
+  return r;
+}
+bool invert(const M4& a, M4& out) {
+  double m[4][8];
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 8; j++) m[i][j] = j < 4 ? a.m[4 * i + j] : (j - 4 == i ? 1.0 : 0.0);
+  for (int c = 0; c < 4; c++) {
+    int p = c;
+    for (int r = c + 1; r < 4; r++)
+      if (std::fabs(m[r][c]) > std::fabs(m[p][c])) p = r;
+    if (std::fabs(m[p][c]) < 1e-300) return false;
+    for (int j = 0; j < 8; j++) std::swap(m[c][j], m[p][j]);
+    double d = m[c][c];
+    for (int j = 0; j < 8; j++) m[c][j] /= d;
+    for (int r = 0; r < 4; r++)
+      if (r != c) {
+        double f = m[r][c];
+        for (int j = 0; j < 8; j++) m[r][j] -= f * m[c][j];
+      }
+  }
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) out.m[4 * i + j] = m[i][j + 4];
+  return true;
+}
+
+// xyz children (<x>, <y>, <z>) with a default value
+void xyz(const XNode& n, double def, double v[3]) {
+  v[0] = v[1] = v[2] = def;
+  for (const XNode& c : n.kids) {
+    if (c.name == "x") v[0] = number(c.text);
+    else if (c.name == "y") v[1] = number(c.text);
+    else if (c.name == "z") v[2] = number(c.text);
+    else perr("unknown transform component <" + c.name + ">");
+  }
+}
+void vec3(const std::string& s, double v[3]) {
+  auto a = numbers(s);
+  if (a.size() != 3) perr("expected three numbers, got '" + s + "'");
+  v[0] = a[0]; v[1] = a[1]; v[2] = a[2];
+}
+
+// core/Transform.cpp:116-227 (angles in degrees; lookAt columns left, up, dir, origin)
+M4 rot_axis(int axis, double deg) {
+  const double a = deg * 3.14159265358979323846 / 180.0, c = std::cos(a), s = std::sin(a);
+  M4 r = M4::id();
+  if (axis == 0) { r.m[5] = c; r.m[6] = -s; r.m[9] = s; r.m[10] = c; }
+  if (axis == 1) { r.m[0] = c; r.m[2] = s; r.m[8] = -s; r.m[10] = c; }
+  if (axis == 2) { r.m[0] = c; r.m[1] = -s; r.m[4] = s; r.m[5] = c; }
+  return r;
+}
+void normalize3(double v[3]) {
+  double l = std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  if (l > 0) { v[0] /= l; v[1] /= l; v[2] /= l; }
+}
+void cross3(const double a[3], const double b[3], double o[3]) {
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+M4 look_at(const double o[3], const double t[3], const double up_in[3]) {
+  double dir[3] = {t[0] - o[0], t[1] - o[1], t[2] - o[2]}, up[3] = {up_in[0], up_in[1], up_in[2]}, left[3], nu[3];
+  normalize3(dir);
+  normalize3(up);
+  cross3(up, dir, left);
+  normalize3(left);
+  cross3(dir, left, nu);
+  normalize3(nu);
+  M4 r = M4::id();
+  for (int i = 0; i < 3; i++) {
+    r.m[4 * i + 0] = left[i];
+    r.m[4 * i + 1] = nu[i];
+    r.m[4 * i + 2] = dir[i];
+    r.m[4 * i + 3] = o[i];
+  }
+  return r;
+}
+
+// XMLLoader::handleTransform (core/XMLLoader.cpp:843-904): each child multiplies from the left
+M4 parse_transform(const XNode& n) {
+  M4 T = M4::id();
+  for (const XNode& c : n.kids) {
+    M4 A = M4::id();
+    if (c.name == "rotation") {
+      double r[3];
+      xyz(c, 0.0, r);
+      A = mul(mul(rot_axis(0, r[0]), rot_axis(1, r[1])), rot_axis(2, r[2]));
+    } else if (c.name == "translation") {
+      double t[3];
+      xyz(c, 0.0, t);
+      A.m[3] = t[0]; A.m[7] = t[1]; A.m[11] = t[2];
+    } else if (c.name == "scale") {
+      double s[3];
+      xyz(c, 1.0, s);
+      A.m[0] = s[0]; A.m[5] = s[1]; A.m[10] = s[2];
+    } else if (c.name == "matrix") {
+      const std::string* v = c.get("value");
+      if (!v) perr("<matrix> without value");
+      auto a = numbers(*v);
+      if (a.size() != 16) perr("<matrix> needs 16 values");
+      for (int i = 0; i < 16; i++) A.m[i] = a[i];
+    } else if (c.name == "lookAt") {
+      double o[3] = {0, 0, 0}, t[3] = {0, 0, 1}, u[3] = {0, 1, 0};
+      for (const XNode& k : c.kids) {
+        if (k.name == "origin") vec3(k.text, o);
+        else if (k.name == "lookAt") vec3(k.text, t);
+        else if (k.name == "up") vec3(k.text, u);
+        else perr("unknown lookAt component <" + k.name + ">");
+      }
+      A = look_at(o, t, u);
+    } else {
+      perr("unknown transform element <" + c.name + ">");
+    }
+    T = mul(A, T);
+  }
+  return T;
+}
+
+// Property list of an object (XMLLoader::handleProperty, core/XMLLoader.cpp:687-813)
+struct Props {
+  std::map<std::string, std::string> val;   // name -> text value
+  std::map<std::string, std::string> kind;  // name -> element name (integer, color, ...)
+  std::map<std::string, M4> xf;
+  std::string owner;
+  bool has(const std::string& k) const { return val.count(k) || xf.count(k); }
+  std::string str(const std::string& k, const std::string& d) const {
+    auto it = val.find(k);
+    return it == val.end() ? d : it->second;
+  }
+  double real(const std::string& k, double d) const {
+    auto it = val.find(k);
+    return it == val.end() ? d : number(it->second);
+  }
+  int integer(const std::string& k, int d) const { return (int)std::lround(real(k, d)); }
+  void color(const std::string& k, const float d[3], float out[3]) const {
+    auto it = val.find(k);
+    if (it == val.end()) {
+      out[0] = d[0]; out[1] = d[1]; out[2] = d[2];
+      return;
+    }
+    if (kind.at(k) == "spectrum") perr(owner + ": spectral '" + k + "' values are not supported (give an RGB color)");
+    auto v = numbers(it->second);
+    if (v.size() == 1) v = {v[0], v[0], v[0]};
+    if (v.size() != 3) perr(owner + ": color '" + k + "' needs 3 values");
+    out[0] = (float)v[0]; out[1] = (float)v[1]; out[2] = (float)v[2];
+  }
+  M4 transform(const std::string& k) const {
+    auto it = xf.find(k);
+    return it == xf.end() ? M4::id() : it->second;
+  }
+};
+
+struct Loader {
+  std::string dir;  // directory of the scene file (relative paths)
+  std::map<std::string, XNode> ref_bsdf;
+  std::map<std::string, M4> ref_xf;
+  // output
+  std::vector<rtg_material> mats;
+  struct Mesh {
+    std::vector<float> v, n, uv;
+    std::vector<int32_t> idx;
+    int material = 0;
+    bool emit = false;
+    float emission[3] = {0, 0, 0};
+  };
+  std::vector<Mesh> meshes;
+  std::vector<rtg_sphere_desc> spheres;
+  std::vector<float> env;
+  int env_w = 0, env_h = 0;
+  float env_scale = 1.f;
+  rtg_render_desc rd{};
+  int strategy = RTG_LIGHTS_UNIFORM;
+  bool have_camera = false, have_integrator = false;
+
+  std::string resolve(const std::string& p) const {
+    if (p.empty() || p[0] == '/') return p;
+    std::string q = p;
+    if (q.compare(0, 2, "./") == 0) q = q.substr(2);
+    std::string a = dir + "/" + q;
+    std::ifstream f(a);
+    if (f.good()) return a;
+    // the reference also searches the program's working directory
+    return p;
+  }
+
+  Props props(const XNode& n, const std::string& owner) {
+    Props P;
+    P.owner = owner;
+    for (const XNode& c : n.kids) {
+      if (c.name == "bsdf" || c.name == "light" || c.name == "filter" || c.name == "medium" || c.name == "bssrdf")
+        continue;  // handled by the caller
+      const std::string* nm = c.get("name");
+      if (c.name == "texture") {
+        // XMLLoader::handleTexture (core/XMLLoader.cpp:556-571): only ConstantTexture, whose
+        // value is its "albedo" color (texture/ConstantTexture.cpp:6-10), kept as a color
+        const std::string* ty = c.get("type");
+        if (!nm) perr(owner + ": <texture> without a name");
+        if (!ty || lower(*ty) != "constanttexture")
+          perr(owner + ": texture '" + *nm + "' of type " + (ty ? *ty : std::string("?")) +
+               " is not supported (ConstantTexture only)");
+        Props T = props(c, owner + " texture " + *nm);
+        static const float half[3] = {0.5f, 0.5f, 0.5f};
+        float col[3];
+        T.color("albedo", half, col);
+        char buf[128];
+        std::snprintf(buf, sizeof(buf), "%.9g %.9g %.9g", col[0], col[1], col[2]);
+        P.val[*nm] = buf;
+        P.kind[*nm] = "color";
+        continue;
+      }
+      if (c.name == "transform") {
+        if (const std::string* r = c.get("ref")) {
+          auto it = ref_xf.find(*r);
+          if (it == ref_xf.end()) perr(owner + ": unknown transform ref " + *r);
+          P.xf[nm ? *nm : "toWorld"] = it->second;
+        } else {
+          P.xf[nm ? *nm : "toWorld"] = parse_transform(c);
+        }
+        continue;
+      }
+      if (!nm) perr(owner + ": property <" + c.name + "> without a name");
+      const std::string* v = c.get("value");
+      if (!v) perr(owner + ": property " + *nm + " without a value");
+      static const char* kinds[] = {"bool", "integer", "double", "real", "string", "point", "color", "spectrum",
+                                    "vector"};
+      bool ok = false;
+      for (const char* k : kinds) ok = ok || c.name == k;
+      if (!ok) perr(owner + ": unsupported property <" + c.name + "> (" + *nm + ")");
+      P.val[*nm] = *v;
+      P.kind[*nm] = c.name;
+    }
+    return P;
+  }
+
+  int bsdf(const XNode& n) {
+    const XNode* b = &n;
+    if (const std::string* r = n.get("ref")) {
+      auto it = ref_bsdf.find(*r);
+      if (it == ref_bsdf.end()) perr("unknown bsdf ref " + *r);
+      b = &it->second;
+    }
+    const std::string* type = b->get("type");
+    if (!type) perr("<bsdf> without a type");
+    const std::string lt = lower(*type);
+    Props P = props(*b, "bsdf " + *type);
+    rtg_material m;
+    std::memset(&m, 0, sizeof(m));
+    static const float one[3] = {1, 1, 1}, half[3] = {0.5f, 0.5f, 0.5f}, zero[3] = {0, 0, 0};
+    auto dist = [&](const std::string& d) {
+      if (d == "beckmann") return (int)RTG_DIST_BECKMANN;
+      if (d == "ggx") return (int)RTG_DIST_GGX;
+      perr("bsdf " + *type + ": distribution '" + d + "' is not supported (beckmann, ggx)");
+    };
+    if (lt == "diffuse") {
+      m.type = RTG_MAT_DIFFUSE;
+      P.color("albedo", half, m.albedo);  // bsdf/Diffuse.cpp:15-17
+    } else if (lt == "roughconductor") {
+      m.type = RTG_MAT_ROUGH_CONDUCTOR;  // bsdf/RoughConductor.cpp:13-30
+      m.alpha = (float)P.real("alpha", 0.2);
+      m.distribution = dist(P.str("distribution", "beckmann"));
+      if (P.has("material") && !P.str("material", "").empty())
+        perr("bsdf RoughConductor: named material '" + P.str("material", "") +
+             "' needs spectral IOR data; give 'eta' and 'absorption' colors");
+      P.color("eta", zero, m.eta);  // tools/Common.cpp:56-80 fallback when no material name
+      P.color("absorption", zero, m.k);
+      P.color("reflectanceTexture", one, m.albedo);
+    } else if (lt == "roughdielectric") {
+      m.type = RTG_MAT_ROUGH_DIELECTRIC;  // bsdf/RoughDielectric.cpp:13-27
+      m.alpha = (float)P.real("alpha", 0.2);
+      m.eta_ext = (float)P.real("etaExt", 1.000277);
+      m.eta_int = (float)P.real("etaInt", 1.5046);
+      m.distribution = dist(P.str("distribution", "beckmann"));
+      P.color("reflectanceTexture", one, m.albedo);
+      P.color("transmittedTexture", one, m.transmitted);
+    } else {
+      perr("bsdf type " + *type + " is not rendered by the GPU path");
+    }
+    mats.push_back(m);
+    return (int)mats.size() - 1;
+  }
+
+  // io/ObjLoader.cpp:97-289: positions and normals transformed by toWorld (normals by the inverse
+  // transpose), quads split into (0,1,2) and (3,0,2), vertices deduplicated by (position, uv,
+  // normal) index triple, negative indices counted back from the end of the file's lists.
+  void obj(const std::string& path, const M4& T, Mesh& out) {
+    std::ifstream is(path);
+    if (!is) perr("cannot open mesh file " + path);
+    M4 inv;
+    if (!invert(T, inv)) perr("singular toWorld for " + path);
+    std::vector<double> P, N, UV;
+    struct VK { int p, t, n; };
+    struct VH {
+      size_t operator()(const VK& k) const { return ((size_t)k.p * 73856093u) ^ ((size_t)k.t * 19349663u) ^ ((size_t)k.n * 83492791u); }
+    };
+    struct VE {
+      bool operator()(const VK& a, const VK& b) const { return a.p == b.p && a.t == b.t && a.n == b.n; }
+    };
+    std::unordered_map<VK, int, VH, VE> map;
+    std::vector<VK> verts;
+    std::string line;
+    auto parse_v = [&](const std::string& tok) {
+      VK k{1, 1, 1};
+      std::vector<std::string> parts;
+      std::string cur;
+      for (char c : tok) {
+        if (c == '/') { parts.push_back(cur); cur.clear(); } else cur += c;
+      }
+      parts.push_back(cur);
+      if (parts.empty() || parts.size() > 3 || parts[0].empty()) perr("bad face vertex '" + tok + "' in " + path);
+      k.p = std::atoi(parts[0].c_str());
+      if (parts.size() > 1 && !parts[1].empty()) k.t = std::atoi(parts[1].c_str());
+      if (parts.size() > 2 && !parts[2].empty()) k.n = std::atoi(parts[2].c_str());
+      return k;
+    };
+    while (std::getline(is, line)) {
+      std::istringstream sl(line);
+      std::string type;
+      sl >> type;
+      if (type == "v") {
+        double x, y, z;
+        if (!(sl >> x >> y >> z)) perr("bad vertex line '" + line + "' in " + path);
+        const double* m = T.m;
+        P.push_back(m[0] * x + m[1] * y + m[2] * z + m[3]);
+        P.push_back(m[4] * x + m[5] * y + m[6] * z + m[7]);
+        P.push_back(m[8] * x + m[9] * y + m[10] * z + m[11]);
+      } else if (type == "vn") {
+        double x, y, z;
+        if (!(sl >> x >> y >> z)) perr("bad normal line '" + line + "' in " + path);
+        const double* m = inv.m;  // transpose of the inverse's 3x3
+        N.push_back(m[0] * x + m[4] * y + m[8] * z);
+        N.push_back(m[1] * x + m[5] * y + m[9] * z);
+        N.push_back(m[2] * x + m[6] * y + m[10] * z);
+      } else if (type == "vt") {
+        double u, v;
+        if (!(sl >> u)) perr("bad uv line '" + line + "' in " + path);
+        if (!(sl >> v)) v = 0.0;
+        UV.push_back(u);
+        UV.push_back(v);
+      } else if (type == "f") {
+        std::string t[5];
+        sl >> t[0] >> t[1] >> t[2] >> t[3] >> t[4];
+        if (t[2].empty()) perr("face with fewer than 3 vertices in " + path);
+        // a polygon with more than four vertices keeps its first four, as in the reference, which
+        // only logs it (io/ObjLoader.cpp:176-177)
+        VK f[6] = {parse_v(t[0]), parse_v(t[1]), parse_v(t[2])};
+        int nv = 3;
+        if (!t[3].empty()) {
+          f[3] = parse_v(t[3]);
+          f[4] = f[0];
+          f[5] = f[2];
+          nv = 6;
+        }
+        for (int i = 0; i < nv; i++) {
+          auto it = map.find(f[i]);
+          if (it == map.end()) {
+            map[f[i]] = (int)verts.size();
+            out.idx.push_back((int)verts.size());
+            verts.push_back(f[i]);
+          } else {
+            out.idx.push_back(it->second);
+          }
+        }
+      }
+    }
+    const int np = (int)P.size() / 3, nn = (int)N.size() / 3, nt = (int)UV.size() / 2;
+    auto fix = [](int i, int n) { return i < 0 ? n + i + 1 : i; };
+    for (const VK& k : verts) {
+      int p = fix(k.p, np) - 1;
+      if (p < 0 || p >= np) perr("face references a missing position in " + path);
+      for (int c = 0; c < 3; c++) out.v.push_back((float)P[3 * p + c]);
+    }
+    if (nn) {
+      for (const VK& k : verts) {
+        int q = fix(k.n, nn) - 1;
+        if (q < 0 || q >= nn) perr("face references a missing normal in " + path);
+        for (int c = 0; c < 3; c++) out.n.push_back((float)N[3 * q + c]);
+      }
+    }
+    if (nt) {
+      for (const VK& k : verts) {
+        int q = fix(k.t, nt) - 1;
+        if (q < 0 || q >= nt) perr("face references a missing uv in " + path);
+        for (int c = 0; c < 2; c++) out.uv.push_back((float)UV[2 * q + c]);
+      }
+    }
+  }
+
+  // geometricShape/Cube.cpp: the [-1,1]^3 cube, 24 vertices with per-face normals and UVs, two
+  // triangles (0,1,2), (3,0,2) per face; positions and normals transformed by toWorld
+  void cube(const M4& T, Mesh& out) {
+    static const int pos[24][3] = {{1, -1, -1}, {1, -1, 1}, {-1, -1, 1}, {-1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                   {-1, 1, 1}, {1, 1, 1}, {1, -1, -1}, {1, 1, -1}, {1, 1, 1}, {1, -1, 1},
+                                   {1, -1, 1}, {1, 1, 1}, {-1, 1, 1}, {-1, -1, 1}, {-1, -1, 1}, {-1, 1, 1},
+                                   {-1, 1, -1}, {-1, -1, -1}, {1, 1, -1}, {1, -1, -1}, {-1, -1, -1}, {-1, 1, -1}};
+    static const int nrm[6][3] = {{0, -1, 0}, {0, 1, 0}, {1, 0, 0}, {0, 0, 1}, {-1, 0, 0}, {0, 0, -1}};
+    static const int uv[4][2] = {{0, 1}, {1, 1}, {1, 0}, {0, 0}};
+    M4 inv;
+    if (!invert(T, inv)) perr("Cube: singular toWorld");
+    const double* m = T.m;
+    const double* q = inv.m;
+    for (int i = 0; i < 24; i++) {
+      const double x = pos[i][0], y = pos[i][1], z = pos[i][2];
+      out.v.push_back((float)(m[0] * x + m[1] * y + m[2] * z + m[3]));
+      out.v.push_back((float)(m[4] * x + m[5] * y + m[6] * z + m[7]));
+      out.v.push_back((float)(m[8] * x + m[9] * y + m[10] * z + m[11]));
+      const double a = nrm[i / 4][0], b = nrm[i / 4][1], c = nrm[i / 4][2];
+      out.n.push_back((float)(q[0] * a + q[4] * b + q[8] * c));
+      out.n.push_back((float)(q[1] * a + q[5] * b + q[9] * c));
+      out.n.push_back((float)(q[2] * a + q[6] * b + q[10] * c));
+      out.uv.push_back((float)uv[i % 4][0]);
+      out.uv.push_back((float)uv[i % 4][1]);
+    }
+    for (int f = 0; f < 6; f++) {
+      const int b = 4 * f;
+      const int t[6] = {b, b + 1, b + 2, b + 3, b, b + 2};
+      for (int k : t) out.idx.push_back(k);
+    }
+  }
+
+  // geometricShape/Plane.cpp: the square [-1,1]^2 at z = 0 of object space, normal +z; rendered as
+  // two triangles spanning the same square
+  void plane(const M4& T, Mesh& out) {
+    static const int pos[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+    M4 inv;
+    if (!invert(T, inv)) perr("Plane: singular toWorld");
+    const double* m = T.m;
+    const double* q = inv.m;
+    for (int i = 0; i < 4; i++) {
+      const double x = pos[i][0], y = pos[i][1];
+      out.v.push_back((float)(m[0] * x + m[1] * y + m[3]));
+      out.v.push_back((float)(m[4] * x + m[5] * y + m[7]));
+      out.v.push_back((float)(m[8] * x + m[9] * y + m[11]));
+      out.n.push_back((float)q[8]);
+      out.n.push_back((float)q[9]);
+      out.n.push_back((float)q[10]);
+      out.uv.push_back(0.5f * (float)(x + 1));
+      out.uv.push_back(0.5f * (float)(y + 1));
+    }
+    const int t[6] = {0, 1, 2, 0, 2, 3};
+    for (int k : t) out.idx.push_back(k);
+  }
+
+  void object(const XNode& n) {
+    const std::string* type = n.get("type");
+    if (!type) perr("<" + n.name + "> without a type");
+    const std::string lt = lower(*type);
+    int mat = -1;
+    const XNode* light = nullptr;
+    for (const XNode& c : n.kids) {
+      if (c.name == "bsdf") {
+        if (mat >= 0) perr("shape " + *type + ": only one bsdf allowed");
+        mat = bsdf(c);
+      } else if (c.name == "light") {
+        light = &c;
+      } else if (c.name == "bssrdf" || c.name == "medium") {
+        perr("shape " + *type + ": <" + c.name + "> is not rendered by the GPU path");
+      }
+    }
+    if (mat < 0) {  // shapes without a BSDF get the reference's default diffuse (0.5)
+      rtg_material m;
+      std::memset(&m, 0, sizeof(m));
+      m.type = RTG_MAT_DIFFUSE;
+      m.albedo[0] = m.albedo[1] = m.albedo[2] = 0.5f;
+      mats.push_back(m);
+      mat = (int)mats.size() - 1;
+    }
+    Props P = props(n, "shape " + *type);
+    const M4 T = P.transform("toWorld");
+    if (lt == "sphere") {
+      if (light) perr("shape Sphere: sphere emitters are not rendered by the GPU path");
+      rtg_sphere_desc s;
+      std::memset(&s, 0, sizeof(s));
+      for (int i = 0; i < 16; i++) s.to_world[i] = (float)T.m[i];
+      s.radius = (float)P.real("radius", 1.0);
+      s.material = mat;
+      spheres.push_back(s);
+      return;
+    }
+    Mesh m;
+    m.material = mat;
+    if (lt == "mesh") {
+      obj(resolve(P.str("path", "")), T, m);
+    } else if (lt == "cube") {
+      cube(T, m);
+    } else if (lt == "plane") {
+      plane(T, m);
+    } else {
+      perr("shape type " + *type + " is not rendered by the GPU path");
+    }
+    if (light) {
+      const std::string ltype = type_of(*light);
+      if (ltype != "meshlight" && ltype != "arealight") perr("shape light type must be MeshLight or AreaLight");
+      Props L = props(*light, "light " + ltype);
+      static const float zero[3] = {0, 0, 0};
+      L.color("radiance", zero, m.emission);  // light/AreaLight.cpp:12
+      m.emit = true;
+    }
+    meshes.push_back(std::move(m));
+  }
+
+  // Radiance RGBE (.hdr): new-style run-length encoded scanlines or flat pixels, -Y H +X W
+  void read_hdr(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) perr("cannot open environment map " + path);
+    std::string line;
+    bool fmt = false;
+    while (std::getline(f, line)) {
+      if (line.empty()) break;
+      if (line.find("FORMAT=32-bit_rle_rgbe") != std::string::npos) fmt = true;
+    }
+    if (!fmt) perr(path + ": not a Radiance RGBE image (only .hdr environment maps are supported)");
+    std::getline(f, line);
+    char ya[3], xa[3];
+    int H = 0, W = 0;
+    if (std::sscanf(line.c_str(), "%2s %d %2s %d", ya, &H, xa, &W) != 4 || std::string(ya) != "-Y" ||
+        std::string(xa) != "+X" || W <= 0 || H <= 0)
+      perr(path + ": unsupported RGBE orientation '" + line + "'");
+    std::vector<unsigned char> row((size_t)W * 4);
+    env.assign((size_t)W * H * 3, 0.f);
+    for (int y = 0; y < H; y++) {
+      unsigned char h[4];
+      if (!f.read((char*)h, 4)) perr(path + ": truncated");
+      if (W >= 8 && W < 32768 && h[0] == 2 && h[1] == 2 && ((h[2] << 8) | h[3]) == W) {
+        for (int c = 0; c < 4; c++) {
+          int x = 0;
+          while (x < W) {
+            unsigned char cnt;
+            if (!f.read((char*)&cnt, 1)) perr(path + ": truncated");
+            if (cnt > 128) {
+              unsigned char v;
+              if (!f.read((char*)&v, 1)) perr(path + ": truncated");
+              cnt -= 128;
+              if (x + cnt > W) perr(path + ": bad run length");
+              for (int k = 0; k < cnt; k++) row[(size_t)(x++) * 4 + c] = v;
+            } else {
+              if (cnt == 0 || x + cnt > W) perr(path + ": bad run length");
+              for (int k = 0; k < cnt; k++) {
+                unsigned char v;
+                if (!f.read((char*)&v, 1)) perr(path + ": truncated");
+                row[(size_t)(x++) * 4 + c] = v;
+              }
+            }
+          }
+        }
+      } else {  // flat
+        std::memcpy(row.data(), h, 4);
+        if (!f.read((char*)row.data() + 4, (std::streamsize)W * 4 - 4)) perr(path + ": truncated");
+      }
+      // the reference reads .hdr through stb_image's stbi_loadf (io/ImageLoader.cpp:80, gamma 1 for
+      // HDR files at :45-48); its decode (ext/include/stb/stb_image.h:6830-6849) is mantissa x
+      // 2^(exponent - 136), zero when the exponent byte is zero
+      for (int x = 0; x < W; x++) {
+        const unsigned char* p = &row[(size_t)x * 4];
+        float s = p[3] ? std::ldexp(1.0f, (int)p[3] - (128 + 8)) : 0.f;
+        for (int c = 0; c < 3; c++) env[((size_t)y * W + x) * 3 + c] = (float)p[c] * s;
+      }
+    }
+    env_w = W;
+    env_h = H;
+  }
+
+  void lights(const XNode& n) {
+    for (const XNode& c : n.kids) {
+      const std::string* type = c.get("type");
+      if (!type) perr("light without a type");
+      if (lower(*type) != "environmentlight") perr("light type " + *type + " is not rendered by the GPU path");
+      Props P = props(c, "light EnvironmentLight");  // light/EnvironmentLight.cpp:8-73
+      if (P.xf.count("toWorld")) {
+        const M4 T = P.transform("toWorld");
+        for (int i = 0; i < 16; i++)
+          if (std::fabs(T.m[i] - M4::id().m[i]) > 1e-9) perr("EnvironmentLight: only the identity toWorld is supported");
+      }
+      if (P.real("gamma", 0.0) != 0.0 || P.real("offsetU", 0.0) != 0.0 || P.real("offsetV", 0.0) != 0.0 ||
+          P.str("isProbe", "false") == "true")
+        perr("EnvironmentLight: gamma, offsets and light probes are not supported");
+      read_hdr(resolve(P.str("path", "")));
+      env_scale = (float)P.real("scale", 1.0);
+    }
+  }
+
+  void camera(const XNode& n) {
+    if (type_of(n) != "noriperspective") perr("camera type must be NoriPerspective");
+    M4 T = M4::id();
+    for (const XNode& c : n.kids) {
+      if (c.name == "transform") T = parse_transform(c);
+      if (c.name == "filter") {
+        Props F = props(c, "filter");
+        if (type_of(c) != "boxfilter" || F.real("radiusX", 0.5) != 0.5 || F.real("radiusY", 0.5) != 0.5)
+          perr("camera filter: only the box filter of radius 0.5 is rendered by the GPU path");
+      }
+    }
+    Props P = props(n, "camera NoriPerspective");
+    if (P.real("apertureRadius", 0.0) != 0.0) perr("camera: depth of field is not rendered by the GPU path");
+    rd.fov = (float)P.real("fov", 30.0);  // camera/NoriPerspective.cpp:15-17, camera/Camera.cpp:11-14
+    rd.near_clip = (float)P.real("nearClip", 1e-4);
+    rd.far_clip = (float)P.real("farClip", 1e4);
+    rd.width = P.integer("sizeX", 768);
+    rd.height = P.integer("sizeY", 768);
+    rd.use_camera_matrix = 1;
+    for (int i = 0; i < 16; i++) rd.camera_to_world[i] = (float)T.m[i];
+    have_camera = true;
+  }
+
+  void load(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) perr("cannot open scene file " + path);
+    std::stringstream ss;
+    ss << f.rdbuf();
+    const std::string text = ss.str();
+    size_t sl = path.find_last_of('/');
+    dir = sl == std::string::npos ? "." : path.substr(0, sl);
+    XNode root = XmlParser(text).parse();
+    if (root.name != "scene") perr("root element must be <scene>");
+    rd.spp = 1;
+    rd.sampler = RTG_SAMPLER_RANDOM;
+    rd.dimension = 4;
+    rd.min_depth = 3;
+    rd.max_depth = 1000;
+    rd.seed = 1;
+    // top-level references first so that objects may use them (the reference requires the same
+    // order in the file; accepting either order is a superset)
+    for (const XNode& c : root.kids) {
+      if (c.name == "bsdf" && c.get("id")) ref_bsdf[*c.get("id")] = c;
+      if (c.name == "transform" && c.get("id")) ref_xf[*c.get("id")] = parse_transform(c);
+    }
+    for (const XNode& c : root.kids) {
+      if (c.name == "camera") {
+        camera(c);
+      } else if (c.name == "integrator") {
+        const std::string* t = c.get("type");
+        if (type_of(c) != "pathtracingmis")
+          perr("integrator " + (t ? *t : std::string("?")) + " is not rendered by the GPU path (PathTracingMIS only)");
+        Props P = props(c, "integrator PathTracingMIS");  // integrator/PathTracingMIS.cpp:12-20
+        rd.min_depth = P.integer("minDepth", 3);
+        rd.max_depth = P.integer("maxDepth", 1000);
+        // integrator/Integrator.h:18-23 strategy names
+        std::string s = P.str("strategy", "one_light_uniform");
+        if (s == "one_light_uniform") strategy = RTG_LIGHTS_UNIFORM;
+        else if (s == "one_light_weighted") strategy = RTG_LIGHTS_WEIGHTED;
+        else perr("integrator strategy '" + s + "' is not rendered by the GPU path");
+        have_integrator = true;
+      } else if (c.name == "sampler") {
+        Props P = props(c, "sampler");  // sampler/RandomSampler.cpp:10, sampler/StratifiedSampler.cpp:23-24
+        if (type_of(c) == "randomsampler") rd.sampler = RTG_SAMPLER_RANDOM;
+        else if (type_of(c) == "stratifiedsampler") rd.sampler = RTG_SAMPLER_STRATIFIED;
+        else perr("sampler type must be RandomSampler or StratifiedSampler");
+        rd.spp = P.integer("sampleNumber", 1);
+        rd.dimension = P.integer("dimension", 4);
+      } else if (c.name == "objects") {
+        for (const XNode& o : c.kids) object(o);
+      } else if (c.name == "lights") {
+        lights(c);
+      } else if (c.name == "parameters" || c.name == "bsdf" || c.name == "transform") {
+        // threads, block size and output file do not change the image
+      } else {
+        perr("element <" + c.name + "> is not rendered by the GPU path");
+      }
+    }
+    if (!have_camera) perr("scene has no camera");
+  }
+};
+
+}  // namespace
+
+extern "C" int rtg_scene_load_xml(rtg_ctx* ctx, const char* path, rtg_scene** out, rtg_render_desc* desc) {
+  if (!ctx || !path || !out) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  try {
+    Loader L;
+    L.load(path);
+    std::vector<rtg_mesh_desc> md(L.meshes.size());
+    for (size_t i = 0; i < L.meshes.size(); i++) {
+      Loader::Mesh& m = L.meshes[i];
+      rtg_mesh_desc& d = md[i];
+      std::memset(&d, 0, sizeof(d));
+      d.vertices = m.v.data();
+      d.num_vertices = (int32_t)(m.v.size() / 3);
+      d.normals = m.n.empty() ? nullptr : m.n.data();
+      d.uvs = m.uv.empty() ? nullptr : m.uv.data();
+      d.indices = m.idx.data();
+      d.num_triangles = (int32_t)(m.idx.size() / 3);
+      d.material = m.material;
+      d.emission = m.emit ? m.emission : nullptr;
+    }
+    rtg_scene_desc sd;
+    std::memset(&sd, 0, sizeof(sd));
+    sd.materials = L.mats.data();
+    sd.num_materials = (int32_t)L.mats.size();
+    sd.meshes = md.data();
+    sd.num_meshes = (int32_t)md.size();
+    sd.spheres = L.spheres.data();
+    sd.num_spheres = (int32_t)L.spheres.size();
+    sd.env_rgb = L.env.empty() ? nullptr : L.env.data();
+    sd.env_width = L.env_w;
+    sd.env_height = L.env_h;
+    sd.env_scale = L.env_scale;
+    sd.light_strategy = L.strategy;
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    *out = build_scene(ctx, &sd);
+    if (desc) *desc = L.rd;
+    return RTG_OK;
+  } catch (const RtgError& e) {
+    set_error(e.what());
+    return e.code;
+  } catch (const std::exception& e) {
+    set_error(e.what());
+    return RTG_ERR_PARSE;
+  }
+}

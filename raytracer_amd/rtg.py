@@ -53,6 +53,7 @@ class rtg_render_desc(C.Structure):
         ("near_clip", C.c_float), ("far_clip", C.c_float), ("width", C.c_int32), ("height", C.c_int32),
         ("spp", C.c_int32), ("sampler", C.c_int32), ("dimension", C.c_int32), ("min_depth", C.c_int32),
         ("max_depth", C.c_int32), ("seed", C.c_uint64), ("guiding", C.c_int32),
+        ("use_camera_matrix", C.c_int32), ("camera_to_world", C.c_float * 16),
     ]
 
 
@@ -64,6 +65,13 @@ class rtg_stats(C.Structure):
         ("shadow_prims_tested", C.c_uint64), ("stage_ms", C.c_double * 8), ("stage_launches", C.c_uint64 * 8),
         ("guide_train_steps", C.c_uint64), ("guide_samples", C.c_uint64), ("guide_loss", C.c_double),
         ("guide_train_ms", C.c_double), ("guide_queries", C.c_uint64),
+    ]
+
+
+class rtg_scene_info(C.Structure):
+    _fields_ = [
+        ("triangles", C.c_int32), ("spheres", C.c_int32), ("materials", C.c_int32), ("lights", C.c_int32),
+        ("env_width", C.c_int32), ("env_height", C.c_int32), ("bbox_min", C.c_float * 3), ("bbox_max", C.c_float * 3),
     ]
 
 
@@ -92,6 +100,7 @@ SIGNATURES = {
     "rtg_scene_load_xml": (C.c_int, [P, C.c_char_p, C.POINTER(P), C.POINTER(rtg_render_desc)]),
     "rtg_scene_release": (C.c_int, [P, P]),
     "rtg_scene_bytes": (C.c_int, [P, C.POINTER(C.c_uint64)]),
+    "rtg_scene_query": (C.c_int, [P, C.POINTER(rtg_scene_info)]),
     "rtg_render": (C.c_int, [P, P, C.POINTER(rtg_render_desc), P, C.POINTER(rtg_stats)]),
     "rtg_render_device": (C.c_int, [P, P, C.POINTER(rtg_render_desc), P, C.POINTER(rtg_stats)]),
     "rtg_trace": (C.c_int, [P, P, P, C.c_int32, C.c_int32, P, P, P, P, P]),
@@ -107,6 +116,7 @@ SIGNATURES = {
     "rtg_guide_grad": (C.c_int, [P, P, P, C.c_int32, P, P]),
     "rtg_guide_train": (C.c_int, [P, P, P, C.c_int32, C.c_int32, C.c_float, P, P]),
     "rtg_guide_sample": (C.c_int, [P, P, P, C.c_int32, P, P]),
+    "rtg_guide_params": (C.c_int, [P, P]),
 }
 GUIDE_PARAMS = 20200
 
@@ -176,6 +186,9 @@ def render_struct(cam, rp):
     d.max_depth = int(rp.get("max_depth", 1000))
     d.seed = int(rp.get("seed", 1))
     d.guiding = int(rp.get("guiding", 0))
+    if cam.get("to_world") is not None:
+        d.use_camera_matrix = 1
+        d.camera_to_world[:] = [float(x) for x in np.asarray(cam["to_world"], np.float64).reshape(16)]
     return d
 
 
@@ -207,6 +220,19 @@ class Context:
             self.close()
         except Exception:
             pass
+
+    def load_xml(self, path):
+        """Scene file in the reference's XML schema -> (Scene, camera dict, render params dict)."""
+        h = P()
+        d = rtg_render_desc()
+        self.check(self.lib.rtg_scene_load_xml(self.h, str(path).encode(), C.byref(h), C.byref(d)))
+        cam = dict(origin=(0, 0, 0), target=(0, 0, 1), up=(0, 1, 0), fov=d.fov, near=d.near_clip, far=d.far_clip,
+                   width=d.width, height=d.height)
+        if d.use_camera_matrix:
+            cam["to_world"] = np.array(d.camera_to_world[:], np.float64).reshape(4, 4)
+        rp = dict(spp=d.spp, sampler=d.sampler, dimension=d.dimension, min_depth=d.min_depth,
+                  max_depth=d.max_depth, seed=d.seed)
+        return Scene.from_handle(self, h), cam, rp
 
     def set_pool(self, slots):
         self.check(self.lib.rtg_ctx_set_pool(self.h, int(slots)))
@@ -255,6 +281,12 @@ class Context:
                                             float(lr), _ptr(out), _ptr(losses)))
         return out, losses[:steps]
 
+    def guide_params(self):
+        """The network trained by the last guided render on this context."""
+        out = np.zeros(GUIDE_PARAMS, np.float32)
+        self.check(self.lib.rtg_guide_params(self.h, _ptr(out)))
+        return out
+
     def guide_sample(self, mix, u):
         mix = _f32(mix, (-1, 40)); u = _f32(u, (-1, 3))
         n = mix.shape[0]
@@ -275,6 +307,14 @@ class Context:
 
 class Scene:
     """A resident scene (BVH + tables in HBM)."""
+
+    @classmethod
+    def from_handle(cls, ctx, h):
+        obj = cls.__new__(cls)
+        obj.ctx = ctx
+        obj._keep = []
+        obj.h = h
+        return obj
 
     def __init__(self, ctx, sd):
         self.ctx = ctx
@@ -325,6 +365,11 @@ class Scene:
         if self.h:
             self.ctx.check(self.ctx.lib.rtg_scene_release(self.ctx.h, self.h))
             self.h = None
+
+    def info(self):
+        i = rtg_scene_info()
+        self.ctx.check(self.ctx.lib.rtg_scene_query(self.h, C.byref(i)))
+        return {k: (list(getattr(i, k)) if k.startswith("bbox") else getattr(i, k)) for k, _ in i._fields_}
 
     def bytes(self):
         b = C.c_uint64()

@@ -321,3 +321,132 @@ def glossy_cornell(width=512, height=512, spp=64, max_depth=1000, min_depth=3, s
     sd.render = dict(spp=spp, sampler=R.SAMPLER_STRATIFIED, dimension=4, min_depth=min_depth, max_depth=max_depth,
                      seed=seed)
     return sd
+
+
+# ---------------------------------------------------------------- scene files
+def write_rgbe(path, rgb):
+    """Radiance RGBE (.hdr) image from an (H, W, 3) float array (flat scanlines, -Y H +X W)."""
+    rgb = np.asarray(rgb, np.float64)
+    H, W = rgb.shape[:2]
+    m = rgb.max(axis=2)
+    e = np.zeros((H, W), np.int32)
+    mant = np.zeros((H, W), np.float64)
+    nz = m > 1e-32
+    mant[nz], e[nz] = np.frexp(m[nz])
+    px = np.zeros((H, W, 4), np.uint8)
+    scale = np.where(nz, mant * 256.0 / np.where(nz, m, 1.0), 0.0)
+    px[..., :3] = np.clip(np.floor(rgb * scale[..., None]), 0, 255).astype(np.uint8)
+    px[..., 3] = np.where(nz, e + 128, 0).astype(np.uint8)
+    with open(path, "wb") as f:
+        f.write(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n")
+        f.write(f"-Y {H} +X {W}\n".encode())
+        f.write(px.tobytes())
+
+
+def _fmt(v):
+    return " ".join(repr(float(x)) for x in v)
+
+
+def write_xml(sd, directory, name="scene.xml"):
+    """Write a SceneDesc as a scene file in the reference's XML schema (one OBJ per mesh and
+    material, a MeshLight on emitting meshes, Sphere shapes with a toWorld matrix, an RGBE
+    environment map). Returns the path of the XML file."""
+    import os
+
+    os.makedirs(directory, exist_ok=True)
+    cam, rp = sd.camera, sd.render
+    out = ['<?xml version="1.0" encoding="utf-8"?>', "<scene>"]
+    strat = {R.LIGHTS_UNIFORM: "one_light_uniform", R.LIGHTS_WEIGHTED: "one_light_weighted"}[sd.light_strategy]
+    out += ['  <integrator type="PathTracingMIS">',
+            f'    <integer name="minDepth" value="{int(rp.get("min_depth", 3))}"/>',
+            f'    <integer name="maxDepth" value="{int(rp.get("max_depth", 1000))}"/>',
+            f'    <string name="strategy" value="{strat}"/>', "  </integrator>"]
+    stype = "StratifiedSampler" if rp.get("sampler", R.SAMPLER_STRATIFIED) == R.SAMPLER_STRATIFIED else "RandomSampler"
+    out += [f'  <sampler type="{stype}">', f'    <integer name="sampleNumber" value="{int(rp["spp"])}"/>',
+            f'    <integer name="dimension" value="{int(rp.get("dimension", 4))}"/>', "  </sampler>"]
+    out += ['  <camera type="NoriPerspective">', '    <transform name="toWorld">', "      <lookAt>",
+            f"        <origin>{_fmt(cam['origin'])}</origin>", f"        <lookAt>{_fmt(cam['target'])}</lookAt>",
+            f"        <up>{_fmt(cam['up'])}</up>", "      </lookAt>", "    </transform>",
+            f'    <double name="fov" value="{float(cam["fov"])!r}"/>',
+            f'    <double name="nearClip" value="{float(cam.get("near", 1e-4))!r}"/>',
+            f'    <double name="farClip" value="{float(cam.get("far", 1e4))!r}"/>',
+            f'    <integer name="sizeX" value="{int(cam["width"])}"/>',
+            f'    <integer name="sizeY" value="{int(cam["height"])}"/>',
+            '    <filter type="BoxFilter"/>', "  </camera>"]
+
+    def bsdf(i, ind="      "):
+        m = sd.materials[i]
+        dist = "ggx" if m.get("distribution", R.DIST_BECKMANN) == R.DIST_GGX else "beckmann"
+        if m["type"] == R.MAT_DIFFUSE:
+            return [f'{ind}<bsdf type="Diffuse">', f'{ind}  <color name="albedo" value="{_fmt(m["albedo"])}"/>',
+                    f"{ind}</bsdf>"]
+        def tex(name, key):  # constant textures (texture/ConstantTexture.cpp:6-10)
+            c = tuple(float(x) for x in m.get(key, (1.0, 1.0, 1.0)))
+            if c == (1.0, 1.0, 1.0):
+                return []
+            return [f'{ind}  <texture name="{name}" type="ConstantTexture"><color name="albedo" value="{_fmt(c)}"/>'
+                    "</texture>"]
+
+        if m["type"] == R.MAT_ROUGH_CONDUCTOR:
+            return ([f'{ind}<bsdf type="RoughConductor">', f'{ind}  <double name="alpha" value="{float(m["alpha"])!r}"/>',
+                     f'{ind}  <string name="distribution" value="{dist}"/>',
+                     f'{ind}  <color name="eta" value="{_fmt(m["eta"])}"/>',
+                     f'{ind}  <color name="absorption" value="{_fmt(m["k"])}"/>']
+                    + tex("reflectanceTexture", "albedo") + [f"{ind}</bsdf>"])
+        return ([f'{ind}<bsdf type="RoughDielectric">', f'{ind}  <double name="alpha" value="{float(m["alpha"])!r}"/>',
+                 f'{ind}  <string name="distribution" value="{dist}"/>',
+                 f'{ind}  <double name="etaInt" value="{float(m["eta_int"])!r}"/>',
+                 f'{ind}  <double name="etaExt" value="{float(m["eta_ext"])!r}"/>']
+                + tex("reflectanceTexture", "albedo") + tex("transmittedTexture", "transmitted") + [f"{ind}</bsdf>"])
+
+    out.append("  <objects>")
+    k = 0
+    for mi, mesh in enumerate(sd.meshes):
+        idx = np.asarray(mesh.indices, np.int64).reshape(-1, 3)
+        tm = (np.full(len(idx), mesh.material, np.int64) if mesh.tri_material is None
+              else np.asarray(mesh.tri_material, np.int64))
+        for mat in dict.fromkeys(tm.tolist()):  # materials in order of first use
+            sel = idx[tm == mat]
+            used, first, inv = np.unique(sel.reshape(-1), return_index=True, return_inverse=True)
+            # number vertices in order of first use (the file then reads like the mesh)
+            order = np.argsort(first)
+            remap = np.empty(len(used), np.int64)
+            remap[order] = np.arange(len(used))
+            tri = remap[inv].reshape(-1, 3)
+            verts = np.asarray(mesh.vertices, np.float64).reshape(-1, 3)[used[order]]
+            fn = f"mesh{k}.obj"
+            k += 1
+            with open(os.path.join(directory, fn), "w") as f:
+                for v in verts.tolist():
+                    f.write(f"v {v[0]!r} {v[1]!r} {v[2]!r}\n")
+                has_n = mesh.normals is not None
+                if has_n:
+                    for v in np.asarray(mesh.normals, np.float64).reshape(-1, 3)[used[order]].tolist():
+                        f.write(f"vn {v[0]!r} {v[1]!r} {v[2]!r}\n")
+                for t in tri + 1:
+                    if has_n:
+                        f.write(f"f {t[0]}//{t[0]} {t[1]}//{t[1]} {t[2]}//{t[2]}\n")
+                    else:
+                        f.write(f"f {t[0]} {t[1]} {t[2]}\n")
+            out += ['    <shape type="Mesh">', f'      <string name="path" value="{fn}"/>']
+            out += bsdf(int(mat))
+            if mesh.emission is not None:
+                out += ['      <light type="MeshLight">', f'        <color name="radiance" value="{_fmt(mesh.emission)}"/>',
+                        "      </light>"]
+            out.append("    </shape>")
+    for s in sd.spheres:
+        M = np.asarray(s.to_world, np.float64).reshape(16)
+        out += ['    <shape type="Sphere">', f'      <double name="radius" value="{float(s.radius)!r}"/>',
+                '      <transform name="toWorld">', f'        <matrix value="{_fmt(M)}"/>', "      </transform>"]
+        out += bsdf(int(s.material))
+        out.append("    </shape>")
+    out.append("  </objects>")
+    if sd.env is not None:
+        write_rgbe(os.path.join(directory, "env.hdr"), sd.env)
+        out += ["  <lights>", '    <light type="EnvironmentLight">', '      <string name="path" value="env.hdr"/>',
+                f'      <double name="scale" value="{float(sd.env_scale)!r}"/>', "    </light>", "  </lights>"]
+    out.append("</scene>")
+    path = os.path.join(directory, name)
+    with open(path, "w") as f:
+        f.write("\n".join(out) + "\n")
+    return path

@@ -29,13 +29,28 @@ def test_library_exports_every_declared_symbol():
     assert set(syms) <= set(rtg.SIGNATURES)
 
 
-def test_struct_layouts_match_header():
-    # sizes implied by include/rtg.h on x86-64 (natural alignment)
-    assert ctypes.sizeof(rtg.rtg_material) == 4 * 17
-    assert ctypes.sizeof(rtg.rtg_sphere_desc) == 4 * 18
-    assert ctypes.sizeof(rtg.rtg_render_desc) == 4 * 20 + 8 + 8  # 20 words, u64 seed, int + pad
-    assert ctypes.sizeof(rtg.rtg_stats) == 8 * 6 + 8 * 4 + 8 * 8 * 2 + 8 * 5
-    assert ctypes.sizeof(rtg.rtg_server_stats) == 8 * 8 + 4 * 4
+def test_struct_layouts_match_header(tmp_path):
+    """The ctypes structures have the sizes and field offsets the C compiler gives include/rtg.h."""
+    structs = {"rtg_material": rtg.rtg_material, "rtg_mesh_desc": rtg.rtg_mesh_desc,
+               "rtg_sphere_desc": rtg.rtg_sphere_desc, "rtg_scene_desc": rtg.rtg_scene_desc,
+               "rtg_render_desc": rtg.rtg_render_desc, "rtg_stats": rtg.rtg_stats,
+               "rtg_server_stats": rtg.rtg_server_stats, "rtg_scene_info": rtg.rtg_scene_info}
+    src = ['#include <stdio.h>', '#include <stddef.h>', '#include "rtg.h"', "int main(void) {"]
+    for name, cls in structs.items():
+        src.append(f'printf("{name} %zu\\n", sizeof({name}));')
+        for fname, _ in cls._fields_:
+            src.append(f'printf("{name}.{fname} %zu\\n", offsetof({name}, {fname}));')
+    src.append("return 0; }")
+    c = tmp_path / "sizes.c"
+    c.write_text("\n".join(src))
+    exe = tmp_path / "sizes"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(c), "-o", str(exe)], check=True)
+    out = dict(line.split() for line in subprocess.run([str(exe)], capture_output=True, text=True,
+                                                         check=True).stdout.splitlines())
+    for name, cls in structs.items():
+        assert int(out[name]) == ctypes.sizeof(cls), name
+        for fname, _ in cls._fields_:
+            assert int(out[f"{name}.{fname}"]) == getattr(cls, fname).offset, (name, fname)
 
 
 def test_icosphere_counts_and_closure():

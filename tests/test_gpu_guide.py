@@ -249,12 +249,12 @@ def _block_means(img, block=8):
 
 def test_guided_render_matches_plain(ctx):
     """One-sample MIS keeps the plain estimator's expectation: per-block means of guided and plain
-    renders (4 seeds each, 32x32, 2048 spp) agree within their standard errors."""
+    renders (8 seeds each, 32x32, 2048 spp) agree within their standard errors."""
     sd = scenes.glossy_cornell()
     cam = dict(sd.camera, width=32, height=32)
     g = ctx.upload(sd)
     plain, guided = [], []
-    for s in range(4):
+    for s in range(8):
         out, _ = g.render(cam, dict(sd.render, spp=2048, seed=100 + s))
         plain.append(_block_means(rtg.resolve(out).astype(np.float64)))
         out, st = g.render(cam, dict(sd.render, spp=2048, seed=200 + s, guiding=1))
@@ -262,7 +262,7 @@ def test_guided_render_matches_plain(ctx):
         assert st.nan_samples == 0
         assert st.guide_train_steps > 0
     P, Gd = np.stack(plain), np.stack(guided)
-    se = np.sqrt(P.var(0, ddof=1) / 4 + Gd.var(0, ddof=1) / 4) + 1e-4 * (np.abs(P.mean(0)) + 1e-3)
+    se = np.sqrt(P.var(0, ddof=1) / 8 + Gd.var(0, ddof=1) / 8) + 1e-4 * (np.abs(P.mean(0)) + 1e-3)
     z = (Gd.mean(0) - P.mean(0)) / se
     print(f"  guided vs plain: max|z| {np.abs(z).max():.2f} mean z^2 {np.mean(z * z):.2f} "
           f"means {Gd.mean():.5f} {P.mean():.5f} train steps {st.guide_train_steps}", flush=True)

@@ -108,7 +108,7 @@ def _block_z_test(ctx, sd, cam, spp_cpu, n_cpu, spp_gpu, label, block=8):
 def test_glossy_cornell_unbiased(ctx):
     sd = scenes.glossy_cornell()
     cam = dict(sd.camera, width=32, height=32)
-    _block_z_test(ctx, sd, cam, 128, 8, 16384, "glossy cornell (dielectric spheres, conductor walls)")
+    _block_z_test(ctx, sd, cam, 128, 16, 16384, "glossy cornell (dielectric spheres, conductor walls)")
 
 
 def test_displaced_sphere_env_unbiased(ctx):

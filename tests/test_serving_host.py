@@ -41,7 +41,7 @@ def build_mock_lib():
     os.makedirs(BUILD, exist_ok=True)
     out = os.path.join(BUILD, "librtg_hostmock.so")
     srcs = [os.path.join(CSRC, f) for f in ("rtg_serve.cpp", "rtg_host.cpp", "rtg_guide_host.cpp", "rtg_bvh.cpp",
-                                            "rtg_stubs.cpp")]
+                                            "rtg_xml.cpp")]
     srcs.append(os.path.join(MOCK, "mock_kernels.cpp"))
     cmd = ["g++", "-std=c++17", "-O1", "-g", "-fPIC", "-shared", "-pthread", "-Wl,--no-undefined", "-I" + MOCK,
            "-I" + CSRC] + srcs + ["-o", out]

@@ -1,0 +1,271 @@
+"""Scene-file loading (rtg_scene_load_xml) on the CPU: librtg's host sources built with g++ against
+the HIP stand-in in tests/mock_hip (device memory is host memory there), driven through the ctypes
+binding. Each case runs in a child process so that the stand-in library never meets the real one.
+
+What the loader must reproduce (core/XMLLoader.cpp:33-904, io/ObjLoader.cpp:97-289):
+  * a scene written by scenes.write_xml loads back with the same triangle, sphere, material and
+    light counts and the same bounding box as the same SceneDesc uploaded directly, and the same
+    render parameters and camera frame (lookAt columns left, up, dir, origin: core/Transform.cpp);
+  * OBJ features: quads split (0,1,2), (3,0,2), negative (relative) indices, v/vt/vn forms,
+    comments and group lines, toWorld applied to positions;
+  * Cube (12 triangles) and Plane (2 triangles) shapes, bsdf and transform references by id,
+    case-insensitive type names, constant textures;
+  * errors name what failed, and the context stays usable after one.
+"""
+import os
+import subprocess
+import sys
+import textwrap
+
+import numpy as np
+import pytest
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+
+@pytest.fixture(scope="module")
+def mock_lib():
+    from test_serving_host import build_mock_lib
+
+    return build_mock_lib()
+
+
+def _run(mock_lib, case, tmp_path):
+    code = textwrap.dedent(f"""
+        import sys
+        sys.path.insert(0, {ROOT!r}); sys.path.insert(0, {HERE!r})
+        from raytracer_amd import rtg
+        rtg.LIB_PATH = {mock_lib!r}
+        import test_xml_host as T
+        ctx = rtg.Context(0)
+        T.{case}(ctx, {str(tmp_path)!r})
+        ctx.close()
+        print("CASE OK", flush=True)
+    """)
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300,
+                       env=dict(os.environ, MALLOC_CHECK_="3"))
+    assert r.returncode == 0 and "CASE OK" in r.stdout, (r.stdout[-2000:], r.stderr[-3000:])
+
+
+# ------------------------------------------------------------------ cases (run in the child)
+def lookat_matrix(origin, target, up):
+    d = np.subtract(target, origin).astype(np.float64)
+    d /= np.linalg.norm(d)
+    u = np.asarray(up, np.float64) / np.linalg.norm(up)
+    left = np.cross(u, d)
+    left /= np.linalg.norm(left)
+    nu = np.cross(d, left)
+    nu /= np.linalg.norm(nu)
+    M = np.eye(4)
+    M[:3, 0], M[:3, 1], M[:3, 2], M[:3, 3] = left, nu, d, origin
+    return M
+
+
+def _same_info(a, b):
+    # materials are not compared: every shape in a scene file carries its own BSDF instance
+    for k in ("triangles", "spheres", "lights", "env_width", "env_height"):
+        assert a[k] == b[k], (k, a[k], b[k])
+    assert np.array_equal(a["bbox_min"], b["bbox_min"]), (a["bbox_min"], b["bbox_min"])
+    assert np.array_equal(a["bbox_max"], b["bbox_max"]), (a["bbox_max"], b["bbox_max"])
+
+
+def case_cornell_roundtrip(ctx, tmp):
+    from raytracer_amd import scenes
+
+    sd = scenes.cornell(width=96, height=64, spp=36, max_depth=7, min_depth=2)
+    path = scenes.write_xml(sd, tmp)
+    sc, cam, rp = ctx.load_xml(path)
+    _same_info(sc.info(), ctx.upload(sd).info())
+    assert (cam["width"], cam["height"], cam["fov"]) == (96, 64, 40.0)
+    assert (rp["spp"], rp["min_depth"], rp["max_depth"]) == (36, 2, 7)
+    assert rp["sampler"] == scenes.R.SAMPLER_STRATIFIED and rp["dimension"] == 4
+    M = lookat_matrix(sd.camera["origin"], sd.camera["target"], sd.camera["up"])
+    assert np.allclose(cam["to_world"], M, atol=1e-6), cam["to_world"]
+
+
+def case_glossy_roundtrip(ctx, tmp):
+    from raytracer_amd import scenes
+
+    sd = scenes.glossy_cornell(width=64, height=64, spp=16)
+    path = scenes.write_xml(sd, tmp)
+    sc, cam, rp = ctx.load_xml(path)
+    i = sc.info()
+    _same_info(i, ctx.upload(sd).info())
+    assert i["spheres"] == len(sd.spheres) == 2 and i["lights"] == 1
+
+
+def case_env_roundtrip(ctx, tmp):
+    from raytracer_amd import scenes
+
+    sd = scenes.displaced_sphere(width=64, height=64, spp=4, level=3)
+    path = scenes.write_xml(sd, tmp)
+    sc, _, _ = ctx.load_xml(path)
+    i = sc.info()
+    _same_info(i, ctx.upload(sd).info())
+    assert (i["env_height"], i["env_width"]) == sd.env.shape[:2]
+
+
+CAMERA = """
+  <camera type="NoriPerspective">
+    <transform name="toWorld"><lookAt><origin>0 0 5</origin><lookAt>0 0 0</lookAt><up>0 1 0</up></lookAt></transform>
+    <double name="fov" value="30"/><integer name="sizeX" value="8"/><integer name="sizeY" value="8"/>
+  </camera>"""
+
+
+def _scene(tmp, body, files=None, name="s.xml"):
+    for fn, text in (files or {}).items():
+        with open(os.path.join(tmp, fn), "w") as f:
+            f.write(text)
+    p = os.path.join(tmp, name)
+    with open(p, "w") as f:
+        f.write(f'<?xml version="1.0"?>\n<!-- test scene -->\n<scene>{CAMERA}\n{body}\n</scene>\n')
+    return p
+
+
+OBJ_QUADS = """# two quads and a triangle
+o thing
+g part
+v 0 0 0
+v 1 0 0
+v 1 1 0
+v 0 1 0
+vt 0 0
+vt 1 0
+vt 1 1
+vn 0 0 1
+s off
+f 1/1/1 2/2/1 3/3/1 4/1/1
+v 0 0 2
+v 2 0 2
+v 2 3 2
+v 0 3 2
+f -4//1 -3//1 -2//1 -1//1
+f 1 2 6
+"""
+
+
+def case_obj_features(ctx, tmp):
+    body = """<objects>
+      <shape type="mesh"><string name="path" value="q.obj"/>
+        <transform name="toWorld"><scale><x>2</x><y>2</y><z>2</z></scale>
+          <translation><x>1</x><y>-1</y><z>0.5</z></translation></transform>
+        <bsdf type="diffuse"><color name="albedo" value="0.2 0.3 0.4"/></bsdf></shape>
+    </objects>"""
+    p = _scene(tmp, body, {"q.obj": OBJ_QUADS})
+    sc, cam, rp = ctx.load_xml(p)
+    i = sc.info()
+    assert i["triangles"] == 5 and i["materials"] == 1 and i["lights"] == 0
+    # object-space box [0,2]x[0,3]x[0,2], scaled by 2 then translated: translate applies last
+    # (left multiplication, core/XMLLoader.cpp handleTransform)
+    assert np.allclose(i["bbox_min"], [1, -1, 0.5], atol=1e-5), i["bbox_min"]
+    assert np.allclose(i["bbox_max"], [5, 5, 4.5], atol=1e-5), i["bbox_max"]
+    assert rp["sampler"] == scenes_R().SAMPLER_RANDOM and rp["spp"] == 1  # no <sampler>: defaults
+
+
+def scenes_R():
+    from raytracer_amd import scenes
+
+    return scenes.R
+
+
+def case_shapes_refs_textures(ctx, tmp):
+    body = """
+    <bsdf type="RoughConductor" id="metal">
+      <double name="alpha" value="0.3"/><string name="distribution" value="ggx"/>
+      <color name="eta" value="0.2 0.4 1.2"/><color name="absorption" value="3 2.3 1.7"/>
+      <texture name="reflectanceTexture" type="ConstantTexture"><color name="albedo" value="0.9 0.8 0.7"/></texture>
+    </bsdf>
+    <transform id="lift"><translation><y>3</y></translation></transform>
+    <objects>
+      <shape type="CUBE"><bsdf ref="metal"/></shape>
+      <shape type="Plane"><transform ref="lift"/><bsdf ref="metal"/>
+        <light type="MeshLight"><color name="radiance" value="5 5 5"/></light></shape>
+      <shape type="Sphere"><double name="radius" value="0.5"/>
+        <transform name="toWorld"><translation><x>4</x></translation></transform>
+        <bsdf type="RoughDielectric"><double name="alpha" value="0.1"/></bsdf></shape>
+      <shape type="Plane"><transform name="toWorld"><translation><z>-3</z></translation></transform></shape>
+    </objects>
+    <integrator type="pathtracingmis"><integer name="maxDepth" value="5"/>
+      <string name="strategy" value="one_light_weighted"/></integrator>
+    <sampler type="stratifiedsampler"><integer name="sampleNumber" value="10"/></sampler>"""
+    sc, cam, rp = ctx.load_xml(_scene(tmp, body))
+    i = sc.info()
+    assert i["triangles"] == 12 + 2 + 2 and i["spheres"] == 1 and i["lights"] == 1
+    # metal is instanced per reference (two), one dielectric, one default diffuse
+    assert i["materials"] == 4
+    assert np.allclose(i["bbox_min"], [-1, -1, -3], atol=1e-6), i["bbox_min"]
+    # the Plane spans [-1,1]^2 of its xy plane (geometricShape/Plane.cpp), lifted to y in [2, 4]
+    assert np.allclose(i["bbox_max"], [4.5, 4, 1], atol=1e-6), i["bbox_max"]
+    assert rp["max_depth"] == 5 and rp["spp"] == 10
+
+
+def case_errors(ctx, tmp):
+    from raytracer_amd import rtg
+
+    cases = [
+        ("<objects><shape type='mesh'><string name='path' value='q.obj'/></objects>", "closes"),
+        ("<objects><shape type='mesh'><string name='path' value='missing.obj'/></shape></objects>",
+         "cannot open mesh file"),
+        ("<objects><shape type='Disk'/></objects>", "not rendered by the GPU path"),
+        ("<objects><shape type='Cube'><bsdf type='Phong'/></shape></objects>", "bsdf type Phong"),
+        ("<objects><shape type='Cube'><bsdf type='Diffuse'><texture name='albedo' type='Checkerboard'/>"
+         "</bsdf></shape></objects>", "ConstantTexture only"),
+        ("<integrator type='PathTracing'/>", "PathTracingMIS only"),
+        ("<objects><shape type='Cube'><bsdf ref='nothing'/></shape></objects>", "unknown bsdf ref"),
+        ("<objects><shape type='mesh'><string name='path' value='bad.obj'/></shape></objects>",
+         "missing position"),
+    ]
+    files = {"bad.obj": "v 0 0 0\nv 1 0 0\nf 1 2 3\n"}
+    for k, (body, msg) in enumerate(cases):
+        p = _scene(tmp, body, files, name=f"e{k}.xml")
+        with pytest.raises(rtg.RtgError) as e:
+            ctx.load_xml(p)
+        assert msg in str(e.value), (k, str(e.value))
+    with pytest.raises(rtg.RtgError) as e:
+        ctx.load_xml(os.path.join(tmp, "no_such_file.xml"))
+    # still usable
+    sc, _, _ = ctx.load_xml(_scene(tmp, "<objects><shape type='Cube'/></objects>", name="ok.xml"))
+    assert sc.info()["triangles"] == 12
+
+
+def case_rgbe(ctx, tmp):
+    """write_rgbe / the loader's decode: the decoded map is within one mantissa step of the input
+    (stb_image's rule, mantissa x 2^(e-136)), and an exact power-of-two grid decodes exactly."""
+    from raytracer_amd import scenes
+
+    rng = np.random.default_rng(3)
+    env = rng.random((8, 16, 3)) * np.logspace(-3, 3, 16)[None, :, None]
+    env[0, 0] = 0
+    env[1, 1] = (0.5, 0.25, 1.0)
+    scenes.write_rgbe(os.path.join(tmp, "e.hdr"), env)
+    dec = decode_rgbe(os.path.join(tmp, "e.hdr"))
+    m = env.max(axis=2, keepdims=True)
+    assert np.all(np.abs(dec - env) <= m / 128 + 1e-30)
+    assert np.array_equal(dec[1, 1], [0.5, 0.25, 1.0]) and np.all(dec[0, 0] == 0)
+    body = "<objects><shape type='Cube'/></objects><lights><light type='EnvironmentLight'>" \
+           "<string name='path' value='e.hdr'/></light></lights>"
+    sc, _, _ = ctx.load_xml(_scene(tmp, body))
+    i = sc.info()
+    assert (i["env_width"], i["env_height"]) == (16, 8) and i["lights"] == 1
+
+
+def decode_rgbe(path):
+    """Flat-scanline RGBE reader (as written by scenes.write_rgbe), stb_image's decode rule."""
+    with open(path, "rb") as f:
+        data = f.read()
+    head, _, rest = data.partition(b"\n\n")
+    dims, _, px = rest.partition(b"\n")
+    _, H, _, W = dims.split()
+    a = np.frombuffer(px, np.uint8).reshape(int(H), int(W), 4).astype(np.float64)
+    s = np.where(a[..., 3] > 0, np.ldexp(1.0, a[..., 3].astype(int) - 136), 0.0)
+    return a[..., :3] * s[..., None]
+
+
+# ------------------------------------------------------------------ pytest entry points
+@pytest.mark.parametrize("case", ["case_cornell_roundtrip", "case_glossy_roundtrip", "case_env_roundtrip",
+                                  "case_obj_features", "case_shapes_refs_textures", "case_errors", "case_rgbe"])
+def test_xml_loader(mock_lib, case, tmp_path):
+    _run(mock_lib, case, tmp_path)
