@@ -3,11 +3,13 @@ scenes uploaded directly.
 
 scenes.write_xml writes every float with its exact decimal form and keeps triangles in the same order
 where each mesh's materials are contiguous (Cornell box, glossy Cornell), so those scenes load back
-bit-for-bit equal and, with the same seed, render bit-for-bit equal images: the camera frame of the
-file's <lookAt> is built with the same double-precision steps as the library's own. The displaced
-sphere interleaves materials over its triangles (so the file reorders them, and the BVH differs) and
-its environment map goes through 8-bit RGBE: it is compared with the direct upload of the same
-quantised map, where only exact first-hit ties may differ."""
+bit-for-bit equal and, with the same seed, trace the same paths: the camera frame of the file's
+<lookAt> is built with the same double-precision steps as the library's own. The images then agree
+up to the order in which the film's float atomic adds land (a few ulp; any path that differs would
+move a pixel by orders of magnitude more). The displaced sphere interleaves materials over its
+triangles (so the file reorders them, and the BVH differs) and its environment map goes through
+8-bit RGBE: it is compared with the direct upload of the same quantised map, where only exact
+first-hit ties may take another path."""
 import os
 
 import numpy as np
@@ -44,7 +46,7 @@ def _pair(ctx, sd, tmp_path, spp):
 
 
 @pytest.mark.parametrize("name", ["cornell", "glossy_cornell"])
-def test_xml_scene_renders_bit_identical(ctx, tmp_path, name):
+def test_xml_scene_renders_the_same_paths(ctx, tmp_path, name):
     sd = getattr(scenes, name)(width=64, height=64, spp=256)
     print(f"[xml] {name} 64x64 256 spp, file vs direct upload", flush=True)
     a, b = _pair(ctx, sd, tmp_path, 256)
@@ -52,7 +54,7 @@ def test_xml_scene_renders_bit_identical(ctx, tmp_path, name):
     print(f"  max |diff| {d.max():.3e}, differing pixels {np.mean(d.max(axis=2) > 0):.4f}, mean {a.mean():.5f}",
           flush=True)
     assert a.mean() > 0
-    assert np.array_equal(a, b)
+    assert np.all(d <= 1e-5 * np.maximum(1.0, b)), d.max()
 
 
 def test_xml_env_scene_matches(ctx, tmp_path):
