@@ -389,90 +389,135 @@ __device__ __forceinline__ void stk_top(const TravStack& s, int sp, uint32_t& re
 }
 
 // BVH traversal over child-pair nodes, nearer child first. Closest hit keeps the reference's
-// acceptance rules (triangles accept t == maxT, the last accepted primitive wins); ANY returns at
-// the first accepted primitive.
-template <bool ANY>
-__device__ __forceinline__ bool traverse(const DevScene& S, Ray r, const TravStack& stk, HitRec& h,
-                                         uint32_t& nodes_visited, uint32_t& prims_tested) {
-  h.prim = -1;
-  adapt_eps(r);
-  if (S.empty || r.tmax < r.tmin) return false;
+// acceptance rules (triangles accept t == maxT, the last accepted primitive wins); ANY stops at
+// the first accepted primitive. The traversal is resumable one step at a time (TravState), so a
+// persistent kernel can give a lane a new ray as soon as its previous one is done.
+struct TravState {
+  Ray r;
+  uint32_t ref;  // node or leaf to visit next
+  int sp;        // stack entries in use
+  bool zdir;     // a direction component is zero: the reference's per-axis slab branch
+  bool hit;
+  HitRec h;
+};
+// The scene arrays a traversal reads, kept in registers for the duration of a ray.
+struct SceneGeo {
+  const float4* nodes;
+  const float* tri_v;
+  const float4* spheres;
+  int32_t ntri;
+};
+__device__ __forceinline__ SceneGeo scene_geo(const DevScene& S) {
+  SceneGeo g;
+  g.nodes = S.nodes;
+  g.tri_v = S.tri_v;
+  g.spheres = S.spheres;
+  g.ntri = S.ntri;
+  return g;
+}
+
+// Starts the traversal of s.r; false when there is nothing to traverse (empty scene, empty ray
+// interval or a miss of the scene box), with s.hit false and s.h.prim -1.
+__device__ __forceinline__ bool trav_begin(const DevScene& S, TravState& s) {
+  s.h.prim = -1;
+  s.hit = false;
+  s.sp = 0;
+  adapt_eps(s.r);
+  if (S.empty || s.r.tmax < s.r.tmin) return false;
   float tn;
-  if (!slab(r, S.root_min[0], S.root_min[1], S.root_min[2], S.root_max[0], S.root_max[1], S.root_max[2], tn))
+  if (!slab(s.r, S.root_min[0], S.root_min[1], S.root_min[2], S.root_max[0], S.root_max[1], S.root_max[2], tn))
     return false;
-  uint32_t ref = S.root_ref;
-  int sp = 0;
-  bool hit = false;
-  // rays with a zero direction component take the reference's per-axis branch (rare)
-  const bool zdir = has_zero_dir(r);
-  while (true) {
-    if (!(ref & kLeafBit)) {
-      const float4* nd = S.nodes + 4u * ref;
-      float4 a0 = nd[0], a1 = nd[1], c0 = nd[2], c1 = nd[3];
-      nodes_visited++;
-      float t0, t1;
-      bool h0, h1;
-      if (!zdir) {
-        h0 = slab_fast(r, a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, t0);
-        h1 = slab_fast(r, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, t1);
-      } else {
-        h0 = slab(r, a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, t0);
-        h1 = slab(r, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, t1);
-      }
-      uint32_t r0 = __float_as_uint(a1.z), r1 = __float_as_uint(c1.z);
-      if (h0 && h1) {
-        uint32_t nearR = r0, farR = r1;
-        float farT = t1;
-        if (t1 < t0) { nearR = r1; farR = r0; farT = t0; }
-        stk_push(stk, sp, farR, farT);
-        ref = nearR;
-        continue;
-      } else if (h0) {
-        ref = r0;
-        continue;
-      } else if (h1) {
-        ref = r1;
-        continue;
+  s.ref = S.root_ref;
+  s.zdir = has_zero_dir(s.r);
+  return true;
+}
+
+// One step: an interior node (descend into the nearer hit child, push the farther one) or a leaf
+// followed by the pops up to the next subtree that can still hold a closer hit. True when the
+// traversal has finished; the result is in s.hit / s.h.
+template <bool ANY>
+__device__ __forceinline__ bool trav_step(const SceneGeo& G, TravState& s, const TravStack& stk,
+                                          uint32_t& nodes_visited, uint32_t& prims_tested) {
+  if (!(s.ref & kLeafBit)) {
+    const float4* nd = G.nodes + 4u * s.ref;
+    float4 a0 = nd[0], a1 = nd[1], c0 = nd[2], c1 = nd[3];
+    nodes_visited++;
+    float t0, t1;
+    bool h0, h1;
+    if (!s.zdir) {
+      h0 = slab_fast(s.r, a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, t0);
+      h1 = slab_fast(s.r, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, t1);
+    } else {
+      h0 = slab(s.r, a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, t0);
+      h1 = slab(s.r, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, t1);
+    }
+    uint32_t r0 = __float_as_uint(a1.z), r1 = __float_as_uint(c1.z);
+    if (h0 && h1) {
+      uint32_t nearR = r0, farR = r1;
+      float farT = t1;
+      if (t1 < t0) { nearR = r1; farR = r0; farT = t0; }
+      stk_push(stk, s.sp, farR, farT);
+      s.ref = nearR;
+      return false;
+    } else if (h0) {
+      s.ref = r0;
+      return false;
+    } else if (h1) {
+      s.ref = r1;
+      return false;
+    }
+  } else {
+    uint32_t first = leaf_first(s.ref), cnt = leaf_count(s.ref);
+    prims_tested += cnt;
+    if ((int32_t)first < G.ntri) {
+      for (uint32_t i = 0; i < cnt; i++) {
+        float t, b1, b2;
+        if (tri_hit(s.r, G.tri_v + 9u * (first + i), t, b1, b2)) {
+          s.h.t = t; s.h.b1 = b1; s.h.b2 = b2; s.h.prim = (int32_t)(first + i);
+          s.hit = true;
+          if (ANY) return true;
+          s.r.tmax = t;
+        }
       }
     } else {
-      uint32_t first = leaf_first(ref), cnt = leaf_count(ref);
-      prims_tested += cnt;
-      if ((int32_t)first < S.ntri) {
-        for (uint32_t i = 0; i < cnt; i++) {
-          float t, b1, b2;
-          if (tri_hit(r, S.tri_v + 9u * (first + i), t, b1, b2)) {
-            h.t = t; h.b1 = b1; h.b2 = b2; h.prim = (int32_t)(first + i);
-            if (ANY) return true;
-            r.tmax = t;
-            hit = true;
-          }
+      for (uint32_t i = 0; i < cnt; i++) {
+        float t;
+        uint32_t si = first + i - (uint32_t)G.ntri;
+        if (sphere_hit(s.r, G.spheres + 5u * si, t)) {
+          s.h.t = t; s.h.b1 = 0.f; s.h.b2 = 0.f; s.h.prim = (int32_t)(first + i);
+          s.hit = true;
+          if (ANY) return true;
+          s.r.tmax = t;
         }
-      } else {
-        for (uint32_t i = 0; i < cnt; i++) {
-          float t;
-          uint32_t si = first + i - (uint32_t)S.ntri;
-          if (sphere_hit(r, S.spheres + 5u * si, t)) {
-            h.t = t; h.b1 = 0.f; h.b2 = 0.f; h.prim = (int32_t)(first + i);
-            if (ANY) return true;
-            r.tmax = t;
-            hit = true;
-          }
-        }
-      }
-    }
-    // pop; a popped subtree whose box starts beyond the current maxT fails its box test
-    while (true) {
-      if (sp == 0) return hit;
-      sp--;
-      float pt;
-      uint32_t pr;
-      stk_top(stk, sp, pr, pt);
-      if (r.tmax >= pt) {
-        ref = pr;
-        break;
       }
     }
   }
+  // pop; a popped subtree whose box starts beyond the current maxT fails its box test
+  while (true) {
+    if (s.sp == 0) return true;
+    s.sp--;
+    float pt;
+    uint32_t pr;
+    stk_top(stk, s.sp, pr, pt);
+    if (s.r.tmax >= pt) {
+      s.ref = pr;
+      return false;
+    }
+  }
+}
+
+template <bool ANY>
+__device__ __forceinline__ bool traverse(const DevScene& S, Ray r, const TravStack& stk, HitRec& h,
+                                         uint32_t& nodes_visited, uint32_t& prims_tested) {
+  TravState s;
+  s.r = r;
+  if (trav_begin(S, s)) {
+    const SceneGeo G = scene_geo(S);
+    while (!trav_step<ANY>(G, s, stk, nodes_visited, prims_tested)) {
+    }
+  }
+  h = s.h;
+  return s.hit;
 }
 
 // Surface data at a hit (Triangle.cpp:130-215, Sphere.cpp:115-139, Primitive.cpp:28)

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -533,6 +534,7 @@ LaunchCfg launch_cfg(rtg_ctx* ctx) {
   cfg.block = 256;
   cfg.stack = stack_for_resident(ctx);
   cfg.gstack = ctx->gstack;
+  cfg.refill = ctx->refill;
   return cfg;
 }
 
@@ -625,6 +627,8 @@ int rtg_ctx_create(int device, rtg_ctx** out) {
     ctx->scene_slots.assign(kMaxScenes, nullptr);
     ctx->pool_slots = 1 << 21;
     ctx->grid = std::max(64, ctx->cu_count * 5);
+    // tuning knob for the persistent traversal (lanes idle before a wave refills, 1..64)
+    if (const char* e = std::getenv("RTG_TRACE_REFILL")) ctx->refill = std::min(64, std::max(1, std::atoi(e)));
     // traversal stack spill area for every lane of a traversal launch (2 * grid blocks of 128)
     size_t lanes = (size_t)ctx->grid * 2 * 128;
     hip_check(hipMalloc(&ctx->gstack, lanes * (kMaxStack - 16) * sizeof(uint2)), "hipMalloc stack spill");
@@ -934,6 +938,7 @@ int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int3
       cfg.block = 128;
       cfg.stack = scene->stack;
       cfg.gstack = ctx->gstack;
+      cfg.refill = ctx->refill;
       launch_trace_tap(ctx->scenes_dev + scene->slot, dr, n, any_hit, dp, dt, duv, dpp, dn, cfg, ctx->stream);
       hip_check(hipGetLastError(), "trace launch");
       hip_check(hipMemcpyAsync(prim, dp, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "prim");

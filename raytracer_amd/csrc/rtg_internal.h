@@ -127,6 +127,7 @@ struct DevPool {
 enum {
   C_EXT = 0, C_SHADE = 1, C_SHADOW = 5, C_ACTIVE = 6, C_NAN = 7, C_EXTRAYS = 8, C_SHRAYS = 10, C_DONE = 12,
   C_NODES = 14, C_PRIMS = 16, C_SHNODES = 18, C_SHPRIMS = 20,  // u64 work counters (counting mode)
+  C_XFETCH = 22, C_SFETCH = 23,  // batch cursors of the persistent extension / shadow traversals
   C_COUNT = 24
 };
 
@@ -172,11 +173,13 @@ struct DevGuide {
 };
 
 // Launch wrappers implemented in rtg_kernels.hip
+static const int kTraceRefill = 16;
 struct LaunchCfg {
   int grid;      // blocks for grid-stride kernels (traversal kernels use 2x grid blocks of 128)
   int block;     // threads per block
   int stack;     // traversal stack capacity (entries, >= tree height)
   uint2* gstack; // spill area: (stack - 16) entries x (2 * grid * 128) lanes
+  int refill = kTraceRefill;  // persistent traversal: refill a wave once this many lanes are idle
 };
 static const int kMaxStack = 64;
 // ev (optional): 8 events recorded before regen and after each of the 7 stage launches

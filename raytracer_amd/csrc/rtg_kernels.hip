@@ -123,6 +123,7 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, const in
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     // shade queues are filled by the classify stage of this iteration
     C[C_SHADE + 0] = 0; C[C_SHADE + 1] = 0; C[C_SHADE + 2] = 0; C[C_SHADE + 3] = 0;
+    C[C_XFETCH] = 0;  // extension-ray batches of this iteration
     if (GUIDED) *G.gq_count = 0;
   }
   for (int j = threadIdx.x; j < kMaxJobsDev; j += blockDim.x) s_done[j] = 0;
@@ -296,41 +297,123 @@ __device__ __forceinline__ void add_work(uint32_t* C, int idx, uint32_t nv, uint
   __syncthreads();
 }
 
+// Persistent traversal over a ray queue. A lane that finishes its ray takes the next entry of its
+// wave's current batch of 64 queue entries, and a wave takes a new batch (Src::batch) when it runs
+// out, so a wave keeps all its lanes traversing while rays are left instead of waiting for its
+// slowest lane before the next 64. Idle lanes are refilled once at least `refill` of them are idle
+// (or all are). Src: batch() -> first entry of a new batch or kNoBatch (wave-uniform);
+// load(k, ctx, state, geo) -> loads entry k and starts its traversal, false when it finished at
+// once (its result already stored); finish(ctx, state) stores a result.
+constexpr uint32_t kNoBatch = 0xffffffffu;
+constexpr uint32_t kTraceBatch = 64;
+
+template <bool ANY, class Src>
+__device__ __forceinline__ void trace_persistent(Src& src, uint32_t n, const TravStack& stk, int refill,
+                                                 uint32_t& nv, uint32_t& np) {
+  const uint32_t lane = __lane_id();
+  const uint64_t below = (1ull << lane) - 1ull;
+  const uint64_t wave_mask = __ballot(true);
+  bool busy = false;
+  TravState s;
+  SceneGeo geo;
+  typename Src::Ctx c;
+  uint32_t wnext = 0, wend = 0;  // the wave's current batch [wnext, wend)
+  bool more = true;              // batches may remain
+  while (true) {
+    const uint64_t idle = __ballot(!busy);
+    if (more && (idle == wave_mask || __popcll(idle) >= refill)) {
+      uint64_t pending = idle;
+      while (pending) {
+        if (wnext >= wend) {
+          const uint32_t b = src.batch();
+          if (b == kNoBatch || b >= n) {
+            more = false;
+            break;
+          }
+          wnext = b;
+          wend = min(b + kTraceBatch, n);
+        }
+        const uint32_t rank = (uint32_t)__popcll(pending & below);
+        const bool mine = ((pending >> lane) & 1ull) && rank < wend - wnext;
+        const uint64_t took = __ballot(mine);
+        if (mine) busy = src.load(wnext + rank, c, s, geo);
+        wnext += (uint32_t)__popcll(took);
+        pending &= ~took;
+      }
+    }
+    if (!__any(busy)) {
+      if (!more) break;
+      continue;
+    }
+    if (busy && trav_step<ANY>(geo, s, stk, nv, np)) {
+      src.finish(c, s);
+      busy = false;
+    }
+  }
+}
+
+// One wave-uniform atomic per batch on a queue's fetch counter.
+__device__ __forceinline__ uint32_t fetch_batch(uint32_t* ctr) {
+  uint32_t b = 0;
+  if (__lane_id() == (uint32_t)__builtin_amdgcn_readfirstlane(__lane_id())) b = atomicAdd(ctr, kTraceBatch);
+  return (uint32_t)__builtin_amdgcn_readfirstlane(b);
+}
+
+// Extension rays: closest hit, hit record and material class per queue entry.
+struct ExtSrc {
+  DevPool P;
+  const DevScene* scenes;
+  const DevJob* jobs;
+  struct Ctx {
+    uint32_t k, slot;
+    const DevScene* S;
+  };
+  __device__ __forceinline__ uint32_t batch() { return fetch_batch(&P.counters[C_XFETCH]); }
+  __device__ __forceinline__ bool load(uint32_t k, Ctx& c, TravState& s, SceneGeo& geo) {
+    c.k = k;
+    c.slot = P.q_ext[k];
+    const uint32_t f = P.flags[c.slot];
+    c.S = &scenes[jobs[fl_job(f)].scene];
+    const float4 o = P.ray_o[c.slot], d = P.ray_d[c.slot];
+    s.r.o = mk(o.x, o.y, o.z);
+    s.r.d = mk(d.x, d.y, d.z);
+    s.r.inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    s.r.tmin = o.w;
+    s.r.tmax = d.w;
+    if (!trav_begin(*c.S, s)) {
+      finish(c, s);
+      return false;
+    }
+    geo = scene_geo(*c.S);
+    return true;
+  }
+  __device__ __forceinline__ void finish(const Ctx& c, const TravState& s) {
+    P.hit[c.slot] = make_float4(s.h.t, s.h.b1, s.h.b2, __int_as_float(s.h.prim));
+    // material class for the shade queues: 0..2 BSDF type, 3 miss or no BSDF
+    uint8_t cls = 3;
+    if (s.h.prim >= 0) {
+      const uint32_t m = c.S->prim_info[s.h.prim].y;
+      if (m != 0xffffffffu) cls = (uint8_t)c.S->mats[m].type;
+    }
+    P.cls_q[c.k] = cls;
+  }
+};
+
 __global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scenes, const DevJob* jobs,
-                                                uint2* gstack, int stack_depth) {
+                                                uint2* gstack, int stack_depth, int refill) {
   extern __shared__ uint32_t smem[];
   __shared__ uint32_t s_w[4];
   const TravStack stk = make_stack(smem, gstack, stack_depth);
   uint32_t* C = P.counters;
   const uint32_t n = C[C_EXT];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    C[C_SHADOW] = 0;  // compact stage of this iteration fills it
+    C[C_SHADOW] = 0;   // compact stage of this iteration fills it
+    C[C_SFETCH] = 0;   // shadow-ray batches of this iteration
     atomicAdd((unsigned long long*)&C[C_EXTRAYS], (unsigned long long)n);
   }
-  const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t nv = 0, np = 0;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-    uint32_t slot = P.q_ext[k];
-    uint32_t f = P.flags[slot];
-    const DevScene& S = scenes[jobs[fl_job(f)].scene];
-    float4 o = P.ray_o[slot], d = P.ray_d[slot];
-    Ray r;
-    r.o = mk(o.x, o.y, o.z);
-    r.d = mk(d.x, d.y, d.z);
-    r.inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
-    r.tmin = o.w;
-    r.tmax = d.w;
-    HitRec h;
-    traverse<false>(S, r, stk, h, nv, np);
-    P.hit[slot] = make_float4(h.t, h.b1, h.b2, __int_as_float(h.prim));
-    // material class for the shade queues: 0..2 BSDF type, 3 miss or no BSDF
-    uint8_t cls = 3;
-    if (h.prim >= 0) {
-      uint32_t m = S.prim_info[h.prim].y;
-      if (m != 0xffffffffu) cls = (uint8_t)S.mats[m].type;
-    }
-    P.cls_q[k] = cls;
-  }
+  ExtSrc src{P, scenes, jobs};
+  trace_persistent<false>(src, n, stk, refill, nv, np);
   if (P.count_work) add_work(C, C_NODES, nv, np, s_w);
 }
 
@@ -665,8 +748,44 @@ __global__ void __launch_bounds__(256) k_compact_shadow(DevPool P) {
 }
 
 // ---------------------------------------------------------------- shadow rays (Integrator.cpp:228-235)
+// Shadow rays: any hit; an unoccluded ray adds its stored NEE contribution to the path.
+struct ShadowSrc {
+  DevPool P;
+  const DevScene* scenes;
+  const DevJob* jobs;
+  struct Ctx {
+    uint32_t slot;
+  };
+  __device__ __forceinline__ uint32_t batch() { return fetch_batch(&P.counters[C_SFETCH]); }
+  __device__ __forceinline__ bool load(uint32_t k, Ctx& c, TravState& s, SceneGeo& geo) {
+    c.slot = P.q_shadow[k];
+    const uint32_t f = P.flags[c.slot];
+    const DevScene& S = scenes[jobs[fl_job(f)].scene];
+    const float4 o = P.ray_o[c.slot], d = P.sh_d[c.slot];
+    s.r.o = mk(o.x, o.y, o.z);
+    s.r.d = mk(d.x, d.y, d.z);
+    s.r.inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
+    s.r.tmin = EPS;
+    s.r.tmax = d.w;
+    if (!trav_begin(S, s)) {
+      finish(c, s);
+      return false;
+    }
+    geo = scene_geo(S);
+    return true;
+  }
+  __device__ __forceinline__ void finish(const Ctx& c, const TravState& s) {
+    if (!s.hit) {
+      const float4 cc = P.sh_c[c.slot];
+      float4 L = P.L[c.slot];
+      L.x += cc.x; L.y += cc.y; L.z += cc.z;
+      P.L[c.slot] = L;
+    }
+  }
+};
+
 __global__ void __launch_bounds__(128) k_shadow(DevPool P, const DevScene* scenes, const DevJob* jobs,
-                                                uint2* gstack, int stack_depth) {
+                                                uint2* gstack, int stack_depth, int refill) {
   extern __shared__ uint32_t smem[];
   __shared__ uint32_t s_w[4];
   const TravStack stk = make_stack(smem, gstack, stack_depth);
@@ -676,28 +795,9 @@ __global__ void __launch_bounds__(128) k_shadow(DevPool P, const DevScene* scene
     C[C_EXT] = 0;  // regen of the next iteration fills it
     atomicAdd((unsigned long long*)&C[C_SHRAYS], (unsigned long long)n);
   }
-  const uint32_t stride = gridDim.x * blockDim.x;
   uint32_t nv = 0, np = 0;
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-    uint32_t slot = P.q_shadow[k];
-    uint32_t f = P.flags[slot];
-    const DevScene& S = scenes[jobs[fl_job(f)].scene];
-    float4 o = P.ray_o[slot], d = P.sh_d[slot];
-    Ray r;
-    r.o = mk(o.x, o.y, o.z);
-    r.d = mk(d.x, d.y, d.z);
-    r.inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
-    r.tmin = EPS;
-    r.tmax = d.w;
-    HitRec h;
-    bool occ = traverse<true>(S, r, stk, h, nv, np);
-    if (!occ) {
-      float4 c = P.sh_c[slot];
-      float4 L = P.L[slot];
-      L.x += c.x; L.y += c.y; L.z += c.z;
-      P.L[slot] = L;
-    }
-  }
+  ShadowSrc src{P, scenes, jobs};
+  trace_persistent<true>(src, n, stk, refill, nv, np);
   if (P.count_work) add_work(C, C_SHNODES, nv, np, s_w);
 }
 
@@ -713,33 +813,53 @@ __global__ void k_fill_zero(uint32_t* p, size_t n) {
 }
 
 // ---------------------------------------------------------------- parity taps
-__global__ void __launch_bounds__(128) k_trace_tap(const DevScene* scene, const float* rays, int n, int any_hit,
-                                                   int32_t* prim, float* t, float* uv, float* p, float* nrm,
-                                                   uint2* gstack, int stack_depth) {
-  extern __shared__ uint32_t smem[];
-  const TravStack stk = make_stack(smem, gstack, stack_depth);
-  const DevScene& S = *scene;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+// Rays from a host array (8 floats each), one scene; batches are taken in a fixed grid-stride order
+// (no counter), so the tap runs the same persistent traversal as the render kernels.
+struct TapSrc {
+  const DevScene* scene;
+  const float* rays;
+  int32_t *prim;
+  float *t, *uv, *p, *nrm;
+  bool any;
+  uint32_t next, stride;  // wave-uniform batch cursor
+  struct Ctx {
+    uint32_t i;
+  };
+  __device__ __forceinline__ uint32_t batch() {
+    const uint32_t b = next;
+    next += stride;
+    return b;
+  }
+  __device__ __forceinline__ bool load(uint32_t i, Ctx& c, TravState& s, SceneGeo& geo) {
+    c.i = i;
     const float* q = rays + 8 * (size_t)i;
-    Ray r;
-    r.o = mk(q[0], q[1], q[2]);
-    r.d = mk(q[3], q[4], q[5]);
-    r.inv = mk(1.f / q[3], 1.f / q[4], 1.f / q[5]);
-    r.tmin = q[6];
-    r.tmax = q[7];
-    HitRec h;
-    uint32_t nv = 0, np = 0;
-    bool hit = any_hit ? traverse<true>(S, r, stk, h, nv, np) : traverse<false>(S, r, stk, h, nv, np);
-    prim[i] = hit ? (int32_t)S.prim_info[h.prim].x : -1;
-    if (t) t[i] = hit ? h.t : 0.f;
+    s.r.o = mk(q[0], q[1], q[2]);
+    s.r.d = mk(q[3], q[4], q[5]);
+    s.r.inv = mk(1.f / q[3], 1.f / q[4], 1.f / q[5]);
+    s.r.tmin = q[6];
+    s.r.tmax = q[7];
+    if (!trav_begin(*scene, s)) {
+      finish(c, s);
+      return false;
+    }
+    geo = scene_geo(*scene);
+    return true;
+  }
+  __device__ __forceinline__ void finish(const Ctx& c, const TravState& s) {
+    const uint32_t i = c.i;
+    const DevScene& S = *scene;
+    const bool hit = s.hit;
+    prim[i] = hit ? (int32_t)S.prim_info[s.h.prim].x : -1;
+    if (t) t[i] = hit ? s.h.t : 0.f;
     if (uv) {
-      uv[2 * i] = hit ? h.b1 : 0.f;
-      uv[2 * i + 1] = hit ? h.b2 : 0.f;
+      uv[2 * i] = hit ? s.h.b1 : 0.f;
+      uv[2 * i + 1] = hit ? s.h.b2 : 0.f;
     }
     if (p || nrm) {
       f3 pp = mk(0.f, 0.f, 0.f), nn = mk(0.f, 0.f, 0.f);
-      if (hit && !any_hit) {
-        Surface sf = surface_at(S, h, r.o, r.d);
+      if (hit && !any) {
+        // surface_at takes the ray as given (adapt_eps only changes tmin)
+        Surface sf = surface_at(S, s.h, s.r.o, s.r.d);
         pp = sf.p;
         nn = sf.sh.n;
       }
@@ -747,6 +867,21 @@ __global__ void __launch_bounds__(128) k_trace_tap(const DevScene* scene, const 
       if (nrm) { nrm[3 * i] = nn.x; nrm[3 * i + 1] = nn.y; nrm[3 * i + 2] = nn.z; }
     }
   }
+};
+
+__global__ void __launch_bounds__(128) k_trace_tap(const DevScene* scene, const float* rays, int n, int any_hit,
+                                                   int32_t* prim, float* t, float* uv, float* p, float* nrm,
+                                                   uint2* gstack, int stack_depth, int refill) {
+  extern __shared__ uint32_t smem[];
+  const TravStack stk = make_stack(smem, gstack, stack_depth);
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
+  const uint32_t nwaves = (gridDim.x * blockDim.x) / 64u;
+  TapSrc src{scene, rays, prim, t, uv, p, nrm, any_hit != 0, wave * kTraceBatch, nwaves * kTraceBatch};
+  uint32_t nv = 0, np = 0;
+  if (any_hit)
+    trace_persistent<true>(src, (uint32_t)n, stk, refill, nv, np);
+  else
+    trace_persistent<false>(src, (uint32_t)n, stk, refill, nv, np);
 }
 
 __global__ void k_bsdf_tap(const DevMaterial* mat, int op, const float* wi, const float* wo_or_u, int n,
@@ -816,7 +951,7 @@ void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* j
     hipLaunchKernelGGL(k_regen<false>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, norder, G);
   if (ev) (void)hipEventRecord(ev[1], stream);
   hipLaunchKernelGGL(k_extend, dim3(g * 2), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
-                     cfg.gstack, cfg.stack);
+                     cfg.gstack, cfg.stack, cfg.refill);
   if (gd)
     hipLaunchKernelGGL(k_classify<true>, dim3(chunks), dim3(256), 0, stream, pool, G);
   else
@@ -845,7 +980,7 @@ void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* j
   if (ev) (void)hipEventRecord(ev[7], stream);
   hipLaunchKernelGGL(k_compact_shadow, dim3(chunks), dim3(256), 0, stream, pool);
   hipLaunchKernelGGL(k_shadow, dim3(g * 2), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
-                     cfg.gstack, cfg.stack);
+                     cfg.gstack, cfg.stack, cfg.refill);
   if (ev) (void)hipEventRecord(ev[8], stream);
 }
 
@@ -858,7 +993,7 @@ void launch_trace_tap(const DevScene* scene_dev, const float* rays, int n, int a
   if (n == 0) return;
   int blocks = std::min((n + 127) / 128, cfg.grid * 2);
   hipLaunchKernelGGL(k_trace_tap, dim3(blocks), dim3(128), stack_bytes(128), stream, scene_dev, rays, n, any_hit,
-                     prim, t, uv, p, nrm, cfg.gstack, cfg.stack);
+                     prim, t, uv, p, nrm, cfg.gstack, cfg.stack, cfg.refill);
 }
 
 void launch_bsdf_tap(const DevMaterial* mat, int op, const float* wi, const float* wo_or_u, int n, float* out_dir,
