@@ -22,7 +22,7 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes of the extension-traversal kernel
 RAY_IO_BYTES = 4 + 4 + 32 + 16 + 4   # queue entry, flags, ray (2 x float4), hit record, shade-queue entry
-NODE_BYTES = 64                      # one interior node = two 32-byte child slots
+NODE_BYTES = 112                     # one 4-wide node visit reads 6 box float4 + the 4 child refs
 TRI_BYTES = 36                       # three float3 vertices
 
 

@@ -1,5 +1,6 @@
 // Host BVH builder: binned SAH (16 bins per axis over centroid bounds, all three axes),
-// leaves of up to 16 primitives, flattened into child-pair nodes of two 32-byte slots.
+// leaves of up to 16 primitives, built as a binary tree and then collapsed into 4-wide nodes of
+// one 128-byte cache line (the child with the largest surface area is opened first).
 // The tree topology is this library's own; the reference builds a different tree
 // (accelStruct/Bvh.cpp:51-79,611-771). First hits agree up to exact/near ties because
 // both return the closest primitive under the same FP32 intersection arithmetic.
@@ -144,6 +145,76 @@ struct Builder {
   }
 };
 
+// ---------------------------------------------------------------- 4-wide collapse
+struct Child {
+  float mn[3], mx[3];
+  uint32_t ref;
+  float area() const {
+    float d0 = mx[0] - mn[0], d1 = mx[1] - mn[1], d2 = mx[2] - mn[2];
+    return 2.f * (d0 * d1 + d1 * d2 + d0 * d2);
+  }
+};
+
+struct Collapser {
+  const std::vector<float>& bin;  // 16 floats per binary node
+  std::vector<float> out;         // 32 floats per 4-wide node
+  int max_depth = 0;              // 4-wide levels
+  int max_stack = 0;              // most traversal stack entries any path can hold
+
+  explicit Collapser(const std::vector<float>& b) : bin(b) {}
+
+  void children(uint32_t ref, Child c[2]) const {
+    const float* nd = &bin[(size_t)ref * 16];
+    for (int k = 0; k < 2; k++) {
+      const float* sl = nd + 8 * k;
+      for (int a = 0; a < 3; a++) {
+        c[k].mn[a] = sl[a];
+        c[k].mx[a] = sl[3 + a];
+      }
+      std::memcpy(&c[k].ref, &sl[6], 4);
+    }
+  }
+
+  // 4-wide node over the binary interior node `ref`; `stk` = stack entries its ancestors can leave
+  uint32_t collapse(uint32_t ref, int depth, int stk) {
+    Child ch[4];
+    children(ref, ch);
+    int n = 2;
+    while (n < 4) {
+      int best = -1;
+      float ba = -1.f;
+      for (int i = 0; i < n; i++)
+        if (!(ch[i].ref & kLeafBit) && ch[i].area() > ba) {
+          ba = ch[i].area();
+          best = i;
+        }
+      if (best < 0) break;
+      Child sub[2];
+      children(ch[best].ref, sub);
+      ch[best] = sub[0];
+      ch[n++] = sub[1];
+    }
+    const uint32_t me = (uint32_t)(out.size() / 32);
+    out.resize(out.size() + 32, 0.f);
+    max_depth = std::max(max_depth, depth + 1);
+    // a visit pushes every hit child but the nearest
+    const int here = stk + (n - 1);
+    max_stack = std::max(max_stack, here);
+    uint32_t refs[4] = {kEmptyChild, kEmptyChild, kEmptyChild, kEmptyChild};
+    for (int i = 0; i < n; i++) refs[i] = (ch[i].ref & kLeafBit) ? ch[i].ref : collapse(ch[i].ref, depth + 1, here);
+    float* nd = &out[(size_t)me * 32];  // after the recursion: `out` may have grown
+    for (int j = 0; j < 4; j++) {
+      const bool v = j < n;
+      for (int a = 0; a < 3; a++) {
+        nd[4 * a + j] = v ? ch[j].mn[a] : 0.f;
+        nd[12 + 4 * a + j] = v ? ch[j].mx[a] : 0.f;
+      }
+      std::memcpy(&nd[24 + j], &refs[j], 4);
+    }
+    return me;
+  }
+};
+
 }  // namespace
 
 BvhResult build_bvh(const std::vector<BuildPrim>& tris, const std::vector<BuildPrim>& spheres) {
@@ -197,8 +268,17 @@ BvhResult build_bvh(const std::vector<BuildPrim>& tris, const std::vector<BuildP
   } else {
     res.root_ref = tris.empty() ? sphRef : triRef;
   }
-  res.nodes.swap(bt.nodes);
-  res.max_depth = depth;
+  (void)depth;
+  if (res.root_ref & kLeafBit) {
+    res.max_depth = 0;
+    res.max_stack = 0;
+  } else {
+    Collapser cl(bt.nodes);
+    res.root_ref = cl.collapse(res.root_ref, 0, 0);
+    res.nodes.swap(cl.out);
+    res.max_depth = cl.max_depth;
+    res.max_stack = cl.max_stack;
+  }
   return res;
 }
 

@@ -388,7 +388,7 @@ __device__ __forceinline__ void stk_top(const TravStack& s, int sp, uint32_t& re
   }
 }
 
-// BVH traversal over child-pair nodes, nearer child first. Closest hit keeps the reference's
+// BVH traversal over 4-wide nodes, nearest child first. Closest hit keeps the reference's
 // acceptance rules (triangles accept t == maxT, the last accepted primitive wins); ANY stops at
 // the first accepted primitive. The traversal is resumable one step at a time (TravState), so a
 // persistent kernel can give a lane a new ray as soon as its previous one is done.
@@ -432,38 +432,63 @@ __device__ __forceinline__ bool trav_begin(const DevScene& S, TravState& s) {
   return true;
 }
 
-// One step: an interior node (descend into the nearer hit child, push the farther one) or a leaf
+// One step: an interior node (descend into the nearest hit child, push the others) or a leaf
 // followed by the pops up to the next subtree that can still hold a closer hit. True when the
 // traversal has finished; the result is in s.hit / s.h.
 template <bool ANY>
 __device__ __forceinline__ bool trav_step(const SceneGeo& G, TravState& s, const TravStack& stk,
                                           uint32_t& nodes_visited, uint32_t& prims_tested) {
   if (!(s.ref & kLeafBit)) {
-    const float4* nd = G.nodes + 4u * s.ref;
-    float4 a0 = nd[0], a1 = nd[1], c0 = nd[2], c1 = nd[3];
+    const float4* nd = G.nodes + 8u * s.ref;
+    const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
+    const uint4 rf = *reinterpret_cast<const uint4*>(nd + 6);
     nodes_visited++;
-    float t0, t1;
-    bool h0, h1;
+    float t0, t1, t2, t3;
+    bool h0, h1, h2, h3;
     if (!s.zdir) {
-      h0 = slab_fast(s.r, a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, t0);
-      h1 = slab_fast(s.r, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, t1);
+      h0 = slab_fast(s.r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, t0);
+      h1 = slab_fast(s.r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, t1);
+      h2 = slab_fast(s.r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, t2);
+      h3 = slab_fast(s.r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, t3);
     } else {
-      h0 = slab(s.r, a0.x, a0.y, a0.z, a0.w, a1.x, a1.y, t0);
-      h1 = slab(s.r, c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, t1);
+      h0 = slab(s.r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, t0);
+      h1 = slab(s.r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, t1);
+      h2 = slab(s.r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, t2);
+      h3 = slab(s.r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, t3);
     }
-    uint32_t r0 = __float_as_uint(a1.z), r1 = __float_as_uint(c1.z);
-    if (h0 && h1) {
-      uint32_t nearR = r0, farR = r1;
-      float farT = t1;
-      if (t1 < t0) { nearR = r1; farR = r0; farT = t0; }
-      stk_push(stk, s.sp, farR, farT);
-      s.ref = nearR;
-      return false;
-    } else if (h0) {
-      s.ref = r0;
-      return false;
-    } else if (h1) {
-      s.ref = r1;
+    h0 = h0 && rf.x != kEmptyChild;
+    h1 = h1 && rf.y != kEmptyChild;
+    h2 = h2 && rf.z != kEmptyChild;
+    h3 = h3 && rf.w != kEmptyChild;
+    const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+    if (nh > 0) {
+      // order the children by entry distance, visit the nearest, push the rest farthest first so
+      // that the next nearest is popped next. Misses sort last (key +inf); a hit's key is always
+      // below +inf (a NaN or infinite entry distance sorts as the largest finite float), so the
+      // first nh children after the sort are exactly the hit ones.
+      auto key = [](bool h, float t) { return h ? (t < INFINITY ? t : 3.0e38f) : INFINITY; };
+      float k0 = key(h0, t0), k1 = key(h1, t1), k2 = key(h2, t2), k3 = key(h3, t3);
+      uint32_t c0 = rf.x, c1 = rf.y, c2 = rf.z, c3 = rf.w;
+#define RTG_CSWAP(ka, ca, kb, cb)          \
+  {                                        \
+    const bool sw = kb < ka;               \
+    const float kt = sw ? kb : ka;         \
+    kb = sw ? ka : kb;                     \
+    ka = kt;                               \
+    const uint32_t ct = sw ? cb : ca;      \
+    cb = sw ? ca : cb;                     \
+    ca = ct;                               \
+  }
+      RTG_CSWAP(k0, c0, k1, c1);
+      RTG_CSWAP(k2, c2, k3, c3);
+      RTG_CSWAP(k0, c0, k2, c2);
+      RTG_CSWAP(k1, c1, k3, c3);
+      RTG_CSWAP(k1, c1, k2, c2);
+#undef RTG_CSWAP
+      if (nh > 3) stk_push(stk, s.sp, c3, k3);
+      if (nh > 2) stk_push(stk, s.sp, c2, k2);
+      if (nh > 1) stk_push(stk, s.sp, c1, k1);
+      s.ref = c0;
       return false;
     }
   } else {
@@ -499,7 +524,7 @@ __device__ __forceinline__ bool trav_step(const SceneGeo& G, TravState& s, const
     float pt;
     uint32_t pr;
     stk_top(stk, s.sp, pr, pt);
-    if (s.r.tmax >= pt) {
+    if (s.r.tmax >= pt && pr != kEmptyChild) {
       s.ref = pr;
       return false;
     }

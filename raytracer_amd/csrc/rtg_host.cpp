@@ -337,7 +337,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   }
   const int ntri = (int)tprims.size(), nsph = (int)sprims.size();
   BvhResult bvh = build_bvh(tprims, sprims);
-  if (bvh.max_depth > 62) fail(RTG_ERR_INVALID, "BVH deeper than the traversal stack");
+  if (bvh.max_stack + 1 > kMaxStack) fail(RTG_ERR_INVALID, "BVH deeper than the traversal stack");
   // leaf-order tables
   std::vector<float> lv((size_t)ntri * 9);
   std::vector<uint4> info((size_t)ntri + nsph);
@@ -473,7 +473,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
             "scene table");
   hip_check(hipStreamSynchronize(ctx->stream), "scene upload");
   sc->bytes = bytes;
-  sc->stack = std::min(kMaxStack, std::max(16, bvh.max_depth + 2));
+  sc->stack = std::min(kMaxStack, std::max(16, bvh.max_stack + 1));
   sc->ntri = ntri;
   sc->nsph = nsph;
   ctx->scene_slots[slot] = sc.get();
@@ -531,6 +531,7 @@ int stack_for_resident(rtg_ctx* ctx) {
 LaunchCfg launch_cfg(rtg_ctx* ctx) {
   LaunchCfg cfg;
   cfg.grid = ctx->grid;
+  cfg.trace_blocks = ctx->trace_blocks;
   cfg.block = 256;
   cfg.stack = stack_for_resident(ctx);
   cfg.gstack = ctx->gstack;
@@ -627,10 +628,14 @@ int rtg_ctx_create(int device, rtg_ctx** out) {
     ctx->scene_slots.assign(kMaxScenes, nullptr);
     ctx->pool_slots = 1 << 21;
     ctx->grid = std::max(64, ctx->cu_count * 5);
-    // tuning knob for the persistent traversal (lanes idle before a wave refills, 1..64)
+    // tuning knobs of the persistent traversal: lanes idle before a wave refills (1..64), and
+    // waves per SIMD of a traversal launch (blocks of 128 = 2 waves; 4 SIMDs per CU)
     if (const char* e = std::getenv("RTG_TRACE_REFILL")) ctx->refill = std::min(64, std::max(1, std::atoi(e)));
-    // traversal stack spill area for every lane of a traversal launch (2 * grid blocks of 128)
-    size_t lanes = (size_t)ctx->grid * 2 * 128;
+    int waves = kTraceWaves;
+    if (const char* e = std::getenv("RTG_TRACE_WAVES")) waves = std::min(8, std::max(1, std::atoi(e)));
+    ctx->trace_blocks = std::max(32, ctx->cu_count * 2 * waves);
+    // traversal stack spill area for every lane of a traversal launch
+    size_t lanes = (size_t)ctx->trace_blocks * 128;
     hip_check(hipMalloc(&ctx->gstack, lanes * (kMaxStack - 16) * sizeof(uint2)), "hipMalloc stack spill");
   } catch (const RtgError& e) {
     int code = e.code;
@@ -935,6 +940,7 @@ int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int3
       hip_check(hipMemcpyAsync(dr, rays, (size_t)n * 32, hipMemcpyHostToDevice, ctx->stream), "rays");
       LaunchCfg cfg;
       cfg.grid = ctx->grid;
+      cfg.trace_blocks = ctx->trace_blocks;
       cfg.block = 128;
       cfg.stack = scene->stack;
       cfg.gstack = ctx->gstack;

@@ -73,6 +73,7 @@ struct rtg_ctx {
   rtg::DevJob* jobs_dev = nullptr;
   uint2* gstack = nullptr;
   int refill = rtg::kTraceRefill;
+  int trace_blocks = 0;
   int32_t* order_dev = nullptr;    // claim order of job-table slots
   rtg_server* server = nullptr;    // the serving loop owning the pool, if any
   rtg::GuideState* guide = nullptr;  // path-guiding network and buffers, created on first use

@@ -14,6 +14,10 @@ namespace rtg {
 static const uint32_t kLeafBit = 0x80000000u;
 static const int kMaxJobsDev = 256;  // job table entries (the flags word holds 8 bits of job slot)
 static const int kMaxLeafPrims = 16;
+// 4-wide interior node (128 bytes, 8 float4): min x, min y, min z, max x, max y, max z of the four
+// children (one float4 each, child j in component j), then the four child refs, then padding.
+// An unused child slot holds kEmptyChild.
+static const uint32_t kEmptyChild = 0x7FFFFFFFu;
 __host__ __device__ inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
   return kLeafBit | ((count - 1u) << 27) | first;
 }
@@ -46,7 +50,7 @@ struct DevLight {
 };
 
 struct DevScene {
-  const float4* nodes;       // 4 float4 per interior node
+  const float4* nodes;       // 8 float4 per 4-wide interior node
   uint32_t root_ref;         // ref of the root (interior index 0, or a leaf)
   int32_t empty;             // no primitives at all
   float root_min[3], root_max[3];
@@ -99,7 +103,8 @@ struct DevJob {
 //   bits 0..11 depth of the pending hit, bits 12..19 job slot, bit 20 LAST (only the emission
 //   of the pending segment remains), bit 21 EMISSION (depth-0 emission enabled)
 enum : uint32_t {
-  ST_FREE = 0u, ST_ACTIVE = 1u, ST_DONE = 2u
+  ST_FREE = 0u, ST_ACTIVE = 1u, ST_DONE = 2u,
+  ST_NONE = 0xffffffffu  // past the pool's end (never stored)
 };
 struct DevPool {
   int32_t capacity;
@@ -174,14 +179,16 @@ struct DevGuide {
 
 // Launch wrappers implemented in rtg_kernels.hip
 static const int kTraceRefill = 16;
+static const int kTraceWaves = 5;  // waves per SIMD of a traversal launch
 struct LaunchCfg {
-  int grid;      // blocks for grid-stride kernels (traversal kernels use 2x grid blocks of 128)
+  int grid;      // blocks for grid-stride kernels
+  int trace_blocks;  // blocks of 128 for the persistent traversal kernels
   int block;     // threads per block
   int stack;     // traversal stack capacity (entries, >= tree height)
-  uint2* gstack; // spill area: (stack - 16) entries x (2 * grid * 128) lanes
+  uint2* gstack; // spill area: (stack - 16) entries x (trace_blocks * 128) lanes
   int refill = kTraceRefill;  // persistent traversal: refill a wave once this many lanes are idle
 };
-static const int kMaxStack = 64;
+static const int kMaxStack = 96;
 // ev (optional): 8 events recorded before regen and after each of the 7 stage launches
 // order_dev: job-table slots in claim order (free path slots go to the first job with samples left)
 void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, const int32_t* order_dev,

@@ -112,7 +112,7 @@ __device__ __forceinline__ bool guided_vertex(int mt, uint32_t depth) {
 // training sample (position, normal, direction, luminance of the radiance gathered after the
 // vertex divided by the throughput before it) to the guide's ring; one 64-bit atomic per chunk.
 template <bool GUIDED>
-__global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, const int32_t* order, int norder,
+__global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const int32_t* order, int norder,
                                                DevGuide G) {
   __shared__ uint32_t s_w[4];
   __shared__ uint32_t s_base;
@@ -136,16 +136,31 @@ __global__ void __launch_bounds__(256) k_regen(DevPool P, DevJob* jobs, const in
     uint32_t need = 0;  // bit k: slot base + k*256 is free
     uint32_t act = 0;   // bit k: slot is active after this kernel
     uint32_t trn = 0;   // bit k: slot yields a guide training sample
-#pragma unroll 1
+    // the loads of all kRegenK slots are issued before any of them is used
+    uint32_t stv[kRegenK], flv[kRegenK];
+    float4 Lv[kRegenK];
+#pragma unroll
+    for (int k = 0; k < kRegenK; k++) {
+      const int i = base + k * 256;
+      stv[k] = i < cap ? P.state[i] : ST_NONE;
+    }
+#pragma unroll
+    for (int k = 0; k < kRegenK; k++) {
+      const int i = base + k * 256;
+      if (stv[k] == ST_DONE) {
+        flv[k] = P.flags[i];
+        Lv[k] = P.L[i];
+      }
+    }
+#pragma unroll
     for (int k = 0; k < kRegenK; k++) {
       int i = base + k * 256;
-      if (i >= cap) continue;
-      uint32_t st = P.state[i];
+      uint32_t st = stv[k];
       if (st == ST_DONE) {
-        uint32_t f = P.flags[i];
+        uint32_t f = flv[k];
         uint32_t jb = fl_job(f);
         const DevJob& J = jobs[jb];
-        float4 L = P.L[i];
+        float4 L = Lv[k];
         f3 l = mk(L.x, L.y, L.z);
         const bool bad = isnan(l.x) || isnan(l.y) || isnan(l.z);
         if (bad) {
@@ -950,7 +965,7 @@ void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* j
   else
     hipLaunchKernelGGL(k_regen<false>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, norder, G);
   if (ev) (void)hipEventRecord(ev[1], stream);
-  hipLaunchKernelGGL(k_extend, dim3(g * 2), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
+  hipLaunchKernelGGL(k_extend, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
                      cfg.gstack, cfg.stack, cfg.refill);
   if (gd)
     hipLaunchKernelGGL(k_classify<true>, dim3(chunks), dim3(256), 0, stream, pool, G);
@@ -979,7 +994,7 @@ void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* j
   }
   if (ev) (void)hipEventRecord(ev[7], stream);
   hipLaunchKernelGGL(k_compact_shadow, dim3(chunks), dim3(256), 0, stream, pool);
-  hipLaunchKernelGGL(k_shadow, dim3(g * 2), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
+  hipLaunchKernelGGL(k_shadow, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
                      cfg.gstack, cfg.stack, cfg.refill);
   if (ev) (void)hipEventRecord(ev[8], stream);
 }
@@ -991,7 +1006,7 @@ void launch_reset_pool(const DevPool& pool, hipStream_t stream) {
 void launch_trace_tap(const DevScene* scene_dev, const float* rays, int n, int any_hit, int32_t* prim, float* t,
                       float* uv, float* p, float* nrm, const LaunchCfg& cfg, hipStream_t stream) {
   if (n == 0) return;
-  int blocks = std::min((n + 127) / 128, cfg.grid * 2);
+  int blocks = std::min((n + 127) / 128, cfg.trace_blocks);
   hipLaunchKernelGGL(k_trace_tap, dim3(blocks), dim3(128), stack_bytes(128), stream, scene_dev, rays, n, any_hit,
                      prim, t, uv, p, nrm, cfg.gstack, cfg.stack, cfg.refill);
 }
