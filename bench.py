@@ -280,6 +280,18 @@ def main():
     ms_per_step = wall * 1e3 / a.steps
     value = samples / wall
 
+    # HBM bytes per extension-traversal dispatch from the committed rocprofv3 PMC summary of this
+    # workload (FETCH_SIZE and WRITE_SIZE, each scaled by what the counter reads for a copy of known
+    # size); null when the summary is absent or was taken on another workload
+    workload = f"displaced_sphere plain MIS path tracing {W}x{H} {rp['spp']}spp depth{rp['max_depth']}"
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "k_extend_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            tj = json.load(f)
+        if tj.get("workload") == workload:
+            traffic = float(tj["hbm_bytes_per_dispatch"])
+
     # roofline of the dominant kernel (extension traversal)
     ext_bytes = (counted["ext_rays"] * RAY_IO_BYTES + counted["nodes"] * NODE_BYTES
                  + counted["prims"] * TRI_BYTES)
@@ -300,7 +312,7 @@ def main():
         "dtype": "fp32",
         "data": "synthetic (procedural scene generated in-process)",
         "config": {
-            "workload": f"displaced_sphere plain MIS path tracing {W}x{H} {rp['spp']}spp depth{rp['max_depth']}",
+            "workload": workload,
             "triangles": sd.num_triangles(), "width": W, "height": H, "spp": rp["spp"],
             "max_depth": rp["max_depth"], "min_depth": rp["min_depth"], "sampler": "stratified",
         },
@@ -314,7 +326,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic,
             "bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": avg_launch_s * 1e3,
             "per_ray": {"nodes": counted["nodes"] / max(1, counted["ext_rays"]),

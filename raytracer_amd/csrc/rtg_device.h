@@ -286,23 +286,17 @@ __device__ __forceinline__ bool slab(const Ray& r, float bx0, float by0, float b
 // the flag is taken per axis so NaN distances fail exactly where slab() fails).
 __device__ __forceinline__ bool slab_fast(const Ray& r, float bx0, float by0, float bz0, float bx1, float by1,
                                           float bz1, float& nearT) {
-  float a1 = (bx0 - r.o.x) * r.inv.x, a2 = (bx1 - r.o.x) * r.inv.x;
-  float b1 = (by0 - r.o.y) * r.inv.y, b2 = (by1 - r.o.y) * r.inv.y;
-  float c1 = (bz0 - r.o.z) * r.inv.z, c2 = (bz1 - r.o.z) * r.inv.z;
-  bool sa = a1 > a2, sb = b1 > b2, sc = c1 > c2;
-  float alo = sa ? a2 : a1, ahi = sa ? a1 : a2;
-  float blo = sb ? b2 : b1, bhi = sb ? b1 : b2;
-  float clo = sc ? c2 : c1, chi = sc ? c1 : c2;
-  float nT = smax(alo, -MAX_REAL), fT = smin(ahi, MAX_REAL);
-  bool ok = nT <= fT;
-  nT = smax(blo, nT);
-  fT = smin(bhi, fT);
-  ok = ok && (nT <= fT);
-  nT = smax(clo, nT);
-  fT = smin(chi, fT);
-  ok = ok && (nT <= fT);
+  // The per-axis entry/exit distances as in core/BoundingBox.h:154-183; the reference's running
+  // clamps (nT <= fT checked after every axis) equal one check of the overall max/min, since nT
+  // only grows and fT only shrinks. Same result as slab() for finite distances (rays with a zero
+  // direction component take slab()).
+  const float a1 = (bx0 - r.o.x) * r.inv.x, a2 = (bx1 - r.o.x) * r.inv.x;
+  const float b1 = (by0 - r.o.y) * r.inv.y, b2 = (by1 - r.o.y) * r.inv.y;
+  const float c1 = (bz0 - r.o.z) * r.inv.z, c2 = (bz1 - r.o.z) * r.inv.z;
+  const float nT = fmaxf(fmaxf(fminf(a1, a2), fminf(b1, b2)), fmaxf(fminf(c1, c2), -MAX_REAL));
+  const float fT = fminf(fminf(fmaxf(a1, a2), fmaxf(b1, b2)), fminf(fmaxf(c1, c2), MAX_REAL));
   nearT = nT;
-  return ok && (r.tmin <= fT && r.tmax >= nT);
+  return nT <= fT && r.tmin <= fT && r.tmax >= nT;
 }
 
 // geometricShape/Triangle.cpp:46-89

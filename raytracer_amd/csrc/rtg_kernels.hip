@@ -97,7 +97,7 @@ __device__ __forceinline__ uint32_t block_reserve(uint32_t* counter, uint32_t to
 }
 
 // ---------------------------------------------------------------- regen
-// Each block owns chunks of kRegenSlots consecutive slots (K per thread, strided by the block size).
+// Each block owns chunks of kRegenSlots consecutive slots (kRegenK per thread, strided by the block size).
 // It finalises finished paths into their job's film, claims new samples for its free slots (one
 // 64-bit atomic per chunk and job) and appends its active slots to the extension queue (one atomic
 // per chunk).
@@ -117,6 +117,7 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
   __shared__ uint32_t s_w[4];
   __shared__ uint32_t s_base;
   __shared__ unsigned long long s_base64;
+  __shared__ uint32_t s_kc[kRegenK * 4];
   __shared__ uint32_t s_done[kMaxJobsDev];
   __shared__ uint32_t s_nan;
   uint32_t* C = P.counters;
@@ -132,8 +133,9 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
   const int cap = P.capacity;
   const int nchunks = (cap + kRegenSlots - 1) / kRegenSlots;
   for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    // slot base + k * 256 is this thread's k-th: every pool access of a wave is coalesced
     const int base = chunk * kRegenSlots + threadIdx.x;
-    uint32_t need = 0;  // bit k: slot base + k*256 is free
+    uint32_t need = 0;  // bit k: slot base + k * 256 is free
     uint32_t act = 0;   // bit k: slot is active after this kernel
     uint32_t trn = 0;   // bit k: slot yields a guide training sample
     // the loads of all kRegenK slots are issued before any of them is used
@@ -270,14 +272,32 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
         idx++;
       }
     }
-    // extension queue
-    uint32_t total;
-    uint32_t off = block_excl_scan<4, uint32_t>((uint32_t)__popc(act), s_w, total);
-    uint32_t qb = block_reserve(&C[C_EXT], total, &s_base);
-    uint32_t pos = qb + off;
+    // extension queue, in slot order: entry (k, thread) for slot base + k * 256, so that the
+    // stages reading the queue (kRegenK consecutive entries per thread) access the pool coalesced
+    {
+      const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+      const unsigned long long below = (1ull << lane) - 1ull;
+      uint32_t rank[kRegenK];
 #pragma unroll
-    for (int k = 0; k < kRegenK; k++)
-      if (act & (1u << k)) P.q_ext[pos++] = (uint32_t)(base + k * 256);
+      for (int k = 0; k < kRegenK; k++) {
+        const unsigned long long m = __ballot((act >> k) & 1u);
+        rank[k] = (uint32_t)__popcll(m & below);
+        if (lane == 0) s_kc[k * 4 + wave] = (uint32_t)__popcll(m);
+      }
+      __syncthreads();
+      uint32_t pre[kRegenK], run = 0;
+#pragma unroll
+      for (int k = 0; k < kRegenK; k++)
+#pragma unroll
+        for (uint32_t w = 0; w < 4; w++) {
+          if (w == wave) pre[k] = run;
+          run += s_kc[k * 4 + w];
+        }
+      const uint32_t qb = block_reserve(&C[C_EXT], run, &s_base);
+#pragma unroll
+      for (int k = 0; k < kRegenK; k++)
+        if (act & (1u << k)) P.q_ext[qb + pre[k] + rank[k]] = (uint32_t)(base + k * 256);
+    }
   }
   __syncthreads();
   for (int j = threadIdx.x; j < kMaxJobsDev; j += blockDim.x)
@@ -447,12 +467,12 @@ __global__ void __launch_bounds__(256) k_classify(DevPool P, DevGuide G) {
   const uint32_t n = C[C_EXT];
   const uint32_t nchunks = (n + kRegenSlots - 1) / kRegenSlots;
   for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
-    const uint32_t base = chunk * kRegenSlots + threadIdx.x;
+    const uint32_t base = chunk * kRegenSlots + threadIdx.x * kRegenK;  // kRegenK consecutive entries
     uint32_t cls[kRegenK];
     unsigned long long cnt = 0;
 #pragma unroll
     for (int k = 0; k < kRegenK; k++) {
-      uint32_t e = base + k * 256;
+      uint32_t e = base + k;
       cls[k] = e < n ? (uint32_t)P.cls_q[e] : 4u;
       if (cls[k] < 4) cnt += 1ull << (16 * cls[k]);
     }
@@ -471,7 +491,7 @@ __global__ void __launch_bounds__(256) k_classify(DevPool P, DevGuide G) {
     for (int k = 0; k < kRegenK; k++) {
       uint32_t c = cls[k];
       if (c < 4) {
-        uint32_t e = base + k * 256;
+        uint32_t e = base + k;
         P.q_shade[(size_t)c * P.capacity + pos[c]] = P.q_ext[e];
         pos[c]++;
       }
@@ -481,7 +501,7 @@ __global__ void __launch_bounds__(256) k_classify(DevPool P, DevGuide G) {
 #pragma unroll
       for (int k = 0; k < kRegenK; k++) {
         if (G.sample && cls[k] < 2) {
-          const uint32_t f = P.flags[P.q_ext[base + k * 256]];
+          const uint32_t f = P.flags[P.q_ext[base + k]];
           if (guided_vertex((int)cls[k], fl_depth(f)) && (int)fl_job(f) == G.job) gmask |= 1u << k;
         }
       }
@@ -491,7 +511,7 @@ __global__ void __launch_bounds__(256) k_classify(DevPool P, DevGuide G) {
       uint32_t gp = gb + goff;
 #pragma unroll
       for (int k = 0; k < kRegenK; k++)
-        if (gmask & (1u << k)) G.gq[gp++] = P.q_ext[base + k * 256];
+        if (gmask & (1u << k)) G.gq[gp++] = P.q_ext[base + k];
     }
   }
 }
@@ -741,12 +761,12 @@ __global__ void __launch_bounds__(256) k_compact_shadow(DevPool P) {
   const uint32_t n = C[C_EXT];
   const uint32_t nchunks = (n + kRegenSlots - 1) / kRegenSlots;
   for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
-    const uint32_t base = chunk * kRegenSlots + threadIdx.x;
+    const uint32_t base = chunk * kRegenSlots + threadIdx.x * kRegenK;  // kRegenK consecutive entries
     uint32_t bits = 0;
     uint32_t slots[kRegenK];
 #pragma unroll
     for (int k = 0; k < kRegenK; k++) {
-      uint32_t e = base + k * 256;
+      uint32_t e = base + k;
       slots[k] = 0;
       if (e < n && P.cls_q[e] < 3) {
         slots[k] = P.q_ext[e];
