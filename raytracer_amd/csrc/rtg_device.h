@@ -354,32 +354,44 @@ struct HitRec {
 // beyond the current closest hit is skipped without fetching its node.
 constexpr int kLdsStack = 16;
 struct TravStack {
-  uint32_t* lref;  // LDS refs, this lane's column
-  float* lt;       // LDS distances
+  uint2* l;        // LDS entries (ref, entry distance bits), this lane's column
   int lstride;     // threads per block
   uint2* g;        // global spill, this lane's column
   int gstride;     // threads in the grid
-  int cap;         // total entries (>= tree height, set by the host)
+  int cap;         // total entries (>= the deepest stack any ray needs, set by the host)
 };
 __device__ __forceinline__ void stk_push(const TravStack& s, int& sp, uint32_t ref, float t) {
   if (sp < kLdsStack) {
-    s.lref[sp * s.lstride] = ref;
-    s.lt[sp * s.lstride] = t;
+    s.l[sp * s.lstride] = make_uint2(ref, __float_as_uint(t));
     sp++;
   } else if (sp < s.cap) {
     s.g[(size_t)(sp - kLdsStack) * s.gstride] = make_uint2(ref, __float_as_uint(t));
     sp++;
   }
 }
-__device__ __forceinline__ void stk_top(const TravStack& s, int sp, uint32_t& ref, float& t) {
-  if (sp < kLdsStack) {
-    ref = s.lref[sp * s.lstride];
-    t = s.lt[sp * s.lstride];
+// Pushes the nh - 1 farther hit children of a node (sorted near to far: 1, 2, 3), farthest at the
+// bottom. With room for three entries in LDS it writes three slots unconditionally (slots above the
+// new top are free) instead of branching per entry.
+__device__ __forceinline__ void stk_push_children(const TravStack& s, int& sp, int nh, uint32_t c1, float k1,
+                                                  uint32_t c2, float k2, uint32_t c3, float k3) {
+  if (sp + 3 <= kLdsStack) {
+    const uint2 e1 = make_uint2(c1, __float_as_uint(k1));
+    const uint2 e2 = make_uint2(c2, __float_as_uint(k2));
+    const uint2 e3 = make_uint2(c3, __float_as_uint(k3));
+    s.l[sp * s.lstride] = nh == 4 ? e3 : (nh == 3 ? e2 : e1);
+    s.l[(sp + 1) * s.lstride] = nh == 4 ? e2 : e1;
+    s.l[(sp + 2) * s.lstride] = e1;
+    sp += nh - 1;
   } else {
-    uint2 e = s.g[(size_t)(sp - kLdsStack) * s.gstride];
-    ref = e.x;
-    t = __uint_as_float(e.y);
+    if (nh > 3) stk_push(s, sp, c3, k3);
+    if (nh > 2) stk_push(s, sp, c2, k2);
+    stk_push(s, sp, c1, k1);
   }
+}
+__device__ __forceinline__ void stk_top(const TravStack& s, int sp, uint32_t& ref, float& t) {
+  const uint2 e = sp < kLdsStack ? s.l[sp * s.lstride] : s.g[(size_t)(sp - kLdsStack) * s.gstride];
+  ref = e.x;
+  t = __uint_as_float(e.y);
 }
 
 // BVH traversal over 4-wide nodes, nearest child first. Closest hit keeps the reference's
@@ -426,43 +438,56 @@ __device__ __forceinline__ bool trav_begin(const DevScene& S, TravState& s) {
   return true;
 }
 
-// One step: an interior node (descend into the nearest hit child, push the others) or a leaf
-// followed by the pops up to the next subtree that can still hold a closer hit. True when the
-// traversal has finished; the result is in s.hit / s.h.
-template <bool ANY>
-__device__ __forceinline__ bool trav_step(const SceneGeo& G, TravState& s, const TravStack& stk,
-                                          uint32_t& nodes_visited, uint32_t& prims_tested) {
-  if (!(s.ref & kLeafBit)) {
-    const float4* nd = G.nodes + 8u * s.ref;
-    const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
-    const uint4 rf = *reinterpret_cast<const uint4*>(nd + 6);
-    nodes_visited++;
-    float t0, t1, t2, t3;
-    bool h0, h1, h2, h3;
-    if (!s.zdir) {
-      h0 = slab_fast(s.r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, t0);
-      h1 = slab_fast(s.r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, t1);
-      h2 = slab_fast(s.r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, t2);
-      h3 = slab_fast(s.r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, t3);
-    } else {
-      h0 = slab(s.r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, t0);
-      h1 = slab(s.r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, t1);
-      h2 = slab(s.r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, t2);
-      h3 = slab(s.r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, t3);
-    }
-    h0 = h0 && rf.x != kEmptyChild;
-    h1 = h1 && rf.y != kEmptyChild;
-    h2 = h2 && rf.z != kEmptyChild;
-    h3 = h3 && rf.w != kEmptyChild;
-    const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
-    if (nh > 0) {
-      // order the children by entry distance, visit the nearest, push the rest farthest first so
-      // that the next nearest is popped next. Misses sort last (key +inf); a hit's key is always
-      // below +inf (a NaN or infinite entry distance sorts as the largest finite float), so the
-      // first nh children after the sort are exactly the hit ones.
-      auto key = [](bool h, float t) { return h ? (t < INFINITY ? t : 3.0e38f) : INFINITY; };
-      float k0 = key(h0, t0), k1 = key(h1, t1), k2 = key(h2, t2), k3 = key(h3, t3);
-      uint32_t c0 = rf.x, c1 = rf.y, c2 = rf.z, c3 = rf.w;
+// Next subtree from the stack that can still hold a closer hit (its box starts at or before maxT),
+// or kTravDone when the stack is empty.
+constexpr uint32_t kTravDone = 0xFFFFFFFFu;
+__device__ __forceinline__ uint32_t trav_pop(const TravStack& stk, TravState& s) {
+  while (s.sp > 0) {
+    s.sp--;
+    float pt;
+    uint32_t pr;
+    stk_top(stk, s.sp, pr, pt);
+    if (s.r.tmax >= pt && pr != kEmptyChild) return pr;
+  }
+  return kTravDone;
+}
+
+// Visits the interior node s.ref: descends into the nearest hit child and pushes the other hit
+// children farthest first (so the next nearest is popped next), or pops when no child is hit.
+__device__ __forceinline__ void trav_node(const SceneGeo& G, TravState& s, const TravStack& stk,
+                                          uint32_t& nodes_visited) {
+  const float4* nd = G.nodes + 8u * s.ref;
+  const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
+  const uint4 rf = *reinterpret_cast<const uint4*>(nd + 6);
+  nodes_visited++;
+  float t0, t1, t2, t3;
+  bool h0, h1, h2, h3;
+  if (!s.zdir) {
+    h0 = slab_fast(s.r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, t0);
+    h1 = slab_fast(s.r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, t1);
+    h2 = slab_fast(s.r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, t2);
+    h3 = slab_fast(s.r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, t3);
+  } else {
+    h0 = slab(s.r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, t0);
+    h1 = slab(s.r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, t1);
+    h2 = slab(s.r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, t2);
+    h3 = slab(s.r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, t3);
+  }
+  h0 = h0 && rf.x != kEmptyChild;
+  h1 = h1 && rf.y != kEmptyChild;
+  h2 = h2 && rf.z != kEmptyChild;
+  h3 = h3 && rf.w != kEmptyChild;
+  const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
+  if (nh == 0) {
+    s.ref = trav_pop(stk, s);
+    return;
+  }
+  // Misses sort last (key +inf); a hit's key is always below +inf (a NaN or infinite entry
+  // distance sorts as the largest finite float), so the first nh children after the sort are
+  // exactly the hit ones.
+  auto key = [](bool h, float t) { return h ? (t < INFINITY ? t : 3.0e38f) : INFINITY; };
+  float k0 = key(h0, t0), k1 = key(h1, t1), k2 = key(h2, t2), k3 = key(h3, t3);
+  uint32_t c0 = rf.x, c1 = rf.y, c2 = rf.z, c3 = rf.w;
 #define RTG_CSWAP(ka, ca, kb, cb)          \
   {                                        \
     const bool sw = kb < ka;               \
@@ -473,56 +498,61 @@ __device__ __forceinline__ bool trav_step(const SceneGeo& G, TravState& s, const
     cb = sw ? ca : cb;                     \
     ca = ct;                               \
   }
-      RTG_CSWAP(k0, c0, k1, c1);
-      RTG_CSWAP(k2, c2, k3, c3);
-      RTG_CSWAP(k0, c0, k2, c2);
-      RTG_CSWAP(k1, c1, k3, c3);
-      RTG_CSWAP(k1, c1, k2, c2);
+  RTG_CSWAP(k0, c0, k1, c1);
+  RTG_CSWAP(k2, c2, k3, c3);
+  RTG_CSWAP(k0, c0, k2, c2);
+  RTG_CSWAP(k1, c1, k3, c3);
+  RTG_CSWAP(k1, c1, k2, c2);
 #undef RTG_CSWAP
-      if (nh > 3) stk_push(stk, s.sp, c3, k3);
-      if (nh > 2) stk_push(stk, s.sp, c2, k2);
-      if (nh > 1) stk_push(stk, s.sp, c1, k1);
-      s.ref = c0;
-      return false;
+  if (nh > 1) stk_push_children(stk, s.sp, nh, c1, k1, c2, k2, c3, k3);
+  s.ref = c0;
+}
+
+// Tests the primitives of leaf `ref`; true when ANY found an accepted primitive.
+template <bool ANY>
+__device__ __forceinline__ bool trav_leaf(const SceneGeo& G, TravState& s, uint32_t ref, uint32_t& prims_tested) {
+  const uint32_t first = leaf_first(ref), cnt = leaf_count(ref);
+  prims_tested += cnt;
+  if ((int32_t)first < G.ntri) {
+    for (uint32_t i = 0; i < cnt; i++) {
+      float t, b1, b2;
+      if (tri_hit(s.r, G.tri_v + 9u * (first + i), t, b1, b2)) {
+        s.h.t = t; s.h.b1 = b1; s.h.b2 = b2; s.h.prim = (int32_t)(first + i);
+        s.hit = true;
+        if (ANY) return true;
+        s.r.tmax = t;
+      }
     }
   } else {
-    uint32_t first = leaf_first(s.ref), cnt = leaf_count(s.ref);
-    prims_tested += cnt;
-    if ((int32_t)first < G.ntri) {
-      for (uint32_t i = 0; i < cnt; i++) {
-        float t, b1, b2;
-        if (tri_hit(s.r, G.tri_v + 9u * (first + i), t, b1, b2)) {
-          s.h.t = t; s.h.b1 = b1; s.h.b2 = b2; s.h.prim = (int32_t)(first + i);
-          s.hit = true;
-          if (ANY) return true;
-          s.r.tmax = t;
-        }
-      }
-    } else {
-      for (uint32_t i = 0; i < cnt; i++) {
-        float t;
-        uint32_t si = first + i - (uint32_t)G.ntri;
-        if (sphere_hit(s.r, G.spheres + 5u * si, t)) {
-          s.h.t = t; s.h.b1 = 0.f; s.h.b2 = 0.f; s.h.prim = (int32_t)(first + i);
-          s.hit = true;
-          if (ANY) return true;
-          s.r.tmax = t;
-        }
+    for (uint32_t i = 0; i < cnt; i++) {
+      float t;
+      uint32_t si = first + i - (uint32_t)G.ntri;
+      if (sphere_hit(s.r, G.spheres + 5u * si, t)) {
+        s.h.t = t; s.h.b1 = 0.f; s.h.b2 = 0.f; s.h.prim = (int32_t)(first + i);
+        s.hit = true;
+        if (ANY) return true;
+        s.r.tmax = t;
       }
     }
   }
-  // pop; a popped subtree whose box starts beyond the current maxT fails its box test
-  while (true) {
-    if (s.sp == 0) return true;
-    s.sp--;
-    float pt;
-    uint32_t pr;
-    stk_top(stk, s.sp, pr, pt);
-    if (s.r.tmax >= pt && pr != kEmptyChild) {
-      s.ref = pr;
-      return false;
-    }
+  return false;
+}
+
+// One step: an interior node, or a leaf followed by the pops up to the next subtree that can
+// still hold a closer hit. True when the traversal has finished; the result is in s.hit / s.h.
+// (Speculative while-while traversal, where lanes that reached a leaf keep visiting nodes until
+// every lane of the wave has one, measured slower here: it visits more nodes, and node fetches
+// are what bounds this traversal.)
+template <bool ANY>
+__device__ __forceinline__ bool trav_step(const SceneGeo& G, TravState& s, const TravStack& stk,
+                                          uint32_t& nodes_visited, uint32_t& prims_tested) {
+  if (!(s.ref & kLeafBit)) {
+    trav_node(G, s, stk, nodes_visited);
+  } else {
+    if (trav_leaf<ANY>(G, s, s.ref, prims_tested)) return true;
+    s.ref = trav_pop(stk, s);
   }
+  return s.ref == kTravDone;
 }
 
 template <bool ANY>

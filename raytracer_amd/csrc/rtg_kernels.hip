@@ -306,10 +306,9 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
 }
 
 // ---------------------------------------------------------------- extend: closest hit
-__device__ __forceinline__ TravStack make_stack(uint32_t* smem, uint2* gstack, int cap) {
+__device__ __forceinline__ TravStack make_stack(uint2* smem, uint2* gstack, int cap) {
   TravStack s;
-  s.lref = smem + threadIdx.x;
-  s.lt = reinterpret_cast<float*>(smem + (size_t)kLdsStack * blockDim.x) + threadIdx.x;
+  s.l = smem + threadIdx.x;
   s.lstride = blockDim.x;
   s.gstride = gridDim.x * blockDim.x;
   s.g = gstack + blockIdx.x * blockDim.x + threadIdx.x;
@@ -436,7 +435,7 @@ struct ExtSrc {
 
 __global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scenes, const DevJob* jobs,
                                                 uint2* gstack, int stack_depth, int refill) {
-  extern __shared__ uint32_t smem[];
+  extern __shared__ uint2 smem[];
   __shared__ uint32_t s_w[4];
   const TravStack stk = make_stack(smem, gstack, stack_depth);
   uint32_t* C = P.counters;
@@ -821,7 +820,7 @@ struct ShadowSrc {
 
 __global__ void __launch_bounds__(128) k_shadow(DevPool P, const DevScene* scenes, const DevJob* jobs,
                                                 uint2* gstack, int stack_depth, int refill) {
-  extern __shared__ uint32_t smem[];
+  extern __shared__ uint2 smem[];
   __shared__ uint32_t s_w[4];
   const TravStack stk = make_stack(smem, gstack, stack_depth);
   uint32_t* C = P.counters;
@@ -907,7 +906,7 @@ struct TapSrc {
 __global__ void __launch_bounds__(128) k_trace_tap(const DevScene* scene, const float* rays, int n, int any_hit,
                                                    int32_t* prim, float* t, float* uv, float* p, float* nrm,
                                                    uint2* gstack, int stack_depth, int refill) {
-  extern __shared__ uint32_t smem[];
+  extern __shared__ uint2 smem[];
   const TravStack stk = make_stack(smem, gstack, stack_depth);
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
   const uint32_t nwaves = (gridDim.x * blockDim.x) / 64u;

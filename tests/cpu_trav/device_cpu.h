@@ -23,3 +23,4 @@ template <class T, class U> inline T atomicAdd(T* p, U v) { T o = *p; *p += (T)v
 struct CpuThreadIdx { uint32_t x = 0, y = 0, z = 0; };
 static CpuThreadIdx threadIdx;
 inline void __syncthreads() {}
+inline bool __any(bool p) { return p; }

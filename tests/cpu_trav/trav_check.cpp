@@ -122,12 +122,10 @@ int main(int argc, char** argv) {
   for (int i = 0; i < ntri + nsph; i++) bad = bad || seen[i] != 1;
   if (bad) { printf("FAIL structure\n"); return 1; }
   const int cap = std::max(16, bvh.max_stack + 1);
-  std::vector<uint32_t> lref(kLdsStack);
-  std::vector<float> lt(kLdsStack);
+  std::vector<uint2> ls(kLdsStack);
   std::vector<uint2> gs(cap > kLdsStack ? cap - kLdsStack : 1);
   TravStack stk;
-  stk.lref = lref.data();
-  stk.lt = lt.data();
+  stk.l = ls.data();
   stk.lstride = 1;
   stk.g = gs.data();
   stk.gstride = 1;
