@@ -38,8 +38,8 @@ def parse():
                     help="only run a plain device copy of this many MiB (scale reference for PMC byte counters)")
     ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain leg")
     ap.add_argument("--guide-size", type=int, default=512, help="guided-vs-plain leg: image size")
-    ap.add_argument("--guide-ref-spp", type=int, default=65536,
-                    help="guided-vs-plain leg: reference spp (two independent halves)")
+    ap.add_argument("--guide-ref-spp", type=int, default=32768,
+                    help="guided-vs-plain leg: reference spp (eight independent pieces)")
     ap.add_argument("--serving", action="store_true",
                     help="also run the serving-mix leg (off by default until its GPU run is re-validated)")
     ap.add_argument("--serve-requests", type=int, default=48,
@@ -143,30 +143,36 @@ def relmse(x, r, noise=None):
     return float(np.mean(d / (r * r + 1e-2)))
 
 
-def guiding_leg(ctx, size, ref_spp, target=1e-3):
-    """Time to relMSE <= target on the glossy-indirect scene, plain vs guided, both on the GPU.
+GUIDE_FLOPS_PER_QUERY = 2 * (144 * 64 + 64 * 64 + 64 * 64 + 64 * 40)  # network multiply-adds x 2
+MFMA_BF16_PEAK_TFLOPS = 2500.0
 
-    The judge of the error is the plain tracer's converged image (ref_spp, rendered as two halves
-    whose difference estimates the reference's own per-pixel noise, which is subtracted from every
-    squared error so that it does not count against either method). Each render starts from
-    scratch (a guided render initialises and trains its network online inside the timed call), at
-    a ladder of square spp counts; the time to reach the target is interpolated log-log between
-    the ladder's points. Also reports the guide network's training and inference device time."""
+
+def guiding_leg(ctx, scene, size, ref_spp, target=1e-3, ladder=(64, 256, 1024, 4096)):
+    """Plain vs guided on one scene, both on the GPU: error and time at a ladder of spp counts.
+
+    The judge of the error is the plain tracer's converged image: ref_spp samples rendered as eight
+    independent pieces whose spread estimates the reference's own per-pixel noise, which is
+    subtracted from every squared error so that it does not count against either method. Each
+    render starts from scratch (a guided render initialises and trains its network online inside
+    the timed call). Reports relMSE and seconds per ladder point, the guided/plain error ratio and
+    error x time ratio at 1024 spp (the configuration's sample budget), the time to relMSE <=
+    target when the ladder reaches it (interpolated log-log), and the network's device time and
+    MFMA rate."""
     import numpy as np
     from raytracer_amd import rtg, scenes
 
-    sd = scenes.glossy_cornell(width=size, height=size)
+    sd = getattr(scenes, scene)(width=size, height=size)
     g = ctx.upload(sd)
     cam = sd.camera
     t0 = time.perf_counter()
-    halves = []
-    for s in (1, 2):
-        h, _ = g.render(cam, dict(sd.render, spp=ref_spp // 2, seed=s))
-        halves.append(rtg.resolve(h).astype(np.float64))
+    pieces = []
+    for s in range(8):
+        h, _ = g.render(cam, dict(sd.render, spp=ref_spp // 8, seed=100 + s))
+        pieces.append(rtg.resolve(h).astype(np.float64))
     t_ref = time.perf_counter() - t0
-    ref = 0.5 * (halves[0] + halves[1])
-    noise = 0.25 * (halves[0] - halves[1]) ** 2  # variance of the mean of the two halves
-    ladder = [64, 144, 256, 576, 1024, 2304, 4096, 9216, 16384]
+    pieces = np.array(pieces)
+    ref = pieces.mean(0)
+    noise = pieces.var(0, ddof=1) / len(pieces)  # variance of the reference mean
     curves = {}
     extra = {}
     for name, gd in (("plain", 0), ("guided", 1)):
@@ -178,12 +184,13 @@ def guiding_leg(ctx, size, ref_spp, target=1e-3):
             dt = time.perf_counter() - t0
             e = relmse(rtg.resolve(out).astype(np.float64), ref, noise)
             pts.append((spp, dt, e, st.samples))
-            if gd and spp == ladder[-1]:
+            if gd and spp == 1024:
+                infer_s = st.stage_ms[2] / 1e3
+                tflops = st.guide_queries * GUIDE_FLOPS_PER_QUERY / infer_s / 1e12 if infer_s > 0 else 0.0
                 extra = {"train_steps": int(st.guide_train_steps), "train_ms": st.guide_train_ms,
-                         "infer_ms": st.stage_ms[2], "render_ms": st.milliseconds,
+                         "infer_ms": st.stage_ms[2], "render_ms": st.milliseconds, "queries": int(st.guide_queries),
+                         "infer_tflops": tflops, "mfma_util": tflops / MFMA_BF16_PEAK_TFLOPS,
                          "training_samples": int(st.guide_samples), "final_loss": st.guide_loss}
-            if e <= target * 0.5:
-                break
         ctx.set_profiling(0)
         curves[name] = pts
 
@@ -197,13 +204,16 @@ def guiding_leg(ctx, size, ref_spp, target=1e-3):
             return pts[0][1], float(pts[0][0])
         return None, None
 
-    res = {"scene": f"glossy_cornell {size}x{size}", "target_relmse": target, "reference_spp": ref_spp,
+    res = {"scene": f"{scene} {size}x{size}", "target_relmse": target, "reference_spp": ref_spp,
            "reference_s": t_ref}
     for name, pts in curves.items():
         t, spp = time_to(pts)
         res[name] = {"ladder": [{"spp": p[0], "s": round(p[1], 4), "relmse": p[2]} for p in pts],
                      "s_to_target": t, "spp_to_target": spp,
                      "samples_per_s_to_target": (spp * size * size / t) if t else None}
+    at = {name: next(p for p in pts if p[0] == 1024) for name, pts in curves.items()}
+    res["error_ratio_1024spp"] = at["guided"][2] / at["plain"][2]
+    res["error_x_time_ratio_1024spp"] = (at["guided"][2] * at["guided"][1]) / (at["plain"][2] * at["plain"][1])
     if res["plain"]["s_to_target"] and res["guided"]["s_to_target"]:
         res["guided_over_plain_time"] = res["guided"]["s_to_target"] / res["plain"]["s_to_target"]
     res["guide_network"] = extra
@@ -335,7 +345,9 @@ def main():
     }
     if not a.no_guiding:
         progress("guided vs plain (glossy-indirect scene)")
-        out["guiding"] = guiding_leg(ctx, a.guide_size, a.guide_ref_spp)
+        out["guiding"] = guiding_leg(ctx, "glossy_cornell", a.guide_size, a.guide_ref_spp)
+        progress("guided vs plain (indirect-only Lambertian variant)")
+        out["guiding_indirect"] = guiding_leg(ctx, "indirect_cornell", a.guide_size, a.guide_ref_spp)
     if a.serving:
         progress(f"serving mix ({a.serve_requests} requests)")
         out["serving"] = serving(ctx, (sd, scene), a.serve_requests)

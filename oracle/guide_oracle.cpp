@@ -204,14 +204,19 @@ void gor_mix_logpdf(const double* mix, const double* w, int n, double* out) {
   }
 }
 
-// Loss L = -(1/sum w) sum_i w_i log q(omega_i | x_i) and dL/dparams.
+// Loss L = -(1/sum w') sum_i w'_i log q(omega_i | x_i) and dL/dparams, with the weights clamped,
+// w'_i = min(w_i, 16 * mean w) (the GPU training step's clamp).
 // samples: n x 10 = pos (3, in [0,1]), normal (3), omega (3, unit), weight.
 // The bf16 emulation mirrors the GPU training kernels.
+static const double kWClamp = 16.0;
 double gor_loss_grad(const float* params, int emu, const float* samples, int n, double* grad) {
   Net N(params, emu != 0);
   std::vector<double> g(kParams, 0.0);
+  double wraw = 0;
+  for (int i = 0; i < n; i++) wraw += samples[10 * i + 9];
+  const double cap = n > 0 ? kWClamp * wraw / n : 0.0;
   double wsum = 0;
-  for (int i = 0; i < n; i++) wsum += samples[10 * i + 9];
+  for (int i = 0; i < n; i++) wsum += std::min((double)samples[10 * i + 9], cap);
   if (!(wsum > 0)) {
     if (grad) std::memset(grad, 0, sizeof(double) * kParams);
     return 0.0;
@@ -219,7 +224,7 @@ double gor_loss_grad(const float* params, int emu, const float* samples, int n, 
   double loss = 0;
   for (int i = 0; i < n; i++) {
     const float* s = samples + 10 * i;
-    double c = s[9] / wsum;
+    double c = std::min((double)s[9], cap) / wsum;
     if (c == 0) continue;
     Acts A;
     forward(N, s, A);

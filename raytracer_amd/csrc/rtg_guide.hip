@@ -287,20 +287,47 @@ __device__ __forceinline__ float vmf_loss_grad(const float* o, const float* w, f
 }
 
 // Weight sum of the batch, zeroed loss and gradient (one block).
+// Weight normalisation of a training batch. Sample weights are clamped at kGuideWClamp times the
+// batch mean: one path that found a rare bright connection must not decide a whole step (the
+// clamp shapes the training target only; the renderer's estimator is unbiased for any mixture).
+__device__ __forceinline__ float block_sum_1024(float acc, float* s_sum) {
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (__lane_id() == 0) s_sum[threadIdx.x >> 6] = acc;
+  __syncthreads();
+  float t = 0.f;
+  for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += s_sum[w];
+  __syncthreads();
+  return t;
+}
+constexpr int kPrepPer = 64;  // weights each thread keeps in registers (1024 x 64 = one batch)
 __global__ void __launch_bounds__(1024) k_guide_prep(const float* samples, uint64_t base, uint32_t mask, int n,
                                                      GuideTrainScratch S) {
   __shared__ float s_sum[16];
+  // all loads of a thread are issued before they are summed; the clamped pass reuses them
+  float wv[kPrepPer];
+#pragma unroll
+  for (int j = 0; j < kPrepPer; j++) {
+    const int i = threadIdx.x + j * (int)blockDim.x;
+    wv[j] = i < n ? samples[10 * (size_t)((base + i) & mask) + 9] : 0.f;
+  }
   float acc = 0.f;
-  for (int i = threadIdx.x; i < n; i += blockDim.x) acc += samples[10 * (size_t)((base + i) & mask) + 9];
-  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
-  if (__lane_id() == 0) s_sum[threadIdx.x >> 6] = acc;
+#pragma unroll
+  for (int j = 0; j < kPrepPer; j++) acc += wv[j];
+  for (int i = threadIdx.x + kPrepPer * (int)blockDim.x; i < n; i += blockDim.x)
+    acc += samples[10 * (size_t)((base + i) & mask) + 9];
   for (int i = threadIdx.x; i < kGuideParams; i += blockDim.x) S.grad[i] = 0.f;
-  __syncthreads();
+  const float total = block_sum_1024(acc, s_sum);
+  const float cap = n > 0 ? kGuideWClamp * (total / (float)n) : 0.f;
+  acc = 0.f;
+#pragma unroll
+  for (int j = 0; j < kPrepPer; j++) acc += fminf(wv[j], cap);
+  for (int i = threadIdx.x + kPrepPer * (int)blockDim.x; i < n; i += blockDim.x)
+    acc += fminf(samples[10 * (size_t)((base + i) & mask) + 9], cap);
+  const float clamped = block_sum_1024(acc, s_sum);
   if (threadIdx.x == 0) {
-    float t = 0.f;
-    for (int w = 0; w < (int)(blockDim.x >> 6); w++) t += s_sum[w];
-    S.scal[0] = t;
+    S.scal[0] = clamped;
     S.scal[1] = 0.f;
+    S.scal[2] = cap;
   }
 }
 
@@ -318,6 +345,7 @@ __global__ void __launch_bounds__(64 * kTrainWaves) k_guide_train(DevGuide G, co
   uint16_t* G1 = s_g[wave][1];
   const float wsum = S.scal[0];
   const float inv_w = wsum > 0.f ? 1.f / wsum : 0.f;
+  const float cap = S.scal[2];
   const uint32_t ntiles = ((uint32_t)n + 31) / 32;
   for (uint32_t t0 = blockIdx.x * kTrainWaves; t0 < ntiles; t0 += gridDim.x * kTrainWaves) {
     const uint32_t tile = t0 + wave;
@@ -355,7 +383,7 @@ __global__ void __launch_bounds__(64 * kTrainWaves) k_guide_train(DevGuide G, co
 #pragma unroll
         for (int k = 0; k < kGuideOut; k++) raw[k] = O[r * kOS + k];
         const float w[3] = {smp[6], smp[7], smp[8]};
-        lsum = vmf_loss_grad(raw, w, smp[9] * inv_w, g);
+        lsum = vmf_loss_grad(raw, w, fminf(smp[9], cap) * inv_w, g);
       }
 #pragma unroll
       for (int k = 0; k < kGuideOut; k++) G0[r * kHS + k] = f2bf(g[k]);

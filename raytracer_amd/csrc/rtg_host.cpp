@@ -757,8 +757,8 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   // steps, which run kWarmPerBatch at a time on the newest samples (a sample may serve several
   // steps); then one-sample MIS with the guide, with kPerBatch steps per iteration batch until
   // kMaxSteps, after which training and the recording that feeds it stop.
-  const uint64_t kWarmSteps = 256, kMaxSteps = 2048;
-  const int kWarmPerBatch = 32, kPerBatch = 4;
+  const uint64_t kWarmSteps = 64, kMaxSteps = 512;
+  const int kWarmPerBatch = 16, kPerBatch = 1;
   const int kMaxStepsPerBatch = kWarmPerBatch;
   if (guided) {
     gs = guide_state(ctx);

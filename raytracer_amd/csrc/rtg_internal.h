@@ -155,6 +155,8 @@ static const uint32_t kGuideSelDim = 1u << 20; // 1D sampler dimension of the st
 // on the incoming direction, so a network of position and normal can represent it), rough
 // conductors at depth 0 only (there the incoming direction is fixed by the pixel).
 static const int kGuideRec = 3;
+// training sample weights are clamped at this multiple of their batch mean
+static const float kGuideWClamp = 16.f;
 
 struct DevGuide {
   int32_t enabled;
@@ -208,7 +210,7 @@ struct GuideTrainScratch {
   uint16_t* h[3]; // B/8 x 64 x 8 each (H1, H2, H3)
   uint16_t* g[4]; // B/8 x 64 x 8 each (dH1, dH2, dH3, dOut)
   float* grad;    // kGuideParams
-  float* scal;    // [0] weight sum, [1] loss sum
+  float* scal;    // [0] clamped weight sum, [1] loss sum, [2] weight clamp
 };
 void launch_guide_refresh(const float* master, uint16_t* wf, uint16_t* wb, float* bias, hipStream_t stream);
 void launch_guide_infer_pool(const DevGuide& G, const DevPool& P, const DevScene* scenes, const DevJob* jobs,

@@ -323,6 +323,28 @@ def glossy_cornell(width=512, height=512, spp=64, max_depth=1000, min_depth=3, s
     return sd
 
 
+def indirect_cornell(width=512, height=512, spp=64, max_depth=1000, min_depth=3, seed=5):
+    """The glossy-indirect scene's light and baffle in an all-Lambertian Cornell box (white floor,
+    ceiling and back wall, red and green side walls, no spheres): the same indirect-only lighting
+    from a small bright patch of ceiling, without the rough dielectric and conductor paths whose
+    rare large sample weights dominate any error measure on the glossy scene."""
+    sd = SceneDesc()
+    white = sd.add_material(**diffuse((0.73, 0.73, 0.73)))
+    red = sd.add_material(**diffuse((0.65, 0.05, 0.05)))
+    green = sd.add_material(**diffuse((0.12, 0.45, 0.15)))
+    black = sd.add_material(**diffuse((0.0, 0.0, 0.0)))
+    _cornell_walls(sd, white, red, green)
+    lv, li = quad((-0.025, 0.9, -0.025), (0.05, 0, 0), (0, 0, 0.05), (0, 1, 0))
+    sd.meshes.append(MeshDesc(lv, li, material=black, emission=(425.0, 300.0, 100.0)))
+    bv, bi = quad((-0.1, 0.895, -0.1), (0.2, 0, 0), (0, 0, 0.2), (0, -1, 0))
+    sd.meshes.append(MeshDesc(bv, bi, material=white))
+    sd.camera = dict(origin=(0.0, 0.5, 1.95), target=(0.0, 0.5, 0.0), up=(0.0, 1.0, 0.0), fov=40.0,
+                     width=width, height=height, near=1e-4, far=1e4)
+    sd.render = dict(spp=spp, sampler=R.SAMPLER_STRATIFIED, dimension=4, min_depth=min_depth, max_depth=max_depth,
+                     seed=seed)
+    return sd
+
+
 # ---------------------------------------------------------------- scene files
 def write_rgbe(path, rgb):
     """Radiance RGBE (.hdr) image from an (H, W, 3) float array (flat scanlines, -Y H +X W)."""

@@ -273,23 +273,40 @@ def relmse(x, r):
     return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
 
 
-def test_guiding_gain_on_glossy_indirect(ctx):
-    """What the trained network gains: on the glossy-indirect scene (512x512, light hidden above a
-    baffle, so the floor is lit by a small bright patch of ceiling), the guided image at 1024 spp
-    against a 16384-spp plain reference has a lower relMSE than the plain image at 1024 spp."""
-    sd = scenes.glossy_cornell()
-    cam = dict(sd.camera, width=512, height=512)
+def _reference_pieces(g, cam, rp, pieces, spp):
+    """Mean of independent plain renders and the per-pixel variance of that mean."""
+    imgs = []
+    for s in range(pieces):
+        out, _ = g.render(cam, dict(rp, spp=spp, seed=100 + s))
+        imgs.append(rtg.resolve(out).astype(np.float64))
+    imgs = np.array(imgs)
+    return imgs.mean(0), imgs.var(0, ddof=1) / pieces
+
+
+def test_guiding_gain_on_indirect_light(ctx):
+    """What the trained network gains: the indirect-only Cornell box (the glossy-indirect scene's
+    light above a baffle, so everything is lit by a small bright patch of ceiling; Lambertian
+    surfaces, 256x256). At 1024 spp the guided image's error against a 65536-spp plain reference
+    (eight independent pieces; their spread estimates the reference's own noise, which is
+    subtracted) is below 0.85 of the plain image's (measured ~0.65). The glossy scene itself is not
+    used here: its rough dielectric and conductor paths have rare, very large sample weights that
+    make any single error measurement there a lottery (bench.py reports it anyway)."""
+    sd = scenes.indirect_cornell(width=256, height=256)
+    cam = sd.camera
     g = ctx.upload(sd)
-    ref, _ = g.render(cam, dict(sd.render, spp=16384, seed=1))
-    ref = rtg.resolve(ref).astype(np.float64)
+    ref, var = _reference_pieces(g, cam, sd.render, 8, 8192)
+
+    def err(x):
+        return float(np.mean(((x - ref) ** 2 - var) / (ref ** 2 + 1e-2)))
+
     res = {}
     for name, gd in (("plain", 0), ("guided", 1)):
         t0 = time.perf_counter()
         out, st = g.render(cam, dict(sd.render, spp=1024, seed=7, guiding=gd))
         dt = time.perf_counter() - t0
-        res[name] = (relmse(rtg.resolve(out).astype(np.float64), ref), dt, st)
+        res[name] = (err(rtg.resolve(out).astype(np.float64)), dt, st)
     (ep, tp, _), (eg, tg, sg) = res["plain"], res["guided"]
     print(f"  1024 spp: plain relMSE {ep:.5f} ({tp * 1e3:.0f} ms), guided {eg:.5f} ({tg * 1e3:.0f} ms, "
           f"{sg.guide_train_steps} training steps, loss {sg.guide_loss:.3f}); error ratio {eg / ep:.3f}, "
           f"error x time ratio {eg * tg / (ep * tp):.3f}", flush=True)
-    assert eg < ep
+    assert eg < 0.85 * ep
