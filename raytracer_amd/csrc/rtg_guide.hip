@@ -116,8 +116,17 @@ __device__ __forceinline__ void encode_half(uint16_t* X, int r, int h, const flo
   }
 }
 
+// Every LDS tile here belongs to one wave, so ordering the wave's own LDS accesses is enough: a
+// wave's LDS instructions complete in issue order; this keeps the compiler from moving them.
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Forward through the four layers for the wave's tile; returns with the FP32 outputs in O
-// (which may alias X). HA/HB are scratch tiles; when keep != nullptr, H1..H3 go to keep[0..2].
+// (which may alias X). HA/HB are scratch tiles (HB may alias X, which is dead after layer 1);
+// when keep != nullptr, H1..H3 go to keep[0..2].
 __device__ __forceinline__ void forward_tile(const DevGuide& G, uint16_t* X, uint16_t* HA, uint16_t* HB, float* O,
                                              uint16_t* const* keep) {
   uint16_t* h1 = keep ? keep[0] : HA;
@@ -127,23 +136,23 @@ __device__ __forceinline__ void forward_tile(const DevGuide& G, uint16_t* X, uin
   gemm_tile<9>(X, kXS, G.wf + kGWf1, kGuideIn, c0, c1);
   store_relu(h1, c0, 0, G.bias + kGBias1);
   store_relu(h1, c1, 32, G.bias + kGBias1);
-  __syncthreads();
+  wave_sync();
   c0 = f32x16{}; c1 = f32x16{};
   gemm_tile<4>(h1, kHS, G.wf + kGWf2, kGuideHid, c0, c1);
   store_relu(h2, c0, 0, G.bias + kGBias2);
   store_relu(h2, c1, 32, G.bias + kGBias2);
-  __syncthreads();
+  wave_sync();
   c0 = f32x16{}; c1 = f32x16{};
   gemm_tile<4>(h2, kHS, G.wf + kGWf3, kGuideHid, c0, c1);
-  __syncthreads();  // h3 may alias h1 (inference)
+  wave_sync();  // h3 may alias h1 (inference)
   store_relu(h3, c0, 0, G.bias + kGBias3);
   store_relu(h3, c1, 32, G.bias + kGBias3);
-  __syncthreads();
+  wave_sync();
   c0 = f32x16{}; c1 = f32x16{};
   gemm_tile<4>(h3, kHS, G.wf + kGWf4, kGuideHid, c0, c1);
   store_out(O, c0, 0, G.bias + kGBias4);
   store_out(O, c1, 32, G.bias + kGBias4);
-  __syncthreads();
+  wave_sync();
 }
 
 // Mixture parameters (pi, mu xyz, kappa) x 8 from the 40 raw outputs of one row.
@@ -170,16 +179,17 @@ __device__ __forceinline__ void mixture_from_raw(const float* o, float* mix) {
 // ---------------------------------------------------------------- inference
 // MODE 0: rows are the guide queue's pool slots (hit surface -> inputs, mixture -> G.mix[slot]).
 // MODE 1: rows are in6[] inputs (position in [0,1]^3, normal), mixture -> out[row].
-constexpr int kInferWaves = 4;
+constexpr int kInferWaves = 2;  // 14 KB of LDS per wave: 10 waves per CU
 template <int MODE>
 __global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(DevGuide G, DevPool P, const DevScene* scenes,
                                                                    const DevJob* jobs, const float* in6, int n_arr,
                                                                    float* out) {
   __shared__ __attribute__((aligned(16))) uint16_t s_x[kInferWaves][32 * kXS];
-  __shared__ __attribute__((aligned(16))) uint16_t s_h[kInferWaves][2][32 * kHS];
+  __shared__ __attribute__((aligned(16))) uint16_t s_h[kInferWaves][32 * kHS];
   const int wave = threadIdx.x >> 6, l = __lane_id(), r = l & 31, h = l >> 5;
   uint16_t* X = s_x[wave];
   float* O = reinterpret_cast<float*>(X);
+  static_assert(32 * kOS * 4 <= 32 * kXS * 2 && 32 * kHS <= 32 * kXS, "outputs and H2 alias the feature tile");
   const uint32_t n = MODE == 0 ? *G.gq_count : (uint32_t)n_arr;
   if (MODE == 0 && blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(G.infer_count, (unsigned long long)n);
   const uint32_t ntiles = (n + 31) / 32;
@@ -216,8 +226,8 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(DevGuide G, De
       }
     }
     encode_half(X, r, h, v);
-    __syncthreads();
-    forward_tile(G, X, s_h[wave][0], s_h[wave][1], O, nullptr);
+    wave_sync();
+    forward_tile(G, X, s_h[wave], X, O, nullptr);
     if (valid && h == 0) {
       float raw[kGuideOut], mix[kGuideOut];
 #pragma unroll
@@ -228,7 +238,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(DevGuide G, De
 #pragma unroll
       for (int k = 0; k < kGuideOut / 4; k++) d4[k] = make_float4(mix[4 * k], mix[4 * k + 1], mix[4 * k + 2], mix[4 * k + 3]);
     }
-    __syncthreads();
+    wave_sync();
   }
 }
 
