@@ -33,7 +33,16 @@ enum {
 };
 
 /* Material types: bsdf/Diffuse.cpp, bsdf/RoughConductor.cpp, bsdf/RoughDielectric.cpp */
-enum { RTG_MAT_DIFFUSE = 0, RTG_MAT_ROUGH_CONDUCTOR = 1, RTG_MAT_ROUGH_DIELECTRIC = 2 };
+/* Diffuse (bsdf/Diffuse.cpp), RoughConductor (bsdf/RoughConductor.cpp), RoughDielectric
+ * (bsdf/RoughDielectric.cpp), and the smooth (delta) Conductor (bsdf/Conductor.cpp:33-77) and
+ * Dielectric (bsdf/Dielectric.cpp:28-110). */
+enum {
+  RTG_MAT_DIFFUSE = 0,
+  RTG_MAT_ROUGH_CONDUCTOR = 1,
+  RTG_MAT_ROUGH_DIELECTRIC = 2,
+  RTG_MAT_CONDUCTOR = 3,
+  RTG_MAT_DIELECTRIC = 4
+};
 /* Microfacet distributions: bsdf/MicrofacetDistribution.h:8-12 */
 enum { RTG_DIST_BECKMANN = 0, RTG_DIST_GGX = 2 };
 /* Light selection strategies: integrator/Integrator.h:18-22 */
@@ -196,7 +205,9 @@ int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int3
  * op 1: sample from wi with u (n x 2) -> wo (n x 3), f = weight (n x 3), pdf (n), rel_eta (n).
  * op 2 (diffuse and rough conductor; used by guided sampling, no reference counterpart): for
  *   (wi, wo) the density with which op 1 produces wo -> pdf, the weight op 1 returns for wo -> f,
- *   the pdf op 1 reports for wo -> rel_eta. */
+ *   the pdf op 1 reports for wo -> rel_eta.
+ * Smooth conductors and dielectrics: op 0 gives f = 0 and pdf = 0 (their eval and pdf in the
+ * solid-angle measure), op 1 the mirror / refracted direction with pdf 1, fr or 1 - fr. */
 int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const float* wi, const float* wo_or_u,
                    int32_t n, float* out_dir, float* f, float* pdf, float* rel_eta);
 

@@ -554,7 +554,9 @@ static inline float adjustedAlpha(float alpha, float cosThetaI) {
 }
 
 // ---------------------------------------------------------------- BSDFs
-enum { MAT_DIFFUSE = 0, MAT_CONDUCTOR = 1, MAT_DIELECTRIC = 2 };
+// rough conductor / dielectric (RoughConductor, RoughDielectric) and the smooth, delta ones
+// (Conductor, Dielectric); the values are rtg.h's RTG_MAT_*
+enum { MAT_DIFFUSE = 0, MAT_CONDUCTOR = 1, MAT_DIELECTRIC = 2, MAT_SMOOTH_CONDUCTOR = 3, MAT_SMOOTH_DIELECTRIC = 4 };
 // bsdf/BSDF.h:24-40
 enum : uint32_t {
     BT_DELTA_REFLECTION = 0x2, BT_GLOSSY_REFLECTION = 0x4, BT_DELTA_TRANSMISSION = 0x8,
@@ -728,10 +730,45 @@ static C3 dielectricSample(const Material& m, BSDFInfo& in, V2 s) {  // :194-258
     return dielectricEvalRefraction(m, in) * std::fabs(cosThetaO) / in.pdf;
 }
 
+// ---- Conductor (bsdf/Conductor.cpp:26-77). The integrator's infos carry the solid-angle measure
+// (BSDF.h:95), so eval and pdf are 0 there; sample returns the mirror direction, pdf 1.
+static C3 smoothConductorEval(const Material&, const BSDFInfo&) { return c3(0.f); }
+static float smoothConductorPdf(const Material&, const BSDFInfo&) { return 0.f; }
+static C3 smoothConductorSample(const Material& m, BSDFInfo& in) {
+    if (in.wi.z <= 0.f) return c3(0.f);
+    in.sampledType = BT_DELTA_REFLECTION;
+    in.wo = reflectLocal(in.wi);
+    in.pdf = 1.f;
+    float cosI = in.wi.z;
+    return m.reflectance * fresnelConductor(m.eta, m.k, cosI);
+}
+// ---- Dielectric (bsdf/Dielectric.cpp:28-104): eval and pdf 0; sample reflects with probability
+// fr (weight: reflectance), else refracts (weight: transmitted * relEta^2); relativeEta is set
+// for both outcomes.
+static C3 smoothDielectricEval(const Material&, const BSDFInfo&) { return c3(0.f); }
+static float smoothDielectricPdf(const Material&, const BSDFInfo&) { return 0.f; }
+static C3 smoothDielectricSample(const Material& m, BSDFInfo& in, V2 s) {
+    float etaI, etaT, rel, cosT;
+    float fr = estimateDielectric(m.etaExt, m.etaInt, in.wi.z, etaI, etaT, rel, cosT);
+    in.relativeEta = rel;
+    if (s.x <= fr) {
+        in.sampledType = BT_DELTA_REFLECTION;
+        in.wo = reflectLocal(in.wi);
+        in.pdf = fr;
+        return m.reflectance;
+    }
+    in.sampledType = BT_DELTA_TRANSMISSION;
+    in.wo = v3(rel * -in.wi.x, rel * -in.wi.y, cosT);  // BSDF.h:51
+    in.pdf = 1.f - fr;
+    return m.transmitted * rel * rel;
+}
+
 static C3 bsdfEval(const Material& m, const BSDFInfo& in) {
     switch (m.type) {
         case MAT_DIFFUSE: return diffuseEval(m, in);
         case MAT_CONDUCTOR: return conductorEval(m, in);
+        case MAT_SMOOTH_CONDUCTOR: return smoothConductorEval(m, in);
+        case MAT_SMOOTH_DIELECTRIC: return smoothDielectricEval(m, in);
         default: return dielectricEval(m, in);
     }
 }
@@ -739,6 +776,8 @@ static float bsdfPdf(const Material& m, const BSDFInfo& in) {
     switch (m.type) {
         case MAT_DIFFUSE: return diffusePdf(m, in);
         case MAT_CONDUCTOR: return conductorPdf(m, in);
+        case MAT_SMOOTH_CONDUCTOR: return smoothConductorPdf(m, in);
+        case MAT_SMOOTH_DIELECTRIC: return smoothDielectricPdf(m, in);
         default: return dielectricPdf(m, in);
     }
 }
@@ -746,6 +785,8 @@ static C3 bsdfSample(const Material& m, BSDFInfo& in, V2 s) {
     switch (m.type) {
         case MAT_DIFFUSE: return diffuseSample(m, in, s);
         case MAT_CONDUCTOR: return conductorSample(m, in, s);
+        case MAT_SMOOTH_CONDUCTOR: return smoothConductorSample(m, in);
+        case MAT_SMOOTH_DIELECTRIC: return smoothDielectricSample(m, in, s);
         default: return dielectricSample(m, in, s);
     }
 }

@@ -869,14 +869,46 @@ __device__ __forceinline__ bool dielectric_sample(const DevMaterial& m, f3 wi, f
   return true;
 }
 
+// bsdf/Conductor.cpp:50-64: mirror reflection, weight R * Fresnel(cos_i), pdf 1
+__device__ __forceinline__ bool smooth_conductor_sample(const DevMaterial& m, f3 wi, BsdfSample& bs) {
+  if (wi.z <= 0.f) return false;
+  bs.wo = mk(-wi.x, -wi.y, wi.z);
+  bs.pdf = 1.f;
+  bs.rel_eta = 1.f;
+  bs.f = cmul(mat_col(m.albedo), fresnel_conductor(mat_col(m.eta), mat_col(m.k), wi.z));
+  return true;
+}
+// bsdf/Dielectric.cpp:37-95: reflect with probability fr (weight R), else refract without
+// renormalising (weight T * eta_rel^2); the relative IOR is reported for both branches.
+__device__ __forceinline__ bool smooth_dielectric_sample(const DevMaterial& m, f3 wi, float ux, BsdfSample& bs) {
+  float etaI, etaT, rel, cosT;
+  const float fr = est_dielectric(m.eta_ext, m.eta_int, wi.z, etaI, etaT, rel, cosT);
+  bs.rel_eta = rel;
+  if (ux <= fr) {
+    bs.wo = mk(-wi.x, -wi.y, wi.z);
+    bs.pdf = fr;
+    bs.f = mat_col(m.albedo);
+  } else {
+    bs.wo = mk(rel * -wi.x, rel * -wi.y, cosT);
+    bs.pdf = 1.f - fr;
+    bs.f = mat_col(m.trans) * (rel * rel);
+  }
+  return true;
+}
+// Smooth conductors and dielectrics share the shade kernels (and template MT) of their rough
+// kind; in the solid-angle measure their eval and pdf are 0 (Conductor.cpp:26-35, 68-77,
+// Dielectric.cpp:28-35, 98-104), so next-event estimation from them contributes nothing.
+__device__ __forceinline__ bool is_delta(const DevMaterial& m) { return m.type >= MAT_CONDUCTOR; }
 template <int MT>
 __device__ __forceinline__ f3 bsdf_eval(const DevMaterial& m, f3 wi, f3 wo) {
+  if (MT != 0 && is_delta(m)) return mk(0.f, 0.f, 0.f);
   if (MT == 0) return diffuse_eval(m, wi, wo);
   if (MT == 1) return conductor_eval(m, wi, wo);
   return dielectric_eval(m, wi, wo);
 }
 template <int MT>
 __device__ __forceinline__ float bsdf_pdf(const DevMaterial& m, f3 wi, f3 wo) {
+  if (MT != 0 && is_delta(m)) return 0.f;
   if (MT == 0) return diffuse_pdf(wi, wo);
   if (MT == 1) return conductor_pdf(m, wi, wo);
   return dielectric_pdf(m, wi, wo);
@@ -884,8 +916,8 @@ __device__ __forceinline__ float bsdf_pdf(const DevMaterial& m, f3 wi, f3 wo) {
 template <int MT>
 __device__ __forceinline__ bool bsdf_sample(const DevMaterial& m, f3 wi, float ux, float uy, BsdfSample& bs) {
   if (MT == 0) return diffuse_sample(m, wi, ux, uy, bs);
-  if (MT == 1) return conductor_sample(m, wi, ux, uy, bs);
-  return dielectric_sample(m, wi, ux, uy, bs);
+  if (MT == 1) return m.type == MAT_CONDUCTOR ? smooth_conductor_sample(m, wi, bs) : conductor_sample(m, wi, ux, uy, bs);
+  return m.type == MAT_DIELECTRIC ? smooth_dielectric_sample(m, wi, ux, bs) : dielectric_sample(m, wi, ux, uy, bs);
 }
 
 // ---------------------------------------------------------------- guiding: one-sample MIS support
@@ -898,7 +930,7 @@ template <int MT>
 __device__ __forceinline__ bool bsdf_density(const DevMaterial& m, f3 wi, f3 wo, f3& weight, float& ps, float& pref,
                                              float& rel) {
   rel = 1.f;
-  if (wi.z <= 0.f || wo.z <= 0.f) return false;
+  if (wi.z <= 0.f || wo.z <= 0.f || is_delta(m)) return false;
   if (MT == 0) {
     ps = wo.z * INV_PI;
     pref = diffuse_pdf(wi, wo);

@@ -72,6 +72,33 @@ static inline float avg3(float a) {  // Color(a).average()
   return s / 3.f;
 }
 
+static_assert(MAT_CONDUCTOR == RTG_MAT_CONDUCTOR && MAT_DIELECTRIC == RTG_MAT_DIELECTRIC &&
+                  MAT_ROUGH_DIELECTRIC == RTG_MAT_ROUGH_DIELECTRIC,
+              "device material types follow rtg.h");
+// rtg_material -> the device record (texture/ConstantTexture.h; bsdf constructors). Smooth
+// conductors and dielectrics are shaded by the rough kernels of their kind (shade class 1, 2).
+DevMaterial dev_material(const rtg_material& m) {
+  if (m.type < RTG_MAT_DIFFUSE || m.type > RTG_MAT_DIELECTRIC) fail(RTG_ERR_INVALID, "unknown material type");
+  if ((m.type == RTG_MAT_ROUGH_CONDUCTOR || m.type == RTG_MAT_ROUGH_DIELECTRIC) &&
+      m.distribution != RTG_DIST_GGX && m.distribution != RTG_DIST_BECKMANN)
+    fail(RTG_ERR_INVALID, "distribution must be ggx or beckmann");
+  DevMaterial d;
+  std::memset(&d, 0, sizeof(d));
+  d.type = m.type;
+  d.cls = m.type == RTG_MAT_CONDUCTOR ? 1 : (m.type == RTG_MAT_DIELECTRIC ? 2 : m.type);
+  d.dist = m.distribution;
+  d.alpha = avg3(m.alpha);
+  d.eta_int = m.eta_int;
+  d.eta_ext = m.eta_ext;
+  for (int c = 0; c < 3; c++) {
+    d.albedo[c] = m.albedo[c];
+    d.eta[c] = m.eta[c];
+    d.k[c] = m.k[c];
+    d.trans[c] = m.transmitted[c];
+  }
+  return d;
+}
+
 // 4x4 inverse in double, rounded to float (row-major)
 static void inverse44(const float* a, float* out) {
   double m[4][8];
@@ -194,26 +221,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   const int nmat = desc->num_materials;
   // materials (texture/ConstantTexture.h; bsdf constructors)
   std::vector<DevMaterial> mats(nmat);
-  for (int i = 0; i < nmat; i++) {
-    const rtg_material& m = desc->materials[i];
-    if (m.type < 0 || m.type > 2) fail(RTG_ERR_INVALID, "unknown material type");
-    if (m.distribution != RTG_DIST_GGX && m.distribution != RTG_DIST_BECKMANN)
-      fail(RTG_ERR_INVALID, "distribution must be ggx or beckmann");
-    DevMaterial d;
-    std::memset(&d, 0, sizeof(d));
-    d.type = m.type;
-    d.dist = m.distribution;
-    d.alpha = avg3(m.alpha);
-    d.eta_int = m.eta_int;
-    d.eta_ext = m.eta_ext;
-    for (int c = 0; c < 3; c++) {
-      d.albedo[c] = m.albedo[c];
-      d.eta[c] = m.eta[c];
-      d.k[c] = m.k[c];
-      d.trans[c] = m.transmitted[c];
-    }
-    mats[i] = d;
-  }
+  for (int i = 0; i < nmat; i++) mats[i] = dev_material(desc->materials[i]);
   // triangles in global id order
   std::vector<BuildPrim> tprims;
   std::vector<float> tv;           // 9 per triangle, global order
@@ -969,25 +977,11 @@ int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const
     if (!material || !wi || !wo_or_u || !f || !pdf || n < 0 || op < 0 || op > 2)
       fail(RTG_ERR_INVALID, "bad bsdf arguments");
     if (op == 1 && (!out_dir || !rel_eta)) fail(RTG_ERR_INVALID, "sample needs out_dir and rel_eta");
-    if (op == 2 && (!rel_eta || material->type == RTG_MAT_ROUGH_DIELECTRIC))
+    if (op == 2 && (!rel_eta || (material->type != RTG_MAT_DIFFUSE && material->type != RTG_MAT_ROUGH_CONDUCTOR)))
       fail(RTG_ERR_INVALID, "sampling density needs rel_eta and a diffuse or conductor material");
     if (n == 0) return RTG_OK;
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
-    const rtg_material& m = *material;
-    if (m.type < 0 || m.type > 2) fail(RTG_ERR_INVALID, "unknown material type");
-    DevMaterial d;
-    std::memset(&d, 0, sizeof(d));
-    d.type = m.type;
-    d.dist = m.distribution;
-    d.alpha = avg3(m.alpha);
-    d.eta_int = m.eta_int;
-    d.eta_ext = m.eta_ext;
-    for (int c = 0; c < 3; c++) {
-      d.albedo[c] = m.albedo[c];
-      d.eta[c] = m.eta[c];
-      d.k[c] = m.k[c];
-      d.trans[c] = m.transmitted[c];
-    }
+    const DevMaterial d = dev_material(*material);
     std::vector<void*> own;
     DevMaterial* dm = (DevMaterial*)dev_alloc(ctx, sizeof(DevMaterial), own);
     float* dwi = (float*)dev_alloc(ctx, (size_t)n * 12, own);

@@ -7,16 +7,14 @@
 namespace rtg {
 
 // ---------------------------------------------------------------- BVH layout
-// Interior node = two 32-byte child slots (64 bytes, one node pair per load):
-//   slot: float bmin[3], float bmax[3], uint32 ref, uint32 pad
-// A child ref with bit 31 set is a leaf: bits 0..26 first primitive (leaf order),
-// bits 27..30 count-1 (1..16 primitives). Otherwise the ref is an interior node index.
+// 4-wide interior node (128 bytes, 8 float4): min x, min y, min z, max x, max y, max z of the four
+// children (one float4 each, child j in component j), then the four child refs, then padding.
+// An unused child slot holds kEmptyChild. A child ref with bit 31 set is a leaf: bits 0..26
+// first primitive (leaf order), bits 27..30 count-1 (1..16 primitives). Otherwise the ref is an
+// interior node index.
 static const uint32_t kLeafBit = 0x80000000u;
 static const int kMaxJobsDev = 256;  // job table entries (the flags word holds 8 bits of job slot)
 static const int kMaxLeafPrims = 16;
-// 4-wide interior node (128 bytes, 8 float4): min x, min y, min z, max x, max y, max z of the four
-// children (one float4 each, child j in component j), then the four child refs, then padding.
-// An unused child slot holds kEmptyChild.
 static const uint32_t kEmptyChild = 0x7FFFFFFFu;
 __host__ __device__ inline uint32_t make_leaf_ref(uint32_t first, uint32_t count) {
   return kLeafBit | ((count - 1u) << 27) | first;
@@ -24,13 +22,17 @@ __host__ __device__ inline uint32_t make_leaf_ref(uint32_t first, uint32_t count
 __host__ __device__ inline uint32_t leaf_first(uint32_t ref) { return ref & 0x07FFFFFFu; }
 __host__ __device__ inline uint32_t leaf_count(uint32_t ref) { return ((ref >> 27) & 0xFu) + 1u; }
 
+// Material types (the RTG_MAT_* values of rtg.h)
+enum { MAT_DIFFUSE = 0, MAT_ROUGH_CONDUCTOR = 1, MAT_ROUGH_DIELECTRIC = 2, MAT_CONDUCTOR = 3, MAT_DIELECTRIC = 4 };
+// type: MAT_*; cls: the shade kernel and queue (0 diffuse, 1 conductors, 2 dielectrics)
 struct DevMaterial {
   int32_t type;
   int32_t dist;
   float alpha;     // Color(alpha).average(), as the reference evaluates the texture
   float eta_int;
   float eta_ext;
-  float pad0, pad1, pad2;
+  int32_t cls;
+  float pad1, pad2;
   float albedo[4]; // Diffuse albedo or reflectance texture
   float eta[4];
   float k[4];

@@ -427,7 +427,7 @@ struct ExtSrc {
     uint8_t cls = 3;
     if (s.h.prim >= 0) {
       const uint32_t m = c.S->prim_info[s.h.prim].y;
-      if (m != 0xffffffffu) cls = (uint8_t)c.S->mats[m].type;
+      if (m != 0xffffffffu) cls = (uint8_t)c.S->mats[m].cls;
     }
     P.cls_q[c.k] = cls;
   }
@@ -526,7 +526,9 @@ __device__ __forceinline__ float power_heuristic(float a, float b) {  // Integra
 __device__ __forceinline__ void prev_segment(const DevScene& S, f3 Le, float pdfLight, bool caught, float4& T,
                                              const float4& F, float4& L) {
   if (caught) {
-    float w = power_heuristic(F.w, pdfLight);
+    // a delta bounce (F.w < 0) cannot be reached by light sampling: light pdf 0, weight 1
+    // (PathTracingMIS.cpp:205)
+    float w = F.w < 0.f ? power_heuristic(-F.w, 0.f) : power_heuristic(F.w, pdfLight);
     f3 c = cmul(cmul(mk(T.x, T.y, T.z), Le), mk(F.x, F.y, F.z)) * w;
     L.x += c.x; L.y += c.y; L.z += c.z;
   }
@@ -597,8 +599,9 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
         const DevMaterial& M = S.mats[sf.material];
         PathRng rng = make_rng(J.seed, P.pixel[slot], P.sample[slot], J);
         f3 wo_l = to_local(sf.sh, -d);
-        // one-light next-event estimation (Integrator.cpp:176-238, PathTracingMIS.cpp:119-153)
-        if (S.nlights > 0) {
+        // one-light next-event estimation (Integrator.cpp:176-238, PathTracingMIS.cpp:119-153);
+        // from a smooth conductor or dielectric it contributes 0 (eval is 0), so it is skipped
+        if (S.nlights > 0 && !is_delta(M)) {
           float ux, uy;
           sample2d(rng, 1u + 2u * depth, ux, uy);
           float lw;
@@ -628,7 +631,7 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
         sample2d(rng, 2u + 2u * depth, bx, by);
         BsdfSample bs;
         bool ok;
-        if (GUIDED && guided_vertex(MT, depth) && (int)fl_job(f) == G.job) {
+        if (GUIDED && guided_vertex(MT, depth) && (int)fl_job(f) == G.job && !is_delta(M)) {
           // one-sample MIS of BSDF sampling and the guide, each chosen with probability 1/2:
           // weight = (sampler weight * sampler density) / mixture density, so the estimator keeps
           // the plain tracer's expectation; the MIS weights of the next hit use the BSDF's pdf.
@@ -677,7 +680,7 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
         } else {
           f3 nd = to_world(sf.sh, bs.wo);
           P.ray_d[slot] = make_float4(nd.x, nd.y, nd.z, MAX_REAL);
-          F = make_float4(bs.f.x, bs.f.y, bs.f.z, bs.pdf);
+          F = make_float4(bs.f.x, bs.f.y, bs.f.z, is_delta(M) ? -bs.pdf : bs.pdf);
           float eta = T.w * bs.rel_eta;
           float q = 1.f;
           nf = (f & ~(DEPTH_MASK | FL_EMIT)) | (depth + 1u);
@@ -928,8 +931,8 @@ __global__ void k_bsdf_tap(const DevMaterial* mat, int op, const float* wi, cons
     f3 b = ld3(wo_or_u + 3 * (size_t)i);
     f3 e;
     float pd;
-    if (M.type == 0) { e = bsdf_eval<0>(M, a, b); pd = bsdf_pdf<0>(M, a, b); }
-    else if (M.type == 1) { e = bsdf_eval<1>(M, a, b); pd = bsdf_pdf<1>(M, a, b); }
+    if (M.cls == 0) { e = bsdf_eval<0>(M, a, b); pd = bsdf_pdf<0>(M, a, b); }
+    else if (M.cls == 1) { e = bsdf_eval<1>(M, a, b); pd = bsdf_pdf<1>(M, a, b); }
     else { e = bsdf_eval<2>(M, a, b); pd = bsdf_pdf<2>(M, a, b); }
     f[3 * i] = e.x; f[3 * i + 1] = e.y; f[3 * i + 2] = e.z;
     pdf[i] = pd;
@@ -938,7 +941,7 @@ __global__ void k_bsdf_tap(const DevMaterial* mat, int op, const float* wi, cons
     f3 b = ld3(wo_or_u + 3 * (size_t)i);
     f3 w = mk(0.f, 0.f, 0.f);
     float ps = 0.f, pref = 0.f, rel = 1.f;
-    bool ok = M.type == 0 ? bsdf_density<0>(M, a, b, w, ps, pref, rel) : bsdf_density<1>(M, a, b, w, ps, pref, rel);
+    bool ok = M.cls == 0 ? bsdf_density<0>(M, a, b, w, ps, pref, rel) : bsdf_density<1>(M, a, b, w, ps, pref, rel);
     if (!ok) { w = mk(0.f, 0.f, 0.f); ps = 0.f; pref = 0.f; }
     f[3 * i] = w.x; f[3 * i + 1] = w.y; f[3 * i + 2] = w.z;
     pdf[i] = ps;
@@ -951,8 +954,8 @@ __global__ void k_bsdf_tap(const DevMaterial* mat, int op, const float* wi, cons
     bs.pdf = -1.f;
     bs.rel_eta = 1.f;
     bool ok;
-    if (M.type == 0) ok = bsdf_sample<0>(M, a, ux, uy, bs);
-    else if (M.type == 1) ok = bsdf_sample<1>(M, a, ux, uy, bs);
+    if (M.cls == 0) ok = bsdf_sample<0>(M, a, ux, uy, bs);
+    else if (M.cls == 1) ok = bsdf_sample<1>(M, a, ux, uy, bs);
     else ok = bsdf_sample<2>(M, a, ux, uy, bs);
     if (!ok) bs.f = mk(0.f, 0.f, 0.f);
     out_dir[3 * i] = bs.wo.x; out_dir[3 * i + 1] = bs.wo.y; out_dir[3 * i + 2] = bs.wo.z;

@@ -1,8 +1,8 @@
 // Scene files in the reference's XML schema (core/XMLLoader.cpp:33-904), for the types the GPU
 // path renders: PathTracingMIS, RandomSampler / StratifiedSampler, NoriPerspective with the box
-// filter, Mesh (OBJ, io/ObjLoader.cpp:97-289) / Sphere / Cube shapes, Diffuse / RoughConductor /
-// RoughDielectric BSDFs with constant parameters, area lights on meshes, and an environment light
-// from a Radiance RGBE (.hdr) image. Anything else fails with RTG_ERR_PARSE and names the element.
+// filter, Mesh (OBJ, io/ObjLoader.cpp:97-289) / Sphere / Cube / Plane shapes, Diffuse /
+// RoughConductor / RoughDielectric / Conductor / Dielectric BSDFs with constant parameters, area
+// lights on meshes, and an environment light from a Radiance RGBE (.hdr) image. Anything else fails with RTG_ERR_PARSE and names the element.
 #include <algorithm>
 #include <cctype>
 #include <cmath>
@@ -471,6 +471,20 @@ struct Loader {
       P.color("eta", zero, m.eta);  // tools/Common.cpp:56-80 fallback when no material name
       P.color("absorption", zero, m.k);
       P.color("reflectanceTexture", one, m.albedo);
+    } else if (lt == "conductor") {
+      m.type = RTG_MAT_CONDUCTOR;  // bsdf/Conductor.cpp:12-19
+      if (P.has("material") && !P.str("material", "").empty())
+        perr("bsdf Conductor: named material '" + P.str("material", "") +
+             "' needs spectral IOR data; give 'eta' and 'absorption' colors");
+      P.color("eta", zero, m.eta);
+      P.color("absorption", zero, m.k);
+      P.color("reflectanceTexture", one, m.albedo);
+    } else if (lt == "dielectric") {
+      m.type = RTG_MAT_DIELECTRIC;  // bsdf/Dielectric.cpp:10-17
+      m.eta_ext = (float)P.real("etaExt", 1.000277);
+      m.eta_int = (float)P.real("etaInt", 1.5046);
+      P.color("reflectanceTexture", one, m.albedo);
+      P.color("transmittedTexture", one, m.transmitted);
     } else if (lt == "roughdielectric") {
       m.type = RTG_MAT_ROUGH_DIELECTRIC;  // bsdf/RoughDielectric.cpp:13-27
       m.alpha = (float)P.real("alpha", 0.2);

@@ -13,7 +13,7 @@ LIB_PATH = os.path.join(_HERE, "librtg.so")
 RTG_OK = 0
 ERRORS = {-1: "INVALID", -2: "HIP", -3: "OOM", -4: "PARSE", -5: "NOT_READY", -6: "CANCELLED"}
 
-MAT_DIFFUSE, MAT_ROUGH_CONDUCTOR, MAT_ROUGH_DIELECTRIC = 0, 1, 2
+MAT_DIFFUSE, MAT_ROUGH_CONDUCTOR, MAT_ROUGH_DIELECTRIC, MAT_CONDUCTOR, MAT_DIELECTRIC = 0, 1, 2, 3, 4
 DIST_BECKMANN, DIST_GGX = 0, 2
 LIGHTS_ALL, LIGHTS_UNIFORM, LIGHTS_WEIGHTED = 0, 1, 2
 SAMPLER_RANDOM, SAMPLER_STRATIFIED = 0, 1

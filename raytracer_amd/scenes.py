@@ -78,6 +78,17 @@ def dielectric(alpha, eta_int=1.5, eta_ext=1.000277, dist=R.DIST_GGX):
                 albedo=(1.0, 1.0, 1.0), transmitted=(1.0, 1.0, 1.0))
 
 
+def smooth_conductor(eta=GOLD_ETA, k=GOLD_K, reflectance=(1.0, 1.0, 1.0)):
+    """Perfectly specular conductor (bsdf/Conductor.cpp)."""
+    return dict(type=R.MAT_CONDUCTOR, eta=tuple(eta), k=tuple(k), albedo=tuple(reflectance))
+
+
+def smooth_dielectric(eta_int=1.5046, eta_ext=1.000277, reflectance=(1.0, 1.0, 1.0), transmitted=(1.0, 1.0, 1.0)):
+    """Perfectly specular dielectric (bsdf/Dielectric.cpp)."""
+    return dict(type=R.MAT_DIELECTRIC, eta_int=eta_int, eta_ext=eta_ext, albedo=tuple(reflectance),
+                transmitted=tuple(transmitted))
+
+
 def quad(corner, e1, e2, normal):
     """Two triangles spanning corner + [0,1]e1 + [0,1]e2, wound so (p1-p0)x(p2-p0) follows `normal`."""
     c = np.asarray(corner, np.float64); a = np.asarray(e1, np.float64); b = np.asarray(e2, np.float64)
@@ -323,6 +334,18 @@ def glossy_cornell(width=512, height=512, spp=64, max_depth=1000, min_depth=3, s
     return sd
 
 
+def specular_cornell(width=256, height=256, spp=64, max_depth=16, min_depth=3, seed=6):
+    """Cornell box with perfectly specular surfaces: the tall box a smooth gold mirror, a smooth
+    glass sphere (IOR 1.5) in front of the short box, the ceiling light of cornell()."""
+    sd = cornell(width=width, height=height, spp=spp, max_depth=max_depth, min_depth=min_depth, seed=seed)
+    mirror = sd.add_material(**smooth_conductor())
+    glass = sd.add_material(**smooth_dielectric(eta_int=1.5))
+    sd.meshes[2].material = mirror   # the tall box
+    M = np.eye(4, dtype=np.float32); M[:3, 3] = (0.2, 0.43, 0.15)
+    sd.spheres.append(SphereDesc(M, 0.12, glass))
+    return sd
+
+
 def indirect_cornell(width=512, height=512, spp=64, max_depth=1000, min_depth=3, seed=5):
     """The glossy-indirect scene's light and baffle in an all-Lambertian Cornell box (white floor,
     ceiling and back wall, red and green side walls, no spheres): the same indirect-only lighting
@@ -409,6 +432,14 @@ def write_xml(sd, directory, name="scene.xml"):
             return [f'{ind}  <texture name="{name}" type="ConstantTexture"><color name="albedo" value="{_fmt(c)}"/>'
                     "</texture>"]
 
+        if m["type"] == R.MAT_CONDUCTOR:
+            return ([f'{ind}<bsdf type="Conductor">', f'{ind}  <color name="eta" value="{_fmt(m["eta"])}"/>',
+                     f'{ind}  <color name="absorption" value="{_fmt(m["k"])}"/>']
+                    + tex("reflectanceTexture", "albedo") + [f"{ind}</bsdf>"])
+        if m["type"] == R.MAT_DIELECTRIC:
+            return ([f'{ind}<bsdf type="Dielectric">', f'{ind}  <double name="etaInt" value="{float(m["eta_int"])!r}"/>',
+                     f'{ind}  <double name="etaExt" value="{float(m["eta_ext"])!r}"/>']
+                    + tex("reflectanceTexture", "albedo") + tex("transmittedTexture", "transmitted") + [f"{ind}</bsdf>"])
         if m["type"] == R.MAT_ROUGH_CONDUCTOR:
             return ([f'{ind}<bsdf type="RoughConductor">', f'{ind}  <double name="alpha" value="{float(m["alpha"])!r}"/>',
                      f'{ind}  <string name="distribution" value="{dist}"/>',

@@ -81,6 +81,8 @@ MATS = [
     scenes.conductor(0.4, dist=rtg.DIST_BECKMANN),
     scenes.dielectric(0.1),
     scenes.dielectric(0.3, dist=rtg.DIST_BECKMANN),
+    scenes.smooth_conductor(eta=(0.2, 0.92, 1.1), k=(3.9, 2.45, 2.14), reflectance=(0.9, 0.8, 0.7)),
+    scenes.smooth_dielectric(eta_int=1.5, eta_ext=1.000277, transmitted=(0.9, 0.95, 1.0)),
 ]
 
 
@@ -123,7 +125,7 @@ def test_bsdf_sample(ctx, mi):
     o = O.OracleScene(sd)
     rng = np.random.default_rng(20 + mi)
     n = 100000
-    wi = _dirs(rng, n, upper=(m["type"] != rtg.MAT_ROUGH_DIELECTRIC))
+    wi = _dirs(rng, n, upper=(m["type"] not in (rtg.MAT_ROUGH_DIELECTRIC, rtg.MAT_DIELECTRIC)))
     u = rng.random((n, 2)).astype(np.float32)
     wg, fg, pg, eg = ctx.bsdf_sample(m, wi, u)
     wo, fo, po, eo, _ = o.bsdf_sample(0, wi, u)

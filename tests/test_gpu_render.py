@@ -111,6 +111,14 @@ def test_glossy_cornell_unbiased(ctx):
     _block_z_test(ctx, sd, cam, 128, 16, 16384, "glossy cornell (dielectric spheres, conductor walls)")
 
 
+def test_specular_cornell_unbiased(ctx):
+    sd = scenes.specular_cornell()
+    cam = dict(sd.camera, width=32, height=32)
+    # caustics through the glass sphere (diffuse -> glass -> glass -> light, all by BSDF sampling)
+    # make the per-block CPU variance heavy-tailed: more CPU renders than for the other scenes
+    _block_z_test(ctx, sd, cam, 256, 32, 16384, "specular cornell (smooth mirror box, smooth glass sphere)")
+
+
 def test_displaced_sphere_env_unbiased(ctx):
     sd = scenes.displaced_sphere(level=5, width=32, height=32)
     _block_z_test(ctx, sd, sd.camera, 64, 8, 8192, "displaced sphere level 5 + env map")

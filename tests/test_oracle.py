@@ -455,3 +455,67 @@ def test_direct_light_on_plane_matches_quadrature():
     expected = nee + bsdf
     se = vals.std() / np.sqrt(n)
     assert abs(vals.mean() - expected) < 4 * se + 1e-4, (vals.mean(), expected, se)
+
+
+# ---------------------------------------------------------------- smooth (delta) BSDFs
+def _fresnel_conductor_normal(eta, k):
+    """Closed form at normal incidence: ((eta - 1)^2 + k^2) / ((eta + 1)^2 + k^2)."""
+    eta = np.asarray(eta, np.float64); k = np.asarray(k, np.float64)
+    return ((eta - 1) ** 2 + k ** 2) / ((eta + 1) ** 2 + k ** 2)
+
+
+def test_smooth_conductor_mirror_and_fresnel():
+    """bsdf/Conductor.cpp: mirror direction, pdf 1, weight = reflectance x Fresnel(cos_i); eval and
+    pdf are 0 in the solid-angle measure; nothing below the surface."""
+    m = scenes.smooth_conductor(eta=(0.2, 0.92, 1.1), k=(3.9, 2.45, 2.14), reflectance=(0.9, 0.8, 0.7))
+    o = O.OracleScene(scenes.SceneDesc(materials=[m]))
+    rng = np.random.default_rng(3)
+    wi = rng.normal(size=(2000, 3)); wi /= np.linalg.norm(wi, axis=1, keepdims=True)
+    wi[:, 2] = np.abs(wi[:, 2]); wi[0] = (0, 0, 1)
+    wo, w, pdf, eta, typ = o.bsdf_sample(0, wi, rng.random((2000, 2)))
+    assert np.allclose(wo[:, :2], -wi[:, :2], atol=1e-7) and np.allclose(wo[:, 2], wi[:, 2], atol=1e-7)
+    assert np.all(pdf == 1.0) and np.all(eta == 1.0)
+    # the reference's fresnelConductor (core/Fresnel.h:30-48) averages the squares of two terms that
+    # are already reflectances, so at normal incidence it gives R^2 for the closed-form R
+    r = _fresnel_conductor_normal((0.2, 0.92, 1.1), (3.9, 2.45, 2.14))
+    assert np.allclose(w[0], np.array((0.9, 0.8, 0.7)) * r * r, rtol=1e-5)
+    f, p = o.bsdf_eval(0, wi, wo)
+    assert np.all(f == 0) and np.all(p == 0)
+    wo2, w2, _, _, _ = o.bsdf_sample(0, -wi[1:], rng.random((1999, 2)))
+    assert np.all(w2 == 0)
+
+
+def test_smooth_dielectric_fresnel_split_and_snell():
+    """bsdf/Dielectric.cpp: reflection with probability fr (weight = reflectance, pdf fr), else
+    refraction with Snell's law (sin_t = eta_rel sin_i, weight = transmitted x eta_rel^2, pdf 1 - fr);
+    from inside past the critical angle only reflection. fr at normal incidence = ((n1-n2)/(n1+n2))^2."""
+    m = scenes.smooth_dielectric(eta_int=1.5, eta_ext=1.0, transmitted=(0.9, 0.95, 1.0))
+    o = O.OracleScene(scenes.SceneDesc(materials=[m]))
+    n = 200000
+    rng = np.random.default_rng(4)
+    for cos_i in (1.0, 0.6, -0.9, -0.5):   # outside, outside, inside, inside past critical (sin 0.87 > 1/1.5)
+        sin_i = np.sqrt(1 - cos_i * cos_i)
+        wi = np.tile(np.array([sin_i, 0.0, cos_i]), (n, 1))
+        wo, w, pdf, eta, typ = o.bsdf_sample(0, wi, rng.random((n, 2)))
+        refl = wo[:, 2] * cos_i > 0
+        n1, n2 = (1.0, 1.5) if cos_i > 0 else (1.5, 1.0)
+        rel = n1 / n2
+        sin_t = rel * sin_i
+        if sin_t >= 1:
+            assert np.all(refl) and np.allclose(pdf, 1.0)
+            continue
+        cos_t = np.sqrt(1 - sin_t ** 2)
+        rpar = (n2 * abs(cos_i) - n1 * cos_t) / (n2 * abs(cos_i) + n1 * cos_t)
+        rperp = (n1 * abs(cos_i) - n2 * cos_t) / (n1 * abs(cos_i) + n2 * cos_t)
+        fr = 0.5 * (rpar ** 2 + rperp ** 2)
+        if cos_i == 1.0:
+            assert abs(fr - ((1 - 1.5) / 2.5) ** 2) < 1e-12
+        frac = refl.mean()
+        assert abs(frac - fr) < 5 * np.sqrt(fr * (1 - fr) / n) + 1e-6, (cos_i, frac, fr)
+        assert np.allclose(pdf[refl], fr, rtol=1e-5) and np.allclose(pdf[~refl], 1 - fr, rtol=1e-5)
+        assert np.allclose(w[refl], 1.0) and np.allclose(w[~refl], np.array((0.9, 0.95, 1.0)) * rel * rel, rtol=1e-5)
+        assert np.allclose(eta, rel, rtol=1e-6)   # reported for both outcomes, as the reference does
+        # refracted: tangential part scaled by -rel, |wo| = 1, on the other side
+        t = wo[~refl]
+        assert np.allclose(t[:, 0], -rel * sin_i, rtol=1e-5) and np.allclose(np.abs(t[:, 2]), cos_t, rtol=1e-5)
+        assert np.all(t[:, 2] * cos_i < 0)

@@ -202,6 +202,18 @@ def case_shapes_refs_textures(ctx, tmp):
     assert rp["max_depth"] == 5 and rp["spp"] == 10
 
 
+def case_smooth_bsdfs(ctx, tmp):
+    from raytracer_amd import scenes
+
+    sd = scenes.specular_cornell(width=32, height=32)
+    path = scenes.write_xml(sd, tmp)
+    text = open(path).read()
+    assert 'type="Conductor"' in text and 'type="Dielectric"' in text
+    sc, _, rp = ctx.load_xml(path)
+    _same_info(sc.info(), ctx.upload(sd).info())
+    assert rp["max_depth"] == 16
+
+
 def case_errors(ctx, tmp):
     from raytracer_amd import rtg
 
@@ -266,6 +278,7 @@ def decode_rgbe(path):
 
 # ------------------------------------------------------------------ pytest entry points
 @pytest.mark.parametrize("case", ["case_cornell_roundtrip", "case_glossy_roundtrip", "case_env_roundtrip",
-                                  "case_obj_features", "case_shapes_refs_textures", "case_errors", "case_rgbe"])
+                                  "case_obj_features", "case_shapes_refs_textures", "case_smooth_bsdfs", "case_errors",
+                                  "case_rgbe"])
 def test_xml_loader(mock_lib, case, tmp_path):
     _run(mock_lib, case, tmp_path)
