@@ -48,6 +48,7 @@ enum { RTG_DIST_BECKMANN = 0, RTG_DIST_GGX = 2 };
 /* Light selection strategies: integrator/Integrator.h:18-22 */
 enum { RTG_LIGHTS_ALL = 0, RTG_LIGHTS_UNIFORM = 1, RTG_LIGHTS_WEIGHTED = 2 };
 /* Samplers: sampler/RandomSampler.cpp, sampler/StratifiedSampler.cpp */
+enum { RTG_FILTER_BOX = 0, RTG_FILTER_GAUSSIAN = 1, RTG_FILTER_MITCHELL = 2 };
 enum { RTG_SAMPLER_RANDOM = 0, RTG_SAMPLER_STRATIFIED = 1 };
 
 /* Constant-texture material parameters (texture/ConstantTexture.h:6-37).
@@ -129,6 +130,18 @@ typedef struct rtg_render_desc {
   int32_t use_camera_matrix;   /* 1: camera_to_world (row-major, the XML camera's toWorld, e.g. a
                                   lookAt with a mirroring scale) replaces origin/target/up */
   float camera_to_world[16];
+  /* Reconstruction filter (the camera's <filter>; filter/BoxFilter.h, GaussianFilter.cpp,
+   * MitchellNetravaliFilter.cpp): RTG_FILTER_BOX (weight 1), RTG_FILTER_GAUSSIAN (param = alpha,
+   * omega; a value <= 0 takes the reference's default, 2 and radius x) or RTG_FILTER_MITCHELL
+   * (param = B, C as given; the reference's defaults are 1/3, 1/3), radius x/y (<= 0: 0.5; at
+   * most 8), weights looked up in the reference's 32x32 table (ReconstructionFilter.h:53-62). A sample is splatted into the pixels of its
+   * 32x32 block's sub-screen (block plus the filter's overlap border, Screen.cpp:44-111) at their
+   * true positions; contributions outside the image are dropped. (The reference's final image
+   * reads its bordered array from index 0, which shifts filtered images by the overlap; the film
+   * returned here is not shifted.) All zero = the box filter of radius 0.5. */
+  int32_t filter;
+  float filter_radius[2];
+  float filter_param[2];
 } rtg_render_desc;
 
 typedef struct rtg_stats {

@@ -1722,10 +1722,44 @@ struct Film {
     std::mutex mtx;
 };
 
+// ---- reconstruction filters (filter/ReconstructionFilter.{h,cpp}, BoxFilter, GaussianFilter,
+// MitchellNetravaliFilter): getValue tabulated at 32x32 points of [0, rx) x [0, ry), looked up
+// by truncation (no interpolation)
+struct Filter {
+    int type = 0;  // 0 box, 1 gaussian, 2 mitchell
+    float rx = 0.5f, ry = 0.5f, p0 = 0.f, p1 = 0.f;
+    float lut[32 * 32];
+    float gauss1(float v) const {  // GaussianFilter.cpp:40-43
+        return std::max(0.f, std::exp(-p0 * v * v) - std::exp(-p0 * p1 * p1));
+    }
+    float mitchell1(float v) const {  // MitchellNetravaliFilter.cpp:44-59
+        const float B = p0, C = p1;
+        v = std::fabs(2.f * v);
+        if (v > 1.f)
+            return ((-B - 6 * C) * v * v * v + (6 * B + 30 * C) * v * v + (-12 * B - 48 * C) * v + 8 * B + 24 * C) *
+                   (1.f / 6.f);
+        return ((12 - 9 * B - 6 * C) * v * v * v + (-18 + 12 * B + 6 * C) * v * v + (6 - 2 * B)) * (1.f / 6.f);
+    }
+    float value(float x, float y) const {
+        if (type == 1) return gauss1(x) * gauss1(y);
+        if (type == 2) return mitchell1(x / rx) * mitchell1(y / ry);
+        return 1.f;
+    }
+    void precompute() {  // ReconstructionFilter.cpp:26-39
+        for (int x = 0; x < 32; x++)
+            for (int y = 0; y < 32; y++) lut[x + y * 32] = value(x * rx / 32, y * ry / 32);
+    }
+    float operator()(float x, float y) const {  // ReconstructionFilter.h:53-62
+        int ix = std::min((int)((32 * std::fabs(x)) / rx), 31);
+        int iy = std::min((int)((32 * std::fabs(y)) / ry), 31);
+        return lut[ix + iy * 32];
+    }
+};
+
 struct RenderParams {
     int spp = 16, samplerType = 1 /*0 random, 1 stratified*/, dimension = 4;
     int minDepth = 3, maxDepth = 1000, threads = 1, block = 32;
-    float radius = 0.5f;
+    Filter filter;
 };
 
 // SampleJob.cpp:28-72 for one 32x32 block, merged into the film (Screen.cpp:136-158)
@@ -1747,8 +1781,9 @@ static void runBlock(const Scene& scene, const Camera& cam, const RenderParams& 
     pt.maxDepth = rp.maxDepth;
     int endX = std::min(cam.W, bx + rp.block), endY = std::min(cam.H, by + rp.block);
     int sx = endX - bx, sy = endY - by;
-    int ov = (int)std::ceil(rp.radius - 0.5f);  // Screen.cpp:101-111
-    int pw = sx + 2 * ov, ph = sy + 2 * ov;
+    const Filter& flt = rp.filter;
+    int ovx = (int)std::ceil(flt.rx - 0.5f), ovy = (int)std::ceil(flt.ry - 0.5f);  // Screen.cpp:101-111
+    int pw = sx + 2 * ovx, ph = sy + 2 * ovy;
     std::vector<double> sub((size_t)pw * ph * 3, 0.0), subw((size_t)pw * ph, 0.0);
     uint64_t nans = 0;
     for (int y = by; y < endY; y++)
@@ -1760,23 +1795,24 @@ static void runBlock(const Scene& scene, const Camera& cam, const RenderParams& 
                 float yy = (float)y + s.y;
                 Ray ray = cam.getRay(xx, yy);
                 C3 col = pt.li(*sampler, ray);
-                // Screen.cpp:44-97 (box filter, weight 1)
+                // Screen.cpp:44-97
                 if (isNan(col)) {
                     nans++;
                 } else {
-                    float fx = xx - 0.5f - (bx - ov);
-                    float fy = yy - 0.5f - (by - ov);
-                    int x0 = std::max(0, (int)std::ceil(fx - rp.radius));
-                    int x1 = std::min(pw - 1, (int)std::floor(fx + rp.radius));
-                    int y0 = std::max(0, (int)std::ceil(fy - rp.radius));
-                    int y1 = std::min(ph - 1, (int)std::floor(fy + rp.radius));
+                    float fx = xx - 0.5f - (bx - ovx);
+                    float fy = yy - 0.5f - (by - ovy);
+                    int x0 = std::max(0, (int)std::ceil(fx - flt.rx));
+                    int x1 = std::min(pw - 1, (int)std::floor(fx + flt.rx));
+                    int y0 = std::max(0, (int)std::ceil(fy - flt.ry));
+                    int y1 = std::min(ph - 1, (int)std::floor(fy + flt.ry));
                     for (int px = x0; px <= x1; px++)
                         for (int py = y0; py <= y1; py++) {
+                            const float wgt = flt(px - fx, py - fy);
                             size_t k = (size_t)py * pw + px;
-                            sub[3 * k] += col.r;
-                            sub[3 * k + 1] += col.g;
-                            sub[3 * k + 2] += col.b;
-                            subw[k] += 1.0;
+                            sub[3 * k] += col.r * wgt;
+                            sub[3 * k + 1] += col.g * wgt;
+                            sub[3 * k + 2] += col.b * wgt;
+                            subw[k] += wgt;
                         }
                 }
                 sampler->advance();
@@ -1786,7 +1822,7 @@ static void runBlock(const Scene& scene, const Camera& cam, const RenderParams& 
         std::lock_guard<std::mutex> lock(film.mtx);
         for (int py = 0; py < ph; py++)
             for (int px = 0; px < pw; px++) {
-                int gx = bx - ov + px, gy = by - ov + py;
+                int gx = bx - ovx + px, gy = by - ovy + py;
                 if (gx < 0 || gy < 0 || gx >= film.W || gy >= film.H) continue;
                 size_t k = (size_t)py * pw + px, g = (size_t)gy * film.W + gx;
                 film.rgb[3 * g] += sub[3 * k];
@@ -2162,9 +2198,22 @@ void orc_li(void* sp, const float* ray, const float* u2, int n2, const float* u1
 // seeded (blockX, blockY), threads popping jobs LIFO. Accumulates in double.
 // cam: origin3 target3 up3 fov near far. stats: [rays, nan samples]. Optional block range restricts work
 // to blocks [b0, b1) in creation order (for bounded CPU baselines).
+// One table lookup of a filter (type, radius x, radius y, param 0, param 1).
+float orc_filter_value(const float* filter, float x, float y) {
+    Filter f;
+    f.type = (int)filter[0];
+    f.rx = filter[1];
+    f.ry = filter[2];
+    f.p0 = filter[3];
+    f.p1 = filter[4];
+    f.precompute();
+    return f(x, y);
+}
+
+// filter: type, radius x, radius y, param 0, param 1 (NULL: box of radius 0.5)
 int orc_render(void* sp, const float* cam, int W, int H, int spp, int samplerType, int minDepth, int maxDepth,
                int threads, int b0, int b1, int bstride, uint64_t seedOffset, double* rgb, double* wgt,
-               uint64_t* stats) {
+               uint64_t* stats, const float* filter) {
     Scene* s = (Scene*)sp;
     Camera c;
     buildCamera(c, cam, cam + 3, cam + 6, cam[9], W, H, cam[10], cam[11]);
@@ -2174,6 +2223,14 @@ int orc_render(void* sp, const float* cam, int W, int H, int spp, int samplerTyp
     rp.minDepth = minDepth;
     rp.maxDepth = maxDepth;
     rp.threads = std::max(1, threads);
+    if (filter) {
+        rp.filter.type = (int)filter[0];
+        rp.filter.rx = filter[1];
+        rp.filter.ry = filter[2];
+        rp.filter.p0 = filter[3];
+        rp.filter.p1 = filter[4];
+    }
+    rp.filter.precompute();
     Film film;
     film.W = W;
     film.H = H;

@@ -585,8 +585,60 @@ DevJob make_job(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc& d, float4
   J.scene = scene->slot;
   J.active = 1;
   J.tiles_x = (d.width + 7) / 8;
+  float lut[kFilterLut];
+  filter_setup(d, J, lut);
   (void)ctx;
   return J;
+}
+
+// Reconstruction filter of a render (filter/ReconstructionFilter.cpp:26-39 and the filters'
+// getValue): validates the description, fills the job's filter fields and the 32x32 weight table
+// (evaluated in float at (x rx / 32, y ry / 32), as the reference tabulates it).
+void filter_setup(const rtg_render_desc& d, DevJob& J, float* lut) {
+  if (d.filter < RTG_FILTER_BOX || d.filter > RTG_FILTER_MITCHELL) fail(RTG_ERR_INVALID, "unknown filter");
+  const float rx = d.filter_radius[0] > 0.f ? d.filter_radius[0] : 0.5f;
+  const float ry = d.filter_radius[1] > 0.f ? d.filter_radius[1] : 0.5f;
+  if (!(rx <= 8.f && ry <= 8.f)) fail(RTG_ERR_INVALID, "filter radius above 8");
+  float p0 = d.filter_param[0], p1 = d.filter_param[1];
+  if (d.filter == RTG_FILTER_GAUSSIAN) {  // GaussianFilter.cpp:21-27 defaults
+    if (!(p0 > 0.f)) p0 = 1.f / (2.f * 0.5f * 0.5f);
+    if (!(p1 > 0.f)) p1 = rx;
+  }
+  auto gauss = [&](float v) { return std::max(0.f, std::exp(-p0 * v * v) - std::exp(-p0 * p1 * p1)); };
+  auto mitchell = [&](float v) {
+    const float B = p0, C = p1;
+    v = std::fabs(2.f * v);
+    if (v > 1.f)
+      return ((-B - 6 * C) * v * v * v + (6 * B + 30 * C) * v * v + (-12 * B - 48 * C) * v + 8 * B + 24 * C) *
+             (1.f / 6.f);
+    return ((12 - 9 * B - 6 * C) * v * v * v + (-18 + 12 * B + 6 * C) * v * v + (6 - 2 * B)) * (1.f / 6.f);
+  };
+  for (int x = 0; x < 32; x++)
+    for (int y = 0; y < 32; y++) {
+      const float fx = x * rx / 32, fy = y * ry / 32;
+      float v = 1.f;
+      if (d.filter == RTG_FILTER_GAUSSIAN) v = gauss(fx) * gauss(fy);
+      if (d.filter == RTG_FILTER_MITCHELL) v = mitchell(fx / rx) * mitchell(fy / ry);
+      lut[x + y * 32] = v;
+    }
+  const bool plain_box = d.filter == RTG_FILTER_BOX && rx == 0.5f && ry == 0.5f;
+  J.filter = plain_box ? 0 : 1;
+  J.rx = rx;
+  J.ry = ry;
+  J.ox = (int)std::ceil(rx - 0.5f);  // Screen.cpp:101-111
+  J.oy = (int)std::ceil(ry - 0.5f);
+  J.lut = nullptr;
+}
+
+// Uploads a tabulated filter into the context's table of job slot `slot` and points the job at it.
+void upload_job_filter(rtg_ctx* ctx, const rtg_render_desc& d, DevJob& J, int slot) {
+  if (!J.filter) return;
+  float lut[kFilterLut];
+  DevJob tmp;
+  filter_setup(d, tmp, lut);
+  float* dst = ctx->lut_dev + (size_t)slot * kFilterLut;
+  hip_check(hipMemcpy(dst, lut, sizeof(lut), hipMemcpyHostToDevice), "filter table upload");
+  J.lut = dst;
 }
 
 }  // namespace rtg
@@ -632,6 +684,7 @@ int rtg_ctx_create(int device, rtg_ctx** out) {
     hip_check(hipMalloc(&ctx->scenes_dev, sizeof(DevScene) * kMaxScenes), "hipMalloc scenes");
     hip_check(hipMalloc(&ctx->jobs_dev, sizeof(DevJob) * kMaxJobs), "hipMalloc jobs");
     hip_check(hipMalloc(&ctx->order_dev, sizeof(int32_t) * kMaxJobs), "hipMalloc job order");
+    hip_check(hipMalloc(&ctx->lut_dev, sizeof(float) * kFilterLut * kMaxJobs), "hipMalloc filter tables");
     hip_check(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault), "hipHostMalloc");
     ctx->scene_slots.assign(kMaxScenes, nullptr);
     ctx->pool_slots = 1 << 21;
@@ -672,6 +725,7 @@ int rtg_ctx_destroy(rtg_ctx* ctx) {
   if (ctx->jobs_dev) (void)hipFree(ctx->jobs_dev);
   if (ctx->gstack) (void)hipFree(ctx->gstack);
   if (ctx->order_dev) (void)hipFree(ctx->order_dev);
+  if (ctx->lut_dev) (void)hipFree(ctx->lut_dev);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
   for (void* p : ctx->scratch_owned) (void)hipFree(p);
   for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
@@ -741,6 +795,7 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   if (ctx->server) fail(RTG_ERR_INVALID, "the context is serving requests; submit the render instead");
   hip_check(hipSetDevice(ctx->device), "hipSetDevice");
   DevJob J = make_job(ctx, scene, *desc, (float4*)out_dev);
+  upload_job_filter(ctx, *desc, J, 0);
   // pinned page: [0] done count, [16] guide ring and inference counts, [40] guide loss, [64] claim
   // order, [256] job
   uint64_t* done = (uint64_t*)ctx->pinned;

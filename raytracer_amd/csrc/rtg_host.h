@@ -73,6 +73,7 @@ struct rtg_ctx {
   rtg::DevJob* jobs_dev = nullptr;
   uint2* gstack = nullptr;
   int refill = rtg::kTraceRefill;
+  float* lut_dev = nullptr;  // kMaxJobs filter tables of kFilterLut weights
   int trace_blocks = 0;
   int32_t* order_dev = nullptr;    // claim order of job-table slots
   rtg_server* server = nullptr;    // the serving loop owning the pool, if any
@@ -89,6 +90,8 @@ void release_scene(rtg_ctx* ctx, rtg_scene* s);
 void ensure_pool(rtg_ctx* ctx, int capacity);
 int stack_for_resident(rtg_ctx* ctx);
 DevJob make_job(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc& d, float4* accum);
+void filter_setup(const rtg_render_desc& d, DevJob& J, float* lut);
+void upload_job_filter(rtg_ctx* ctx, const rtg_render_desc& d, DevJob& J, int slot);
 LaunchCfg launch_cfg(rtg_ctx* ctx);
 void camera_matrices(const rtg_render_desc& d, float* s2c, float* c2w);
 void* dev_alloc(rtg_ctx* ctx, size_t bytes, std::vector<void*>& owned);

@@ -98,8 +98,14 @@ struct DevJob {
   int32_t scene;        // index into the scene table
   int32_t active;       // 1 while samples may be claimed
   int32_t tiles_x;      // 8x8 pixel tiles per row (sample ordering)
-  int32_t pad;
+  // reconstruction filter: 0 the box of radius 0.5 (one pixel, or two for a jitter of exactly 0),
+  // 1 a tabulated filter (lut: 32x32 weights, radius rx/ry, sub-screen border ox/oy)
+  int32_t filter;
+  int32_t ox, oy;
+  float rx, ry;
+  const float* lut;
 };
+static const int kFilterLut = 32 * 32;
 
 // Path-pool state (structure of arrays). Slot flags word (rtg_kernels.hip):
 //   bits 0..11 depth of the pending hit, bits 12..19 job slot, bit 20 LAST (only the emission

@@ -45,22 +45,48 @@ __device__ __forceinline__ void decode_sample(const DevJob& J, uint64_t s, uint3
   pixel = (uint32_t)y * (uint32_t)J.width + (uint32_t)x;
 }
 
-// Box-filter splat (core/Screen.cpp:44-97 with 32x32 sub-screens, r = 0.5): the sample lands in
-// its own pixel; a jitter of exactly 0 also reaches the previous pixel inside the same block.
+__device__ __forceinline__ void film_add(const DevJob& J, int x, int y, f3 L, float w) {
+  float4* a = J.accum + (size_t)y * J.width + x;
+  // hardware float atomics (the film is ordinary coarse-grained device memory)
+  unsafeAtomicAdd(&a->x, L.x * w);
+  unsafeAtomicAdd(&a->y, L.y * w);
+  unsafeAtomicAdd(&a->z, L.z * w);
+  unsafeAtomicAdd(&a->w, w);
+}
+
+// Splat into the film (core/Screen.cpp:44-111, SampleJob.cpp:35-71): the sample goes to the
+// pixels of its 32x32 block's sub-screen (the block plus the filter's overlap border) within the
+// filter radius of (x + sx - 0.5, y + sy - 0.5), weighted by the filter table; sub-screen pixels
+// outside the image are dropped. Box filter of radius 0.5 (filter 0): its own pixel, and for a
+// jitter of exactly 0 also the previous pixel inside the same block.
 __device__ __forceinline__ void splat(const DevJob& J, uint32_t pixel, float sx, float sy, f3 L) {
-  int x = (int)(pixel % (uint32_t)J.width), y = (int)(pixel / (uint32_t)J.width);
-  int x0 = x, y0 = y;
-  if (sx == 0.f && (x & 31) != 0) x0 = x - 1;
-  if (sy == 0.f && (y & 31) != 0) y0 = y - 1;
-  for (int yy = y0; yy <= y; yy++)
-    for (int xx = x0; xx <= x; xx++) {
-      float4* a = J.accum + (size_t)yy * J.width + xx;
-      // hardware float atomics (the film is ordinary coarse-grained device memory)
-      unsafeAtomicAdd(&a->x, L.x);
-      unsafeAtomicAdd(&a->y, L.y);
-      unsafeAtomicAdd(&a->z, L.z);
-      unsafeAtomicAdd(&a->w, 1.0f);
+  const int x = (int)(pixel % (uint32_t)J.width), y = (int)(pixel / (uint32_t)J.width);
+  if (J.filter == 0) {
+    int x0 = x, y0 = y;
+    if (sx == 0.f && (x & 31) != 0) x0 = x - 1;
+    if (sy == 0.f && (y & 31) != 0) y0 = y - 1;
+    for (int yy = y0; yy <= y; yy++)
+      for (int xx = x0; xx <= x; xx++) film_add(J, xx, yy, L, 1.0f);
+    return;
+  }
+  const int bx0 = x & ~31, by0 = y & ~31;
+  const int pw = min(32, J.width - bx0) + 2 * J.ox, ph = min(32, J.height - by0) + 2 * J.oy;
+  const float fx = ((float)x + sx) - 0.5f - (float)(bx0 - J.ox);
+  const float fy = ((float)y + sy) - 0.5f - (float)(by0 - J.oy);
+  const int x0 = max(0, (int)ceilf(fx - J.rx)), x1 = min(pw - 1, (int)floorf(fx + J.rx));
+  const int y0 = max(0, (int)ceilf(fy - J.ry)), y1 = min(ph - 1, (int)floorf(fy + J.ry));
+  for (int py = y0; py <= y1; py++) {
+    const int gy = by0 - J.oy + py;
+    if (gy < 0 || gy >= J.height) continue;
+    // ReconstructionFilter.h:53-62: table index by truncation
+    const int iy = min((int)((32.f * fabsf((float)py - fy)) / J.ry), 31);
+    for (int px = x0; px <= x1; px++) {
+      const int gx = bx0 - J.ox + px;
+      if (gx < 0 || gx >= J.width) continue;
+      const int ix = min((int)((32.f * fabsf((float)px - fx)) / J.rx), 31);
+      film_add(J, gx, gy, L, J.lut[ix + iy * 32]);
     }
+  }
 }
 
 // ---------------------------------------------------------------- block primitives

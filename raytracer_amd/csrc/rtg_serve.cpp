@@ -130,6 +130,7 @@ void admit(rtg_server* s) {
     s->st.admitted++;
     launch_fill_zero(r->film, bytes, ctx->stream);
     r->job = make_job(ctx, r->scene, r->desc, r->film);
+    upload_job_filter(ctx, r->desc, r->job, slot);
     r->claimed = 0;
     r->slot = slot;
     s->slots[slot] = r;

@@ -805,9 +805,26 @@ struct Loader {
     for (const XNode& c : n.kids) {
       if (c.name == "transform") T = parse_transform(c);
       if (c.name == "filter") {
+        // XMLLoader::handleReconstructionFilter (core/XMLLoader.cpp:425-438); parameters and
+        // defaults of filter/ReconstructionFilter.cpp:13-19, GaussianFilter.cpp:21-27,
+        // MitchellNetravaliFilter.cpp:15-21
         Props F = props(c, "filter");
-        if (type_of(c) != "boxfilter" || F.real("radiusX", 0.5) != 0.5 || F.real("radiusY", 0.5) != 0.5)
-          perr("camera filter: only the box filter of radius 0.5 is rendered by the GPU path");
+        const std::string ft = type_of(c);
+        rd.filter_radius[0] = (float)F.real("radiusX", 0.5);
+        rd.filter_radius[1] = (float)F.real("radiusY", 0.5);
+        if (ft == "boxfilter") {
+          rd.filter = RTG_FILTER_BOX;
+        } else if (ft == "gaussianfilter") {
+          rd.filter = RTG_FILTER_GAUSSIAN;
+          rd.filter_param[0] = (float)F.real("alpha", 1.0 / (2.0 * 0.5 * 0.5));
+          rd.filter_param[1] = (float)F.real("omega", rd.filter_radius[0]);
+        } else if (ft == "mitchellnetravalifilter") {
+          rd.filter = RTG_FILTER_MITCHELL;
+          rd.filter_param[0] = (float)F.real("B", 1.0 / 3);
+          rd.filter_param[1] = (float)F.real("C", 1.0 / 3);
+        } else {
+          perr("camera filter type " + ft + " is not known");
+        }
       }
     }
     Props P = props(n, "camera NoriPerspective");

@@ -15,6 +15,7 @@ ERRORS = {-1: "INVALID", -2: "HIP", -3: "OOM", -4: "PARSE", -5: "NOT_READY", -6:
 
 MAT_DIFFUSE, MAT_ROUGH_CONDUCTOR, MAT_ROUGH_DIELECTRIC, MAT_CONDUCTOR, MAT_DIELECTRIC = 0, 1, 2, 3, 4
 DIST_BECKMANN, DIST_GGX = 0, 2
+FILTER_BOX, FILTER_GAUSSIAN, FILTER_MITCHELL = 0, 1, 2
 LIGHTS_ALL, LIGHTS_UNIFORM, LIGHTS_WEIGHTED = 0, 1, 2
 SAMPLER_RANDOM, SAMPLER_STRATIFIED = 0, 1
 
@@ -54,6 +55,7 @@ class rtg_render_desc(C.Structure):
         ("spp", C.c_int32), ("sampler", C.c_int32), ("dimension", C.c_int32), ("min_depth", C.c_int32),
         ("max_depth", C.c_int32), ("seed", C.c_uint64), ("guiding", C.c_int32),
         ("use_camera_matrix", C.c_int32), ("camera_to_world", C.c_float * 16),
+        ("filter", C.c_int32), ("filter_radius", C.c_float * 2), ("filter_param", C.c_float * 2),
     ]
 
 
@@ -189,7 +191,24 @@ def render_struct(cam, rp):
     if cam.get("to_world") is not None:
         d.use_camera_matrix = 1
         d.camera_to_world[:] = [float(x) for x in np.asarray(cam["to_world"], np.float64).reshape(16)]
+    f = rp.get("filter")
+    if f:
+        d.filter = int(f.get("type", FILTER_BOX))
+        d.filter_radius[:] = [float(x) for x in f.get("radius", (0.5, 0.5))]
+        d.filter_param[:] = [float(x) for x in f.get("param", (0.0, 0.0))]
     return d
+
+
+def filter_desc(kind="box", radius=(0.5, 0.5), alpha=2.0, omega=None, B=1.0 / 3, C_=1.0 / 3):
+    """Render-parameter entry for a reconstruction filter, with the reference's defaults
+    (GaussianFilter: alpha 1/(2 0.5^2), omega = radius x; MitchellNetravali: B = C = 1/3)."""
+    if kind == "box":
+        return dict(type=FILTER_BOX, radius=tuple(radius), param=(0.0, 0.0))
+    if kind == "gaussian":
+        return dict(type=FILTER_GAUSSIAN, radius=tuple(radius), param=(alpha, radius[0] if omega is None else omega))
+    if kind == "mitchell":
+        return dict(type=FILTER_MITCHELL, radius=tuple(radius), param=(B, C_))
+    raise ValueError(kind)
 
 
 class Context:
@@ -231,7 +250,8 @@ class Context:
         if d.use_camera_matrix:
             cam["to_world"] = np.array(d.camera_to_world[:], np.float64).reshape(4, 4)
         rp = dict(spp=d.spp, sampler=d.sampler, dimension=d.dimension, min_depth=d.min_depth,
-                  max_depth=d.max_depth, seed=d.seed)
+                  max_depth=d.max_depth, seed=d.seed,
+                  filter=dict(type=d.filter, radius=tuple(d.filter_radius[:]), param=tuple(d.filter_param[:])))
         return Scene.from_handle(self, h), cam, rp
 
     def set_pool(self, slots):

@@ -392,6 +392,21 @@ def _fmt(v):
     return " ".join(repr(float(x)) for x in v)
 
 
+def _filter_xml(f):
+    if not f:
+        return ['    <filter type="BoxFilter"/>']
+    kind = {R.FILTER_BOX: "BoxFilter", R.FILTER_GAUSSIAN: "GaussianFilter", R.FILTER_MITCHELL: "MitchellNetravaliFilter"}
+    rx, ry = f.get("radius", (0.5, 0.5))
+    out = [f'    <filter type="{kind[f["type"]]}">', f'      <double name="radiusX" value="{float(rx)!r}"/>',
+           f'      <double name="radiusY" value="{float(ry)!r}"/>']
+    p0, p1 = f.get("param", (0.0, 0.0))
+    if f["type"] == R.FILTER_GAUSSIAN:
+        out += [f'      <double name="alpha" value="{float(p0)!r}"/>', f'      <double name="omega" value="{float(p1)!r}"/>']
+    if f["type"] == R.FILTER_MITCHELL:
+        out += [f'      <double name="B" value="{float(p0)!r}"/>', f'      <double name="C" value="{float(p1)!r}"/>']
+    return out + ["    </filter>"]
+
+
 def write_xml(sd, directory, name="scene.xml"):
     """Write a SceneDesc as a scene file in the reference's XML schema (one OBJ per mesh and
     material, a MeshLight on emitting meshes, Sphere shapes with a toWorld matrix, an RGBE
@@ -417,7 +432,7 @@ def write_xml(sd, directory, name="scene.xml"):
             f'    <double name="farClip" value="{float(cam.get("far", 1e4))!r}"/>',
             f'    <integer name="sizeX" value="{int(cam["width"])}"/>',
             f'    <integer name="sizeY" value="{int(cam["height"])}"/>',
-            '    <filter type="BoxFilter"/>', "  </camera>"]
+            *_filter_xml(rp.get("filter")), "  </camera>"]
 
     def bsdf(i, ind="      "):
         m = sd.materials[i]

@@ -53,8 +53,9 @@ def load():
             "orc_light_cdf": (None, [P, P]),
             "orc_li": (None, [P, P, P, C.c_int, P, C.c_int, C.c_int, C.c_int, P]),
             "orc_render": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
-                                     C.c_int, C.c_int, C.c_uint64, P, P, P]),
+                                     C.c_int, C.c_int, C.c_uint64, P, P, P, P]),
             "orc_bvh_info": (C.c_int, [P, P]),
+            "orc_filter_value": (C.c_float, [P, C.c_float, C.c_float]),
             "gor_num_params": (C.c_int, []),
             "gor_bf16": (C.c_double, [C.c_double]),
             "gor_encode": (None, [P, C.c_int, P]),
@@ -77,6 +78,18 @@ def _p(a):
 def f32(a, shape=None):
     a = np.ascontiguousarray(a, dtype=np.float32)
     return a if shape is None else a.reshape(shape)
+
+
+def filter_params(rp):
+    """(type, radius x, radius y, param 0, param 1) from render parameters' "filter" entry."""
+    f = rp.get("filter") or {}
+    return f32([f.get("type", 0), *f.get("radius", (0.5, 0.5)), *f.get("param", (0.0, 0.0))])
+
+
+def filter_value(f, x, y):
+    """The oracle's table lookup for filter entry f (see filter_params) at offset (x, y)."""
+    p = filter_params({"filter": f})
+    return float(load().orc_filter_value(_p(p), float(x), float(y)))
 
 
 def material_params(m):
@@ -190,7 +203,7 @@ class OracleScene:
                                       int(rp["min_depth"] if min_depth is None else min_depth),
                                       int(rp["max_depth"] if max_depth is None else max_depth), int(threads),
                                       int(blocks[0]), int(blocks[1]), int(blocks[2] if len(blocks) > 2 else 1),
-                                      int(seed_offset), _p(rgb), _p(wgt), _p(stats))
+                                      int(seed_offset), _p(rgb), _p(wgt), _p(stats), _p(filter_params(rp)))
         return rgb, wgt, stats, nblocks
 
 

@@ -519,3 +519,58 @@ def test_smooth_dielectric_fresnel_split_and_snell():
         t = wo[~refl]
         assert np.allclose(t[:, 0], -rel * sin_i, rtol=1e-5) and np.allclose(np.abs(t[:, 2]), cos_t, rtol=1e-5)
         assert np.all(t[:, 2] * cos_i < 0)
+
+
+# ---------------------------------------------------------------- reconstruction filters
+def _constant_scene(w=40, h=24, spp=16, flt=None):
+    sd = scenes.SceneDesc()
+    sd.env = np.full((8, 16, 3), 0.75, np.float32)
+    sd.materials = [scenes.diffuse((0.5, 0.5, 0.5))]   # unused: the scene holds no shape
+    sd.camera = dict(origin=(0.0, 0.0, 0.0), target=(0.0, 0.0, 1.0), up=(0.0, 1.0, 0.0), fov=60.0, width=w,
+                     height=h, near=1e-4, far=1e4)
+    sd.render = dict(spp=spp, sampler=rtg.SAMPLER_STRATIFIED, dimension=4, min_depth=3, max_depth=8, seed=1,
+                     filter=flt)
+    return sd
+
+
+@pytest.mark.parametrize("flt", [rtg.filter_desc("box", (1.0, 1.0)), rtg.filter_desc("gaussian", (1.5, 1.5)),
+                                 rtg.filter_desc("mitchell", (2.0, 2.0)), rtg.filter_desc("box")])
+def test_filtered_film_of_a_constant_scene(flt):
+    """Screen.cpp:44-158 with a tabulated filter: every sample of a constant scene carries the same
+    radiance, so each pixel's weighted mean is that radiance whatever the weights (the image is
+    40x24: edge blocks narrower than 32, sub-screen borders clipped at the image edge), and every
+    pixel receives weight; the box of radius 0.5 gives each pixel exactly spp."""
+    sd = _constant_scene(flt=flt)
+    rgb, w, _, _ = O.OracleScene(sd).render(threads=4)
+    assert np.all(w > 0)
+    img = rgb / w[..., None]
+    c = img[0, 0]
+    assert np.all(c > 0) and np.allclose(img, c[None, None, :], rtol=1e-6)
+    if flt["radius"] == (0.5, 0.5):
+        assert np.all(w == 16.0)
+
+
+def test_filter_table_values():
+    """The 32x32 table (ReconstructionFilter.cpp:26-39, looked up by truncation,
+    ReconstructionFilter.h:53-62) against the filters' formulas at table nodes and just past them:
+    Gaussian max(0, e^{-alpha v^2} - e^{-alpha omega^2}) per axis (GaussianFilter.cpp:40-43),
+    Mitchell-Netravali with B = C = 1/3 on |2 v / r| (MitchellNetravaliFilter.cpp:44-59), box 1."""
+    def mitchell(v, B=1 / 3, C=1 / 3):
+        v = abs(2 * v)
+        if v > 1:
+            return ((-B - 6 * C) * v ** 3 + (6 * B + 30 * C) * v ** 2 + (-12 * B - 48 * C) * v + 8 * B + 24 * C) / 6
+        return ((12 - 9 * B - 6 * C) * v ** 3 + (-18 + 12 * B + 6 * C) * v ** 2 + (6 - 2 * B)) / 6
+
+    cases = [(rtg.filter_desc("gaussian", (1.5, 1.0), alpha=2.0, omega=1.5),
+              lambda x, y: max(0, np.exp(-2 * x * x) - np.exp(-2 * 2.25)) * max(0, np.exp(-2 * y * y) - np.exp(-2 * 2.25))),
+             (rtg.filter_desc("mitchell", (2.0, 2.0)), lambda x, y: mitchell(x / 2) * mitchell(y / 2)),
+             (rtg.filter_desc("box", (1.0, 1.0)), lambda x, y: 1.0)]
+    for f, ref in cases:
+        rx, ry = f["radius"]
+        for ix, iy in [(0, 0), (5, 0), (13, 7), (31, 31), (20, 3)]:
+            x, y = ix * rx / 32, iy * ry / 32
+            want = ref(x, y)
+            for dx in (0.0, 0.4 * rx / 32):   # inside the same table cell
+                got = O.filter_value(f, x + dx, -(y + dx))
+                assert abs(got - want) <= 1e-5 * max(1.0, abs(want)), (f["type"], ix, iy, got, want)
+        assert O.filter_value(f, 2 * rx, 0.0) == O.filter_value(f, 31 * rx / 32, 0.0)  # clamped to the last cell

@@ -214,6 +214,30 @@ def case_smooth_bsdfs(ctx, tmp):
     assert rp["max_depth"] == 16
 
 
+def case_filters(ctx, tmp):
+    """<filter> inside the camera (filter/*.cpp parameter names and defaults): written by
+    scenes.write_xml and read back, and the defaults of a bare Gaussian / Mitchell element."""
+    from raytracer_amd import scenes
+
+    R = scenes.R
+    for f in [R.filter_desc("gaussian", (1.5, 1.25), alpha=3.0, omega=1.0), R.filter_desc("mitchell", (2.0, 2.0), B=0.5, C_=0.25),
+              R.filter_desc("box", (1.0, 0.75)), None]:
+        sd = scenes.cornell(width=16, height=16)
+        sd.render = dict(sd.render, filter=f)
+        _, _, rp = ctx.load_xml(scenes.write_xml(sd, os.path.join(tmp, "f"), name="f.xml"))
+        want = f or R.filter_desc("box")
+        got = rp["filter"]
+        assert got["type"] == want["type"] and np.allclose(got["radius"], want["radius"]), (got, want)
+        assert np.allclose(got["param"], want["param"]), (got, want)
+    for kind, params in [("GaussianFilter", (2.0, 0.75)), ("MitchellNetravaliFilter", (1 / 3, 1 / 3))]:
+        p = os.path.join(tmp, kind + ".xml")
+        with open(p, "w") as fh:
+            fh.write(CAMERA.replace("</camera>", f"<filter type='{kind}'><double name='radiusX' value='0.75'/></filter></camera>")
+                     .join(["<scene>", "<objects><shape type='Cube'/></objects></scene>"]))
+        _, _, rp = ctx.load_xml(p)
+        assert np.allclose(rp["filter"]["radius"], (0.75, 0.5)) and np.allclose(rp["filter"]["param"], params), rp["filter"]
+
+
 def case_errors(ctx, tmp):
     from raytracer_amd import rtg
 
@@ -278,7 +302,7 @@ def decode_rgbe(path):
 
 # ------------------------------------------------------------------ pytest entry points
 @pytest.mark.parametrize("case", ["case_cornell_roundtrip", "case_glossy_roundtrip", "case_env_roundtrip",
-                                  "case_obj_features", "case_shapes_refs_textures", "case_smooth_bsdfs", "case_errors",
+                                  "case_obj_features", "case_shapes_refs_textures", "case_smooth_bsdfs", "case_filters", "case_errors",
                                   "case_rgbe"])
 def test_xml_loader(mock_lib, case, tmp_path):
     _run(mock_lib, case, tmp_path)
