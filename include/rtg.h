@@ -50,6 +50,8 @@ enum { RTG_LIGHTS_ALL = 0, RTG_LIGHTS_UNIFORM = 1, RTG_LIGHTS_WEIGHTED = 2 };
 /* Samplers: sampler/RandomSampler.cpp, sampler/StratifiedSampler.cpp */
 enum { RTG_FILTER_BOX = 0, RTG_FILTER_GAUSSIAN = 1, RTG_FILTER_MITCHELL = 2 };
 enum { RTG_SAMPLER_RANDOM = 0, RTG_SAMPLER_STRATIFIED = 1 };
+/* integrator/PathTracingMIS.cpp, integrator/VolPathTracing.cpp */
+enum { RTG_INTEGRATOR_PATH_MIS = 0, RTG_INTEGRATOR_VOLPATH = 1 };
 
 /* Constant-texture material parameters (texture/ConstantTexture.h:6-37).
  * albedo: Diffuse albedo, or reflectanceTexture of the rough BSDFs.
@@ -104,6 +106,16 @@ typedef struct rtg_scene_desc {
   int32_t env_height;
   float env_scale;
   int32_t light_strategy;      /* RTG_LIGHTS_UNIFORM or RTG_LIGHTS_WEIGHTED */
+  /* Camera medium (the scene file's top-level <medium>, core/XMLLoader.cpp:88-92,159-172): a
+   * Heterogeneous medium (medium/Heterogeneous.cpp) over a GridVolume density grid
+   * (volume/GridVolume.cpp) with a Henyey-Greenstein phase function (medium/HGPhaseFunction.cpp).
+   * As a camera medium it is never placed in an owner box, so the grid spans the world box
+   * [0,1]^3. Only the RTG_INTEGRATOR_VOLPATH integrator reads it. NULL density = no medium. */
+  const float* medium_density; /* res_x * res_y * res_z values, x fastest then y then z */
+  int32_t medium_res[3];       /* each >= 2 */
+  float medium_albedo[3];      /* single-scattering albedo ("albedo") */
+  float medium_g;              /* HGPhaseFunction "g" (< 1e-4: isotropic sampling) */
+  float medium_scale;          /* density multiplier ("scale"; <= 0 takes the default 1) */
 } rtg_scene_desc;
 
 /* Camera (camera/NoriPerspective.cpp:10-37 + core/Transform.cpp:199-227 lookAt),
@@ -142,6 +154,10 @@ typedef struct rtg_render_desc {
   int32_t filter;
   float filter_radius[2];
   float filter_param[2];
+  /* RTG_INTEGRATOR_PATH_MIS (0, integrator/PathTracingMIS.cpp) or RTG_INTEGRATOR_VOLPATH
+   * (integrator/VolPathTracing.cpp:25-426: delta tracking in the scene's camera medium, ratio
+   * tracking for transmittance; rtg_render / rtg_render_device only, no guiding) */
+  int32_t integrator;
 } rtg_render_desc;
 
 typedef struct rtg_stats {
@@ -223,6 +239,18 @@ int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int3
  * solid-angle measure), op 1 the mirror / refracted direction with pdf 1, fr or 1 - fr. */
 int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const float* wi, const float* wo_or_u,
                    int32_t n, float* out_dir, float* f, float* pdf, float* rel_eta);
+
+/* Parity tap for the scene's camera medium (medium/Heterogeneous.cpp, volume/GridVolume.cpp,
+ * medium/HGPhaseFunction.cpp); the scene must have one.
+ * op 0 GridVolume::lookupSigmaT x scale: in = n points (x 3) -> out n.
+ * op 1 Heterogeneous::sampleDistance (delta tracking) and op 2 Heterogeneous::transmittance (ratio
+ *   tracking): in = n rays (x 8: origin, direction, minT, maxT), u = k given 1D draws per ray (a ray
+ *   that needs more draws gets 0.5) -> out n x 3: (scattered 0/1, t, draws used) for op 1,
+ *   (transmittance, draws used, 0) for op 2.
+ * op 3 HGPhaseFunction::eval: in = n x 6 (wi, wo) -> out n.
+ * op 4 HGPhaseFunction::sample: in = n x 3 (wi), u = n x 2 (k = 2) -> out n x 4 (wo, pdf). */
+int rtg_medium_batch(rtg_ctx* ctx, rtg_scene* scene, int32_t op, const float* in, int32_t n, const float* u,
+                     int32_t k, float* out);
 
 /* Path-guiding network parity taps (no reference counterpart; the network is this library's own).
  * params: RTG_GUIDE_PARAMS FP32 values, per layer W[out][in] row-major then bias:

@@ -1010,8 +1010,118 @@ struct Bvh {
     }
 };
 
+// ---------------------------------------------------------------- participating medium
+// The camera medium (core/XMLLoader.cpp:159-172): medium/Heterogeneous.cpp over
+// volume/GridVolume.cpp with medium/HGPhaseFunction.cpp. A camera medium never receives
+// setOwnerBBox, so its world-to-array transform is the identity and its world box is [0,1]^3
+// (Heterogeneous.cpp:15-41).
+struct Medium {
+    std::vector<float> data;  // x fastest, then y, then z
+    int res[3] = {0, 0, 0};
+    C3 albedo = c3(0.f);
+    float g = 0.f, scale = 1.f, maxSigmaT = 0.f;
+    bool present() const { return !data.empty(); }
+    // GridVolume.cpp:96-105 getMaxSigmaT x scale (Heterogeneous.cpp:40)
+    void finalize() {
+        float m = data[0];
+        for (size_t i = 1; i < data.size(); i++)
+            if (data[i] > m) m = data[i];
+        maxSigmaT = m * scale;
+    }
+    // GridVolume.cpp:118-148 x Heterogeneous.cpp:403-406
+    float lookupSigmaT(V3 p) const {
+        float qx = p.x * (float)res[0], qy = p.y * (float)res[1], qz = p.z * (float)res[2];
+        float fx = std::floor(qx), fy = std::floor(qy), fz = std::floor(qz);
+        if (!(fx >= 0.f && fy >= 0.f && fz >= 0.f)) return 0.f;
+        if (fx >= (float)(res[0] - 1) || fy >= (float)(res[1] - 1) || fz >= (float)(res[2] - 1)) return 0.f;
+        size_t row = (size_t)res[0], slab = row * (size_t)res[1];
+        size_t off = (size_t)fz * slab + (size_t)fy * row + (size_t)fx;
+        const float d000 = data[off], d001 = data[off + 1], d010 = data[off + row], d011 = data[off + row + 1];
+        const float d100 = data[off + slab], d101 = data[off + slab + 1], d110 = data[off + slab + row],
+                    d111 = data[off + slab + row + 1];
+        float w1x = qx - fx, w1y = qy - fy, w1z = qz - fz;
+        float w0x = 1 - w1x, w0y = 1 - w1y, w0z = 1 - w1z;
+        float v = ((d000 * w0x + d001 * w1x) * w0y + (d010 * w0x + d011 * w1x) * w1y) * w0z +
+                  ((d100 * w0x + d101 * w1x) * w0y + (d110 * w0x + d111 * w1x) * w1y) * w1z;
+        return v * scale;
+    }
+    // Heterogeneous.cpp:491-546 (delta tracking). The reference starts at ray.minT; its tentative
+    // collisions before the box entry meet zero density and are all rejected, so starting at the
+    // entry draws the same distribution. Bounded at 16384 tentative collisions.
+    bool sampleDistance(const Ray& ray, Sampler& sampler, float& t, C3& weight) const {
+        Box box{v3(0, 0, 0), v3(1, 1, 1)};
+        float nearT, farT;
+        weight = c3(1.f);
+        if (!boxIntersect(box, ray, nearT, farT)) {
+            t = ray.maxT;
+            return false;
+        }
+        float tMax = std::min(ray.maxT, farT);
+        t = std::max(ray.minT, nearT);
+        float sigma = 0.f;
+        for (int it = 0;; it++) {
+            if (it >= 16384) return false;
+            t += -std::log(1.f - sampler.next1D()) / maxSigmaT;
+            if (t > tMax) {
+                t = tMax;
+                return false;
+            }
+            sigma = lookupSigmaT(ray.o + t * ray.d);
+            if (sampler.next1D() < sigma / maxSigmaT) break;
+        }
+        weight = sigma != 0.f ? albedo : c3(0.f);
+        return sigma != 0.f;
+    }
+    // Heterogeneous.cpp:419-489 (ratio tracking with roulette below 0.1; length maxT - minT)
+    float transmittance(const Ray& ray, Sampler& sampler) const {
+        float tMax = ray.maxT - ray.minT;
+        float tr = 1.f, sampledT;
+        float nearT, farT;
+        Box box{v3(0, 0, 0), v3(1, 1, 1)};
+        if (!boxIntersect(box, ray, nearT, farT)) return 1.f;
+        tMax = std::min(tMax, farT);
+        sampledT = std::max(nearT, ray.minT);
+        int breaker = 0;
+        while (true) {
+            sampledT += -std::log(1.f - sampler.next1D()) / maxSigmaT;
+            float proba = lookupSigmaT(ray.o + sampledT * ray.d);
+            if (sampledT >= tMax) return tr;
+            tr *= 1 - std::max(0.f, proba / maxSigmaT);
+            const float threshold = .1f;
+            if (tr < threshold) {
+                float q = std::max(.05f, 1 - tr);
+                if (sampler.next1D() < q) return 0.f;
+                tr /= 1 - q;
+            }
+            if (breaker > 1024) return 0.f;
+            breaker++;
+        }
+    }
+    // HGPhaseFunction.cpp:61-65
+    float hgEval(V3 wi, V3 wo) const {
+        float cosTheta = dot(-wi, wo);
+        return 0.07957747154594766788450883936693f * (1 - g * g) / std::pow(1 + g * g - 2 * g * cosTheta, 1.5f);
+    }
+    // HGPhaseFunction.cpp:27-45: the inversion as written, the direction around world +y
+    // (Geometry.h:181-188) whatever wi is, weight 1
+    V3 hgSample(V3 wi, V2 s, float& pdf) const {
+        float cosTheta;
+        if (g < EPSILON)
+            cosTheta = 1.f - 2.f * s.x;
+        else
+            cosTheta = (1 + g * g - ((1 - g * g) * (1 - g * g) / (1 - g + 2 * g * s.x))) / (2 * g);
+        float theta = std::acos(cosTheta);
+        float phi = 2 * PI * s.y;
+        V3 wo = cartesianFromSpherical(theta, phi);
+        pdf = hgEval(wi, wo);
+        return wo;
+    }
+};
+
 // ---------------------------------------------------------------- scene
 struct Scene {
+    Medium medium;           // camera medium (empty: none)
+    int integrator = 0;      // 0 PathTracingMIS, 1 VolPathTracing
     std::vector<Material> mats;
     std::vector<Mesh> meshes;
     std::vector<Sphere> spheres;
@@ -1621,6 +1731,177 @@ struct PathTracer {
     }
 };
 
+// VolPathTracing::li (integrator/VolPathTracing.cpp:25-346) and its transmittance helpers
+// (Integrator.cpp:81-170, VolPathTracing.cpp:348-423), for scenes whose only medium is the camera
+// medium and whose shapes all carry a BSDF: no medium transitions, and the first surface on a
+// shadow ray occludes it.
+struct VolPathTracer {
+    const Scene* scene;
+    int minDepth = 3, maxDepth = 1000;
+    uint64_t rays = 0;
+
+    const Medium* medium() const { return scene->medium.present() ? &scene->medium : nullptr; }
+    // Integrator.cpp:81-135
+    float evalTransmittance(const Ray& shadowRay, Sampler& sampler) {
+        Hit tmp;
+        rays++;
+        if (scene->intersect(shadowRay, tmp, true)) return 0.f;
+        const Medium* m = medium();
+        return m ? m->transmittance(shadowRay, sampler) : 1.f;
+    }
+    // Integrator.cpp:137-170
+    C3 sampleAttenuatedLightDirect(V3 p, Sampler& sampler, Scene::LightSample& info) {
+        if (scene->lights.empty()) return c3(0.f);
+        V2 s = sampler.next2D();
+        float lightWeight;
+        int li = scene->lightWeights.sampleAndReuse(s.x, lightWeight);
+        info = scene->lightSample(li, p, s);
+        C3 value = c3(0.f);
+        if (info.pdf > 0 && luminance(info.intensity) > 0) {
+            Ray shadowRay = makeRay(p, info.toLight, EPSILON, info.distance - EPSILON);
+            float tr = evalTransmittance(shadowRay, sampler);
+            info.pdf *= lightWeight;
+            value = tr * info.intensity / info.pdf;
+        }
+        return value;
+    }
+    // Mesh::pdf (Mesh.cpp:70-93) as VolPathTracing.cpp:152 calls it after a medium vertex: from the
+    // point where the sampled ray met the light to the point light sampling drew. On a planar
+    // emitter the two are coplanar and the cosine is 0 in exact arithmetic (pdf 0); rounding makes
+    // the reference's sign arbitrary, so a cosine within 1e-4 of 0 is taken as 0.
+    float mediumHitLightPdf(int li, V3 pFrom, V3 sampled, V3 normal) const {
+        const Light& L = scene->lights[li];
+        const Mesh& m = scene->meshes[L.mesh];
+        V3 v = pFrom - sampled;
+        float distance = norm(v);
+        v = normalized(v);
+        float cosine = dot(normal, v);
+        if (cosine > 1e-4f) return distance * distance / (m.area * cosine);
+        return 0.f;
+    }
+    static float maxCoeff(C3 c) { return std::max(c.r, std::max(c.g, c.b)); }
+
+    C3 li(Sampler& sampler, const Ray& _ray) {
+        const Medium* med = medium();
+        C3 radiance = c3(0.f), throughput = c3(1.f);
+        float eta = 1.f;
+        int depth = 0;
+        Ray ray = _ray;
+        Hit intersection;  // keeps its t when an intersection fails (Bvh.cpp:392-480)
+        rays++;
+        bool wasIntersected = scene->intersect(ray, intersection, false);
+        bool emission = true;
+        const int env = scene->envLight;
+        while (depth < maxDepth) {
+            float t;
+            C3 weightMedium;
+            if (med && med->sampleDistance(makeRay(ray.o, ray.d, ray.minT, intersection.t), sampler, t, weightMedium)) {
+                // :62-157
+                throughput = throughput * weightMedium;
+                V3 p = ray.o + t * ray.d;
+                Scene::LightSample info;
+                C3 value = sampleAttenuatedLightDirect(p, sampler, info);
+                if (!isZero(value)) {
+                    float pfPDF = med->hgEval(info.toLight, -ray.d);
+                    float weight = PathTracer::powerHeuristic(info.pdf, pfPDF);
+                    radiance += throughput * value * med->hgEval(info.toLight, -ray.d) * weight;
+                }
+                float pfPdf;
+                V3 wo = med->hgSample(-ray.d, sampler.next2D(), pfPdf);
+                ray = makeRay(p, wo, 0.f, ray.maxT);
+                // evalTransmittanceFromSampledBSDF (:348-423)
+                rays++;
+                wasIntersected = scene->intersect(ray, intersection, false);
+                float tr = med->transmittance(makeRay(ray.o, ray.d, ray.minT, wasIntersected ? intersection.t : ray.maxT),
+                                              sampler);
+                if (wasIntersected) {
+                    int hl = scene->primLight(intersection.prim);
+                    if (hl >= 0) {
+                        C3 Le = scene->meshLe(scene->lights[hl], -ray.d, intersection.sh.n);
+                        float pdfLight = mediumHitLightPdf(hl, intersection.p, info.point, info.normal);
+                        float weight = PathTracer::powerHeuristic(pfPdf, pdfLight);
+                        radiance += throughput * Le * tr * weight;
+                    }
+                }
+            } else {
+                // :159-326
+                if (!wasIntersected) {
+                    if (emission && env >= 0) radiance += throughput * scene->envLe(scene->lights[env], ray.d);
+                    return radiance;
+                }
+                const Hit hit = intersection;
+                if (scene->primMaterial(hit.prim) < 0) {
+                    // :187-195: a shape without a BSDF is passed through, depth unchanged
+                    ray = makeRay(hit.p, ray.d);
+                    rays++;
+                    wasIntersected = scene->intersect(ray, intersection, false);
+                    continue;
+                }
+                const Material& bsdf = scene->mats[scene->primMaterial(hit.prim)];
+                int hitLight = scene->primLight(hit.prim);
+                if (emission && hitLight >= 0)
+                    radiance += throughput * scene->meshLe(scene->lights[hitLight], -ray.d, hit.sh.n);
+                Scene::LightSample info;
+                C3 value = sampleAttenuatedLightDirect(hit.p, sampler, info);
+                if (!isZero(value)) {
+                    BSDFInfo bi;
+                    bi.wi = toLocal(hit.sh, info.toLight);
+                    bi.wo = toLocal(hit.sh, -ray.d);
+                    float cosTheta = std::fabs(bi.wi.z);
+                    float bsdfPDF = bsdfPdf(bsdf, bi);
+                    float weight = PathTracer::powerHeuristic(info.pdf, bsdfPDF);
+                    radiance += throughput * value * bsdfEval(bsdf, bi) * cosTheta * weight;
+                }
+                BSDFInfo bi;
+                bi.wi = toLocal(hit.sh, -ray.d);
+                C3 bsdfValue = bsdfSample(bsdf, bi, sampler.next2D());
+                if (isZero(bsdfValue)) break;
+                V3 pFrom = hit.p;
+                ray = makeRay(hit.p, toWorld(hit.sh, bi.wo));
+                rays++;
+                wasIntersected = scene->intersect(ray, intersection, false);
+                float tr = med ? med->transmittance(
+                                     makeRay(ray.o, ray.d, ray.minT, wasIntersected ? intersection.t : ray.maxT), sampler)
+                               : 1.f;
+                int caught = -1;
+                C3 radianceLight = c3(0.f);
+                if (wasIntersected) {
+                    int hl = scene->primLight(intersection.prim);
+                    if (hl >= 0) {
+                        caught = hl;
+                        radianceLight = scene->meshLe(scene->lights[hl], -ray.d, intersection.sh.n);
+                    }
+                } else if (env >= 0) {
+                    caught = env;
+                    radianceLight = scene->envLe(scene->lights[env], ray.d);
+                }
+                if (caught >= 0) {
+                    float pdfLight = (bi.sampledType & BT_DELTA)
+                                         ? 0.f
+                                         : scene->lightPdf(caught, pFrom, intersection.p, intersection.sh.n, ray.d);
+                    float weight = PathTracer::powerHeuristic(bi.pdf, pdfLight);
+                    radiance += throughput * radianceLight * bsdfValue * tr * weight;
+                    if (caught == env && !med) return radiance;
+                }
+                throughput = throughput * bsdfValue;
+                eta *= bi.relativeEta;
+            }
+            // :331-343
+            if (depth >= minDepth) {
+                float stopVal = std::min(0.95f, maxCoeff(throughput) * eta * eta);
+                if (sampler.next1D() < stopVal)
+                    throughput = throughput / stopVal;
+                else
+                    return radiance;
+            }
+            // :345 traces the same ray again; the result (and a failure's kept t) is unchanged
+            emission = false;
+            depth++;
+        }
+        return radiance;
+    }
+};
+
 // ---------------------------------------------------------------- camera (camera/NoriPerspective.cpp:10-118)
 struct Camera {
     int W = 0, H = 0;
@@ -1779,6 +2060,10 @@ static void runBlock(const Scene& scene, const Camera& cam, const RenderParams& 
     pt.scene = &scene;
     pt.minDepth = rp.minDepth;
     pt.maxDepth = rp.maxDepth;
+    VolPathTracer vpt;
+    vpt.scene = &scene;
+    vpt.minDepth = rp.minDepth;
+    vpt.maxDepth = rp.maxDepth;
     int endX = std::min(cam.W, bx + rp.block), endY = std::min(cam.H, by + rp.block);
     int sx = endX - bx, sy = endY - by;
     const Filter& flt = rp.filter;
@@ -1794,7 +2079,7 @@ static void runBlock(const Scene& scene, const Camera& cam, const RenderParams& 
                 float xx = (float)x + s.x;
                 float yy = (float)y + s.y;
                 Ray ray = cam.getRay(xx, yy);
-                C3 col = pt.li(*sampler, ray);
+                C3 col = scene.integrator == 1 ? vpt.li(*sampler, ray) : pt.li(*sampler, ray);
                 // Screen.cpp:44-97
                 if (isNan(col)) {
                     nans++;
@@ -1831,7 +2116,7 @@ static void runBlock(const Scene& scene, const Camera& cam, const RenderParams& 
                 film.wgt[g] += subw[k];
             }
     }
-    rayCount += pt.rays;
+    rayCount += pt.rays + vpt.rays;
     nanCount += nans;
 }
 
@@ -2198,6 +2483,67 @@ void orc_li(void* sp, const float* ray, const float* u2, int n2, const float* u1
 // seeded (blockX, blockY), threads popping jobs LIFO. Accumulates in double.
 // cam: origin3 target3 up3 fov near far. stats: [rays, nan samples]. Optional block range restricts work
 // to blocks [b0, b1) in creation order (for bounded CPU baselines).
+// Camera medium: density grid (x fastest), albedo rgb, HG g, scale (<= 0: 1).
+int orc_set_medium(void* sp, const float* data, int rx, int ry, int rz, const float* albedo, float g, float scale) {
+    Scene* s = (Scene*)sp;
+    Medium& m = s->medium;
+    m.res[0] = rx; m.res[1] = ry; m.res[2] = rz;
+    m.data.assign(data, data + (size_t)rx * ry * rz);
+    m.albedo = c3(albedo[0], albedo[1], albedo[2]);
+    m.g = g;
+    m.scale = scale > 0.f ? scale : 1.f;
+    m.finalize();
+    return 0;
+}
+// 0 PathTracingMIS, 1 VolPathTracing
+void orc_set_integrator(void* sp, int type) { ((Scene*)sp)->integrator = type; }
+// GridVolume::lookupSigmaT x scale at n points
+void orc_medium_sigma(void* sp, const float* pts, int n, float* out) {
+    const Medium& m = ((Scene*)sp)->medium;
+    for (int i = 0; i < n; i++) out[i] = m.lookupSigmaT(v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
+}
+// Trackers on n rays (ox oy oz dx dy dz minT maxT) with K given 1D draws per ray.
+// op 0 delta tracking: out = (scattered, t, draws used); op 1 ratio tracking: out = (tr, draws used, 0)
+void orc_medium_track(void* sp, int op, const float* rays, int n, const float* u, int K, float* out) {
+    const Medium& m = ((Scene*)sp)->medium;
+    for (int i = 0; i < n; i++) {
+        const float* r = rays + 8 * i;
+        Ray ray = makeRay(v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), r[6], r[7]);
+        ReplaySampler smp;
+        smp.u1 = u + (size_t)i * K;
+        smp.n1 = K;
+        smp.u2 = nullptr;
+        smp.n2 = 0;
+        if (op == 0) {
+            float t;
+            C3 w;
+            bool sc = m.sampleDistance(ray, smp, t, w);
+            out[3 * i] = sc ? 1.f : 0.f;
+            out[3 * i + 1] = t;
+            out[3 * i + 2] = (float)smp.i1;
+        } else {
+            out[3 * i] = m.transmittance(ray, smp);
+            out[3 * i + 1] = (float)smp.i1;
+            out[3 * i + 2] = 0.f;
+        }
+    }
+}
+// Henyey-Greenstein: op 0 eval(wi, wo) -> out[i]; op 1 sample(wi, u) -> out 4 per item (wo, pdf)
+void orc_hg(float g, int op, const float* wi, const float* x, int n, float* out) {
+    Medium m;
+    m.g = g;
+    for (int i = 0; i < n; i++) {
+        V3 a = v3(wi[3 * i], wi[3 * i + 1], wi[3 * i + 2]);
+        if (op == 0) {
+            out[i] = m.hgEval(a, v3(x[3 * i], x[3 * i + 1], x[3 * i + 2]));
+        } else {
+            float pdf;
+            V3 wo = m.hgSample(a, V2{x[2 * i], x[2 * i + 1]}, pdf);
+            out[4 * i] = wo.x; out[4 * i + 1] = wo.y; out[4 * i + 2] = wo.z; out[4 * i + 3] = pdf;
+        }
+    }
+}
+
 // One table lookup of a filter (type, radius x, radius y, param 0, param 1).
 float orc_filter_value(const float* filter, float x, float y) {
     Filter f;

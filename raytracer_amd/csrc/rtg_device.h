@@ -1003,6 +1003,8 @@ struct LightSample {
   float dist;
   f3 intensity;
   float pdf;
+  f3 point;   // sampled point (environment: at the bounding-sphere radius)
+  f3 normal;  // its normal (environment: zero, the default of LightSamplingInfos)
 };
 // light/MeshLight.cpp:32-64 with geometricShape/Mesh.cpp:38-68, Mapping.h:94-105
 __device__ __forceinline__ LightSample mesh_light_sample(const DevScene& S, const DevLight& L, f3 pFrom, float ux,
@@ -1024,6 +1026,8 @@ __device__ __forceinline__ LightSample mesh_light_sample(const DevScene& S, cons
     n = normalized(cross(p1 - p0, p2 - p0));
     n = normalized(n);
   }
+  ls.point = pt;
+  ls.normal = n;
   ls.dir = pt - pFrom;
   ls.dist = norm(ls.dir);
   ls.dir = ls.dir / ls.dist;
@@ -1119,12 +1123,145 @@ __device__ __forceinline__ LightSample env_sample(const DevScene& S, f3 pFrom, f
   ls.dir = mk(sinPhi * sinTheta, cosTheta, -cosPhi * sinTheta);
   ls.dist = S.env_radius;
   ls.dir = normalized(ls.dir);
+  ls.point = pFrom + S.env_radius * ls.dir;
+  ls.normal = mk(0.f, 0.f, 0.f);
   int cc = min(max(col, 0), W - 1), rr = min(max(row, 0), H - 1);
   ls.intensity = ld3(S.env_map + 3 * (rr * W + cc)) * S.env_factor;
   ls.pdf = luminance(ls.intensity) * S.env_weight[row] /
            (S.env_marg_sum * (2.f * PI / (float)W) * (PI / (float)H));
   ls.pdf /= smax(fabsf(sinTheta), EPS);
   return ls;
+}
+
+// ---------------------------------------------------------------- participating medium
+// The camera medium: medium/Heterogeneous.cpp over volume/GridVolume.cpp with
+// medium/HGPhaseFunction.cpp. A camera medium is never placed in an owner box, so its
+// world-to-array transform is the identity and the grid spans [0,1]^3 in world space.
+
+// volume/GridVolume.cpp:118-148 (x Heterogeneous.cpp:403-406 scale): trilinear, 0 outside the
+// cells [0, res-1) on each axis
+__device__ __forceinline__ float medium_sigma(const DevScene& S, f3 p) {
+  const float qx = p.x * (float)S.gres[0], qy = p.y * (float)S.gres[1], qz = p.z * (float)S.gres[2];
+  const float fx = floorf(qx), fy = floorf(qy), fz = floorf(qz);
+  if (!(fx >= 0.f && fy >= 0.f && fz >= 0.f && fx < (float)(S.gres[0] - 1) && fy < (float)(S.gres[1] - 1) &&
+        fz < (float)(S.gres[2] - 1)))
+    return 0.f;
+  const size_t row = (size_t)S.gres[0], slab = row * (size_t)S.gres[1];
+  const float* g = S.grid + (size_t)fz * slab + (size_t)fy * row + (size_t)fx;
+  const float d000 = g[0], d001 = g[1], d010 = g[row], d011 = g[row + 1];
+  const float d100 = g[slab], d101 = g[slab + 1], d110 = g[slab + row], d111 = g[slab + row + 1];
+  const float w1x = qx - fx, w1y = qy - fy, w1z = qz - fz;
+  const float w0x = 1.f - w1x, w0y = 1.f - w1y, w0z = 1.f - w1z;
+  const float v = ((d000 * w0x + d001 * w1x) * w0y + (d010 * w0x + d011 * w1x) * w1y) * w0z +
+                  ((d100 * w0x + d101 * w1x) * w0y + (d110 * w0x + d111 * w1x) * w1y) * w1z;
+  return v * S.gscale;
+}
+
+// core/BoundingBox.h:154-183 on the box [0,1]^3: unclipped entry/exit distances; true when
+// [minT, maxT] reaches the box
+__device__ __forceinline__ bool unit_box(f3 o, f3 d, float minT, float maxT, float& nearT, float& farT) {
+  nearT = -MAX_REAL;
+  farT = MAX_REAL;
+  const float oo[3] = {o.x, o.y, o.z}, dd[3] = {d.x, d.y, d.z};
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    if (dd[i] == 0.f) {
+      if (oo[i] < 0.f || oo[i] > 1.f) return false;
+    } else {
+      const float inv = 1.f / dd[i];
+      float t1 = (0.f - oo[i]) * inv, t2 = (1.f - oo[i]) * inv;
+      if (t1 > t2) {
+        const float x = t1;
+        t1 = t2;
+        t2 = x;
+      }
+      nearT = fmaxf(t1, nearT);
+      farT = fminf(t2, farT);
+      if (!(nearT <= farT)) return false;
+    }
+  }
+  return minT <= farT && maxT >= nearT;
+}
+
+// Delta tracking (Heterogeneous.cpp:491-546): true on a scattering event at distance t. The
+// reference starts its tentative collisions at minT; before the box entry the density is 0 and
+// every one of them is rejected, so starting at max(minT, entry) draws the same distribution.
+// At most kMediumSteps tentative collisions per call (the reference has no bound; the expected
+// count is sigma_max times the length inside the box).
+constexpr int kMediumSteps = 16384;
+template <class U>
+__device__ __forceinline__ bool medium_sample_distance(const DevScene& S, f3 o, f3 d, float minT, float maxT, U& u,
+                                                       float& t) {
+  float nearT, farT;
+  if (!unit_box(o, d, minT, maxT, nearT, farT)) return false;
+  const float tMax = fminf(maxT, farT);
+  t = fmaxf(minT, nearT);
+  for (int it = 0; it < kMediumSteps; it++) {
+    t += -logf(1.f - u()) / S.sigma_max;
+    if (t > tMax) return false;
+    const float sigma = medium_sigma(S, o + t * d);
+    if (u() < sigma / S.sigma_max) return sigma != 0.f;
+  }
+  return false;
+}
+
+// Ratio tracking with Russian roulette below 0.1 (Heterogeneous.cpp:419-489), including its
+// segment length maxT - minT and its limit of 1025 steps
+template <class U>
+__device__ __forceinline__ float medium_transmittance(const DevScene& S, f3 o, f3 d, float minT, float maxT, U& u) {
+  float tMax = maxT - minT;
+  float nearT, farT;
+  if (!unit_box(o, d, minT, maxT, nearT, farT)) return 1.f;
+  tMax = fminf(tMax, farT);
+  float s = fmaxf(nearT, minT);
+  float tr = 1.f;
+  for (int breaker = 0;; breaker++) {
+    s += -logf(1.f - u()) / S.sigma_max;
+    if (s >= tMax) return tr;
+    const float sigma = medium_sigma(S, o + s * d);
+    tr *= 1.f - fmaxf(0.f, sigma / S.sigma_max);
+    if (tr < 0.1f) {
+      const float q = fmaxf(0.05f, 1.f - tr);
+      if (u() < q) return 0.f;
+      tr /= 1.f - q;
+    }
+    if (breaker >= 1024) return 0.f;
+  }
+}
+
+// HGPhaseFunction.cpp:61-79: cos = (-wi).wo
+__device__ __forceinline__ float hg_eval(float g, f3 wi, f3 wo) {
+  const float c = dot(-wi, wo);
+  return (0.0795774715459476678844f * (1.f - g * g)) / powf(1.f + g * g - 2.f * g * c, 1.5f);
+}
+// HGPhaseFunction.cpp:27-45 as written: the inversion divides (1 - g^2)^2 by (1 - g + 2 g u) once,
+// and the direction is built around world +y (core/Geometry.h:181-188), not around wi; weight 1
+__device__ __forceinline__ f3 hg_sample(float g, f3 wi, float ux, float uy, float& pdf) {
+  float cosTheta;
+  if (g < EPS)
+    cosTheta = 1.f - 2.f * ux;
+  else
+    cosTheta = (1.f + g * g - ((1.f - g * g) * (1.f - g * g) / (1.f - g + 2.f * g * ux))) / (2.f * g);
+  const float theta = acosf(cosTheta);
+  const float phi = 2.f * PI * uy;
+  const float sinT = sinf(theta), cosT = cosf(theta), cosP = cosf(phi), sinP = sinf(phi);
+  const f3 wo = mk(sinP * sinT, cosT, -cosP * sinT);
+  pdf = hg_eval(g, wi, wo);
+  return wo;
+}
+
+// Mesh::pdf (geometricShape/Mesh.cpp:70-93) as VolPathTracing.cpp:152 calls it after a medium
+// vertex: from the point where the sampled ray met the light to the point that light sampling
+// drew at that vertex. On a planar emitter the two points are coplanar and the cosine is 0 in
+// exact arithmetic (pdf 0, MIS weight 1); rounding makes the reference's sign arbitrary, so a
+// cosine within 1e-4 of 0 is taken as 0 here.
+__device__ __forceinline__ float medium_hit_light_pdf(const DevLight& L, f3 pFrom, f3 sampled, f3 n) {
+  f3 v = pFrom - sampled;
+  const float dist = norm(v);
+  v = normalized(v);
+  const float c = dot(n, v);
+  if (c > 1e-4f) return dist * dist / (L.area * c);
+  return 0.f;
 }
 
 }  // namespace dev

@@ -477,6 +477,35 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   D.env_row_cdf = upload(ctx, row_cdf, own, bytes);
   D.env_marg_cdf = upload(ctx, marg_cdf, own, bytes);
   D.env_weight = upload(ctx, env_w, own, bytes);
+  // camera medium: GridVolume data as given (GridVolume.cpp:7-64), majorant of
+  // Heterogeneous.cpp:40 (GridVolume::getMaxSigmaT x scale)
+  if (desc->medium_density) {
+    const int* r = desc->medium_res;
+    if (r[0] < 2 || r[1] < 2 || r[2] < 2) fail(RTG_ERR_INVALID, "medium grid must be at least 2x2x2");
+    const size_t nv = (size_t)r[0] * r[1] * r[2];
+    if (nv > ((size_t)1 << 31)) fail(RTG_ERR_INVALID, "medium grid above 2^31 values");
+    float mx = desc->medium_density[0];
+    for (size_t i = 0; i < nv; i++) {
+      const float v = desc->medium_density[i];
+      if (!std::isfinite(v)) fail(RTG_ERR_INVALID, "medium density must be finite");
+      if (v > mx) mx = v;
+    }
+    const float scale = desc->medium_scale > 0.f ? desc->medium_scale : 1.f;
+    if (!std::isfinite(desc->medium_g) || desc->medium_g >= 1.f || desc->medium_g <= -1.f)
+      fail(RTG_ERR_INVALID, "phase function g must lie in (-1, 1)");
+    float* g = (float*)dev_alloc(ctx, nv * sizeof(float), own);
+    hip_check(hipMemcpyAsync(g, desc->medium_density, nv * sizeof(float), hipMemcpyHostToDevice, ctx->stream),
+              "medium upload");
+    bytes += nv * sizeof(float);
+    D.grid = g;
+    for (int k = 0; k < 3; k++) {
+      D.gres[k] = r[k];
+      D.albedo[k] = desc->medium_albedo[k];
+    }
+    D.gscale = scale;
+    D.sigma_max = mx * scale;
+    D.hg_g = desc->medium_g;
+  }
   hip_check(hipMemcpyAsync(ctx->scenes_dev + slot, &D, sizeof(DevScene), hipMemcpyHostToDevice, ctx->stream),
             "scene table");
   hip_check(hipStreamSynchronize(ctx->stream), "scene upload");
@@ -520,6 +549,8 @@ void ensure_pool(rtg_ctx* ctx, int capacity) {
   P.L = (float4*)A(n * 16);
   P.sh_d = (float4*)A(n * 16);
   P.sh_c = (float4*)A(n * 16);
+  P.vol_a = (float4*)A(n * 16);
+  P.vol_b = (float4*)A(n * 16);
   P.q_ext = (uint32_t*)A(n * 4);
   P.q_shade = (uint32_t*)A(n * 4 * 4);
   P.q_shadow = (uint32_t*)A(n * 4);
@@ -813,6 +844,9 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   launch_reset_pool(ctx->pool, ctx->stream);
   hip_check(hipMemcpyAsync(ctx->jobs_dev, up_job, sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream), "job upload");
   const bool guided = desc->guiding != 0;
+  const bool vol = desc->integrator == RTG_INTEGRATOR_VOLPATH;
+  if (desc->integrator != RTG_INTEGRATOR_PATH_MIS && !vol) fail(RTG_ERR_INVALID, "unknown integrator");
+  if (vol && guided) fail(RTG_ERR_INVALID, "guiding is not available with the volumetric integrator");
   GuideState* gs = nullptr;
   DevGuide G;
   std::memset(&G, 0, sizeof(G));
@@ -836,7 +870,7 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   const int batch = 4;
   const int kEv = 9;             // events per iteration (8 stages)
   const int kTrainEv = 2 * kMaxStepsPerBatch;
-  const bool prof = (ctx->profiling & RTG_PROFILE_EVENTS) != 0;
+  const bool prof = (ctx->profiling & RTG_PROFILE_EVENTS) != 0 && !vol;
   double stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
   double train_ms = 0.0;
   uint64_t train_steps = 0, trained_upto = 0, ring_total = 0;
@@ -850,9 +884,13 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   }
   int pending_train = 0;  // training steps enqueued before the batch now in flight
   while (true) {
-    for (int b = 0; b < batch; b++)
-      launch_iteration(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, 1, cfg, ctx->stream,
-                       prof ? &ctx->events[(size_t)b * kEv] : nullptr, guided ? &G : nullptr);
+    for (int b = 0; b < batch; b++) {
+      if (vol)
+        launch_iteration_vol(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, 1, cfg, ctx->stream);
+      else
+        launch_iteration(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, 1, cfg, ctx->stream,
+                         prof ? &ctx->events[(size_t)b * kEv] : nullptr, guided ? &G : nullptr);
+    }
     iters += batch;
     hip_check(hipGetLastError(), "kernel launch");
     hip_check(hipMemcpyAsync(done, &ctx->jobs_dev[0].done, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream),
@@ -1016,6 +1054,42 @@ int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int3
       if (p) hip_check(hipMemcpyAsync(p, dpp, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "p");
       if (nrm) hip_check(hipMemcpyAsync(nrm, dn, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "n");
       hip_check(hipStreamSynchronize(ctx->stream), "trace");
+    } catch (...) {
+      for (void* q : own) (void)hipFree(q);
+      throw;
+    }
+    for (void* q : own) (void)hipFree(q);
+  })
+}
+
+// values per item of rtg_medium_batch's input and output, by op
+static const int kMediumIn[5] = {3, 8, 8, 6, 3}, kMediumOut[5] = {1, 3, 3, 1, 4};
+
+int rtg_medium_batch(rtg_ctx* ctx, rtg_scene* scene, int32_t op, const float* in, int32_t n, const float* u,
+                     int32_t k, float* out) {
+  if (!ctx) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  RTG_GUARD(ctx, {
+    if (!scene || !in || !out || n < 0 || op < 0 || op > 4) fail(RTG_ERR_INVALID, "bad medium arguments");
+    if (!scene->dev.grid) fail(RTG_ERR_INVALID, "the scene has no medium");
+    if ((op == 1 || op == 2) && (k < 0 || (k > 0 && !u))) fail(RTG_ERR_INVALID, "tracking needs k draws per ray");
+    if (op == 4 && (!u || k != 2)) fail(RTG_ERR_INVALID, "phase sampling needs 2 draws per item");
+    if (n == 0) return RTG_OK;
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const size_t nin = (size_t)n * kMediumIn[op];
+    const size_t nout = (size_t)n * kMediumOut[op];
+    const size_t nu = (op == 1 || op == 2 || op == 4) ? (size_t)n * k : 0;
+    std::vector<void*> own;
+    float* din = (float*)dev_alloc(ctx, nin * 4, own);
+    float* du = (float*)dev_alloc(ctx, std::max<size_t>(nu, 1) * 4, own);
+    float* dout = (float*)dev_alloc(ctx, nout * 4, own);
+    try {
+      hip_check(hipMemcpyAsync(din, in, nin * 4, hipMemcpyHostToDevice, ctx->stream), "in");
+      if (nu) hip_check(hipMemcpyAsync(du, u, nu * 4, hipMemcpyHostToDevice, ctx->stream), "u");
+      launch_medium_tap(ctx->scenes_dev + scene->slot, op, din, n, du, nu ? k : 0, dout, ctx->stream);
+      hip_check(hipGetLastError(), "medium launch");
+      hip_check(hipMemcpyAsync(out, dout, nout * 4, hipMemcpyDeviceToHost, ctx->stream), "out");
+      hip_check(hipStreamSynchronize(ctx->stream), "medium");
     } catch (...) {
       for (void* q : own) (void)hipFree(q);
       throw;

@@ -81,6 +81,12 @@ struct DevScene {
   int32_t env_w, env_h;
   float env_factor, env_marg_sum, env_radius;
   int32_t pad;
+  // camera medium (medium/Heterogeneous.cpp, volume/GridVolume.cpp, medium/HGPhaseFunction.cpp):
+  // density grid over the world box [0,1]^3, x fastest; sigma_max = max density x scale
+  const float* grid;         // null: no medium
+  int32_t gres[3];
+  float gscale, sigma_max, hg_g;
+  float albedo[3];
 };
 
 // Per-render (per-request) state, one slot of the job table.
@@ -129,6 +135,8 @@ struct DevPool {
   float4* L;            // radiance, Russian-roulette divisor of the last step
   float4* sh_d;         // shadow direction, maxT
   float4* sh_c;         // shadow contribution if unoccluded
+  float4* vol_a;        // volumetric paths: light-sample point of the last medium vertex, last hit t
+  float4* vol_b;        // volumetric paths: light-sample normal of the last medium vertex
   // queues
   uint32_t* q_ext;      // extension queue
   uint32_t* q_shade;    // 4 queues of capacity each: diffuse, conductor, dielectric, miss
@@ -204,9 +212,14 @@ static const int kMaxStack = 96;
 void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, const int32_t* order_dev,
                       int norder, const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev = nullptr,
                       const DevGuide* guide = nullptr);
+// One iteration of the volumetric integrator (regen, extension traversal, k_vol, shadow rays).
+void launch_iteration_vol(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, const int32_t* order_dev,
+                          int norder, const LaunchCfg& cfg, hipStream_t stream);
 void launch_reset_pool(const DevPool& pool, hipStream_t stream);
 void launch_trace_tap(const DevScene* scene_dev, const float* rays, int n, int any_hit, int32_t* prim,
                       float* t, float* uv, float* p, float* nrm, const LaunchCfg& cfg, hipStream_t stream);
+void launch_medium_tap(const DevScene* scene_dev, int op, const float* in, int n, const float* u, int k, float* out,
+                       hipStream_t stream);
 void launch_bsdf_tap(const DevMaterial* mat, int op, const float* wi, const float* wo_or_u, int n,
                      float* out_dir, float* f, float* pdf, float* rel_eta, hipStream_t stream);
 void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream);

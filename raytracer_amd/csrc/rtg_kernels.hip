@@ -781,6 +781,237 @@ __global__ void __launch_bounds__(256) k_shade_miss(DevPool P, const DevScene* s
   }
 }
 
+// ---------------------------------------------------------------- volumetric path tracing
+// VolPathTracing::li (integrator/VolPathTracing.cpp:25-346) for scenes whose only medium is the
+// camera medium and whose shapes all carry a BSDF (no medium transitions, no null-BSDF
+// boundaries, no emissive media). One kernel per iteration after the shared closest-hit
+// traversal; per extension-queue entry it
+//   1. adds the emission met by the direction sampled at the previous vertex, weighted by the
+//      ratio-tracking transmittance of that segment and the MIS weight (:112-157 after a medium
+//      vertex, :257-325 after a surface vertex), and applies that vertex's throughput update and
+//      roulette divisor;
+//   2. runs delta tracking along the segment (:60): a collision is a medium vertex (albedo,
+//      light sampling attenuated by ratio tracking and weighted against the phase function,
+//      phase sampling), otherwise the segment ends at the surface (depth-0 emission, attenuated
+//      light sampling with the BSDF MIS weight, BSDF sampling) or escapes (depth-0 environment);
+//   3. plays Russian roulette on max(throughput) * eta^2 (:331-343).
+// A segment that missed the scene is tracked up to the distance of the path's last hit, as the
+// reference's Intersection keeps its previous t when computeIntersection fails.
+static const uint32_t FL_PEND = 1u << 22;  // the sampled direction's emission is still to be added
+static const uint32_t FL_PMED = 1u << 23;  // ... and that direction was sampled at a medium vertex
+constexpr uint32_t kVolDim = 1u << 24;      // raw 1D draws of depth k: kVolDim + k * kVolDimDepth + j
+constexpr uint32_t kVolDimDepth = 1u << 16;
+
+struct VolDraws {
+  const PathRng* r;
+  uint32_t k;
+  __device__ __forceinline__ float operator()() { return sample1d(*r, k++); }
+};
+
+__device__ __forceinline__ f3 bsdf_eval_any(const DevMaterial& m, f3 wi, f3 wo) {
+  return m.cls == 0 ? bsdf_eval<0>(m, wi, wo) : m.cls == 1 ? bsdf_eval<1>(m, wi, wo) : bsdf_eval<2>(m, wi, wo);
+}
+__device__ __forceinline__ float bsdf_pdf_any(const DevMaterial& m, f3 wi, f3 wo) {
+  return m.cls == 0 ? bsdf_pdf<0>(m, wi, wo) : m.cls == 1 ? bsdf_pdf<1>(m, wi, wo) : bsdf_pdf<2>(m, wi, wo);
+}
+__device__ __forceinline__ bool bsdf_sample_any(const DevMaterial& m, f3 wi, float ux, float uy, BsdfSample& bs) {
+  return m.cls == 0   ? bsdf_sample<0>(m, wi, ux, uy, bs)
+         : m.cls == 1 ? bsdf_sample<1>(m, wi, ux, uy, bs)
+                      : bsdf_sample<2>(m, wi, ux, uy, bs);
+}
+__device__ __forceinline__ float max3(f3 v) { return fmaxf(v.x, fmaxf(v.y, v.z)); }
+
+__global__ void __launch_bounds__(256) k_vol(DevPool P, const DevScene* scenes, const DevJob* jobs) {
+  uint32_t* C = P.counters;
+  const uint32_t n = C[C_EXT];
+  const uint32_t stride = gridDim.x * blockDim.x;
+  for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
+    const uint32_t slot = P.q_ext[e];
+    const uint32_t f = P.flags[slot];
+    const uint32_t depth = fl_depth(f);
+    const DevJob& J = jobs[fl_job(f)];
+    const DevScene& S = scenes[J.scene];
+    const float4 ro = P.ray_o[slot], rd = P.ray_d[slot], hv = P.hit[slot];
+    float4 T = P.T[slot], L = P.L[slot];
+    const float4 F = P.F[slot];
+    float4 va = P.vol_a[slot];
+    const f3 o = mk(ro.x, ro.y, ro.z), d = mk(rd.x, rd.y, rd.z);
+    const bool medium = S.grid != nullptr;
+    HitRec h;
+    h.t = hv.x; h.b1 = hv.y; h.b2 = hv.z; h.prim = __float_as_int(hv.w);
+    const bool hit = h.prim >= 0;
+    Surface sf;
+    if (hit) sf = surface_at(S, h, o, d);
+    const float tEnd = hit ? h.t : (depth == 0 ? MAX_REAL : va.w);
+    if (hit || depth == 0) va.w = tEnd;
+    const PathRng rng = make_rng(J.seed, P.pixel[slot], P.sample[slot], J);
+    VolDraws u{&rng, kVolDim + depth * kVolDimDepth};
+    f3 tp = mk(T.x, T.y, T.z);
+    // 1. emission along the previous vertex's sampled direction
+    if (f & FL_PEND) {
+      const bool after_medium = (f & FL_PMED) != 0;
+      bool caught = false;
+      f3 Le = mk(0.f, 0.f, 0.f);
+      float pl = 0.f;
+      if (hit) {
+        if (sf.light != 0xffffffffu) {
+          caught = true;
+          const DevLight& LL = S.lights[sf.light];
+          if (dot(-d, sf.sh.n) > 0.f) Le = mat_col(LL.radiance);
+          if (F.w >= 0.f) {
+            if (after_medium) {
+              const float4 vb = P.vol_b[slot];
+              pl = medium_hit_light_pdf(LL, sf.p, mk(va.x, va.y, va.z), mk(vb.x, vb.y, vb.z));
+            } else {
+              pl = mesh_light_pdf(LL, o, sf.p, sf.sh.n);
+            }
+          }
+        }
+      } else if (!after_medium && S.env_light >= 0) {
+        caught = true;
+        Le = env_le(S, d);
+        if (F.w >= 0.f) pl = env_pdf(S, d);
+      }
+      if (caught) {
+        // evalTransmittanceFromSampledBSDF (:372-423): ratio tracking over [minT, hit or maxT]
+        const float tr = medium ? medium_transmittance(S, o, d, ro.w, hit ? h.t : rd.w, u) : 1.f;
+        const float w = power_heuristic(fabsf(F.w), pl);
+        const f3 c = (cmul(cmul(tp, Le), mk(F.x, F.y, F.z)) * tr) * w;
+        L.x += c.x; L.y += c.y; L.z += c.z;
+      }
+      tp = cmul(tp, mk(F.x, F.y, F.z)) / L.w;
+    }
+    bool done = (f & FL_LAST) || (int)depth >= J.max_depth;
+    uint32_t nf = f & ~(FL_PEND | FL_PMED | FL_EMIT);
+    bool shadow = false;
+    f3 vp = o, nd = d;
+    float nmin = 0.f;
+    float4 nF = F;
+    bool pass = false;
+    if (!done) {
+      const bool emission = depth == 0;  // RadianceType::EMISSION in the first iteration only
+      float ts = 0.f;
+      const bool scat = medium && medium_sample_distance(S, o, d, ro.w, tEnd, u, ts);
+      if (scat) {
+        // medium vertex (:60-103)
+        tp = cmul(tp, mk(S.albedo[0], S.albedo[1], S.albedo[2]));
+        vp = o + ts * d;
+        if (S.nlights > 0) {
+          float ux, uy;
+          sample2d(rng, 1u + 2u * depth, ux, uy);
+          float lw;
+          const int li = cdf_sample_reuse(S.light_cdf, S.nlights + 1, ux, lw);
+          const DevLight& LL = S.lights[li];
+          const LightSample ls =
+              (LL.type == LIGHT_ENV) ? env_sample(S, vp, ux, uy) : mesh_light_sample(S, LL, vp, ux, uy);
+          va.x = ls.point.x; va.y = ls.point.y; va.z = ls.point.z;
+          P.vol_b[slot] = make_float4(ls.normal.x, ls.normal.y, ls.normal.z, 0.f);
+          if (ls.pdf > 0.f && luminance(ls.intensity) > 0.f) {
+            const float tr = medium_transmittance(S, vp, ls.dir, EPS, ls.dist - EPS, u);
+            const float lpdf = ls.pdf * lw;
+            const f3 value = (tr * ls.intensity) / lpdf;
+            if (!is_zero(value)) {
+              const float pf = hg_eval(S.hg_g, ls.dir, -d);
+              const float w = power_heuristic(lpdf, pf);
+              const f3 c = (cmul(tp, value) * pf) * w;
+              P.sh_d[slot] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - EPS);
+              P.sh_c[slot] = make_float4(c.x, c.y, c.z, 0.f);
+              shadow = true;
+            }
+          }
+        }
+        float bx, by, ppdf;
+        sample2d(rng, 2u + 2u * depth, bx, by);
+        nd = hg_sample(S.hg_g, -d, bx, by, ppdf);
+        nmin = 0.f;
+        nF = make_float4(1.f, 1.f, 1.f, ppdf);
+        nf |= FL_PEND | FL_PMED;
+      } else if (!hit) {
+        // escaped (:170-183)
+        if (emission && S.env_light >= 0) {
+          const f3 Le = env_le(S, d);
+          L.x += tp.x * Le.x; L.y += tp.y * Le.y; L.z += tp.z * Le.z;
+        }
+        done = true;
+      } else if (sf.material == 0xffffffffu) {
+        // a shape without a BSDF is passed through (:187-195): same direction, same depth, no roulette
+        vp = sf.p;
+        nmin = EPS;
+        nF = make_float4(1.f, 1.f, 1.f, 0.f);
+        pass = true;
+      } else {
+        // surface vertex (:185-330)
+        vp = sf.p;
+        if (emission && sf.light != 0xffffffffu && dot(-d, sf.sh.n) > 0.f) {
+          const f3 c = cmul(tp, mat_col(S.lights[sf.light].radiance));
+          L.x += c.x; L.y += c.y; L.z += c.z;
+        }
+        const DevMaterial& M = S.mats[sf.material];
+        const f3 wo_l = to_local(sf.sh, -d);
+        if (S.nlights > 0 && !is_delta(M)) {
+          float ux, uy;
+          sample2d(rng, 1u + 2u * depth, ux, uy);
+          float lw;
+          const int li = cdf_sample_reuse(S.light_cdf, S.nlights + 1, ux, lw);
+          const DevLight& LL = S.lights[li];
+          const LightSample ls =
+              (LL.type == LIGHT_ENV) ? env_sample(S, sf.p, ux, uy) : mesh_light_sample(S, LL, sf.p, ux, uy);
+          if (ls.pdf > 0.f && luminance(ls.intensity) > 0.f) {
+            const float tr = medium ? medium_transmittance(S, sf.p, ls.dir, EPS, ls.dist - EPS, u) : 1.f;
+            const float lpdf = ls.pdf * lw;
+            const f3 value = (tr * ls.intensity) / lpdf;
+            if (!is_zero(value)) {
+              const f3 wi_l = to_local(sf.sh, ls.dir);
+              const float cosT = fabsf(wi_l.z);
+              const float w = power_heuristic(lpdf, bsdf_pdf_any(M, wi_l, wo_l));
+              const f3 c = (cmul(cmul(tp, value), bsdf_eval_any(M, wi_l, wo_l)) * cosT) * w;
+              P.sh_d[slot] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - EPS);
+              P.sh_c[slot] = make_float4(c.x, c.y, c.z, 0.f);
+              shadow = true;
+            }
+          }
+        }
+        float bx, by;
+        sample2d(rng, 2u + 2u * depth, bx, by);
+        BsdfSample bs;
+        const bool ok = bsdf_sample_any(M, wo_l, bx, by, bs);
+        if (!ok || is_zero(bs.f)) {
+          done = true;
+        } else {
+          nd = to_world(sf.sh, bs.wo);
+          nmin = EPS;
+          nF = make_float4(bs.f.x, bs.f.y, bs.f.z, is_delta(M) ? -bs.pdf : bs.pdf);
+          T.w *= bs.rel_eta;
+          nf |= FL_PEND;
+        }
+      }
+      if (!done && pass) {
+        L.w = 1.f;
+        P.ray_d[slot] = make_float4(nd.x, nd.y, nd.z, MAX_REAL);
+      } else if (!done) {
+        float q = 1.f;
+        if ((int)depth >= J.min_depth) {
+          const float stop = smin(0.95f, max3(cmul(tp, mk(nF.x, nF.y, nF.z))) * T.w * T.w);
+          if (u() < stop) q = stop;
+          else nf |= FL_LAST;
+        }
+        L.w = q;
+        nf = (nf & ~DEPTH_MASK) | (depth + 1u);
+        P.ray_d[slot] = make_float4(nd.x, nd.y, nd.z, MAX_REAL);
+      }
+      P.ray_o[slot] = make_float4(vp.x, vp.y, vp.z, nmin);
+    }
+    P.T[slot] = make_float4(tp.x, tp.y, tp.z, T.w);
+    P.F[slot] = nF;
+    P.L[slot] = L;
+    P.vol_a[slot] = va;
+    P.flags[slot] = nf;
+    P.state[slot] = done ? ST_DONE : ST_ACTIVE;
+    P.shflag[slot] = shadow ? 1 : 0;
+    P.cls_q[e] = 0;  // the shadow compaction reads every entry's flag
+  }
+}
+
 // ---------------------------------------------------------------- compact: shadow queue
 __global__ void __launch_bounds__(256) k_compact_shadow(DevPool P) {
   __shared__ uint32_t s_w[4];
@@ -991,7 +1222,52 @@ __global__ void k_bsdf_tap(const DevMaterial* mat, int op, const float* wi, cons
   }
 }
 
+// Camera-medium parity tap (the C ABI's rtg_medium_batch): one item per thread.
+struct TapDraws {
+  const float* u;
+  int k, i;
+  __device__ __forceinline__ float operator()() {
+    const float v = i < k ? u[i] : 0.5f;
+    i++;
+    return v;
+  }
+};
+__global__ void k_medium_tap(const DevScene* scene, int op, const float* in, int n, const float* u, int k, float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DevScene& S = *scene;
+  if (op == 0) {
+    out[i] = medium_sigma(S, mk(in[3 * i], in[3 * i + 1], in[3 * i + 2]));
+  } else if (op == 1 || op == 2) {
+    const float* r = in + 8 * i;
+    const f3 o = mk(r[0], r[1], r[2]), d = mk(r[3], r[4], r[5]);
+    TapDraws dr{u + (size_t)i * k, k, 0};
+    if (op == 1) {
+      float t = r[7];
+      const bool sc = medium_sample_distance(S, o, d, r[6], r[7], dr, t);
+      out[3 * i] = sc ? 1.f : 0.f;
+      out[3 * i + 1] = sc ? t : fminf(t, r[7]);
+      out[3 * i + 2] = (float)dr.i;
+    } else {
+      out[3 * i] = medium_transmittance(S, o, d, r[6], r[7], dr);
+      out[3 * i + 1] = (float)dr.i;
+      out[3 * i + 2] = 0.f;
+    }
+  } else if (op == 3) {
+    const float* r = in + 6 * i;
+    out[i] = hg_eval(S.hg_g, mk(r[0], r[1], r[2]), mk(r[3], r[4], r[5]));
+  } else {
+    float pdf;
+    const f3 wo = hg_sample(S.hg_g, mk(in[3 * i], in[3 * i + 1], in[3 * i + 2]), u[2 * i], u[2 * i + 1], pdf);
+    out[4 * i] = wo.x; out[4 * i + 1] = wo.y; out[4 * i + 2] = wo.z; out[4 * i + 3] = pdf;
+  }
+}
+
 // ---------------------------------------------------------------- launchers
+void launch_medium_tap(const DevScene* scene_dev, int op, const float* in, int n, const float* u, int k, float* out,
+                       hipStream_t stream) {
+  hipLaunchKernelGGL(k_medium_tap, dim3((n + 127) / 128), dim3(128), 0, stream, scene_dev, op, in, n, u, k, out);
+}
 static size_t stack_bytes(int block) { return (size_t)block * kLdsStack * 8u; }
 
 void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, const int32_t* order_dev,
@@ -1045,6 +1321,21 @@ void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* j
   hipLaunchKernelGGL(k_shadow, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
                      cfg.gstack, cfg.stack, cfg.refill);
   if (ev) (void)hipEventRecord(ev[8], stream);
+}
+
+void launch_iteration_vol(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, const int32_t* order_dev,
+                          int norder, const LaunchCfg& cfg, hipStream_t stream) {
+  const int chunks = (pool.capacity + kRegenSlots - 1) / kRegenSlots;
+  DevGuide G;
+  memset(&G, 0, sizeof(G));
+  G.job = -1;
+  hipLaunchKernelGGL(k_regen<false>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, norder, G);
+  hipLaunchKernelGGL(k_extend, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev, jobs_dev,
+                     cfg.gstack, cfg.stack, cfg.refill);
+  hipLaunchKernelGGL(k_vol, dim3(cfg.grid), dim3(256), 0, stream, pool, scenes_dev, jobs_dev);
+  hipLaunchKernelGGL(k_compact_shadow, dim3(chunks), dim3(256), 0, stream, pool);
+  hipLaunchKernelGGL(k_shadow, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev,
+                     jobs_dev, cfg.gstack, cfg.stack, cfg.refill);
 }
 
 void launch_reset_pool(const DevPool& pool, hipStream_t stream) {

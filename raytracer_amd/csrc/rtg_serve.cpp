@@ -210,6 +210,8 @@ int rtg_submit(rtg_server* s, rtg_scene* scene, const rtg_render_desc* desc, int
   try {
     // validate now so a bad request fails at submission
     if (desc->guiding) fail(RTG_ERR_INVALID, "guided renders are not served (use rtg_render)");
+    if (desc->integrator != RTG_INTEGRATOR_PATH_MIS)
+      fail(RTG_ERR_INVALID, "volumetric renders are not served (use rtg_render)");
     (void)make_job(ctx, scene, *desc, nullptr);
     std::unique_ptr<Request> r(new Request());
     r->id = s->next_id++;

@@ -18,6 +18,7 @@ DIST_BECKMANN, DIST_GGX = 0, 2
 FILTER_BOX, FILTER_GAUSSIAN, FILTER_MITCHELL = 0, 1, 2
 LIGHTS_ALL, LIGHTS_UNIFORM, LIGHTS_WEIGHTED = 0, 1, 2
 SAMPLER_RANDOM, SAMPLER_STRATIFIED = 0, 1
+INTEGRATOR_PATH_MIS, INTEGRATOR_VOLPATH = 0, 1
 
 
 class rtg_material(C.Structure):
@@ -45,6 +46,8 @@ class rtg_scene_desc(C.Structure):
         ("materials", C.c_void_p), ("num_materials", C.c_int32), ("meshes", C.c_void_p), ("num_meshes", C.c_int32),
         ("spheres", C.c_void_p), ("num_spheres", C.c_int32), ("env_rgb", C.c_void_p), ("env_width", C.c_int32),
         ("env_height", C.c_int32), ("env_scale", C.c_float), ("light_strategy", C.c_int32),
+        ("medium_density", C.c_void_p), ("medium_res", C.c_int32 * 3), ("medium_albedo", C.c_float * 3),
+        ("medium_g", C.c_float), ("medium_scale", C.c_float),
     ]
 
 
@@ -56,6 +59,7 @@ class rtg_render_desc(C.Structure):
         ("max_depth", C.c_int32), ("seed", C.c_uint64), ("guiding", C.c_int32),
         ("use_camera_matrix", C.c_int32), ("camera_to_world", C.c_float * 16),
         ("filter", C.c_int32), ("filter_radius", C.c_float * 2), ("filter_param", C.c_float * 2),
+        ("integrator", C.c_int32),
     ]
 
 
@@ -107,6 +111,7 @@ SIGNATURES = {
     "rtg_render_device": (C.c_int, [P, P, C.POINTER(rtg_render_desc), P, C.POINTER(rtg_stats)]),
     "rtg_trace": (C.c_int, [P, P, P, C.c_int32, C.c_int32, P, P, P, P, P]),
     "rtg_bsdf_batch": (C.c_int, [P, C.POINTER(rtg_material), C.c_int32, P, P, C.c_int32, P, P, P, P]),
+    "rtg_medium_batch": (C.c_int, [P, P, C.c_int32, P, C.c_int32, P, C.c_int32, P]),
     "rtg_server_create": (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
     "rtg_server_destroy": (C.c_int, [P]),
     "rtg_submit": (C.c_int, [P, P, C.POINTER(rtg_render_desc), C.POINTER(C.c_int64)]),
@@ -196,6 +201,7 @@ def render_struct(cam, rp):
         d.filter = int(f.get("type", FILTER_BOX))
         d.filter_radius[:] = [float(x) for x in f.get("radius", (0.5, 0.5))]
         d.filter_param[:] = [float(x) for x in f.get("param", (0.0, 0.0))]
+    d.integrator = int(rp.get("integrator", INTEGRATOR_PATH_MIS))
     return d
 
 
@@ -376,6 +382,17 @@ class Scene:
             desc.env_rgb = _ptr(env); desc.env_height, desc.env_width = env.shape[0], env.shape[1]
             desc.env_scale = float(sd.env_scale)
         desc.light_strategy = int(sd.light_strategy)
+        med = getattr(sd, "medium", None)
+        if med is not None:
+            grid = _f32(med["density"]); keep.append(grid)
+            if grid.ndim != 3:
+                raise ValueError("medium density must be a (z, y, x) array")
+            desc.medium_density = _ptr(grid)
+            desc.medium_res[0], desc.medium_res[1], desc.medium_res[2] = grid.shape[2], grid.shape[1], grid.shape[0]
+            for k in range(3):
+                desc.medium_albedo[k] = float(med.get("albedo", (0.0, 0.0, 0.0))[k])
+            desc.medium_g = float(med.get("g", 0.0))
+            desc.medium_scale = float(med.get("scale", 1.0))
         h = P()
         ctx.check(ctx.lib.rtg_scene_upload(ctx.h, C.byref(desc), C.byref(h)))
         self.h = h
@@ -404,6 +421,21 @@ class Scene:
         self.ctx.check(self.ctx.lib.rtg_trace(self.ctx.h, self.h, _ptr(rays), n, int(any_hit), _ptr(prim), _ptr(t),
                                               _ptr(uv), _ptr(p), _ptr(nrm)))
         return prim, t, uv, p, nrm
+
+    def medium(self, op, x, u=None):
+        """Camera-medium tap (rtg_medium_batch): op 0 density at points (n, 3); op 1 delta tracking /
+        op 2 ratio tracking on rays (n, 8) with draws u (n, k); op 3 phase eval on (n, 6) = (wi, wo);
+        op 4 phase sample of wi (n, 3) with u (n, 2)."""
+        w_in = {0: 3, 1: 8, 2: 8, 3: 6, 4: 3}[op]
+        w_out = {0: 1, 1: 3, 2: 3, 3: 1, 4: 4}[op]
+        x = _f32(x, (-1, w_in)); n = x.shape[0]
+        k = 0
+        if u is not None:
+            u = _f32(u).reshape(n, -1); k = u.shape[1]
+        out = np.zeros((n, w_out), np.float32)
+        self.ctx.check(self.ctx.lib.rtg_medium_batch(self.ctx.h, self.h, int(op), _ptr(x), n,
+                                                     None if u is None else _ptr(u), k, _ptr(out)))
+        return out[:, 0] if w_out == 1 else out
 
     def render(self, camera, params):
         d = render_struct(camera, params)

@@ -14,7 +14,11 @@ Procedural scenes:
                    depth 6.
   glossy_cornell() Cornell variant: rough dielectric spheres, rough conductor walls, a small
                    light hidden behind a baffle; 512^2.
+  vol_cornell()    the Cornell box over [0,1]^3 filled by a 256^3 fBm camera medium (albedo 0.9,
+                   Henyey-Greenstein g 0.3), volumetric integrator, 512^2, 128 spp, depth 16.
 """
+import functools
+import os
 from dataclasses import dataclass, field
 from typing import List, Optional
 
@@ -55,6 +59,8 @@ class SceneDesc:
     light_strategy: int = R.LIGHTS_UNIFORM
     camera: dict = field(default_factory=dict)
     render: dict = field(default_factory=dict)
+    # camera medium over the world box [0,1]^3: density (z, y, x) float32, albedo (3), g, scale
+    medium: Optional[dict] = None
 
     def add_material(self, **m):
         self.materials.append(m)
@@ -303,6 +309,52 @@ def displaced_sphere(width=1024, height=1024, spp=64, max_depth=6, min_depth=3, 
                      width=width, height=height, near=1e-4, far=1e4)
     sd.render = dict(spp=spp, sampler=R.SAMPLER_STRATIFIED, dimension=4, min_depth=min_depth, max_depth=max_depth,
                      seed=seed)
+    return sd
+
+
+@functools.lru_cache(maxsize=2)
+def fbm_density(res=256, octaves=5, seed=3, freq=4.0, gain=6.0, bias=1.0, lo=0.0, hi=4.0):
+    """Density grid for the volumetric scene: clip(bias + gain * fBm(x), lo, hi) at the grid
+    vertices x = (i, j, k) / res, fBm = sum over octaves of 0.5^o Perlin(freq 2^o x) with a seeded
+    permutation. Returns a (z, y, x) float32 array (read-only, cached). One z-slice per task on a
+    thread pool."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    perm = perlin_perm(seed)
+    c = np.arange(res, dtype=np.float64) / res
+    y, x = np.meshgrid(c, c, indexing="ij")
+    out = np.empty((res, res, res), np.float32)
+
+    def slab(k):
+        z = np.full_like(x, c[k])
+        acc = np.zeros_like(x)
+        amp, f = 1.0, freq
+        for _ in range(octaves):
+            acc += amp * perlin3(x * f, y * f, z * f, perm)
+            amp *= 0.5; f *= 2.0
+        out[k] = np.clip(bias + gain * acc, lo, hi)
+
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as ex:
+        list(ex.map(slab, range(res)))
+    out.setflags(write=False)  # cached: shared by every caller
+    return out
+
+
+def vol_cornell(width=512, height=512, spp=128, max_depth=16, min_depth=3, seed=1, res=256, albedo=0.9, g=0.3):
+    """Heterogeneous volumetric scene: the Cornell box moved to span [0,1]^3 (x and z shifted by
+    0.5), filled by a camera medium over that box (a camera medium's grid spans the world box
+    [0,1]^3): res^3 fBm density (5 octaves, seed 3, clipped to [0,4]), single-scattering albedo 0.9,
+    Henyey-Greenstein g = 0.3; rendered by the volumetric integrator."""
+    sd = cornell(width, height, spp, max_depth, min_depth, seed)
+    off = np.array([0.5, 0.0, 0.5], np.float32)
+    for m in sd.meshes:
+        m.vertices = (np.asarray(m.vertices, np.float32) + off).astype(np.float32)
+    cam = dict(sd.camera)
+    cam["origin"] = tuple(float(a + b) for a, b in zip(cam["origin"], off))
+    cam["target"] = tuple(float(a + b) for a, b in zip(cam["target"], off))
+    sd.camera = cam
+    sd.medium = dict(density=fbm_density(res), albedo=(albedo, albedo, albedo), g=g, scale=1.0)
+    sd.render = dict(sd.render, integrator=R.INTEGRATOR_VOLPATH)
     return sd
 
 

@@ -28,6 +28,12 @@ void launch_iteration(const DevPool& P, const DevScene*, DevJob* jobs, const int
     free_slots -= take;
   }
 }
+// the volumetric integrator's iteration finishes samples the same way
+void launch_iteration_vol(const DevPool& P, const DevScene* S, DevJob* jobs, const int32_t* order, int norder,
+                          const LaunchCfg& cfg, hipStream_t st) {
+  launch_iteration(P, S, jobs, order, norder, cfg, st, nullptr, nullptr);
+}
+void launch_medium_tap(const DevScene*, int, const float*, int, const float*, int, float*, hipStream_t) {}
 void launch_reset_pool(const DevPool&, hipStream_t) {}
 void launch_trace_tap(const DevScene*, const float*, int, int, int32_t*, float*, float*, float*, float*,
                       const LaunchCfg&, hipStream_t) {}

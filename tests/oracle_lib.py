@@ -56,6 +56,11 @@ def load():
                                      C.c_int, C.c_int, C.c_uint64, P, P, P, P]),
             "orc_bvh_info": (C.c_int, [P, P]),
             "orc_filter_value": (C.c_float, [P, C.c_float, C.c_float]),
+            "orc_set_medium": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, P, C.c_float, C.c_float]),
+            "orc_set_integrator": (None, [P, C.c_int]),
+            "orc_medium_sigma": (None, [P, P, C.c_int, P]),
+            "orc_medium_track": (None, [P, C.c_int, P, C.c_int, P, C.c_int, P]),
+            "orc_hg": (None, [C.c_float, C.c_int, P, P, C.c_int, P]),
             "gor_num_params": (C.c_int, []),
             "gor_bf16": (C.c_double, [C.c_double]),
             "gor_encode": (None, [P, C.c_int, P]),
@@ -84,6 +89,19 @@ def filter_params(rp):
     """(type, radius x, radius y, param 0, param 1) from render parameters' "filter" entry."""
     f = rp.get("filter") or {}
     return f32([f.get("type", 0), *f.get("radius", (0.5, 0.5)), *f.get("param", (0.0, 0.0))])
+
+
+def hg_eval(g, wi, wo):
+    wi = f32(wi, (-1, 3)); wo = f32(wo, (-1, 3)); out = np.zeros(wi.shape[0], np.float32)
+    load().orc_hg(float(g), 0, _p(wi), _p(wo), wi.shape[0], _p(out))
+    return out
+
+
+def hg_sample(g, wi, u):
+    """-> (n, 4): sampled direction, pdf."""
+    wi = f32(wi, (-1, 3)); u = f32(u, (-1, 2)); out = np.zeros((wi.shape[0], 4), np.float32)
+    load().orc_hg(float(g), 1, _p(wi), _p(u), wi.shape[0], _p(out))
+    return out
 
 
 def filter_value(f, x, y):
@@ -130,6 +148,13 @@ class OracleScene:
             env = f32(sd.env)
             self.lib.orc_set_env(self.h, _p(env), env.shape[1], env.shape[0], float(sd.env_scale))
         self.nprim = self.lib.orc_finalize(self.h, int(sd.light_strategy))
+        med = getattr(sd, "medium", None)
+        if med is not None:
+            g = f32(med["density"])
+            alb = f32(med.get("albedo", (0.0, 0.0, 0.0)), (3,))
+            self.lib.orc_set_medium(self.h, _p(g), g.shape[2], g.shape[1], g.shape[0], _p(alb), float(med.get("g", 0.0)),
+                                    float(med.get("scale", 1.0)))
+        self.lib.orc_set_integrator(self.h, int((sd.render or {}).get("integrator", 0)))
         self.sd = sd
 
     def __del__(self):
@@ -137,6 +162,18 @@ class OracleScene:
             self.lib.orc_scene_free(self.h)
         except Exception:
             pass
+
+    def medium_sigma(self, pts):
+        pts = f32(pts, (-1, 3)); out = np.zeros(pts.shape[0], np.float32)
+        self.lib.orc_medium_sigma(self.h, _p(pts), pts.shape[0], _p(out))
+        return out
+
+    def medium_track(self, op, rays, u):
+        """op 0 delta tracking -> (scattered, t, draws used); op 1 ratio tracking -> (tr, draws used, 0)."""
+        rays = f32(rays, (-1, 8)); u = f32(u); n = rays.shape[0]
+        u = u.reshape(n, -1); out = np.zeros((n, 3), np.float32)
+        self.lib.orc_medium_track(self.h, int(op), _p(rays), n, _p(u), u.shape[1], _p(out))
+        return out
 
     def trace(self, rays, any_hit=False, brute=False):
         rays = f32(rays, (-1, 8)); n = rays.shape[0]
