@@ -44,8 +44,13 @@ def parse():
                     help="also run the serving-mix leg (off by default until its GPU run is re-validated)")
     ap.add_argument("--serve-requests", type=int, default=48,
                     help="serving leg: requests of the Poisson stream (lambda 40/s, seed 4)")
-    ap.add_argument("--cpu-stride", type=int, default=7,
-                    help="CPU sample: every k-th 32x32 block in creation order (7 is coprime with 32 blocks per row)")
+    ap.add_argument("--no-volume", action="store_true", help="skip the volumetric leg")
+    ap.add_argument("--volume-stride", type=int, default=1,
+                    help="volumetric leg's CPU sample: every k-th 32x32 block of the 512x512 render")
+    ap.add_argument("--only-volume", action="store_true",
+                    help="run only the volumetric leg and print its JSON object (for profiling)")
+    ap.add_argument("--cpu-stride", type=int, default=1,
+                    help="CPU sample: every k-th 32x32 block in creation order (1: the whole render)")
     return ap.parse_args()
 
 
@@ -64,8 +69,9 @@ def cpu_baseline(sd, spp, stride):
         "unit": "samples/s",
         "cores": threads,
         "kind": "port",
-        "sample": f"every {stride}th 32x32 block of the same 1024x1024 render ({nblocks} blocks, {samples} samples, "
-                  f"{dt:.1f} s), reference schedule (blocks seeded (x, y), LIFO pop)",
+        "sample": (f"the whole render" if stride == 1 else f"every {stride}th 32x32 block of the same render")
+                  + f" ({nblocks} blocks, {samples} samples, {dt:.1f} s), reference schedule (blocks seeded (x, y), "
+                  "LIFO pop)",
         "rays_per_s": float(st[0]) / dt,
     }
 
@@ -220,6 +226,55 @@ def guiding_leg(ctx, scene, size, ref_spp, target=1e-3, ladder=(64, 256, 1024, 4
     return res
 
 
+def volumetric_leg(ctx, run_cpu, stride):
+    """Heterogeneous volumetric path tracing: the Cornell box over [0,1]^3 filled by a 256^3 fBm
+    density (5 octaves, seed 3, clipped to [0,4]), albedo 0.9, Henyey-Greenstein g 0.3, 512x512,
+    128 spp, depth 16, delta tracking with ratio-tracking transmittance. One warm-up render, one
+    timed render; the CPU oracle on every stride-th 32x32 block of the same render."""
+    import torch
+    from raytracer_amd import scenes
+
+    sd = scenes.vol_cornell()
+    scene = ctx.upload(sd)
+    cam, rp = sd.camera, sd.render
+    W, H = cam["width"], cam["height"]
+    film = torch.zeros(H * W * 4, dtype=torch.float32, device="cuda")
+    scene.render_device(cam, rp, film.data_ptr())
+    torch.cuda.synchronize()
+    film.zero_()
+    t0 = time.perf_counter()
+    st = scene.render_device(cam, rp, film.data_ptr())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    out = {
+        "workload": f"vol_cornell heterogeneous medium 256^3 VolPathTracing {W}x{H} {rp['spp']}spp depth{rp['max_depth']}",
+        "value": st.samples / wall,
+        "unit": "samples/s",
+        "ms": wall * 1e3,
+        "rays_per_s": (st.extension_rays + st.shadow_rays) / wall,
+        "iterations": st.iterations,
+        "nan_samples": st.nan_samples,
+    }
+    if run_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+
+        threads = max(1, min(16, os.cpu_count() or 1))
+        o = O.OracleScene(sd)
+        t0 = time.perf_counter()
+        _, _, cst, nblocks = o.render(spp=rp["spp"], threads=threads, blocks=(0, 0, stride))
+        dt = time.perf_counter() - t0
+        samples = nblocks * 32 * 32 * rp["spp"]
+        out["cpu_baseline"] = {
+            "value": samples / dt, "unit": "samples/s", "cores": threads, "kind": "port",
+            "sample": (f"the whole {W}x{H} render" if stride == 1 else f"every {stride}th 32x32 block of the same "
+                       f"{W}x{H} render") + f" ({nblocks} blocks, {samples} samples, {dt:.1f} s)",
+        }
+        out["speedup_vs_cpu"] = out["value"] / out["cpu_baseline"]["value"]
+    del scene
+    return out
+
+
 def copy_reference(mb):
     import torch
 
@@ -247,6 +302,11 @@ def main():
 
     def progress(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+    if a.only_volume:
+        progress("volumetric leg only")
+        print(json.dumps(volumetric_leg(rtg.Context(0), not a.no_cpu, a.volume_stride)), flush=True)
+        return
 
     progress("building the displaced-sphere scene")
     sd = scenes.displaced_sphere(width=a.size, height=a.size, spp=a.spp)
@@ -348,6 +408,9 @@ def main():
         out["guiding"] = guiding_leg(ctx, "glossy_cornell", a.guide_size, a.guide_ref_spp)
         progress("guided vs plain (indirect-only Lambertian variant)")
         out["guiding_indirect"] = guiding_leg(ctx, "indirect_cornell", a.guide_size, a.guide_ref_spp)
+    if not a.no_volume:
+        progress("volumetric leg (heterogeneous medium, 512x512 128spp)")
+        out["volumetric"] = volumetric_leg(ctx, not a.no_cpu, a.volume_stride)
     if a.serving:
         progress(f"serving mix ({a.serve_requests} requests)")
         out["serving"] = serving(ctx, (sd, scene), a.serve_requests)

@@ -212,6 +212,11 @@ int rtg_scene_bytes(rtg_scene* scene, uint64_t* bytes);
 typedef struct rtg_scene_info {
   int32_t triangles, spheres, materials, lights, env_width, env_height;
   float bbox_min[3], bbox_max[3];
+  /* camera medium (all 0 without one): grid resolution, albedo, phase g, density scale and the
+   * tracking majorant (maximum density x scale, Heterogeneous.cpp:40) */
+  int32_t medium_res[3];
+  float medium_albedo[3];
+  float medium_g, medium_scale, medium_sigma_max;
 } rtg_scene_info;
 int rtg_scene_query(rtg_scene* scene, rtg_scene_info* info);
 

@@ -812,6 +812,15 @@ int rtg_scene_query(rtg_scene* scene, rtg_scene_info* info) {
     info->bbox_min[k] = scene->dev.root_min[k];
     info->bbox_max[k] = scene->dev.root_max[k];
   }
+  if (scene->dev.grid) {
+    for (int k = 0; k < 3; k++) {
+      info->medium_res[k] = scene->dev.gres[k];
+      info->medium_albedo[k] = scene->dev.albedo[k];
+    }
+    info->medium_g = scene->dev.hg_g;
+    info->medium_scale = scene->dev.gscale;
+    info->medium_sigma_max = scene->dev.sigma_max;
+  }
   return RTG_OK;
 }
 

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <map>
 #include <memory>
 #include <sstream>
@@ -378,6 +379,11 @@ struct Loader {
   rtg_render_desc rd{};
   int strategy = RTG_LIGHTS_UNIFORM;
   bool have_camera = false, have_integrator = false;
+  // camera medium
+  std::vector<float> grid;
+  int gres[3] = {0, 0, 0};
+  float medium_albedo[3] = {0.f, 0.f, 0.f};
+  float medium_g = 0.f, medium_scale = 1.f;
 
   std::string resolve(const std::string& p) const {
     if (p.empty() || p[0] == '/') return p;
@@ -394,7 +400,8 @@ struct Loader {
     Props P;
     P.owner = owner;
     for (const XNode& c : n.kids) {
-      if (c.name == "bsdf" || c.name == "light" || c.name == "filter" || c.name == "medium" || c.name == "bssrdf")
+      if (c.name == "bsdf" || c.name == "light" || c.name == "filter" || c.name == "medium" || c.name == "bssrdf" ||
+          c.name == "volume" || c.name == "phase")
         continue;  // handled by the caller
       const std::string* nm = c.get("name");
       if (c.name == "texture") {
@@ -864,11 +871,17 @@ struct Loader {
     for (const XNode& c : root.kids) {
       if (c.name == "camera") {
         camera(c);
+      } else if (c.name == "medium") {
+        camera_medium(c);
       } else if (c.name == "integrator") {
         const std::string* t = c.get("type");
-        if (type_of(c) != "pathtracingmis")
-          perr("integrator " + (t ? *t : std::string("?")) + " is not rendered by the GPU path (PathTracingMIS only)");
-        Props P = props(c, "integrator PathTracingMIS");  // integrator/PathTracingMIS.cpp:12-20
+        const std::string ty = type_of(c);
+        if (ty != "pathtracingmis" && ty != "volpathtracing")
+          perr("integrator " + (t ? *t : std::string("?")) +
+               " is not rendered by the GPU path (PathTracingMIS and VolPathTracing only)");
+        // integrator/PathTracingMIS.cpp:12-20, VolPathTracing.cpp:9-17, Integrator.cpp:6-14
+        Props P = props(c, "integrator " + (t ? *t : std::string("?")));
+        rd.integrator = ty == "volpathtracing" ? RTG_INTEGRATOR_VOLPATH : RTG_INTEGRATOR_PATH_MIS;
         rd.min_depth = P.integer("minDepth", 3);
         rd.max_depth = P.integer("maxDepth", 1000);
         // integrator/Integrator.h:18-23 strategy names
@@ -895,6 +908,65 @@ struct Loader {
       }
     }
     if (!have_camera) perr("scene has no camera");
+  }
+
+  // The scene's camera medium (XMLLoader::handleCameraMedium, core/XMLLoader.cpp:159-172):
+  // Heterogeneous (medium/Heterogeneous.cpp:15-41: "albedo" default 0, "scale" default 1; as a
+  // camera medium its "toWorld" and "centered" do not apply) over a GridVolume, with the
+  // "phaseFunction" HGPhaseFunction ("g", default -0.8, HGPhaseFunction.cpp:7-10) or, by default,
+  // IsotropicPhaseFunction (Medium.cpp:7-10; the same density 1/(4 pi) and uniform sampling as
+  // HG with g = 0, which stands for it here).
+  void camera_medium(const XNode& c) {
+    const std::string* t = c.get("type");
+    if (type_of(c) != "heterogeneous")
+      perr("medium " + (t ? *t : std::string("?")) + " is not rendered by the GPU path (Heterogeneous only)");
+    Props P = props(c, "medium Heterogeneous");
+    static const float zero[3] = {0.f, 0.f, 0.f};
+    P.color("albedo", zero, medium_albedo);
+    medium_scale = (float)P.real("scale", 1.0);
+    const XNode* vol = nullptr;
+    const XNode* ph = nullptr;
+    for (const XNode& k : c.kids) {
+      const std::string* nm = k.get("name");
+      if (k.name == "volume" && nm && *nm == "volume") vol = &k;
+      if (k.name == "phase" && nm && *nm == "phaseFunction") ph = &k;
+    }
+    if (!vol) perr("medium Heterogeneous: no <volume name=\"volume\">");
+    if (type_of(*vol) != "gridvolume") perr("medium Heterogeneous: volume type must be GridVolume");
+    Props V = props(*vol, "volume GridVolume");
+    read_vol(resolve(V.str("path", "")));
+    medium_g = 0.f;
+    if (ph) {
+      const std::string pt = type_of(*ph);
+      if (pt == "hgphasefunction") {
+        Props H = props(*ph, "phase HGPhaseFunction");
+        medium_g = (float)H.real("g", -0.8);
+      } else if (pt != "isotropicphasefunction") {
+        perr("phase function type must be HGPhaseFunction or IsotropicPhaseFunction");
+      }
+    }
+  }
+
+  // volume/GridVolume.cpp:7-64: "VOL", version 3, encoding, resolution x y z, channels, bounding
+  // box (6 floats), 48 bytes in all; the reference then skips 4 more bytes before reading
+  // x*y*z floats, so its grid starts at the file's second value, and its last value lies past
+  // a standard file's end (left unset there; 0 here). Float32 single-channel files only.
+  void read_vol(const std::string& path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) perr("cannot open volume file " + path);
+    std::vector<char> b((std::istreambuf_iterator<char>(f)), std::istreambuf_iterator<char>());
+    if (b.size() < 48 || b[0] != 'V' || b[1] != 'O' || b[2] != 'L' || b[3] != 3)
+      perr("volume file " + path + " is not a version 3 VOL file");
+    int32_t hdr[5];
+    std::memcpy(hdr, b.data() + 4, sizeof(hdr));  // encoding, x, y, z, channels
+    if (hdr[0] != 1 || hdr[4] != 1) perr("volume file " + path + ": only float32 single-channel grids are supported");
+    if (hdr[1] < 2 || hdr[2] < 2 || hdr[3] < 2) perr("volume file " + path + ": resolution below 2");
+    const size_t n = (size_t)hdr[1] * hdr[2] * hdr[3];
+    if (n > ((size_t)1 << 31)) perr("volume file " + path + ": grid too large");
+    for (int k = 0; k < 3; k++) gres[k] = hdr[1 + k];
+    grid.assign(n, 0.f);
+    const size_t avail = b.size() > 52 ? (b.size() - 52) / 4 : 0;
+    std::memcpy(grid.data(), b.data() + 52, std::min(n, avail) * 4);
   }
 };
 
@@ -933,6 +1005,15 @@ extern "C" int rtg_scene_load_xml(rtg_ctx* ctx, const char* path, rtg_scene** ou
     sd.env_height = L.env_h;
     sd.env_scale = L.env_scale;
     sd.light_strategy = L.strategy;
+    if (!L.grid.empty()) {
+      sd.medium_density = L.grid.data();
+      for (int k = 0; k < 3; k++) {
+        sd.medium_res[k] = L.gres[k];
+        sd.medium_albedo[k] = L.medium_albedo[k];
+      }
+      sd.medium_g = L.medium_g;
+      sd.medium_scale = L.medium_scale;
+    }
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     *out = build_scene(ctx, &sd);
     if (desc) *desc = L.rd;

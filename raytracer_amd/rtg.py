@@ -78,6 +78,8 @@ class rtg_scene_info(C.Structure):
     _fields_ = [
         ("triangles", C.c_int32), ("spheres", C.c_int32), ("materials", C.c_int32), ("lights", C.c_int32),
         ("env_width", C.c_int32), ("env_height", C.c_int32), ("bbox_min", C.c_float * 3), ("bbox_max", C.c_float * 3),
+        ("medium_res", C.c_int32 * 3), ("medium_albedo", C.c_float * 3), ("medium_g", C.c_float),
+        ("medium_scale", C.c_float), ("medium_sigma_max", C.c_float),
     ]
 
 
@@ -256,7 +258,7 @@ class Context:
         if d.use_camera_matrix:
             cam["to_world"] = np.array(d.camera_to_world[:], np.float64).reshape(4, 4)
         rp = dict(spp=d.spp, sampler=d.sampler, dimension=d.dimension, min_depth=d.min_depth,
-                  max_depth=d.max_depth, seed=d.seed,
+                  max_depth=d.max_depth, seed=d.seed, integrator=d.integrator,
                   filter=dict(type=d.filter, radius=tuple(d.filter_radius[:]), param=tuple(d.filter_param[:])))
         return Scene.from_handle(self, h), cam, rp
 
@@ -406,7 +408,8 @@ class Scene:
     def info(self):
         i = rtg_scene_info()
         self.ctx.check(self.ctx.lib.rtg_scene_query(self.h, C.byref(i)))
-        return {k: (list(getattr(i, k)) if k.startswith("bbox") else getattr(i, k)) for k, _ in i._fields_}
+        return {k: (list(getattr(i, k)) if k.startswith("bbox") or k in ("medium_res", "medium_albedo") else getattr(i, k))
+                for k, _ in i._fields_}
 
     def bytes(self):
         b = C.c_uint64()

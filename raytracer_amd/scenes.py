@@ -444,6 +444,34 @@ def _fmt(v):
     return " ".join(repr(float(x)) for x in v)
 
 
+def write_vol(path, density, for_reference_reader=True):
+    """Mitsuba VOL v3 grid file (float32, one channel, bounding box [0,1]^3) of a (z, y, x)
+    density array. The reference's GridVolume reader skips 4 bytes after the 48-byte header
+    (volume/GridVolume.cpp:56-60); with for_reference_reader one padding value is written there
+    so that it reads the grid as given."""
+    d = np.ascontiguousarray(density, np.float32)
+    nz, ny, nx = d.shape
+    with open(path, "wb") as f:
+        f.write(b"VOL" + bytes([3]))
+        f.write(np.array([1, nx, ny, nz, 1], np.int32).tobytes())
+        f.write(np.array([0, 0, 0, 1, 1, 1], np.float32).tobytes())
+        if for_reference_reader:
+            f.write(np.zeros(1, np.float32).tobytes())
+        f.write(d.tobytes())
+
+
+def _medium_xml(med, directory):
+    import os
+
+    write_vol(os.path.join(directory, "medium.vol"), med["density"])
+    return ['  <medium type="Heterogeneous">',
+            '    <volume name="volume" type="GridVolume"><string name="path" value="medium.vol"/></volume>',
+            f'    <color name="albedo" value="{_fmt(med.get("albedo", (0.0, 0.0, 0.0)))}"/>',
+            f'    <double name="scale" value="{float(med.get("scale", 1.0))!r}"/>',
+            '    <phase name="phaseFunction" type="HGPhaseFunction">',
+            f'      <double name="g" value="{float(med.get("g", 0.0))!r}"/>', "    </phase>", "  </medium>"]
+
+
 def _filter_xml(f):
     if not f:
         return ['    <filter type="BoxFilter"/>']
@@ -469,7 +497,8 @@ def write_xml(sd, directory, name="scene.xml"):
     cam, rp = sd.camera, sd.render
     out = ['<?xml version="1.0" encoding="utf-8"?>', "<scene>"]
     strat = {R.LIGHTS_UNIFORM: "one_light_uniform", R.LIGHTS_WEIGHTED: "one_light_weighted"}[sd.light_strategy]
-    out += ['  <integrator type="PathTracingMIS">',
+    integ = "VolPathTracing" if rp.get("integrator", R.INTEGRATOR_PATH_MIS) == R.INTEGRATOR_VOLPATH else "PathTracingMIS"
+    out += [f'  <integrator type="{integ}">',
             f'    <integer name="minDepth" value="{int(rp.get("min_depth", 3))}"/>',
             f'    <integer name="maxDepth" value="{int(rp.get("max_depth", 1000))}"/>',
             f'    <string name="strategy" value="{strat}"/>', "  </integrator>"]
@@ -485,6 +514,8 @@ def write_xml(sd, directory, name="scene.xml"):
             f'    <integer name="sizeX" value="{int(cam["width"])}"/>',
             f'    <integer name="sizeY" value="{int(cam["height"])}"/>',
             *_filter_xml(rp.get("filter")), "  </camera>"]
+    if sd.medium is not None:
+        out += _medium_xml(sd.medium, directory)
 
     def bsdf(i, ind="      "):
         m = sd.materials[i]
