@@ -69,7 +69,29 @@ typedef struct rtg_material {
   float eta_int;
   float eta_ext;
   float transmitted[3];
+  /* Image textures (texture/ImageTexture.cpp, ScaleTexture.cpp) in place of the constant values
+   * above: 0 = the constant, k >= 1 = rtg_scene_desc.textures[k - 1], evaluated at the hit's uv.
+   * tex_albedo replaces albedo (Diffuse "albedo"; "reflectanceTexture" of the others),
+   * tex_alpha the roughness (the texture's channel average, as the reference evaluates it),
+   * tex_transmitted the dielectrics' "transmittedTexture". */
+  int32_t tex_albedo;
+  int32_t tex_alpha;
+  int32_t tex_transmitted;
 } rtg_material;
+
+/* An ImageTexture (texture/ImageTexture.cpp:7-64): the decoded linear RGB texel array that the
+ * reference's ImageLoader produces, looked up at uv as ImageTexture::eval does (x = u (width - 1),
+ * y = v height, times uv_scale; optional inversion; wrapped; bilinear, core/BasicMath.h:74-90),
+ * times value_scale (a ScaleTexture around it, texture/ScaleTexture.cpp:19-22; 1 without). */
+typedef struct rtg_texture_desc {
+  const float* rgb;            /* height x width x 3, row 0 first */
+  int32_t width;
+  int32_t height;
+  float uv_scale;              /* ImageTexture "scale" (its absolute value is used) */
+  int32_t invert_x;            /* ImageTexture "invertX" */
+  int32_t invert_y;            /* ImageTexture "invertY" */
+  float value_scale;           /* ScaleTexture "scale" */
+} rtg_texture_desc;
 
 /* A triangle mesh (geometricShape/Mesh.h:19-58). emission != NULL makes the
  * mesh a MeshLight with that radiance (light/MeshLight.cpp:32-64). */
@@ -116,6 +138,8 @@ typedef struct rtg_scene_desc {
   float medium_albedo[3];      /* single-scattering albedo ("albedo") */
   float medium_g;              /* HGPhaseFunction "g" (< 1e-4: isotropic sampling) */
   float medium_scale;          /* density multiplier ("scale"; <= 0 takes the default 1) */
+  const rtg_texture_desc* textures;  /* referenced by rtg_material.tex_* (1-based) */
+  int32_t num_textures;
 } rtg_scene_desc;
 
 /* Camera (camera/NoriPerspective.cpp:10-37 + core/Transform.cpp:199-227 lookAt),
@@ -217,6 +241,7 @@ typedef struct rtg_scene_info {
   int32_t medium_res[3];
   float medium_albedo[3];
   float medium_g, medium_scale, medium_sigma_max;
+  int32_t textures;            /* image textures */
 } rtg_scene_info;
 int rtg_scene_query(rtg_scene* scene, rtg_scene_info* info);
 

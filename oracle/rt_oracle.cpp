@@ -574,6 +574,7 @@ struct Material {
     C3 eta = c3(0.f), k = c3(0.f);
     float etaInt = 1.5046f, etaExt = 1.000277f;
     int dist = DIST_BECKMANN;
+    int tex[3] = {-1, -1, -1};  // image textures for albedo/reflectance, alpha, transmitted
 };
 struct BSDFInfo {  // bsdf/BSDF.h:82-139
     V3 wi, wo;
@@ -1010,6 +1011,55 @@ struct Bvh {
     }
 };
 
+// ---------------------------------------------------------------- image textures
+// texture/ImageTexture.cpp:7-64 on the decoded texel array (the reference's ImageLoader output),
+// times a ScaleTexture factor (texture/ScaleTexture.cpp:19-22)
+struct Texture {
+    int W = 0, H = 0;
+    bool invertX = false, invertY = false;
+    float scale = 1.f;       // ImageTexture "scale" (absolute value)
+    float valueScale = 1.f;  // ScaleTexture "scale"
+    std::vector<C3> px;      // row-major, H rows of W
+    // BasicMath.h:74-90
+    C3 interp2(float x, float y) const {
+        float ax = x - (int)x;
+        float ay = y - (int)y;
+        int xMin = std::min(std::max((int)x, 0), W - 2);
+        int yMin = std::min(std::max((int)y, 0), H - 2);
+        int xMax = xMin + 1 < W ? xMin + 1 : xMin;
+        xMax = std::min(std::max(xMax, 0), W - 1);
+        int yMax = yMin + 1 < H ? yMin + 1 : yMin;
+        yMax = std::min(std::max(yMax, 0), H - 1);
+        auto A = [&](int xx, int yy) { return px[(size_t)yy * W + xx]; };
+        return (1 - ax) * (1 - ay) * A(xMin, yMin) + (1 - ax) * ay * A(xMin, yMax) +
+               ax * (1 - ay) * A(xMax, yMin) + ax * ay * A(xMax, yMax);
+    }
+    static float sign(float v) { return v >= 0.f ? 1.f : -1.f; }  // BasicMath.h:60-66
+    // ImageTexture.cpp:27-64
+    C3 eval(V2 uv) const {
+        float x, y;
+        if (!invertY)
+            y = uv.y * H;
+        else
+            y = (int)uv.y + (sign(uv.y) * 1.f - (uv.y - (int)uv.y)) * H;
+        if (!invertX)
+            x = uv.x * (W - 1.f);
+        else
+            x = (int)uv.x + (sign(uv.x) * 1.f - (uv.x - (int)uv.x)) * W;
+        x *= scale;
+        y *= scale;
+        if (x >= W)
+            x = (int)x % W + (x - (int)x);
+        else if (x < 0)
+            x = W - std::abs((int)x) % W + (x - (int)x);
+        if (y >= H)
+            y = (int)y % H + (y - (int)y);
+        else if (y < 0)
+            y = H - std::abs((int)y) % H + (y - (int)y);
+        return interp2(x, y) * valueScale;
+    }
+};
+
 // ---------------------------------------------------------------- participating medium
 // The camera medium (core/XMLLoader.cpp:159-172): medium/Heterogeneous.cpp over
 // volume/GridVolume.cpp with medium/HGPhaseFunction.cpp. A camera medium never receives
@@ -1120,6 +1170,7 @@ struct Medium {
 
 // ---------------------------------------------------------------- scene
 struct Scene {
+    std::vector<Texture> textures;
     Medium medium;           // camera medium (empty: none)
     int integrator = 0;      // 0 PathTracingMIS, 1 VolPathTracing
     std::vector<Material> mats;
@@ -1450,6 +1501,29 @@ struct Scene {
             hit.sh = hit.tr;
         }
     }
+    // Texture coordinates of a hit: a triangle's uv (barycentrics, or interpolated from its mesh's
+    // uvs, Triangle.cpp:141-171); a sphere's are the spherical angles of the normalized world-space
+    // hit point (Sphere.cpp:100-104)
+    V2 textureUv(const Hit& hit) const {
+        if (hit.prim < ntri) return hit.uv;
+        V3 n = normalized(hit.p);
+        return V2{sphericalPhiFromCartesian(n) / (2 * PI), sphericalThetaFromCartesian(n) / PI};
+    }
+    // The material at a hit with its textures evaluated (Diffuse.cpp:34,91, RoughConductor.cpp:87-90,
+    // RoughDielectric.cpp:108-140, Conductor.cpp:42; alpha = the texture's average)
+    Material materialAt(const Hit& hit) const {
+        Material m = mats[primMaterial(hit.prim)];
+        if (m.tex[0] < 0 && m.tex[1] < 0 && m.tex[2] < 0) return m;
+        V2 uv = textureUv(hit);
+        if (m.tex[0] >= 0) {
+            C3 c = textures[m.tex[0]].eval(uv);
+            if (m.type == MAT_DIFFUSE) m.albedo = c;
+            else m.reflectance = c;
+        }
+        if (m.tex[1] >= 0) m.alpha = average(textures[m.tex[1]].eval(uv));
+        if (m.tex[2] >= 0) m.transmitted = textures[m.tex[2]].eval(uv);
+        return m;
+    }
     int primMaterial(int prim) const {
         if (prim < ntri) {
             const Mesh& m = meshes[primMesh[prim]];
@@ -1647,7 +1721,7 @@ struct PathTracer {
             return radiance;
         }
         while (depth < maxDepth) {
-            const Material& bsdf = scene->mats[scene->primMaterial(hit.prim)];
+            const Material bsdf = scene->materialAt(hit);
             int hitLight = scene->primLight(hit.prim);
             if (emission && hitLight >= 0)
                 radiance += throughput * scene->meshLe(scene->lights[hitLight], -ray.d, hit.sh.n);
@@ -1837,7 +1911,7 @@ struct VolPathTracer {
                     wasIntersected = scene->intersect(ray, intersection, false);
                     continue;
                 }
-                const Material& bsdf = scene->mats[scene->primMaterial(hit.prim)];
+                const Material bsdf = scene->materialAt(hit);
                 int hitLight = scene->primLight(hit.prim);
                 if (emission && hitLight >= 0)
                     radiance += throughput * scene->meshLe(scene->lights[hitLight], -ray.d, hit.sh.n);
@@ -2147,6 +2221,7 @@ int orc_add_material(void* sp, const float* p) {
     m.etaExt = p[12];
     m.dist = (int)p[13];
     m.transmitted = c3(p[14], p[15], p[16]);
+    for (int k = 0; k < 3; k++) m.tex[k] = (int)p[17 + k] - 1;  // 1-based, 0 = none
     s->mats.push_back(m);
     return (int)s->mats.size() - 1;
 }
@@ -2194,6 +2269,52 @@ int orc_add_sphere(void* sp, const float* toWorld, float radius, int mat) {
 }
 
 // EnvironmentLight.cpp:8-44 (identity toWorld, no offsets, lat-long)
+// ImageTexture::eval of texture ti at n uv pairs -> n x 3
+void orc_texture_eval(void* sp, int ti, const float* uv, int n, float* out) {
+    const Texture& t = ((Scene*)sp)->textures[ti];
+    for (int i = 0; i < n; i++) {
+        C3 c = t.eval(V2{uv[2 * i], uv[2 * i + 1]});
+        out[3 * i] = c.r; out[3 * i + 1] = c.g; out[3 * i + 2] = c.b;
+    }
+}
+// The material at the closest hit of n rays (x 8) with its textures evaluated: out n x 8 =
+// (albedo or reflectance rgb, alpha, transmitted rgb, texture u) ; v in uvout (n x 2 = u, v)
+void orc_material_at(void* sp, const float* rays, int n, float* out, float* uvout) {
+    const Scene* s = (const Scene*)sp;
+    for (int i = 0; i < n; i++) {
+        const float* r = rays + 8 * i;
+        Hit hit;
+        float* o = out + 8 * i;
+        for (int k = 0; k < 8; k++) o[k] = 0.f;
+        uvout[2 * i] = uvout[2 * i + 1] = 0.f;
+        if (!s->intersect(makeRay(v3(r[0], r[1], r[2]), v3(r[3], r[4], r[5]), r[6], r[7]), hit, false)) continue;
+        Material m = s->materialAt(hit);
+        C3 a = m.type == MAT_DIFFUSE ? m.albedo : m.reflectance;
+        o[0] = a.r; o[1] = a.g; o[2] = a.b; o[3] = m.alpha;
+        o[4] = m.transmitted.r; o[5] = m.transmitted.g; o[6] = m.transmitted.b;
+        V2 uv = s->textureUv(hit);
+        uvout[2 * i] = uv.x;
+        uvout[2 * i + 1] = uv.y;
+    }
+}
+
+// Image texture: H x W x 3 texels, uv scale, inversion flags, value scale; returns its index
+int orc_add_texture(void* sp, const float* rgb, int W, int H, float uvScale, int invertX, int invertY,
+                    float valueScale) {
+    Scene* s = (Scene*)sp;
+    Texture t;
+    t.W = W;
+    t.H = H;
+    t.scale = std::fabs(uvScale);
+    t.invertX = invertX != 0;
+    t.invertY = invertY != 0;
+    t.valueScale = valueScale;
+    t.px.resize((size_t)W * H);
+    for (size_t i = 0; i < t.px.size(); i++) t.px[i] = c3(rgb[3 * i], rgb[3 * i + 1], rgb[3 * i + 2]);
+    s->textures.push_back(t);
+    return (int)s->textures.size() - 1;
+}
+
 int orc_set_env(void* sp, const float* rgb, int W, int H, float scale) {
     Scene* s = (Scene*)sp;
     Light L;

@@ -1064,6 +1064,20 @@ __device__ __forceinline__ void check_xy(const DevScene& S, float& x, float& y) 
   if (y >= (float)H) y = (float)((int)y % H) + (y - (float)(int)y);
   else if (y < 0.f) y = (float)(H - abs((int)y) % H) + (y - (float)(int)y);
 }
+// core/BasicMath.h:74-90 on an H x W RGB array
+__device__ __forceinline__ f3 interp2_map(const float* m, int W, int H, float x, float y) {
+  float ax = x - (float)(int)x;
+  float ay = y - (float)(int)y;
+  int x0 = min(max((int)x, 0), W - 2);
+  int y0 = min(max((int)y, 0), H - 2);
+  int x1 = x0 + 1 < W ? x0 + 1 : x0;
+  x1 = min(max(x1, 0), W - 1);
+  int y1 = y0 + 1 < H ? y0 + 1 : y0;
+  y1 = min(max(y1, 0), H - 1);
+  f3 A = ld3(m + 3 * (y0 * W + x0)), B = ld3(m + 3 * (y1 * W + x0));
+  f3 C = ld3(m + 3 * (y0 * W + x1)), D = ld3(m + 3 * (y1 * W + x1));
+  return ((1.f - ax) * (1.f - ay)) * A + ((1.f - ax) * ay) * B + (ax * (1.f - ay)) * C + (ax * ay) * D;
+}
 // core/BasicMath.h:74-90
 __device__ __forceinline__ f3 interp2(const DevScene& S, float x, float y) {
   int W = S.env_w, H = S.env_h;
@@ -1131,6 +1145,70 @@ __device__ __forceinline__ LightSample env_sample(const DevScene& S, f3 pFrom, f
            (S.env_marg_sum * (2.f * PI / (float)W) * (PI / (float)H));
   ls.pdf /= smax(fabsf(sinTheta), EPS);
   return ls;
+}
+
+// ---------------------------------------------------------------- image textures
+// Texture coordinates of a hit: a triangle's are its barycentrics (b1, b2) unless its mesh has
+// uvs, which are then interpolated (geometricShape/Triangle.cpp:141-171); a sphere's are the
+// spherical angles of the normalized world-space hit point, phi / 2pi and theta / pi
+// (Sphere.cpp:100-104: the world ray's point, not the point relative to the centre).
+__device__ __forceinline__ void surface_uv(const DevScene& S, const HitRec& h, f3 ro, f3 rd, float& u, float& v) {
+  if (h.prim < S.ntri) {
+    u = h.b1;
+    v = h.b2;
+    if (S.tri_has_uv && S.tri_has_uv[h.prim]) {
+      const int32_t* vi = S.tri_vidx + 3u * (uint32_t)h.prim;
+      const float b0 = 1.f - (h.b1 + h.b2);
+      const float* a = S.vuv + 2 * vi[0];
+      const float* b = S.vuv + 2 * vi[1];
+      const float* c = S.vuv + 2 * vi[2];
+      u = b0 * a[0] + h.b1 * b[0] + h.b2 * c[0];
+      v = b0 * a[1] + h.b1 * b[1] + h.b2 * c[1];
+    }
+  } else {
+    const f3 n = normalized(ro + h.t * rd);
+    u = sph_phi(n) / (2.f * PI);
+    v = sph_theta(n) / PI;
+  }
+}
+// ImageTexture::eval (texture/ImageTexture.cpp:27-64) times the ScaleTexture factor
+__device__ __forceinline__ f3 tex_eval(const DevScene& S, int ti, float u, float v) {
+  const DevTexture& T = S.textures[ti];
+  const float W = (float)T.w, H = (float)T.h;
+  float x, y;
+  if (!T.invert_y) y = v * H;
+  else y = (float)(int)v + (sgn(v) * 1.f - (v - (float)(int)v)) * H;
+  if (!T.invert_x) x = u * (W - 1.f);
+  else x = (float)(int)u + (sgn(u) * 1.f - (u - (float)(int)u)) * W;
+  x *= T.uv_scale;
+  y *= T.uv_scale;
+  if (x >= W) x = (float)((int)x % T.w) + (x - (float)(int)x);
+  else if (x < 0.f) x = (float)(T.w - abs((int)x) % T.w) + (x - (float)(int)x);
+  if (y >= H) y = (float)((int)y % T.h) + (y - (float)(int)y);
+  else if (y < 0.f) y = (float)(T.h - abs((int)y) % T.h) + (y - (float)(int)y);
+  return interp2_map(S.texels + T.offset, T.w, T.h, x, y) * T.value_scale;
+}
+// The BSDF parameters at a hit: each textured one evaluated at the hit's uv (Diffuse.cpp:34,91,
+// RoughConductor.cpp:87-90, RoughDielectric.cpp:108-140, Conductor.cpp:42, Dielectric.cpp); a
+// roughness texture contributes its channel average (Color.h:151-158).
+__device__ __forceinline__ void resolve_material(const DevScene& S, DevMaterial& M, const HitRec& h, f3 ro, f3 rd) {
+  if (M.tex[0] < 0 && M.tex[1] < 0 && M.tex[2] < 0) return;
+  float u, v;
+  surface_uv(S, h, ro, rd, u, v);
+  if (M.tex[0] >= 0) {
+    const f3 c = tex_eval(S, M.tex[0], u, v);
+    M.albedo[0] = c.x; M.albedo[1] = c.y; M.albedo[2] = c.z;
+  }
+  if (M.tex[1] >= 0) {
+    const f3 c = tex_eval(S, M.tex[1], u, v);
+    float a = 0.f;
+    a += c.x; a += c.y; a += c.z;
+    M.alpha = a / 3.f;
+  }
+  if (M.tex[2] >= 0) {
+    const f3 c = tex_eval(S, M.tex[2], u, v);
+    M.trans[0] = c.x; M.trans[1] = c.y; M.trans[2] = c.z;
+  }
 }
 
 // ---------------------------------------------------------------- participating medium

@@ -96,6 +96,11 @@ DevMaterial dev_material(const rtg_material& m) {
     d.k[c] = m.k[c];
     d.trans[c] = m.transmitted[c];
   }
+  // rtg_material's 1-based texture references -> 0-based, -1 for none (range-checked by the scene)
+  d.tex[0] = m.tex_albedo - 1;
+  d.tex[1] = m.tex_alpha - 1;
+  d.tex[2] = m.tex_transmitted - 1;
+  d.tex[3] = -1;
   return d;
 }
 
@@ -221,18 +226,48 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   const int nmat = desc->num_materials;
   // materials (texture/ConstantTexture.h; bsdf constructors)
   std::vector<DevMaterial> mats(nmat);
-  for (int i = 0; i < nmat; i++) mats[i] = dev_material(desc->materials[i]);
+  for (int i = 0; i < nmat; i++) {
+    mats[i] = dev_material(desc->materials[i]);
+    for (int k = 0; k < 3; k++)
+      if (mats[i].tex[k] < -1 || mats[i].tex[k] >= desc->num_textures)
+        fail(RTG_ERR_INVALID, "material " + std::to_string(i) + ": texture reference out of range");
+  }
+  // image textures (texture/ImageTexture.cpp:7-64)
+  std::vector<DevTexture> texs;
+  std::vector<float> texels;
+  if (desc->num_textures < 0 || (desc->num_textures > 0 && !desc->textures))
+    fail(RTG_ERR_INVALID, "bad texture table");
+  for (int i = 0; i < desc->num_textures; i++) {
+    const rtg_texture_desc& T = desc->textures[i];
+    if (!T.rgb || T.width < 2 || T.height < 2) fail(RTG_ERR_INVALID, "texture " + std::to_string(i) + ": needs at least 2x2 texels");
+    if ((size_t)T.width * T.height * 3 > ((size_t)1 << 31) - texels.size())
+      fail(RTG_ERR_INVALID, "textures above 2^31 values in all");
+    DevTexture d;
+    std::memset(&d, 0, sizeof(d));
+    d.w = T.width;
+    d.h = T.height;
+    d.invert_x = T.invert_x ? 1 : 0;
+    d.invert_y = T.invert_y ? 1 : 0;
+    d.uv_scale = std::fabs(T.uv_scale);
+    d.value_scale = T.value_scale;
+    d.offset = (uint32_t)texels.size();
+    texels.insert(texels.end(), T.rgb, T.rgb + (size_t)T.width * T.height * 3);
+    texs.push_back(d);
+  }
   // triangles in global id order
   std::vector<BuildPrim> tprims;
   std::vector<float> tv;           // 9 per triangle, global order
   std::vector<uint32_t> tmat, tlight, tmesh;
   std::vector<int32_t> tvidx;      // global vertex ids
   std::vector<uint8_t> thasn;
-  std::vector<float> vnormals;
+  std::vector<float> vnormals, vuvs;
+  std::vector<uint8_t> thasuv;
   std::vector<DevLight> lights;
   std::vector<float> emit_tri, emit_nrm, emit_cdf;
   int vbase = 0;
-  bool anyNormals = false;
+  bool anyNormals = false, anyUvs = false;
+  for (int mi = 0; mi < desc->num_meshes; mi++)
+    if (desc->meshes[mi].uvs && desc->meshes[mi].num_triangles > 0) anyUvs = true;
   for (int mi = 0; mi < desc->num_meshes; mi++) {
     const rtg_mesh_desc& M = desc->meshes[mi];
     if (M.num_triangles < 0 || M.num_vertices < 0 || (M.num_triangles > 0 && (!M.indices || !M.vertices)))
@@ -301,9 +336,13 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
       tlight.push_back(lightId);
       tmesh.push_back((uint32_t)mi);
       thasn.push_back(M.normals ? 1 : 0);
+      thasuv.push_back(M.uvs ? 1 : 0);
     }
     for (int i = 0; i < M.num_vertices; i++)
       for (int c = 0; c < 3; c++) vnormals.push_back(M.normals ? M.normals[3 * i + c] : 0.f);
+    if (anyUvs)
+      for (int i = 0; i < M.num_vertices; i++)
+        for (int c = 0; c < 2; c++) vuvs.push_back(M.uvs ? M.uvs[2 * i + c] : 0.f);
     vbase += M.num_vertices;
   }
   // spheres
@@ -350,20 +389,19 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   std::vector<float> lv((size_t)ntri * 9);
   std::vector<uint4> info((size_t)ntri + nsph);
   std::vector<int32_t> lvidx;
-  std::vector<uint8_t> lhasn;
-  if (anyNormals) {
-    lvidx.resize((size_t)ntri * 3);
-    lhasn.resize(ntri);
-  }
+  std::vector<uint8_t> lhasn, lhasuv;
+  if (anyNormals || anyUvs) lvidx.resize((size_t)ntri * 3);
+  if (anyNormals) lhasn.resize(ntri);
+  if (anyUvs) lhasuv.resize(ntri);
   uint32_t sphBase = (uint32_t)ntri;
   for (int i = 0; i < ntri; i++) {
     uint32_t g = bvh.order_tri[i];
     std::memcpy(&lv[(size_t)i * 9], &tv[(size_t)g * 9], 9 * sizeof(float));
     info[i] = make_uint4(g, tmat[g], tlight[g], tmesh[g]);
-    if (anyNormals) {
+    if (anyNormals || anyUvs)
       for (int k = 0; k < 3; k++) lvidx[(size_t)i * 3 + k] = tvidx[(size_t)g * 3 + k];
-      lhasn[i] = thasn[g];
-    }
+    if (anyNormals) lhasn[i] = thasn[g];
+    if (anyUvs) lhasuv[i] = thasuv[g];
   }
   std::vector<float4> lsph((size_t)nsph * 5);
   for (int i = 0; i < nsph; i++) {
@@ -456,8 +494,16 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   }
   D.tri_v = upload(ctx, lv, own, bytes);
   D.prim_info = upload(ctx, info, own, bytes);
+  if (anyNormals || anyUvs) D.tri_vidx = upload(ctx, lvidx, own, bytes);
+  if (anyUvs) {
+    D.vuv = upload(ctx, vuvs, own, bytes);
+    D.tri_has_uv = upload(ctx, lhasuv, own, bytes);
+  }
+  if (!texs.empty()) {
+    D.textures = upload(ctx, texs, own, bytes);
+    D.texels = upload(ctx, texels, own, bytes);
+  }
   if (anyNormals) {
-    D.tri_vidx = upload(ctx, lvidx, own, bytes);
     D.tri_has_n = upload(ctx, lhasn, own, bytes);
     D.vnormals = upload(ctx, vnormals, own, bytes);
   }
@@ -513,6 +559,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   sc->stack = std::min(kMaxStack, std::max(16, bvh.max_stack + 1));
   sc->ntri = ntri;
   sc->nsph = nsph;
+  sc->ntex = (int)texs.size();
   ctx->scene_slots[slot] = sc.get();
   return sc.release();
 }
@@ -812,6 +859,7 @@ int rtg_scene_query(rtg_scene* scene, rtg_scene_info* info) {
     info->bbox_min[k] = scene->dev.root_min[k];
     info->bbox_max[k] = scene->dev.root_max[k];
   }
+  info->textures = scene->ntex;
   if (scene->dev.grid) {
     for (int k = 0; k < 3; k++) {
       info->medium_res[k] = scene->dev.gres[k];

@@ -53,7 +53,7 @@ struct rtg_scene {
   std::vector<void*> owned;
   uint64_t bytes = 0;
   int stack = 16;
-  int ntri = 0, nsph = 0;
+  int ntri = 0, nsph = 0, ntex = 0;
   std::atomic<int> users{0};  // requests of the serving loop that reference this scene
 };
 

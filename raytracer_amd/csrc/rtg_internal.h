@@ -37,6 +37,15 @@ struct DevMaterial {
   float eta[4];
   float k[4];
   float trans[4];
+  int32_t tex[4];  // image textures (index into DevScene::textures, -1 none): albedo, alpha, transmitted
+};
+
+// An image texture (texture/ImageTexture.cpp, ScaleTexture.cpp): texels at `offset` floats into
+// DevScene::texels, h rows of w RGB values.
+struct DevTexture {
+  int32_t w, h, invert_x, invert_y;
+  float uv_scale, value_scale;
+  uint32_t offset, pad;
 };
 
 enum { LIGHT_MESH = 0, LIGHT_ENV = 1 };
@@ -81,6 +90,12 @@ struct DevScene {
   int32_t env_w, env_h;
   float env_factor, env_marg_sum, env_radius;
   int32_t pad;
+  // image textures and per-vertex texture coordinates (2 floats per global vertex; tri_has_uv per
+  // leaf-order triangle: its mesh has uvs, else the hit's barycentrics are its uv)
+  const DevTexture* textures;
+  const float* texels;
+  const float* vuv;
+  const uint8_t* tri_has_uv;
   // camera medium (medium/Heterogeneous.cpp, volume/GridVolume.cpp, medium/HGPhaseFunction.cpp):
   // density grid over the world box [0,1]^3, x fastest; sigma_max = max density x scale
   const float* grid;         // null: no medium

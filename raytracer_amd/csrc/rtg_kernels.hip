@@ -622,7 +622,8 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
             L.x += c.x; L.y += c.y; L.z += c.z;
           }
         }
-        const DevMaterial& M = S.mats[sf.material];
+        DevMaterial M = S.mats[sf.material];
+        resolve_material(S, M, h, o, d);
         PathRng rng = make_rng(J.seed, P.pixel[slot], P.sample[slot], J);
         f3 wo_l = to_local(sf.sh, -d);
         // one-light next-event estimation (Integrator.cpp:176-238, PathTracingMIS.cpp:119-153);
@@ -946,7 +947,8 @@ __global__ void __launch_bounds__(256) k_vol(DevPool P, const DevScene* scenes, 
           const f3 c = cmul(tp, mat_col(S.lights[sf.light].radiance));
           L.x += c.x; L.y += c.y; L.z += c.z;
         }
-        const DevMaterial& M = S.mats[sf.material];
+        DevMaterial M = S.mats[sf.material];
+        resolve_material(S, M, h, o, d);
         const f3 wo_l = to_local(sf.sh, -d);
         if (S.nlights > 0 && !is_delta(M)) {
           float ux, uy;

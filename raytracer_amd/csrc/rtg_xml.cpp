@@ -329,7 +329,16 @@ struct Props {
   std::map<std::string, std::string> val;   // name -> text value
   std::map<std::string, std::string> kind;  // name -> element name (integer, color, ...)
   std::map<std::string, M4> xf;
+  std::map<std::string, int> tex;           // name -> image texture (0-based) in place of a value
   std::string owner;
+  bool boolean(const std::string& k, bool d) const {
+    auto it = val.find(k);
+    if (it == val.end()) return d;
+    const std::string v = lower(it->second);
+    if (v == "true" || v == "1") return true;
+    if (v == "false" || v == "0") return false;
+    perr(owner + ": '" + k + "' must be true or false");
+  }
   bool has(const std::string& k) const { return val.count(k) || xf.count(k); }
   std::string str(const std::string& k, const std::string& d) const {
     auto it = val.find(k);
@@ -379,6 +388,9 @@ struct Loader {
   rtg_render_desc rd{};
   int strategy = RTG_LIGHTS_UNIFORM;
   bool have_camera = false, have_integrator = false;
+  // image textures (their texels in texdata; rgb pointers are set when the scene is built)
+  std::vector<rtg_texture_desc> textures;
+  std::vector<std::vector<float>> texdata;
   // camera medium
   std::vector<float> grid;
   int gres[3] = {0, 0, 0};
@@ -405,21 +417,19 @@ struct Loader {
         continue;  // handled by the caller
       const std::string* nm = c.get("name");
       if (c.name == "texture") {
-        // XMLLoader::handleTexture (core/XMLLoader.cpp:556-571): only ConstantTexture, whose
-        // value is its "albedo" color (texture/ConstantTexture.cpp:6-10), kept as a color
-        const std::string* ty = c.get("type");
+        // XMLLoader::handleTexture (core/XMLLoader.cpp:556-571): a ConstantTexture is kept as its
+        // color, an ImageTexture (or a ScaleTexture around one) becomes an image texture
         if (!nm) perr(owner + ": <texture> without a name");
-        if (!ty || lower(*ty) != "constanttexture")
-          perr(owner + ": texture '" + *nm + "' of type " + (ty ? *ty : std::string("?")) +
-               " is not supported (ConstantTexture only)");
-        Props T = props(c, owner + " texture " + *nm);
-        static const float half[3] = {0.5f, 0.5f, 0.5f};
         float col[3];
-        T.color("albedo", half, col);
-        char buf[128];
-        std::snprintf(buf, sizeof(buf), "%.9g %.9g %.9g", col[0], col[1], col[2]);
-        P.val[*nm] = buf;
-        P.kind[*nm] = "color";
+        int ti = texture(c, owner + " texture " + *nm, 1.f, col);
+        if (ti >= 0) {
+          P.tex[*nm] = ti;
+        } else {
+          char buf[128];
+          std::snprintf(buf, sizeof(buf), "%.9g %.9g %.9g", col[0], col[1], col[2]);
+          P.val[*nm] = buf;
+          P.kind[*nm] = "color";
+        }
         continue;
       }
       if (c.name == "transform") {
@@ -444,6 +454,67 @@ struct Loader {
       P.kind[*nm] = c.name;
     }
     return P;
+  }
+
+  // A texture element, times `scale`: returns its image-texture index, or -1 for a constant
+  // (then `col`). ConstantTexture "albedo" (texture/ConstantTexture.cpp:6-10); ScaleTexture
+  // "scale" x "texture" (texture/ScaleTexture.cpp:4-22); ImageTexture "path", "invertX",
+  // "invertY", "scale", "gamma", "channel" (texture/ImageTexture.cpp:7-20) over a Radiance .hdr
+  // image, which the reference decodes with stb_image at gamma 1 (io/ImageLoader.cpp:40-100,
+  // 183-215: "channel" r, g, b or a replicates one channel; an RGBE file has no alpha, so a is 0).
+  int texture(const XNode& c, const std::string& owner, float scale, float col[3]) {
+    const std::string* ty = c.get("type");
+    const std::string t = ty ? lower(*ty) : std::string("?");
+    if (t == "scaletexture") {
+      // its own properties are read here: props() would also load the inner texture
+      const XNode* inner = nullptr;
+      double s = 1.0;
+      for (const XNode& k : c.kids) {
+        const std::string* nm = k.get("name");
+        if (k.name == "texture" && nm && *nm == "texture") inner = &k;
+        if (nm && *nm == "scale" && k.get("value")) s = number(*k.get("value"));
+      }
+      if (!inner) perr(owner + ": ScaleTexture without a texture named 'texture'");
+      return texture(*inner, owner + " texture", scale * (float)s, col);
+    }
+    Props T = props(c, owner);
+    if (t == "constanttexture") {
+      static const float half[3] = {0.5f, 0.5f, 0.5f};
+      T.color("albedo", half, col);
+      for (int k = 0; k < 3; k++) col[k] *= scale;
+      return -1;
+    }
+    if (t != "imagetexture")
+      perr(owner + ": texture type " + (ty ? *ty : std::string("?")) +
+           " is not supported (ConstantTexture, ImageTexture, ScaleTexture)");
+    const std::string path = resolve(T.str("path", ""));
+    const double gamma = T.real("gamma", 0.0);
+    if (gamma != 0.0 && gamma != 1.0) perr(owner + ": only gamma 0 or 1 is supported for .hdr images");
+    const std::string ch = lower(T.str("channel", "all"));
+    if (ch != "all" && ch != "r" && ch != "g" && ch != "b" && ch != "a")
+      perr(owner + ": channel must be all, r, g, b or a");
+    std::vector<float> px;
+    int W = 0, H = 0;
+    read_hdr(path, px, W, H);
+    if (W < 2 || H < 2) perr(owner + ": image must be at least 2x2");
+    if (ch != "all") {
+      const int k = ch == "r" ? 0 : ch == "g" ? 1 : ch == "b" ? 2 : -1;
+      for (size_t i = 0; i < px.size(); i += 3) {
+        const float v = k < 0 ? 0.f : px[i + k];
+        px[i] = px[i + 1] = px[i + 2] = v;
+      }
+    }
+    rtg_texture_desc d;
+    std::memset(&d, 0, sizeof(d));
+    d.width = W;
+    d.height = H;
+    d.uv_scale = (float)std::fabs(T.real("scale", 1.0));
+    d.invert_x = T.boolean("invertX", false) ? 1 : 0;
+    d.invert_y = T.boolean("invertY", false) ? 1 : 0;
+    d.value_scale = scale;
+    textures.push_back(d);
+    texdata.push_back(std::move(px));
+    return (int)textures.size() - 1;
   }
 
   int bsdf(const XNode& n) {
@@ -503,6 +574,11 @@ struct Loader {
     } else {
       perr("bsdf type " + *type + " is not rendered by the GPU path");
     }
+    // image textures in place of constants (1-based in rtg_material)
+    auto slot = [&](const char* k) { return P.tex.count(k) ? P.tex.at(k) + 1 : 0; };
+    m.tex_albedo = lt == "diffuse" ? slot("albedo") : slot("reflectanceTexture");
+    m.tex_alpha = (lt == "roughconductor" || lt == "roughdielectric") ? slot("alpha") : 0;
+    m.tex_transmitted = (lt == "dielectric" || lt == "roughdielectric") ? slot("transmittedTexture") : 0;
     mats.push_back(m);
     return (int)mats.size() - 1;
   }
@@ -727,16 +803,16 @@ struct Loader {
   }
 
   // Radiance RGBE (.hdr): new-style run-length encoded scanlines or flat pixels, -Y H +X W
-  void read_hdr(const std::string& path) {
+  void read_hdr(const std::string& path, std::vector<float>& img, int& img_w, int& img_h) {
     std::ifstream f(path, std::ios::binary);
-    if (!f) perr("cannot open environment map " + path);
+    if (!f) perr("cannot open image " + path);
     std::string line;
     bool fmt = false;
     while (std::getline(f, line)) {
       if (line.empty()) break;
       if (line.find("FORMAT=32-bit_rle_rgbe") != std::string::npos) fmt = true;
     }
-    if (!fmt) perr(path + ": not a Radiance RGBE image (only .hdr environment maps are supported)");
+    if (!fmt) perr(path + ": not a Radiance RGBE image (only .hdr images are supported)");
     std::getline(f, line);
     char ya[3], xa[3];
     int H = 0, W = 0;
@@ -744,6 +820,7 @@ struct Loader {
         std::string(xa) != "+X" || W <= 0 || H <= 0)
       perr(path + ": unsupported RGBE orientation '" + line + "'");
     std::vector<unsigned char> row((size_t)W * 4);
+    std::vector<float>& env = img;
     env.assign((size_t)W * H * 3, 0.f);
     for (int y = 0; y < H; y++) {
       unsigned char h[4];
@@ -783,8 +860,8 @@ struct Loader {
         for (int c = 0; c < 3; c++) env[((size_t)y * W + x) * 3 + c] = (float)p[c] * s;
       }
     }
-    env_w = W;
-    env_h = H;
+    img_w = W;
+    img_h = H;
   }
 
   void lights(const XNode& n) {
@@ -801,7 +878,7 @@ struct Loader {
       if (P.real("gamma", 0.0) != 0.0 || P.real("offsetU", 0.0) != 0.0 || P.real("offsetV", 0.0) != 0.0 ||
           P.str("isProbe", "false") == "true")
         perr("EnvironmentLight: gamma, offsets and light probes are not supported");
-      read_hdr(resolve(P.str("path", "")));
+      read_hdr(resolve(P.str("path", "")), env, env_w, env_h);
       env_scale = (float)P.real("scale", 1.0);
     }
   }
@@ -1005,6 +1082,9 @@ extern "C" int rtg_scene_load_xml(rtg_ctx* ctx, const char* path, rtg_scene** ou
     sd.env_height = L.env_h;
     sd.env_scale = L.env_scale;
     sd.light_strategy = L.strategy;
+    for (size_t i = 0; i < L.textures.size(); i++) L.textures[i].rgb = L.texdata[i].data();
+    sd.textures = L.textures.empty() ? nullptr : L.textures.data();
+    sd.num_textures = (int32_t)L.textures.size();
     if (!L.grid.empty()) {
       sd.medium_density = L.grid.data();
       for (int k = 0; k < 3; k++) {

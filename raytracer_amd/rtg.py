@@ -25,7 +25,15 @@ class rtg_material(C.Structure):
     _fields_ = [
         ("type", C.c_int32), ("distribution", C.c_int32), ("albedo", C.c_float * 3), ("alpha", C.c_float),
         ("eta", C.c_float * 3), ("k", C.c_float * 3), ("eta_int", C.c_float), ("eta_ext", C.c_float),
-        ("transmitted", C.c_float * 3),
+        ("transmitted", C.c_float * 3), ("tex_albedo", C.c_int32), ("tex_alpha", C.c_int32),
+        ("tex_transmitted", C.c_int32),
+    ]
+
+
+class rtg_texture_desc(C.Structure):
+    _fields_ = [
+        ("rgb", C.c_void_p), ("width", C.c_int32), ("height", C.c_int32), ("uv_scale", C.c_float),
+        ("invert_x", C.c_int32), ("invert_y", C.c_int32), ("value_scale", C.c_float),
     ]
 
 
@@ -47,7 +55,7 @@ class rtg_scene_desc(C.Structure):
         ("spheres", C.c_void_p), ("num_spheres", C.c_int32), ("env_rgb", C.c_void_p), ("env_width", C.c_int32),
         ("env_height", C.c_int32), ("env_scale", C.c_float), ("light_strategy", C.c_int32),
         ("medium_density", C.c_void_p), ("medium_res", C.c_int32 * 3), ("medium_albedo", C.c_float * 3),
-        ("medium_g", C.c_float), ("medium_scale", C.c_float),
+        ("medium_g", C.c_float), ("medium_scale", C.c_float), ("textures", C.c_void_p), ("num_textures", C.c_int32),
     ]
 
 
@@ -79,7 +87,7 @@ class rtg_scene_info(C.Structure):
         ("triangles", C.c_int32), ("spheres", C.c_int32), ("materials", C.c_int32), ("lights", C.c_int32),
         ("env_width", C.c_int32), ("env_height", C.c_int32), ("bbox_min", C.c_float * 3), ("bbox_max", C.c_float * 3),
         ("medium_res", C.c_int32 * 3), ("medium_albedo", C.c_float * 3), ("medium_g", C.c_float),
-        ("medium_scale", C.c_float), ("medium_sigma_max", C.c_float),
+        ("medium_scale", C.c_float), ("medium_sigma_max", C.c_float), ("textures", C.c_int32),
     ]
 
 
@@ -175,6 +183,10 @@ def material_struct(m):
     s.eta_int = float(m.get("eta_int", 1.5046))
     s.eta_ext = float(m.get("eta_ext", 1.000277))
     s.transmitted[:] = [float(x) for x in m.get("transmitted", (1.0, 1.0, 1.0))]
+    # image textures: 1-based indices into the scene's texture list, 0 = the constant above
+    s.tex_albedo = int(m.get("tex_albedo", 0))
+    s.tex_alpha = int(m.get("tex_alpha", 0))
+    s.tex_transmitted = int(m.get("tex_transmitted", 0))
     return s
 
 
@@ -384,6 +396,17 @@ class Scene:
             desc.env_rgb = _ptr(env); desc.env_height, desc.env_width = env.shape[0], env.shape[1]
             desc.env_scale = float(sd.env_scale)
         desc.light_strategy = int(sd.light_strategy)
+        texs = getattr(sd, "textures", None) or []
+        if texs:
+            tarr = (rtg_texture_desc * len(texs))(); keep.append(tarr)
+            for i, t in enumerate(texs):
+                rgb = _f32(t["rgb"]); keep.append(rgb)
+                if rgb.ndim != 3 or rgb.shape[2] != 3:
+                    raise ValueError("texture rgb must be (height, width, 3)")
+                tarr[i].rgb = _ptr(rgb); tarr[i].height, tarr[i].width = rgb.shape[0], rgb.shape[1]
+                tarr[i].uv_scale = float(t.get("uv_scale", 1.0)); tarr[i].value_scale = float(t.get("value_scale", 1.0))
+                tarr[i].invert_x = int(t.get("invert_x", 0)); tarr[i].invert_y = int(t.get("invert_y", 0))
+            desc.textures = C.cast(tarr, C.c_void_p); desc.num_textures = len(texs)
         med = getattr(sd, "medium", None)
         if med is not None:
             grid = _f32(med["density"]); keep.append(grid)

@@ -61,6 +61,9 @@ class SceneDesc:
     render: dict = field(default_factory=dict)
     # camera medium over the world box [0,1]^3: density (z, y, x) float32, albedo (3), g, scale
     medium: Optional[dict] = None
+    # image textures: dicts of rgb (H x W x 3 linear), uv_scale, invert_x, invert_y, value_scale;
+    # materials refer to them by 1-based index (tex_albedo, tex_alpha, tex_transmitted)
+    textures: List[dict] = field(default_factory=list)
 
     def add_material(self, **m):
         self.materials.append(m)
@@ -358,6 +361,42 @@ def vol_cornell(width=512, height=512, spp=128, max_depth=16, min_depth=3, seed=
     return sd
 
 
+def checker_texture(n=16, cells=4, lo=(0.1, 0.1, 0.1), hi=(0.9, 0.8, 0.6), smooth=False):
+    """An n x n RGB texture: a checkerboard of `cells` x `cells` squares, or (smooth) a linear ramp
+    lo + (hi - lo) * (i + j) / (2 (n - 1)) over texel column i and row j."""
+    lo, hi = np.array(lo, np.float32), np.array(hi, np.float32)
+    j, i = np.meshgrid(np.arange(n), np.arange(n), indexing="ij")
+    if smooth:
+        t = ((i + j) / (2.0 * (n - 1)))[..., None]
+    else:
+        t = (((i * cells // n) + (j * cells // n)) % 2)[..., None].astype(np.float32)
+    return np.ascontiguousarray((lo + (hi - lo) * t).astype(np.float32))
+
+
+def textured_cornell(width=256, height=256, spp=64, max_depth=8, min_depth=3, seed=7):
+    """Cornell box with image textures: a checkerboard albedo on the back wall (uvs over the quad),
+    a smooth ramp albedo on the short box (barycentric uvs), a rough-conductor sphere whose
+    roughness comes from a texture (spherical uvs), and a ScaleTexture-style value scale."""
+    sd = cornell(width, height, spp, max_depth, min_depth, seed)
+    sd.textures.append(dict(rgb=checker_texture(32, 8), uv_scale=1.0))
+    sd.textures.append(dict(rgb=checker_texture(16, smooth=True), uv_scale=2.0, value_scale=0.9))
+    sd.textures.append(dict(rgb=checker_texture(8, 2, lo=(0.15, 0.15, 0.15), hi=(0.5, 0.5, 0.5)), invert_y=1))
+    back = sd.add_material(**dict(diffuse((0.5, 0.5, 0.5)), tex_albedo=1))
+    ramp = sd.add_material(**dict(diffuse((0.5, 0.5, 0.5)), tex_albedo=2))
+    rough = sd.add_material(**dict(conductor(0.2), tex_alpha=3))
+    # the walls (floor, ceiling, back, left, right; 4 vertices and 2 triangles each) get per-quad
+    # uvs; the back wall's triangles the checkerboard material
+    walls = sd.meshes[0]
+    walls.uvs = np.tile(np.array([[0, 0], [1, 0], [1, 1], [0, 1]], np.float32), (5, 1))
+    tm = np.array(walls.tri_material, np.int32)
+    tm[4:6] = back
+    walls.tri_material = tm
+    sd.meshes[1].material = ramp  # short box: no uvs, its barycentrics are the uvs
+    sd.spheres.append(SphereDesc(np.array([[1, 0, 0, -0.2], [0, 1, 0, 0.75], [0, 0, 1, 0.1], [0, 0, 0, 1]], np.float32),
+                                 0.12, rough))
+    return sd
+
+
 def glossy_cornell(width=512, height=512, spp=64, max_depth=1000, min_depth=3, seed=5):
     """Glossy-indirect scene: rough dielectric spheres (IOR 1.5, alpha 0.1), rough GGX conductor walls
     (alpha 0.15), a white diffuse floor and ceiling, and a 0.05x0.05 light 0.1 below the ceiling that
@@ -517,18 +556,43 @@ def write_xml(sd, directory, name="scene.xml"):
     if sd.medium is not None:
         out += _medium_xml(sd.medium, directory)
 
+    def image(name, ti, ind):  # ImageTexture (in a ScaleTexture when it has a value scale)
+        t = sd.textures[ti - 1]
+        fn = f"tex{ti}.hdr"
+        write_rgbe(os.path.join(directory, fn), t["rgb"])
+        img = [f'<texture name="{{}}" type="ImageTexture"><string name="path" value="{fn}"/>'
+               f'<double name="scale" value="{float(t.get("uv_scale", 1.0))!r}"/>'
+               f'<bool name="invertX" value="{"true" if t.get("invert_x") else "false"}"/>'
+               f'<bool name="invertY" value="{"true" if t.get("invert_y") else "false"}"/></texture>']
+        vs = float(t.get("value_scale", 1.0))
+        if vs == 1.0:
+            return [ind + "  " + img[0].format(name)]
+        return [f'{ind}  <texture name="{name}" type="ScaleTexture"><double name="scale" value="{vs!r}"/>'
+                + img[0].format("texture") + "</texture>"]
+
     def bsdf(i, ind="      "):
         m = sd.materials[i]
         dist = "ggx" if m.get("distribution", R.DIST_BECKMANN) == R.DIST_GGX else "beckmann"
         if m["type"] == R.MAT_DIFFUSE:
+            if m.get("tex_albedo", 0):
+                return [f'{ind}<bsdf type="Diffuse">'] + image("albedo", m["tex_albedo"], ind) + [f"{ind}</bsdf>"]
             return [f'{ind}<bsdf type="Diffuse">', f'{ind}  <color name="albedo" value="{_fmt(m["albedo"])}"/>',
                     f"{ind}</bsdf>"]
-        def tex(name, key):  # constant textures (texture/ConstantTexture.cpp:6-10)
+
+        def tex(name, key):  # constant textures (texture/ConstantTexture.cpp:6-10) or images
+            tk = {"albedo": "tex_albedo", "transmitted": "tex_transmitted"}[key]
+            if m.get(tk, 0):
+                return image(name, m[tk], ind)
             c = tuple(float(x) for x in m.get(key, (1.0, 1.0, 1.0)))
             if c == (1.0, 1.0, 1.0):
                 return []
             return [f'{ind}  <texture name="{name}" type="ConstantTexture"><color name="albedo" value="{_fmt(c)}"/>'
                     "</texture>"]
+
+        def alpha():
+            if m.get("tex_alpha", 0):
+                return image("alpha", m["tex_alpha"], ind)
+            return [f'{ind}  <double name="alpha" value="{float(m["alpha"])!r}"/>']
 
         if m["type"] == R.MAT_CONDUCTOR:
             return ([f'{ind}<bsdf type="Conductor">', f'{ind}  <color name="eta" value="{_fmt(m["eta"])}"/>',
@@ -539,12 +603,12 @@ def write_xml(sd, directory, name="scene.xml"):
                      f'{ind}  <double name="etaExt" value="{float(m["eta_ext"])!r}"/>']
                     + tex("reflectanceTexture", "albedo") + tex("transmittedTexture", "transmitted") + [f"{ind}</bsdf>"])
         if m["type"] == R.MAT_ROUGH_CONDUCTOR:
-            return ([f'{ind}<bsdf type="RoughConductor">', f'{ind}  <double name="alpha" value="{float(m["alpha"])!r}"/>',
+            return ([f'{ind}<bsdf type="RoughConductor">'] + alpha() + [
                      f'{ind}  <string name="distribution" value="{dist}"/>',
                      f'{ind}  <color name="eta" value="{_fmt(m["eta"])}"/>',
                      f'{ind}  <color name="absorption" value="{_fmt(m["k"])}"/>']
                     + tex("reflectanceTexture", "albedo") + [f"{ind}</bsdf>"])
-        return ([f'{ind}<bsdf type="RoughDielectric">', f'{ind}  <double name="alpha" value="{float(m["alpha"])!r}"/>',
+        return ([f'{ind}<bsdf type="RoughDielectric">'] + alpha() + [
                  f'{ind}  <string name="distribution" value="{dist}"/>',
                  f'{ind}  <double name="etaInt" value="{float(m["eta_int"])!r}"/>',
                  f'{ind}  <double name="etaExt" value="{float(m["eta_ext"])!r}"/>']
@@ -571,14 +635,23 @@ def write_xml(sd, directory, name="scene.xml"):
                 for v in verts.tolist():
                     f.write(f"v {v[0]!r} {v[1]!r} {v[2]!r}\n")
                 has_n = mesh.normals is not None
+                has_t = mesh.uvs is not None
                 if has_n:
                     for v in np.asarray(mesh.normals, np.float64).reshape(-1, 3)[used[order]].tolist():
                         f.write(f"vn {v[0]!r} {v[1]!r} {v[2]!r}\n")
+                if has_t:
+                    for v in np.asarray(mesh.uvs, np.float32).reshape(-1, 2)[used[order]].astype(np.float64).tolist():
+                        f.write(f"vt {v[0]!r} {v[1]!r}\n")
                 for t in tri + 1:
-                    if has_n:
-                        f.write(f"f {t[0]}//{t[0]} {t[1]}//{t[1]} {t[2]}//{t[2]}\n")
+                    a, b, c = t
+                    if has_n and has_t:
+                        f.write(f"f {a}/{a}/{a} {b}/{b}/{b} {c}/{c}/{c}\n")
+                    elif has_t:
+                        f.write(f"f {a}/{a} {b}/{b} {c}/{c}\n")
+                    elif has_n:
+                        f.write(f"f {a}//{a} {b}//{b} {c}//{c}\n")
                     else:
-                        f.write(f"f {t[0]} {t[1]} {t[2]}\n")
+                        f.write(f"f {a} {b} {c}\n")
             out += ['    <shape type="Mesh">', f'      <string name="path" value="{fn}"/>']
             out += bsdf(int(mat))
             if mesh.emission is not None:

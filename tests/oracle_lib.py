@@ -58,6 +58,9 @@ def load():
             "orc_filter_value": (C.c_float, [P, C.c_float, C.c_float]),
             "orc_set_medium": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, P, C.c_float, C.c_float]),
             "orc_set_integrator": (None, [P, C.c_int]),
+            "orc_add_texture": (C.c_int, [P, P, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float]),
+            "orc_texture_eval": (None, [P, C.c_int, P, C.c_int, P]),
+            "orc_material_at": (None, [P, P, C.c_int, P, P]),
             "orc_medium_sigma": (None, [P, P, C.c_int, P]),
             "orc_medium_track": (None, [P, C.c_int, P, C.c_int, P, C.c_int, P]),
             "orc_hg": (None, [C.c_float, C.c_int, P, P, C.c_int, P]),
@@ -121,6 +124,9 @@ def material_params(m):
     p[12] = m.get("eta_ext", 1.000277)
     p[13] = m.get("distribution", 0)
     p[14:17] = m.get("transmitted", (1, 1, 1))
+    p[17] = m.get("tex_albedo", 0)
+    p[18] = m.get("tex_alpha", 0)
+    p[19] = m.get("tex_transmitted", 0)
     return p
 
 
@@ -130,6 +136,11 @@ class OracleScene:
     def __init__(self, sd):
         self.lib = load()
         self.h = self.lib.orc_scene_new()
+        for t in getattr(sd, "textures", None) or []:
+            rgb = f32(t["rgb"])
+            self.lib.orc_add_texture(self.h, _p(rgb), rgb.shape[1], rgb.shape[0], float(t.get("uv_scale", 1.0)),
+                                     int(t.get("invert_x", 0)), int(t.get("invert_y", 0)),
+                                     float(t.get("value_scale", 1.0)))
         for m in sd.materials:
             mp = material_params(m)
             self.lib.orc_add_material(self.h, _p(mp))
@@ -162,6 +173,19 @@ class OracleScene:
             self.lib.orc_scene_free(self.h)
         except Exception:
             pass
+
+    def texture_eval(self, ti, uv):
+        uv = f32(uv, (-1, 2)); out = np.zeros((uv.shape[0], 3), np.float32)
+        self.lib.orc_texture_eval(self.h, int(ti), _p(uv), uv.shape[0], _p(out))
+        return out
+
+    def material_at(self, rays):
+        """Material at the closest hit with textures evaluated -> (n x 8: albedo/reflectance rgb, alpha,
+        transmitted rgb, 0), (n x 2 texture uv)."""
+        rays = f32(rays, (-1, 8)); n = rays.shape[0]
+        out = np.zeros((n, 8), np.float32); uv = np.zeros((n, 2), np.float32)
+        self.lib.orc_material_at(self.h, _p(rays), n, _p(out), _p(uv))
+        return out, uv
 
     def medium_sigma(self, pts):
         pts = f32(pts, (-1, 3)); out = np.zeros(pts.shape[0], np.float32)

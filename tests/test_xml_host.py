@@ -279,6 +279,36 @@ def case_volume(ctx, tmp):
         assert msg in str(e.value), (msg, str(e.value))
 
 
+def case_textures(ctx, tmp):
+    """ImageTexture / ScaleTexture elements over .hdr images (written by scenes.write_xml for the
+    textured Cornell box, with per-quad uvs in the OBJ files) load as the scene's image textures;
+    other texture types, non-RGBE images and unknown channels fail with the element named."""
+    from raytracer_amd import scenes
+
+    sd = scenes.textured_cornell(width=16, height=16)
+    sc, _, _ = ctx.load_xml(scenes.write_xml(sd, os.path.join(tmp, "t"), name="t.xml"))
+    i = sc.info()
+    assert i["textures"] == 3 and i["spheres"] == 1, i
+    scenes.write_rgbe(os.path.join(tmp, "img.hdr"), np.full((4, 4, 3), 0.5, np.float32))
+    ok = ("<objects><shape type='Cube'><bsdf type='Diffuse'><texture name='albedo' type='ScaleTexture'>"
+          "<double name='scale' value='2'/><texture name='texture' type='ImageTexture'>"
+          "<string name='path' value='img.hdr'/><string name='channel' value='g'/></texture></texture></bsdf>"
+          "</shape></objects>")
+    assert ctx.load_xml(_scene(tmp, ok, name="ok.xml"))[0].info()["textures"] == 1
+    with open(os.path.join(tmp, "img.png"), "wb") as f:
+        f.write(b"\x89PNG\r\n")
+    bad = [("<texture name='albedo' type='CheckerTexture'/>", "CheckerTexture"),
+           ("<texture name='albedo' type='ImageTexture'><string name='path' value='img.png'/></texture>",
+            "not a Radiance RGBE image"),
+           ("<texture name='albedo' type='ImageTexture'><string name='path' value='img.hdr'/>"
+            "<string name='channel' value='luminance'/></texture>", "channel")]
+    for tex, msg in bad:
+        body = f"<objects><shape type='Cube'><bsdf type='Diffuse'>{tex}</bsdf></shape></objects>"
+        with pytest.raises(Exception) as e:
+            ctx.load_xml(_scene(tmp, body, name="bad.xml"))
+        assert msg in str(e.value), (msg, str(e.value))
+
+
 def case_errors(ctx, tmp):
     from raytracer_amd import rtg
 
@@ -289,7 +319,7 @@ def case_errors(ctx, tmp):
         ("<objects><shape type='Disk'/></objects>", "not rendered by the GPU path"),
         ("<objects><shape type='Cube'><bsdf type='Phong'/></shape></objects>", "bsdf type Phong"),
         ("<objects><shape type='Cube'><bsdf type='Diffuse'><texture name='albedo' type='Checkerboard'/>"
-         "</bsdf></shape></objects>", "ConstantTexture only"),
+         "</bsdf></shape></objects>", "ConstantTexture, ImageTexture, ScaleTexture"),
         ("<integrator type='PathTracing'/>", "PathTracingMIS and VolPathTracing only"),
         ("<objects><shape type='Cube'><bsdf ref='nothing'/></shape></objects>", "unknown bsdf ref"),
         ("<objects><shape type='mesh'><string name='path' value='bad.obj'/></shape></objects>",
@@ -343,7 +373,7 @@ def decode_rgbe(path):
 
 # ------------------------------------------------------------------ pytest entry points
 @pytest.mark.parametrize("case", ["case_cornell_roundtrip", "case_glossy_roundtrip", "case_env_roundtrip",
-                                  "case_obj_features", "case_shapes_refs_textures", "case_smooth_bsdfs", "case_filters", "case_volume", "case_errors",
+                                  "case_obj_features", "case_shapes_refs_textures", "case_smooth_bsdfs", "case_filters", "case_volume", "case_textures", "case_errors",
                                   "case_rgbe"])
 def test_xml_loader(mock_lib, case, tmp_path):
     _run(mock_lib, case, tmp_path)
