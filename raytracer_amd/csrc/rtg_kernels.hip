@@ -822,7 +822,7 @@ __device__ __forceinline__ bool bsdf_sample_any(const DevMaterial& m, f3 wi, flo
 }
 __device__ __forceinline__ float max3(f3 v) { return fmaxf(v.x, fmaxf(v.y, v.z)); }
 
-__global__ void __launch_bounds__(256) k_vol(DevPool P, const DevScene* scenes, const DevJob* jobs) {
+__global__ void __launch_bounds__(256, 3) k_vol(DevPool P, const DevScene* scenes, const DevJob* jobs) {
   uint32_t* C = P.counters;
   const uint32_t n = C[C_EXT];
   const uint32_t stride = gridDim.x * blockDim.x;
@@ -841,8 +841,6 @@ __global__ void __launch_bounds__(256) k_vol(DevPool P, const DevScene* scenes, 
     HitRec h;
     h.t = hv.x; h.b1 = hv.y; h.b2 = hv.z; h.prim = __float_as_int(hv.w);
     const bool hit = h.prim >= 0;
-    Surface sf;
-    if (hit) sf = surface_at(S, h, o, d);
     const float tEnd = hit ? h.t : (depth == 0 ? MAX_REAL : va.w);
     if (hit || depth == 0) va.w = tEnd;
     const PathRng rng = make_rng(J.seed, P.pixel[slot], P.sample[slot], J);
@@ -855,6 +853,9 @@ __global__ void __launch_bounds__(256) k_vol(DevPool P, const DevScene* scenes, 
       f3 Le = mk(0.f, 0.f, 0.f);
       float pl = 0.f;
       if (hit) {
+        // the surface is built again for the vertex below: not keeping it live across the
+        // tracking loops saves registers
+        const Surface sf = surface_at(S, h, o, d);
         if (sf.light != 0xffffffffu) {
           caught = true;
           const DevLight& LL = S.lights[sf.light];
@@ -934,14 +935,15 @@ __global__ void __launch_bounds__(256) k_vol(DevPool P, const DevScene* scenes, 
           L.x += tp.x * Le.x; L.y += tp.y * Le.y; L.z += tp.z * Le.z;
         }
         done = true;
-      } else if (sf.material == 0xffffffffu) {
+      } else if (S.prim_info[h.prim].y == 0xffffffffu) {
         // a shape without a BSDF is passed through (:187-195): same direction, same depth, no roulette
-        vp = sf.p;
+        vp = surface_at(S, h, o, d).p;
         nmin = EPS;
         nF = make_float4(1.f, 1.f, 1.f, 0.f);
         pass = true;
       } else {
         // surface vertex (:185-330)
+        const Surface sf = surface_at(S, h, o, d);
         vp = sf.p;
         if (emission && sf.light != 0xffffffffu && dot(-d, sf.sh.n) > 0.f) {
           const f3 c = cmul(tp, mat_col(S.lights[sf.light].radiance));
