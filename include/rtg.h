@@ -140,6 +140,11 @@ typedef struct rtg_scene_desc {
   float medium_scale;          /* density multiplier ("scale"; <= 0 takes the default 1) */
   const rtg_texture_desc* textures;  /* referenced by rtg_material.tex_* (1-based) */
   int32_t num_textures;
+  /* 1: the camera medium is Homogeneous (medium/Homogeneous.cpp: "sigmaA", "sigmaS", "scale" as
+   * medium_scale; unbounded; medium_density must be NULL; medium_g as above) */
+  int32_t medium_homogeneous;
+  float medium_sigma_a[3];
+  float medium_sigma_s[3];
 } rtg_scene_desc;
 
 /* Camera (camera/NoriPerspective.cpp:10-37 + core/Transform.cpp:199-227 lookAt),
@@ -242,6 +247,8 @@ typedef struct rtg_scene_info {
   float medium_albedo[3];
   float medium_g, medium_scale, medium_sigma_max;
   int32_t textures;            /* image textures */
+  int32_t medium_kind;         /* 0 none, 1 heterogeneous grid, 2 homogeneous */
+  float medium_sigma_a[3], medium_sigma_s[3];  /* homogeneous, after the scale */
 } rtg_scene_info;
 int rtg_scene_query(rtg_scene* scene, rtg_scene_info* info);
 

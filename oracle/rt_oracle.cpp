@@ -1066,11 +1066,32 @@ struct Texture {
 // setOwnerBBox, so its world-to-array transform is the identity and its world box is [0,1]^3
 // (Heterogeneous.cpp:15-41).
 struct Medium {
+    bool homogeneous = false;  // medium/Homogeneous.cpp (unbounded, chromatic coefficients)
+    C3 sigA = c3(0.f), sigS = c3(0.f), sigT = c3(0.f);
     std::vector<float> data;  // x fastest, then y, then z
     int res[3] = {0, 0, 0};
     C3 albedo = c3(0.f);
     float g = 0.f, scale = 1.f, maxSigmaT = 0.f;
-    bool present() const { return !data.empty(); }
+    bool present() const { return homogeneous || !data.empty(); }
+    static C3 expC(C3 c) { return c3(std::exp(c.r), std::exp(c.g), std::exp(c.b)); }
+    // Homogeneous.cpp:26-60: channel chosen uniformly, exponential distance with its extinction
+    // (+EPSILON) from minT compared with maxT as an absolute distance, spectral-MIS weights
+    bool sampleDistanceHomogeneous(const Ray& ray, Sampler& sampler, float& t, C3& weight) const {
+        int channel = (int)(sampler.next1D() * 3);
+        float sig = (channel == 0 ? sigT.r : channel == 1 ? sigT.g : sigT.b) + EPSILON;
+        t = -std::log(1.f - sampler.next1D()) / sig;
+        if (t > ray.maxT) {
+            t = ray.maxT;
+            float pdf = average(expC(-1.f * sigT * t));
+            weight = expC(-1.f * sigT * (ray.maxT - ray.minT)) / pdf;
+            return false;
+        }
+        float pdf = average(sigT * expC(-1.f * sigT * t));
+        float len = t;
+        t += ray.minT;
+        weight = sigS * expC(-1.f * sigT * len) / pdf;
+        return true;
+    }
     // GridVolume.cpp:96-105 getMaxSigmaT x scale (Heterogeneous.cpp:40)
     void finalize() {
         float m = data[0];
@@ -1099,6 +1120,7 @@ struct Medium {
     // collisions before the box entry meet zero density and are all rejected, so starting at the
     // entry draws the same distribution. Bounded at 16384 tentative collisions.
     bool sampleDistance(const Ray& ray, Sampler& sampler, float& t, C3& weight) const {
+        if (homogeneous) return sampleDistanceHomogeneous(ray, sampler, t, weight);
         Box box{v3(0, 0, 0), v3(1, 1, 1)};
         float nearT, farT;
         weight = c3(1.f);
@@ -1121,6 +1143,11 @@ struct Medium {
         }
         weight = sigma != 0.f ? albedo : c3(0.f);
         return sigma != 0.f;
+    }
+    // Homogeneous.cpp:26-30 (exp(-sigma_t (maxT - minT))) or the ratio tracking below, per channel
+    C3 transmittanceC(const Ray& ray, Sampler& sampler) const {
+        if (homogeneous) return expC(-1.f * sigT * (ray.maxT - ray.minT));
+        return c3(transmittance(ray, sampler));
     }
     // Heterogeneous.cpp:419-489 (ratio tracking with roulette below 0.1; length maxT - minT)
     float transmittance(const Ray& ray, Sampler& sampler) const {
@@ -1816,12 +1843,12 @@ struct VolPathTracer {
 
     const Medium* medium() const { return scene->medium.present() ? &scene->medium : nullptr; }
     // Integrator.cpp:81-135
-    float evalTransmittance(const Ray& shadowRay, Sampler& sampler) {
+    C3 evalTransmittance(const Ray& shadowRay, Sampler& sampler) {
         Hit tmp;
         rays++;
-        if (scene->intersect(shadowRay, tmp, true)) return 0.f;
+        if (scene->intersect(shadowRay, tmp, true)) return c3(0.f);
         const Medium* m = medium();
-        return m ? m->transmittance(shadowRay, sampler) : 1.f;
+        return m ? m->transmittanceC(shadowRay, sampler) : c3(1.f);
     }
     // Integrator.cpp:137-170
     C3 sampleAttenuatedLightDirect(V3 p, Sampler& sampler, Scene::LightSample& info) {
@@ -1833,7 +1860,7 @@ struct VolPathTracer {
         C3 value = c3(0.f);
         if (info.pdf > 0 && luminance(info.intensity) > 0) {
             Ray shadowRay = makeRay(p, info.toLight, EPSILON, info.distance - EPSILON);
-            float tr = evalTransmittance(shadowRay, sampler);
+            C3 tr = evalTransmittance(shadowRay, sampler);
             info.pdf *= lightWeight;
             value = tr * info.intensity / info.pdf;
         }
@@ -1886,8 +1913,8 @@ struct VolPathTracer {
                 // evalTransmittanceFromSampledBSDF (:348-423)
                 rays++;
                 wasIntersected = scene->intersect(ray, intersection, false);
-                float tr = med->transmittance(makeRay(ray.o, ray.d, ray.minT, wasIntersected ? intersection.t : ray.maxT),
-                                              sampler);
+                C3 tr = med->transmittanceC(makeRay(ray.o, ray.d, ray.minT, wasIntersected ? intersection.t : ray.maxT),
+                                            sampler);
                 if (wasIntersected) {
                     int hl = scene->primLight(intersection.prim);
                     if (hl >= 0) {
@@ -1899,6 +1926,7 @@ struct VolPathTracer {
                 }
             } else {
                 // :159-326
+                if (med) throughput = throughput * weightMedium;  // :160-165
                 if (!wasIntersected) {
                     if (emission && env >= 0) radiance += throughput * scene->envLe(scene->lights[env], ray.d);
                     return radiance;
@@ -1934,9 +1962,9 @@ struct VolPathTracer {
                 ray = makeRay(hit.p, toWorld(hit.sh, bi.wo));
                 rays++;
                 wasIntersected = scene->intersect(ray, intersection, false);
-                float tr = med ? med->transmittance(
-                                     makeRay(ray.o, ray.d, ray.minT, wasIntersected ? intersection.t : ray.maxT), sampler)
-                               : 1.f;
+                C3 tr = med ? med->transmittanceC(
+                                  makeRay(ray.o, ray.d, ray.minT, wasIntersected ? intersection.t : ray.maxT), sampler)
+                            : c3(1.f);
                 int caught = -1;
                 C3 radianceLight = c3(0.f);
                 if (wasIntersected) {
@@ -2616,6 +2644,21 @@ int orc_set_medium(void* sp, const float* data, int rx, int ry, int rz, const fl
     m.finalize();
     return 0;
 }
+// Homogeneous camera medium: sigmaA, sigmaS (rgb), HG g, scale (<= 0: 1)
+int orc_set_medium_homogeneous(void* sp, const float* sigA, const float* sigS, float g, float scale) {
+    Medium& m = ((Scene*)sp)->medium;
+    m.homogeneous = true;
+    float sc = scale > 0.f ? scale : 1.f;
+    m.sigA = c3(sigA[0], sigA[1], sigA[2]);
+    m.sigS = c3(sigS[0], sigS[1], sigS[2]);
+    if (sc != 1.f) {  // Homogeneous.cpp:16-20
+        m.sigA = m.sigA * sc;
+        m.sigS = m.sigS * sc;
+    }
+    m.sigT = m.sigA + m.sigS;
+    m.g = g;
+    return 0;
+}
 // 0 PathTracingMIS, 1 VolPathTracing
 void orc_set_integrator(void* sp, int type) { ((Scene*)sp)->integrator = type; }
 // GridVolume::lookupSigmaT x scale at n points
@@ -2623,8 +2666,9 @@ void orc_medium_sigma(void* sp, const float* pts, int n, float* out) {
     const Medium& m = ((Scene*)sp)->medium;
     for (int i = 0; i < n; i++) out[i] = m.lookupSigmaT(v3(pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]));
 }
-// Trackers on n rays (ox oy oz dx dy dz minT maxT) with K given 1D draws per ray.
-// op 0 delta tracking: out = (scattered, t, draws used); op 1 ratio tracking: out = (tr, draws used, 0)
+// Trackers on n rays (ox oy oz dx dy dz minT maxT) with K given 1D draws per ray; out n x 6.
+// op 0 distance sampling: (scattered, t, draws used, weight rgb);
+// op 1 transmittance: (tr (red channel), draws used, 0, tr rgb)
 void orc_medium_track(void* sp, int op, const float* rays, int n, const float* u, int K, float* out) {
     const Medium& m = ((Scene*)sp)->medium;
     for (int i = 0; i < n; i++) {
@@ -2635,17 +2679,21 @@ void orc_medium_track(void* sp, int op, const float* rays, int n, const float* u
         smp.n1 = K;
         smp.u2 = nullptr;
         smp.n2 = 0;
+        float* o = out + 6 * i;
         if (op == 0) {
             float t;
             C3 w;
             bool sc = m.sampleDistance(ray, smp, t, w);
-            out[3 * i] = sc ? 1.f : 0.f;
-            out[3 * i + 1] = t;
-            out[3 * i + 2] = (float)smp.i1;
+            o[0] = sc ? 1.f : 0.f;
+            o[1] = t;
+            o[2] = (float)smp.i1;
+            o[3] = w.r; o[4] = w.g; o[5] = w.b;
         } else {
-            out[3 * i] = m.transmittance(ray, smp);
-            out[3 * i + 1] = (float)smp.i1;
-            out[3 * i + 2] = 0.f;
+            C3 tr = m.transmittanceC(ray, smp);
+            o[0] = tr.r;
+            o[1] = (float)smp.i1;
+            o[2] = 0.f;
+            o[3] = tr.r; o[4] = tr.g; o[5] = tr.b;
         }
     }
 }

@@ -1307,6 +1307,54 @@ __device__ __forceinline__ float medium_transmittance(const DevScene& S, f3 o, f
   }
 }
 
+// The camera medium's two operations for the integrator, for either kind (RGB weights):
+// Homogeneous (medium/Homogeneous.cpp:26-60) samples a channel uniformly, an exponential distance
+// with that channel's extinction (+1e-4) measured from minT but compared with maxT as an absolute
+// distance, and returns the spectral-MIS weight sigma_s T / avg(sigma_t T) on a collision,
+// T / avg(T) otherwise (T = exp(-sigma_t t)); its transmittance is exp(-sigma_t (maxT - minT)),
+// without bounds. Heterogeneous: delta tracking (weight = albedo on a collision, 1 otherwise) and
+// ratio tracking as above.
+template <class U>
+__device__ __forceinline__ bool medium_distance(const DevScene& S, f3 o, f3 d, float minT, float maxT, U& u,
+                                                float& t, f3& weight) {
+  if (S.medium_kind == 2) {
+    const float uc = u();
+    const int ch = min((int)(uc * 3.f), 2);
+    const float sig = (ch == 0 ? S.sig_t[0] : ch == 1 ? S.sig_t[1] : S.sig_t[2]) + EPS;
+    t = -logf(1.f - u()) / sig;
+    const f3 st = mk(S.sig_t[0], S.sig_t[1], S.sig_t[2]);
+    if (t > maxT) {
+      t = maxT;
+      const f3 e = mk(expf(-st.x * t), expf(-st.y * t), expf(-st.z * t));
+      const float pdf = (e.x + e.y + e.z) / 3.f;
+      const float len = maxT - minT;
+      weight = mk(expf(-st.x * len), expf(-st.y * len), expf(-st.z * len)) / pdf;
+      return false;
+    }
+    const f3 e = mk(expf(-st.x * t), expf(-st.y * t), expf(-st.z * t));
+    const f3 se = cmul(st, e);
+    const float pdf = (se.x + se.y + se.z) / 3.f;
+    const float len = t;  // transmittance over [minT, minT + t]
+    t += minT;
+    weight = cmul(mk(S.sig_s[0], S.sig_s[1], S.sig_s[2]), mk(expf(-st.x * len), expf(-st.y * len), expf(-st.z * len))) /
+             pdf;
+    return true;
+  }
+  weight = mk(1.f, 1.f, 1.f);
+  const bool sc = medium_sample_distance(S, o, d, minT, maxT, u, t);
+  if (sc) weight = mk(S.albedo[0], S.albedo[1], S.albedo[2]);
+  return sc;
+}
+template <class U>
+__device__ __forceinline__ f3 medium_tr(const DevScene& S, f3 o, f3 d, float minT, float maxT, U& u) {
+  if (S.medium_kind == 2) {
+    const float len = maxT - minT;
+    return mk(expf(-S.sig_t[0] * len), expf(-S.sig_t[1] * len), expf(-S.sig_t[2] * len));
+  }
+  const float tr = medium_transmittance(S, o, d, minT, maxT, u);
+  return mk(tr, tr, tr);
+}
+
 // HGPhaseFunction.cpp:61-79: cos = (-wi).wo
 __device__ __forceinline__ float hg_eval(float g, f3 wi, f3 wo) {
   const float c = dot(-wi, wo);

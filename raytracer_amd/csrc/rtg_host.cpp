@@ -525,6 +525,24 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   D.env_weight = upload(ctx, env_w, own, bytes);
   // camera medium: GridVolume data as given (GridVolume.cpp:7-64), majorant of
   // Heterogeneous.cpp:40 (GridVolume::getMaxSigmaT x scale)
+  if (desc->medium_homogeneous) {
+    // medium/Homogeneous.cpp:7-22 (the scale multiplies both coefficients)
+    if (desc->medium_density) fail(RTG_ERR_INVALID, "a homogeneous medium takes no density grid");
+    const float scale = desc->medium_scale > 0.f ? desc->medium_scale : 1.f;
+    for (int k = 0; k < 3; k++) {
+      const float a = desc->medium_sigma_a[k], sc = desc->medium_sigma_s[k];
+      if (!(std::isfinite(a) && std::isfinite(sc) && a >= 0.f && sc >= 0.f))
+        fail(RTG_ERR_INVALID, "medium coefficients must be finite and non-negative");
+      D.sig_a[k] = scale != 1.f ? a * scale : a;
+      D.sig_s[k] = scale != 1.f ? sc * scale : sc;
+      D.sig_t[k] = D.sig_a[k] + D.sig_s[k];
+    }
+    if (!std::isfinite(desc->medium_g) || desc->medium_g >= 1.f || desc->medium_g <= -1.f)
+      fail(RTG_ERR_INVALID, "phase function g must lie in (-1, 1)");
+    D.hg_g = desc->medium_g;
+    D.gscale = scale;
+    D.medium_kind = 2;
+  }
   if (desc->medium_density) {
     const int* r = desc->medium_res;
     if (r[0] < 2 || r[1] < 2 || r[2] < 2) fail(RTG_ERR_INVALID, "medium grid must be at least 2x2x2");
@@ -551,6 +569,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
     D.gscale = scale;
     D.sigma_max = mx * scale;
     D.hg_g = desc->medium_g;
+    D.medium_kind = 1;
   }
   hip_check(hipMemcpyAsync(ctx->scenes_dev + slot, &D, sizeof(DevScene), hipMemcpyHostToDevice, ctx->stream),
             "scene table");
@@ -860,6 +879,15 @@ int rtg_scene_query(rtg_scene* scene, rtg_scene_info* info) {
     info->bbox_max[k] = scene->dev.root_max[k];
   }
   info->textures = scene->ntex;
+  info->medium_kind = scene->dev.medium_kind;
+  for (int k = 0; k < 3; k++) {
+    info->medium_sigma_a[k] = scene->dev.sig_a[k];
+    info->medium_sigma_s[k] = scene->dev.sig_s[k];
+  }
+  if (scene->dev.medium_kind == 2) {
+    info->medium_g = scene->dev.hg_g;
+    info->medium_scale = scene->dev.gscale;
+  }
   if (scene->dev.grid) {
     for (int k = 0; k < 3; k++) {
       info->medium_res[k] = scene->dev.gres[k];

@@ -98,10 +98,13 @@ struct DevScene {
   const uint8_t* tri_has_uv;
   // camera medium (medium/Heterogeneous.cpp, volume/GridVolume.cpp, medium/HGPhaseFunction.cpp):
   // density grid over the world box [0,1]^3, x fastest; sigma_max = max density x scale
-  const float* grid;         // null: no medium
+  const float* grid;         // null: no heterogeneous medium
   int32_t gres[3];
   float gscale, sigma_max, hg_g;
   float albedo[3];
+  // homogeneous camera medium (medium/Homogeneous.cpp), scaled: medium_kind 2
+  int32_t medium_kind;       // 0 none, 1 heterogeneous grid, 2 homogeneous
+  float sig_a[3], sig_s[3], sig_t[3];
 };
 
 // Per-render (per-request) state, one slot of the job table.

@@ -837,7 +837,7 @@ __global__ void __launch_bounds__(256, 3) k_vol(DevPool P, const DevScene* scene
     const float4 F = P.F[slot];
     float4 va = P.vol_a[slot];
     const f3 o = mk(ro.x, ro.y, ro.z), d = mk(rd.x, rd.y, rd.z);
-    const bool medium = S.grid != nullptr;
+    const bool medium = S.medium_kind != 0;
     HitRec h;
     h.t = hv.x; h.b1 = hv.y; h.b2 = hv.z; h.prim = __float_as_int(hv.w);
     const bool hit = h.prim >= 0;
@@ -876,9 +876,9 @@ __global__ void __launch_bounds__(256, 3) k_vol(DevPool P, const DevScene* scene
       }
       if (caught) {
         // evalTransmittanceFromSampledBSDF (:372-423): ratio tracking over [minT, hit or maxT]
-        const float tr = medium ? medium_transmittance(S, o, d, ro.w, hit ? h.t : rd.w, u) : 1.f;
+        const f3 tr = medium ? medium_tr(S, o, d, ro.w, hit ? h.t : rd.w, u) : mk(1.f, 1.f, 1.f);
         const float w = power_heuristic(fabsf(F.w), pl);
-        const f3 c = (cmul(cmul(tp, Le), mk(F.x, F.y, F.z)) * tr) * w;
+        const f3 c = cmul(cmul(cmul(tp, Le), mk(F.x, F.y, F.z)), tr) * w;
         L.x += c.x; L.y += c.y; L.z += c.z;
       }
       tp = cmul(tp, mk(F.x, F.y, F.z)) / L.w;
@@ -893,10 +893,11 @@ __global__ void __launch_bounds__(256, 3) k_vol(DevPool P, const DevScene* scene
     if (!done) {
       const bool emission = depth == 0;  // RadianceType::EMISSION in the first iteration only
       float ts = 0.f;
-      const bool scat = medium && medium_sample_distance(S, o, d, ro.w, tEnd, u, ts);
+      f3 wm = mk(1.f, 1.f, 1.f);
+      const bool scat = medium && medium_distance(S, o, d, ro.w, tEnd, u, ts, wm);
+      tp = cmul(tp, wm);  // the medium's weight, with or without a collision (:64-66, :160-165)
       if (scat) {
         // medium vertex (:60-103)
-        tp = cmul(tp, mk(S.albedo[0], S.albedo[1], S.albedo[2]));
         vp = o + ts * d;
         if (S.nlights > 0) {
           float ux, uy;
@@ -909,9 +910,9 @@ __global__ void __launch_bounds__(256, 3) k_vol(DevPool P, const DevScene* scene
           va.x = ls.point.x; va.y = ls.point.y; va.z = ls.point.z;
           P.vol_b[slot] = make_float4(ls.normal.x, ls.normal.y, ls.normal.z, 0.f);
           if (ls.pdf > 0.f && luminance(ls.intensity) > 0.f) {
-            const float tr = medium_transmittance(S, vp, ls.dir, EPS, ls.dist - EPS, u);
+            const f3 tr = medium_tr(S, vp, ls.dir, EPS, ls.dist - EPS, u);
             const float lpdf = ls.pdf * lw;
-            const f3 value = (tr * ls.intensity) / lpdf;
+            const f3 value = cmul(tr, ls.intensity) / lpdf;
             if (!is_zero(value)) {
               const float pf = hg_eval(S.hg_g, ls.dir, -d);
               const float w = power_heuristic(lpdf, pf);
@@ -961,9 +962,9 @@ __global__ void __launch_bounds__(256, 3) k_vol(DevPool P, const DevScene* scene
           const LightSample ls =
               (LL.type == LIGHT_ENV) ? env_sample(S, sf.p, ux, uy) : mesh_light_sample(S, LL, sf.p, ux, uy);
           if (ls.pdf > 0.f && luminance(ls.intensity) > 0.f) {
-            const float tr = medium ? medium_transmittance(S, sf.p, ls.dir, EPS, ls.dist - EPS, u) : 1.f;
+            const f3 tr = medium ? medium_tr(S, sf.p, ls.dir, EPS, ls.dist - EPS, u) : mk(1.f, 1.f, 1.f);
             const float lpdf = ls.pdf * lw;
-            const f3 value = (tr * ls.intensity) / lpdf;
+            const f3 value = cmul(tr, ls.intensity) / lpdf;
             if (!is_zero(value)) {
               const f3 wi_l = to_local(sf.sh, ls.dir);
               const float cosT = fabsf(wi_l.z);

@@ -396,6 +396,8 @@ struct Loader {
   int gres[3] = {0, 0, 0};
   float medium_albedo[3] = {0.f, 0.f, 0.f};
   float medium_g = 0.f, medium_scale = 1.f;
+  bool medium_homogeneous = false;
+  float medium_sigma_a[3] = {0.f, 0.f, 0.f}, medium_sigma_s[3] = {0.f, 0.f, 0.f};
 
   std::string resolve(const std::string& p) const {
     if (p.empty() || p[0] == '/') return p;
@@ -988,40 +990,60 @@ struct Loader {
   }
 
   // The scene's camera medium (XMLLoader::handleCameraMedium, core/XMLLoader.cpp:159-172):
-  // Heterogeneous (medium/Heterogeneous.cpp:15-41: "albedo" default 0, "scale" default 1; as a
+  // Homogeneous (below), or Heterogeneous (medium/Heterogeneous.cpp:15-41: "albedo" default 0, "scale" default 1; as a
   // camera medium its "toWorld" and "centered" do not apply) over a GridVolume, with the
   // "phaseFunction" HGPhaseFunction ("g", default -0.8, HGPhaseFunction.cpp:7-10) or, by default,
   // IsotropicPhaseFunction (Medium.cpp:7-10; the same density 1/(4 pi) and uniform sampling as
   // HG with g = 0, which stands for it here).
   void camera_medium(const XNode& c) {
     const std::string* t = c.get("type");
+    if (type_of(c) == "homogeneous") {
+      // medium/Homogeneous.cpp:7-22: "sigmaA", "sigmaS" (default 0.15 each), "scale";
+      // emission ("temperature" > 0) is not rendered
+      Props P = props(c, "medium Homogeneous");
+      static const float d15[3] = {0.15f, 0.15f, 0.15f};
+      P.color("sigmaA", d15, medium_sigma_a);
+      P.color("sigmaS", d15, medium_sigma_s);
+      medium_scale = (float)P.real("scale", 1.0);
+      if (P.real("temperature", 0.0) > 0.0) perr("medium Homogeneous: emission (temperature) is not rendered");
+      medium_homogeneous = true;
+      medium_g = phase(c);
+      return;
+    }
     if (type_of(c) != "heterogeneous")
-      perr("medium " + (t ? *t : std::string("?")) + " is not rendered by the GPU path (Heterogeneous only)");
+      perr("medium " + (t ? *t : std::string("?")) +
+           " is not rendered by the GPU path (Heterogeneous and Homogeneous only)");
     Props P = props(c, "medium Heterogeneous");
     static const float zero[3] = {0.f, 0.f, 0.f};
     P.color("albedo", zero, medium_albedo);
     medium_scale = (float)P.real("scale", 1.0);
     const XNode* vol = nullptr;
-    const XNode* ph = nullptr;
     for (const XNode& k : c.kids) {
       const std::string* nm = k.get("name");
       if (k.name == "volume" && nm && *nm == "volume") vol = &k;
-      if (k.name == "phase" && nm && *nm == "phaseFunction") ph = &k;
     }
     if (!vol) perr("medium Heterogeneous: no <volume name=\"volume\">");
     if (type_of(*vol) != "gridvolume") perr("medium Heterogeneous: volume type must be GridVolume");
     Props V = props(*vol, "volume GridVolume");
     read_vol(resolve(V.str("path", "")));
-    medium_g = 0.f;
-    if (ph) {
-      const std::string pt = type_of(*ph);
-      if (pt == "hgphasefunction") {
-        Props H = props(*ph, "phase HGPhaseFunction");
-        medium_g = (float)H.real("g", -0.8);
-      } else if (pt != "isotropicphasefunction") {
-        perr("phase function type must be HGPhaseFunction or IsotropicPhaseFunction");
-      }
+    medium_g = phase(c);
+  }
+
+  // The medium's "phaseFunction" (Medium.cpp:7-10): HG g (default -0.8), or isotropic (g = 0)
+  float phase(const XNode& c) {
+    const XNode* ph = nullptr;
+    for (const XNode& k : c.kids) {
+      const std::string* nm = k.get("name");
+      if (k.name == "phase" && nm && *nm == "phaseFunction") ph = &k;
     }
+    if (!ph) return 0.f;
+    const std::string pt = type_of(*ph);
+    if (pt == "hgphasefunction") {
+      Props H = props(*ph, "phase HGPhaseFunction");
+      return (float)H.real("g", -0.8);
+    }
+    if (pt != "isotropicphasefunction") perr("phase function type must be HGPhaseFunction or IsotropicPhaseFunction");
+    return 0.f;
   }
 
   // volume/GridVolume.cpp:7-64: "VOL", version 3, encoding, resolution x y z, channels, bounding
@@ -1085,6 +1107,15 @@ extern "C" int rtg_scene_load_xml(rtg_ctx* ctx, const char* path, rtg_scene** ou
     for (size_t i = 0; i < L.textures.size(); i++) L.textures[i].rgb = L.texdata[i].data();
     sd.textures = L.textures.empty() ? nullptr : L.textures.data();
     sd.num_textures = (int32_t)L.textures.size();
+    if (L.medium_homogeneous) {
+      sd.medium_homogeneous = 1;
+      for (int k = 0; k < 3; k++) {
+        sd.medium_sigma_a[k] = L.medium_sigma_a[k];
+        sd.medium_sigma_s[k] = L.medium_sigma_s[k];
+      }
+      sd.medium_g = L.medium_g;
+      sd.medium_scale = L.medium_scale;
+    }
     if (!L.grid.empty()) {
       sd.medium_density = L.grid.data();
       for (int k = 0; k < 3; k++) {

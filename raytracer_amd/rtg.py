@@ -56,6 +56,7 @@ class rtg_scene_desc(C.Structure):
         ("env_height", C.c_int32), ("env_scale", C.c_float), ("light_strategy", C.c_int32),
         ("medium_density", C.c_void_p), ("medium_res", C.c_int32 * 3), ("medium_albedo", C.c_float * 3),
         ("medium_g", C.c_float), ("medium_scale", C.c_float), ("textures", C.c_void_p), ("num_textures", C.c_int32),
+        ("medium_homogeneous", C.c_int32), ("medium_sigma_a", C.c_float * 3), ("medium_sigma_s", C.c_float * 3),
     ]
 
 
@@ -88,6 +89,7 @@ class rtg_scene_info(C.Structure):
         ("env_width", C.c_int32), ("env_height", C.c_int32), ("bbox_min", C.c_float * 3), ("bbox_max", C.c_float * 3),
         ("medium_res", C.c_int32 * 3), ("medium_albedo", C.c_float * 3), ("medium_g", C.c_float),
         ("medium_scale", C.c_float), ("medium_sigma_max", C.c_float), ("textures", C.c_int32),
+        ("medium_kind", C.c_int32), ("medium_sigma_a", C.c_float * 3), ("medium_sigma_s", C.c_float * 3),
     ]
 
 
@@ -408,7 +410,14 @@ class Scene:
                 tarr[i].invert_x = int(t.get("invert_x", 0)); tarr[i].invert_y = int(t.get("invert_y", 0))
             desc.textures = C.cast(tarr, C.c_void_p); desc.num_textures = len(texs)
         med = getattr(sd, "medium", None)
-        if med is not None:
+        if med is not None and med.get("homogeneous"):
+            desc.medium_homogeneous = 1
+            for k in range(3):
+                desc.medium_sigma_a[k] = float(med["sigma_a"][k])
+                desc.medium_sigma_s[k] = float(med["sigma_s"][k])
+            desc.medium_g = float(med.get("g", 0.0))
+            desc.medium_scale = float(med.get("scale", 1.0))
+        elif med is not None:
             grid = _f32(med["density"]); keep.append(grid)
             if grid.ndim != 3:
                 raise ValueError("medium density must be a (z, y, x) array")
@@ -431,7 +440,8 @@ class Scene:
     def info(self):
         i = rtg_scene_info()
         self.ctx.check(self.ctx.lib.rtg_scene_query(self.h, C.byref(i)))
-        return {k: (list(getattr(i, k)) if k.startswith("bbox") or k in ("medium_res", "medium_albedo") else getattr(i, k))
+        return {k: (list(getattr(i, k)) if k.startswith("bbox") or k in ("medium_res", "medium_albedo", "medium_sigma_a",
+                                                                          "medium_sigma_s") else getattr(i, k))
                 for k, _ in i._fields_}
 
     def bytes(self):

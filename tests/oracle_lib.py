@@ -58,6 +58,7 @@ def load():
             "orc_filter_value": (C.c_float, [P, C.c_float, C.c_float]),
             "orc_set_medium": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, P, C.c_float, C.c_float]),
             "orc_set_integrator": (None, [P, C.c_int]),
+            "orc_set_medium_homogeneous": (C.c_int, [P, P, P, C.c_float, C.c_float]),
             "orc_add_texture": (C.c_int, [P, P, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float]),
             "orc_texture_eval": (None, [P, C.c_int, P, C.c_int, P]),
             "orc_material_at": (None, [P, P, C.c_int, P, P]),
@@ -160,7 +161,11 @@ class OracleScene:
             self.lib.orc_set_env(self.h, _p(env), env.shape[1], env.shape[0], float(sd.env_scale))
         self.nprim = self.lib.orc_finalize(self.h, int(sd.light_strategy))
         med = getattr(sd, "medium", None)
-        if med is not None:
+        if med is not None and med.get("homogeneous"):
+            sa, ss = f32(med["sigma_a"], (3,)), f32(med["sigma_s"], (3,))
+            self.lib.orc_set_medium_homogeneous(self.h, _p(sa), _p(ss), float(med.get("g", 0.0)),
+                                                float(med.get("scale", 1.0)))
+        elif med is not None:
             g = f32(med["density"])
             alb = f32(med.get("albedo", (0.0, 0.0, 0.0)), (3,))
             self.lib.orc_set_medium(self.h, _p(g), g.shape[2], g.shape[1], g.shape[0], _p(alb), float(med.get("g", 0.0)),
@@ -193,9 +198,10 @@ class OracleScene:
         return out
 
     def medium_track(self, op, rays, u):
-        """op 0 delta tracking -> (scattered, t, draws used); op 1 ratio tracking -> (tr, draws used, 0)."""
+        """op 0 distance sampling -> (scattered, t, draws used, weight rgb); op 1 transmittance ->
+        (tr, draws used, 0, tr rgb)."""
         rays = f32(rays, (-1, 8)); u = f32(u); n = rays.shape[0]
-        u = u.reshape(n, -1); out = np.zeros((n, 3), np.float32)
+        u = u.reshape(n, -1); out = np.zeros((n, 6), np.float32)
         self.lib.orc_medium_track(self.h, int(op), _p(rays), n, _p(u), u.shape[1], _p(out))
         return out
 

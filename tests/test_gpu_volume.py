@@ -126,3 +126,35 @@ def test_vol_cornell_full_size_runs(ctx):
           f"{st.extension_rays} extension rays, mean {img.mean():.4f}", flush=True)
     assert st.samples == 512 * 512 * 16 and st.nan_samples == 0
     assert 0.01 < img.mean() < 5.0
+
+
+def test_homogeneous_fog_closed_form(ctx):
+    """The oracle test's chromatic fog on the GPU: absorption 0.5 x (0.1, 0.3, 0.6) over the distance
+    3 to an emitter of radiance 2: per channel 2 exp(-sigma_t 3) within 4 standard errors."""
+    import test_oracle_volume as TV
+    Le = 2.0
+    v, i = scenes.quad((-5.0, -5.0, 3.0), (0.0, 10.0, 0.0), (10.0, 0.0, 0.0), (0.0, 0.0, -1.0))
+    em = scenes.MeshDesc(v, i, material=0, emission=(Le, Le, Le))
+    cam = dict(origin=(0.0, 0.0, 0.0), target=(0.0, 0.0, 1.0), up=(0.0, 1.0, 0.0), fov=1.0, width=32, height=32)
+    rp = dict(spp=1024, sampler=rtg.SAMPLER_RANDOM, max_depth=4, min_depth=3, seed=3,
+              integrator=rtg.INTEGRATOR_VOLPATH)
+    sd = TV._homog_scene((0.1, 0.3, 0.6), (0.0, 0.0, 0.0), [em], cam, rp, scale=0.5)
+    print("[volume] homogeneous fog 32x32 1024spp", flush=True)
+    img, st = gpu_image(ctx.upload(sd), cam, rp)
+    stc = 0.5 * np.array([0.1, 0.3, 0.6])
+    want = Le * np.exp(-stc * 3.0)
+    p = np.mean(np.exp(-stc * 3.0))
+    se = want / p * math.sqrt(p * (1 - p) / (32 * 32 * 1024))
+    m = img.reshape(-1, 3).mean(0)
+    print(f"  mean {m} want {want} se {se}", flush=True)
+    assert st.nan_samples == 0 and np.all(np.abs(m - want) < 4 * se)
+
+
+def test_fog_cornell_unbiased(ctx):
+    """The Cornell box in a scattering homogeneous camera medium (sigma_a 0.3, sigma_s 0.6 per unit,
+    HG g 0.3): block means of the GPU render at 8192 spp within the error bars of 8 CPU renders of
+    128 spp."""
+    sd = scenes.cornell(width=32, height=32)
+    sd.medium = dict(homogeneous=True, sigma_a=(0.3, 0.3, 0.3), sigma_s=(0.6, 0.5, 0.4), g=0.3, scale=1.0)
+    sd.render = dict(sd.render, integrator=rtg.INTEGRATOR_VOLPATH, max_depth=16)
+    _block_z_test(ctx, sd, sd.camera, 128, 8, 8192, "fog cornell (homogeneous camera medium)")

@@ -270,7 +270,16 @@ def case_volume(ctx, tmp):
         assert j["medium_res"] == [3, 3, 3] and j["medium_sigma_max"] == 1.0, j
         assert abs(j["medium_g"] - want_g) < 1e-7 and np.allclose(j["medium_albedo"], 0.0)
         assert rp2["integrator"] == R.INTEGRATOR_VOLPATH
-    bad = [("<medium type='Homogeneous'/>", "Heterogeneous only"),
+    # the reference's fog (Scenes/cbox_fog.xml): a Homogeneous camera medium, scaled coefficients
+    fog = ("<integrator type='VolPathTracing'/><medium type='Homogeneous'><color name='sigmaA' value='0.002 0.002 0.002'/>"
+           "<color name='sigmaS' value='0.0005 0.0005 0.0005'/><double name='scale' value='2'/></medium>"
+           "<objects><shape type='Cube'/></objects>")
+    f = ctx.load_xml(_scene(tmp, fog, name="fog.xml"))[0].info()
+    assert f["medium_kind"] == 2 and np.allclose(f["medium_sigma_a"], 0.004) and np.allclose(f["medium_sigma_s"], 0.001)
+    d = ctx.load_xml(_scene(tmp, "<medium type='Homogeneous'/><objects><shape type='Cube'/></objects>", name="d.xml"))[0]
+    assert np.allclose(d.info()["medium_sigma_a"], 0.15) and d.info()["medium_g"] == 0.0
+    bad = [("<medium type='Homogeneous'><double name='temperature' value='1000'/></medium>", "temperature"),
+           ("<medium type='Participating'/>", "Heterogeneous and Homogeneous only"),
            ("<medium type='Heterogeneous'><volume name='volume' type='OpenVDBVolume'/></medium>", "GridVolume"),
            ("<integrator type='DirectLighting'/>", "and VolPathTracing only")]
     for body, msg in bad:
