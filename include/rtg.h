@@ -278,7 +278,7 @@ int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const
                    int32_t n, float* out_dir, float* f, float* pdf, float* rel_eta);
 
 /* Parity tap for the scene's camera medium (medium/Heterogeneous.cpp, volume/GridVolume.cpp,
- * medium/HGPhaseFunction.cpp); the scene must have one.
+ * medium/HGPhaseFunction.cpp); the scene must have one, a grid one for ops 0-2.
  * op 0 GridVolume::lookupSigmaT x scale: in = n points (x 3) -> out n.
  * op 1 Heterogeneous::sampleDistance (delta tracking) and op 2 Heterogeneous::transmittance (ratio
  *   tracking): in = n rays (x 8: origin, direction, minT, maxT), u = k given 1D draws per ray (a ray

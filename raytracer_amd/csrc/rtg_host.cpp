@@ -1156,7 +1156,9 @@ int rtg_medium_batch(rtg_ctx* ctx, rtg_scene* scene, int32_t op, const float* in
   std::lock_guard<std::mutex> lk(ctx->mtx);
   RTG_GUARD(ctx, {
     if (!scene || !in || !out || n < 0 || op < 0 || op > 4) fail(RTG_ERR_INVALID, "bad medium arguments");
-    if (!scene->dev.grid) fail(RTG_ERR_INVALID, "the scene has no medium");
+    if (op <= 2 && scene->dev.medium_kind != 1)
+      fail(RTG_ERR_INVALID, "density and tracking queries need a heterogeneous (grid) medium");
+    if (scene->dev.medium_kind == 0) fail(RTG_ERR_INVALID, "the scene has no medium");
     if ((op == 1 || op == 2) && (k < 0 || (k > 0 && !u))) fail(RTG_ERR_INVALID, "tracking needs k draws per ray");
     if (op == 4 && (!u || k != 2)) fail(RTG_ERR_INVALID, "phase sampling needs 2 draws per item");
     if (n == 0) return RTG_OK;
