@@ -50,6 +50,8 @@ enum { RTG_LIGHTS_ALL = 0, RTG_LIGHTS_UNIFORM = 1, RTG_LIGHTS_WEIGHTED = 2 };
 /* Samplers: sampler/RandomSampler.cpp, sampler/StratifiedSampler.cpp */
 enum { RTG_FILTER_BOX = 0, RTG_FILTER_GAUSSIAN = 1, RTG_FILTER_MITCHELL = 2 };
 enum { RTG_SAMPLER_RANDOM = 0, RTG_SAMPLER_STRATIFIED = 1 };
+/* core/main.cpp:185-290 tone mappers */
+enum { RTG_TONEMAP_CLAMP = 0, RTG_TONEMAP_GAMMA = 1, RTG_TONEMAP_REINHARD = 2 };
 /* integrator/PathTracingMIS.cpp, integrator/VolPathTracing.cpp */
 enum { RTG_INTEGRATOR_PATH_MIS = 0, RTG_INTEGRATOR_VOLPATH = 1 };
 
@@ -259,6 +261,15 @@ int rtg_render(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, floa
 /* Same with a caller-owned device buffer (width*height*4 floats), not cleared. */
 int rtg_render_device(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, float* out_rgbw_device,
                       rtg_stats* stats);
+
+/* The image the reference displays and saves (core/main.cpp:185-290, after Pixel::applyColor,
+ * core/Screen.cpp:8-15): rgbw (width*height*4 floats, host, as rtg_render returns it) is resolved
+ * (rgb / w where w != 0), then RTG_TONEMAP_CLAMP (x 255), RTG_TONEMAP_GAMMA (times
+ * 2^((exposure - 0.5) * 20), then the sRGB curve; the reference's default exposure is 0.5) or
+ * RTG_TONEMAP_REINHARD (global: L' = L (1 + L / Lmax^2) / (1 + L) on the luminance, then sRGB),
+ * clamped to [0, 255] and truncated to 8 bits -> out_rgb (width*height*3 bytes, host). */
+int rtg_tonemap(rtg_ctx* ctx, const float* rgbw, int32_t width, int32_t height, int32_t mode, float exposure,
+                uint8_t* out_rgb);
 
 /* Parity tap for Scene::computeIntersection (core/Scene.cpp:106-113, accelStruct/Bvh.cpp:392-481).
  * rays: n x 8 floats (origin, direction, minT, maxT). prim: id or -1. t/uv/n may be NULL.

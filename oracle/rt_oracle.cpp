@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <atomic>
 #include <cmath>
+#include <limits>
 #include <cstdint>
 #include <cstring>
 #include <memory>
@@ -2710,6 +2711,50 @@ void orc_hg(float g, int op, const float* wi, const float* x, int n, float* out)
             V3 wo = m.hgSample(a, V2{x[2 * i], x[2 * i + 1]}, pdf);
             out[4 * i] = wo.x; out[4 * i + 1] = wo.y; out[4 * i + 2] = wo.z; out[4 * i + 3] = pdf;
         }
+    }
+}
+
+// core/main.cpp:185-290 tone mappers after Pixel::applyColor (core/Screen.cpp:8-15):
+// mode 0 clamp, 1 gamma (sRGB, exposure), 2 global Reinhard (sRGB); out w*h*3 bytes
+void orc_tonemap(const float* rgbw, int w, int h, int mode, float exposure, unsigned char* out) {
+    const int n = w * h;
+    auto color = [&](int i) {
+        C3 c = c3(rgbw[4 * i], rgbw[4 * i + 1], rgbw[4 * i + 2]);
+        float wt = rgbw[4 * i + 3];
+        if (wt != 0.f) c = c / wt;
+        return c;
+    };
+    auto toSRGB = [](float v) {  // Color.h:450-461
+        if (v < 0.0031308) return 12.92f * v;
+        return 1.055f * std::pow(v, 0.41666f) - 0.055f;
+    };
+    auto byte = [](float v) {  // thresholding(v * 255, 0, 255) as sf::Uint8 (NaN: 0)
+        float x = v * 255;
+        float c = x < 0.f ? 0.f : (x > 255.f ? 255.f : x);
+        return (unsigned char)(c == c ? (int)c : 0);
+    };
+    float expo = (float)std::pow(2., (exposure - 0.5) * 20);  // main.cpp:193
+    float maxLum = std::numeric_limits<float>::min();
+    if (mode == 2)
+        for (int i = 0; i < n; i++) maxLum = std::max(maxLum, luminance(color(i)));
+    for (int i = 0; i < n; i++) {
+        C3 c = color(i);
+        float r, g, b;
+        if (mode == 0) {
+            r = c.r; g = c.g; b = c.b;
+        } else if (mode == 1) {
+            C3 e = c * expo;
+            r = toSRGB(e.r); g = toSRGB(e.g); b = toSRGB(e.b);
+        } else {
+            float oldLum = luminance(c);
+            float numerator = oldLum * (1.0 + (oldLum / (maxLum * maxLum)));
+            float lum = numerator / (1.0f + oldLum);
+            C3 f = c * lum / oldLum;
+            r = toSRGB(f.r); g = toSRGB(f.g); b = toSRGB(f.b);
+        }
+        out[3 * i] = byte(r);
+        out[3 * i + 1] = byte(g);
+        out[3 * i + 2] = byte(b);
     }
 }
 

@@ -1147,6 +1147,38 @@ int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int3
   })
 }
 
+int rtg_tonemap(rtg_ctx* ctx, const float* rgbw, int32_t width, int32_t height, int32_t mode, float exposure,
+                uint8_t* out_rgb) {
+  if (!ctx) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  RTG_GUARD(ctx, {
+    if (!rgbw || !out_rgb || width <= 0 || height <= 0 || mode < RTG_TONEMAP_CLAMP || mode > RTG_TONEMAP_REINHARD)
+      fail(RTG_ERR_INVALID, "bad tone-mapping arguments");
+    if ((int64_t)width * height > (1ll << 28)) fail(RTG_ERR_INVALID, "image too large");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const int n = width * height;
+    // gammaToneMapper (core/main.cpp:193): exposure = pow(2., (exposure - 0.5) * 20) in double
+    const float expo = (float)std::pow(2., ((double)exposure - 0.5) * 20);
+    std::vector<void*> own;
+    float4* dfilm = (float4*)dev_alloc(ctx, (size_t)n * 16, own);
+    uint8_t* dout = (uint8_t*)dev_alloc(ctx, (size_t)n * 3, own);
+    unsigned int* dmax = (unsigned int*)dev_alloc(ctx, 4, own);
+    try {
+      const unsigned int zero = 0;
+      hip_check(hipMemcpyAsync(dfilm, rgbw, (size_t)n * 16, hipMemcpyHostToDevice, ctx->stream), "film");
+      hip_check(hipMemcpyAsync(dmax, &zero, 4, hipMemcpyHostToDevice, ctx->stream), "max");
+      launch_tonemap(dfilm, n, mode, expo, dmax, dout, ctx->stream);
+      hip_check(hipGetLastError(), "tonemap launch");
+      hip_check(hipMemcpyAsync(out_rgb, dout, (size_t)n * 3, hipMemcpyDeviceToHost, ctx->stream), "out");
+      hip_check(hipStreamSynchronize(ctx->stream), "tonemap");
+    } catch (...) {
+      for (void* q : own) (void)hipFree(q);
+      throw;
+    }
+    for (void* q : own) (void)hipFree(q);
+  })
+}
+
 // values per item of rtg_medium_batch's input and output, by op
 static const int kMediumIn[5] = {3, 8, 8, 6, 3}, kMediumOut[5] = {1, 3, 3, 1, 4};
 

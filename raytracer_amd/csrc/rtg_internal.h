@@ -236,6 +236,8 @@ void launch_iteration_vol(const DevPool& pool, const DevScene* scenes_dev, DevJo
 void launch_reset_pool(const DevPool& pool, hipStream_t stream);
 void launch_trace_tap(const DevScene* scene_dev, const float* rays, int n, int any_hit, int32_t* prim,
                       float* t, float* uv, float* p, float* nrm, const LaunchCfg& cfg, hipStream_t stream);
+void launch_tonemap(const float4* film, int n, int mode, float expo, unsigned int* maxbits, uint8_t* out,
+                    hipStream_t stream);
 void launch_medium_tap(const DevScene* scene_dev, int op, const float* in, int n, const float* u, int k, float* out,
                        hipStream_t stream);
 void launch_bsdf_tap(const DevMaterial* mat, int op, const float* wi, const float* wo_or_u, int n,

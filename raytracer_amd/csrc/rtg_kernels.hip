@@ -1268,7 +1268,59 @@ __global__ void k_medium_tap(const DevScene* scene, int op, const float* in, int
   }
 }
 
+// ---------------------------------------------------------------- tone mapping (core/main.cpp:185-290)
+__device__ __forceinline__ f3 resolved(const float4 a) {  // Pixel::applyColor (core/Screen.cpp:8-15)
+  return a.w != 0.f ? mk(a.x / a.w, a.y / a.w, a.z / a.w) : mk(a.x, a.y, a.z);
+}
+__device__ __forceinline__ float to_srgb(float v) {  // Color.h:450-461
+  if (v < 0.0031308f) return 12.92f * v;
+  return 1.055f * powf(v, 0.41666f) - 0.055f;
+}
+// math::thresholding(v * 255, 0, 255) converted to sf::Uint8 (truncation; NaN gives 0 here)
+__device__ __forceinline__ uint8_t to_byte(float v) {
+  const float x = v * 255.f;
+  const float c = x < 0.f ? 0.f : (x > 255.f ? 255.f : x);
+  return (c == c) ? (uint8_t)(int)c : (uint8_t)0;
+}
+// maximum luminance, starting from the smallest normal float (numeric_limits<real>::min()):
+// every candidate is then positive, so the unsigned bit patterns order like the values
+__global__ void k_tonemap_max(const float4* film, int n, unsigned int* maxbits) {
+  float m = 1.17549435e-38f;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+    m = fmaxf(m, luminance(resolved(film[i])));
+  for (int off = 32; off > 0; off >>= 1) m = fmaxf(m, __shfl_down(m, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(maxbits, __float_as_uint(m));
+}
+__global__ void k_tonemap(const float4* film, int n, int mode, float expo, const unsigned int* maxbits,
+                          uint8_t* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const f3 c = resolved(film[i]);
+  float r, g, b;
+  if (mode == 0) {  // EXRToClampedRGB
+    r = c.x; g = c.y; b = c.z;
+  } else if (mode == 1) {  // gammaToneMapper with gamma -1 (sRGB)
+    const f3 e = c * expo;
+    r = to_srgb(e.x); g = to_srgb(e.y); b = to_srgb(e.z);
+  } else {  // globalReinhardToneMapper with gamma -1
+    const float mx = __uint_as_float(*maxbits);
+    const float oldLum = luminance(c);
+    const float numerator = (float)((double)oldLum * (1.0 + (double)(oldLum / (mx * mx))));
+    const float lum = numerator / (1.0f + oldLum);
+    const f3 f = (c * lum) / oldLum;
+    r = to_srgb(f.x); g = to_srgb(f.y); b = to_srgb(f.z);
+  }
+  out[3 * i] = to_byte(r);
+  out[3 * i + 1] = to_byte(g);
+  out[3 * i + 2] = to_byte(b);
+}
+
 // ---------------------------------------------------------------- launchers
+void launch_tonemap(const float4* film, int n, int mode, float expo, unsigned int* maxbits, uint8_t* out,
+                    hipStream_t stream) {
+  if (mode == 2) hipLaunchKernelGGL(k_tonemap_max, dim3(1024), dim3(256), 0, stream, film, n, maxbits);
+  hipLaunchKernelGGL(k_tonemap, dim3((n + 255) / 256), dim3(256), 0, stream, film, n, mode, expo, maxbits, out);
+}
 void launch_medium_tap(const DevScene* scene_dev, int op, const float* in, int n, const float* u, int k, float* out,
                        hipStream_t stream) {
   hipLaunchKernelGGL(k_medium_tap, dim3((n + 127) / 128), dim3(128), 0, stream, scene_dev, op, in, n, u, k, out);

@@ -19,6 +19,7 @@ FILTER_BOX, FILTER_GAUSSIAN, FILTER_MITCHELL = 0, 1, 2
 LIGHTS_ALL, LIGHTS_UNIFORM, LIGHTS_WEIGHTED = 0, 1, 2
 SAMPLER_RANDOM, SAMPLER_STRATIFIED = 0, 1
 INTEGRATOR_PATH_MIS, INTEGRATOR_VOLPATH = 0, 1
+TONEMAP_CLAMP, TONEMAP_GAMMA, TONEMAP_REINHARD = 0, 1, 2
 
 
 class rtg_material(C.Structure):
@@ -124,6 +125,7 @@ SIGNATURES = {
     "rtg_trace": (C.c_int, [P, P, P, C.c_int32, C.c_int32, P, P, P, P, P]),
     "rtg_bsdf_batch": (C.c_int, [P, C.POINTER(rtg_material), C.c_int32, P, P, C.c_int32, P, P, P, P]),
     "rtg_medium_batch": (C.c_int, [P, P, C.c_int32, P, C.c_int32, P, C.c_int32, P]),
+    "rtg_tonemap": (C.c_int, [P, P, C.c_int32, C.c_int32, C.c_int32, C.c_float, P]),
     "rtg_server_create": (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
     "rtg_server_destroy": (C.c_int, [P]),
     "rtg_submit": (C.c_int, [P, P, C.POINTER(rtg_render_desc), C.POINTER(C.c_int64)]),
@@ -275,6 +277,14 @@ class Context:
                   max_depth=d.max_depth, seed=d.seed, integrator=d.integrator,
                   filter=dict(type=d.filter, radius=tuple(d.filter_radius[:]), param=tuple(d.filter_param[:])))
         return Scene.from_handle(self, h), cam, rp
+
+    def tonemap(self, rgbw, mode=TONEMAP_CLAMP, exposure=0.5):
+        """The reference's displayed / saved 8-bit image of a film (rtg_tonemap) -> H x W x 3 uint8."""
+        f = _f32(rgbw)
+        H, W = f.shape[0], f.shape[1]
+        out = np.zeros((H, W, 3), np.uint8)
+        self.check(self.lib.rtg_tonemap(self.h, _ptr(f), W, H, int(mode), float(exposure), _ptr(out)))
+        return out
 
     def set_pool(self, slots):
         self.check(self.lib.rtg_ctx_set_pool(self.h, int(slots)))

@@ -34,6 +34,7 @@ void launch_iteration_vol(const DevPool& P, const DevScene* S, DevJob* jobs, con
   launch_iteration(P, S, jobs, order, norder, cfg, st, nullptr, nullptr);
 }
 void launch_medium_tap(const DevScene*, int, const float*, int, const float*, int, float*, hipStream_t) {}
+void launch_tonemap(const float4*, int, int, float, unsigned int*, uint8_t*, hipStream_t) {}
 void launch_reset_pool(const DevPool&, hipStream_t) {}
 void launch_trace_tap(const DevScene*, const float*, int, int, int32_t*, float*, float*, float*, float*,
                       const LaunchCfg&, hipStream_t) {}

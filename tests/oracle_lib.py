@@ -59,6 +59,7 @@ def load():
             "orc_set_medium": (C.c_int, [P, P, C.c_int, C.c_int, C.c_int, P, C.c_float, C.c_float]),
             "orc_set_integrator": (None, [P, C.c_int]),
             "orc_set_medium_homogeneous": (C.c_int, [P, P, P, C.c_float, C.c_float]),
+            "orc_tonemap": (None, [P, C.c_int, C.c_int, C.c_int, C.c_float, P]),
             "orc_add_texture": (C.c_int, [P, P, C.c_int, C.c_int, C.c_float, C.c_int, C.c_int, C.c_float]),
             "orc_texture_eval": (None, [P, C.c_int, P, C.c_int, P]),
             "orc_material_at": (None, [P, P, C.c_int, P, P]),
@@ -411,3 +412,12 @@ def guide_loss_grad(params, samples, emulate_bf16=True):
     g = np.zeros(GUIDE_PARAMS)
     loss = lib.gor_loss_grad(_p(params), int(emulate_bf16), _p(samples), samples.shape[0], _p(g))
     return loss, g
+
+
+def tonemap(rgbw, mode, exposure=0.5):
+    """The oracle's tone mappers (core/main.cpp:185-290) -> H x W x 3 uint8."""
+    f = f32(rgbw)
+    H, W = f.shape[0], f.shape[1]
+    out = np.zeros((H, W, 3), np.uint8)
+    load().orc_tonemap(_p(f), W, H, int(mode), float(exposure), _p(out))
+    return out

@@ -574,3 +574,52 @@ def test_filter_table_values():
                 got = O.filter_value(f, x + dx, -(y + dx))
                 assert abs(got - want) <= 1e-5 * max(1.0, abs(want)), (f["type"], ix, iy, got, want)
         assert O.filter_value(f, 2 * rx, 0.0) == O.filter_value(f, 31 * rx / 32, 0.0)  # clamped to the last cell
+
+
+def _srgb_byte(v):
+    """The sRGB curve of Color.h:450-461 in float32, then x 255, clamped, truncated."""
+    v = np.asarray(v, np.float32)
+    f = np.float32
+    s = np.where(v < f(0.0031308), f(12.92) * v, f(1.055) * np.power(np.maximum(v, f(0)), f(0.41666)) - f(0.055))
+    return np.clip(s.astype(np.float32) * f(255), 0, 255).astype(np.uint8)
+
+
+def test_tone_mappers_closed_forms():
+    """core/main.cpp:185-290 after Pixel::applyColor (rgb / w where w != 0, core/Screen.cpp:8-15):
+    clamp maps 0.5 to 127 and 1.5 to 255; the gamma mapper at exposure 0.5 is the sRGB curve
+    (linear 0.18 -> 117, 0.5 -> 187; 1 -> 254, as 1.055 - 0.055 rounds below 1 in float and the
+    byte is truncated; the linear toe below 0.0031308) and an exposure step of 0.05 doubles the input; Reinhard maps the brightest pixel to luminance 1 and keeps
+    each pixel's chromaticity (a black pixel gives 0)."""
+    vals = np.array([0.0, 0.001, 0.0031, 0.18, 0.5, 1.0, 1.5, 4.0], np.float32)
+    w = 4.0
+    film = np.zeros((1, len(vals), 4), np.float32)
+    film[0, :, :3] = (vals * w)[:, None]
+    film[0, :, 3] = w
+    clamp = O.tonemap(film, 0)[0, :, 0]
+    assert list(clamp[[4, 6]]) == [127, 255]
+    gam = O.tonemap(film, 1, 0.5)[0, :, 0]
+    assert gam[3] == 117 and gam[4] == 187 and gam[5] == 254 and gam[1] == int(12.92 * 0.001 * 255)
+    assert np.all(np.abs(gam.astype(int) - _srgb_byte(vals).astype(int)) <= 1)
+    g2 = O.tonemap(film, 1, 0.55)[0, :, 0]
+    assert np.all(np.abs(g2.astype(int) - _srgb_byte(vals * 2).astype(int)) <= 1) and g2[3] > gam[3]
+    rein = O.tonemap(np.concatenate([film, np.array([[[2.0, 1.0, 0.5, 1.0]]], np.float32)], axis=1), 2)[0]
+    L = np.array([0.2126, 0.7152, 0.0722])
+    lum_max = max(vals.max(), np.dot([2.0, 1.0, 0.5], L))
+    c = np.array([2.0, 1.0, 0.5])
+    l0 = np.dot(c, L)
+    lum = l0 * (1 + l0 / lum_max ** 2) / (1 + l0)
+    assert np.all(np.abs(rein[-1].astype(int) - _srgb_byte(c * lum / l0).astype(int)) <= 1)
+    assert np.all(rein[0] == 0)
+    # zero weight: the sum is shown as it is
+    assert O.tonemap(np.array([[[0.25, 0.25, 0.25, 0.0]]], np.float32), 0)[0, 0, 0] == 63
+
+
+def test_png_round_trip(tmp_path):
+    """raytracer_amd.image writes an 8-bit RGB PNG (signature, IHDR, zlib IDAT, CRCs) that reads
+    back to the same bytes."""
+    from raytracer_amd import image
+    img = np.random.default_rng(1).integers(0, 256, (13, 17, 3), dtype=np.uint8)
+    p = tmp_path / "a.png"
+    image.write_png(str(p), img)
+    assert open(p, "rb").read()[:8] == b"\x89PNG\r\n\x1a\n"
+    assert np.array_equal(image.read_png(str(p)), img)
