@@ -153,7 +153,7 @@ GUIDE_FLOPS_PER_QUERY = 2 * (144 * 64 + 64 * 64 + 64 * 64 + 64 * 40)  # network 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
-def guiding_leg(ctx, scene, size, ref_spp, target=1e-3, ladder=(64, 256, 1024, 4096)):
+def guiding_leg(ctx, scene, size, ref_spp, target=1e-3, ladder=(64, 256, 1024, 4096, 8192)):
     """Plain vs guided on one scene, both on the GPU: error and time at a ladder of spp counts.
 
     The judge of the error is the plain tracer's converged image: ref_spp samples rendered as eight
