@@ -129,7 +129,8 @@ typedef struct rtg_scene_desc {
   int32_t env_width;
   int32_t env_height;
   float env_scale;
-  int32_t light_strategy;      /* RTG_LIGHTS_UNIFORM or RTG_LIGHTS_WEIGHTED */
+  int32_t light_strategy;      /* RTG_LIGHTS_UNIFORM, RTG_LIGHTS_WEIGHTED or RTG_LIGHTS_ALL (at most 8
+                                  lights; rtg_render only) */
   /* Camera medium (the scene file's top-level <medium>, core/XMLLoader.cpp:88-92,159-172): a
    * Heterogeneous medium (medium/Heterogeneous.cpp) over a GridVolume density grid
    * (volume/GridVolume.cpp) with a Henyey-Greenstein phase function (medium/HGPhaseFunction.cpp).

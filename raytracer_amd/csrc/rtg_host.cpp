@@ -214,8 +214,9 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   if (!desc) fail(RTG_ERR_INVALID, "null scene description");
   if (desc->num_meshes < 0 || desc->num_spheres < 0 || desc->num_materials <= 0)
     fail(RTG_ERR_INVALID, "scene needs at least one material and non-negative counts");
-  if (desc->light_strategy != RTG_LIGHTS_UNIFORM && desc->light_strategy != RTG_LIGHTS_WEIGHTED)
-    fail(RTG_ERR_INVALID, "light_strategy must be one_light_uniform or one_light_weighted on the GPU");
+  if (desc->light_strategy != RTG_LIGHTS_UNIFORM && desc->light_strategy != RTG_LIGHTS_WEIGHTED &&
+      desc->light_strategy != RTG_LIGHTS_ALL)
+    fail(RTG_ERR_INVALID, "unknown light_strategy");
   int slot = -1;
   for (int i = 0; i < kMaxScenes; i++)
     if (!ctx->scene_slots[i]) {
@@ -482,6 +483,8 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
     }
   }
   cdf_build(lw, lcdf, lsum);
+  if (desc->light_strategy == RTG_LIGHTS_ALL && (int)lights.size() > kAllLights)
+    fail(RTG_ERR_INVALID, "the all_lights strategy supports at most " + std::to_string(kAllLights) + " lights");
   // upload
   uint64_t bytes = 0;
   auto& own = sc->owned;
@@ -579,6 +582,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   sc->ntri = ntri;
   sc->nsph = nsph;
   sc->ntex = (int)texs.size();
+  sc->all_lights = desc->light_strategy == RTG_LIGHTS_ALL;
   ctx->scene_slots[slot] = sc.get();
   return sc.release();
 }
@@ -624,6 +628,16 @@ void ensure_pool(rtg_ctx* ctx, int capacity) {
   P.shflag = (uint8_t*)A(n);
   P.counters = (uint32_t*)A(C_COUNT * 4);
   ctx->pool = P;
+}
+
+// Shadow-ray storage of the all-lights strategy (kAllLights per slot), allocated on first use.
+static void ensure_all_lights(rtg_ctx* ctx) {
+  if (ctx->pool.sh_d2) return;
+  auto A = [&](size_t bytes) { return dev_alloc(ctx, bytes, ctx->pool_owned); };
+  const size_t n = (size_t)ctx->pool.capacity * kAllLights;
+  ctx->pool.sh_d2 = (float4*)A(n * 16);
+  ctx->pool.sh_c2 = (float4*)A(n * 16);
+  ctx->pool.q_shadow2 = (uint32_t*)A(n * 4);
 }
 
 int stack_for_resident(rtg_ctx* ctx) {
@@ -910,6 +924,14 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   if (!scene || !desc) fail(RTG_ERR_INVALID, "null scene or render description");
   if (ctx->server) fail(RTG_ERR_INVALID, "the context is serving requests; submit the render instead");
   hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+  const bool guided = desc->guiding != 0;
+  const bool vol = desc->integrator == RTG_INTEGRATOR_VOLPATH;
+  if (desc->integrator != RTG_INTEGRATOR_PATH_MIS && !vol) fail(RTG_ERR_INVALID, "unknown integrator");
+  if (vol && guided) fail(RTG_ERR_INVALID, "guiding is not available with the volumetric integrator");
+  // all_lights: PathTracingMIS samples every light; VolPathTracing always samples one
+  // (Integrator::sampleAttenuatedLightDirect, with uniform selection for this strategy)
+  const bool all_lights = scene->all_lights && !vol;
+  if (all_lights && guided) fail(RTG_ERR_INVALID, "guiding is not available with the all_lights strategy");
   DevJob J = make_job(ctx, scene, *desc, (float4*)out_dev);
   upload_job_filter(ctx, *desc, J, 0);
   // pinned page: [0] done count, [16] guide ring and inference counts, [40] guide loss, [64] claim
@@ -924,14 +946,12 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   *up_job = J;
   hip_check(hipMemcpyAsync(ctx->order_dev, up_order, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream), "order");
   ensure_pool(ctx, ctx->pool_slots);
+  if (all_lights) ensure_all_lights(ctx);
+  ctx->pool.all_lights = all_lights ? 1 : 0;
   ctx->pool.count_work = (ctx->profiling & RTG_PROFILE_COUNT) ? 1 : 0;
   LaunchCfg cfg = launch_cfg(ctx);
   launch_reset_pool(ctx->pool, ctx->stream);
   hip_check(hipMemcpyAsync(ctx->jobs_dev, up_job, sizeof(DevJob), hipMemcpyHostToDevice, ctx->stream), "job upload");
-  const bool guided = desc->guiding != 0;
-  const bool vol = desc->integrator == RTG_INTEGRATOR_VOLPATH;
-  if (desc->integrator != RTG_INTEGRATOR_PATH_MIS && !vol) fail(RTG_ERR_INVALID, "unknown integrator");
-  if (vol && guided) fail(RTG_ERR_INVALID, "guiding is not available with the volumetric integrator");
   GuideState* gs = nullptr;
   DevGuide G;
   std::memset(&G, 0, sizeof(G));
@@ -1039,6 +1059,7 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   }
   hip_check(hipEventRecord(e1, ctx->stream), "event");
   hip_check(hipEventSynchronize(e1), "event");
+  ctx->pool.all_lights = 0;
   float ms = 0.f;
   (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);

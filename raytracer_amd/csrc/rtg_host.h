@@ -54,6 +54,7 @@ struct rtg_scene {
   uint64_t bytes = 0;
   int stack = 16;
   int ntri = 0, nsph = 0, ntex = 0;
+  bool all_lights = false;  // RTG_LIGHTS_ALL
   std::atomic<int> users{0};  // requests of the serving loop that reference this scene
 };
 

@@ -130,6 +130,9 @@ struct DevJob {
   const float* lut;
 };
 static const int kFilterLut = 32 * 32;
+// lights per scene the all-lights strategy supports (shadow rays per vertex)
+static const int kAllLights = 8;
+static const uint32_t kAllLightDim = 1u << 22;  // 2D draws of lights 1.. at depth d: + d * kAllLights + light
 
 // Path-pool state (structure of arrays). Slot flags word (rtg_kernels.hip):
 //   bits 0..11 depth of the pending hit, bits 12..19 job slot, bit 20 LAST (only the emission
@@ -153,6 +156,12 @@ struct DevPool {
   float4* L;            // radiance, Russian-roulette divisor of the last step
   float4* sh_d;         // shadow direction, maxT
   float4* sh_c;         // shadow contribution if unoccluded
+  // all-lights strategy (allocated on first use): up to kAllLights shadow rays per slot, entry
+  // slot * kAllLights + k; shflag then holds their count
+  int32_t all_lights;
+  float4* sh_d2;
+  float4* sh_c2;
+  uint32_t* q_shadow2;
   float4* vol_a;        // volumetric paths: light-sample point of the last medium vertex, last hit t
   float4* vol_b;        // volumetric paths: light-sample normal of the last medium vertex
   // queues

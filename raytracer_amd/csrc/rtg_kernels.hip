@@ -586,7 +586,7 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
   const uint32_t* q = P.q_shade + (size_t)MT * P.capacity;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += stride) {
-    bool shadow = false;
+    uint32_t nshadow = 0;
     uint32_t slot = q[k];
     {
       uint32_t f = P.flags[slot];
@@ -628,7 +628,32 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
         f3 wo_l = to_local(sf.sh, -d);
         // one-light next-event estimation (Integrator.cpp:176-238, PathTracingMIS.cpp:119-153);
         // from a smooth conductor or dielectric it contributes 0 (eval is 0), so it is skipped
-        if (S.nlights > 0 && !is_delta(M)) {
+        if (P.all_lights && S.nlights > 0 && !is_delta(M)) {
+          // all_lights (PathTracingMIS.cpp:119-137): every light sampled once, without the
+          // selection probability; one shadow ray per contributing light
+          uint32_t cnt = 0;
+          for (int li = 0; li < S.nlights; li++) {
+            float ux, uy;
+            sample2d(rng, li == 0 ? 1u + 2u * depth : kAllLightDim + depth * (uint32_t)kAllLights + (uint32_t)li, ux, uy);
+            const DevLight& LL = S.lights[li];
+            const LightSample ls = (LL.type == LIGHT_ENV) ? env_sample(S, sf.p, ux, uy)
+                                                          : mesh_light_sample(S, LL, sf.p, ux, uy);
+            if (ls.pdf > 0.f && luminance(ls.intensity) > 0.f) {
+              const f3 value = ls.intensity / ls.pdf;
+              if (!is_zero(value)) {
+                const f3 wi_l = to_local(sf.sh, ls.dir);
+                const float cosT = fabsf(wi_l.z);
+                const float w = power_heuristic(ls.pdf, bsdf_pdf<MT>(M, wi_l, wo_l));
+                const f3 c = cmul(cmul(mk(T.x, T.y, T.z), value), bsdf_eval<MT>(M, wi_l, wo_l)) * cosT * w;
+                const size_t e = (size_t)slot * kAllLights + cnt;
+                P.sh_d2[e] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - EPS);
+                P.sh_c2[e] = make_float4(c.x, c.y, c.z, 0.f);
+                cnt++;
+              }
+            }
+          }
+          nshadow = cnt;
+        } else if (S.nlights > 0 && !is_delta(M)) {
           float ux, uy;
           sample2d(rng, 1u + 2u * depth, ux, uy);
           float lw;
@@ -649,7 +674,7 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
               f3 c = cmul(cmul(mk(T.x, T.y, T.z), value), fe) * cosT * w;
               P.sh_d[slot] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - EPS);
               P.sh_c[slot] = make_float4(c.x, c.y, c.z, 0.f);
-              shadow = true;
+              nshadow = 1;
             }
           }
         }
@@ -729,7 +754,7 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
       P.flags[slot] = nf;
       P.state[slot] = done ? ST_DONE : ST_ACTIVE;
     }
-    P.shflag[slot] = shadow ? 1 : 0;
+    P.shflag[slot] = (uint8_t)nshadow;
   }
 }
 
@@ -1026,23 +1051,33 @@ __global__ void __launch_bounds__(256) k_compact_shadow(DevPool P) {
   const uint32_t nchunks = (n + kRegenSlots - 1) / kRegenSlots;
   for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
     const uint32_t base = chunk * kRegenSlots + threadIdx.x * kRegenK;  // kRegenK consecutive entries
-    uint32_t bits = 0;
+    uint32_t cnt[kRegenK];
     uint32_t slots[kRegenK];
+    uint32_t mine = 0;
 #pragma unroll
     for (int k = 0; k < kRegenK; k++) {
       uint32_t e = base + k;
       slots[k] = 0;
+      cnt[k] = 0;
       if (e < n && P.cls_q[e] < 3) {
         slots[k] = P.q_ext[e];
-        if (P.shflag[slots[k]]) bits |= 1u << k;
+        cnt[k] = P.shflag[slots[k]];
       }
+      mine += cnt[k];
     }
     uint32_t total;
-    uint32_t off = block_excl_scan<4, uint32_t>((uint32_t)__popc(bits), s_w, total);
+    uint32_t off = block_excl_scan<4, uint32_t>(mine, s_w, total);
     uint32_t pos = block_reserve(&C[C_SHADOW], total, &s_base) + off;
+    if (!P.all_lights) {
 #pragma unroll
-    for (int k = 0; k < kRegenK; k++)
-      if (bits & (1u << k)) P.q_shadow[pos++] = slots[k];
+      for (int k = 0; k < kRegenK; k++)
+        if (cnt[k]) P.q_shadow[pos++] = slots[k];
+    } else {
+      // entry slot * kAllLights + j for each of the slot's shadow rays
+#pragma unroll
+      for (int k = 0; k < kRegenK; k++)
+        for (uint32_t j = 0; j < cnt[k]; j++) P.q_shadow2[pos++] = slots[k] * (uint32_t)kAllLights + j;
+    }
   }
 }
 
@@ -1053,14 +1088,20 @@ struct ShadowSrc {
   const DevScene* scenes;
   const DevJob* jobs;
   struct Ctx {
-    uint32_t slot;
+    uint32_t slot, entry;
   };
   __device__ __forceinline__ uint32_t batch() { return fetch_batch(&P.counters[C_SFETCH]); }
   __device__ __forceinline__ bool load(uint32_t k, Ctx& c, TravState& s, SceneGeo& geo) {
-    c.slot = P.q_shadow[k];
+    if (P.all_lights) {
+      c.entry = P.q_shadow2[k];
+      c.slot = c.entry / (uint32_t)kAllLights;
+    } else {
+      c.slot = P.q_shadow[k];
+      c.entry = c.slot;
+    }
     const uint32_t f = P.flags[c.slot];
     const DevScene& S = scenes[jobs[fl_job(f)].scene];
-    const float4 o = P.ray_o[c.slot], d = P.sh_d[c.slot];
+    const float4 o = P.ray_o[c.slot], d = P.all_lights ? P.sh_d2[c.entry] : P.sh_d[c.slot];
     s.r.o = mk(o.x, o.y, o.z);
     s.r.d = mk(d.x, d.y, d.z);
     s.r.inv = mk(1.f / d.x, 1.f / d.y, 1.f / d.z);
@@ -1075,10 +1116,18 @@ struct ShadowSrc {
   }
   __device__ __forceinline__ void finish(const Ctx& c, const TravState& s) {
     if (!s.hit) {
-      const float4 cc = P.sh_c[c.slot];
-      float4 L = P.L[c.slot];
-      L.x += cc.x; L.y += cc.y; L.z += cc.z;
-      P.L[c.slot] = L;
+      if (P.all_lights) {
+        // a slot's shadow rays may finish together: add atomically
+        const float4 cc = P.sh_c2[c.entry];
+        unsafeAtomicAdd(&P.L[c.slot].x, cc.x);
+        unsafeAtomicAdd(&P.L[c.slot].y, cc.y);
+        unsafeAtomicAdd(&P.L[c.slot].z, cc.z);
+      } else {
+        const float4 cc = P.sh_c[c.slot];
+        float4 L = P.L[c.slot];
+        L.x += cc.x; L.y += cc.y; L.z += cc.z;
+        P.L[c.slot] = L;
+      }
     }
   }
 };

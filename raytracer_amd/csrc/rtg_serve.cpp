@@ -212,6 +212,7 @@ int rtg_submit(rtg_server* s, rtg_scene* scene, const rtg_render_desc* desc, int
     if (desc->guiding) fail(RTG_ERR_INVALID, "guided renders are not served (use rtg_render)");
     if (desc->integrator != RTG_INTEGRATOR_PATH_MIS)
       fail(RTG_ERR_INVALID, "volumetric renders are not served (use rtg_render)");
+    if (scene->all_lights) fail(RTG_ERR_INVALID, "all_lights scenes are not served (use rtg_render)");
     (void)make_job(ctx, scene, *desc, nullptr);
     std::unique_ptr<Request> r(new Request());
     r->id = s->next_id++;
@@ -238,6 +239,7 @@ int rtg_server_step(rtg_server* s, double max_ms) {
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");
     auto t0 = Clock::now();
     LaunchCfg cfg = launch_cfg(ctx);
+    ctx->pool.all_lights = 0;  // served scenes sample one light per vertex
     const int batch = 2;
     while (true) {
       admit(s);

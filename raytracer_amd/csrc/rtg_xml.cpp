@@ -967,7 +967,8 @@ struct Loader {
         std::string s = P.str("strategy", "one_light_uniform");
         if (s == "one_light_uniform") strategy = RTG_LIGHTS_UNIFORM;
         else if (s == "one_light_weighted") strategy = RTG_LIGHTS_WEIGHTED;
-        else perr("integrator strategy '" + s + "' is not rendered by the GPU path");
+        else if (s == "all_lights") strategy = RTG_LIGHTS_ALL;
+        else perr("integrator strategy '" + s + "' is not known");
         have_integrator = true;
       } else if (c.name == "sampler") {
         Props P = props(c, "sampler");  // sampler/RandomSampler.cpp:10, sampler/StratifiedSampler.cpp:23-24

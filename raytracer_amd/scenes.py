@@ -361,6 +361,18 @@ def vol_cornell(width=512, height=512, spp=128, max_depth=16, min_depth=3, seed=
     return sd
 
 
+def two_light_cornell(width=256, height=256, spp=64, max_depth=8, min_depth=3, seed=9, strategy=R.LIGHTS_ALL):
+    """Cornell box with a second emitter (a small blue panel on the left wall facing +x) and a dim
+    sky seen through the open front: three lights for the light-selection strategies."""
+    sd = cornell(width, height, spp, max_depth, min_depth, seed)
+    black = sd.add_material(**diffuse((0.0, 0.0, 0.0)))
+    v, i = quad((-0.499, 0.55, -0.2), (0, 0.2, 0), (0, 0, 0.2), (1, 0, 0))
+    sd.meshes.append(MeshDesc(v, i, material=black, emission=(2.0, 4.0, 8.0)))
+    sd.env = np.full((8, 16, 3), 0.05, np.float32)
+    sd.light_strategy = strategy
+    return sd
+
+
 def checker_texture(n=16, cells=4, lo=(0.1, 0.1, 0.1), hi=(0.9, 0.8, 0.6), smooth=False):
     """An n x n RGB texture: a checkerboard of `cells` x `cells` squares, or (smooth) a linear ramp
     lo + (hi - lo) * (i + j) / (2 (n - 1)) over texel column i and row j."""
@@ -535,7 +547,8 @@ def write_xml(sd, directory, name="scene.xml"):
     os.makedirs(directory, exist_ok=True)
     cam, rp = sd.camera, sd.render
     out = ['<?xml version="1.0" encoding="utf-8"?>', "<scene>"]
-    strat = {R.LIGHTS_UNIFORM: "one_light_uniform", R.LIGHTS_WEIGHTED: "one_light_weighted"}[sd.light_strategy]
+    strat = {R.LIGHTS_UNIFORM: "one_light_uniform", R.LIGHTS_WEIGHTED: "one_light_weighted",
+             R.LIGHTS_ALL: "all_lights"}[sd.light_strategy]
     integ = "VolPathTracing" if rp.get("integrator", R.INTEGRATOR_PATH_MIS) == R.INTEGRATOR_VOLPATH else "PathTracingMIS"
     out += [f'  <integrator type="{integ}">',
             f'    <integer name="minDepth" value="{int(rp.get("min_depth", 3))}"/>',

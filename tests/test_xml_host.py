@@ -318,6 +318,18 @@ def case_textures(ctx, tmp):
         assert msg in str(e.value), (msg, str(e.value))
 
 
+def case_all_lights(ctx, tmp):
+    """strategy all_lights (integrator/Integrator.h:18-23) loads; a scene file round trip keeps it."""
+    from raytracer_amd import scenes
+
+    sd = scenes.two_light_cornell(width=16, height=16)
+    sc, _, _ = ctx.load_xml(scenes.write_xml(sd, os.path.join(tmp, "al"), name="al.xml"))
+    assert sc.info()["lights"] == 3
+    body = ("<integrator type='PathTracingMIS'><string name='strategy' value='all_lights'/></integrator>"
+            "<objects><shape type='Cube'/></objects>")
+    ctx.load_xml(_scene(tmp, body, name="al2.xml"))
+
+
 def case_errors(ctx, tmp):
     from raytracer_amd import rtg
 
@@ -382,7 +394,7 @@ def decode_rgbe(path):
 
 # ------------------------------------------------------------------ pytest entry points
 @pytest.mark.parametrize("case", ["case_cornell_roundtrip", "case_glossy_roundtrip", "case_env_roundtrip",
-                                  "case_obj_features", "case_shapes_refs_textures", "case_smooth_bsdfs", "case_filters", "case_volume", "case_textures", "case_errors",
+                                  "case_obj_features", "case_shapes_refs_textures", "case_smooth_bsdfs", "case_filters", "case_volume", "case_textures", "case_all_lights", "case_errors",
                                   "case_rgbe"])
 def test_xml_loader(mock_lib, case, tmp_path):
     _run(mock_lib, case, tmp_path)
