@@ -237,6 +237,12 @@ int rtg_scene_upload(rtg_ctx* ctx, const rtg_scene_desc* desc, rtg_scene** out);
 /* Scene file in the reference's XML schema (core/XMLLoader.cpp:33-904): the
  * hot-path types only. Fills *render from <camera>, <sampler>, <integrator>. */
 int rtg_scene_load_xml(rtg_ctx* ctx, const char* path, rtg_scene** out, rtg_render_desc* render);
+/* A sampled spectrum (n >= 2 (lambda nm, value) pairs) -> linear sRGB, as the scene loader turns a
+ * named metal's IOR spectra (data/IOR/<metal>.eta.spd, .k.spd) into the conductor's eta and k.
+ * Replaces Color3::setFromSampledSpectrum (core/Color.cpp:423-443) as called from
+ * IORHelper::extractAndConvertSpectrumFromFile (tools/Common.cpp:83-110): CIE 1931 2-degree
+ * integration at 1 nm over 360-830 nm, negative channels clamped to zero. Host only. */
+int rtg_spectrum_to_rgb(const float* lambda, const float* values, int32_t n, float* out_rgb);
 int rtg_scene_release(rtg_ctx* ctx, rtg_scene* scene);
 /* Device bytes held by a scene. */
 int rtg_scene_bytes(rtg_scene* scene, uint64_t* bytes);

@@ -17,6 +17,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "rtg_cie.h"
 #include "rtg_host.h"
 
 using namespace rtg;
@@ -24,6 +25,77 @@ using namespace rtg;
 namespace {
 
 [[noreturn]] void perr(const std::string& m) { fail(RTG_ERR_PARSE, m); }
+
+// Color3::setFromSampledSpectrum (core/Color.cpp:423-443): the spectrum, piecewise linear between
+// its samples and constant beyond its ends (Color.h:239-248), is read at each nanometre of the CIE
+// table and summed against x, y, z; the sums are scaled by (830 - 360) / (kYIntegral * 471) and
+// turned into linear sRGB by the matrix of Color.h:196-205, negative channels clamped to zero.
+// All in float, in the reference's order of operations.
+void spectrum_to_rgb(const float* lam, const float* val, int n, float rgb[3]) {
+  float xyz[3] = {0.f, 0.f, 0.f};
+  for (int i = 0; i < rtg_cie::kSamples; i++) {
+    const float l = (float)(rtg_cie::kFirstNm + i);
+    float v;
+    if (l <= lam[0]) {
+      v = val[0];
+    } else if (l >= lam[n - 1]) {
+      v = val[n - 1];
+    } else {
+      // bisection for the first index whose wavelength exceeds l; the segment starts one before it,
+      // clamped to [0, n - 2] (Color.h:219-234: also defined on unsorted input)
+      int first = 0, len = n;
+      while (len > 0) {
+        const int half = len >> 1, mid = first + half;
+        if (lam[mid] <= l) {
+          first = mid + 1;
+          len -= half + 1;
+        } else {
+          len = half;
+        }
+      }
+      const int o = std::min(std::max(first - 1, 0), n - 2);
+      const float t = (l - lam[o]) / (lam[o + 1] - lam[o]);
+      v = (1.f - t) * val[o] + t * val[o + 1];
+    }
+    xyz[0] += v * rtg_cie::kCmf[i][0];
+    xyz[1] += v * rtg_cie::kCmf[i][1];
+    xyz[2] += v * rtg_cie::kCmf[i][2];
+  }
+  const float scale = (float)(rtg_cie::kFirstNm + rtg_cie::kSamples - 1 - rtg_cie::kFirstNm) /
+                      (rtg_cie::kYIntegral * rtg_cie::kSamples);
+  for (float& c : xyz) c *= scale;
+  rgb[0] = 3.240479f * xyz[0] - 1.537150f * xyz[1] - 0.498535f * xyz[2];
+  rgb[1] = -0.969256f * xyz[0] + 1.875991f * xyz[1] + 0.041556f * xyz[2];
+  rgb[2] = 0.055648f * xyz[0] - 0.204043f * xyz[1] + 1.057311f * xyz[2];
+  for (int c = 0; c < 3; c++) rgb[c] = std::max(rgb[c], 0.f);
+}
+
+// IORHelper::extractAndConvertSpectrumFromFile (tools/Common.cpp:83-110): "lambda value" lines,
+// '#' lines skipped. A line of two words that are not numbers (Fe.k.spd's header) enters as the
+// pair (0, 0), as failed stream extraction leaves it in the reference; a line of fewer than two
+// words (blank) is skipped, where the reference reads an undefined pair.
+void read_spd(const std::string& path, float rgb[3]) {
+  std::ifstream f(path);
+  if (!f) perr("spectrum file " + path + " does not exist");
+  std::vector<float> lam, val;
+  std::string line;
+  while (std::getline(f, line)) {
+    if (!line.empty() && line[0] == '#') continue;
+    std::istringstream ls(line);
+    std::string a, b;
+    if (!(ls >> a >> b)) continue;
+    auto num = [](const std::string& w) {
+      std::istringstream is(w);
+      float x = 0.f;
+      if (!(is >> x)) x = 0.f;
+      return x;
+    };
+    lam.push_back(num(a));
+    val.push_back(num(b));
+  }
+  if (lam.size() < 2) perr("spectrum file " + path + ": fewer than two samples");
+  spectrum_to_rgb(lam.data(), val.data(), (int)lam.size(), rgb);
+}
 
 // object types are looked up case-insensitively (factory/ObjectFactoryManager.h:82-89)
 std::string lower(const std::string& s) {
@@ -399,6 +471,28 @@ struct Loader {
   bool medium_homogeneous = false;
   float medium_sigma_a[3] = {0.f, 0.f, 0.f}, medium_sigma_s[3] = {0.f, 0.f, 0.f};
 
+  // IORHelper::evalConductorIOR (tools/Common.cpp:44-81): a known metal name reads
+  // data/IOR/<symbol>.eta.spd and .k.spd (searched beside the scene file, then in the working
+  // directory) through the spectral conversion; no name, or an unknown one, takes the 'eta' and
+  // 'absorption' colors.
+  void conductor_ior(Props& P, rtg_material& m) {
+    static const float zero[3] = {0.f, 0.f, 0.f};
+    static const std::map<std::string, std::string> symbol = {
+        {"gold", "Au"},      {"silver", "Ag"}, {"aluminium", "Al"},   {"copper", "Cu"},
+        {"chromium", "Cr"},  {"iron", "Fe"},   {"nickel", "Ni_palik"}, {"tin", "SnTe_palik"}};
+    const std::string name = P.str("material", "");
+    const auto it = symbol.find(name);
+    if (it == symbol.end()) {
+      if (!name.empty())
+        std::fprintf(stderr, "rtg: material %s not found, using the eta and absorption colors\n", name.c_str());
+      P.color("eta", zero, m.eta);
+      P.color("absorption", zero, m.k);
+      return;
+    }
+    read_spd(resolve("./data/IOR/" + it->second + ".eta.spd"), m.eta);
+    read_spd(resolve("./data/IOR/" + it->second + ".k.spd"), m.k);
+  }
+
   std::string resolve(const std::string& p) const {
     if (p.empty() || p[0] == '/') return p;
     std::string q = p;
@@ -545,19 +639,11 @@ struct Loader {
       m.type = RTG_MAT_ROUGH_CONDUCTOR;  // bsdf/RoughConductor.cpp:13-30
       m.alpha = (float)P.real("alpha", 0.2);
       m.distribution = dist(P.str("distribution", "beckmann"));
-      if (P.has("material") && !P.str("material", "").empty())
-        perr("bsdf RoughConductor: named material '" + P.str("material", "") +
-             "' needs spectral IOR data; give 'eta' and 'absorption' colors");
-      P.color("eta", zero, m.eta);  // tools/Common.cpp:56-80 fallback when no material name
-      P.color("absorption", zero, m.k);
+      conductor_ior(P, m);
       P.color("reflectanceTexture", one, m.albedo);
     } else if (lt == "conductor") {
       m.type = RTG_MAT_CONDUCTOR;  // bsdf/Conductor.cpp:12-19
-      if (P.has("material") && !P.str("material", "").empty())
-        perr("bsdf Conductor: named material '" + P.str("material", "") +
-             "' needs spectral IOR data; give 'eta' and 'absorption' colors");
-      P.color("eta", zero, m.eta);
-      P.color("absorption", zero, m.k);
+      conductor_ior(P, m);
       P.color("reflectanceTexture", one, m.albedo);
     } else if (lt == "dielectric") {
       m.type = RTG_MAT_DIELECTRIC;  // bsdf/Dielectric.cpp:10-17
@@ -1071,6 +1157,12 @@ struct Loader {
 };
 
 }  // namespace
+
+extern "C" int rtg_spectrum_to_rgb(const float* lambda, const float* values, int32_t n, float* out_rgb) {
+  if (!lambda || !values || !out_rgb || n < 2) return RTG_ERR_INVALID;
+  spectrum_to_rgb(lambda, values, n, out_rgb);
+  return RTG_OK;
+}
 
 extern "C" int rtg_scene_load_xml(rtg_ctx* ctx, const char* path, rtg_scene** out, rtg_render_desc* desc) {
   if (!ctx || !path || !out) return RTG_ERR_INVALID;

@@ -126,6 +126,7 @@ SIGNATURES = {
     "rtg_bsdf_batch": (C.c_int, [P, C.POINTER(rtg_material), C.c_int32, P, P, C.c_int32, P, P, P, P]),
     "rtg_medium_batch": (C.c_int, [P, P, C.c_int32, P, C.c_int32, P, C.c_int32, P]),
     "rtg_tonemap": (C.c_int, [P, P, C.c_int32, C.c_int32, C.c_int32, C.c_float, P]),
+    "rtg_spectrum_to_rgb": (C.c_int, [P, P, C.c_int32, P]),
     "rtg_server_create": (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
     "rtg_server_destroy": (C.c_int, [P]),
     "rtg_submit": (C.c_int, [P, P, C.POINTER(rtg_render_desc), C.POINTER(C.c_int64)]),
@@ -563,6 +564,20 @@ class Server:
         st = rtg_server_stats()
         self.ctx.check(self.lib.rtg_server_query(self.h, C.byref(st)))
         return {k: getattr(st, k) for k, _ in st._fields_ if k != "reserved"}
+
+
+def spectrum_to_rgb(lam, values):
+    """Sampled spectrum (wavelengths in nm, values) -> linear sRGB (rtg_spectrum_to_rgb), the
+    conversion the scene loader applies to a named metal's IOR spectra (core/Color.cpp:423-443)."""
+    lib = load()
+    la, va = _f32(lam).ravel(), _f32(values).ravel()
+    if la.size != va.size:
+        raise ValueError("spectrum_to_rgb: wavelengths and values differ in length")
+    out = np.zeros(3, np.float32)
+    code = lib.rtg_spectrum_to_rgb(_ptr(la), _ptr(va), int(la.size), _ptr(out))
+    if code != 0:
+        raise RtgError(code, "rtg_spectrum_to_rgb needs at least two samples")
+    return out
 
 
 def resolve(rgbw):

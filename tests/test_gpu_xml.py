@@ -89,3 +89,35 @@ def test_xml_volumetric_scene_renders_the_same_paths(ctx, tmp_path):
     print(f"  max |diff| {d.max():.3e}, mean {a.mean():.5f}", flush=True)
     assert a.mean() > 0
     assert np.all(d <= 1e-5 * np.maximum(1.0, b)), d.max()
+
+
+def test_xml_named_metal_matches_explicit_colors(ctx, tmp_path):
+    """The mirror of the specular Cornell box named 'gold' (IOR spectra in data/IOR beside the
+    scene file) renders, with the same seed, like the same scene with its eta and absorption given
+    as the RGB that rtg_spectrum_to_rgb makes of those spectra (the header line of the k file read
+    as (0, 0))."""
+    import re
+
+    from test_xml_host import named_metal_files
+
+    tmp = str(tmp_path)
+    named_metal_files(tmp)
+    eta = rtg.spectrum_to_rgb(np.array([400, 500, 600, 700], np.float32), np.array([1.5, 0.9, 0.3, 0.2], np.float32))
+    k = rtg.spectrum_to_rgb(np.array([0, 0, 400, 550, 700], np.float32), np.array([0, 0, 1.9, 2.4, 4.0], np.float32))
+    col = lambda n, c: f'<color name="{n}" value="{" ".join("%.9g" % x for x in c)}"/>'
+    text = open(scenes.write_xml(scenes.specular_cornell(width=32, height=32), tmp)).read()
+    pat = re.compile(r'<color name="eta" value="[^"]*"/>\s*<color name="absorption" value="[^"]*"/>')
+    assert len(pat.findall(text)) == 1
+    paths = []
+    for name, repl in (("named.xml", '<string name="material" value="gold"/>'),
+                       ("explicit.xml", col("eta", eta) + col("absorption", k))):
+        paths.append(os.path.join(tmp, name))
+        with open(paths[-1], "w") as f:
+            f.write(pat.sub(repl, text))
+    print(f"[xml] named metal: eta {eta}, k {k}", flush=True)
+    a_sc, cam, rp = ctx.load_xml(paths[0])
+    b_sc, _, _ = ctx.load_xml(paths[1])
+    rp = dict(rp, spp=64, seed=3)
+    a, b = _render(a_sc, cam, rp), _render(b_sc, cam, rp)
+    print(f"  mean {a.mean():.5f}, max |diff| {np.abs(a - b).max():.3e}", flush=True)
+    assert a.mean() > 0 and np.all(np.abs(a - b) <= 1e-5 * np.maximum(1.0, b))

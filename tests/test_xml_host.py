@@ -359,6 +359,43 @@ def case_errors(ctx, tmp):
     assert sc.info()["triangles"] == 12
 
 
+SPD_GOLD_ETA = "# test spectrum\n# ;  Lambda  n\n400.0 1.5\n500.0 0.9\n600.0 0.3\n700.0 0.2\n"
+SPD_GOLD_K = "\nProduct: gold\nlambda(nm), k\n400 1.9\n550 2.4\n700 4.0\n"
+
+
+def named_metal_files(root):
+    d = os.path.join(root, "data", "IOR")
+    os.makedirs(d, exist_ok=True)
+    for fn, text in (("Au.eta.spd", SPD_GOLD_ETA), ("Au.k.spd", SPD_GOLD_K)):
+        with open(os.path.join(d, fn), "w") as f:
+            f.write(text)
+
+
+def named_metal_body(material, eta="", kind="Conductor"):
+    return f"""<objects><shape type="Cube"><bsdf type="{kind}">
+      <string name="material" value="{material}"/>{eta}</bsdf></shape>
+      <shape type="Plane"><transform name="toWorld"><translation><y>3</y></translation></transform>
+        <light type="MeshLight"><color name="radiance" value="5 5 5"/></light></shape></objects>"""
+
+
+def case_named_metal(ctx, tmp):
+    """tools/Common.cpp:44-110: a known metal name reads data/IOR/<symbol>.eta.spd and .k.spd
+    beside the scene ('#' lines skipped, a text header line read as (0, 0)); an unknown name falls
+    back to the colors; a known name without its files is an error naming the file."""
+    from raytracer_amd import rtg
+
+    named_metal_files(tmp)
+    for kind in ("Conductor", "RoughConductor"):
+        sc, _, _ = ctx.load_xml(_scene(tmp, named_metal_body("gold", kind=kind), name=f"g{kind}.xml"))
+        assert sc.info()["materials"] == 2
+    sc, _, _ = ctx.load_xml(_scene(tmp, named_metal_body("unobtainium", '<color name="eta" value="0.2 0.4 1"/>'),
+                                   name="u.xml"))
+    assert sc.info()["materials"] == 2
+    with pytest.raises(rtg.RtgError) as e:
+        ctx.load_xml(_scene(tmp, named_metal_body("silver"), name="s2.xml"))
+    assert "Ag.eta.spd" in str(e.value), str(e.value)
+
+
 def case_rgbe(ctx, tmp):
     """write_rgbe / the loader's decode: the decoded map is within one mantissa step of the input
     (stb_image's rule, mantissa x 2^(e-136)), and an exact power-of-two grid decodes exactly."""
@@ -394,7 +431,7 @@ def decode_rgbe(path):
 
 # ------------------------------------------------------------------ pytest entry points
 @pytest.mark.parametrize("case", ["case_cornell_roundtrip", "case_glossy_roundtrip", "case_env_roundtrip",
-                                  "case_obj_features", "case_shapes_refs_textures", "case_smooth_bsdfs", "case_filters", "case_volume", "case_textures", "case_all_lights", "case_errors",
+                                  "case_obj_features", "case_shapes_refs_textures", "case_smooth_bsdfs", "case_filters", "case_volume", "case_textures", "case_all_lights", "case_named_metal", "case_errors",
                                   "case_rgbe"])
 def test_xml_loader(mock_lib, case, tmp_path):
     _run(mock_lib, case, tmp_path)
