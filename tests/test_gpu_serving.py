@@ -12,15 +12,7 @@ import pytest
 
 from raytracer_amd import rtg, scenes
 
-pytestmark = [
-    pytest.mark.gpu,
-    # An earlier build of rtg_server_destroy unlocked the request-table mutex after deleting the
-    # server that owns it (use-after-free; glibc aborted with "corrupted unsorted chunks" after
-    # this file's first test). The cause was reproduced and fixed on the CPU
-    # (tests/test_serving_host.py); the GPU cases stay off until a later session reruns them.
-    pytest.mark.skip(reason="serving GPU cases held back after a host-side use-after-free; fixed and "
-                            "covered on the CPU by tests/test_serving_host.py, GPU rerun pending"),
-]
+pytestmark = [pytest.mark.gpu]
 
 
 @pytest.fixture(scope="module")
