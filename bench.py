@@ -40,8 +40,9 @@ def parse():
     ap.add_argument("--guide-size", type=int, default=512, help="guided-vs-plain leg: image size")
     ap.add_argument("--guide-ref-spp", type=int, default=32768,
                     help="guided-vs-plain leg: reference spp (eight independent pieces)")
-    ap.add_argument("--serving", action="store_true",
-                    help="also run the serving-mix leg (off by default until its GPU run is re-validated)")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg")
+    ap.add_argument("--only-serving", action="store_true",
+                    help="run only the serving-mix leg and print its JSON object")
     ap.add_argument("--serve-requests", type=int, default=48,
                     help="serving leg: requests of the Poisson stream (lambda 40/s, seed 4)")
     ap.add_argument("--no-volume", action="store_true", help="skip the volumetric leg")
@@ -54,11 +55,34 @@ def parse():
     return ap.parse_args()
 
 
+def host_cpu():
+    """Threads of the CPU baseline and what the host reports. One GPU's share of the host is 16
+    hardware threads (OMP_NUM_THREADS is set to it), whatever nproc shows for the whole machine."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "16") or 16)
+    threads = max(1, min(share, affinity))
+    return threads, {"host_nproc": os.cpu_count(), "host_affinity": affinity, "cpu_model": model,
+                     "threads_rule": "one GPU's share of the host: min(OMP_NUM_THREADS (16 on the GPU host), "
+                                     "CPUs in this process's affinity mask)"}
+
+
 def cpu_baseline(sd, spp, stride):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib as O
 
-    threads = max(1, min(16, os.cpu_count() or 1))
+    threads, host = host_cpu()
     o = O.OracleScene(sd)
     t0 = time.perf_counter()
     rgb, w, st, nblocks = o.render(spp=spp, threads=threads, blocks=(0, 0, stride))
@@ -73,23 +97,26 @@ def cpu_baseline(sd, spp, stride):
                   + f" ({nblocks} blocks, {samples} samples, {dt:.1f} s), reference schedule (blocks seeded (x, y), "
                   "LIFO pop)",
         "rays_per_s": float(st[0]) / dt,
+        **host,
     }
 
 
 def serving(ctx, big, n_req, lam=40.0, seed=4):
-    """Serving mix: a Poisson stream (lambda req/s) of requests, each drawing a scene uniformly from
-    {Cornell box, large displaced sphere, glossy Cornell}, a resolution from {128,256,512,1024}^2
-    and spp from {4,16,64,256}. A producer thread submits at the arrival times while this thread
-    steps the loop; latency = submit to finish. The heterogeneous-volume scene of the same mix is not
-    part of this renderer, so the glossy Cornell variant takes its place."""
+    """Serving mix: a Poisson stream (lambda req/s, seed 4) of requests, each drawing a scene
+    uniformly from the Cornell box (plain MIS, depth 8), the large displaced sphere (plain MIS,
+    depth 6) and the heterogeneous-medium Cornell box (VolPathTracing, 256^3 grid, depth 16), a
+    resolution from {128,256,512,1024}^2 and spp from {4,16,64,256}. The three scenes stay
+    resident; the server's own thread runs the loop (rtg_server_start); a producer thread submits
+    at the arrival times and this thread waits for each request (rtg_wait); latency = submit to
+    finish as the library measures it."""
     import threading
 
     import numpy as np
     from raytracer_amd import rtg, scenes
 
-    sds = [scenes.cornell(), None, scenes.glossy_cornell()]
+    sds = [scenes.cornell(), big[0], scenes.vol_cornell()]
     resident = [ctx.upload(sds[0]), big[1], ctx.upload(sds[2])]
-    sds[1] = big[0]
+    names = ["cornell", "displaced_sphere", "vol_cornell"]
     rng = np.random.default_rng(seed)
     gaps = rng.exponential(1.0 / lam, n_req)
     kinds = rng.integers(0, 3, n_req)
@@ -101,7 +128,9 @@ def serving(ctx, big, n_req, lam=40.0, seed=4):
         reqs.append((float(g), int(k), dict(sd.camera, width=int(sz), height=int(sz)),
                      dict(sd.render, spp=int(spp), seed=int(rng.integers(1, 1 << 30)))))
     offered = sum(c["width"] * c["height"] * r["spp"] for _, _, c, r in reqs)
+    arrival_s = float(gaps.sum())
     with rtg.Server(ctx) as srv:
+        srv.start()
         ids = []
 
         def producer():
@@ -116,26 +145,37 @@ def serving(ctx, big, n_req, lam=40.0, seed=4):
         t0 = time.perf_counter()
         th = threading.Thread(target=producer)
         th.start()
-        while True:
-            alive = th.is_alive()
-            srv.step(max_ms=5.0)
-            st = srv.stats()
-            if not alive and st["running"] == 0 and st["pending"] == 0:
-                break
+        lat = []
+        done = 0
+        while done < n_req:
+            if done >= len(ids):
+                time.sleep(0.001)
+                continue
+            status, _, ms = srv.wait(ids[done], want_image=False)
+            assert status == "done", status
+            lat.append(ms)
+            done += 1
         th.join()
         makespan = time.perf_counter() - t0
-        lat = np.array([srv.poll(j, want_image=False)[2] for j in ids])
         st = srv.stats()
+    lat = np.array(lat)
     for r in (resident[0], resident[2]):
         r.release()
+    per_scene = {}
+    for k in range(3):
+        sel = kinds == k
+        if sel.any():
+            per_scene[names[k]] = {"requests": int(sel.sum()), "p50_ms": float(np.percentile(lat[sel], 50)),
+                                   "p99_ms": float(np.percentile(lat[sel], 99))}
     return {
-        "requests": n_req, "lambda_per_s": lam, "seed": seed,
+        "requests": n_req, "lambda_per_s": lam, "seed": seed, "scenes": names,
         "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
         "mean_ms": float(lat.mean()), "max_ms": float(lat.max()),
-        "samples_per_s": st["samples"] / makespan, "makespan_s": makespan,
-        "offered_samples_per_s": offered * lam / n_req,
+        "samples_per_s": st["samples"] / makespan, "makespan_s": makespan, "arrival_span_s": arrival_s,
+        "offered_samples_per_s": offered / arrival_s,
         "max_running": st["max_running"], "film_bytes_peak": st["film_bytes_peak"],
-        "completed": st["completed"],
+        "completed": st["completed"], "iterations": st["iterations"], "graph_launches": st["graph_launches"],
+        "per_scene": per_scene,
     }
 
 
@@ -259,14 +299,14 @@ def volumetric_leg(ctx, run_cpu, stride):
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         import oracle_lib as O
 
-        threads = max(1, min(16, os.cpu_count() or 1))
+        threads, host = host_cpu()
         o = O.OracleScene(sd)
         t0 = time.perf_counter()
         _, _, cst, nblocks = o.render(spp=rp["spp"], threads=threads, blocks=(0, 0, stride))
         dt = time.perf_counter() - t0
         samples = nblocks * 32 * 32 * rp["spp"]
         out["cpu_baseline"] = {
-            "value": samples / dt, "unit": "samples/s", "cores": threads, "kind": "port",
+            "value": samples / dt, "unit": "samples/s", "cores": threads, "kind": "port", **host,
             "sample": (f"the whole {W}x{H} render" if stride == 1 else f"every {stride}th 32x32 block of the same "
                        f"{W}x{H} render") + f" ({nblocks} blocks, {samples} samples, {dt:.1f} s)",
         }
@@ -312,6 +352,10 @@ def main():
     sd = scenes.displaced_sphere(width=a.size, height=a.size, spp=a.spp)
     ctx = rtg.Context(0)
     scene = ctx.upload(sd)
+    if a.only_serving:
+        progress(f"serving mix only ({a.serve_requests} requests)")
+        print(json.dumps(serving(ctx, (sd, scene), a.serve_requests)), flush=True)
+        return
     cam, rp = sd.camera, sd.render
     W, H = cam["width"], cam["height"]
     film = torch.zeros(H * W * 4, dtype=torch.float32, device="cuda")
@@ -411,7 +455,7 @@ def main():
     if not a.no_volume:
         progress("volumetric leg (heterogeneous medium, 512x512 128spp)")
         out["volumetric"] = volumetric_leg(ctx, not a.no_cpu, a.volume_stride)
-    if a.serving:
+    if not a.no_serving:
         progress(f"serving mix ({a.serve_requests} requests)")
         out["serving"] = serving(ctx, (sd, scene), a.serve_requests)
     if not a.no_cpu:

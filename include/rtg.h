@@ -328,27 +328,41 @@ int rtg_guide_params(rtg_ctx* ctx, float* params_out);
  * the lobe) -> dirs (n x 3) and the mixture density at each (n). */
 int rtg_guide_sample(rtg_ctx* ctx, const float* mix, const float* u, int32_t n, float* dirs, float* pdf);
 
-/* Continuous-batching serving (no reference counterpart; for a host that renders many requests
- * on one GPU, e.g. an interactive preview service). Requests of any resident scene are admitted
- * in arrival order while a job slot (256) and the HBM budget for their FP32 films allow; every
- * wavefront iteration hands free path slots to the admitted requests, shortest remaining work
- * first, so their paths share the queues. A request's image equals its rtg_render image for the
- * same seed up to the order of the film's float additions. While a server exists on a context,
- * rtg_render / rtg_render_device on that context fail with RTG_ERR_INVALID.
+/* Continuous-batching serving (no reference counterpart; it replaces the reference's job loop,
+ * JobManager::addJobs / jobRun in core/JobManager.cpp:103-232, for a host that renders many
+ * requests on one GPU, e.g. an interactive preview service). Requests of any resident scene —
+ * PathTracingMIS or VolPathTracing — are admitted in arrival order while a job slot (256) and the
+ * HBM budget for their FP32 films allow; every wavefront iteration hands free path slots to the
+ * admitted requests, shortest remaining work first, so their paths share the queues. A request's
+ * image equals its rtg_render image for the same seed up to the order of the film's float
+ * additions. Steady-state iteration batches are replayed hipGraphs.
+ * While a server exists on a context, rtg_render / rtg_render_device on that context become
+ * requests of the server and wait for their result; guided renders and all-lights PathTracingMIS
+ * renders (which the server does not take) fail with RTG_ERR_INVALID.
  * hbm_budget_bytes: film bytes that may be resident at once (0 = 8 GiB); a request whose film
  * alone exceeds it is cancelled at admission. */
 typedef struct rtg_server rtg_server;
 int rtg_server_create(rtg_ctx* ctx, uint64_t hbm_budget_bytes, rtg_server** out);
+/* Stops the server thread if it runs, frees every request's film. */
 int rtg_server_destroy(rtg_server* srv);
+/* Start the server's own loop thread: it runs the device loop whenever requests are pending or
+ * running (slices of 20 ms under the context lock) and sleeps otherwise. Afterwards
+ * rtg_server_step fails; use rtg_wait / rtg_poll. */
+int rtg_server_start(rtg_server* srv);
 /* Enqueue a request (validated now); returns its id in *job_id. Does not block. */
 int rtg_submit(rtg_server* srv, rtg_scene* scene, const rtg_render_desc* desc, int64_t* job_id);
-/* Run the device loop until no request is pending or running, or max_ms elapsed (0 = until idle).
- * Requests submitted from another thread between calls join at the next iteration batch. */
+/* Run the device loop on the calling thread until no request is pending or running, or max_ms
+ * elapsed (0 = until idle). Requests submitted from another thread between calls join at the next
+ * iteration batch. */
 int rtg_server_step(rtg_server* srv, double max_ms);
 /* RTG_OK when finished (copies width*height*4 floats of (rgb sum, weight) if out_rgbw != NULL),
  * RTG_ERR_NOT_READY while queued or running, RTG_ERR_CANCELLED after a cancel. *latency_ms =
  * submit-to-finish wall time. */
 int rtg_poll(rtg_server* srv, int64_t job_id, float* out_rgbw, double* latency_ms);
+/* Block until the request has finished or was cancelled (results as rtg_poll), or timeout_ms
+ * passed (RTG_ERR_NOT_READY; timeout_ms <= 0 waits without limit). Without a server thread the
+ * calling thread runs the device loop meanwhile. */
+int rtg_wait(rtg_server* srv, int64_t job_id, double timeout_ms, float* out_rgbw, double* latency_ms);
 /* Stop claiming new samples for the request; its paths in flight drain, then it ends cancelled. */
 int rtg_cancel(rtg_server* srv, int64_t job_id);
 
@@ -362,6 +376,7 @@ typedef struct rtg_server_stats {
   int32_t pending;              /* requests waiting for admission */
   int32_t max_running;          /* most requests sharing one iteration */
   int32_t reserved;
+  uint64_t graph_launches;      /* iteration batches replayed from hipGraphs on the context */
 } rtg_server_stats;
 int rtg_server_query(rtg_server* srv, rtg_server_stats* out);
 

@@ -72,6 +72,8 @@ static inline float avg3(float a) {  // Color(a).average()
   return s / 3.f;
 }
 
+static_assert(INTEGRATOR_PATH_MIS == RTG_INTEGRATOR_PATH_MIS && INTEGRATOR_VOLPATH == RTG_INTEGRATOR_VOLPATH,
+              "integrator ids");
 static_assert(MAT_CONDUCTOR == RTG_MAT_CONDUCTOR && MAT_DIELECTRIC == RTG_MAT_DIELECTRIC &&
                   MAT_ROUGH_DIELECTRIC == RTG_MAT_ROUGH_DIELECTRIC,
               "device material types follow rtg.h");
@@ -600,6 +602,8 @@ void release_scene(rtg_ctx* ctx, rtg_scene* s) {
 void ensure_pool(rtg_ctx* ctx, int capacity) {
   if (ctx->pool.capacity == capacity && ctx->pool_owned.size()) return;
   (void)hipStreamSynchronize(ctx->stream);
+  graphs_clear(ctx);
+  ctx->pool_epoch++;
   for (void* p : ctx->pool_owned) (void)hipFree(p);
   ctx->pool_owned.clear();
   DevPool P;
@@ -622,7 +626,7 @@ void ensure_pool(rtg_ctx* ctx, int capacity) {
   P.vol_a = (float4*)A(n * 16);
   P.vol_b = (float4*)A(n * 16);
   P.q_ext = (uint32_t*)A(n * 4);
-  P.q_shade = (uint32_t*)A(n * 4 * 4);
+  P.q_shade = (uint32_t*)A(n * 4 * 5);
   P.q_shadow = (uint32_t*)A(n * 4);
   P.cls_q = (uint8_t*)A(n);
   P.shflag = (uint8_t*)A(n);
@@ -633,6 +637,9 @@ void ensure_pool(rtg_ctx* ctx, int capacity) {
 // Shadow-ray storage of the all-lights strategy (kAllLights per slot), allocated on first use.
 static void ensure_all_lights(rtg_ctx* ctx) {
   if (ctx->pool.sh_d2) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  graphs_clear(ctx);
+  ctx->pool_epoch++;
   auto A = [&](size_t bytes) { return dev_alloc(ctx, bytes, ctx->pool_owned); };
   const size_t n = (size_t)ctx->pool.capacity * kAllLights;
   ctx->pool.sh_d2 = (float4*)A(n * 16);
@@ -656,6 +663,69 @@ LaunchCfg launch_cfg(rtg_ctx* ctx) {
   cfg.gstack = ctx->gstack;
   cfg.refill = ctx->refill;
   return cfg;
+}
+
+void graphs_clear(rtg_ctx* ctx) {
+  for (auto& g : ctx->graphs) (void)hipGraphExecDestroy(g.exec);
+  ctx->graphs.clear();
+}
+
+void run_iterations(rtg_ctx* ctx, int mode, const LaunchCfg& cfg, int count) {
+  auto launch = [&]() {
+    for (int b = 0; b < count; b++) {
+      if (mode == IT_VOL)
+        launch_iteration_vol(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, cfg, ctx->stream);
+      else if (mode == IT_MIXED)
+        launch_iteration_mixed(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, cfg, ctx->stream);
+      else
+        launch_iteration(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, cfg, ctx->stream);
+    }
+  };
+  if (!ctx->use_graphs) {
+    launch();
+    return;
+  }
+  GraphKey k;
+  std::memset(&k, 0, sizeof(k));
+  k.mode = mode;
+  k.count = count;
+  k.capacity = ctx->pool.capacity;
+  k.all_lights = ctx->pool.all_lights;
+  k.count_work = ctx->pool.count_work;
+  k.grid = cfg.grid;
+  k.trace_blocks = cfg.trace_blocks;
+  k.stack = cfg.stack;
+  k.refill = cfg.refill;
+  k.epoch = ctx->pool_epoch;
+  for (auto& g : ctx->graphs)
+    if (g.key == k) {
+      hip_check(hipGraphLaunch(g.exec, ctx->stream), "hipGraphLaunch");
+      ctx->graph_launches++;
+      return;
+    }
+  // Capture: the batch's kernels only (no memset nodes; the order and job table are uploaded
+  // before the launch, outside the graph).
+  if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeRelaxed) != hipSuccess) {
+    (void)hipGetLastError();
+    ctx->use_graphs = false;  // capture unavailable: direct launches from now on
+    launch();
+    return;
+  }
+  launch();
+  hipGraph_t graph = nullptr;
+  hip_check(hipStreamEndCapture(ctx->stream, &graph), "hipStreamEndCapture");
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  hip_check(e, "hipGraphInstantiate");
+  if (ctx->graphs.size() >= 16) {  // a bounded cache: drop the oldest configuration
+    (void)hipGraphExecDestroy(ctx->graphs.front().exec);
+    ctx->graphs.erase(ctx->graphs.begin());
+  }
+  ctx->graphs.push_back(GraphEntry{k, exec});
+  ctx->graph_captures++;
+  hip_check(hipGraphLaunch(exec, ctx->stream), "hipGraphLaunch");
+  ctx->graph_launches++;
 }
 
 DevJob make_job(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc& d, float4* accum) {
@@ -696,6 +766,7 @@ DevJob make_job(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc& d, float4
   J.scene = scene->slot;
   J.active = 1;
   J.tiles_x = (d.width + 7) / 8;
+  J.integrator = d.integrator;
   float lut[kFilterLut];
   filter_setup(d, J, lut);
   (void)ctx;
@@ -794,7 +865,7 @@ int rtg_ctx_create(int device, rtg_ctx** out) {
     if (ctx->arch.rfind("gfx950", 0) != 0) fail(RTG_ERR_INVALID, "librtg is built for gfx950, device is " + ctx->arch);
     hip_check(hipMalloc(&ctx->scenes_dev, sizeof(DevScene) * kMaxScenes), "hipMalloc scenes");
     hip_check(hipMalloc(&ctx->jobs_dev, sizeof(DevJob) * kMaxJobs), "hipMalloc jobs");
-    hip_check(hipMalloc(&ctx->order_dev, sizeof(int32_t) * kMaxJobs), "hipMalloc job order");
+    hip_check(hipMalloc(&ctx->order_dev, sizeof(int32_t) * (kMaxJobs + 1)), "hipMalloc job order");
     hip_check(hipMalloc(&ctx->lut_dev, sizeof(float) * kFilterLut * kMaxJobs), "hipMalloc filter tables");
     hip_check(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault), "hipHostMalloc");
     ctx->scene_slots.assign(kMaxScenes, nullptr);
@@ -831,6 +902,7 @@ int rtg_ctx_destroy(rtg_ctx* ctx) {
       } catch (const std::exception&) {
       }
     }
+  graphs_clear(ctx);
   for (void* p : ctx->pool_owned) (void)hipFree(p);
   if (ctx->scenes_dev) (void)hipFree(ctx->scenes_dev);
   if (ctx->jobs_dev) (void)hipFree(ctx->jobs_dev);
@@ -922,7 +994,8 @@ int rtg_scene_bytes(rtg_scene* scene, uint64_t* bytes) {
 
 static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, float* out_dev, rtg_stats* st) {
   if (!scene || !desc) fail(RTG_ERR_INVALID, "null scene or render description");
-  if (ctx->server) fail(RTG_ERR_INVALID, "the context is serving requests; submit the render instead");
+  if (ctx->server)
+    fail(RTG_ERR_INVALID, "the context is serving requests, which do not take guided or all-lights renders");
   hip_check(hipSetDevice(ctx->device), "hipSetDevice");
   const bool guided = desc->guiding != 0;
   const bool vol = desc->integrator == RTG_INTEGRATOR_VOLPATH;
@@ -942,9 +1015,11 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   int32_t* up_order = (int32_t*)((char*)ctx->pinned + 64);
   DevJob* up_job = (DevJob*)((char*)ctx->pinned + 256);
   static_assert(256 + sizeof(DevJob) <= 4096, "pinned staging page too small");
-  *up_order = 0;
+  up_order[0] = 1;  // one job, in slot 0
+  up_order[1] = 0;
   *up_job = J;
-  hip_check(hipMemcpyAsync(ctx->order_dev, up_order, sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream), "order");
+  hip_check(hipMemcpyAsync(ctx->order_dev, up_order, 2 * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream),
+            "order");
   ensure_pool(ctx, ctx->pool_slots);
   if (all_lights) ensure_all_lights(ctx);
   ctx->pool.all_lights = all_lights ? 1 : 0;
@@ -989,12 +1064,13 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   }
   int pending_train = 0;  // training steps enqueued before the batch now in flight
   while (true) {
-    for (int b = 0; b < batch; b++) {
-      if (vol)
-        launch_iteration_vol(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, 1, cfg, ctx->stream);
-      else
-        launch_iteration(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, 1, cfg, ctx->stream,
+    if (prof || guided) {
+      // per-stage events and the guide's changing state: direct launches
+      for (int b = 0; b < batch; b++)
+        launch_iteration(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, cfg, ctx->stream,
                          prof ? &ctx->events[(size_t)b * kEv] : nullptr, guided ? &G : nullptr);
+    } else {
+      run_iterations(ctx, vol ? IT_VOL : IT_PLAIN, cfg, batch);
     }
     iters += batch;
     hip_check(hipGetLastError(), "kernel launch");
@@ -1097,18 +1173,52 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   }
 }
 
+// With a server on the context, a render that the server can take (not guided, not all-lights
+// PathTracingMIS) joins its mix as a request and the call waits for it; other renders fail.
+static bool servable(const rtg_scene* scene, const rtg_render_desc* desc) {
+  return scene && desc && !desc->guiding && !(scene->all_lights && desc->integrator == RTG_INTEGRATOR_PATH_MIS);
+}
+
 int rtg_render_device(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, float* out_dev, rtg_stats* st) {
   if (!ctx) return RTG_ERR_INVALID;
-  std::lock_guard<std::mutex> lk(ctx->mtx);
-  RTG_GUARD(ctx, {
-    if (!out_dev) fail(RTG_ERR_INVALID, "null output");
-    render_impl(ctx, scene, desc, out_dev, st);
-  })
+  rtg_server* srv = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    srv = ctx->server;
+    if (!srv || !servable(scene, desc)) {
+      RTG_GUARD(ctx, {
+        if (!out_dev) fail(RTG_ERR_INVALID, "null output");
+        render_impl(ctx, scene, desc, out_dev, st);
+      })
+    }
+  }
+  if (!out_dev) {
+    set_error("null output");
+    return RTG_ERR_INVALID;
+  }
+  return serve_render(srv, scene, desc, nullptr, out_dev, st);
 }
+
+static int render_blocking_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, float* out_rgbw,
+                                rtg_stats* st);
 
 int rtg_render(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, float* out_rgbw, rtg_stats* st) {
   if (!ctx) return RTG_ERR_INVALID;
-  std::lock_guard<std::mutex> lk(ctx->mtx);
+  rtg_server* srv = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mtx);
+    srv = ctx->server;
+    if (!srv || !servable(scene, desc)) return render_blocking_impl(ctx, scene, desc, out_rgbw, st);
+  }
+  if (!out_rgbw) {
+    set_error("null output");
+    return RTG_ERR_INVALID;
+  }
+  return serve_render(srv, scene, desc, out_rgbw, nullptr, st);
+}
+
+static int render_blocking_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, float* out_rgbw,
+                                rtg_stats* st) {
   RTG_GUARD(ctx, {
     if (!out_rgbw || !desc) fail(RTG_ERR_INVALID, "null output");
     hip_check(hipSetDevice(ctx->device), "hipSetDevice");

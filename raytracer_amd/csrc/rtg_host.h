@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <atomic>
+#include <cstring>
 #include <mutex>
 #include <stdexcept>
 #include <string>
@@ -58,6 +59,21 @@ struct rtg_scene {
   std::atomic<int> users{0};  // requests of the serving loop that reference this scene
 };
 
+namespace rtg {
+// Kinds of wavefront iteration (run_iterations)
+enum IterMode { IT_PLAIN = 0, IT_VOL = 1, IT_MIXED = 2 };
+// A captured batch of iterations: everything its kernels receive by value is in the key.
+struct GraphKey {
+  int32_t mode, count, capacity, all_lights, count_work, grid, trace_blocks, stack, refill;
+  uint64_t epoch;  // pool allocation generation
+  bool operator==(const GraphKey& o) const { return std::memcmp(this, &o, sizeof(GraphKey)) == 0; }
+};
+struct GraphEntry {
+  GraphKey key;
+  hipGraphExec_t exec;
+};
+}  // namespace rtg
+
 struct rtg_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -80,6 +96,11 @@ struct rtg_ctx {
   rtg_server* server = nullptr;    // the serving loop owning the pool, if any
   rtg::GuideState* guide = nullptr;  // path-guiding network and buffers, created on first use
   std::vector<rtg_scene*> scene_slots;
+  // steady-state iteration batches captured as hipGraphs (one per launch configuration)
+  std::vector<rtg::GraphEntry> graphs;
+  uint64_t pool_epoch = 0;
+  bool use_graphs = true;
+  std::atomic<uint64_t> graph_launches{0}, graph_captures{0};
   void* pinned = nullptr;
   std::recursive_mutex mtx_r;
   std::mutex mtx;
@@ -94,6 +115,14 @@ DevJob make_job(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc& d, float4
 void filter_setup(const rtg_render_desc& d, DevJob& J, float* lut);
 void upload_job_filter(rtg_ctx* ctx, const rtg_render_desc& d, DevJob& J, int slot);
 LaunchCfg launch_cfg(rtg_ctx* ctx);
+// Enqueues `count` iterations of kind `mode` (IterMode) on the context's stream, replaying a cached
+// hipGraph of the batch when graphs are on (captured on first use of a configuration).
+void run_iterations(rtg_ctx* ctx, int mode, const LaunchCfg& cfg, int count);
+void graphs_clear(rtg_ctx* ctx);
+// rtg_render / rtg_render_device while a server runs on the context: the render becomes a request
+// (rtg_serve.cpp); film_dev = the caller's device film (not cleared) or NULL.
+int serve_render(rtg_server* s, rtg_scene* scene, const rtg_render_desc* desc, float* out_host, float* film_dev,
+                 rtg_stats* st);
 void camera_matrices(const rtg_render_desc& d, float* s2c, float* c2w);
 void* dev_alloc(rtg_ctx* ctx, size_t bytes, std::vector<void*>& owned);
 GuideState* guide_state(rtg_ctx* ctx);

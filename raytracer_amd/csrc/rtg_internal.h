@@ -22,6 +22,8 @@ __host__ __device__ inline uint32_t make_leaf_ref(uint32_t first, uint32_t count
 __host__ __device__ inline uint32_t leaf_first(uint32_t ref) { return ref & 0x07FFFFFFu; }
 __host__ __device__ inline uint32_t leaf_count(uint32_t ref) { return ((ref >> 27) & 0xFu) + 1u; }
 
+// Integrators (the RTG_INTEGRATOR_* values of rtg.h)
+enum { INTEGRATOR_PATH_MIS = 0, INTEGRATOR_VOLPATH = 1 };
 // Material types (the RTG_MAT_* values of rtg.h)
 enum { MAT_DIFFUSE = 0, MAT_ROUGH_CONDUCTOR = 1, MAT_ROUGH_DIELECTRIC = 2, MAT_CONDUCTOR = 3, MAT_DIELECTRIC = 4 };
 // type: MAT_*; cls: the shade kernel and queue (0 diffuse, 1 conductors, 2 dielectrics)
@@ -128,6 +130,8 @@ struct DevJob {
   int32_t ox, oy;
   float rx, ry;
   const float* lut;
+  int32_t integrator;   // RTG_INTEGRATOR_PATH_MIS or RTG_INTEGRATOR_VOLPATH
+  int32_t pad_j;
 };
 static const int kFilterLut = 32 * 32;
 // lights per scene the all-lights strategy supports (shadow rays per vertex)
@@ -166,7 +170,7 @@ struct DevPool {
   float4* vol_b;        // volumetric paths: light-sample normal of the last medium vertex
   // queues
   uint32_t* q_ext;      // extension queue
-  uint32_t* q_shade;    // 4 queues of capacity each: diffuse, conductor, dielectric, miss
+  uint32_t* q_shade;    // 5 queues of capacity each: diffuse, conductor, dielectric, miss, volumetric
   uint32_t* q_shadow;
   uint8_t* cls_q;       // material class per extension-queue entry (0..2 BSDF type, 3 miss)
   uint8_t* shflag;      // per slot: the shade stage produced a shadow ray
@@ -176,7 +180,8 @@ enum {
   C_EXT = 0, C_SHADE = 1, C_SHADOW = 5, C_ACTIVE = 6, C_NAN = 7, C_EXTRAYS = 8, C_SHRAYS = 10, C_DONE = 12,
   C_NODES = 14, C_PRIMS = 16, C_SHNODES = 18, C_SHPRIMS = 20,  // u64 work counters (counting mode)
   C_XFETCH = 22, C_SFETCH = 23,  // batch cursors of the persistent extension / shadow traversals
-  C_COUNT = 24
+  C_VOLQ = 24,  // volumetric queue of a mixed iteration (entries of VolPathTracing jobs)
+  C_COUNT = 25
 };
 
 // ---------------------------------------------------------------- path guiding (rtg_guide.hip)
@@ -235,13 +240,18 @@ struct LaunchCfg {
 };
 static const int kMaxStack = 96;
 // ev (optional): 8 events recorded before regen and after each of the 7 stage launches
-// order_dev: job-table slots in claim order (free path slots go to the first job with samples left)
+// order_dev: [0] number of entries n, [1..n] job-table slots in claim order (free path slots go to
+// the first job with samples left); read on the device so that a captured launch stays valid
 void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, const int32_t* order_dev,
-                      int norder, const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev = nullptr,
+                      const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev = nullptr,
                       const DevGuide* guide = nullptr);
 // One iteration of the volumetric integrator (regen, extension traversal, k_vol, shadow rays).
 void launch_iteration_vol(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, const int32_t* order_dev,
-                          int norder, const LaunchCfg& cfg, hipStream_t stream);
+                          const LaunchCfg& cfg, hipStream_t stream);
+// One iteration over a pool holding paths of PathTracingMIS and VolPathTracing jobs: the plain
+// shading stages for the former, k_vol over the volumetric queue for the latter.
+void launch_iteration_mixed(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev,
+                            const int32_t* order_dev, const LaunchCfg& cfg, hipStream_t stream);
 void launch_reset_pool(const DevPool& pool, hipStream_t stream);
 void launch_trace_tap(const DevScene* scene_dev, const float* rays, int n, int any_hit, int32_t* prim,
                       float* t, float* uv, float* p, float* nrm, const LaunchCfg& cfg, hipStream_t stream);

@@ -138,8 +138,7 @@ __device__ __forceinline__ bool guided_vertex(int mt, uint32_t depth) {
 // training sample (position, normal, direction, luminance of the radiance gathered after the
 // vertex divided by the throughput before it) to the guide's ring; one 64-bit atomic per chunk.
 template <bool GUIDED>
-__global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const int32_t* order, int norder,
-                                               DevGuide G) {
+__global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const int32_t* order, DevGuide G) {
   __shared__ uint32_t s_w[4];
   __shared__ uint32_t s_base;
   __shared__ unsigned long long s_base64;
@@ -149,7 +148,7 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
   uint32_t* C = P.counters;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     // shade queues are filled by the classify stage of this iteration
-    C[C_SHADE + 0] = 0; C[C_SHADE + 1] = 0; C[C_SHADE + 2] = 0; C[C_SHADE + 3] = 0;
+    C[C_SHADE + 0] = 0; C[C_SHADE + 1] = 0; C[C_SHADE + 2] = 0; C[C_SHADE + 3] = 0; C[C_VOLQ] = 0;
     C[C_XFETCH] = 0;  // extension-ray batches of this iteration
     if (GUIDED) *G.gq_count = 0;
   }
@@ -249,8 +248,9 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
       }
     }
     // claims, jobs in the host's claim order
+    const int norder = order[0];
     for (int jj = 0; jj < norder; jj++) {
-      const int j = order[jj];
+      const int j = order[1 + jj];
       uint32_t total;
       uint32_t my = (uint32_t)__popc(need);
       uint32_t off = block_excl_scan<4, uint32_t>(my, s_w, total);
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
           P.L[i] = make_float4(0.f, 0.f, 0.f, 1.f);
           P.pixel[i] = pix;
           P.sample[i] = si;
-          P.flags[i] = ((uint32_t)j << JOB_SHIFT) | FL_EMIT;
+          P.flags[i] = ((uint32_t)j << JOB_SHIFT) | FL_EMIT | (J.integrator == INTEGRATOR_VOLPATH ? FL_VOL : 0u);
           P.state[i] = ST_ACTIVE;
           need &= ~(1u << k);
           act |= 1u << k;
@@ -478,14 +478,20 @@ __global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scene
 }
 
 // ---------------------------------------------------------------- classify: shade queues
-// Chunks of 2048 extension-queue entries per block; the four per-class counts are packed into
-// 16-bit fields of one 64-bit word so a single block scan serves all four queues.
+// Chunks of 2048 extension-queue entries per block; the five per-class counts (at most 2048 each)
+// are packed into 12-bit fields of one 64-bit word so a single block scan serves all queues.
+// MIXED: entries of VolPathTracing paths (flag FL_VOL) go to the volumetric queue (class 4), hit
+// or miss, for k_vol; the others keep their material class.
 // GUIDED: also queues the guided vertices of the guided job (guided_vertex) for the guide
 // network's inference.
-template <bool GUIDED>
+constexpr int kClassBits = 12;
+constexpr unsigned long long kClassMask = (1ull << kClassBits) - 1ull;
+static_assert(kRegenSlots < (1 << kClassBits), "class counts of a chunk must fit their field");
+
+template <bool GUIDED, bool MIXED>
 __global__ void __launch_bounds__(256) k_classify(DevPool P, DevGuide G) {
   __shared__ unsigned long long s_w[4];
-  __shared__ uint32_t s_base[4];
+  __shared__ uint32_t s_base[5];
   __shared__ uint32_t s_w32[4];
   __shared__ uint32_t s_gbase;
   uint32_t* C = P.counters;
@@ -498,24 +504,28 @@ __global__ void __launch_bounds__(256) k_classify(DevPool P, DevGuide G) {
 #pragma unroll
     for (int k = 0; k < kRegenK; k++) {
       uint32_t e = base + k;
-      cls[k] = e < n ? (uint32_t)P.cls_q[e] : 4u;
-      if (cls[k] < 4) cnt += 1ull << (16 * cls[k]);
+      cls[k] = e < n ? (uint32_t)P.cls_q[e] : 5u;
+      if (MIXED && e < n && (P.flags[P.q_ext[e]] & FL_VOL)) {
+        cls[k] = 4u;
+        P.cls_q[e] = 4;  // the shadow compaction takes the entry's shadow flag
+      }
+      if (cls[k] < 5) cnt += 1ull << (kClassBits * cls[k]);
     }
     unsigned long long total;
     unsigned long long off = block_excl_scan<4, unsigned long long>(cnt, s_w, total);
-    if (threadIdx.x < 4) {
-      uint32_t t = (uint32_t)((total >> (16 * threadIdx.x)) & 0xFFFFull);
-      s_base[threadIdx.x] = t ? atomicAdd(&C[C_SHADE + threadIdx.x], t) : 0u;
+    if (threadIdx.x < 5) {
+      uint32_t t = (uint32_t)((total >> (kClassBits * threadIdx.x)) & kClassMask);
+      s_base[threadIdx.x] = t ? atomicAdd(&C[threadIdx.x < 4 ? C_SHADE + threadIdx.x : C_VOLQ], t) : 0u;
     }
     __syncthreads();
-    uint32_t pos[4];
+    uint32_t pos[5];
 #pragma unroll
-    for (int q = 0; q < 4; q++) pos[q] = s_base[q] + (uint32_t)((off >> (16 * q)) & 0xFFFFull);
+    for (int q = 0; q < 5; q++) pos[q] = s_base[q] + (uint32_t)((off >> (kClassBits * q)) & kClassMask);
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kRegenK; k++) {
       uint32_t c = cls[k];
-      if (c < 4) {
+      if (c < 5) {
         uint32_t e = base + k;
         P.q_shade[(size_t)c * P.capacity + pos[c]] = P.q_ext[e];
         pos[c]++;
@@ -847,12 +857,16 @@ __device__ __forceinline__ bool bsdf_sample_any(const DevMaterial& m, f3 wi, flo
 }
 __device__ __forceinline__ float max3(f3 v) { return fmaxf(v.x, fmaxf(v.y, v.z)); }
 
+// MIXED = false: every extension-queue entry is volumetric (a render of one VolPathTracing job).
+// MIXED = true: the volumetric queue of a mixed iteration (classify wrote the entries' classes).
+template <bool MIXED>
 __global__ void __launch_bounds__(256, 3) k_vol(DevPool P, const DevScene* scenes, const DevJob* jobs) {
   uint32_t* C = P.counters;
-  const uint32_t n = C[C_EXT];
+  const uint32_t n = MIXED ? C[C_VOLQ] : C[C_EXT];
+  const uint32_t* queue = MIXED ? P.q_shade + (size_t)4 * P.capacity : P.q_ext;
   const uint32_t stride = gridDim.x * blockDim.x;
   for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += stride) {
-    const uint32_t slot = P.q_ext[e];
+    const uint32_t slot = queue[e];
     const uint32_t f = P.flags[slot];
     const uint32_t depth = fl_depth(f);
     const DevJob& J = jobs[fl_job(f)];
@@ -1038,7 +1052,7 @@ __global__ void __launch_bounds__(256, 3) k_vol(DevPool P, const DevScene* scene
     P.flags[slot] = nf;
     P.state[slot] = done ? ST_DONE : ST_ACTIVE;
     P.shflag[slot] = shadow ? 1 : 0;
-    P.cls_q[e] = 0;  // the shadow compaction reads every entry's flag
+    if (!MIXED) P.cls_q[e] = 0;  // the shadow compaction reads every entry's flag
   }
 }
 
@@ -1059,7 +1073,7 @@ __global__ void __launch_bounds__(256) k_compact_shadow(DevPool P) {
       uint32_t e = base + k;
       slots[k] = 0;
       cnt[k] = 0;
-      if (e < n && P.cls_q[e] < 3) {
+      if (e < n && P.cls_q[e] != 3) {  // misses of plain paths produce no shadow ray
         slots[k] = P.q_ext[e];
         cnt[k] = P.shflag[slots[k]];
       }
@@ -1377,7 +1391,7 @@ void launch_medium_tap(const DevScene* scene_dev, int op, const float* in, int n
 static size_t stack_bytes(int block) { return (size_t)block * kLdsStack * 8u; }
 
 void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, const int32_t* order_dev,
-                      int norder, const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev, const DevGuide* guide) {
+                      const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev, const DevGuide* guide) {
   const int g = cfg.grid;
   const DevJob* cj = jobs_dev;
   const int chunks = (pool.capacity + kRegenSlots - 1) / kRegenSlots;
@@ -1391,16 +1405,16 @@ void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* j
   }
   if (ev) (void)hipEventRecord(ev[0], stream);
   if (gd)
-    hipLaunchKernelGGL(k_regen<true>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, norder, G);
+    hipLaunchKernelGGL(k_regen<true>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, G);
   else
-    hipLaunchKernelGGL(k_regen<false>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, norder, G);
+    hipLaunchKernelGGL(k_regen<false>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, G);
   if (ev) (void)hipEventRecord(ev[1], stream);
   hipLaunchKernelGGL(k_extend, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
                      cfg.gstack, cfg.stack, cfg.refill);
   if (gd)
-    hipLaunchKernelGGL(k_classify<true>, dim3(chunks), dim3(256), 0, stream, pool, G);
+    hipLaunchKernelGGL((k_classify<true, false>), dim3(chunks), dim3(256), 0, stream, pool, G);
   else
-    hipLaunchKernelGGL(k_classify<false>, dim3(chunks), dim3(256), 0, stream, pool, G);
+    hipLaunchKernelGGL((k_classify<false, false>), dim3(chunks), dim3(256), 0, stream, pool, G);
   if (ev) (void)hipEventRecord(ev[2], stream);
   if (gd) {
     if (G.sample) launch_guide_infer_pool(G, pool, scenes_dev, cj, g, stream);
@@ -1430,15 +1444,36 @@ void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* j
 }
 
 void launch_iteration_vol(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, const int32_t* order_dev,
-                          int norder, const LaunchCfg& cfg, hipStream_t stream) {
+                          const LaunchCfg& cfg, hipStream_t stream) {
   const int chunks = (pool.capacity + kRegenSlots - 1) / kRegenSlots;
   DevGuide G;
   memset(&G, 0, sizeof(G));
   G.job = -1;
-  hipLaunchKernelGGL(k_regen<false>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, norder, G);
+  hipLaunchKernelGGL(k_regen<false>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, G);
   hipLaunchKernelGGL(k_extend, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev, jobs_dev,
                      cfg.gstack, cfg.stack, cfg.refill);
-  hipLaunchKernelGGL(k_vol, dim3(cfg.grid), dim3(256), 0, stream, pool, scenes_dev, jobs_dev);
+  hipLaunchKernelGGL(k_vol<false>, dim3(cfg.grid), dim3(256), 0, stream, pool, scenes_dev, jobs_dev);
+  hipLaunchKernelGGL(k_compact_shadow, dim3(chunks), dim3(256), 0, stream, pool);
+  hipLaunchKernelGGL(k_shadow, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev,
+                     jobs_dev, cfg.gstack, cfg.stack, cfg.refill);
+}
+
+void launch_iteration_mixed(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev,
+                            const int32_t* order_dev, const LaunchCfg& cfg, hipStream_t stream) {
+  const int g = cfg.grid;
+  const int chunks = (pool.capacity + kRegenSlots - 1) / kRegenSlots;
+  DevGuide G;
+  memset(&G, 0, sizeof(G));
+  G.job = -1;
+  hipLaunchKernelGGL(k_regen<false>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, G);
+  hipLaunchKernelGGL(k_extend, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev, jobs_dev,
+                     cfg.gstack, cfg.stack, cfg.refill);
+  hipLaunchKernelGGL((k_classify<false, true>), dim3(chunks), dim3(256), 0, stream, pool, G);
+  hipLaunchKernelGGL((k_shade<0, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
+  hipLaunchKernelGGL((k_shade<1, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
+  hipLaunchKernelGGL((k_shade<2, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
+  hipLaunchKernelGGL(k_shade_miss<false>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
+  hipLaunchKernelGGL(k_vol<true>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev);
   hipLaunchKernelGGL(k_compact_shadow, dim3(chunks), dim3(256), 0, stream, pool);
   hipLaunchKernelGGL(k_shadow, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev,
                      jobs_dev, cfg.gstack, cfg.stack, cfg.refill);

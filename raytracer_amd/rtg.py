@@ -99,7 +99,7 @@ class rtg_server_stats(C.Structure):
         ("submitted", C.c_uint64), ("admitted", C.c_uint64), ("completed", C.c_uint64), ("cancelled", C.c_uint64),
         ("iterations", C.c_uint64), ("samples", C.c_uint64), ("film_bytes", C.c_uint64),
         ("film_bytes_peak", C.c_uint64), ("running", C.c_int32), ("pending", C.c_int32),
-        ("max_running", C.c_int32), ("reserved", C.c_int32),
+        ("max_running", C.c_int32), ("reserved", C.c_int32), ("graph_launches", C.c_uint64),
     ]
 
 
@@ -132,6 +132,8 @@ SIGNATURES = {
     "rtg_submit": (C.c_int, [P, P, C.POINTER(rtg_render_desc), C.POINTER(C.c_int64)]),
     "rtg_server_step": (C.c_int, [P, C.c_double]),
     "rtg_poll": (C.c_int, [P, C.c_int64, P, C.POINTER(C.c_double)]),
+    "rtg_wait": (C.c_int, [P, C.c_int64, C.c_double, P, C.POINTER(C.c_double)]),
+    "rtg_server_start": (C.c_int, [P]),
     "rtg_cancel": (C.c_int, [P, C.c_int64]),
     "rtg_server_query": (C.c_int, [P, C.POINTER(rtg_server_stats)]),
     "rtg_guide_eval": (C.c_int, [P, P, P, C.c_int32, P]),
@@ -550,6 +552,24 @@ class Server:
         out = np.zeros((H, W, 4), np.float32) if want_image else None
         lat = C.c_double(0.0)
         rc = self.lib.rtg_poll(self.h, C.c_int64(jid), _ptr(out), C.byref(lat))
+        if rc == -5:
+            return "running", None, None
+        if rc == -6:
+            return "cancelled", None, lat.value
+        self.ctx.check(rc)
+        return "done", out, lat.value
+
+    def start(self):
+        """Run the loop on the server's own thread (rtg_server_start); use wait() afterwards."""
+        self.ctx.check(self.lib.rtg_server_start(self.h))
+
+    def wait(self, jid, timeout_ms=0.0, want_image=True):
+        """Block until the request ends (rtg_wait) -> (status, image or None, latency_ms); status
+        "running" only when timeout_ms > 0 passed first."""
+        H, W = self.shapes[jid]
+        out = np.zeros((H, W, 4), np.float32) if want_image else None
+        lat = C.c_double(0.0)
+        rc = self.lib.rtg_wait(self.h, C.c_int64(jid), C.c_double(float(timeout_ms)), _ptr(out), C.byref(lat))
         if rc == -5:
             return "running", None, None
         if rc == -6:

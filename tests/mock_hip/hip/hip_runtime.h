@@ -15,7 +15,7 @@ struct uint2 { uint32_t x, y; };
 struct uint4 { uint32_t x, y, z, w; };
 typedef struct ihipStream_t* hipStream_t;
 typedef struct ihipEvent_t* hipEvent_t;
-enum hipError_t { hipSuccess = 0, hipErrorOutOfMemory = 2 };
+enum hipError_t { hipSuccess = 0, hipErrorOutOfMemory = 2, hipErrorNotSupported = 801 };
 enum hipMemcpyKind { hipMemcpyHostToDevice = 1, hipMemcpyDeviceToHost = 2, hipMemcpyDeviceToDevice = 3 };
 #define hipHostMallocDefault 0
 inline hipError_t hipSetDevice(int) { return hipSuccess; }
@@ -44,3 +44,15 @@ inline hipError_t hipEventElapsedTime(float* ms, hipEvent_t, hipEvent_t) { *ms =
 inline uint4 make_uint4(uint32_t a, uint32_t b, uint32_t c, uint32_t d) { return uint4{a, b, c, d}; }
 inline float4 make_float4(float a, float b, float c, float d) { return float4{a, b, c, d}; }
 inline uint2 make_uint2(uint32_t a, uint32_t b) { return uint2{a, b}; }
+// ---- graphs: capture is reported as unavailable, so the host launches directly
+typedef struct ihipGraph_t* hipGraph_t;
+typedef struct ihipGraphExec_t* hipGraphExec_t;
+typedef struct ihipGraphNode_t* hipGraphNode_t;
+enum hipStreamCaptureMode { hipStreamCaptureModeGlobal = 0, hipStreamCaptureModeThreadLocal = 1,
+                            hipStreamCaptureModeRelaxed = 2 };
+inline hipError_t hipStreamBeginCapture(hipStream_t, hipStreamCaptureMode) { return hipErrorNotSupported; }
+inline hipError_t hipStreamEndCapture(hipStream_t, hipGraph_t*) { return hipErrorNotSupported; }
+inline hipError_t hipGraphInstantiate(hipGraphExec_t*, hipGraph_t, hipGraphNode_t*, char*, size_t) { return hipErrorNotSupported; }
+inline hipError_t hipGraphDestroy(hipGraph_t) { return hipSuccess; }
+inline hipError_t hipGraphExecDestroy(hipGraphExec_t) { return hipSuccess; }
+inline hipError_t hipGraphLaunch(hipGraphExec_t, hipStream_t) { return hipErrorNotSupported; }

@@ -5,8 +5,8 @@
 #include <cstdio>
 #include "rtg_internal.h"
 namespace rtg {
-void launch_iteration(const DevPool& P, const DevScene*, DevJob* jobs, const int32_t* order, int norder,
-                      const LaunchCfg&, hipStream_t, hipEvent_t*, const DevGuide*) {
+void launch_iteration(const DevPool& P, const DevScene*, DevJob* jobs, const int32_t* order, const LaunchCfg&,
+                      hipStream_t, hipEvent_t*, const DevGuide*) {
   uint64_t free_slots = (uint64_t)P.capacity;
   for (int j = 0; j < kMaxJobsDev; j++) {  // finish everything claimed earlier
     DevJob& J = jobs[j];
@@ -18,8 +18,9 @@ void launch_iteration(const DevPool& P, const DevScene*, DevJob* jobs, const int
     }
     J.done = fin;
   }
+  const int norder = order[0];
   for (int jj = 0; jj < norder; jj++) {
-    int j = order[jj];
+    int j = order[1 + jj];
     if (j < 0 || j >= kMaxJobsDev) { fprintf(stderr, "bad order entry\n"); abort(); }
     DevJob& J = jobs[j];
     if (!J.active || J.next >= J.total || free_slots == 0) continue;
@@ -28,10 +29,14 @@ void launch_iteration(const DevPool& P, const DevScene*, DevJob* jobs, const int
     free_slots -= take;
   }
 }
-// the volumetric integrator's iteration finishes samples the same way
-void launch_iteration_vol(const DevPool& P, const DevScene* S, DevJob* jobs, const int32_t* order, int norder,
+// the volumetric and mixed iterations finish samples the same way
+void launch_iteration_vol(const DevPool& P, const DevScene* S, DevJob* jobs, const int32_t* order,
                           const LaunchCfg& cfg, hipStream_t st) {
-  launch_iteration(P, S, jobs, order, norder, cfg, st, nullptr, nullptr);
+  launch_iteration(P, S, jobs, order, cfg, st, nullptr, nullptr);
+}
+void launch_iteration_mixed(const DevPool& P, const DevScene* S, DevJob* jobs, const int32_t* order,
+                            const LaunchCfg& cfg, hipStream_t st) {
+  launch_iteration(P, S, jobs, order, cfg, st, nullptr, nullptr);
 }
 void launch_medium_tap(const DevScene*, int, const float*, int, const float*, int, float*, hipStream_t) {}
 void launch_tonemap(const float4*, int, int, float, unsigned int*, uint8_t*, hipStream_t) {}

@@ -1,6 +1,8 @@
 """Serving loop (continuous batching on the shared path pool): a request renders the same image
-alone (rtg_render, same seed) and inside a mix of scenes, resolutions and spp that arrive while
-others run; HBM-budget admission; cancellation; concurrent submission from another thread.
+alone (rtg_render without a server, same seed) and inside a mix of scenes — plain and volumetric
+(VolPathTracing) — resolutions and spp that arrive while others run; HBM-budget admission;
+cancellation; concurrent submission from another thread; the server's own loop thread with
+rtg_wait; rtg_render and rtg_render_device on a serving context joining the mix.
 
 Tolerance: the film accumulates with float atomics, so only the order of additions may differ
 between a request alone and in the mix: allclose(rtol=1e-5, atol=1e-6) on (rgb sum, weight)."""
@@ -9,6 +11,9 @@ import time
 
 import numpy as np
 import pytest
+# torch (device buffers for the caller-owned film case) is imported before librtg creates a
+# context: torch ships its own HIP runtime, which then serves both; imported after, it finds no GPU
+import torch
 
 from raytracer_amd import rtg, scenes
 
@@ -26,7 +31,7 @@ def ctx():
 def resident(ctx):
     out = {}
     for name, sd in (("cornell", scenes.cornell()), ("glossy", scenes.glossy_cornell()),
-                     ("displaced", scenes.displaced_sphere(level=5))):
+                     ("displaced", scenes.displaced_sphere(level=5)), ("vol", scenes.vol_cornell(res=32))):
         out[name] = (sd, ctx.upload(sd))
     return out
 
@@ -79,17 +84,92 @@ def test_same_alone_and_in_mix(ctx, resident):
     assert st["samples"] == sum(w * h * spp for _, w, h, spp, _ in MIX)
 
 
-def test_render_refused_while_server_alive_and_restored_after(ctx, resident):
+def test_render_on_a_serving_context_joins_the_mix(ctx, resident):
+    """rtg_render while a server exists becomes a request of the server (its caller runs the loop
+    when no server thread does) and returns the image it gives without the server; a guided
+    render, which the server does not take, fails; one server per context."""
     sc, cam, rp = _args(resident, *MIX[0])
+    vsc, vcam, vrp = _args(resident, "vol", 40, 32, 16, 3)
     before = sc.render(cam, rp)[0]
+    vbefore = vsc.render(vcam, vrp)[0]
     srv = rtg.Server(ctx)
-    with pytest.raises(rtg.RtgError):
-        sc.render(cam, rp)
-    with pytest.raises(rtg.RtgError):
-        rtg.Server(ctx)  # one server per context
-    srv.close()
+    try:
+        during, st = sc.render(cam, rp)
+        assert st.samples == cam["width"] * cam["height"] * rp["spp"]
+        assert np.allclose(before, during, rtol=1e-5, atol=1e-6)
+        assert np.allclose(vbefore, vsc.render(vcam, vrp)[0], rtol=1e-5, atol=1e-6)
+        with pytest.raises(rtg.RtgError):
+            sc.render(cam, dict(rp, guiding=1))
+        with pytest.raises(rtg.RtgError):
+            rtg.Server(ctx)  # one server per context
+        assert srv.stats()["completed"] == 2
+    finally:
+        srv.close()
     after = sc.render(cam, rp)[0]
     assert np.allclose(before, after, rtol=1e-5, atol=1e-6)
+
+
+def test_volumetric_requests_share_the_pool(ctx, resident):
+    """VolPathTracing requests run in mixed iterations next to PathTracingMIS requests of other
+    scenes; each equals its own render alone."""
+    reqs = [("vol", 48, 40, 16, 51), ("cornell", 64, 48, 16, 52), ("vol", 32, 32, 64, 53),
+            ("displaced", 64, 64, 4, 54), ("glossy", 48, 48, 16, 55)]
+    alone = []
+    for r in reqs:
+        sc, cam, rp = _args(resident, *r)
+        alone.append(sc.render(cam, rp)[0])
+    with rtg.Server(ctx) as srv:
+        ids = [srv.submit(*_args(resident, *r)) for r in reqs]
+        srv.step()
+        st = srv.stats()
+        for i, jid in enumerate(ids):
+            status, img, _ = srv.poll(jid)
+            assert status == "done", (i, status)
+            assert np.allclose(img, alone[i], rtol=1e-5, atol=1e-6), (reqs[i], np.abs(img - alone[i]).max())
+    assert st["completed"] == len(reqs) and st["max_running"] >= 3
+
+
+def server_thread_and_wait(ctx, resident, device_film=True):
+    """rtg_server_start: the server's thread runs the loop; rtg_wait blocks until each request is
+    done; (device_film) a request accumulates into a caller-owned device film through
+    rtg_render_device; a wait with a short timeout on a long request reports it still running; the
+    loop sleeps when idle. Returns the server's final stats."""
+    reqs = [("cornell", 64, 64, 16, 61), ("vol", 32, 32, 16, 62), ("glossy", 48, 32, 64, 63)]
+    alone = []
+    for r in reqs:
+        sc, cam, rp = _args(resident, *r)
+        alone.append(sc.render(cam, rp)[0])
+    with rtg.Server(ctx) as srv:
+        srv.start()
+        with pytest.raises(rtg.RtgError):
+            srv.step()  # the server thread owns the loop
+        ids = [srv.submit(*_args(resident, *r)) for r in reqs]
+        for i, jid in enumerate(ids):
+            status, img, lat = srv.wait(jid)
+            assert status == "done" and lat > 0
+            assert np.allclose(img, alone[i], rtol=1e-5, atol=1e-6), reqs[i]
+        if device_film:
+            # a device film of the caller's (not cleared: the render adds into it)
+            sc, cam, rp = _args(resident, *reqs[0])
+            film = torch.full((cam["height"] * cam["width"] * 4,), 1.0, dtype=torch.float32, device="cuda")
+            st = sc.render_device(cam, rp, film.data_ptr())
+            torch.cuda.synchronize()
+            got = film.cpu().numpy().reshape(cam["height"], cam["width"], 4) - 1.0
+            assert st.samples == cam["width"] * cam["height"] * rp["spp"]
+            assert np.allclose(got, alone[0], rtol=1e-5, atol=1e-5)
+        big = srv.submit(*_args(resident, "displaced", 512, 512, 256, 64))
+        status, _, _ = srv.wait(big, timeout_ms=1.0, want_image=False)
+        assert status == "running"
+        srv.cancel(big)
+        assert srv.wait(big)[0] == "cancelled"
+        stt = srv.stats()
+        assert stt["running"] == 0 and stt["pending"] == 0
+    return stt
+
+
+def test_server_thread_wait_and_device_film(ctx, resident):
+    stt = server_thread_and_wait(ctx, resident)
+    assert stt["graph_launches"] > 0  # iteration batches replayed from hipGraphs
 
 
 def test_budget_admission_serialises_films(ctx, resident):

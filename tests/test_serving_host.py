@@ -25,13 +25,14 @@ ctx = rtg.Context(0)
 ctx.set_pool(1 << 16)
 res = {{}}
 for name, sd in (("cornell", scenes.cornell()), ("glossy", scenes.glossy_cornell()),
-                 ("displaced", scenes.displaced_sphere(level=3))):
+                 ("displaced", scenes.displaced_sphere(level=3)), ("vol", scenes.vol_cornell(res=16))):
     res[name] = (sd, ctx.upload(sd))
-for fn in [T.test_same_alone_and_in_mix, T.test_render_refused_while_server_alive_and_restored_after,
-           T.test_budget_admission_serialises_films, T.test_cancel_drains_and_others_finish,
-           T.test_submit_from_another_thread_while_stepping]:
+for fn in [T.test_same_alone_and_in_mix, T.test_render_on_a_serving_context_joins_the_mix,
+           T.test_volumetric_requests_share_the_pool, T.test_budget_admission_serialises_films,
+           T.test_cancel_drains_and_others_finish, T.test_submit_from_another_thread_while_stepping,
+           lambda c, r: T.server_thread_and_wait(c, r, device_film=False)]:
     fn(ctx, res)
-    print("PASS", fn.__name__, flush=True)
+    print("PASS", getattr(fn, "__name__", "?"), flush=True)
 ctx.close()
 print("ALL OK", flush=True)
 """
