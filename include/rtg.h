@@ -45,8 +45,10 @@ enum {
 };
 /* Microfacet distributions: bsdf/MicrofacetDistribution.h:8-12 */
 enum { RTG_DIST_BECKMANN = 0, RTG_DIST_GGX = 2 };
-/* Light selection strategies: integrator/Integrator.h:18-22 */
-enum { RTG_LIGHTS_ALL = 0, RTG_LIGHTS_UNIFORM = 1, RTG_LIGHTS_WEIGHTED = 2 };
+/* Light selection strategies: integrator/Integrator.h:18-22. RTG_LIGHTS_DEFAULT (0, what a
+ * zero-initialised description holds) is the reference's default, one_light_uniform
+ * (Integrator.cpp:14). */
+enum { RTG_LIGHTS_DEFAULT = 0, RTG_LIGHTS_UNIFORM = 1, RTG_LIGHTS_WEIGHTED = 2, RTG_LIGHTS_ALL = 3 };
 /* Samplers: sampler/RandomSampler.cpp, sampler/StratifiedSampler.cpp */
 enum { RTG_FILTER_BOX = 0, RTG_FILTER_GAUSSIAN = 1, RTG_FILTER_MITCHELL = 2 };
 enum { RTG_SAMPLER_RANDOM = 0, RTG_SAMPLER_STRATIFIED = 1 };
@@ -129,8 +131,8 @@ typedef struct rtg_scene_desc {
   int32_t env_width;
   int32_t env_height;
   float env_scale;
-  int32_t light_strategy;      /* RTG_LIGHTS_UNIFORM, RTG_LIGHTS_WEIGHTED or RTG_LIGHTS_ALL (at most 8
-                                  lights; rtg_render only) */
+  int32_t light_strategy;      /* RTG_LIGHTS_DEFAULT (= UNIFORM), RTG_LIGHTS_UNIFORM, RTG_LIGHTS_WEIGHTED or
+                                  RTG_LIGHTS_ALL (at most 8 lights; not served) */
   /* Camera medium (the scene file's top-level <medium>, core/XMLLoader.cpp:88-92,159-172): a
    * Heterogeneous medium (medium/Heterogeneous.cpp) over a GridVolume density grid
    * (volume/GridVolume.cpp) with a Henyey-Greenstein phase function (medium/HGPhaseFunction.cpp).
@@ -215,6 +217,8 @@ typedef struct rtg_stats {
   double guide_loss;           /* loss of the last training step */
   double guide_train_ms;       /* device time of the training steps (RTG_PROFILE_EVENTS) */
   uint64_t guide_queries;      /* network inferences (guided camera vertices) */
+  uint64_t stack_spills;       /* traversal-stack pushes past the 16 on-chip (LDS) entries into the
+                                  global spill area (RTG_PROFILE_COUNT) */
 } rtg_stats;
 
 enum { RTG_PROFILE_EVENTS = 1, RTG_PROFILE_COUNT = 2 };
@@ -283,6 +287,22 @@ int rtg_tonemap(rtg_ctx* ctx, const float* rgbw, int32_t width, int32_t height, 
  * uv: barycentrics (b1, b2) for triangles. p: hit point, nrm: shading normal (closest hit only). */
 int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int32_t any_hit, int32_t* prim,
               float* t, float* uv, float* p, float* nrm);
+/* Traversal-stack pushes of the context's last rtg_trace call that went past the 16 on-chip (LDS)
+ * entries of a ray's stack into the global spill area (no reference counterpart: the reference's
+ * stack is a 64-entry array, accelStruct/Bvh.cpp:392-400). A stack deeper than the BVH builder's
+ * bound fails the render or trace with RTG_ERR_HIP instead of losing geometry. */
+int rtg_trace_spills(rtg_ctx* ctx, uint64_t* spilled_pushes);
+
+/* Sampler tap for Sampler::getNextSample2D / getNextSample1D (sampler/Sampler.h:18-31;
+ * StratifiedSampler.cpp:18-124, RandomSampler.cpp:28-45): the draws the renderer's sampler gives
+ * each of n (pixel index y * width + x, sample index) pairs of the render `desc` (sampler, spp —
+ * rounded up to a square for the stratified sampler —, dimension, seed): out2 (n x n2 x 2) the
+ * first n2 2D draws, out1 (n x n1) the first n1 1D draws. The GPU sampler is counter-based, not
+ * the reference's sequential PCG32 stream; with the stratified sampler the first `dimension` 2D
+ * draws of a pixel's samples cover its sqrt(spp) x sqrt(spp) strata once each, its first
+ * `dimension` 1D draws its spp strata once each, as the reference's shuffled strata do. */
+int rtg_sampler_batch(rtg_ctx* ctx, const rtg_render_desc* desc, const uint32_t* pixel_sample, int32_t n,
+                      int32_t n2, int32_t n1, float* out2, float* out1);
 
 /* Parity tap for BSDF::eval/pdf/sample (bsdf/BSDF.h:70-74) on local directions.
  * op 0: eval+pdf of (wi, wo) -> f (n x 3), pdf (n).

@@ -165,6 +165,18 @@ __device__ __forceinline__ PathRng make_rng(uint64_t seed, uint32_t pixel, uint3
   r.dimension = J.sampler == 1 ? J.dimension : 0;
   return r;
 }
+// (cell + jitter) / n for a jitter in [0, 1), kept inside [cell / n, (cell + 1) / n) exactly: the
+// float sum cell + jitter rounds up to cell + 1 for a jitter within an ulp of 1, and the product
+// with a rounded 1 / n can leave the cell at either end. The fused multiply-add's sign is that of
+// the exact x n - bound, so the corrections step one ulp at a time exactly while needed (the two
+// roundings put x at most a few ulps outside the cell, which is many ulps wide).
+__device__ __forceinline__ float in_stratum(uint32_t cell, float jitter, uint32_t n) {
+  float x = ((float)cell + jitter) * (1.f / (float)n);
+  for (int i = 0; i < 8 && fmaf(x, (float)n, -(float)(cell + 1u)) >= 0.f; i++)
+    x = __uint_as_float(__float_as_uint(x) - 1u);
+  for (int i = 0; i < 8 && fmaf(x, (float)n, -(float)cell) < 0.f; i++) x = __uint_as_float(__float_as_uint(x) + 1u);
+  return smin(x, ONE_MINUS_EPS);
+}
 // 2D draw number k: stratified over the pixel's samples for k < dimension
 // (sampler/StratifiedSampler.cpp:145-151,174-215), independent uniform beyond.
 __device__ __forceinline__ void sample2d(const PathRng& r, uint32_t k, float& x, float& y) {
@@ -173,9 +185,8 @@ __device__ __forceinline__ void sample2d(const PathRng& r, uint32_t k, float& x,
     uint32_t sx = j / r.sqrt_spp, sy = j % r.sqrt_spp;
     float jx = to_unit(draw32(r.kpath, 0x40000000u + 2u * k));
     float jy = to_unit(draw32(r.kpath, 0x40000001u + 2u * k));
-    float dx = 1.f / (float)r.sqrt_spp;
-    x = smin(((float)sx + jx) * dx, ONE_MINUS_EPS);
-    y = smin(((float)sy + jy) * dx, ONE_MINUS_EPS);
+    x = in_stratum(sx, jx, r.sqrt_spp);
+    y = in_stratum(sy, jy, r.sqrt_spp);
   } else {
     x = to_unit(draw32(r.kpath, 2u * k));
     y = to_unit(draw32(r.kpath, 2u * k + 1u));
@@ -185,8 +196,7 @@ __device__ __forceinline__ float sample1d(const PathRng& r, uint32_t k) {
   if ((int32_t)k < r.dimension) {
     uint32_t j = permute(r.sample, r.spp, draw32(r.kpix, 0x20000001u + 2u * k));
     float u = to_unit(draw32(r.kpath, 0x60000000u + k));
-    float dx = 1.f / (float)r.spp;
-    return smin(((float)j + u) * dx, ONE_MINUS_EPS);
+    return in_stratum(j, u, r.spp);
   }
   return to_unit(draw32(r.kpath, 0x80000000u + k));
 }
@@ -361,6 +371,8 @@ struct TravStack {
   uint2* g;        // global spill, this lane's column
   int gstride;     // threads in the grid
   int cap;         // total entries (>= the deepest stack any ray needs, set by the host)
+  uint32_t* spill; // counts pushes into the global spill area (null: not counted)
+  uint32_t* ovf;   // set to 1 when a push finds the stack full (the host reports it as an error)
 };
 __device__ __forceinline__ void stk_push(const TravStack& s, int& sp, uint32_t ref, float t) {
   if (sp < kLdsStack) {
@@ -369,6 +381,11 @@ __device__ __forceinline__ void stk_push(const TravStack& s, int& sp, uint32_t r
   } else if (sp < s.cap) {
     s.g[(size_t)(sp - kLdsStack) * s.gstride] = make_uint2(ref, __float_as_uint(t));
     sp++;
+    if (s.spill) atomicAdd(s.spill, 1u);
+  } else {
+    // The builder's bound on the stack depth was exceeded: the subtree would be lost. Never
+    // expected; flagged so that the render fails instead of dropping geometry silently.
+    *s.ovf = 1u;
   }
 }
 // Pushes the nh - 1 farther hit children of a node (sorted near to far: 1, 2, 3), farthest at the

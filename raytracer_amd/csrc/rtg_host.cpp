@@ -216,9 +216,11 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   if (!desc) fail(RTG_ERR_INVALID, "null scene description");
   if (desc->num_meshes < 0 || desc->num_spheres < 0 || desc->num_materials <= 0)
     fail(RTG_ERR_INVALID, "scene needs at least one material and non-negative counts");
-  if (desc->light_strategy != RTG_LIGHTS_UNIFORM && desc->light_strategy != RTG_LIGHTS_WEIGHTED &&
-      desc->light_strategy != RTG_LIGHTS_ALL)
+  if (desc->light_strategy != RTG_LIGHTS_DEFAULT && desc->light_strategy != RTG_LIGHTS_UNIFORM &&
+      desc->light_strategy != RTG_LIGHTS_WEIGHTED && desc->light_strategy != RTG_LIGHTS_ALL)
     fail(RTG_ERR_INVALID, "unknown light_strategy");
+  // the reference's default strategy (integrator/Integrator.cpp:14) for a zero-initialised field
+  const int strategy = desc->light_strategy == RTG_LIGHTS_DEFAULT ? RTG_LIGHTS_UNIFORM : desc->light_strategy;
   int slot = -1;
   for (int i = 0; i < kMaxScenes; i++)
     if (!ctx->scene_slots[i]) {
@@ -471,7 +473,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   std::vector<float> lw, lcdf;
   float lsum;
   for (auto& L : lights) {
-    if (desc->light_strategy == RTG_LIGHTS_WEIGHTED) {
+    if (strategy == RTG_LIGHTS_WEIGHTED) {
       const float PI = 3.14159265358979323846f;
       float pw[3];
       if (L.type == LIGHT_MESH) {
@@ -485,7 +487,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
     }
   }
   cdf_build(lw, lcdf, lsum);
-  if (desc->light_strategy == RTG_LIGHTS_ALL && (int)lights.size() > kAllLights)
+  if (strategy == RTG_LIGHTS_ALL && (int)lights.size() > kAllLights)
     fail(RTG_ERR_INVALID, "the all_lights strategy supports at most " + std::to_string(kAllLights) + " lights");
   // upload
   uint64_t bytes = 0;
@@ -520,7 +522,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   D.nlights = (int)lights.size();
   D.lights = upload(ctx, lights, own, bytes);
   D.light_cdf = upload(ctx, lcdf, own, bytes);
-  D.strategy = desc->light_strategy;
+  D.strategy = strategy;
   D.emit_tri = upload(ctx, emit_tri, own, bytes);
   D.emit_nrm = upload(ctx, emit_nrm, own, bytes);
   D.emit_cdf = upload(ctx, emit_cdf, own, bytes);
@@ -584,7 +586,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   sc->ntri = ntri;
   sc->nsph = nsph;
   sc->ntex = (int)texs.size();
-  sc->all_lights = desc->light_strategy == RTG_LIGHTS_ALL;
+  sc->all_lights = strategy == RTG_LIGHTS_ALL;
   ctx->scene_slots[slot] = sc.get();
   return sc.release();
 }
@@ -880,6 +882,7 @@ int rtg_ctx_create(int device, rtg_ctx** out) {
     // traversal stack spill area for every lane of a traversal launch
     size_t lanes = (size_t)ctx->trace_blocks * 128;
     hip_check(hipMalloc(&ctx->gstack, lanes * (kMaxStack - 16) * sizeof(uint2)), "hipMalloc stack spill");
+    hip_check(hipMalloc(&ctx->tap_ctr, 2 * sizeof(uint32_t)), "hipMalloc trace counters");
   } catch (const RtgError& e) {
     int code = e.code;
     delete ctx;
@@ -907,6 +910,7 @@ int rtg_ctx_destroy(rtg_ctx* ctx) {
   if (ctx->scenes_dev) (void)hipFree(ctx->scenes_dev);
   if (ctx->jobs_dev) (void)hipFree(ctx->jobs_dev);
   if (ctx->gstack) (void)hipFree(ctx->gstack);
+  if (ctx->tap_ctr) (void)hipFree(ctx->tap_ctr);
   if (ctx->order_dev) (void)hipFree(ctx->order_dev);
   if (ctx->lut_dev) (void)hipFree(ctx->lut_dev);
   if (ctx->pinned) (void)hipHostFree(ctx->pinned);
@@ -1007,8 +1011,8 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   if (all_lights && guided) fail(RTG_ERR_INVALID, "guiding is not available with the all_lights strategy");
   DevJob J = make_job(ctx, scene, *desc, (float4*)out_dev);
   upload_job_filter(ctx, *desc, J, 0);
-  // pinned page: [0] done count, [16] guide ring and inference counts, [40] guide loss, [64] claim
-  // order, [256] job
+  // pinned page: [0] done count, [16] guide ring and inference counts, [40] guide loss, [48] trace
+  // tap counters (rtg_trace), [64] claim order, [256] job
   uint64_t* done = (uint64_t*)ctx->pinned;
   unsigned long long* ring_cnt = (unsigned long long*)((char*)ctx->pinned + 16);
   float* loss_h = (float*)((char*)ctx->pinned + 40);
@@ -1140,9 +1144,10 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   (void)hipEventElapsedTime(&ms, e0, e1);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  uint32_t C[C_COUNT];
+  hip_check(hipMemcpy(C, ctx->pool.counters, sizeof(C), hipMemcpyDeviceToHost), "counters");
+  if (C[C_OVF]) fail(RTG_ERR_HIP, "BVH traversal stack overflow (the builder's depth bound was exceeded)");
   if (st) {
-    uint32_t C[C_COUNT];
-    hip_check(hipMemcpy(C, ctx->pool.counters, sizeof(C), hipMemcpyDeviceToHost), "counters");
     auto u64 = [&](int i) {
       uint64_t v;
       std::memcpy(&v, &C[i], 8);
@@ -1159,6 +1164,7 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
     st->prims_tested = u64(C_PRIMS);
     st->shadow_nodes_visited = u64(C_SHNODES);
     st->shadow_prims_tested = u64(C_SHPRIMS);
+    st->stack_spills = C[C_SPILL];
     for (int k = 0; k < 8; k++) {
       st->stage_ms[k] = stage_ms[k];
       st->stage_launches[k] = prof ? iters : 0;
@@ -1262,20 +1268,65 @@ int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int3
       cfg.stack = scene->stack;
       cfg.gstack = ctx->gstack;
       cfg.refill = ctx->refill;
-      launch_trace_tap(ctx->scenes_dev + scene->slot, dr, n, any_hit, dp, dt, duv, dpp, dn, cfg, ctx->stream);
+      launch_fill_zero(ctx->tap_ctr, 2 * sizeof(uint32_t), ctx->stream);
+      launch_trace_tap(ctx->scenes_dev + scene->slot, dr, n, any_hit, dp, dt, duv, dpp, dn, ctx->tap_ctr, cfg,
+                       ctx->stream);
       hip_check(hipGetLastError(), "trace launch");
+      uint32_t* ctr_h = (uint32_t*)((char*)ctx->pinned + 48);
+      hip_check(hipMemcpyAsync(ctr_h, ctx->tap_ctr, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
+                "trace counters");
       hip_check(hipMemcpyAsync(prim, dp, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "prim");
       if (t) hip_check(hipMemcpyAsync(t, dt, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "t");
       if (uv) hip_check(hipMemcpyAsync(uv, duv, (size_t)n * 8, hipMemcpyDeviceToHost, ctx->stream), "uv");
       if (p) hip_check(hipMemcpyAsync(p, dpp, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "p");
       if (nrm) hip_check(hipMemcpyAsync(nrm, dn, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "n");
       hip_check(hipStreamSynchronize(ctx->stream), "trace");
+      ctx->last_trace_spills = ctr_h[0];
+      if (ctr_h[1]) fail(RTG_ERR_HIP, "BVH traversal stack overflow (the builder's depth bound was exceeded)");
     } catch (...) {
       for (void* q : own) (void)hipFree(q);
       throw;
     }
     for (void* q : own) (void)hipFree(q);
   })
+}
+
+int rtg_sampler_batch(rtg_ctx* ctx, const rtg_render_desc* desc, const uint32_t* pixel_sample, int32_t n,
+                      int32_t n2, int32_t n1, float* out2, float* out1) {
+  if (!ctx) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  RTG_GUARD(ctx, {
+    if (!desc || !pixel_sample || n < 0 || n2 < 0 || n1 < 0 || (n2 && !out2) || (n1 && !out1))
+      fail(RTG_ERR_INVALID, "bad sampler tap arguments");
+    if (n == 0) return RTG_OK;
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    rtg_scene dummy;
+    dummy.slot = 0;
+    DevJob J = make_job(ctx, &dummy, *desc, nullptr);  // validates and rounds the sample count
+    std::vector<void*> own;
+    uint32_t* dps = (uint32_t*)dev_alloc(ctx, (size_t)n * 8, own);
+    float* d2 = n2 ? (float*)dev_alloc(ctx, (size_t)n * n2 * 8, own) : nullptr;
+    float* d1 = n1 ? (float*)dev_alloc(ctx, (size_t)n * n1 * 4, own) : nullptr;
+    try {
+      hip_check(hipMemcpyAsync(dps, pixel_sample, (size_t)n * 8, hipMemcpyHostToDevice, ctx->stream), "pairs");
+      launch_sampler_tap(J, dps, n, n2, n1, d2, d1, ctx->stream);
+      hip_check(hipGetLastError(), "sampler tap launch");
+      if (n2) hip_check(hipMemcpyAsync(out2, d2, (size_t)n * n2 * 8, hipMemcpyDeviceToHost, ctx->stream), "2d");
+      if (n1) hip_check(hipMemcpyAsync(out1, d1, (size_t)n * n1 * 4, hipMemcpyDeviceToHost, ctx->stream), "1d");
+      hip_check(hipStreamSynchronize(ctx->stream), "sampler tap");
+    } catch (...) {
+      for (void* q : own) (void)hipFree(q);
+      throw;
+    }
+    for (void* q : own) (void)hipFree(q);
+  })
+}
+
+int rtg_trace_spills(rtg_ctx* ctx, uint64_t* spilled_pushes) {
+  if (!ctx || !spilled_pushes) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  *spilled_pushes = ctx->last_trace_spills;
+  return RTG_OK;
 }
 
 int rtg_tonemap(rtg_ctx* ctx, const float* rgbw, int32_t width, int32_t height, int32_t mode, float exposure,

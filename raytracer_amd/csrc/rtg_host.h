@@ -89,6 +89,8 @@ struct rtg_ctx {
   rtg::DevScene* scenes_dev = nullptr;
   rtg::DevJob* jobs_dev = nullptr;
   uint2* gstack = nullptr;
+  uint32_t* tap_ctr = nullptr;        // trace tap: [0] spilled stack pushes, [1] stack overflow
+  uint64_t last_trace_spills = 0;
   int refill = rtg::kTraceRefill;
   float* lut_dev = nullptr;  // kMaxJobs filter tables of kFilterLut weights
   int trace_blocks = 0;

@@ -181,7 +181,9 @@ enum {
   C_NODES = 14, C_PRIMS = 16, C_SHNODES = 18, C_SHPRIMS = 20,  // u64 work counters (counting mode)
   C_XFETCH = 22, C_SFETCH = 23,  // batch cursors of the persistent extension / shadow traversals
   C_VOLQ = 24,  // volumetric queue of a mixed iteration (entries of VolPathTracing jobs)
-  C_COUNT = 25
+  C_SPILL = 25, // traversal-stack pushes past the LDS entries (counting mode)
+  C_OVF = 26,   // 1 when a traversal stack overflowed (an error)
+  C_COUNT = 27
 };
 
 // ---------------------------------------------------------------- path guiding (rtg_guide.hip)
@@ -253,8 +255,10 @@ void launch_iteration_vol(const DevPool& pool, const DevScene* scenes_dev, DevJo
 void launch_iteration_mixed(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev,
                             const int32_t* order_dev, const LaunchCfg& cfg, hipStream_t stream);
 void launch_reset_pool(const DevPool& pool, hipStream_t stream);
+// ctr: two device words, [0] += pushes into the global stack spill, [1] = 1 on a stack overflow
 void launch_trace_tap(const DevScene* scene_dev, const float* rays, int n, int any_hit, int32_t* prim,
-                      float* t, float* uv, float* p, float* nrm, const LaunchCfg& cfg, hipStream_t stream);
+                      float* t, float* uv, float* p, float* nrm, uint32_t* ctr, const LaunchCfg& cfg,
+                      hipStream_t stream);
 void launch_tonemap(const float4* film, int n, int mode, float expo, unsigned int* maxbits, uint8_t* out,
                     hipStream_t stream);
 void launch_medium_tap(const DevScene* scene_dev, int op, const float* in, int n, const float* u, int k, float* out,
@@ -262,6 +266,9 @@ void launch_medium_tap(const DevScene* scene_dev, int op, const float* in, int n
 void launch_bsdf_tap(const DevMaterial* mat, int op, const float* wi, const float* wo_or_u, int n,
                      float* out_dir, float* f, float* pdf, float* rel_eta, hipStream_t stream);
 void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream);
+// ps: n (pixel, sample) pairs -> out2 (n x n2 x 2) and out1 (n x n1) draws of the job's sampler
+void launch_sampler_tap(const DevJob& J, const uint32_t* ps, int n, int n2, int n1, float* out2, float* out1,
+                        hipStream_t stream);
 void launch_guide_sample_tap(const float* mix, const float* u, int n, float* dirs, float* pdf, hipStream_t stream);
 
 // Guiding kernels (rtg_guide.hip). Scratch for one training step (column-blocked bf16 tensors).

@@ -332,8 +332,11 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
 }
 
 // ---------------------------------------------------------------- extend: closest hit
-__device__ __forceinline__ TravStack make_stack(uint2* smem, uint2* gstack, int cap) {
+__device__ __forceinline__ TravStack make_stack(uint2* smem, uint2* gstack, int cap, uint32_t* spill,
+                                                uint32_t* ovf) {
   TravStack s;
+  s.spill = spill;
+  s.ovf = ovf;
   s.l = smem + threadIdx.x;
   s.lstride = blockDim.x;
   s.gstride = gridDim.x * blockDim.x;
@@ -463,8 +466,8 @@ __global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scene
                                                 uint2* gstack, int stack_depth, int refill) {
   extern __shared__ uint2 smem[];
   __shared__ uint32_t s_w[4];
-  const TravStack stk = make_stack(smem, gstack, stack_depth);
   uint32_t* C = P.counters;
+  const TravStack stk = make_stack(smem, gstack, stack_depth, P.count_work ? &C[C_SPILL] : nullptr, &C[C_OVF]);
   const uint32_t n = C[C_EXT];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     C[C_SHADOW] = 0;   // compact stage of this iteration fills it
@@ -1150,8 +1153,8 @@ __global__ void __launch_bounds__(128) k_shadow(DevPool P, const DevScene* scene
                                                 uint2* gstack, int stack_depth, int refill) {
   extern __shared__ uint2 smem[];
   __shared__ uint32_t s_w[4];
-  const TravStack stk = make_stack(smem, gstack, stack_depth);
   uint32_t* C = P.counters;
+  const TravStack stk = make_stack(smem, gstack, stack_depth, P.count_work ? &C[C_SPILL] : nullptr, &C[C_OVF]);
   const uint32_t n = C[C_SHADOW];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     C[C_EXT] = 0;  // regen of the next iteration fills it
@@ -1233,9 +1236,9 @@ struct TapSrc {
 
 __global__ void __launch_bounds__(128) k_trace_tap(const DevScene* scene, const float* rays, int n, int any_hit,
                                                    int32_t* prim, float* t, float* uv, float* p, float* nrm,
-                                                   uint2* gstack, int stack_depth, int refill) {
+                                                   uint32_t* ctr, uint2* gstack, int stack_depth, int refill) {
   extern __shared__ uint2 smem[];
-  const TravStack stk = make_stack(smem, gstack, stack_depth);
+  const TravStack stk = make_stack(smem, gstack, stack_depth, ctr, ctr + 1);
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;
   const uint32_t nwaves = (gridDim.x * blockDim.x) / 64u;
   TapSrc src{scene, rays, prim, t, uv, p, nrm, any_hit != 0, wave * kTraceBatch, nwaves * kTraceBatch};
@@ -1244,6 +1247,26 @@ __global__ void __launch_bounds__(128) k_trace_tap(const DevScene* scene, const 
     trace_persistent<true>(src, (uint32_t)n, stk, refill, nv, np);
   else
     trace_persistent<false>(src, (uint32_t)n, stk, refill, nv, np);
+}
+
+// Sampler tap: the 2D and 1D draws the renderer's sampler gives (pixel, sample) pairs of a job.
+__global__ void k_sampler_tap(DevJob J, const uint32_t* ps, int n, int n2, int n1, float* out2, float* out1) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const PathRng r = make_rng(J.seed, ps[2 * i], ps[2 * i + 1], J);
+  for (int k = 0; k < n2; k++) {
+    float x, y;
+    sample2d(r, (uint32_t)k, x, y);
+    out2[((size_t)i * n2 + k) * 2] = x;
+    out2[((size_t)i * n2 + k) * 2 + 1] = y;
+  }
+  for (int k = 0; k < n1; k++) out1[(size_t)i * n1 + k] = sample1d(r, (uint32_t)k);
+}
+
+void launch_sampler_tap(const DevJob& J, const uint32_t* ps, int n, int n2, int n1, float* out2, float* out1,
+                        hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_sampler_tap, dim3((n + 255) / 256), dim3(256), 0, stream, J, ps, n, n2, n1, out2, out1);
 }
 
 __global__ void k_bsdf_tap(const DevMaterial* mat, int op, const float* wi, const float* wo_or_u, int n,
@@ -1484,11 +1507,11 @@ void launch_reset_pool(const DevPool& pool, hipStream_t stream) {
 }
 
 void launch_trace_tap(const DevScene* scene_dev, const float* rays, int n, int any_hit, int32_t* prim, float* t,
-                      float* uv, float* p, float* nrm, const LaunchCfg& cfg, hipStream_t stream) {
+                      float* uv, float* p, float* nrm, uint32_t* ctr, const LaunchCfg& cfg, hipStream_t stream) {
   if (n == 0) return;
   int blocks = std::min((n + 127) / 128, cfg.trace_blocks);
   hipLaunchKernelGGL(k_trace_tap, dim3(blocks), dim3(128), stack_bytes(128), stream, scene_dev, rays, n, any_hit,
-                     prim, t, uv, p, nrm, cfg.gstack, cfg.stack, cfg.refill);
+                     prim, t, uv, p, nrm, ctr, cfg.gstack, cfg.stack, cfg.refill);
 }
 
 void launch_bsdf_tap(const DevMaterial* mat, int op, const float* wi, const float* wo_or_u, int n, float* out_dir,

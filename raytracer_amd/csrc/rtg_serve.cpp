@@ -74,6 +74,7 @@ struct rtg_server {
   // completion check, which follows a synchronisation), so no copy reads memory that changed.
   DevJob* up_jobs = nullptr;
   int32_t* up_order = nullptr;    // [0] count, [1..] job slots in claim order
+  uint32_t* pinned_ovf = nullptr; // the pool's traversal-stack overflow flag, read after each batch
   rtg_server_stats st{};
   std::thread loop;
   bool loop_on = false;           // the server thread runs the loop
@@ -203,7 +204,11 @@ void step_impl(rtg_server* s, double max_ms) {
     hip_check(hipMemcpyAsync(s->pinned_jobs, ctx->jobs_dev, sizeof(DevJob) * kMaxJobs, hipMemcpyDeviceToHost,
                              ctx->stream),
               "job readback");
+    hip_check(hipMemcpyAsync(s->pinned_ovf, ctx->pool.counters + C_OVF, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                             ctx->stream),
+              "overflow flag readback");
     hip_check(hipStreamSynchronize(ctx->stream), "server iteration");
+    if (*s->pinned_ovf) fail(RTG_ERR_HIP, "BVH traversal stack overflow (the builder's depth bound was exceeded)");
     // completions
     bool finished = false;
     {
@@ -410,6 +415,8 @@ int rtg_server_create(rtg_ctx* ctx, uint64_t hbm_budget_bytes, rtg_server** out)
               "hipHostMalloc");
     hip_check(hipHostMalloc((void**)&s->up_order, sizeof(int32_t) * (kMaxJobs + 1), hipHostMallocDefault),
               "hipHostMalloc");
+    hip_check(hipHostMalloc((void**)&s->pinned_ovf, 64, hipHostMallocDefault), "hipHostMalloc");
+    *s->pinned_ovf = 0;
     std::memset(s->pinned_jobs, 0, sizeof(DevJob) * kMaxJobs);
     ensure_pool(ctx, ctx->pool_slots);
     ctx->pool.count_work = 0;
@@ -462,6 +469,7 @@ int rtg_server_destroy(rtg_server* s) {
     if (s->pinned_jobs) (void)hipHostFree(s->pinned_jobs);
     if (s->up_jobs) (void)hipHostFree(s->up_jobs);
     if (s->up_order) (void)hipHostFree(s->up_order);
+    if (s->pinned_ovf) (void)hipHostFree(s->pinned_ovf);
   }
   ctx->server = nullptr;
   delete s;

@@ -16,7 +16,8 @@ ERRORS = {-1: "INVALID", -2: "HIP", -3: "OOM", -4: "PARSE", -5: "NOT_READY", -6:
 MAT_DIFFUSE, MAT_ROUGH_CONDUCTOR, MAT_ROUGH_DIELECTRIC, MAT_CONDUCTOR, MAT_DIELECTRIC = 0, 1, 2, 3, 4
 DIST_BECKMANN, DIST_GGX = 0, 2
 FILTER_BOX, FILTER_GAUSSIAN, FILTER_MITCHELL = 0, 1, 2
-LIGHTS_ALL, LIGHTS_UNIFORM, LIGHTS_WEIGHTED = 0, 1, 2
+# 0 = the reference's default strategy (one_light_uniform, integrator/Integrator.cpp:14)
+LIGHTS_DEFAULT, LIGHTS_UNIFORM, LIGHTS_WEIGHTED, LIGHTS_ALL = 0, 1, 2, 3
 SAMPLER_RANDOM, SAMPLER_STRATIFIED = 0, 1
 INTEGRATOR_PATH_MIS, INTEGRATOR_VOLPATH = 0, 1
 TONEMAP_CLAMP, TONEMAP_GAMMA, TONEMAP_REINHARD = 0, 1, 2
@@ -80,7 +81,7 @@ class rtg_stats(C.Structure):
         ("nodes_visited", C.c_uint64), ("prims_tested", C.c_uint64), ("shadow_nodes_visited", C.c_uint64),
         ("shadow_prims_tested", C.c_uint64), ("stage_ms", C.c_double * 8), ("stage_launches", C.c_uint64 * 8),
         ("guide_train_steps", C.c_uint64), ("guide_samples", C.c_uint64), ("guide_loss", C.c_double),
-        ("guide_train_ms", C.c_double), ("guide_queries", C.c_uint64),
+        ("guide_train_ms", C.c_double), ("guide_queries", C.c_uint64), ("stack_spills", C.c_uint64),
     ]
 
 
@@ -123,6 +124,8 @@ SIGNATURES = {
     "rtg_render": (C.c_int, [P, P, C.POINTER(rtg_render_desc), P, C.POINTER(rtg_stats)]),
     "rtg_render_device": (C.c_int, [P, P, C.POINTER(rtg_render_desc), P, C.POINTER(rtg_stats)]),
     "rtg_trace": (C.c_int, [P, P, P, C.c_int32, C.c_int32, P, P, P, P, P]),
+    "rtg_trace_spills": (C.c_int, [P, C.POINTER(C.c_uint64)]),
+    "rtg_sampler_batch": (C.c_int, [P, C.POINTER(rtg_render_desc), P, C.c_int32, C.c_int32, C.c_int32, P, P]),
     "rtg_bsdf_batch": (C.c_int, [P, C.POINTER(rtg_material), C.c_int32, P, P, C.c_int32, P, P, P, P]),
     "rtg_medium_batch": (C.c_int, [P, P, C.c_int32, P, C.c_int32, P, C.c_int32, P]),
     "rtg_tonemap": (C.c_int, [P, P, C.c_int32, C.c_int32, C.c_int32, C.c_float, P]),
@@ -253,6 +256,22 @@ class Context:
     def check(self, rc):
         if rc != RTG_OK:
             raise RtgError(rc, self.lib.rtg_last_error(self.h).decode())
+
+    def sampler_draws(self, camera, params, pixel_sample, n2, n1):
+        """Draws of the renderer's sampler (rtg_sampler_batch) for (pixel, sample) pairs ->
+        (n x n2 x 2, n x n1)."""
+        d = render_struct(camera, params)
+        ps = np.ascontiguousarray(pixel_sample, dtype=np.uint32).reshape(-1, 2)
+        n = ps.shape[0]
+        o2 = np.zeros((n, n2, 2), np.float32); o1 = np.zeros((n, n1), np.float32)
+        self.check(self.lib.rtg_sampler_batch(self.h, C.byref(d), _ptr(ps), n, n2, n1, _ptr(o2), _ptr(o1)))
+        return o2, o1
+
+    def trace_spills(self):
+        """Stack pushes of this context's last trace that spilled past the on-chip entries."""
+        v = C.c_uint64(0)
+        self.check(self.lib.rtg_trace_spills(self.h, C.byref(v)))
+        return v.value
 
     def close(self):
         if self.h:

@@ -547,8 +547,8 @@ def write_xml(sd, directory, name="scene.xml"):
     os.makedirs(directory, exist_ok=True)
     cam, rp = sd.camera, sd.render
     out = ['<?xml version="1.0" encoding="utf-8"?>', "<scene>"]
-    strat = {R.LIGHTS_UNIFORM: "one_light_uniform", R.LIGHTS_WEIGHTED: "one_light_weighted",
-             R.LIGHTS_ALL: "all_lights"}[sd.light_strategy]
+    strat = {R.LIGHTS_DEFAULT: "one_light_uniform", R.LIGHTS_UNIFORM: "one_light_uniform",
+             R.LIGHTS_WEIGHTED: "one_light_weighted", R.LIGHTS_ALL: "all_lights"}[sd.light_strategy]
     integ = "VolPathTracing" if rp.get("integrator", R.INTEGRATOR_PATH_MIS) == R.INTEGRATOR_VOLPATH else "PathTracingMIS"
     out += [f'  <integrator type="{integ}">',
             f'    <integer name="minDepth" value="{int(rp.get("min_depth", 3))}"/>',

@@ -129,7 +129,10 @@ int main(int argc, char** argv) {
   stk.lstride = 1;
   stk.g = gs.data();
   stk.gstride = 1;
-  stk.cap = cap;
+  stk.cap = argc > 5 ? atoi(argv[5]) : cap;  // a sixth argument forces a smaller stack (overflow check)
+  uint32_t spills = 0, ovf = 0;
+  stk.spill = &spills;
+  stk.ovf = &ovf;
   long mism = 0, ties = 0, anymism = 0, hits = 0;
   uint64_t nodes = 0, prims = 0;
   for (int q = 0; q < nrays; q++) {
@@ -193,8 +196,13 @@ int main(int argc, char** argv) {
          "mismatches %ld ties %ld any-hit mismatches %ld nodes/ray %.2f prims/ray %.2f\n",
          seed, ntri, nsph, nrays, nnodes, bvh.max_depth, bvh.max_stack, g_max_sp, hits, mism, ties, anymism,
          (double)nodes / (2.0 * nrays), (double)prims / (2.0 * nrays));
+  printf("spilled pushes %u overflow %u\n", spills, ovf);
+  if (argc > 5) {  // forced small stack: the overflow must be flagged
+    printf(ovf ? "OVERFLOW FLAGGED\n" : "FAIL overflow not flagged\n");
+    return ovf ? 0 : 1;
+  }
   const bool ok = g_max_sp <= bvh.max_stack && mism <= nrays / 10000 && anymism <= nrays / 10000 &&
-                  ties <= nrays / 1000 && hits > 0;
+                  ties <= nrays / 1000 && hits > 0 && ovf == 0 && (g_max_sp <= kLdsStack || spills > 0);
   printf(ok ? "OK\n" : "FAIL\n");
   return ok ? 0 : 1;
 }

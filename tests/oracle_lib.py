@@ -160,7 +160,10 @@ class OracleScene:
         if sd.env is not None:
             env = f32(sd.env)
             self.lib.orc_set_env(self.h, _p(env), env.shape[1], env.shape[0], float(sd.env_scale))
-        self.nprim = self.lib.orc_finalize(self.h, int(sd.light_strategy))
+        # the oracle numbers strategies as the reference's enum (Integrator.h:18-22: all_lights,
+        # one_light_uniform, one_light_weighted); the C ABI's 0 is the reference default (uniform)
+        strategy = {0: 1, 1: 1, 2: 2, 3: 0}[int(sd.light_strategy)]
+        self.nprim = self.lib.orc_finalize(self.h, strategy)
         med = getattr(sd, "medium", None)
         if med is not None and med.get("homogeneous"):
             sa, ss = f32(med["sigma_a"], (3,)), f32(med["sigma_s"], (3,))

@@ -234,3 +234,16 @@ def test_all_lights_matches_one_light(ctx):
     a, b = imgs[rtg.LIGHTS_ALL], imgs[rtg.LIGHTS_UNIFORM]
     print(f"[render] all_lights vs one_light_uniform, one light: max |diff| {np.abs(a - b).max():.3e}", flush=True)
     assert a.mean() > 0 and np.allclose(a, b, rtol=1e-5, atol=1e-6)
+
+
+def test_zero_light_strategy_is_the_reference_default(ctx):
+    """A scene description whose light_strategy is left at 0 (RTG_LIGHTS_DEFAULT, e.g. memset) uses
+    the reference's default one_light_uniform (integrator/Integrator.cpp:14): on the three-light
+    Cornell box the image equals the one_light_uniform image (same numbers drawn; only the film's
+    float addition order may differ) and not the all_lights one."""
+    imgs = {}
+    for strat in (rtg.LIGHTS_DEFAULT, rtg.LIGHTS_UNIFORM, rtg.LIGHTS_ALL):
+        sd = scenes.two_light_cornell(width=24, height=24, strategy=strat)
+        imgs[strat], _ = gpu_image(ctx.upload(sd), sd.camera, sd.render, spp=64, seed=22)
+    assert np.allclose(imgs[rtg.LIGHTS_DEFAULT], imgs[rtg.LIGHTS_UNIFORM], rtol=1e-5, atol=1e-6)
+    assert not np.allclose(imgs[rtg.LIGHTS_DEFAULT], imgs[rtg.LIGHTS_ALL], rtol=1e-3, atol=1e-4)

@@ -267,9 +267,142 @@ def test_dielectric_pdf_and_sample_consistency():
     assert refr.sum() > 1000
     f, p = o.bsdf_eval(0, wi[:20000][refr], wo[refr])
     assert np.allclose(p, pdfs[refr], rtol=1e-4)
-    nrm_co = None  # refraction weight = evalRefraction * |cos_o . m| / pdf: checked through eta
     assert np.allclose(eta[refr], 1.000277 / 1.5, rtol=1e-6)
     assert np.all(wo[refr, 2] < 0)
+
+
+def _fresnel_textbook(cos_i, eta_i, eta_t):
+    """Unpolarised dielectric Fresnel reflectance in float64 (1 under total internal reflection)."""
+    ci = np.abs(np.asarray(cos_i, np.float64))
+    st = eta_i / eta_t * np.sqrt(np.maximum(0.0, 1 - ci * ci))
+    ct = np.sqrt(np.maximum(0.0, 1 - st * st))
+    rs = (eta_i * ci - eta_t * ct) / (eta_i * ci + eta_t * ct)
+    rp = (eta_t * ci - eta_i * ct) / (eta_t * ci + eta_i * ct)
+    return np.where(st >= 1, 1.0, 0.5 * (rs * rs + rp * rp))
+
+
+def _dirs(rng, n):
+    d = rng.normal(size=(n, 3))
+    return (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+
+
+ETA_EXT, ETA_INT = 1.000277, 1.5
+
+
+@pytest.mark.parametrize("alpha,dist", [(0.1, 2), (0.3, 2), (0.1, 0), (0.3, 0)])
+def test_dielectric_reflection_lobe_is_reciprocal(alpha, dist):
+    """evalReflection (RoughDielectric.cpp:95-119): D(h) F(wi.h) G(wi, wo) / (4 cos_i cos_o) with h
+    the normalised wi + wo is symmetric in wi and wo on either side of the surface."""
+    o = _scene_with(scenes.dielectric(alpha, dist=dist))
+    rng = np.random.default_rng(21)
+    n = 100000
+    wi, wo = _dirs(rng, n), _dirs(rng, n)
+    wo[:, 2] = np.sign(wi[:, 2]) * np.abs(wo[:, 2])
+    f1, _ = o.bsdf_eval(0, wi, wo)
+    f2, _ = o.bsdf_eval(0, wo, wi)
+    ok = f1[:, 0] > 1e-6
+    assert ok.sum() > n // 10
+    assert np.array_equal(f1[:, 0] > 1e-30, f2[:, 0] > 1e-30)
+    assert np.allclose(f1[ok], f2[ok], rtol=2e-4)
+
+
+@pytest.mark.parametrize("alpha,dist", [(0.1, 2), (0.3, 2), (0.1, 0), (0.3, 0)])
+def test_dielectric_refraction_lobe_reciprocity_with_macro_fresnel(alpha, dist):
+    """evalRefraction (RoughDielectric.cpp:121-147) is
+        T (wi.h)(wo.h)/(cos_i cos_o) eta_i^2 (1 - F(cos_i)) G D / (eta_i wi.h + eta_o wo.h)^2
+    (etaT^2 relEta^2 = eta_i^2, the IOR on wi's side) with the Fresnel term at the *macro-surface*
+    cosine of wi. Swapping the directions leaves h, G, D and the denominator unchanged, so
+        f(wi, wo) eta_o^2 (1 - F_o) = f(wo, wi) eta_i^2 (1 - F_i),
+    with F the textbook Fresnel (float64) at each direction's macro cosine from its own side. A
+    restatement with eta_t^2, or with the Fresnel term at the microfacet, fails it; so does one
+    whose zero/non-zero pattern differs from the total internal reflection on the inner side."""
+    o = _scene_with(scenes.dielectric(alpha, dist=dist))
+    rng = np.random.default_rng(22)
+    n = 200000
+    wi, wo = _dirs(rng, n), _dirs(rng, n)
+    wo[:, 2] = -np.sign(wi[:, 2]) * np.abs(wo[:, 2])
+    f1, _ = o.bsdf_eval(0, wi, wo)
+    f2, _ = o.bsdf_eval(0, wo, wi)
+    ei = np.where(wi[:, 2] > 0, ETA_EXT, ETA_INT)
+    eo = np.where(wo[:, 2] > 0, ETA_EXT, ETA_INT)
+    Fi = _fresnel_textbook(wi[:, 2], ei, eo)
+    Fo = _fresnel_textbook(wo[:, 2], eo, ei)
+    lhs = f1[:, 0].astype(np.float64) * eo ** 2 * (1 - Fo)
+    rhs = f2[:, 0].astype(np.float64) * ei ** 2 * (1 - Fi)
+    both = (f1[:, 0] > 1e-6) & (f2[:, 0] > 1e-6)
+    assert both.sum() > 2000
+    rel = np.abs(lhs - rhs)[both] / np.maximum(lhs, rhs)[both]
+    assert np.quantile(rel, 0.999) < 2e-4 and np.median(rel) < 1e-5
+    # one side exactly zero while the other is not negligible (Beckmann tails underflow below
+    # 1e-30): only where the zero side's macro Fresnel is total internal reflection
+    one = ((f1[:, 0] == 0) & (f2[:, 0] > 1e-30)) | ((f2[:, 0] == 0) & (f1[:, 0] > 1e-30))
+    assert one.sum() > 100
+    zero_side_F = np.where(f1[:, 0] > 0, Fo, Fi)
+    assert np.all(zero_side_F[one] >= 1.0 - 1e-6)
+
+
+@pytest.mark.parametrize("alpha", [0.1, 0.3])
+@pytest.mark.parametrize("cz", [0.8, 0.5, -0.8])
+def test_dielectric_sample_weights_against_eval_and_pdf(alpha, cz):
+    """Sampling weights (RoughDielectric.cpp:194-258), checked against eval and pdf of the same
+    direction:
+    - refraction returns evalRefraction |wo.m| / pdf with m the sampled microfacet, i.e. the
+      microfacet cosine, not the macro cosine |wo.z| (m = the normalised -(eta_i wi + eta_o wo));
+    - reflection returns (wi.m) G(alpha') / (cos_i cos m), which equals evalReflection evaluated
+      with the *adjusted* roughness alpha' = (1.2 - 0.2 sqrt|cos_i|) alpha times |cos_o| over the
+      returned pdf (reflection sampling and its pdf use alpha', eval uses alpha: quirk A.4).
+    A microfacet seen from the side opposite to the macro surface (sign(wi.m) != sign(cos_i))
+    makes the sampler pick the other pair of IORs; those samples (under 0.5% entering, up to 3%
+    for a rough interface seen from inside) do not satisfy the identity."""
+    o = _scene_with(scenes.dielectric(alpha))
+    ap = np.float32((1.2 - 0.2 * np.sqrt(abs(cz))) * alpha)
+    o_adj = _scene_with(scenes.dielectric(float(ap)))
+    rng = np.random.default_rng(23)
+    n = 60000
+    wi = np.tile(np.array([[np.sqrt(1 - cz * cz), 0.0, cz]], np.float32), (n, 1))
+    wo, w, pdf, eta, typ = o.bsdf_sample(0, wi, rng.random((n, 2)))
+    live = np.any(w != 0, 1)
+    refr = live & (wo[:, 2] * cz < 0)
+    refl = live & (wo[:, 2] * cz > 0)
+    assert refr.sum() > 1000 and refl.sum() > 200
+    # refraction
+    f, p = o.bsdf_eval(0, wi[refr], wo[refr])
+    a, b = wi[refr].astype(np.float64), wo[refr].astype(np.float64)
+    ei, eo = (ETA_EXT, ETA_INT) if cz > 0 else (ETA_INT, ETA_EXT)
+    m = -(ei * a + eo * b)
+    m /= np.linalg.norm(m, axis=1, keepdims=True)
+    r_micro = w[refr, 0] / (f[:, 0] * np.abs(np.sum(b * m, 1)) / p)
+    r_macro = w[refr, 0] / (f[:, 0] * np.abs(b[:, 2]) / p)
+    good = np.abs(r_micro - 1) < 1e-4
+    assert good.mean() > (0.995 if cz > 0 else 0.96) and np.median(np.abs(r_micro - 1)) < 1e-6
+    assert np.median(np.abs(r_macro - 1)) > 1e-2
+    # reflection: the closed-form weight equals the alpha' eval over the pdf
+    f2, _ = o_adj.bsdf_eval(0, wi[refl], wo[refl])
+    r_adj = w[refl, 0] / (f2[:, 0] * np.abs(wo[refl, 2]) / pdf[refl])
+    assert np.quantile(np.abs(r_adj - 1), 0.999) < 2e-4
+    f3, _ = o.bsdf_eval(0, wi[refl], wo[refl])
+    r_raw = w[refl, 0] / (f3[:, 0] * np.abs(wo[refl, 2]) / pdf[refl])
+    assert np.median(np.abs(r_raw - 1)) > 1e-2
+
+
+@pytest.mark.parametrize("cz", [0.95, 0.7, -0.95, -0.8])
+def test_dielectric_near_specular_limit_is_fresnel(cz):
+    """At alpha = 0.01 the rough dielectric is a smooth interface: the mean reflection weight is the
+    textbook Fresnel reflectance at the incident angle, and the mean refraction weight, with the
+    radiance scaling (eta_t / eta_i)^2 undone, is the transmittance 1 - F (energy conservation of
+    the limit). A missing or inverted eta^2 factor is off by 2.25x."""
+    o = _scene_with(scenes.dielectric(0.01))
+    rng = np.random.default_rng(24)
+    n = 200000
+    wi = np.tile(np.array([[np.sqrt(1 - cz * cz), 0.0, cz]], np.float32), (n, 1))
+    wo, w, _, _, _ = o.bsdf_sample(0, wi, rng.random((n, 2)))
+    refr = wo[:, 2] * cz < 0
+    ei, et = (ETA_EXT, ETA_INT) if cz > 0 else (ETA_INT, ETA_EXT)
+    F = float(_fresnel_textbook(cz, ei, et))
+    R = w[~refr, 0].astype(np.float64).sum() / n
+    T = w[refr, 0].astype(np.float64).sum() / n * (et / ei) ** 2
+    assert abs(R - F) < 0.04 * F + 1e-3, (R, F)
+    assert abs(T - (1 - F)) < 0.015 * (1 - F), (T, 1 - F)
 
 
 # ---------------------------------------------------------------- geometry
@@ -407,6 +540,31 @@ def test_env_le_texel_exact_and_sampling_marginal():
     assert np.allclose(got, exp, atol=0.01)
 
 
+@pytest.mark.parametrize("H,W,scale", [(8, 16, 1.0), (16, 32, 2.5)])
+def test_env_pdf_texel_unit_sine_on_a_constant_map(H, W, scale):
+    """EnvironmentLight::pdf (EnvironmentLight.cpp:148-191) on a constant map has the closed form
+        pdf(theta) = H w_i / (2 pi^2 sum_j w_j) / max(|sin(theta H / pi)|, 1e-4),
+    w_j = sin(pi (j + 0.5) / H), i = floor(theta H / pi): the sine is taken of the texel-unit
+    coordinate (quirk A.5), not of theta in radians, for any value and scale of the map."""
+    sd = scenes.SceneDesc(materials=[scenes.diffuse((0.5, 0.5, 0.5))])
+    sd.env = np.full((H, W, 3), 0.7, np.float32)
+    sd.env_scale = scale
+    o = O.OracleScene(sd)
+    theta = np.array([0.13, 0.3, 0.71, 1.1, 1.5, 2.2, 2.9])
+    phi = np.array([0.2, 1.0, 2.5, 3.3, 4.0, 5.1, 6.0])
+    d = np.stack([np.sin(phi) * np.sin(theta), np.cos(theta), -np.cos(phi) * np.sin(theta)], 1)
+    q = np.zeros((len(theta), 12), np.float32)
+    q[:, 9:12] = d
+    got = o.light_pdf(0, q).astype(np.float64)
+    w = np.sin(np.pi * (np.arange(H) + 0.5) / H)
+    tt = theta * H / np.pi
+    i = np.floor(tt).astype(int)
+    expect = H * w[i] / (2 * np.pi ** 2 * w.sum()) / np.maximum(np.abs(np.sin(tt)), 1e-4)
+    assert np.allclose(got, expect, rtol=2e-4), (got, expect)
+    radians = H * w[i] / (2 * np.pi ** 2 * w.sum()) / np.sin(theta)
+    assert np.max(np.abs(got / radians - 1)) > 0.5  # a radians version is far off
+
+
 # ---------------------------------------------------------------- integrator (integrator/PathTracingMIS.cpp)
 def test_emitter_seen_directly_is_exact():
     sd = scenes.cornell()
@@ -455,6 +613,57 @@ def test_direct_light_on_plane_matches_quadrature():
     expected = nee + bsdf
     se = vals.std() / np.sqrt(n)
     assert abs(vals.mean() - expected) < 4 * se + 1e-4, (vals.mean(), expected, se)
+
+
+def test_bsdf_hit_mis_weight_omits_light_selection_probability():
+    """Two emitters under the one_light_uniform strategy (selection probability 1/2 each), one
+    bounce on a Lambertian floor. NEE weights use the light pdf *with* the selection probability
+    (Integrator.cpp:197) and the swapped BSDF pdf cos(view)/pi (quirk A.1); the BSDF-sampled hit
+    on an emitter weighs the BSDF pdf against that light's pdf *without* the selection probability
+    (PathTracingMIS.cpp:205, quirk A.2). The estimator's mean equals the quadrature of
+        sum_k int Le_k f cos [ (p_k/2)^2 / ((p_k/2)^2 + p_rev^2) + p_B^2 / (p_B^2 + p_k^2) ] dw
+    and not the one with (p_k/2)^2 in the BSDF weight (measured 15 standard errors away)."""
+    rho, h = 0.6, 0.6
+    lights = [((-0.35, 0.1), 0.5, 4.0), ((0.4, -0.1), 0.3, 9.0)]
+    sd = scenes.SceneDesc()
+    fl = sd.add_material(**scenes.diffuse((rho, rho, rho)))
+    blk = sd.add_material(**scenes.diffuse((0, 0, 0)))
+    sd.meshes.append(scenes.MeshDesc(*scenes.quad((-5, 0, -5), (10, 0, 0), (0, 0, 10), (0, 1, 0)), material=fl))
+    for (cx, cz), a, Le in lights:
+        lv, li = scenes.quad((cx - a / 2, h, cz - a / 2), (a, 0, 0), (0, 0, a), (0, -1, 0))
+        sd.meshes.append(scenes.MeshDesc(lv, li, material=blk, emission=(Le, Le, Le)))
+    o = O.OracleScene(sd)
+    assert np.allclose(o.light_cdf(2), [0.0, 0.5, 1.0])
+    view = np.array([0.3, -1.0, 0.2]); view /= np.linalg.norm(view)
+    p = np.array([0.05, 0.0, -0.02])
+    ray = np.array([*(p - 0.3 * view), *view, 1e-4, 1e8], np.float32)
+    p_rev = -view[1] / np.pi
+    f = rho / np.pi
+
+    def quadrature(selection_in_bsdf_weight):
+        tot = 0.0
+        for (cx, cz), a, Le in lights:
+            g = 400
+            xs = (np.arange(g) + 0.5) / g * a - a / 2
+            X, Z = np.meshgrid(xs + cx, xs + cz)
+            q = np.stack([X.ravel(), np.full(X.size, h), Z.ravel()], 1) - p
+            d = np.linalg.norm(q, axis=1)
+            cs = q[:, 1] / d  # cosine at the floor = cosine at the light (parallel planes)
+            dw = cs * (a / g) ** 2 / d ** 2
+            pk = d ** 2 / (a * a * cs)
+            pB = cs / np.pi
+            pn = 0.5 * pk
+            pl = pn if selection_in_bsdf_weight else pk
+            tot += np.sum(Le * f * cs * (pn ** 2 / (pn ** 2 + p_rev ** 2) + pB ** 2 / (pB ** 2 + pl ** 2)) * dw)
+        return tot
+
+    rng = np.random.default_rng(16)
+    n = 80000
+    vals = np.array([o.li(ray, rng.random((4, 2)), rng.random(2), 3, 1)[0] for _ in range(n)], np.float64)
+    se = vals.std() / np.sqrt(n)
+    quirk, alt = quadrature(False), quadrature(True)
+    assert abs(vals.mean() - quirk) < 4 * se, (vals.mean(), quirk, se)
+    assert abs(vals.mean() - alt) > 8 * se, (vals.mean(), alt, se)
 
 
 # ---------------------------------------------------------------- smooth (delta) BSDFs

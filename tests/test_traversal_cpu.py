@@ -38,3 +38,29 @@ def test_traversal_matches_brute_force(trav_check, seed, ntri, nsph, nrays):
                        timeout=300)
     print(r.stdout)
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), (r.stdout[-2000:], r.stderr[-3000:])
+
+
+def test_stack_overflow_is_flagged(trav_check):
+    """A stack smaller than what the rays need (forced to the 16 on-chip entries on a 40k-triangle
+    scene whose rays reach depth 21) makes pushes find it full: the traversal raises its overflow
+    flag (which the library turns into a render error) instead of dropping subtrees silently."""
+    r = subprocess.run([trav_check, "5", "40000", "0", "6000", "16"], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0 and "OVERFLOW FLAGGED" in r.stdout, (r.stdout[-2000:], r.stderr[-3000:])
+
+
+def test_sampler_strata_arithmetic():
+    """The device sampler's stratum arithmetic and permutation (rtg_device.h in_stratum, permute),
+    compiled for the host: every stratified value stays inside its cell in exact arithmetic (also
+    for jitters within an ulp of 1, where the float sum cell + jitter rounds up to the next cell),
+    and the permutation is a bijection, for sample counts up to 4225."""
+    exe = os.path.join(BUILD, "sampler_check")
+    os.makedirs(BUILD, exist_ok=True)
+    cmd = ["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-include", os.path.join(HERE, "cpu_trav", "device_cpu.h"),
+           "-I" + os.path.join(HERE, "mock_hip"), "-I" + CSRC, os.path.join(HERE, "cpu_trav", "sampler_check.cpp"),
+           "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-2000:]
