@@ -262,6 +262,7 @@ typedef struct rtg_scene_info {
   int32_t textures;            /* image textures */
   int32_t medium_kind;         /* 0 none, 1 heterogeneous grid, 2 homogeneous */
   float medium_sigma_a[3], medium_sigma_s[3];  /* homogeneous, after the scale */
+  int32_t traversal_stack;     /* deepest traversal stack a ray can need (the BVH builder's bound) */
 } rtg_scene_info;
 int rtg_scene_query(rtg_scene* scene, rtg_scene_info* info);
 

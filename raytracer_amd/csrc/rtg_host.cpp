@@ -969,6 +969,7 @@ int rtg_scene_query(rtg_scene* scene, rtg_scene_info* info) {
     info->bbox_max[k] = scene->dev.root_max[k];
   }
   info->textures = scene->ntex;
+  info->traversal_stack = scene->stack;
   info->medium_kind = scene->dev.medium_kind;
   for (int k = 0; k < 3; k++) {
     info->medium_sigma_a[k] = scene->dev.sig_a[k];

@@ -92,6 +92,7 @@ class rtg_scene_info(C.Structure):
         ("medium_res", C.c_int32 * 3), ("medium_albedo", C.c_float * 3), ("medium_g", C.c_float),
         ("medium_scale", C.c_float), ("medium_sigma_max", C.c_float), ("textures", C.c_int32),
         ("medium_kind", C.c_int32), ("medium_sigma_a", C.c_float * 3), ("medium_sigma_s", C.c_float * 3),
+        ("traversal_stack", C.c_int32),
     ]
 
 

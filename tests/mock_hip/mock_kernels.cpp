@@ -40,12 +40,22 @@ void launch_iteration_mixed(const DevPool& P, const DevScene* S, DevJob* jobs, c
 }
 void launch_medium_tap(const DevScene*, int, const float*, int, const float*, int, float*, hipStream_t) {}
 void launch_tonemap(const float4*, int, int, float, unsigned int*, uint8_t*, hipStream_t) {}
-void launch_reset_pool(const DevPool&, hipStream_t) {}
+// as k_reset_pool: every slot free, every counter zero
+void launch_reset_pool(const DevPool& P, hipStream_t) {
+  for (int i = 0; i < P.capacity; i++) P.state[i] = ST_FREE;
+  memset(P.counters, 0, C_COUNT * sizeof(uint32_t));
+}
 void launch_trace_tap(const DevScene*, const float*, int, int, int32_t*, float*, float*, float*, float*, uint32_t*,
                       const LaunchCfg&, hipStream_t) {}
 void launch_bsdf_tap(const DevMaterial*, int, const float*, const float*, int, float*, float*, float*, float*,
                      hipStream_t) {}
 void launch_fill_zero(void* p, size_t b, hipStream_t) { memset(p, 0, b); }
+void* scratch_address(int) {
+  static unsigned char buf[256];
+  return buf;
+}
+void launch_trace_tap_noalloc(const DevScene*, const float*, int, int, int32_t*, float*, float*, float*, float*,
+                              void*, int, int, hipStream_t) {}
 void launch_sampler_tap(const DevJob&, const uint32_t*, int, int, int, float*, float*, hipStream_t) {}
 // guiding kernels are not exercised by the host-logic test
 void launch_guide_refresh(const float*, uint16_t*, uint16_t*, float*, hipStream_t) {}
