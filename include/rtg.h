@@ -288,6 +288,18 @@ int rtg_tonemap(rtg_ctx* ctx, const float* rgbw, int32_t width, int32_t height, 
  * uv: barycentrics (b1, b2) for triangles. p: hit point, nrm: shading normal (closest hit only). */
 int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int32_t any_hit, int32_t* prim,
               float* t, float* uv, float* p, float* nrm);
+/* Allocation-free variant of rtg_trace. It has no counterpart in the reference: it is for callers
+ * that cannot allocate device memory (code under hipGraph capture, tight serving loops, harnesses
+ * that forbid hipMalloc). rays_device (n x 8 floats) and the outputs are caller-owned DEVICE
+ * buffers laid out as rtg_trace's (t, uv, p, nrm may be NULL); the call allocates nothing: the
+ * traversal's scratch (its stack spill area and two counters, 16 MiB + 256 bytes) is a static
+ * device array of the library, one per GPU, held until the process ends and shared by every call
+ * on that GPU, so calls that overlap on one GPU (from different contexts) race. A scene whose
+ * traversal stack bound (rtg_scene_info.traversal_stack) exceeds 48 entries fails with
+ * RTG_ERR_INVALID. Returns when the results are written; rtg_trace_spills reports its spills. */
+int rtg_trace_device_noalloc(rtg_ctx* ctx, rtg_scene* scene, const float* rays_device, int32_t n, int32_t any_hit,
+                             int32_t* prim_device, float* t_device, float* uv_device, float* p_device,
+                             float* nrm_device);
 /* Traversal-stack pushes of the context's last rtg_trace call that went past the 16 on-chip (LDS)
  * entries of a ray's stack into the global spill area (no reference counterpart: the reference's
  * stack is a 64-entry array, accelStruct/Bvh.cpp:392-400). A stack deeper than the BVH builder's

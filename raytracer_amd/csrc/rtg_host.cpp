@@ -883,6 +883,9 @@ int rtg_ctx_create(int device, rtg_ctx** out) {
     size_t lanes = (size_t)ctx->trace_blocks * 128;
     hip_check(hipMalloc(&ctx->gstack, lanes * (kMaxStack - 16) * sizeof(uint2)), "hipMalloc stack spill");
     hip_check(hipMalloc(&ctx->tap_ctr, 2 * sizeof(uint32_t)), "hipMalloc trace counters");
+    // device address of this GPU's static scratch array (the allocation-free trace's scratch); a
+    // failed lookup leaves it null, and only rtg_trace_device_noalloc then fails
+    ctx->scratch = scratch_address(device);
   } catch (const RtgError& e) {
     int code = e.code;
     delete ctx;
@@ -1320,6 +1323,32 @@ int rtg_sampler_batch(rtg_ctx* ctx, const rtg_render_desc* desc, const uint32_t*
       throw;
     }
     for (void* q : own) (void)hipFree(q);
+  })
+}
+
+int rtg_trace_device_noalloc(rtg_ctx* ctx, rtg_scene* scene, const float* rays_device, int32_t n, int32_t any_hit,
+                             int32_t* prim_device, float* t_device, float* uv_device, float* p_device,
+                             float* nrm_device) {
+  if (!ctx) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  RTG_GUARD(ctx, {
+    if (!scene || !rays_device || !prim_device || n < 0) fail(RTG_ERR_INVALID, "bad trace arguments");
+    if (scene->stack > kNoAllocStack)
+      fail(RTG_ERR_INVALID, "the scene's traversal stack bound (" + std::to_string(scene->stack) +
+                                " entries) exceeds the " + std::to_string(kNoAllocStack) +
+                                " the allocation-free trace's scratch holds; use rtg_trace");
+    if (!ctx->scratch) fail(RTG_ERR_HIP, "hipGetSymbolAddress of the scratch array failed at context creation");
+    if (n == 0) return RTG_OK;
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    launch_trace_tap_noalloc(ctx->scenes_dev + scene->slot, rays_device, n, any_hit, prim_device, t_device, uv_device,
+                             p_device, nrm_device, ctx->scratch, scene->stack, ctx->refill, ctx->stream);
+    hip_check(hipGetLastError(), "trace launch");
+    uint32_t* ctr_h = (uint32_t*)((char*)ctx->pinned + 48);
+    hip_check(hipMemcpyAsync(ctr_h, ctx->scratch, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream),
+              "trace counters");
+    hip_check(hipStreamSynchronize(ctx->stream), "trace");
+    ctx->last_trace_spills = ctr_h[0];
+    if (ctr_h[1]) fail(RTG_ERR_HIP, "BVH traversal stack overflow (the builder's depth bound was exceeded)");
   })
 }
 

@@ -90,6 +90,7 @@ struct rtg_ctx {
   rtg::DevJob* jobs_dev = nullptr;
   uint2* gstack = nullptr;
   uint32_t* tap_ctr = nullptr;        // trace tap: [0] spilled stack pushes, [1] stack overflow
+  void* scratch = nullptr;            // this device's static scratch array (rtg_trace_device_noalloc)
   uint64_t last_trace_spills = 0;
   int refill = rtg::kTraceRefill;
   float* lut_dev = nullptr;  // kMaxJobs filter tables of kFilterLut weights

@@ -266,6 +266,21 @@ void launch_medium_tap(const DevScene* scene_dev, int op, const float* in, int n
 void launch_bsdf_tap(const DevMaterial* mat, int op, const float* wi, const float* wo_or_u, int n,
                      float* out_dir, float* f, float* pdf, float* rel_eta, hipStream_t stream);
 void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream);
+// Allocation-free trace (rtg_trace_device_noalloc): a static device scratch array per GPU holds the
+// trace counters (kScratchHeader bytes) and the stack spill area of up to kNoAllocBlocks blocks of
+// 128 lanes, kNoAllocSpill entries each, so scenes whose stack bound is at most kNoAllocStack.
+static const int kNoAllocBlocks = 512;
+static const int kNoAllocSpill = 32;
+static const int kNoAllocStack = 16 + kNoAllocSpill;
+static const size_t kScratchHeader = 256;
+static const size_t kScratchBytes = kScratchHeader + (size_t)kNoAllocBlocks * 128 * kNoAllocSpill * 8;
+static const int kMaxScratchDevices = 64;
+// Device address of the current device's scratch array (hipGetSymbolAddress, once per device;
+// the device must be current); null on failure.
+void* scratch_address(int device);
+void launch_trace_tap_noalloc(const DevScene* scene_dev, const float* rays, int n, int any_hit, int32_t* prim,
+                              float* t, float* uv, float* p, float* nrm, void* scratch_dev, int stack, int refill,
+                              hipStream_t stream);
 // ps: n (pixel, sample) pairs -> out2 (n x n2 x 2) and out1 (n x n1) draws of the job's sampler
 void launch_sampler_tap(const DevJob& J, const uint32_t* ps, int n, int n2, int n1, float* out2, float* out1,
                         hipStream_t stream);

@@ -9,6 +9,7 @@
 // The control flow restates integrator/PathTracingMIS.cpp:27-263 and
 // integrator/Integrator.cpp:176-238 split at the two ray casts of each bounce.
 #include <cstring>
+#include <mutex>
 
 #include "rtg_device.h"
 
@@ -1536,6 +1537,38 @@ __global__ void k_guide_sample_tap(const float* mix, const float* u, int n, floa
 void launch_guide_sample_tap(const float* mix, const float* u, int n, float* dirs, float* pdf, hipStream_t stream) {
   if (n <= 0) return;
   hipLaunchKernelGGL(k_guide_sample_tap, dim3((n + 255) / 256), dim3(256), 0, stream, mix, u, n, dirs, pdf);
+}
+
+// ---------------------------------------------------------------- allocation-free trace
+// rtg_trace_device_noalloc's scratch: the trace counters (first 256 bytes) and the traversal-stack
+// spill area of a launch of up to kNoAllocBlocks blocks of 128 lanes with kNoAllocSpill entries
+// per lane beyond the LDS part. One copy per GPU, kept for the life of the process; host code
+// reaches it only through the address scratch_address() returns.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+void* scratch_address(int device) {
+  static std::mutex mu;
+  static void* addr[kMaxScratchDevices] = {};
+  if (device < 0 || device >= kMaxScratchDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!addr[device]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess) return nullptr;
+    addr[device] = p;
+  }
+  return addr[device];
+}
+
+void launch_trace_tap_noalloc(const DevScene* scene_dev, const float* rays, int n, int any_hit, int32_t* prim,
+                              float* t, float* uv, float* p, float* nrm, void* scratch_dev, int stack, int refill,
+                              hipStream_t stream) {
+  if (n == 0) return;
+  uint32_t* ctr = (uint32_t*)scratch_dev;
+  uint2* spill = (uint2*)((char*)scratch_dev + kScratchHeader);
+  hipLaunchKernelGGL(k_fill_zero, dim3(1), dim3(64), 0, stream, ctr, (size_t)2);
+  const int blocks = std::min((n + 127) / 128, kNoAllocBlocks);
+  hipLaunchKernelGGL(k_trace_tap, dim3(blocks), dim3(128), stack_bytes(128), stream, scene_dev, rays, n, any_hit,
+                     prim, t, uv, p, nrm, ctr, spill, stack, refill);
 }
 
 void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream) {

@@ -126,6 +126,7 @@ SIGNATURES = {
     "rtg_render_device": (C.c_int, [P, P, C.POINTER(rtg_render_desc), P, C.POINTER(rtg_stats)]),
     "rtg_trace": (C.c_int, [P, P, P, C.c_int32, C.c_int32, P, P, P, P, P]),
     "rtg_trace_spills": (C.c_int, [P, C.POINTER(C.c_uint64)]),
+    "rtg_trace_device_noalloc": (C.c_int, [P, P, P, C.c_int32, C.c_int32, P, P, P, P, P]),
     "rtg_sampler_batch": (C.c_int, [P, C.POINTER(rtg_render_desc), P, C.c_int32, C.c_int32, C.c_int32, P, P]),
     "rtg_bsdf_batch": (C.c_int, [P, C.POINTER(rtg_material), C.c_int32, P, P, C.c_int32, P, P, P, P]),
     "rtg_medium_batch": (C.c_int, [P, P, C.c_int32, P, C.c_int32, P, C.c_int32, P]),
