@@ -265,6 +265,9 @@ typedef struct rtg_scene_info {
   int32_t traversal_stack;     /* deepest traversal stack a ray can need (the BVH builder's bound) */
 } rtg_scene_info;
 int rtg_scene_query(rtg_scene* scene, rtg_scene_info* info);
+/* Material `index` of a resident scene as uploaded (for a scene file: as the loader built it, named
+ * metals converted to eta/k). No reference counterpart (introspection for loaders and tests). */
+int rtg_scene_material(rtg_scene* scene, int32_t index, rtg_material* out);
 
 /* Blocking render. out_rgbw (width*height*4 floats, host) receives per-pixel
  * sums of radiance and the filter weight (box filter r=0.5); divide rgb by w

@@ -2024,6 +2024,13 @@ struct Camera {
         return r;
     }
 };
+// camera/NoriPerspective.cpp:10-37 takes the XML's toWorld as camera-to-world: cam[12] = 1 with
+// cam[13..28] (row-major) replaces the lookAt frame (a toWorld such as a lookAt with a mirroring scale)
+static void cameraMatrix(Camera& c, const float* cam) {
+    if (cam[12] != 1.f) return;
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) c.camToWorld.m[i][j] = cam[13 + 4 * i + j];
+}
 // Transform.cpp:199-227 (fromLookAt) and NoriPerspective.cpp:10-37, in double, rounded to float
 static void buildCamera(Camera& cam, const float* origin, const float* target, const float* up, float fov,
                         int W, int H, float nearC, float farC) {
@@ -2544,6 +2551,7 @@ void orc_cdf(const float* w, int nw, const float* u, int n, int* idx, float* ur,
 void orc_camera_rays(const float* cam, int W, int H, const float* pxy, int n, float* rays, float* mats) {
     Camera c;
     buildCamera(c, cam, cam + 3, cam + 6, cam[9], W, H, cam[10], cam[11]);
+    cameraMatrix(c, cam);
     for (int i = 0; i < n; i++) {
         Ray r = c.getRay(pxy[2 * i], pxy[2 * i + 1]);
         float* o = rays + 8 * i;
@@ -2777,6 +2785,7 @@ int orc_render(void* sp, const float* cam, int W, int H, int spp, int samplerTyp
     Scene* s = (Scene*)sp;
     Camera c;
     buildCamera(c, cam, cam + 3, cam + 6, cam[9], W, H, cam[10], cam[11]);
+    cameraMatrix(c, cam);
     RenderParams rp;
     rp.spp = spp;
     rp.samplerType = samplerType;

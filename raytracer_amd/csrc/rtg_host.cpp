@@ -587,6 +587,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   sc->nsph = nsph;
   sc->ntex = (int)texs.size();
   sc->all_lights = strategy == RTG_LIGHTS_ALL;
+  sc->mats.assign(desc->materials, desc->materials + desc->num_materials);
   ctx->scene_slots[slot] = sc.get();
   return sc.release();
 }
@@ -991,6 +992,16 @@ int rtg_scene_query(rtg_scene* scene, rtg_scene_info* info) {
     info->medium_scale = scene->dev.gscale;
     info->medium_sigma_max = scene->dev.sigma_max;
   }
+  return RTG_OK;
+}
+
+int rtg_scene_material(rtg_scene* scene, int32_t index, rtg_material* out) {
+  if (!scene || !out) return RTG_ERR_INVALID;
+  if (index < 0 || index >= (int32_t)scene->mats.size()) {
+    set_error("material index out of range");
+    return RTG_ERR_INVALID;
+  }
+  *out = scene->mats[(size_t)index];
   return RTG_OK;
 }
 

@@ -57,6 +57,7 @@ struct rtg_scene {
   int ntri = 0, nsph = 0, ntex = 0;
   bool all_lights = false;  // RTG_LIGHTS_ALL
   std::atomic<int> users{0};  // requests of the serving loop that reference this scene
+  std::vector<rtg_material> mats;  // the materials as uploaded (rtg_scene_material)
 };
 
 namespace rtg {

@@ -122,6 +122,7 @@ SIGNATURES = {
     "rtg_scene_release": (C.c_int, [P, P]),
     "rtg_scene_bytes": (C.c_int, [P, C.POINTER(C.c_uint64)]),
     "rtg_scene_query": (C.c_int, [P, C.POINTER(rtg_scene_info)]),
+    "rtg_scene_material": (C.c_int, [P, C.c_int32, C.POINTER(rtg_material)]),
     "rtg_render": (C.c_int, [P, P, C.POINTER(rtg_render_desc), P, C.POINTER(rtg_stats)]),
     "rtg_render_device": (C.c_int, [P, P, C.POINTER(rtg_render_desc), P, C.POINTER(rtg_stats)]),
     "rtg_trace": (C.c_int, [P, P, P, C.c_int32, C.c_int32, P, P, P, P, P]),
@@ -470,6 +471,13 @@ class Scene:
         if self.h:
             self.ctx.check(self.ctx.lib.rtg_scene_release(self.ctx.h, self.h))
             self.h = None
+
+    def material(self, index):
+        """Material `index` as uploaded (rtg_scene_material) -> dict of its fields."""
+        m = rtg_material()
+        self.ctx.check(self.ctx.lib.rtg_scene_material(self.h, int(index), C.byref(m)))
+        return {k: (list(getattr(m, k)) if k in ("albedo", "eta", "k", "transmitted") else getattr(m, k))
+                for k, _ in m._fields_}
 
     def info(self):
         i = rtg_scene_info()

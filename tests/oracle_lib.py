@@ -266,8 +266,7 @@ class OracleScene:
         cam = dict(sd.camera if camera is None else camera)
         rp = sd.render
         W, H = int(cam["width"]), int(cam["height"])
-        cv = f32([*cam["origin"], *cam["target"], *cam["up"], cam["fov"], cam.get("near", 1e-4),
-                  cam.get("far", 1e4)])
+        cv = camera_vector(cam)
         rgb = np.zeros((H, W, 3), np.float64); wgt = np.zeros((H, W), np.float64)
         stats = np.zeros(2, np.uint64)
         nblocks = self.lib.orc_render(self.h, _p(cv), W, H, int(spp or rp["spp"]), int(sampler),
@@ -278,10 +277,19 @@ class OracleScene:
         return rgb, wgt, stats, nblocks
 
 
+def camera_vector(cam):
+    """origin, target, up, fov, near, far, then a flag and the row-major camera-to-world matrix when
+    the camera carries one ("to_world": a scene file's toWorld, e.g. a lookAt with a mirroring scale)."""
+    v = [*cam["origin"], *cam["target"], *cam["up"], cam["fov"], cam.get("near", 1e-4), cam.get("far", 1e4)]
+    M = cam.get("to_world")
+    v += [1.0, *np.asarray(M, np.float64).reshape(16)] if M is not None else [0.0] * 17
+    return f32(v)
+
+
 def camera_rays(camera, pxy):
     lib = load()
     cam = camera
-    cv = f32([*cam["origin"], *cam["target"], *cam["up"], cam["fov"], cam.get("near", 1e-4), cam.get("far", 1e4)])
+    cv = camera_vector(cam)
     pxy = f32(pxy, (-1, 2)); n = pxy.shape[0]
     rays = np.zeros((n, 8), np.float32); mats = np.zeros(32, np.float32)
     lib.orc_camera_rays(_p(cv), int(cam["width"]), int(cam["height"]), _p(pxy), n, _p(rays), _p(mats))

@@ -121,3 +121,37 @@ def test_xml_named_metal_matches_explicit_colors(ctx, tmp_path):
     a, b = _render(a_sc, cam, rp), _render(b_sc, cam, rp)
     print(f"  mean {a.mean():.5f}, max |diff| {np.abs(a - b).max():.3e}", flush=True)
     assert a.mean() > 0 and np.all(np.abs(a - b) <= 1e-5 * np.maximum(1.0, b))
+
+
+def test_reference_cbox_fog_file_renders_like_the_oracle(ctx, tmp_path):
+    """The reference's own cbox_fog.xml, loaded verbatim (stand-in meshes beside it, see
+    tests/reference_standins.py): its VolPathTracing render in the homogeneous fog, with its random
+    sampler, Mitchell filter and mirrored camera, at 64x64 and 4096 spp, sits within the error bars
+    of 8 oracle renders of the scene the file describes (block means, 8x8 blocks)."""
+    import oracle_lib as O
+    import reference_standins as RS
+
+    sc, cam, rp = ctx.load_xml(RS.setup_cbox_fog(str(tmp_path)))
+    cam = dict(cam, width=64, height=64)
+    ref = RS.cbox_fog_scene(width=64, height=64, min_depth=rp["min_depth"], max_depth=rp["max_depth"])
+    o = O.OracleScene(ref)
+    print(f"[xml] cbox_fog.xml (reference file): depth {rp['min_depth']}..{rp['max_depth']}, "
+          f"cpu 8 x 64 spp, gpu 4096 spp", flush=True)
+
+    def blocks(img):
+        return img.reshape(8, 8, 8, 8, 3).mean(axis=(1, 3))
+
+    cpu = []
+    for s in range(8):
+        rgb, w, _, _ = o.render(camera=ref.camera, spp=64, threads=min(16, os.cpu_count()), sampler=0, seed_offset=s)
+        cpu.append(blocks(rgb / np.maximum(w[..., None], 1e-30)))
+    cb = np.stack(cpu)
+    out, st = sc.render(cam, dict(rp, spp=4096, seed=17))
+    G = rtg.resolve(out).astype(np.float64)
+    m = cb.mean(0)
+    se = cb.std(0, ddof=1) / np.sqrt(8) + 1e-4 * (np.abs(m) + 1e-3)
+    z = (blocks(G) - m) / se
+    print(f"  max|z| {np.abs(z).max():.2f} mean z^2 {np.mean(z * z):.2f} mean gpu {G.mean():.5f} cpu {m.mean():.5f}",
+          flush=True)
+    assert G.mean() > 0 and st.nan_samples == 0
+    assert np.abs(z).max() < 5.0 and np.mean(z * z) < 3.0
