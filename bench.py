@@ -193,17 +193,22 @@ GUIDE_FLOPS_PER_QUERY = 2 * (144 * 64 + 64 * 64 + 64 * 64 + 64 * 40)  # network 
 MFMA_BF16_PEAK_TFLOPS = 2500.0
 
 
-def guiding_leg(ctx, scene, size, ref_spp, target=1e-3, ladder=(64, 256, 1024, 4096, 8192)):
+GLOSSY_REFERENCE = os.path.join(ROOT, "tests", "golden", "glossy_cornell_ref.npz")
+
+
+def guiding_leg(ctx, scene, size, ref_spp, target=1e-3, ladder=(64, 256, 1024, 4096, 8192), until_target=False):
     """Plain vs guided on one scene, both on the GPU: error and time at a ladder of spp counts.
 
-    The judge of the error is the plain tracer's converged image: ref_spp samples rendered as eight
-    independent pieces whose spread estimates the reference's own per-pixel noise, which is
-    subtracted from every squared error so that it does not count against either method. Each
-    render starts from scratch (a guided render initialises and trains its network online inside
-    the timed call). Reports relMSE and seconds per ladder point, the guided/plain error ratio and
-    error x time ratio at 1024 spp (the configuration's sample budget), the time to relMSE <=
-    target when the ladder reaches it (interpolated log-log), and the network's device time and
-    MFMA rate."""
+    The judge of the error is the plain tracer's converged image: for the glossy-indirect scene at
+    512x512 the committed reference (tests/golden/glossy_cornell_ref.npz: 2^20 spp as 64 independent
+    pieces, FP64 accumulation), otherwise ref_spp samples rendered here as eight pieces; the spread of
+    the pieces estimates the reference's own per-pixel noise, which is subtracted from every squared
+    error so that it does not count against either method. Each render starts from scratch (a guided
+    render initialises and trains its network online inside the timed call). until_target: each
+    method's ladder stops at its first point at or below the target. Reports relMSE and seconds per
+    ladder point, the guided/plain error ratio and error x time ratio at 1024 spp (the
+    configuration's sample budget), the time to relMSE <= target (interpolated log-log between the
+    ladder points around the crossing), their ratio, and the network's device time and MFMA rate."""
     import numpy as np
     from raytracer_amd import rtg, scenes
 
@@ -211,14 +216,21 @@ def guiding_leg(ctx, scene, size, ref_spp, target=1e-3, ladder=(64, 256, 1024, 4
     g = ctx.upload(sd)
     cam = sd.camera
     t0 = time.perf_counter()
-    pieces = []
-    for s in range(8):
-        h, _ = g.render(cam, dict(sd.render, spp=ref_spp // 8, seed=100 + s))
-        pieces.append(rtg.resolve(h).astype(np.float64))
+    if scene == "glossy_cornell" and size == 512 and os.path.exists(GLOSSY_REFERENCE):
+        z = np.load(GLOSSY_REFERENCE)
+        ref = z["mean"].astype(np.float64)
+        noise = z["var_mean"].astype(np.float64)
+        ref_desc = f"committed, {int(z['pieces'])} x {int(z['spp'])} spp"
+    else:
+        pieces = []
+        for s in range(8):
+            h, _ = g.render(cam, dict(sd.render, spp=ref_spp // 8, seed=100 + s))
+            pieces.append(rtg.resolve(h).astype(np.float64))
+        pieces = np.array(pieces)
+        ref = pieces.mean(0)
+        noise = pieces.var(0, ddof=1) / len(pieces)  # variance of the reference mean
+        ref_desc = f"rendered here, 8 x {ref_spp // 8} spp"
     t_ref = time.perf_counter() - t0
-    pieces = np.array(pieces)
-    ref = pieces.mean(0)
-    noise = pieces.var(0, ddof=1) / len(pieces)  # variance of the reference mean
     curves = {}
     extra = {}
     for name, gd in (("plain", 0), ("guided", 1)):
@@ -237,6 +249,8 @@ def guiding_leg(ctx, scene, size, ref_spp, target=1e-3, ladder=(64, 256, 1024, 4
                          "infer_ms": st.stage_ms[2], "render_ms": st.milliseconds, "queries": int(st.guide_queries),
                          "infer_tflops": tflops, "mfma_util": tflops / MFMA_BF16_PEAK_TFLOPS,
                          "training_samples": int(st.guide_samples), "final_loss": st.guide_loss}
+            if until_target and spp >= 1024 and e <= target:
+                break
         ctx.set_profiling(0)
         curves[name] = pts
 
@@ -250,8 +264,8 @@ def guiding_leg(ctx, scene, size, ref_spp, target=1e-3, ladder=(64, 256, 1024, 4
             return pts[0][1], float(pts[0][0])
         return None, None
 
-    res = {"scene": f"{scene} {size}x{size}", "target_relmse": target, "reference_spp": ref_spp,
-           "reference_s": t_ref}
+    res = {"scene": f"{scene} {size}x{size}", "target_relmse": target, "reference": ref_desc,
+           "reference_own_relmse": float(np.mean(noise / (ref * ref + 1e-2))), "reference_s": t_ref}
     for name, pts in curves.items():
         t, spp = time_to(pts)
         res[name] = {"ladder": [{"spp": p[0], "s": round(p[1], 4), "relmse": p[2]} for p in pts],
@@ -260,8 +274,10 @@ def guiding_leg(ctx, scene, size, ref_spp, target=1e-3, ladder=(64, 256, 1024, 4
     at = {name: next(p for p in pts if p[0] == 1024) for name, pts in curves.items()}
     res["error_ratio_1024spp"] = at["guided"][2] / at["plain"][2]
     res["error_x_time_ratio_1024spp"] = (at["guided"][2] * at["guided"][1]) / (at["plain"][2] * at["plain"][1])
+    res["guided_over_plain_time"] = None
     if res["plain"]["s_to_target"] and res["guided"]["s_to_target"]:
         res["guided_over_plain_time"] = res["guided"]["s_to_target"] / res["plain"]["s_to_target"]
+        res["meets_half_time_target"] = res["guided_over_plain_time"] <= 0.5
     res["guide_network"] = extra
     return res
 
@@ -449,7 +465,8 @@ def main():
     }
     if not a.no_guiding:
         progress("guided vs plain (glossy-indirect scene)")
-        out["guiding"] = guiding_leg(ctx, "glossy_cornell", a.guide_size, a.guide_ref_spp)
+        out["guiding"] = guiding_leg(ctx, "glossy_cornell", a.guide_size, a.guide_ref_spp,
+                                     ladder=(64, 256, 1024, 4096, 16384, 65536, 131072), until_target=True)
         progress("guided vs plain (indirect-only Lambertian variant)")
         out["guiding_indirect"] = guiding_leg(ctx, "indirect_cornell", a.guide_size, a.guide_ref_spp)
     if not a.no_volume:
