@@ -22,7 +22,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # algorithmic bytes of the extension-traversal kernel
 RAY_IO_BYTES = 4 + 4 + 32 + 16 + 4   # queue entry, flags, ray (2 x float4), hit record, shade-queue entry
-NODE_BYTES = 112                     # one 4-wide node visit reads 6 box float4 + the 4 child refs
+NODE_BYTES = 64                      # one 4-wide node visit reads one compressed node: a 16-byte header
+                                     # (origin, exponents), 32 bytes of 8-bit child bounds, 4 child refs
 TRI_BYTES = 36                       # three float3 vertices
 
 
@@ -36,6 +37,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (scale reference for PMC byte counters)")
+    ap.add_argument("--tuning-from", default="",
+                    help="take the launch configuration from the \"tuning\" of an earlier bench line (a JSON "
+                         "file) instead of autotuning: a profiled run then holds only the timed renders' launches")
     ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain leg")
     ap.add_argument("--guide-size", type=int, default=512, help="guided-vs-plain leg: image size")
     ap.add_argument("--guide-ref-spp", type=int, default=32768,
@@ -242,6 +246,7 @@ def guiding_leg(ctx, scene, size, ref_spp, target=1e-3, ladder=(64, 256, 1024, 4
             dt = time.perf_counter() - t0
             e = relmse(rtg.resolve(out).astype(np.float64), ref, noise)
             pts.append((spp, dt, e, st.samples))
+            print(f"[bench]   {scene} {name} {spp} spp: {dt:.2f} s, relMSE {e:.3e}", file=sys.stderr, flush=True)
             if gd and spp == 1024:
                 infer_s = st.stage_ms[2] / 1e3
                 tflops = st.guide_queries * GUIDE_FLOPS_PER_QUERY / infer_s / 1e12 if infer_s > 0 else 0.0
@@ -385,6 +390,17 @@ def main():
                    sh_rays=st.shadow_rays, sh_nodes=st.shadow_nodes_visited, sh_prims=st.shadow_prims_tested,
                    iters=st.iterations)
 
+    # launch configuration for this scene (pool size, traversal waves and refill) from the autotuner
+    if a.tuning_from:
+        with open(a.tuning_from) as f:
+            prev = json.loads(f.read().strip().splitlines()[-1])["tuning"]
+        tune_rate = prev.pop("trial_samples_per_s", None)
+        ctx.set_tuning(**prev)
+        tuning = ctx.tuning()
+    else:
+        progress("autotune")
+        tuning, tune_rate = ctx.autotune(scene, cam, rp, trial_samples=1 << 24)
+
     ctx.set_profiling(rtg.PROFILE_EVENTS)
     for _ in range(a.warmup):
         film.zero_()
@@ -446,8 +462,12 @@ def main():
             "triangles": sd.num_triangles(), "width": W, "height": H, "spp": rp["spp"],
             "max_depth": rp["max_depth"], "min_depth": rp["min_depth"], "sampler": "stratified",
         },
+        "value_measures": "plain MIS samples/s of the large-mesh render named in config.workload (the "
+                          "guided-vs-plain time to relMSE 1e-3 is the guiding object, request latency the "
+                          "serving object)",
         "rays_per_s": rays / wall,
         "device_ms_per_step": dev_ms / a.steps,
+        "tuning": dict(tuning, trial_samples_per_s=tune_rate),
         "stage_ms_per_step": {k: float(v) / a.steps for k, v in zip(rtg.STAGES, stage_ms)},
         "roofline": {
             "bound": "hbm",

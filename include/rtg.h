@@ -230,8 +230,26 @@ typedef struct rtg_scene rtg_scene;
 int rtg_ctx_create(int device, rtg_ctx** out);
 int rtg_ctx_destroy(rtg_ctx* ctx);
 const char* rtg_last_error(rtg_ctx* ctx);
-/* Wavefront pool size (path slots). 0 keeps the autotuned default. */
+/* Wavefront pool size (path slots); 0 restores the default of 2^21 slots (rtg_autotune picks one
+ * per scene). */
 int rtg_ctx_set_pool(rtg_ctx* ctx, int32_t slots);
+/* Launch configuration of the wavefront renderer (no reference counterpart: the reference's
+ * schedule is its JobManager thread count, core/JobManager.cpp:11). */
+typedef struct rtg_tuning {
+  int32_t pool_slots;    /* path slots of the wavefront pool (rounded up to a multiple of 256) */
+  int32_t trace_waves;   /* waves per SIMD of the persistent traversal launches (1..8) */
+  int32_t trace_refill;  /* idle lanes of a traversal wave before it takes new rays (1..64) */
+  int32_t graphs;        /* 1: steady-state iteration batches replay captured hipGraphs */
+} rtg_tuning;
+int rtg_ctx_get_tuning(rtg_ctx* ctx, rtg_tuning* out);
+/* Fails with RTG_ERR_INVALID while a server exists on the context. */
+int rtg_ctx_set_tuning(rtg_ctx* ctx, const rtg_tuning* tuning);
+/* Autotuner: times short renders of `desc` on `scene` (its size, about trial_samples samples each,
+ * 0 = 2^24; no guiding) under a coordinate search over traversal waves, refill threshold and pool
+ * size, starting from the context's current configuration; applies the fastest to the context and
+ * returns it (and its samples/s). Fails with RTG_ERR_INVALID while a server exists. */
+int rtg_autotune(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, uint64_t trial_samples,
+                 rtg_tuning* best, double* best_samples_per_s);
 /* Profiling flags (RTG_PROFILE_*): per-stage HIP events and traversal work counters. */
 int rtg_ctx_set_profiling(rtg_ctx* ctx, int32_t flags);
 

@@ -1,6 +1,7 @@
 // Host BVH builder: binned SAH (16 bins per axis over centroid bounds, all three axes),
-// leaves of up to 16 primitives, built as a binary tree and then collapsed into 4-wide nodes of
-// one 128-byte cache line (the child with the largest surface area is opened first).
+// leaves of up to 16 primitives, built as a binary tree and then collapsed into 4-wide nodes (the
+// child with the largest surface area is opened first), which the device reads compressed to 64
+// bytes (8-bit child bounds against a per-node origin and power-of-two steps, rounded outward).
 // The tree topology is this library's own; the reference builds a different tree
 // (accelStruct/Bvh.cpp:51-79,611-771). First hits agree up to exact/near ties because
 // both return the closest primitive under the same FP32 intersection arithmetic.
@@ -279,7 +280,97 @@ BvhResult build_bvh(const std::vector<BuildPrim>& tris, const std::vector<BuildP
     res.max_depth = cl.max_depth;
     res.max_stack = cl.max_stack;
   }
+  quantize_nodes(res.nodes, res.qnodes);
   return res;
+}
+
+// ---------------------------------------------------------------- node compression
+namespace {
+
+inline float pow2f(int e) { return std::ldexp(1.f, e); }
+
+// One axis of a node: origin = the smallest child lower bound; the step 2^e is the smallest power of
+// two for which every child's bounds round outward into 8 bits (starting from 2^ceil(log2(extent /
+// 255)), never below the float spacing at the node's coordinates, so that a step of q changes the
+// decoded value). qlo/qhi: the largest / smallest codes whose decoded bound is at or below / above
+// the exact bound.
+void quantize_axis(const float* lo, const float* hi, const bool* used, float& origin, int& e, uint8_t* qlo,
+                   uint8_t* qhi) {
+  origin = INFINITY;
+  float top = -INFINITY;
+  for (int j = 0; j < 4; j++)
+    if (used[j]) {
+      origin = std::min(origin, lo[j]);
+      top = std::max(top, hi[j]);
+    }
+  if (!(origin <= top)) {  // no used child (not reached for interior nodes)
+    origin = 0.f;
+    e = 0;
+    for (int j = 0; j < 4; j++) qlo[j] = qhi[j] = 0;
+    return;
+  }
+  const float mag = std::max(std::fabs(origin), std::fabs(top));
+  int e_ulp = -126;
+  if (mag > 0.f) {
+    int ex;
+    std::frexp(mag, &ex);  // mag = m 2^ex, m in [0.5, 1): float spacing there 2^(ex - 24)
+    e_ulp = std::max(-126, ex - 24);
+  }
+  const double ext = (double)top - (double)origin;
+  e = e_ulp;
+  if (ext > 0.0) e = std::max(e_ulp, (int)std::ceil(std::log2(ext / 255.0)));
+  for (;; e++) {
+    if (e > 127) throw std::runtime_error("BVH node too large to compress");
+    const float step = pow2f(e);
+    bool ok = true;
+    for (int j = 0; j < 4 && ok; j++) {
+      if (!used[j]) {
+        qlo[j] = qhi[j] = 0;
+        continue;
+      }
+      double a = std::floor(((double)lo[j] - origin) / step);
+      long ql = (long)std::max(0.0, std::min(255.0, a));
+      while (ql > 0 && dequantize_bound(origin, (uint32_t)ql, step) > lo[j]) ql--;
+      double b = std::ceil(((double)hi[j] - origin) / step);
+      long qh = (long)std::max(0.0, std::min(256.0, b));
+      while (qh <= 255 && dequantize_bound(origin, (uint32_t)qh, step) < hi[j]) qh++;
+      if (qh > 255 || dequantize_bound(origin, (uint32_t)ql, step) > lo[j]) {
+        ok = false;
+        break;
+      }
+      qlo[j] = (uint8_t)ql;
+      qhi[j] = (uint8_t)qh;
+    }
+    if (ok) return;
+  }
+}
+
+}  // namespace
+
+void quantize_nodes(const std::vector<float>& nodes, std::vector<uint32_t>& qnodes) {
+  const size_t n = nodes.size() / 32;
+  qnodes.assign(n * 16, 0u);
+  for (size_t i = 0; i < n; i++) {
+    const float* nd = &nodes[i * 32];
+    uint32_t* q = &qnodes[i * 16];
+    uint32_t refs[4];
+    std::memcpy(refs, nd + 24, 16);
+    bool used[4];
+    for (int j = 0; j < 4; j++) used[j] = refs[j] != kEmptyChild;
+    uint32_t ebits = 0;
+    for (int a = 0; a < 3; a++) {
+      float origin;
+      int e;
+      uint8_t ql[4], qh[4];
+      quantize_axis(nd + 4 * a, nd + 12 + 4 * a, used, origin, e, ql, qh);
+      std::memcpy(&q[a], &origin, 4);
+      ebits |= (uint32_t)(e + 127) << (8 * a);
+      q[4 + a] = (uint32_t)ql[0] | ((uint32_t)ql[1] << 8) | ((uint32_t)ql[2] << 16) | ((uint32_t)ql[3] << 24);
+      q[7 + a] = (uint32_t)qh[0] | ((uint32_t)qh[1] << 8) | ((uint32_t)qh[2] << 16) | ((uint32_t)qh[3] << 24);
+    }
+    q[3] = ebits;
+    std::memcpy(&q[12], refs, 16);
+  }
 }
 
 }  // namespace rtg

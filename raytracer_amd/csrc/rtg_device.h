@@ -427,7 +427,7 @@ struct TravState {
 };
 // The scene arrays a traversal reads, kept in registers for the duration of a ray.
 struct SceneGeo {
-  const float4* nodes;
+  const float4* nodes;  // compressed 4-wide nodes, 4 x 16 bytes each (rtg_bvh.h quantize_nodes)
   const float* tri_v;
   const float4* spheres;
   int32_t ntri;
@@ -473,11 +473,22 @@ __device__ __forceinline__ uint32_t trav_pop(const TravStack& stk, TravState& s)
 
 // Visits the interior node s.ref: descends into the nearest hit child and pushes the other hit
 // children farthest first (so the next nearest is popped next), or pops when no child is hit.
+// A compressed node's child bound: origin + q * step (exact product, one rounded addition; the host
+// rounded q outward for exactly this arithmetic, so the decoded box contains the child's box).
+__device__ __forceinline__ float4 dq4(uint32_t w, float origin, float step) {
+  return make_float4(origin + (float)(w & 0xFFu) * step, origin + (float)((w >> 8) & 0xFFu) * step,
+                     origin + (float)((w >> 16) & 0xFFu) * step, origin + (float)(w >> 24) * step);
+}
 __device__ __forceinline__ void trav_node(const SceneGeo& G, TravState& s, const TravStack& stk,
                                           uint32_t& nodes_visited) {
-  const float4* nd = G.nodes + 8u * s.ref;
-  const float4 lx = nd[0], ly = nd[1], lz = nd[2], hx = nd[3], hy = nd[4], hz = nd[5];
-  const uint4 rf = *reinterpret_cast<const uint4*>(nd + 6);
+  // 64-byte compressed node: origin + step exponents, 8-bit child bounds, child refs
+  const uint4* nd = reinterpret_cast<const uint4*>(G.nodes) + 4u * s.ref;
+  const uint4 hd = nd[0], q0 = nd[1], q1 = nd[2], rf = nd[3];
+  const float ox = __uint_as_float(hd.x), oy = __uint_as_float(hd.y), oz = __uint_as_float(hd.z);
+  const float sx = __uint_as_float((hd.w & 0xFFu) << 23), sy = __uint_as_float(((hd.w >> 8) & 0xFFu) << 23),
+              sz = __uint_as_float(((hd.w >> 16) & 0xFFu) << 23);
+  const float4 lx = dq4(q0.x, ox, sx), ly = dq4(q0.y, oy, sy), lz = dq4(q0.z, oz, sz);
+  const float4 hx = dq4(q0.w, ox, sx), hy = dq4(q1.x, oy, sy), hz = dq4(q1.y, oz, sz);
   nodes_visited++;
   float t0, t1, t2, t3;
   bool h0, h1, h2, h3;

@@ -492,7 +492,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   // upload
   uint64_t bytes = 0;
   auto& own = sc->owned;
-  D.nodes = (const float4*)upload(ctx, bvh.nodes, own, bytes);
+  D.nodes = (const float4*)upload(ctx, bvh.qnodes, own, bytes);
   D.root_ref = bvh.root_ref;
   D.empty = bvh.empty ? 1 : 0;
   for (int k = 0; k < 3; k++) {
@@ -673,7 +673,8 @@ void graphs_clear(rtg_ctx* ctx) {
   ctx->graphs.clear();
 }
 
-void run_iterations(rtg_ctx* ctx, int mode, const LaunchCfg& cfg, int count) {
+void run_iterations(rtg_ctx* ctx, int mode, const LaunchCfg& cfg, int count, hipEvent_t* ev) {
+  if (mode != IT_PLAIN) ev = nullptr;
   auto launch = [&]() {
     for (int b = 0; b < count; b++) {
       if (mode == IT_VOL)
@@ -681,10 +682,13 @@ void run_iterations(rtg_ctx* ctx, int mode, const LaunchCfg& cfg, int count) {
       else if (mode == IT_MIXED)
         launch_iteration_mixed(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, cfg, ctx->stream);
       else
-        launch_iteration(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, cfg, ctx->stream);
+        launch_iteration(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, cfg, ctx->stream,
+                         ev ? ev + (size_t)b * kIterEvents : nullptr);
     }
   };
-  if (!ctx->use_graphs) {
+  // per-stage events are read with hipEventElapsedTime, which does not accept events recorded by a
+  // graph's event nodes: profiled batches launch directly
+  if (!ctx->use_graphs || ev) {
     launch();
     return;
   }
@@ -716,11 +720,11 @@ void run_iterations(rtg_ctx* ctx, int mode, const LaunchCfg& cfg, int count) {
   }
   launch();
   hipGraph_t graph = nullptr;
-  hip_check(hipStreamEndCapture(ctx->stream, &graph), "hipStreamEndCapture");
+  hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
   hipGraphExec_t exec = nullptr;
-  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-  (void)hipGraphDestroy(graph);
-  hip_check(e, "hipGraphInstantiate");
+  if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  if (graph) (void)hipGraphDestroy(graph);
+  hip_check(e, "graph capture");
   if (ctx->graphs.size() >= 16) {  // a bounded cache: drop the oldest configuration
     (void)hipGraphExecDestroy(ctx->graphs.front().exec);
     ctx->graphs.erase(ctx->graphs.begin());
@@ -878,10 +882,11 @@ int rtg_ctx_create(int device, rtg_ctx** out) {
     // waves per SIMD of a traversal launch (blocks of 128 = 2 waves; 4 SIMDs per CU)
     if (const char* e = std::getenv("RTG_TRACE_REFILL")) ctx->refill = std::min(64, std::max(1, std::atoi(e)));
     int waves = kTraceWaves;
-    if (const char* e = std::getenv("RTG_TRACE_WAVES")) waves = std::min(8, std::max(1, std::atoi(e)));
+    if (const char* e = std::getenv("RTG_TRACE_WAVES")) waves = std::min(kMaxTraceWaves, std::max(1, std::atoi(e)));
+    ctx->trace_waves = waves;
     ctx->trace_blocks = std::max(32, ctx->cu_count * 2 * waves);
-    // traversal stack spill area for every lane of a traversal launch
-    size_t lanes = (size_t)ctx->trace_blocks * 128;
+    // traversal stack spill area for every lane of a traversal launch of up to kMaxTraceWaves waves
+    size_t lanes = (size_t)std::max(32, ctx->cu_count * 2 * kMaxTraceWaves) * 128;
     hip_check(hipMalloc(&ctx->gstack, lanes * (kMaxStack - 16) * sizeof(uint2)), "hipMalloc stack spill");
     hip_check(hipMalloc(&ctx->tap_ctr, 2 * sizeof(uint32_t)), "hipMalloc trace counters");
     // device address of this GPU's static scratch array (the allocation-free trace's scratch); a
@@ -935,6 +940,36 @@ int rtg_ctx_set_pool(rtg_ctx* ctx, int32_t slots) {
   RTG_GUARD(ctx, {
     if (slots < 0 || slots > (1 << 26)) fail(RTG_ERR_INVALID, "pool size out of range");
     ctx->pool_slots = slots == 0 ? (1 << 21) : ((slots + 255) / 256) * 256;
+  })
+}
+
+int rtg_ctx_get_tuning(rtg_ctx* ctx, rtg_tuning* out) {
+  if (!ctx || !out) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  out->pool_slots = ctx->pool_slots;
+  out->trace_waves = ctx->trace_waves;
+  out->trace_refill = ctx->refill;
+  out->graphs = ctx->use_graphs ? 1 : 0;
+  return RTG_OK;
+}
+
+static void apply_tuning(rtg_ctx* ctx, const rtg_tuning& t) {
+  if (t.pool_slots < 256 || t.pool_slots > (1 << 26)) fail(RTG_ERR_INVALID, "pool_slots out of range");
+  if (t.trace_waves < 1 || t.trace_waves > kMaxTraceWaves) fail(RTG_ERR_INVALID, "trace_waves out of range");
+  if (t.trace_refill < 1 || t.trace_refill > 64) fail(RTG_ERR_INVALID, "trace_refill out of range");
+  ctx->pool_slots = ((t.pool_slots + 255) / 256) * 256;
+  ctx->trace_waves = t.trace_waves;
+  ctx->trace_blocks = std::max(32, ctx->cu_count * 2 * t.trace_waves);
+  ctx->refill = t.trace_refill;
+  ctx->use_graphs = t.graphs != 0;
+}
+
+int rtg_ctx_set_tuning(rtg_ctx* ctx, const rtg_tuning* t) {
+  if (!ctx || !t) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  RTG_GUARD(ctx, {
+    if (ctx->server) fail(RTG_ERR_INVALID, "the context is serving requests; tune it before creating the server");
+    apply_tuning(ctx, *t);
   })
 }
 
@@ -1067,7 +1102,7 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   hip_check(hipEventRecord(e0, ctx->stream), "event");
   uint64_t iters = 0;
   const int batch = 4;
-  const int kEv = 9;             // events per iteration (8 stages)
+  const int kEv = kIterEvents;   // events per iteration (8 stages)
   const int kTrainEv = 2 * kMaxStepsPerBatch;
   const bool prof = (ctx->profiling & RTG_PROFILE_EVENTS) != 0 && !vol;
   double stage_ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
@@ -1083,13 +1118,13 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   }
   int pending_train = 0;  // training steps enqueued before the batch now in flight
   while (true) {
-    if (prof || guided) {
-      // per-stage events and the guide's changing state: direct launches
+    if (guided) {
+      // the guide's state changes between batches: direct launches
       for (int b = 0; b < batch; b++)
         launch_iteration(ctx->pool, ctx->scenes_dev, ctx->jobs_dev, ctx->order_dev, cfg, ctx->stream,
-                         prof ? &ctx->events[(size_t)b * kEv] : nullptr, guided ? &G : nullptr);
+                         prof ? &ctx->events[(size_t)b * kEv] : nullptr, &G);
     } else {
-      run_iterations(ctx, vol ? IT_VOL : IT_PLAIN, cfg, batch);
+      run_iterations(ctx, vol ? IT_VOL : IT_PLAIN, cfg, batch, prof ? ctx->events.data() : nullptr);
     }
     iters += batch;
     hip_check(hipGetLastError(), "kernel launch");
@@ -1198,6 +1233,81 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
 // PathTracingMIS) joins its mix as a request and the call waits for it; other renders fail.
 static bool servable(const rtg_scene* scene, const rtg_render_desc* desc) {
   return scene && desc && !desc->guiding && !(scene->all_lights && desc->integrator == RTG_INTEGRATOR_PATH_MIS);
+}
+
+int rtg_autotune(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, uint64_t trial_samples, rtg_tuning* best,
+                 double* best_samples_per_s) {
+  if (!ctx) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  RTG_GUARD(ctx, {
+    if (!scene || !desc) fail(RTG_ERR_INVALID, "null scene or render description");
+    if (ctx->server) fail(RTG_ERR_INVALID, "the context is serving requests; tune it before creating the server");
+    if (desc->width <= 0 || desc->height <= 0) fail(RTG_ERR_INVALID, "bad size");
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    rtg_render_desc d = *desc;
+    d.guiding = 0;
+    const uint64_t npix = (uint64_t)d.width * d.height;
+    d.spp = (int32_t)std::max<uint64_t>(1, std::min<uint64_t>(1u << 20, (trial_samples ? trial_samples : (1u << 24)) / npix));
+    const size_t bytes = npix * 16;
+    void* film = nullptr;
+    hip_check(hipMalloc(&film, bytes), "hipMalloc tuning film");
+    const int saved_prof = ctx->profiling;
+    ctx->profiling = 0;
+    rtg_tuning cur;
+    cur.pool_slots = ctx->pool_slots;
+    cur.trace_waves = ctx->trace_waves;
+    cur.trace_refill = ctx->refill;
+    cur.graphs = ctx->use_graphs ? 1 : 0;
+    // samples per second of a configuration: the faster of two renders after a warm-up render
+    auto rate = [&](const rtg_tuning& t) {
+      apply_tuning(ctx, t);
+      double bestr = 0.0;
+      for (int rep = 0; rep < 3; rep++) {
+        launch_fill_zero(film, bytes, ctx->stream);
+        rtg_stats st;
+        const auto t0 = std::chrono::steady_clock::now();
+        render_impl(ctx, scene, &d, (float*)film, &st);
+        const double s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        if (rep > 0) bestr = std::max(bestr, (double)st.samples / s);
+      }
+      return bestr;
+    };
+    try {
+      double r_best = rate(cur);
+      rtg_tuning t_best = cur;
+      // coordinate search: traversal waves, then the refill threshold, then the pool size
+      for (int w : {3, 4, 5, 6, 8}) {
+        rtg_tuning t = t_best;
+        t.trace_waves = w;
+        if (w == t_best.trace_waves) continue;
+        const double r = rate(t);
+        if (r > r_best) { r_best = r; t_best = t; }
+      }
+      for (int f : {4, 8, 16, 32}) {
+        rtg_tuning t = t_best;
+        t.trace_refill = f;
+        if (f == t_best.trace_refill) continue;
+        const double r = rate(t);
+        if (r > r_best) { r_best = r; t_best = t; }
+      }
+      for (int p : {1 << 20, 3 << 19, 1 << 21, 3 << 20, 1 << 22}) {
+        rtg_tuning t = t_best;
+        t.pool_slots = p;
+        if (p == t_best.pool_slots) continue;
+        const double r = rate(t);
+        if (r > r_best) { r_best = r; t_best = t; }
+      }
+      apply_tuning(ctx, t_best);
+      if (best) *best = t_best;
+      if (best_samples_per_s) *best_samples_per_s = r_best;
+    } catch (...) {
+      ctx->profiling = saved_prof;
+      (void)hipFree(film);
+      throw;
+    }
+    ctx->profiling = saved_prof;
+    (void)hipFree(film);
+  })
 }
 
 int rtg_render_device(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, float* out_dev, rtg_stats* st) {

@@ -61,6 +61,7 @@ struct rtg_scene {
 };
 
 namespace rtg {
+static const int kIterEvents = 9;  // events per profiled iteration: before regen, after each of 8 stages
 // Kinds of wavefront iteration (run_iterations)
 enum IterMode { IT_PLAIN = 0, IT_VOL = 1, IT_MIXED = 2 };
 // A captured batch of iterations: everything its kernels receive by value is in the key.
@@ -94,6 +95,7 @@ struct rtg_ctx {
   void* scratch = nullptr;            // this device's static scratch array (rtg_trace_device_noalloc)
   uint64_t last_trace_spills = 0;
   int refill = rtg::kTraceRefill;
+  int trace_waves = rtg::kTraceWaves;  // waves per SIMD of the persistent traversal launches
   float* lut_dev = nullptr;  // kMaxJobs filter tables of kFilterLut weights
   int trace_blocks = 0;
   int32_t* order_dev = nullptr;    // claim order of job-table slots
@@ -121,7 +123,9 @@ void upload_job_filter(rtg_ctx* ctx, const rtg_render_desc& d, DevJob& J, int sl
 LaunchCfg launch_cfg(rtg_ctx* ctx);
 // Enqueues `count` iterations of kind `mode` (IterMode) on the context's stream, replaying a cached
 // hipGraph of the batch when graphs are on (captured on first use of a configuration).
-void run_iterations(rtg_ctx* ctx, int mode, const LaunchCfg& cfg, int count);
+// ev (IT_PLAIN only, or null): per-stage events of each iteration (kIterEvents per iteration); a
+// profiled batch launches directly (its events are timed with hipEventElapsedTime).
+void run_iterations(rtg_ctx* ctx, int mode, const LaunchCfg& cfg, int count, hipEvent_t* ev = nullptr);
 void graphs_clear(rtg_ctx* ctx);
 // rtg_render / rtg_render_device while a server runs on the context: the render becomes a request
 // (rtg_serve.cpp); film_dev = the caller's device film (not cleared) or NULL.

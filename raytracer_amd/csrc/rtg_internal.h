@@ -7,9 +7,10 @@
 namespace rtg {
 
 // ---------------------------------------------------------------- BVH layout
-// 4-wide interior node (128 bytes, 8 float4): min x, min y, min z, max x, max y, max z of the four
-// children (one float4 each, child j in component j), then the four child refs, then padding.
-// An unused child slot holds kEmptyChild. A child ref with bit 31 set is a leaf: bits 0..26
+// 4-wide interior node as built (128 bytes, 8 float4): min x, min y, min z, max x, max y, max z of
+// the four children (one float4 each, child j in component j), then the four child refs, then
+// padding; the device reads it compressed to 64 bytes (rtg_bvh.h). An unused child slot holds
+// kEmptyChild. A child ref with bit 31 set is a leaf: bits 0..26
 // first primitive (leaf order), bits 27..30 count-1 (1..16 primitives). Otherwise the ref is an
 // interior node index.
 static const uint32_t kLeafBit = 0x80000000u;
@@ -63,7 +64,7 @@ struct DevLight {
 };
 
 struct DevScene {
-  const float4* nodes;       // 8 float4 per 4-wide interior node
+  const float4* nodes;       // 4 x 16 bytes per compressed 4-wide interior node (rtg_bvh.h)
   uint32_t root_ref;         // ref of the root (interior index 0, or a leaf)
   int32_t empty;             // no primitives at all
   float root_min[3], root_max[3];
@@ -231,7 +232,8 @@ struct DevGuide {
 
 // Launch wrappers implemented in rtg_kernels.hip
 static const int kTraceRefill = 16;
-static const int kTraceWaves = 5;  // waves per SIMD of a traversal launch
+static const int kTraceWaves = 5;  // waves per SIMD of a traversal launch (default)
+static const int kMaxTraceWaves = 8;  // the spill area is sized for this many
 struct LaunchCfg {
   int grid;      // blocks for grid-stride kernels
   int trace_blocks;  // blocks of 128 for the persistent traversal kernels

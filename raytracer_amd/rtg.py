@@ -105,6 +105,11 @@ class rtg_server_stats(C.Structure):
     ]
 
 
+class rtg_tuning(C.Structure):
+    _fields_ = [("pool_slots", C.c_int32), ("trace_waves", C.c_int32), ("trace_refill", C.c_int32),
+                ("graphs", C.c_int32)]
+
+
 STAGES = ("regen", "extend", "guide", "shade_diffuse", "shade_conductor", "shade_dielectric", "shade_miss", "shadow")
 PROFILE_EVENTS, PROFILE_COUNT = 1, 2
 
@@ -117,6 +122,10 @@ SIGNATURES = {
     "rtg_last_error": (C.c_char_p, [P]),
     "rtg_ctx_set_pool": (C.c_int, [P, C.c_int32]),
     "rtg_ctx_set_profiling": (C.c_int, [P, C.c_int32]),
+    "rtg_ctx_get_tuning": (C.c_int, [P, C.POINTER(rtg_tuning)]),
+    "rtg_ctx_set_tuning": (C.c_int, [P, C.POINTER(rtg_tuning)]),
+    "rtg_autotune": (C.c_int, [P, P, C.POINTER(rtg_render_desc), C.c_uint64, C.POINTER(rtg_tuning),
+                               C.POINTER(C.c_double)]),
     "rtg_scene_upload": (C.c_int, [P, C.POINTER(rtg_scene_desc), C.POINTER(P)]),
     "rtg_scene_load_xml": (C.c_int, [P, C.c_char_p, C.POINTER(P), C.POINTER(rtg_render_desc)]),
     "rtg_scene_release": (C.c_int, [P, P]),
@@ -310,6 +319,26 @@ class Context:
         out = np.zeros((H, W, 3), np.uint8)
         self.check(self.lib.rtg_tonemap(self.h, _ptr(f), W, H, int(mode), float(exposure), _ptr(out)))
         return out
+
+    def tuning(self):
+        t = rtg_tuning()
+        self.check(self.lib.rtg_ctx_get_tuning(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in t._fields_}
+
+    def set_tuning(self, **kw):
+        t = self.tuning()
+        t.update(kw)
+        s = rtg_tuning(**{k: int(v) for k, v in t.items()})
+        self.check(self.lib.rtg_ctx_set_tuning(self.h, C.byref(s)))
+
+    def autotune(self, scene, camera, params, trial_samples=0):
+        """rtg_autotune -> (chosen configuration dict, its samples/s in the trial renders)."""
+        d = render_struct(camera, params)
+        t = rtg_tuning()
+        r = C.c_double(0.0)
+        self.check(self.lib.rtg_autotune(self.h, scene.h, C.byref(d), C.c_uint64(int(trial_samples)), C.byref(t),
+                                         C.byref(r)))
+        return {k: getattr(t, k) for k, _ in t._fields_}, r.value
 
     def set_pool(self, slots):
         self.check(self.lib.rtg_ctx_set_pool(self.h, int(slots)))

@@ -87,7 +87,7 @@ int main(int argc, char** argv) {
   }
   DevScene S;
   std::memset(&S, 0, sizeof(S));
-  S.nodes = (const float4*)bvh.nodes.data();
+  S.nodes = (const float4*)bvh.qnodes.data();  // the compressed nodes the device traverses
   S.root_ref = bvh.root_ref;
   S.empty = bvh.empty ? 1 : 0;
   for (int a = 0; a < 3; a++) {

@@ -247,3 +247,25 @@ def test_zero_light_strategy_is_the_reference_default(ctx):
         imgs[strat], _ = gpu_image(ctx.upload(sd), sd.camera, sd.render, spp=64, seed=22)
     assert np.allclose(imgs[rtg.LIGHTS_DEFAULT], imgs[rtg.LIGHTS_UNIFORM], rtol=1e-5, atol=1e-6)
     assert not np.allclose(imgs[rtg.LIGHTS_DEFAULT], imgs[rtg.LIGHTS_ALL], rtol=1e-3, atol=1e-4)
+
+
+def test_tuning_and_graphs_leave_the_image_unchanged(ctx):
+    """rtg_autotune picks a configuration within its ranges; renders under it, with hipGraph replay
+    off, and under a hand-set configuration equal the default render up to the film's float
+    addition order (a path's random numbers do not depend on slots, launches or iterations)."""
+    sd = scenes.glossy_cornell(width=48, height=32)
+    g = ctx.upload(sd)
+    base = ctx.tuning()
+    ref, _ = gpu_image(g, sd.camera, sd.render, spp=64, seed=23)
+    try:
+        best, rate = ctx.autotune(g, sd.camera, sd.render, trial_samples=1 << 20)
+        print(f"[render] autotune: {best} at {rate / 1e6:.0f} M samples/s", flush=True)
+        assert 1 <= best["trace_waves"] <= 8 and 1 <= best["trace_refill"] <= 64 and best["pool_slots"] >= 256
+        assert ctx.tuning() == best and rate > 0
+        for t in (best, dict(best, graphs=0), dict(base, pool_slots=1 << 16, trace_waves=3, trace_refill=4)):
+            ctx.set_tuning(**t)
+            img, st = gpu_image(g, sd.camera, sd.render, spp=64, seed=23)
+            assert st.samples == 48 * 32 * 64
+            assert np.allclose(img, ref, rtol=1e-4, atol=1e-6), t
+    finally:
+        ctx.set_tuning(**base)

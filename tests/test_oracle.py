@@ -832,3 +832,28 @@ def test_png_round_trip(tmp_path):
     image.write_png(str(p), img)
     assert open(p, "rb").read()[:8] == b"\x89PNG\r\n\x1a\n"
     assert np.array_equal(image.read_png(str(p)), img)
+
+
+def test_exr_writer_round_trip(tmp_path):
+    """The film saved as the reference saves it (core/main.cpp:292-318: RGBA halves, alpha 1): the
+    file starts with the OpenEXR magic and version 2, lists the channels A, B, G, R as halves, and
+    reads back to the image rounded to half (round to nearest even: the ties 1 + 2^-11 -> 1 and
+    1 + 3 2^-11 -> 1 + 2^-9; 65520 overflows to infinity); the scanline table points at every line."""
+    from raytracer_amd.image import read_exr, write_exr
+    import struct as st
+
+    rng = np.random.default_rng(31)
+    img = (rng.random((5, 7, 3)) * 4).astype(np.float32)
+    img[0, 0] = (1 + 2 ** -11, 1 + 3 * 2 ** -11, 65520.0)
+    img[4, 6] = (0.0, 1e-8, -2.5)
+    p = tmp_path / "film.exr"
+    write_exr(str(p), img)
+    raw = p.read_bytes()
+    assert st.unpack("<ii", raw[:8]) == (20000630, 2)
+    ch, attrs = read_exr(str(p))
+    assert list(ch) == ["A", "B", "G", "R"] and attrs["channels"][0] == "chlist"
+    assert np.all(ch["A"] == 1.0)
+    got = np.stack([ch["R"], ch["G"], ch["B"]], -1)
+    assert np.array_equal(got[1:], img[1:].astype(np.float16).astype(np.float32))
+    assert got[0, 0, 0] == 1.0 and got[0, 0, 1] == 1 + 2 ** -9 and np.isinf(got[0, 0, 2])
+    assert got[4, 6, 2] == -2.5 and got[4, 6, 1] == np.float32(np.float16(1e-8))
