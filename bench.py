@@ -25,6 +25,8 @@ RAY_IO_BYTES = 4 + 4 + 32 + 16 + 4   # queue entry, flags, ray (2 x float4), hit
 NODE_BYTES = 64                      # one 4-wide node visit reads one compressed node: a 16-byte header
                                      # (origin, exponents), 32 bytes of 8-bit child bounds, 4 child refs
 TRI_BYTES = 36                       # three float3 vertices
+SEGMENT_IO_BYTES = 292               # per path segment: ray read 32, hit write + read 32, path state read +
+                                     # write 128, shadow ray write + read 64, occlusion 4, next ray 32
 
 
 def parse():
@@ -40,10 +42,14 @@ def parse():
     ap.add_argument("--tuning-from", default="",
                     help="take the launch configuration from the \"tuning\" of an earlier bench line (a JSON "
                          "file) instead of autotuning: a profiled run then holds only the timed renders' launches")
+    ap.add_argument("--pool", type=int, default=0,
+                    help="path-pool slots for the benchmark render, overriding the tuning (0: keep it)")
     ap.add_argument("--no-guiding", action="store_true", help="skip the guided-vs-plain leg")
     ap.add_argument("--guide-size", type=int, default=512, help="guided-vs-plain leg: image size")
     ap.add_argument("--guide-ref-spp", type=int, default=32768,
                     help="guided-vs-plain leg: reference spp (eight independent pieces)")
+    ap.add_argument("--only-guiding", action="store_true",
+                    help="run only the guided-vs-plain legs and print their JSON object")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-mix leg")
     ap.add_argument("--only-serving", action="store_true",
                     help="run only the serving-mix leg and print its JSON object")
@@ -200,19 +206,24 @@ MFMA_BF16_PEAK_TFLOPS = 2500.0
 GLOSSY_REFERENCE = os.path.join(ROOT, "tests", "golden", "glossy_cornell_ref.npz")
 
 
-def guiding_leg(ctx, scene, size, ref_spp, target=1e-3, ladder=(64, 256, 1024, 4096, 8192), until_target=False):
-    """Plain vs guided on one scene, both on the GPU: error and time at a ladder of spp counts.
+def guiding_leg(ctx, scene, size, ref_spp, target=1e-3, ladder=(256, 1024, 4096), piece_spp=4096,
+                max_spp=262144, methods=(("plain", 0), ("guided", 1), ("exit_only", 2))):
+    """Plain vs guided on one scene, both on the GPU: error at a short ladder of spp counts and the
+    time to relMSE <= target.
 
     The judge of the error is the plain tracer's converged image: for the glossy-indirect scene at
     512x512 the committed reference (tests/golden/glossy_cornell_ref.npz: 2^20 spp as 64 independent
     pieces, FP64 accumulation), otherwise ref_spp samples rendered here as eight pieces; the spread of
     the pieces estimates the reference's own per-pixel noise, which is subtracted from every squared
-    error so that it does not count against either method. Each render starts from scratch (a guided
-    render initialises and trains its network online inside the timed call). until_target: each
-    method's ladder stops at its first point at or below the target. Reports relMSE and seconds per
-    ladder point, the guided/plain error ratio and error x time ratio at 1024 spp (the
-    configuration's sample budget), the time to relMSE <= target (interpolated log-log between the
-    ladder points around the crossing), their ratio, and the network's device time and MFMA rate."""
+    error so that it does not count against either method. Every render starts from scratch (a
+    guided render initialises and trains its network online inside the timed call).
+    Methods: rtg_render_desc.guiding 0 (plain, the reference's estimator), 1 (learned guiding with
+    the rough-dielectric exit proposals), 2 (the exit proposals alone).
+    Ladder: one render per point; the guided/plain error ratio and error x time ratio at 1024 spp
+    (the configuration's sample budget). Time to target: independent renders of piece_spp samples
+    (seeds 5000 + k) are averaged progressively, the error measured after each, until the average
+    reaches the target or max_spp; the time is the sum of the pieces' wall times, interpolated
+    log-log between the two pieces around the crossing."""
     import numpy as np
     from raytracer_amd import rtg, scenes
 
@@ -235,54 +246,79 @@ def guiding_leg(ctx, scene, size, ref_spp, target=1e-3, ladder=(64, 256, 1024, 4
         noise = pieces.var(0, ddof=1) / len(pieces)  # variance of the reference mean
         ref_desc = f"rendered here, 8 x {ref_spp // 8} spp"
     t_ref = time.perf_counter() - t0
-    curves = {}
+    res = {"scene": f"{scene} {size}x{size}", "target_relmse": target, "reference": ref_desc,
+           "reference_own_relmse": float(np.mean(noise / (ref * ref + 1e-2))), "reference_s": t_ref,
+           "methods": {name: gd for name, gd in methods}, "piece_spp": piece_spp, "max_spp": max_spp}
     extra = {}
-    for name, gd in (("plain", 0), ("guided", 1)):
-        pts = []
+    for name, gd in methods:
         ctx.set_profiling(rtg.PROFILE_EVENTS)
+        pts = []
         for i, spp in enumerate(ladder):
-            t0 = time.perf_counter()
+            t1 = time.perf_counter()
             out, st = g.render(cam, dict(sd.render, spp=spp, seed=1000 + i, guiding=gd))
-            dt = time.perf_counter() - t0
+            dt = time.perf_counter() - t1
             e = relmse(rtg.resolve(out).astype(np.float64), ref, noise)
-            pts.append((spp, dt, e, st.samples))
+            pts.append((spp, dt, e))
             print(f"[bench]   {scene} {name} {spp} spp: {dt:.2f} s, relMSE {e:.3e}", file=sys.stderr, flush=True)
-            if gd and spp == 1024:
+            if gd == 1 and spp == 1024:
                 infer_s = st.stage_ms[2] / 1e3
                 tflops = st.guide_queries * GUIDE_FLOPS_PER_QUERY / infer_s / 1e12 if infer_s > 0 else 0.0
                 extra = {"train_steps": int(st.guide_train_steps), "train_ms": st.guide_train_ms,
                          "infer_ms": st.stage_ms[2], "render_ms": st.milliseconds, "queries": int(st.guide_queries),
                          "infer_tflops": tflops, "mfma_util": tflops / MFMA_BF16_PEAK_TFLOPS,
                          "training_samples": int(st.guide_samples), "final_loss": st.guide_loss}
-            if until_target and spp >= 1024 and e <= target:
-                break
         ctx.set_profiling(0)
-        curves[name] = pts
-
-    def time_to(pts):
-        for (s0, t0_, e0, _), (s1, t1, e1, _) in zip(pts, pts[1:]):
-            if e0 > target >= e1:
-                f = (np.log(target) - np.log(e0)) / (np.log(e1) - np.log(e0))
-                return float(np.exp(np.log(t0_) + f * (np.log(t1) - np.log(t0_)))), \
-                    float(np.exp(np.log(s0) + f * (np.log(s1) - np.log(s0))))
-        if pts and pts[0][2] <= target:
-            return pts[0][1], float(pts[0][0])
-        return None, None
-
-    res = {"scene": f"{scene} {size}x{size}", "target_relmse": target, "reference": ref_desc,
-           "reference_own_relmse": float(np.mean(noise / (ref * ref + 1e-2))), "reference_s": t_ref}
-    for name, pts in curves.items():
-        t, spp = time_to(pts)
+        # progressive average of independent pieces until the target
+        acc = np.zeros_like(ref)
+        prog = []
+        t_sum = 0.0
+        k = 0
+        while (k + 1) * piece_spp <= max_spp:
+            t1 = time.perf_counter()
+            out, _ = g.render(cam, dict(sd.render, spp=piece_spp, seed=5000 + k, guiding=gd))
+            img = rtg.resolve(out).astype(np.float64)
+            t_sum += time.perf_counter() - t1
+            acc += img
+            k += 1
+            e = relmse(acc / k, ref, noise)
+            prog.append((k * piece_spp, t_sum, e))
+            if e <= target:
+                break
+        print(f"[bench]   {scene} {name} progressive: {k} pieces, {k * piece_spp} spp, {t_sum:.1f} s, relMSE {e:.3e}",
+              file=sys.stderr, flush=True)
+        t_to, spp_to = None, None
+        if prog[-1][2] <= target:
+            if len(prog) == 1:
+                t_to, spp_to = prog[0][1], float(prog[0][0])
+            else:
+                (s0, ta, e0), (s1, tb, e1) = prog[-2], prog[-1]
+                f = (np.log(target) - np.log(e0)) / (np.log(e1) - np.log(e0)) if e0 > e1 else 1.0
+                t_to = float(np.exp(np.log(ta) + f * (np.log(tb) - np.log(ta))))
+                spp_to = float(np.exp(np.log(s0) + f * (np.log(s1) - np.log(s0))))
         res[name] = {"ladder": [{"spp": p[0], "s": round(p[1], 4), "relmse": p[2]} for p in pts],
-                     "s_to_target": t, "spp_to_target": spp,
-                     "samples_per_s_to_target": (spp * size * size / t) if t else None}
-    at = {name: next(p for p in pts if p[0] == 1024) for name, pts in curves.items()}
-    res["error_ratio_1024spp"] = at["guided"][2] / at["plain"][2]
-    res["error_x_time_ratio_1024spp"] = (at["guided"][2] * at["guided"][1]) / (at["plain"][2] * at["plain"][1])
-    res["guided_over_plain_time"] = None
-    if res["plain"]["s_to_target"] and res["guided"]["s_to_target"]:
-        res["guided_over_plain_time"] = res["guided"]["s_to_target"] / res["plain"]["s_to_target"]
+                     "progressive": [{"spp": p[0], "s": round(p[1], 3), "relmse": p[2]} for p in prog],
+                     "s_to_target": t_to, "spp_to_target": spp_to,
+                     "samples_per_s_to_target": (spp_to * size * size / t_to) if t_to else None}
+    at = {name: next(p for p in res[name]["ladder"] if p["spp"] == 1024) for name, _ in methods}
+    for name, _ in methods[1:]:
+        a, b = at[name], at["plain"]
+        res[f"{name}_error_ratio_1024spp"] = a["relmse"] / b["relmse"]
+        res[f"{name}_error_x_time_ratio_1024spp"] = (a["relmse"] * a["s"]) / (b["relmse"] * b["s"])
+        res[f"{name}_over_plain_time"] = None
+        if res["plain"]["s_to_target"] and res[name]["s_to_target"]:
+            res[f"{name}_over_plain_time"] = res[name]["s_to_target"] / res["plain"]["s_to_target"]
+            res[f"{name}_meets_half_time_target"] = res[f"{name}_over_plain_time"] <= 0.5
+    # the headline pair: learned guiding (the network and the exit proposals) against plain; the
+    # fastest guided mode beside it
+    res["error_ratio_1024spp"] = res.get("guided_error_ratio_1024spp")
+    res["error_x_time_ratio_1024spp"] = res.get("guided_error_x_time_ratio_1024spp")
+    res["guided_over_plain_time"] = res.get("guided_over_plain_time")
+    if res["guided_over_plain_time"] is not None:
         res["meets_half_time_target"] = res["guided_over_plain_time"] <= 0.5
+    timed = [(res[f"{n}_over_plain_time"], n) for n, _ in methods[1:] if res.get(f"{n}_over_plain_time")]
+    if timed:
+        res["fastest_guided_mode"] = {"method": min(timed)[1], "over_plain_time": min(timed)[0],
+                                      "meets_half_time_target": min(timed)[0] <= 0.5}
     res["guide_network"] = extra
     return res
 
@@ -368,6 +404,11 @@ def main():
         progress("volumetric leg only")
         print(json.dumps(volumetric_leg(rtg.Context(0), not a.no_cpu, a.volume_stride)), flush=True)
         return
+    if a.only_guiding:
+        gctx = rtg.Context(0)
+        progress("guided vs plain only")
+        print(json.dumps({"guiding": guiding_leg(gctx, "glossy_cornell", a.guide_size, a.guide_ref_spp)}), flush=True)
+        return
 
     progress("building the displaced-sphere scene")
     sd = scenes.displaced_sphere(width=a.size, height=a.size, spp=a.spp)
@@ -381,15 +422,6 @@ def main():
     W, H = cam["width"], cam["height"]
     film = torch.zeros(H * W * 4, dtype=torch.float32, device="cuda")
 
-    # one counted render: traversal work per launch (not timed)
-    progress("counted render")
-    ctx.set_profiling(rtg.PROFILE_COUNT)
-    film.zero_()
-    st = scene.render_device(cam, rp, film.data_ptr())
-    counted = dict(ext_rays=st.extension_rays, nodes=st.nodes_visited, prims=st.prims_tested,
-                   sh_rays=st.shadow_rays, sh_nodes=st.shadow_nodes_visited, sh_prims=st.shadow_prims_tested,
-                   iters=st.iterations)
-
     # launch configuration for this scene (pool size, traversal waves and refill) from the autotuner
     if a.tuning_from:
         with open(a.tuning_from) as f:
@@ -400,6 +432,19 @@ def main():
     else:
         progress("autotune")
         tuning, tune_rate = ctx.autotune(scene, cam, rp, trial_samples=1 << 24)
+    if a.pool:
+        ctx.set_tuning(pool_slots=a.pool)
+        tuning = ctx.tuning()
+
+    # one counted render at the chosen configuration (the pool size sets the rays per launch):
+    # traversal work per launch (not timed)
+    progress("counted render")
+    ctx.set_profiling(rtg.PROFILE_COUNT)
+    film.zero_()
+    st = scene.render_device(cam, rp, film.data_ptr())
+    counted = dict(ext_rays=st.extension_rays, nodes=st.nodes_visited, prims=st.prims_tested,
+                   sh_rays=st.shadow_rays, sh_nodes=st.shadow_nodes_visited, sh_prims=st.shadow_prims_tested,
+                   iters=st.iterations)
 
     ctx.set_profiling(rtg.PROFILE_EVENTS)
     for _ in range(a.warmup):
@@ -436,12 +481,16 @@ def main():
         with open(tpath) as f:
             tj = json.load(f)
         if tj.get("workload") == workload:
-            traffic = float(tj["hbm_bytes_per_dispatch"])
+            per_pool = tj.get("hbm_bytes_per_dispatch_by_pool", {})
+            if str(tuning["pool_slots"]) in per_pool:
+                traffic = float(per_pool[str(tuning["pool_slots"])])
 
     # roofline of the dominant kernel (extension traversal)
     ext_bytes = (counted["ext_rays"] * RAY_IO_BYTES + counted["nodes"] * NODE_BYTES
                  + counted["prims"] * TRI_BYTES)
     bytes_per_launch = ext_bytes / max(1, counted["iters"])
+    path_bytes = (counted["ext_rays"] * SEGMENT_IO_BYTES + (counted["nodes"] + counted["sh_nodes"]) * NODE_BYTES
+                  + (counted["prims"] + counted["sh_prims"]) * TRI_BYTES)
     avg_launch_s = stage_ms[1] / max(1, launches[1]) / 1e3
     achieved = bytes_per_launch / avg_launch_s / 1e9 if avg_launch_s > 0 else 0.0
     out = {
@@ -482,19 +531,33 @@ def main():
             "per_ray": {"nodes": counted["nodes"] / max(1, counted["ext_rays"]),
                         "prims": counted["prims"] / max(1, counted["ext_rays"])},
         },
+        # the whole step under the per-segment byte model of the path (292 B of ray, hit, path-state
+        # and shadow-ray I/O per extension segment, plus node and triangle bytes of both
+        # traversals; node bytes at this BVH's 64-byte nodes) over the step's wall time
+        "path_model": {
+            "bytes_per_step": path_bytes,
+            "achieved": path_bytes / (ms_per_step / 1e3) / 1e9,
+            "unit": "GB/s",
+            "frac": path_bytes / (ms_per_step / 1e3) / 1e9 / HBM_PEAK_GBS,
+            "segment_io_bytes": SEGMENT_IO_BYTES, "node_bytes": NODE_BYTES, "tri_bytes": TRI_BYTES,
+        },
     }
     if not a.no_guiding:
         progress("guided vs plain (glossy-indirect scene)")
-        out["guiding"] = guiding_leg(ctx, "glossy_cornell", a.guide_size, a.guide_ref_spp,
-                                     ladder=(64, 256, 1024, 4096, 16384, 65536, 131072), until_target=True)
+        out["guiding"] = guiding_leg(ctx, "glossy_cornell", a.guide_size, a.guide_ref_spp)
         progress("guided vs plain (indirect-only Lambertian variant)")
-        out["guiding_indirect"] = guiding_leg(ctx, "indirect_cornell", a.guide_size, a.guide_ref_spp)
+        out["guiding_indirect"] = guiding_leg(ctx, "indirect_cornell", a.guide_size, a.guide_ref_spp,
+                                              piece_spp=1024, max_spp=32768, methods=(("plain", 0), ("guided", 1)))
     if not a.no_volume:
         progress("volumetric leg (heterogeneous medium, 512x512 128spp)")
         out["volumetric"] = volumetric_leg(ctx, not a.no_cpu, a.volume_stride)
     if not a.no_serving:
         progress(f"serving mix ({a.serve_requests} requests)")
+        # latency over throughput: the serving loop runs at the default pool of 2^21 path slots
+        # (the benchmark render's tuned pool batches more work per iteration and raises p50)
+        ctx.set_tuning(pool_slots=1 << 21)
         out["serving"] = serving(ctx, (sd, scene), a.serve_requests)
+        out["serving"]["tuning"] = ctx.tuning()
     if not a.no_cpu:
         progress("cpu baseline (oracle)")
         out["cpu_baseline"] = cpu_baseline(sd, rp["spp"], a.cpu_stride)

@@ -56,6 +56,11 @@ enum { RTG_SAMPLER_RANDOM = 0, RTG_SAMPLER_STRATIFIED = 1 };
 enum { RTG_TONEMAP_CLAMP = 0, RTG_TONEMAP_GAMMA = 1, RTG_TONEMAP_REINHARD = 2 };
 /* integrator/PathTracingMIS.cpp, integrator/VolPathTracing.cpp */
 enum { RTG_INTEGRATOR_PATH_MIS = 0, RTG_INTEGRATOR_VOLPATH = 1 };
+/* rtg_render_desc.guiding (no reference counterpart): RTG_GUIDING_OFF is the reference's estimator;
+ * RTG_GUIDING_LEARNED adds the online-trained vMF-mixture guide at camera vertices and the
+ * rough-dielectric exit proposals; RTG_GUIDING_EXIT the exit proposals alone. Both guided modes
+ * keep the plain estimator's expectation. */
+enum { RTG_GUIDING_OFF = 0, RTG_GUIDING_LEARNED = 1, RTG_GUIDING_EXIT = 2 };
 
 /* Constant-texture material parameters (texture/ConstantTexture.h:6-37).
  * albedo: Diffuse albedo, or reflectanceTexture of the rough BSDFs.
@@ -169,10 +174,15 @@ typedef struct rtg_render_desc {
   int32_t min_depth;
   int32_t max_depth;
   uint64_t seed;
-  int32_t guiding;             /* 0 = plain MIS path tracing; 1 = learned path guiding at the camera
-                                  vertices (rtg_render / rtg_render_device only; no reference
-                                  counterpart): a vMF-mixture network trained online on this
-                                  render's own paths, one-sample MIS with BSDF sampling */
+  int32_t guiding;             /* RTG_GUIDING_* (no reference counterpart). OFF: plain MIS path
+                                  tracing. LEARNED: a vMF-mixture network trained online on this
+                                  render's own paths guides diffuse and rough-conductor camera
+                                  vertices (one-sample MIS with BSDF sampling; rtg_render /
+                                  rtg_render_device only), plus EXIT's proposals. EXIT: at
+                                  rough-dielectric vertices on the denser side, one-sample MIS of
+                                  the sampler with two proposals for the normals near the critical
+                                  angle and near grazing, where the sampler's weights are unbounded
+                                  (also served). PathTracingMIS only. */
   int32_t use_camera_matrix;   /* 1: camera_to_world (row-major, the XML camera's toWorld, e.g. a
                                   lookAt with a mirroring scale) replaces origin/target/up */
   float camera_to_world[16];
@@ -344,6 +354,9 @@ int rtg_sampler_batch(rtg_ctx* ctx, const rtg_render_desc* desc, const uint32_t*
  * op 2 (diffuse and rough conductor; used by guided sampling, no reference counterpart): for
  *   (wi, wo) the density with which op 1 produces wo -> pdf, the weight op 1 returns for wo -> f,
  *   the pdf op 1 reports for wo -> rel_eta.
+ * op 3 (rough dielectric; used by guided renders, no reference counterpart): the exit proposals
+ *   with u (n x 3) = (technique choice, ux, uy) -> wo, f = the MIS-weighted estimate, pdf = the
+ *   pdf the sampler reports for wo, rel_eta = the MIS factor applied to the sampler's weight.
  * Smooth conductors and dielectrics: op 0 gives f = 0 and pdf = 0 (their eval and pdf in the
  * solid-angle measure), op 1 the mirror / refracted direction with pdf 1, fr or 1 - fr. */
 int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const float* wi, const float* wo_or_u,

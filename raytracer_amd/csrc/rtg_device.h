@@ -868,10 +868,11 @@ __device__ __forceinline__ float dielectric_pdf(const DevMaterial& m, f3 wi, f3 
   float fr = average(est_dielectric(m.eta_ext, m.eta_int, dot(wi, wt)));
   return diel_pdf_refr(m, wi, wo) * (1.f - fr);
 }
-__device__ __forceinline__ bool dielectric_sample(const DevMaterial& m, f3 wi, float ux, float uy, BsdfSample& bs) {
+// The sampler after its microfacet normal n is drawn (with alpha' = a from ux, uy); ux, which chose
+// n's polar angle, also chooses reflection (ux <= Fresnel at wi.n) or refraction.
+__device__ __forceinline__ bool dielectric_sample_n(const DevMaterial& m, f3 wi, float a, f3 n, float ux,
+                                                    BsdfSample& bs) {
   float ci = wi.z;
-  float a = adj_alpha(m.alpha, ci);
-  f3 n = mf_sample(m.dist, ux, uy, a);
   float etaI, etaT, rel, cosT;
   float fr = average(est_dielectric(m.eta_ext, m.eta_int, dot(wi, n), etaI, etaT, rel, cosT));
   bs.rel_eta = 1.f;
@@ -896,6 +897,92 @@ __device__ __forceinline__ bool dielectric_sample(const DevMaterial& m, f3 wi, f
   bs.pdf = diel_pdf_refr(m, wi, bs.wo) * (1.f - fr);
   if (bs.pdf <= 0.f) return false;
   bs.f = diel_eval_refr(m, wi, bs.wo) * fabsf(co) / bs.pdf;
+  return true;
+}
+__device__ __forceinline__ bool dielectric_sample(const DevMaterial& m, f3 wi, float ux, float uy, BsdfSample& bs) {
+  float a = adj_alpha(m.alpha, wi.z);
+  return dielectric_sample_n(m, wi, a, mf_sample(m.dist, ux, uy, a), ux, bs);
+}
+
+// ---------------------------------------------------------------- rough-dielectric exit proposals
+// (no counterpart in the reference; used by guided renders, rtg_render_desc.guiding 1 and 2)
+// The sampler above is a map from the microfacet normal n (density D_a(n) n.z, a = alpha') to a
+// weight w(n), so the plain estimator integrates w(n) D_a(n) n.z over normals. Where wi lies on the
+// denser side and the macro-surface transmits, w is unbounded in two narrow bands of c = wi.n:
+//   * near the critical angle, x = -c - cos(theta_c) -> 0+: refraction through n has 1 - F ~ sqrt(x)
+//     in the pdf's denominator while the BSDF value carries the macro-surface Fresnel;
+//   * at grazing front-facing normals, c -> 0+: the sampler takes n for an entering facet,
+//     1 - F ~ c.
+// Refraction there needs ux > F, i.e. steep normals where D_a is small, so these samples are rare
+// and carry weights up to 10^3-10^6: the dominant fireflies of rough glass. In both bands the
+// integrand w D_a n.z grows like 1 / sqrt(distance to the band's edge), so two proposals draw n
+// around wi with densities of that shape (azimuth uniform, solid angle dc dphi):
+//   critical: x = B1 s^2, density 1 / (2 sqrt(B1 x)) / 2pi on x in (0, B1];
+//   grazing:  c = G1 s^2, density 1 / (2 sqrt(G1 c)) / 2pi on c in (0, G1].
+// One-sample MIS (balance heuristic) in the space of normals over {sampler, critical, grazing}
+// with probabilities {1 - P1 - P2, P1, P2}: the estimate for a normal n from any of them is
+// w(n) D_a(n) n.z / sum_k P_k p_k(n) (at most 1 / (1 - P1 - P2) times the sampler's weight),
+// whose expectation is the plain estimator's. A proposed normal goes through the sampler's own map
+// (dielectric_sample_n with the ux that mf_sample would have needed for it), so every outcome,
+// weight and reported pdf is one the sampler can produce. The bands' densities are evaluated with
+// their edge distance floored at 1e-10 and 1e-12 (and extended over the float rounding of the
+// edge: est_dielectric decides total internal reflection from sqrt(1 - c^2), not from x), so a
+// normal the sampler transmits at the very edge never meets a zero band density.
+constexpr float kExitP1 = 0.05f;   // probability of the critical-band proposal
+constexpr float kExitP2 = 0.10f;   // probability of the grazing-band proposal
+constexpr float kExitB1 = 0.1f;    // width of the critical band in x
+constexpr float kExitG1 = 0.1f;    // width of the grazing band in c
+
+// the first number mf_sample needs to produce a normal of this polar angle (its inverse)
+__device__ __forceinline__ float mf_u_of(int dist, f3 n, float a) {
+  const float t2 = tan2(n);
+  const float u = dist == 0 ? -expm1f(-t2 / (a * a)) : t2 / (a * a + t2);
+  return smin(u, ONE_MINUS_EPS);
+}
+// densities (solid angle of n) of the two band proposals around wi, for cos_c = cos(theta_c)
+__device__ __forceinline__ float exit_band_density(float c, float cos_c, float& p2) {
+  float x = -c - cos_c;
+  if (x <= 1e-10f && x > -1e-5f) x = 1e-10f;
+  const float p1 = (x > 0.f && x <= kExitB1) ? 1.f / (2.f * sqrtf(kExitB1 * x)) * (0.5f * INV_PI) : 0.f;
+  p2 = (c > -1e-7f && c <= kExitG1) ? 1.f / (2.f * sqrtf(kExitG1 * smax(c, 1e-12f))) * (0.5f * INV_PI) : 0.f;
+  return p1;
+}
+// us picks the technique; (ux, uy) drive it. factor (optional): the MIS factor applied to the
+// sampler's weight, D_a(n) n.z / sum_k P_k p_k(n).
+__device__ __forceinline__ bool dielectric_exit_mis(const DevMaterial& m, f3 wi, float us, float ux, float uy,
+                                                    BsdfSample& bs, float* factor = nullptr) {
+  const float ci = wi.z;
+  const float eI = ci < 0.f ? m.eta_int : m.eta_ext, eT = ci < 0.f ? m.eta_ext : m.eta_int;
+  const float a = adj_alpha(m.alpha, ci);
+  if (factor) *factor = 1.f;
+  // only where wi is on the denser side and the macro-surface transmits (elsewhere plain sampling)
+  if (!(eI > eT) || est_dielectric(m.eta_ext, m.eta_int, ci) >= 1.f) return dielectric_sample(m, wi, ux, uy, bs);
+  const float rr = eT / eI;
+  const float cos_c = sqrtf(smax(0.f, 1.f - rr * rr));
+  f3 n;
+  float un = ux;
+  if (us < kExitP1 + kExitP2) {
+    float c;
+    if (us < kExitP1) c = -smin(cos_c + kExitB1 * ux * ux, 1.f);
+    else c = kExitG1 * ux * ux;
+    const float sc = sqrtf(smax(0.f, 1.f - c * c));
+    float sp, cp;
+    sincosf(2.f * PI * uy, &sp, &cp);
+    n = to_world(make_frame(wi), mk(sc * cp, sc * sp, c));
+    if (!(n.z > 0.f)) return false;  // D_a(n) = 0: the integrand vanishes
+    un = mf_u_of(m.dist, n, a);
+  } else {
+    n = mf_sample(m.dist, ux, uy, a);
+  }
+  if (!dielectric_sample_n(m, wi, a, n, un, bs)) return false;
+  const float ps = mf_D(m.dist, n, a) * n.z;
+  float p2;
+  const float p1 = exit_band_density(dot(wi, n), cos_c, p2);
+  const float den = (1.f - kExitP1 - kExitP2) * ps + kExitP1 * p1 + kExitP2 * p2;
+  if (!(den > 0.f) || !(ps > 0.f)) return false;
+  const float k = ps / den;
+  bs.f = bs.f * k;
+  if (factor) *factor = k;
   return true;
 }
 

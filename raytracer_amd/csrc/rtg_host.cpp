@@ -774,6 +774,8 @@ DevJob make_job(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc& d, float4
   J.active = 1;
   J.tiles_x = (d.width + 7) / 8;
   J.integrator = d.integrator;
+  if (d.guiding < RTG_GUIDING_OFF || d.guiding > RTG_GUIDING_EXIT) fail(RTG_ERR_INVALID, "unknown guiding mode");
+  J.exit_mis = (d.guiding != RTG_GUIDING_OFF && d.integrator == RTG_INTEGRATOR_PATH_MIS) ? 1 : 0;
   float lut[kFilterLut];
   filter_setup(d, J, lut);
   (void)ctx;
@@ -1051,10 +1053,10 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   if (ctx->server)
     fail(RTG_ERR_INVALID, "the context is serving requests, which do not take guided or all-lights renders");
   hip_check(hipSetDevice(ctx->device), "hipSetDevice");
-  const bool guided = desc->guiding != 0;
+  const bool guided = desc->guiding == RTG_GUIDING_LEARNED;  // the network (exit proposals: per job)
   const bool vol = desc->integrator == RTG_INTEGRATOR_VOLPATH;
   if (desc->integrator != RTG_INTEGRATOR_PATH_MIS && !vol) fail(RTG_ERR_INVALID, "unknown integrator");
-  if (vol && guided) fail(RTG_ERR_INVALID, "guiding is not available with the volumetric integrator");
+  if (vol && desc->guiding) fail(RTG_ERR_INVALID, "guiding is not available with the volumetric integrator");
   // all_lights: PathTracingMIS samples every light; VolPathTracing always samples one
   // (Integrator::sampleAttenuatedLightDirect, with uniform selection for this strategy)
   const bool all_lights = scene->all_lights && !vol;
@@ -1229,10 +1231,11 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   }
 }
 
-// With a server on the context, a render that the server can take (not guided, not all-lights
-// PathTracingMIS) joins its mix as a request and the call waits for it; other renders fail.
+// With a server on the context, a render that the server can take (no learned guiding, not
+// all-lights PathTracingMIS) joins its mix as a request and the call waits for it; other renders fail.
 static bool servable(const rtg_scene* scene, const rtg_render_desc* desc) {
-  return scene && desc && !desc->guiding && !(scene->all_lights && desc->integrator == RTG_INTEGRATOR_PATH_MIS);
+  return scene && desc && desc->guiding != RTG_GUIDING_LEARNED &&
+         !(scene->all_lights && desc->integrator == RTG_INTEGRATOR_PATH_MIS);
 }
 
 int rtg_autotune(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, uint64_t trial_samples, rtg_tuning* best,
@@ -1290,7 +1293,7 @@ int rtg_autotune(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, ui
         const double r = rate(t);
         if (r > r_best) { r_best = r; t_best = t; }
       }
-      for (int p : {1 << 20, 3 << 19, 1 << 21, 3 << 20, 1 << 22}) {
+      for (int p : {1 << 20, 3 << 19, 1 << 21, 3 << 20, 1 << 22, 3 << 21, 1 << 23}) {
         rtg_tuning t = t_best;
         t.pool_slots = p;
         if (p == t_best.pool_slots) continue;
@@ -1555,9 +1558,11 @@ int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const
   if (!ctx) return RTG_ERR_INVALID;
   std::lock_guard<std::mutex> lk(ctx->mtx);
   RTG_GUARD(ctx, {
-    if (!material || !wi || !wo_or_u || !f || !pdf || n < 0 || op < 0 || op > 2)
+    if (!material || !wi || !wo_or_u || !f || !pdf || n < 0 || op < 0 || op > 3)
       fail(RTG_ERR_INVALID, "bad bsdf arguments");
-    if (op == 1 && (!out_dir || !rel_eta)) fail(RTG_ERR_INVALID, "sample needs out_dir and rel_eta");
+    if ((op == 1 || op == 3) && (!out_dir || !rel_eta)) fail(RTG_ERR_INVALID, "sample needs out_dir and rel_eta");
+    if (op == 3 && material->type != RTG_MAT_ROUGH_DIELECTRIC)
+      fail(RTG_ERR_INVALID, "exit proposals need a rough dielectric");
     if (op == 2 && (!rel_eta || (material->type != RTG_MAT_DIFFUSE && material->type != RTG_MAT_ROUGH_CONDUCTOR)))
       fail(RTG_ERR_INVALID, "sampling density needs rel_eta and a diffuse or conductor material");
     if (n == 0) return RTG_OK;
@@ -1566,7 +1571,7 @@ int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const
     std::vector<void*> own;
     DevMaterial* dm = (DevMaterial*)dev_alloc(ctx, sizeof(DevMaterial), own);
     float* dwi = (float*)dev_alloc(ctx, (size_t)n * 12, own);
-    size_t w2 = (size_t)n * (op == 1 ? 8 : 12);
+    size_t w2 = (size_t)n * (op == 1 ? 8 : 12);  // wo or (ux, uy); op 3: (technique, ux, uy)
     float* dwo = (float*)dev_alloc(ctx, w2, own);
     float* ddir = (float*)dev_alloc(ctx, (size_t)n * 12, own);
     float* df = (float*)dev_alloc(ctx, (size_t)n * 12, own);
@@ -1580,7 +1585,7 @@ int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const
       hip_check(hipGetLastError(), "bsdf launch");
       hip_check(hipMemcpyAsync(f, df, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "f");
       hip_check(hipMemcpyAsync(pdf, dp, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "pdf");
-      if (op == 1)
+      if (op == 1 || op == 3)
         hip_check(hipMemcpyAsync(out_dir, ddir, (size_t)n * 12, hipMemcpyDeviceToHost, ctx->stream), "dir");
       if (op >= 1)
         hip_check(hipMemcpyAsync(rel_eta, de, (size_t)n * 4, hipMemcpyDeviceToHost, ctx->stream), "eta");

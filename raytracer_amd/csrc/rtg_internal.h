@@ -132,7 +132,7 @@ struct DevJob {
   float rx, ry;
   const float* lut;
   int32_t integrator;   // RTG_INTEGRATOR_PATH_MIS or RTG_INTEGRATOR_VOLPATH
-  int32_t pad_j;
+  int32_t exit_mis;     // 1: rough-dielectric exit vertices use the band proposals (dielectric_exit_mis)
 };
 static const int kFilterLut = 32 * 32;
 // lights per scene the all-lights strategy supports (shadow rays per vertex)

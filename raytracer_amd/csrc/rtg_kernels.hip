@@ -697,7 +697,9 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
         sample2d(rng, 2u + 2u * depth, bx, by);
         BsdfSample bs;
         bool ok;
-        if (GUIDED && guided_vertex(MT, depth) && (int)fl_job(f) == G.job && !is_delta(M)) {
+        if (MT == 2 && J.exit_mis && M.type == MAT_ROUGH_DIELECTRIC) {
+          ok = dielectric_exit_mis(M, wo_l, sample1d(rng, kGuideSelDim + depth), bx, by, bs);
+        } else if (GUIDED && guided_vertex(MT, depth) && (int)fl_job(f) == G.job && !is_delta(M)) {
           // one-sample MIS of BSDF sampling and the guide, each chosen with probability 1/2:
           // weight = (sampler weight * sampler density) / mixture density, so the estimator keeps
           // the plain tracer's expectation; the MIS weights of the next hit use the BSDF's pdf.
@@ -1295,6 +1297,21 @@ __global__ void k_bsdf_tap(const DevMaterial* mat, int op, const float* wi, cons
     f[3 * i] = w.x; f[3 * i + 1] = w.y; f[3 * i + 2] = w.z;
     pdf[i] = ps;
     rel_eta[i] = pref;
+  } else if (op == 3) {
+    // rough-dielectric exit proposals (dielectric_exit_mis) with u = (technique, ux, uy): the
+    // MIS-weighted estimate -> f, the pdf the sampler reports -> pdf, the MIS factor -> rel_eta
+    const float us = wo_or_u[3 * i], ux = wo_or_u[3 * i + 1], uy = wo_or_u[3 * i + 2];
+    BsdfSample bs;
+    bs.wo = mk(0.f, 0.f, 0.f);
+    bs.f = mk(0.f, 0.f, 0.f);
+    bs.pdf = -1.f;
+    bs.rel_eta = 1.f;
+    float k = 0.f;
+    if (!dielectric_exit_mis(M, a, us, ux, uy, bs, &k)) { bs.f = mk(0.f, 0.f, 0.f); k = 0.f; }
+    out_dir[3 * i] = bs.wo.x; out_dir[3 * i + 1] = bs.wo.y; out_dir[3 * i + 2] = bs.wo.z;
+    f[3 * i] = bs.f.x; f[3 * i + 1] = bs.f.y; f[3 * i + 2] = bs.f.z;
+    pdf[i] = bs.pdf;
+    rel_eta[i] = k;
   } else {
     float ux = wo_or_u[2 * i], uy = wo_or_u[2 * i + 1];
     BsdfSample bs;

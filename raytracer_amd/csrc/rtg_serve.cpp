@@ -308,7 +308,8 @@ int submit_impl(rtg_server* s, rtg_scene* scene, const rtg_render_desc* desc, fl
   std::lock_guard<std::mutex> lk(s->qm);
   try {
     // validate now so a bad request fails at submission
-    if (desc->guiding) fail(RTG_ERR_INVALID, "guided renders are not served (use rtg_render without a server)");
+    if (desc->guiding == RTG_GUIDING_LEARNED)
+      fail(RTG_ERR_INVALID, "renders with learned guiding are not served (use rtg_render without a server)");
     if (desc->integrator != RTG_INTEGRATOR_PATH_MIS && desc->integrator != RTG_INTEGRATOR_VOLPATH)
       fail(RTG_ERR_INVALID, "unknown integrator");
     // all_lights: PathTracingMIS samples every light; VolPathTracing samples one whatever the strategy

@@ -20,6 +20,7 @@ FILTER_BOX, FILTER_GAUSSIAN, FILTER_MITCHELL = 0, 1, 2
 LIGHTS_DEFAULT, LIGHTS_UNIFORM, LIGHTS_WEIGHTED, LIGHTS_ALL = 0, 1, 2, 3
 SAMPLER_RANDOM, SAMPLER_STRATIFIED = 0, 1
 INTEGRATOR_PATH_MIS, INTEGRATOR_VOLPATH = 0, 1
+GUIDING_OFF, GUIDING_LEARNED, GUIDING_EXIT = 0, 1, 2
 TONEMAP_CLAMP, TONEMAP_GAMMA, TONEMAP_REINHARD = 0, 1, 2
 
 
@@ -409,6 +410,18 @@ class Context:
         self.check(self.lib.rtg_bsdf_batch(self.h, C.byref(m), 1, _ptr(wi), _ptr(u), n, _ptr(wo), _ptr(f),
                                            _ptr(pdf), _ptr(eta)))
         return wo, f, pdf, eta
+
+    def exit_mis_sample(self, material, wi, u):
+        """Rough-dielectric exit proposals, u = (technique, ux, uy) -> (wo, MIS-weighted estimate,
+        reported pdf, MIS factor)."""
+        wi = _f32(wi, (-1, 3)); u = _f32(u, (-1, 3))
+        n = wi.shape[0]
+        wo = np.zeros((n, 3), np.float32); f = np.zeros((n, 3), np.float32)
+        pdf = np.zeros(n, np.float32); k = np.zeros(n, np.float32)
+        m = material_struct(material)
+        self.check(self.lib.rtg_bsdf_batch(self.h, C.byref(m), 3, _ptr(wi), _ptr(u), n, _ptr(wo), _ptr(f),
+                                           _ptr(pdf), _ptr(k)))
+        return wo, f, pdf, k
 
 
 class Scene:

@@ -21,7 +21,7 @@ BUILD = os.path.join(HERE, "_build")
 @pytest.fixture(scope="module")
 def trav_check():
     os.makedirs(BUILD, exist_ok=True)
-    exe = os.path.join(BUILD, "trav_check")
+    exe = os.path.join(BUILD, f"trav_check.{os.getpid()}")  # per process: parallel workers build their own
     cmd = ["g++", "-std=c++17", "-O1", "-g", "-fsanitize=address,undefined", "-fno-sanitize-recover=undefined",
            "-include", os.path.join(HERE, "cpu_trav", "device_cpu.h"), "-I" + os.path.join(HERE, "mock_hip"),
            "-I" + CSRC, os.path.join(HERE, "cpu_trav", "trav_check.cpp"), os.path.join(CSRC, "rtg_bvh.cpp"),
@@ -54,7 +54,7 @@ def test_sampler_strata_arithmetic():
     compiled for the host: every stratified value stays inside its cell in exact arithmetic (also
     for jitters within an ulp of 1, where the float sum cell + jitter rounds up to the next cell),
     and the permutation is a bijection, for sample counts up to 4225."""
-    exe = os.path.join(BUILD, "sampler_check")
+    exe = os.path.join(BUILD, f"sampler_check.{os.getpid()}")
     os.makedirs(BUILD, exist_ok=True)
     cmd = ["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-include", os.path.join(HERE, "cpu_trav", "device_cpu.h"),
            "-I" + os.path.join(HERE, "mock_hip"), "-I" + CSRC, os.path.join(HERE, "cpu_trav", "sampler_check.cpp"),
