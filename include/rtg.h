@@ -323,7 +323,7 @@ int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int3
  * that cannot allocate device memory (code under hipGraph capture, tight serving loops, harnesses
  * that forbid hipMalloc). rays_device (n x 8 floats) and the outputs are caller-owned DEVICE
  * buffers laid out as rtg_trace's (t, uv, p, nrm may be NULL); the call allocates nothing: the
- * traversal's scratch (its stack spill area and two counters, 16 MiB + 256 bytes) is a static
+ * traversal's scratch (its stack spill area and two counters, 18 MiB + 256 bytes) is a static
  * device array of the library, one per GPU, held until the process ends and shared by every call
  * on that GPU, so calls that overlap on one GPU (from different contexts) race. A scene whose
  * traversal stack bound (rtg_scene_info.traversal_stack) exceeds 48 entries fails with

@@ -293,24 +293,6 @@ __device__ __forceinline__ bool slab(const Ray& r, float bx0, float by0, float b
   nearT = nT;
   return (r.tmin <= fT && r.tmax >= nT);
 }
-// Same result as slab() when no direction component is 0: the per-axis early exits are folded
-// into one flag (nearT never decreases and farT never increases, so a failed axis stays failed;
-// the flag is taken per axis so NaN distances fail exactly where slab() fails).
-__device__ __forceinline__ bool slab_fast(const Ray& r, float bx0, float by0, float bz0, float bx1, float by1,
-                                          float bz1, float& nearT) {
-  // The per-axis entry/exit distances as in core/BoundingBox.h:154-183; the reference's running
-  // clamps (nT <= fT checked after every axis) equal one check of the overall max/min, since nT
-  // only grows and fT only shrinks. Same result as slab() for finite distances (rays with a zero
-  // direction component take slab()).
-  const float a1 = (bx0 - r.o.x) * r.inv.x, a2 = (bx1 - r.o.x) * r.inv.x;
-  const float b1 = (by0 - r.o.y) * r.inv.y, b2 = (by1 - r.o.y) * r.inv.y;
-  const float c1 = (bz0 - r.o.z) * r.inv.z, c2 = (bz1 - r.o.z) * r.inv.z;
-  const float nT = fmaxf(fmaxf(fminf(a1, a2), fminf(b1, b2)), fmaxf(fminf(c1, c2), -MAX_REAL));
-  const float fT = fminf(fminf(fmaxf(a1, a2), fmaxf(b1, b2)), fminf(fmaxf(c1, c2), MAX_REAL));
-  nearT = nT;
-  return nT <= fT && r.tmin <= fT && r.tmax >= nT;
-}
-
 // geometricShape/Triangle.cpp:46-89
 __device__ __forceinline__ bool tri_hit(const Ray& r, const float* v, float& t, float& b1, float& b2) {
   f3 p0 = mk(v[0], v[1], v[2]);
@@ -364,7 +346,6 @@ struct HitRec {
 // conflict-free), deeper entries spill to a global buffer ([entry][grid thread] layout, coalesced).
 // An entry is a subtree ref plus the entry distance of its box, so a popped subtree that starts
 // beyond the current closest hit is skipped without fetching its node.
-constexpr int kLdsStack = 16;
 struct TravStack {
   uint2* l;        // LDS entries (ref, entry distance bits), this lane's column
   int lstride;     // threads per block
@@ -453,7 +434,10 @@ __device__ __forceinline__ bool trav_begin(const DevScene& S, TravState& s) {
   if (!slab(s.r, S.root_min[0], S.root_min[1], S.root_min[2], S.root_max[0], S.root_max[1], S.root_max[2], tn))
     return false;
   s.ref = S.root_ref;
-  s.zdir = has_zero_dir(s.r);
+  // the reference's per-axis branch also takes directions whose reciprocal overflowed (the fused
+  // node test below needs finite reciprocals)
+  s.zdir = has_zero_dir(s.r) || !(fabsf(s.r.inv.x) < INFINITY && fabsf(s.r.inv.y) < INFINITY &&
+                                   fabsf(s.r.inv.z) < INFINITY);
   return true;
 }
 
@@ -479,30 +463,76 @@ __device__ __forceinline__ float4 dq4(uint32_t w, float origin, float step) {
   return make_float4(origin + (float)(w & 0xFFu) * step, origin + (float)((w >> 8) & 0xFFu) * step,
                      origin + (float)((w >> 16) & 0xFFu) * step, origin + (float)(w >> 24) * step);
 }
+// Fused test of the four children for rays without zero direction components: per axis, with
+// A = step / d and B = (origin - o) / d, a child bound q gives the entry or exit distance q A + B
+// (one fma after the byte conversion instead of a decode and a slab), and the bytes of the near
+// and far bound of every child are chosen once per node by the sign of 1 / d. The distances are
+// widened by delta = 2^-20 (|origin / d| + |B| + 256 |A|), which bounds the rounding difference
+// to the reference's slab test on the exact child box (which the decoded box contains) for these
+// operations, so the test accepts every box the reference's would: it only culls, and hits are
+// decided by the triangle test.
+__device__ __forceinline__ void fused_axis(float o, float step, float ro, float inv, uint32_t qlo, uint32_t qhi,
+                                           float& A, float& Bn, float& Bf, uint32_t& qn, uint32_t& qf) {
+  A = step * inv;
+  const float B = (o - ro) * inv;
+  const float delta = 0x1p-20f * (fabsf(o * inv) + fabsf(B) + 256.f * fabsf(A));
+  Bn = B - delta;
+  Bf = B + delta;
+  qn = inv >= 0.f ? qlo : qhi;
+  qf = inv >= 0.f ? qhi : qlo;
+}
+__device__ __forceinline__ bool fused_child(const Ray& r, float qnx, float qfx, float qny, float qfy, float qnz,
+                                            float qfz, float Ax, float Bnx, float Bfx, float Ay, float Bny,
+                                            float Bfy, float Az, float Bnz, float Bfz, float& nearT) {
+  const float nT = fmaxf(fmaxf(fmaf(qnx, Ax, Bnx), fmaf(qny, Ay, Bny)), fmaf(qnz, Az, Bnz));
+  const float fT = fminf(fminf(fmaf(qfx, Ax, Bfx), fmaf(qfy, Ay, Bfy)), fmaf(qfz, Az, Bfz));
+  nearT = nT;
+  return nT <= fT && r.tmin <= fT && r.tmax >= nT;
+}
+__device__ __forceinline__ float qbyte(uint32_t w, int j) { return (float)((w >> (8 * j)) & 0xFFu); }
+
+// Box tests of the four children of a compressed node (header hd, bounds q0, q1): hit flags and
+// entry distances. zdir: the ray has a zero direction component (or an overflowed reciprocal) and
+// takes the reference's per-axis slab test on the decoded boxes.
+__device__ __forceinline__ void node_children(const uint4& hd, const uint4& q0, const uint4& q1, const Ray& r,
+                                              bool zdir, bool& h0, bool& h1, bool& h2, bool& h3, float& t0,
+                                              float& t1, float& t2, float& t3) {
+  const float ox = __uint_as_float(hd.x), oy = __uint_as_float(hd.y), oz = __uint_as_float(hd.z);
+  const float sx = __uint_as_float((hd.w & 0xFFu) << 23), sy = __uint_as_float(((hd.w >> 8) & 0xFFu) << 23),
+              sz = __uint_as_float(((hd.w >> 16) & 0xFFu) << 23);
+  if (!zdir) {
+    float Ax, Bnx, Bfx, Ay, Bny, Bfy, Az, Bnz, Bfz;
+    uint32_t nx, fx, ny, fy, nz, fz;
+    fused_axis(ox, sx, r.o.x, r.inv.x, q0.x, q0.w, Ax, Bnx, Bfx, nx, fx);
+    fused_axis(oy, sy, r.o.y, r.inv.y, q0.y, q1.x, Ay, Bny, Bfy, ny, fy);
+    fused_axis(oz, sz, r.o.z, r.inv.z, q0.z, q1.y, Az, Bnz, Bfz, nz, fz);
+#define RTG_CHILD(j, h, t)                                                                               \
+  h = fused_child(r, qbyte(nx, j), qbyte(fx, j), qbyte(ny, j), qbyte(fy, j), qbyte(nz, j), qbyte(fz, j), Ax, \
+                  Bnx, Bfx, Ay, Bny, Bfy, Az, Bnz, Bfz, t)
+    RTG_CHILD(0, h0, t0);
+    RTG_CHILD(1, h1, t1);
+    RTG_CHILD(2, h2, t2);
+    RTG_CHILD(3, h3, t3);
+#undef RTG_CHILD
+  } else {
+    const float4 lx = dq4(q0.x, ox, sx), ly = dq4(q0.y, oy, sy), lz = dq4(q0.z, oz, sz);
+    const float4 hx = dq4(q0.w, ox, sx), hy = dq4(q1.x, oy, sy), hz = dq4(q1.y, oz, sz);
+    h0 = slab(r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, t0);
+    h1 = slab(r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, t1);
+    h2 = slab(r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, t2);
+    h3 = slab(r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, t3);
+  }
+}
+
 __device__ __forceinline__ void trav_node(const SceneGeo& G, TravState& s, const TravStack& stk,
                                           uint32_t& nodes_visited) {
   // 64-byte compressed node: origin + step exponents, 8-bit child bounds, child refs
   const uint4* nd = reinterpret_cast<const uint4*>(G.nodes) + 4u * s.ref;
   const uint4 hd = nd[0], q0 = nd[1], q1 = nd[2], rf = nd[3];
-  const float ox = __uint_as_float(hd.x), oy = __uint_as_float(hd.y), oz = __uint_as_float(hd.z);
-  const float sx = __uint_as_float((hd.w & 0xFFu) << 23), sy = __uint_as_float(((hd.w >> 8) & 0xFFu) << 23),
-              sz = __uint_as_float(((hd.w >> 16) & 0xFFu) << 23);
-  const float4 lx = dq4(q0.x, ox, sx), ly = dq4(q0.y, oy, sy), lz = dq4(q0.z, oz, sz);
-  const float4 hx = dq4(q0.w, ox, sx), hy = dq4(q1.x, oy, sy), hz = dq4(q1.y, oz, sz);
   nodes_visited++;
   float t0, t1, t2, t3;
   bool h0, h1, h2, h3;
-  if (!s.zdir) {
-    h0 = slab_fast(s.r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, t0);
-    h1 = slab_fast(s.r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, t1);
-    h2 = slab_fast(s.r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, t2);
-    h3 = slab_fast(s.r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, t3);
-  } else {
-    h0 = slab(s.r, lx.x, ly.x, lz.x, hx.x, hy.x, hz.x, t0);
-    h1 = slab(s.r, lx.y, ly.y, lz.y, hx.y, hy.y, hz.y, t1);
-    h2 = slab(s.r, lx.z, ly.z, lz.z, hx.z, hy.z, hz.z, t2);
-    h3 = slab(s.r, lx.w, ly.w, lz.w, hx.w, hy.w, hz.w, t3);
-  }
+  node_children(hd, q0, q1, s.r, s.zdir, h0, h1, h2, h3, t0, t1, t2, t3);
   h0 = h0 && rf.x != kEmptyChild;
   h1 = h1 && rf.y != kEmptyChild;
   h2 = h2 && rf.z != kEmptyChild;

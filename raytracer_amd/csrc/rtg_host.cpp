@@ -582,7 +582,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
             "scene table");
   hip_check(hipStreamSynchronize(ctx->stream), "scene upload");
   sc->bytes = bytes;
-  sc->stack = std::min(kMaxStack, std::max(16, bvh.max_stack + 1));
+  sc->stack = std::min(kMaxStack, std::max(kLdsStack, bvh.max_stack + 1));
   sc->ntri = ntri;
   sc->nsph = nsph;
   sc->ntex = (int)texs.size();
@@ -889,7 +889,7 @@ int rtg_ctx_create(int device, rtg_ctx** out) {
     ctx->trace_blocks = std::max(32, ctx->cu_count * 2 * waves);
     // traversal stack spill area for every lane of a traversal launch of up to kMaxTraceWaves waves
     size_t lanes = (size_t)std::max(32, ctx->cu_count * 2 * kMaxTraceWaves) * 128;
-    hip_check(hipMalloc(&ctx->gstack, lanes * (kMaxStack - 16) * sizeof(uint2)), "hipMalloc stack spill");
+    hip_check(hipMalloc(&ctx->gstack, lanes * (kMaxStack - kLdsStack) * sizeof(uint2)), "hipMalloc stack spill");
     hip_check(hipMalloc(&ctx->tap_ctr, 2 * sizeof(uint32_t)), "hipMalloc trace counters");
     // device address of this GPU's static scratch array (the allocation-free trace's scratch); a
     // failed lookup leaves it null, and only rtg_trace_device_noalloc then fails

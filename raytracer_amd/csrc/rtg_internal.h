@@ -232,17 +232,22 @@ struct DevGuide {
 
 // Launch wrappers implemented in rtg_kernels.hip
 static const int kTraceRefill = 16;
-static const int kTraceWaves = 5;  // waves per SIMD of a traversal launch (default)
-static const int kMaxTraceWaves = 8;  // the spill area is sized for this many
+static const int kTraceWaves = 6;  // waves per SIMD of a traversal launch (default)
+static const int kMaxTraceWaves = 10;  // the spill area is sized for this many
 struct LaunchCfg {
   int grid;      // blocks for grid-stride kernels
   int trace_blocks;  // blocks of 128 for the persistent traversal kernels
   int block;     // threads per block
   int stack;     // traversal stack capacity (entries, >= tree height)
-  uint2* gstack; // spill area: (stack - 16) entries x (trace_blocks * 128) lanes
+  uint2* gstack; // spill area: (stack - kLdsStack) entries x (trace_blocks * 128) lanes
   int refill = kTraceRefill;  // persistent traversal: refill a wave once this many lanes are idle
 };
 static const int kMaxStack = 96;
+// traversal-stack entries of each lane kept in LDS (8 bytes each); the rest spill to global memory
+#ifndef RTG_LDS_STACK
+#define RTG_LDS_STACK 12
+#endif
+constexpr int kLdsStack = RTG_LDS_STACK;
 // ev (optional): 8 events recorded before regen and after each of the 7 stage launches
 // order_dev: [0] number of entries n, [1..n] job-table slots in claim order (free path slots go to
 // the first job with samples left); read on the device so that a captured launch stays valid
@@ -272,8 +277,8 @@ void launch_fill_zero(void* ptr, size_t bytes, hipStream_t stream);
 // trace counters (kScratchHeader bytes) and the stack spill area of up to kNoAllocBlocks blocks of
 // 128 lanes, kNoAllocSpill entries each, so scenes whose stack bound is at most kNoAllocStack.
 static const int kNoAllocBlocks = 512;
-static const int kNoAllocSpill = 32;
-static const int kNoAllocStack = 16 + kNoAllocSpill;
+static const int kNoAllocSpill = 36;
+static const int kNoAllocStack = kLdsStack + kNoAllocSpill;
 static const size_t kScratchHeader = 256;
 static const size_t kScratchBytes = kScratchHeader + (size_t)kNoAllocBlocks * 128 * kNoAllocSpill * 8;
 static const int kMaxScratchDevices = 64;

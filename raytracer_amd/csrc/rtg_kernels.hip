@@ -463,8 +463,13 @@ struct ExtSrc {
   }
 };
 
-__global__ void __launch_bounds__(128) k_extend(DevPool P, const DevScene* scenes, const DevJob* jobs,
-                                                uint2* gstack, int stack_depth, int refill) {
+// Occupancy of the persistent traversal kernels: waves per SIMD the register allocation must allow
+// (the LDS stack, kLdsStack x 8 bytes per lane, bounds it as well).
+#ifndef RTG_TRAV_WAVES_PER_EU
+#define RTG_TRAV_WAVES_PER_EU 6
+#endif
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(RTG_TRAV_WAVES_PER_EU)))
+k_extend(DevPool P, const DevScene* scenes, const DevJob* jobs, uint2* gstack, int stack_depth, int refill) {
   extern __shared__ uint2 smem[];
   __shared__ uint32_t s_w[4];
   uint32_t* C = P.counters;
@@ -1152,8 +1157,8 @@ struct ShadowSrc {
   }
 };
 
-__global__ void __launch_bounds__(128) k_shadow(DevPool P, const DevScene* scenes, const DevJob* jobs,
-                                                uint2* gstack, int stack_depth, int refill) {
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(RTG_TRAV_WAVES_PER_EU)))
+k_shadow(DevPool P, const DevScene* scenes, const DevJob* jobs, uint2* gstack, int stack_depth, int refill) {
   extern __shared__ uint2 smem[];
   __shared__ uint32_t s_w[4];
   uint32_t* C = P.counters;

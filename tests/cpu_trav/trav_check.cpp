@@ -192,6 +192,46 @@ int main(int argc, char** argv) {
       }
     }
   }
+  // The fused child test on the compressed nodes accepts every child box that the reference's slab
+  // test accepts on the builder's exact float box (it may accept a little more), and its entry
+  // distance is never larger: checked for every node and child against a subset of the rays.
+  long fused_viol = 0, fused_extra = 0, fused_tests = 0;
+  {
+    std::mt19937 rng2(seed + 77);
+    const int nprop = std::max(50, (int)std::min<size_t>((size_t)nrays, 3000000 / std::max<size_t>(1, nnodes)));
+    for (int q = 0; q < nprop; q++) {
+      Ray r;
+      float o[3], d[3];
+      for (int a = 0; a < 3; a++) o[a] = -0.2f + 1.4f * U(rng2);
+      for (int a = 0; a < 3; a++) d[a] = 2.f * U(rng2) - 1.f;
+      if (q % 5 == 0) d[q % 3] *= 1e-6f;  // nearly axis-parallel directions: large reciprocals
+      float l = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      r.o = mk(o[0], o[1], o[2]);
+      r.d = mk(d[0] / l, d[1] / l, d[2] / l);
+      r.inv = mk(1.f / r.d.x, 1.f / r.d.y, 1.f / r.d.z);
+      r.tmin = (q % 3 == 0) ? EPS : 1e-3f * U(rng2);
+      r.tmax = (q % 4 == 0) ? 0.5f * U(rng2) : 1e8f;
+      adapt_eps(r);
+      if (has_zero_dir(r)) continue;
+      for (size_t ni = 0; ni < nnodes; ni++) {
+        const float* nd = &bvh.nodes[ni * 32];
+        const uint4* qn = reinterpret_cast<const uint4*>(&bvh.qnodes[ni * 16]);
+        bool h[4];
+        float t[4];
+        node_children(qn[0], qn[1], qn[2], r, false, h[0], h[1], h[2], h[3], t[0], t[1], t[2], t[3]);
+        const uint32_t* rf = (const uint32_t*)(nd + 24);
+        for (int j = 0; j < 4; j++) {
+          if (rf[j] == kEmptyChild) continue;
+          float tr;
+          const bool hr = slab(r, nd[j], nd[4 + j], nd[8 + j], nd[12 + j], nd[16 + j], nd[20 + j], tr);
+          fused_tests++;
+          if (hr && (!h[j] || t[j] > tr)) fused_viol++;
+          if (!hr && h[j]) fused_extra++;
+        }
+      }
+    }
+  }
+  printf("fused child tests %ld: violations %ld, extra acceptances %ld\n", fused_tests, fused_viol, fused_extra);
   printf("seed %u tris %d spheres %d rays %d: nodes4 %zu depth4 %d stack bound %d max sp %d hits %ld "
          "mismatches %ld ties %ld any-hit mismatches %ld nodes/ray %.2f prims/ray %.2f\n",
          seed, ntri, nsph, nrays, nnodes, bvh.max_depth, bvh.max_stack, g_max_sp, hits, mism, ties, anymism,
@@ -201,7 +241,7 @@ int main(int argc, char** argv) {
     printf(ovf ? "OVERFLOW FLAGGED\n" : "FAIL overflow not flagged\n");
     return ovf ? 0 : 1;
   }
-  const bool ok = g_max_sp <= bvh.max_stack && mism <= nrays / 10000 && anymism <= nrays / 10000 &&
+  const bool ok = fused_viol == 0 && g_max_sp <= bvh.max_stack && mism <= nrays / 10000 && anymism <= nrays / 10000 &&
                   ties <= nrays / 1000 && hits > 0 && ovf == 0 && (g_max_sp <= kLdsStack || spills > 0);
   printf(ok ? "OK\n" : "FAIL\n");
   return ok ? 0 : 1;
