@@ -227,7 +227,7 @@ typedef struct rtg_stats {
   double guide_loss;           /* loss of the last training step */
   double guide_train_ms;       /* device time of the training steps (RTG_PROFILE_EVENTS) */
   uint64_t guide_queries;      /* network inferences (guided camera vertices) */
-  uint64_t stack_spills;       /* traversal-stack pushes past the 16 on-chip (LDS) entries into the
+  uint64_t stack_spills;       /* traversal-stack pushes past the 12 on-chip (LDS) entries into the
                                   global spill area (RTG_PROFILE_COUNT) */
 } rtg_stats;
 
@@ -331,7 +331,7 @@ int rtg_trace(rtg_ctx* ctx, rtg_scene* scene, const float* rays, int32_t n, int3
 int rtg_trace_device_noalloc(rtg_ctx* ctx, rtg_scene* scene, const float* rays_device, int32_t n, int32_t any_hit,
                              int32_t* prim_device, float* t_device, float* uv_device, float* p_device,
                              float* nrm_device);
-/* Traversal-stack pushes of the context's last rtg_trace call that went past the 16 on-chip (LDS)
+/* Traversal-stack pushes of the context's last rtg_trace call that went past the 12 on-chip (LDS)
  * entries of a ray's stack into the global spill area (no reference counterpart: the reference's
  * stack is a 64-entry array, accelStruct/Bvh.cpp:392-400). A stack deeper than the BVH builder's
  * bound fails the render or trace with RTG_ERR_HIP instead of losing geometry. */

@@ -74,7 +74,7 @@ def test_trace_displaced_full_size(ctx):
     oracle's reference-structured BVH (itself checked against brute force on a 1,500-ray subset):
     equal ids except near-ties (at most 1e-5 of rays), bit-identical t, barycentrics, point and
     normal; any-hit flags equal. This tree is deep enough that traversal stacks spill past their
-    16 on-chip entries: the spill count of the trace is asserted to be non-zero, so the global
+    on-chip (LDS) entries: the spill count of the trace is asserted to be non-zero, so the global
     spill path is covered by the parity check."""
     sd = scenes.displaced_sphere(level=8)
     rng = np.random.default_rng(4)

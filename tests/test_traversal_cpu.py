@@ -41,8 +41,8 @@ def test_traversal_matches_brute_force(trav_check, seed, ntri, nsph, nrays):
 
 
 def test_stack_overflow_is_flagged(trav_check):
-    """A stack smaller than what the rays need (forced to the 16 on-chip entries on a 40k-triangle
-    scene whose rays reach depth 21) makes pushes find it full: the traversal raises its overflow
+    """A stack smaller than what the rays need (forced to 16 entries on a 40k-triangle scene whose
+    rays reach depth 21) makes pushes find it full: the traversal raises its overflow
     flag (which the library turns into a render error) instead of dropping subtrees silently."""
     r = subprocess.run([trav_check, "5", "40000", "0", "6000", "16"], capture_output=True, text=True, timeout=300)
     print(r.stdout)
