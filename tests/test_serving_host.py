@@ -40,7 +40,8 @@ print("ALL OK", flush=True)
 
 def build_mock_lib():
     os.makedirs(BUILD, exist_ok=True)
-    out = os.path.join(BUILD, "librtg_hostmock.so")
+    # per process: parallel test workers each build (and load) their own copy
+    out = os.path.join(BUILD, f"librtg_hostmock.{os.getpid()}.so")
     srcs = [os.path.join(CSRC, f) for f in ("rtg_serve.cpp", "rtg_host.cpp", "rtg_guide_host.cpp", "rtg_bvh.cpp",
                                             "rtg_xml.cpp")]
     srcs.append(os.path.join(MOCK, "mock_kernels.cpp"))
