@@ -524,6 +524,7 @@ __device__ __forceinline__ void node_children(const uint4& hd, const uint4& q0, 
   }
 }
 
+template <bool ANY>
 __device__ __forceinline__ void trav_node(const SceneGeo& G, TravState& s, const TravStack& stk,
                                           uint32_t& nodes_visited) {
   // 64-byte compressed node: origin + step exponents, 8-bit child bounds, child refs
@@ -540,6 +541,26 @@ __device__ __forceinline__ void trav_node(const SceneGeo& G, TravState& s, const
   const int nh = (int)h0 + (int)h1 + (int)h2 + (int)h3;
   if (nh == 0) {
     s.ref = trav_pop(stk, s);
+    return;
+  }
+  if (ANY) {
+    // Any hit: the ray interval never shrinks, so the visiting order changes neither the result nor
+    // the work much; no sort. Descend into the first hit child in slot order, push the others.
+    // (entry distances as the sorted visit keeps them: infinite or NaN ones as the largest float)
+    const bool f1 = h1 && !h0, f2 = h2 && !h0 && !h1, f3 = h3 && !h0 && !h1 && !h2;
+    const float k1 = t1 < INFINITY ? t1 : 3.0e38f, k2 = t2 < INFINITY ? t2 : 3.0e38f,
+                k3 = t3 < INFINITY ? t3 : 3.0e38f;
+    const uint32_t next = h0 ? rf.x : (h1 ? rf.y : (h2 ? rf.z : rf.w));
+    if (s.sp + 3 <= kLdsStack) {
+      if (h1 && !f1) { stk.l[s.sp * stk.lstride] = make_uint2(rf.y, __float_as_uint(k1)); s.sp++; }
+      if (h2 && !f2) { stk.l[s.sp * stk.lstride] = make_uint2(rf.z, __float_as_uint(k2)); s.sp++; }
+      if (h3 && !f3) { stk.l[s.sp * stk.lstride] = make_uint2(rf.w, __float_as_uint(k3)); s.sp++; }
+    } else {
+      if (h1 && !f1) stk_push(stk, s.sp, rf.y, k1);
+      if (h2 && !f2) stk_push(stk, s.sp, rf.z, k2);
+      if (h3 && !f3) stk_push(stk, s.sp, rf.w, k3);
+    }
+    s.ref = next;
     return;
   }
   // Misses sort last (key +inf); a hit's key is always below +inf (a NaN or infinite entry
@@ -607,7 +628,7 @@ template <bool ANY>
 __device__ __forceinline__ bool trav_step(const SceneGeo& G, TravState& s, const TravStack& stk,
                                           uint32_t& nodes_visited, uint32_t& prims_tested) {
   if (!(s.ref & kLeafBit)) {
-    trav_node(G, s, stk, nodes_visited);
+    trav_node<ANY>(G, s, stk, nodes_visited);
   } else {
     if (trav_leaf<ANY>(G, s, s.ref, prims_tested)) return true;
     s.ref = trav_pop(stk, s);
