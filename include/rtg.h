@@ -362,6 +362,27 @@ int rtg_sampler_batch(rtg_ctx* ctx, const rtg_render_desc* desc, const uint32_t*
 int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const float* wi, const float* wo_or_u,
                    int32_t n, float* out_dir, float* f, float* pdf, float* rel_eta);
 
+/* Parity tap for the lights of next-event estimation and of the MIS weight at an emitter hit:
+ * Light::sample / Light::pdf (light/Light.h:56-64; mesh emitters light/MeshLight.cpp:32-64 with
+ * geometricShape/Mesh.cpp:38-93, the environment light/EnvironmentLight.cpp:56-135),
+ * EnvironmentLight::le (light/EnvironmentLight.cpp:137-166) and the one-light strategy's choice
+ * (integrator/Integrator.cpp:176-190 with core/CDF.h). `light` indexes the scene's lights: the
+ * emitting meshes in upload order, then the environment (rtg_scene_info.lights in all).
+ * op 0 sample: in = n points (x 3), u = n x 2 draws -> out n x 14: unit direction to the light (3),
+ *   distance, pdf (solid angle, without the selection probability), intensity (3), sampled point
+ *   (3), its normal (3; zero for the environment).
+ * op 1 pdf: in = n x 12 (from point, point on the light, its normal, direction) -> out n.
+ * op 2 environment radiance: in = n directions (x 3) -> out n x 3 (`light` is ignored).
+ * op 3 light choice: in = n draws -> out n x 3: light index, the draw rescaled for reuse, selection
+ *   probability (`light` is ignored). */
+int rtg_light_batch(rtg_ctx* ctx, rtg_scene* scene, int32_t op, int32_t light, const float* in, int32_t n,
+                    const float* u, float* out);
+/* Parity tap for NoriPerspective::getRay (camera/NoriPerspective.cpp:80-118) with the camera of a
+ * render description (origin/target/up or camera_to_world, fov, size, clip planes): pxy = n film
+ * positions (pixel + jitter, x 2) -> rays n x 8 (origin, direction, minT, maxT), as the
+ * regeneration stage makes its camera rays. */
+int rtg_camera_rays(rtg_ctx* ctx, const rtg_render_desc* desc, const float* pxy, int32_t n, float* rays);
+
 /* Parity tap for the scene's camera medium (medium/Heterogeneous.cpp, volume/GridVolume.cpp,
  * medium/HGPhaseFunction.cpp); the scene must have one, a grid one for ops 0-2.
  * op 0 GridVolume::lookupSigmaT x scale: in = n points (x 3) -> out n.

@@ -1553,6 +1553,80 @@ int rtg_medium_batch(rtg_ctx* ctx, rtg_scene* scene, int32_t op, const float* in
   })
 }
 
+// floats per item of the light tap's input and output, by op
+static const int kLightIn[4] = {3, 12, 3, 1};
+static const int kLightOut[4] = {14, 1, 3, 3};
+int rtg_light_batch(rtg_ctx* ctx, rtg_scene* scene, int32_t op, int32_t light, const float* in, int32_t n,
+                    const float* u, float* out) {
+  if (!ctx) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  RTG_GUARD(ctx, {
+    if (!scene || !in || !out || n < 0 || op < 0 || op > 3) fail(RTG_ERR_INVALID, "bad light arguments");
+    if ((op == 0 || op == 1) && (light < 0 || light >= scene->dev.nlights)) fail(RTG_ERR_INVALID, "no such light");
+    if (op == 0 && !u) fail(RTG_ERR_INVALID, "light sampling needs 2 draws per point");
+    if (op == 2 && scene->dev.env_w <= 0) fail(RTG_ERR_INVALID, "the scene has no environment map");
+    if (op == 3 && scene->dev.nlights <= 0) fail(RTG_ERR_INVALID, "the scene has no lights");
+    if (n == 0) return RTG_OK;
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    const size_t nin = (size_t)n * kLightIn[op];
+    const size_t nout = (size_t)n * kLightOut[op];
+    const size_t nu = op == 0 ? (size_t)n * 2 : 0;
+    std::vector<void*> own;
+    float* din = (float*)dev_alloc(ctx, nin * 4, own);
+    float* du = (float*)dev_alloc(ctx, std::max<size_t>(nu, 1) * 4, own);
+    float* dout = (float*)dev_alloc(ctx, nout * 4, own);
+    try {
+      hip_check(hipMemcpyAsync(din, in, nin * 4, hipMemcpyHostToDevice, ctx->stream), "in");
+      if (nu) hip_check(hipMemcpyAsync(du, u, nu * 4, hipMemcpyHostToDevice, ctx->stream), "u");
+      launch_light_tap(ctx->scenes_dev + scene->slot, op, light, din, du, n, dout, ctx->stream);
+      hip_check(hipGetLastError(), "light launch");
+      hip_check(hipMemcpyAsync(out, dout, nout * 4, hipMemcpyDeviceToHost, ctx->stream), "out");
+      hip_check(hipStreamSynchronize(ctx->stream), "light");
+    } catch (...) {
+      for (void* q : own) (void)hipFree(q);
+      throw;
+    }
+    for (void* q : own) (void)hipFree(q);
+  })
+}
+
+int rtg_camera_rays(rtg_ctx* ctx, const rtg_render_desc* desc, const float* pxy, int32_t n, float* rays) {
+  if (!ctx) return RTG_ERR_INVALID;
+  std::lock_guard<std::mutex> lk(ctx->mtx);
+  RTG_GUARD(ctx, {
+    if (!desc || !pxy || !rays || n < 0) fail(RTG_ERR_INVALID, "bad camera arguments");
+    const rtg_render_desc& d = *desc;
+    if (d.width <= 0 || d.height <= 0) fail(RTG_ERR_INVALID, "width and height must be positive");
+    if (!(d.fov > 0.f && d.fov < 180.f)) fail(RTG_ERR_INVALID, "fov out of range");
+    if (n == 0) return RTG_OK;
+    hip_check(hipSetDevice(ctx->device), "hipSetDevice");
+    // the camera fields of a render's job, as make_job fills them
+    DevJob J;
+    std::memset(&J, 0, sizeof(J));
+    camera_matrices(d, J.s2c, J.c2w);
+    J.inv_w = 1.f / d.width;
+    J.inv_h = 1.f / d.height;
+    J.near_clip = d.near_clip;
+    J.far_clip = d.far_clip;
+    J.width = d.width;
+    J.height = d.height;
+    std::vector<void*> own;
+    float* dp = (float*)dev_alloc(ctx, (size_t)n * 2 * 4, own);
+    float* dr = (float*)dev_alloc(ctx, (size_t)n * 8 * 4, own);
+    try {
+      hip_check(hipMemcpyAsync(dp, pxy, (size_t)n * 2 * 4, hipMemcpyHostToDevice, ctx->stream), "pxy");
+      launch_camera_tap(J, dp, n, dr, ctx->stream);
+      hip_check(hipGetLastError(), "camera launch");
+      hip_check(hipMemcpyAsync(rays, dr, (size_t)n * 8 * 4, hipMemcpyDeviceToHost, ctx->stream), "rays");
+      hip_check(hipStreamSynchronize(ctx->stream), "camera");
+    } catch (...) {
+      for (void* q : own) (void)hipFree(q);
+      throw;
+    }
+    for (void* q : own) (void)hipFree(q);
+  })
+}
+
 int rtg_bsdf_batch(rtg_ctx* ctx, const rtg_material* material, int32_t op, const float* wi, const float* wo_or_u,
                    int32_t n, float* out_dir, float* f, float* pdf, float* rel_eta) {
   if (!ctx) return RTG_ERR_INVALID;

@@ -268,6 +268,10 @@ void launch_trace_tap(const DevScene* scene_dev, const float* rays, int n, int a
                       hipStream_t stream);
 void launch_tonemap(const float4* film, int n, int mode, float expo, unsigned int* maxbits, uint8_t* out,
                     hipStream_t stream);
+// op 0 light sample (out n x 14), 1 light pdf (n), 2 environment radiance (n x 3), 3 light choice (n x 3)
+void launch_light_tap(const DevScene* scene_dev, int op, int light, const float* in, const float* u, int n,
+                      float* out, hipStream_t stream);
+void launch_camera_tap(const DevJob& job, const float* pxy, int n, float* rays, hipStream_t stream);
 void launch_medium_tap(const DevScene* scene_dev, int op, const float* in, int n, const float* u, int k, float* out,
                        hipStream_t stream);
 void launch_bsdf_tap(const DevMaterial* mat, int op, const float* wi, const float* wo_or_u, int n,

@@ -1377,6 +1377,50 @@ __global__ void k_medium_tap(const DevScene* scene, int op, const float* in, int
   }
 }
 
+// ---------------------------------------------------------------- light and camera taps
+// Light tap (rtg_light_batch): the functions the shading kernels call for next-event estimation
+// and for the MIS weight of an emitter hit, one item per thread.
+__global__ void k_light_tap(const DevScene* scene, int op, int li, const float* in, const float* u, int n,
+                            float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const DevScene& S = *scene;
+  if (op == 0) {  // Light::sample from a point
+    const DevLight& L = S.lights[li];
+    const f3 p = mk(in[3 * i], in[3 * i + 1], in[3 * i + 2]);
+    const LightSample ls =
+        L.type == LIGHT_ENV ? env_sample(S, p, u[2 * i], u[2 * i + 1]) : mesh_light_sample(S, L, p, u[2 * i], u[2 * i + 1]);
+    float* o = out + 14 * (size_t)i;
+    o[0] = ls.dir.x; o[1] = ls.dir.y; o[2] = ls.dir.z; o[3] = ls.dist; o[4] = ls.pdf;
+    o[5] = ls.intensity.x; o[6] = ls.intensity.y; o[7] = ls.intensity.z;
+    o[8] = ls.point.x; o[9] = ls.point.y; o[10] = ls.point.z;
+    o[11] = ls.normal.x; o[12] = ls.normal.y; o[13] = ls.normal.z;
+  } else if (op == 1) {  // Light::pdf (from point, light point, light normal, direction)
+    const DevLight& L = S.lights[li];
+    const float* a = in + 12 * (size_t)i;
+    out[i] = L.type == LIGHT_ENV ? env_pdf(S, mk(a[9], a[10], a[11]))
+                                 : mesh_light_pdf(L, mk(a[0], a[1], a[2]), mk(a[3], a[4], a[5]), mk(a[6], a[7], a[8]));
+  } else if (op == 2) {  // EnvironmentLight::le
+    const f3 c = env_le(S, mk(in[3 * i], in[3 * i + 1], in[3 * i + 2]));
+    out[3 * i] = c.x; out[3 * i + 1] = c.y; out[3 * i + 2] = c.z;
+  } else {  // one-light selection: CDF::sample with the draw reused (Integrator.cpp:176-190)
+    float s = in[i], w;
+    const int k = cdf_sample_reuse(S.light_cdf, S.nlights + 1, s, w);
+    out[3 * i] = (float)k; out[3 * i + 1] = s; out[3 * i + 2] = w;
+  }
+}
+
+// Camera tap (rtg_camera_rays): NoriPerspective::getRay through the renderer's camera_ray.
+__global__ void k_camera_tap(DevJob J, const float* pxy, int n, float* rays) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  f3 o, d;
+  float tmin, tmax;
+  camera_ray(J, pxy[2 * i], pxy[2 * i + 1], o, d, tmin, tmax);
+  float* r = rays + 8 * (size_t)i;
+  r[0] = o.x; r[1] = o.y; r[2] = o.z; r[3] = d.x; r[4] = d.y; r[5] = d.z; r[6] = tmin; r[7] = tmax;
+}
+
 // ---------------------------------------------------------------- tone mapping (core/main.cpp:185-290)
 __device__ __forceinline__ f3 resolved(const float4 a) {  // Pixel::applyColor (core/Screen.cpp:8-15)
   return a.w != 0.f ? mk(a.x / a.w, a.y / a.w, a.z / a.w) : mk(a.x, a.y, a.z);
@@ -1433,6 +1477,13 @@ void launch_tonemap(const float4* film, int n, int mode, float expo, unsigned in
 void launch_medium_tap(const DevScene* scene_dev, int op, const float* in, int n, const float* u, int k, float* out,
                        hipStream_t stream) {
   hipLaunchKernelGGL(k_medium_tap, dim3((n + 127) / 128), dim3(128), 0, stream, scene_dev, op, in, n, u, k, out);
+}
+void launch_light_tap(const DevScene* scene_dev, int op, int light, const float* in, const float* u, int n,
+                      float* out, hipStream_t stream) {
+  hipLaunchKernelGGL(k_light_tap, dim3((n + 127) / 128), dim3(128), 0, stream, scene_dev, op, light, in, u, n, out);
+}
+void launch_camera_tap(const DevJob& job, const float* pxy, int n, float* rays, hipStream_t stream) {
+  hipLaunchKernelGGL(k_camera_tap, dim3((n + 127) / 128), dim3(128), 0, stream, job, pxy, n, rays);
 }
 static size_t stack_bytes(int block) { return (size_t)block * kLdsStack * 8u; }
 

@@ -141,6 +141,8 @@ SIGNATURES = {
     "rtg_sampler_batch": (C.c_int, [P, C.POINTER(rtg_render_desc), P, C.c_int32, C.c_int32, C.c_int32, P, P]),
     "rtg_bsdf_batch": (C.c_int, [P, C.POINTER(rtg_material), C.c_int32, P, P, C.c_int32, P, P, P, P]),
     "rtg_medium_batch": (C.c_int, [P, P, C.c_int32, P, C.c_int32, P, C.c_int32, P]),
+    "rtg_light_batch": (C.c_int, [P, P, C.c_int32, C.c_int32, P, C.c_int32, P, P]),
+    "rtg_camera_rays": (C.c_int, [P, P, P, C.c_int32, P]),
     "rtg_tonemap": (C.c_int, [P, P, C.c_int32, C.c_int32, C.c_int32, C.c_float, P]),
     "rtg_spectrum_to_rgb": (C.c_int, [P, P, C.c_int32, P]),
     "rtg_server_create": (C.c_int, [P, C.c_uint64, C.POINTER(P)]),
@@ -340,6 +342,14 @@ class Context:
         self.check(self.lib.rtg_autotune(self.h, scene.h, C.byref(d), C.c_uint64(int(trial_samples)), C.byref(t),
                                          C.byref(r)))
         return {k: getattr(t, k) for k, _ in t._fields_}, r.value
+
+    def camera_rays(self, camera, pxy):
+        """Camera tap (rtg_camera_rays): film positions (n, 2) -> rays (n, 8) of the camera dict."""
+        d = render_struct(camera, {"spp": 1})
+        pxy = _f32(pxy, (-1, 2)); n = pxy.shape[0]
+        out = np.zeros((n, 8), np.float32)
+        self.check(self.lib.rtg_camera_rays(self.h, C.byref(d), _ptr(pxy), n, _ptr(out)))
+        return out
 
     def set_pool(self, slots):
         self.check(self.lib.rtg_ctx_set_pool(self.h, int(slots)))
@@ -555,6 +565,21 @@ class Scene:
         out = np.zeros((n, w_out), np.float32)
         self.ctx.check(self.ctx.lib.rtg_medium_batch(self.ctx.h, self.h, int(op), _ptr(x), n,
                                                      None if u is None else _ptr(u), k, _ptr(out)))
+        return out[:, 0] if w_out == 1 else out
+
+    def light(self, op, x, light=0, u=None):
+        """Light tap (rtg_light_batch): op 0 sample light `light` from points (n, 3) with draws u (n, 2)
+        -> (n, 14); op 1 its pdf for (n, 12) = (from, light point, light normal, direction) -> (n,);
+        op 2 environment radiance of directions (n, 3) -> (n, 3); op 3 light choice for draws (n,)
+        -> (n, 3) = (index, rescaled draw, probability)."""
+        w_in = {0: 3, 1: 12, 2: 3, 3: 1}[op]
+        w_out = {0: 14, 1: 1, 2: 3, 3: 3}[op]
+        x = _f32(x, (-1, w_in)); n = x.shape[0]
+        if u is not None:
+            u = _f32(u, (n, 2))
+        out = np.zeros((n, w_out), np.float32)
+        self.ctx.check(self.ctx.lib.rtg_light_batch(self.ctx.h, self.h, int(op), int(light), _ptr(x), n,
+                                                    None if u is None else _ptr(u), _ptr(out)))
         return out[:, 0] if w_out == 1 else out
 
     def render(self, camera, params):
