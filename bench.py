@@ -431,7 +431,7 @@ def main():
         tuning = ctx.tuning()
     else:
         progress("autotune")
-        tuning, tune_rate = ctx.autotune(scene, cam, rp, trial_samples=1 << 24)
+        tuning, tune_rate = ctx.autotune(scene, cam, rp, trial_samples=W * H * rp["spp"])
     if a.pool:
         ctx.set_tuning(pool_slots=a.pool)
         tuning = ctx.tuning()

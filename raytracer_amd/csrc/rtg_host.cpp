@@ -634,6 +634,7 @@ void ensure_pool(rtg_ctx* ctx, int capacity) {
   P.cls_q = (uint8_t*)A(n);
   P.shflag = (uint8_t*)A(n);
   P.counters = (uint32_t*)A(C_COUNT * 4);
+  P.chunk_of = (uint32_t*)A((n / kClaimChunk + 1) * 4);
   ctx->pool = P;
 }
 
@@ -665,6 +666,7 @@ LaunchCfg launch_cfg(rtg_ctx* ctx) {
   cfg.stack = stack_for_resident(ctx);
   cfg.gstack = ctx->gstack;
   cfg.refill = ctx->refill;
+  cfg.regen_blocks = std::max(64, 4 * ctx->cu_count);
   return cfg;
 }
 
@@ -1063,6 +1065,9 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   if (all_lights && guided) fail(RTG_ERR_INVALID, "guiding is not available with the all_lights strategy");
   DevJob J = make_job(ctx, scene, *desc, (float4*)out_dev);
   upload_job_filter(ctx, *desc, J, 0);
+  // A pool with a slot for every sample (in whole claim chunks) holds the whole render at once: with
+  // the box filter, pixels are then summed from the slots after the last iteration (k_gather)
+  J.gather = (J.filter == 0 && (uint64_t)ctx->pool_slots >= J.total && ctx->pool_slots % kClaimChunk == 0) ? 1 : 0;
   // pinned page: [0] done count, [16] guide ring and inference counts, [40] guide loss, [48] trace
   // tap counters (rtg_trace), [64] claim order, [256] job
   uint64_t* done = (uint64_t*)ctx->pinned;
@@ -1189,6 +1194,10 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
     double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
     if (iters > 200000 || wall > 600.0) fail(RTG_ERR_HIP, "render did not finish (iteration or time limit)");
   }
+  if (J.gather) {
+    launch_gather(ctx->jobs_dev, 0, ctx->pool, J.width * J.height, ctx->stream);
+    hip_check(hipGetLastError(), "gather launch");
+  }
   hip_check(hipEventRecord(e1, ctx->stream), "event");
   hip_check(hipEventSynchronize(e1), "event");
   ctx->pool.all_lights = 0;
@@ -1251,6 +1260,11 @@ int rtg_autotune(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, ui
     d.guiding = 0;
     const uint64_t npix = (uint64_t)d.width * d.height;
     d.spp = (int32_t)std::max<uint64_t>(1, std::min<uint64_t>(1u << 20, (trial_samples ? trial_samples : (1u << 24)) / npix));
+    if (d.sampler == RTG_SAMPLER_STRATIFIED) {  // the count the sampler will use (rounded up to a square)
+      int q = 1;
+      while (q * q < d.spp) q++;
+      d.spp = q * q;
+    }
     const size_t bytes = npix * 16;
     void* film = nullptr;
     hip_check(hipMalloc(&film, bytes), "hipMalloc tuning film");
@@ -1293,10 +1307,15 @@ int rtg_autotune(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, ui
         const double r = rate(t);
         if (r > r_best) { r_best = r; t_best = t; }
       }
-      for (int p : {1 << 20, 3 << 19, 1 << 21, 3 << 20, 1 << 22, 3 << 21, 1 << 23}) {
+      // Pools up to 2^26 slots (~10 GB at ~150 B per slot): a pool as large as the render keeps
+      // every sample in flight at once, so the render takes its fewest iterations; a pool larger
+      // than twice the trial's sample count could not fill and is not tried.
+      const uint64_t trial_total = npix * (uint64_t)d.spp;
+      for (int p : {1 << 20, 3 << 19, 1 << 21, 3 << 20, 1 << 22, 3 << 21, 1 << 23, 3 << 22, 1 << 24, 3 << 23,
+                    1 << 25, 3 << 24, 1 << 26}) {
         rtg_tuning t = t_best;
         t.pool_slots = p;
-        if (p == t_best.pool_slots) continue;
+        if (p == t_best.pool_slots || (uint64_t)p > 2 * trial_total) continue;
         const double r = rate(t);
         if (r > r_best) { r_best = r; t_best = t; }
       }

@@ -133,8 +133,14 @@ struct DevJob {
   const float* lut;
   int32_t integrator;   // RTG_INTEGRATOR_PATH_MIS or RTG_INTEGRATOR_VOLPATH
   int32_t exit_mis;     // 1: rough-dielectric exit vertices use the band proposals (dielectric_exit_mis)
+  // 1: the whole render is in flight at once (a fresh pool with a slot for every sample) with the
+  // box filter: paths keep their radiance in their slots and k_gather sums each pixel's samples in
+  // sample order after the last iteration, instead of four float atomics per finished sample
+  int32_t gather;
 };
 static const int kFilterLut = 32 * 32;
+// Slots per claim chunk of the regeneration stage (one block, 256 threads x 8 slots).
+static const int kClaimChunk = 2048;
 // lights per scene the all-lights strategy supports (shadow rays per vertex)
 static const int kAllLights = 8;
 static const uint32_t kAllLightDim = 1u << 22;  // 2D draws of lights 1.. at depth d: + d * kAllLights + light
@@ -176,6 +182,8 @@ struct DevPool {
   uint8_t* cls_q;       // material class per extension-queue entry (0..2 BSDF type, 3 miss)
   uint8_t* shflag;      // per slot: the shade stage produced a shadow ray
   uint32_t* counters;   // C_* below; u64 counters occupy two words
+  uint32_t* chunk_of;   // gather renders: slot chunk (kClaimChunk slots) that claimed each chunk of
+                        // kClaimChunk consecutive samples in the first regeneration
 };
 enum {
   C_EXT = 0, C_SHADE = 1, C_SHADOW = 5, C_ACTIVE = 6, C_NAN = 7, C_EXTRAYS = 8, C_SHRAYS = 10, C_DONE = 12,
@@ -241,6 +249,7 @@ struct LaunchCfg {
   int stack;     // traversal stack capacity (entries, >= tree height)
   uint2* gstack; // spill area: (stack - kLdsStack) entries x (trace_blocks * 128) lanes
   int refill = kTraceRefill;  // persistent traversal: refill a wave once this many lanes are idle
+  int regen_blocks = 1024;    // regeneration blocks: one resident round (4 per CU), each looping over chunks
 };
 static const int kMaxStack = 96;
 // traversal-stack entries of each lane kept in LDS (8 bytes each); the rest spill to global memory
@@ -262,6 +271,8 @@ void launch_iteration_vol(const DevPool& pool, const DevScene* scenes_dev, DevJo
 void launch_iteration_mixed(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev,
                             const int32_t* order_dev, const LaunchCfg& cfg, hipStream_t stream);
 void launch_reset_pool(const DevPool& pool, hipStream_t stream);
+// Gather renders (DevJob::gather): adds every pixel's samples of job `job` to its film.
+void launch_gather(const DevJob* jobs_dev, int job, const DevPool& pool, int npix, hipStream_t stream);
 // ctr: two device words, [0] += pushes into the global stack spill, [1] = 1 on a stack overflow
 void launch_trace_tap(const DevScene* scene_dev, const float* rays, int n, int any_hit, int32_t* prim,
                       float* t, float* uv, float* p, float* nrm, uint32_t* ctr, const LaunchCfg& cfg,

@@ -67,7 +67,8 @@ __device__ __forceinline__ void splat(const DevJob& J, uint32_t pixel, float sx,
     if (sx == 0.f && (x & 31) != 0) x0 = x - 1;
     if (sy == 0.f && (y & 31) != 0) y0 = y - 1;
     for (int yy = y0; yy <= y; yy++)
-      for (int xx = x0; xx <= x; xx++) film_add(J, xx, yy, L, 1.0f);
+      for (int xx = x0; xx <= x; xx++)
+        if (!J.gather || xx != x || yy != y) film_add(J, xx, yy, L, 1.0f);  // gather: own pixel later
     return;
   }
   const int bx0 = x & ~31, by0 = y & ~31;
@@ -130,6 +131,7 @@ __device__ __forceinline__ uint32_t block_reserve(uint32_t* counter, uint32_t to
 // per chunk).
 constexpr int kRegenK = 8;
 constexpr int kRegenSlots = 256 * kRegenK;
+static_assert(kRegenSlots == kClaimChunk, "the gather's slot map assumes one claim chunk per regeneration block");
 
 __device__ __forceinline__ bool guided_vertex(int mt, uint32_t depth) {
   return depth < (uint32_t)kGuideRec && (mt == 0 || (mt == 1 && depth == 0));
@@ -146,6 +148,7 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
   __shared__ uint32_t s_kc[kRegenK * 4];
   __shared__ uint32_t s_done[kMaxJobsDev];
   __shared__ uint32_t s_nan;
+  __shared__ uint32_t s_claim;
   uint32_t* C = P.counters;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     // shade queues are filled by the classify stage of this iteration
@@ -154,8 +157,19 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
     if (GUIDED) *G.gq_count = 0;
   }
   for (int j = threadIdx.x; j < kMaxJobsDev; j += blockDim.x) s_done[j] = 0;
-  if (threadIdx.x == 0) s_nan = 0;
+  if (threadIdx.x == 0) {
+    s_nan = 0;
+    // samples left to claim in any job of the claim order (claim counters only grow, so a block
+    // that sees none now would find none later either)
+    uint32_t left = 0;
+    for (int jj = 0; jj < order[0] && !left; jj++) {
+      const DevJob& J = jobs[order[1 + jj]];
+      left = J.active && *(volatile unsigned long long*)&J.next < J.total;
+    }
+    s_claim = left;
+  }
   __syncthreads();
+  const bool claimable = s_claim != 0;
   const int cap = P.capacity;
   const int nchunks = (cap + kRegenSlots - 1) / kRegenSlots;
   for (int chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
@@ -171,6 +185,15 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
     for (int k = 0; k < kRegenK; k++) {
       const int i = base + k * 256;
       stv[k] = i < cap ? P.state[i] : ST_NONE;
+    }
+    // a chunk with no finished or active path, and either no free slot or nothing to claim, has
+    // nothing to do (late iterations of a render whose whole sample set is in flight)
+    {
+      bool busy = false;
+#pragma unroll
+      for (int k = 0; k < kRegenK; k++)
+        busy = busy || stv[k] == ST_DONE || stv[k] == ST_ACTIVE || (stv[k] == ST_FREE && claimable);
+      if (!__syncthreads_or(busy)) continue;
     }
 #pragma unroll
     for (int k = 0; k < kRegenK; k++) {
@@ -266,6 +289,9 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
       unsigned long long b = s_base64;
       __syncthreads();
       if (b == ~0ull || b >= J.total) continue;
+      // gather renders claim every sample in their first regeneration, each block kClaimChunk of
+      // them into its (fully free) chunk of slots: remember which slot chunk holds which samples
+      if (J.gather && threadIdx.x == 0) P.chunk_of[b / kClaimChunk] = (uint32_t)chunk;
       uint64_t got = J.total - b;
       if (got > total) got = total;
       uint32_t idx = off;
@@ -1377,6 +1403,35 @@ __global__ void k_medium_tap(const DevScene* scene, int op, const float* in, int
   }
 }
 
+// ---------------------------------------------------------------- gather (box-filter renders in one pass)
+// Sample s of a gather job sits in slot chunk_of[s / 2048] * 2048 + t + 256 k with s % 2048 = 8 t + k
+// (the regeneration's claim order inside a fully free chunk). One thread per pixel, pixels in the
+// claim order's strip walk; a pixel's samples are summed in sample order (as the reference adds them,
+// core/Screen.cpp:44-97) and added to the film, NaN samples dropped with their weight.
+__global__ void __launch_bounds__(256) k_gather(const DevJob* jobs, int job, DevPool P, int npix) {
+  const DevJob& J = jobs[job];
+  const uint32_t W = (uint32_t)J.width, Hh = (uint32_t)J.height;
+  for (uint32_t r = blockIdx.x * blockDim.x + threadIdx.x; r < (uint32_t)npix; r += gridDim.x * blockDim.x) {
+    const uint32_t strip = r / (8u * W);
+    const uint32_t hs = min(8u, Hh - strip * 8u);
+    const uint32_t within = r - strip * 8u * W;
+    const uint32_t x = within / hs, y = strip * 8u + within % hs;
+    float sr = 0.f, sg = 0.f, sb = 0.f, w = 0.f;
+    for (int si = 0; si < J.spp; si++) {
+      const uint64_t s = (uint64_t)si * (uint64_t)npix + r;
+      const uint32_t rr = (uint32_t)(s % kClaimChunk);
+      const size_t slot = (size_t)P.chunk_of[s / kClaimChunk] * kClaimChunk + (rr >> 3) + (rr & 7u) * 256u;
+      const float4 L = P.L[slot];
+      if (!(isnan(L.x) || isnan(L.y) || isnan(L.z))) {
+        sr += L.x; sg += L.y; sb += L.z; w += 1.f;
+      }
+    }
+    float4* a = J.accum + (size_t)y * W + x;
+    const float4 v = *a;
+    *a = make_float4(v.x + sr, v.y + sg, v.z + sb, v.w + w);
+  }
+}
+
 // ---------------------------------------------------------------- light and camera taps
 // Light tap (rtg_light_batch): the functions the shading kernels call for next-event estimation
 // and for the MIS weight of an emitter hit, one item per thread.
@@ -1502,16 +1557,18 @@ void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* j
   }
   if (ev) (void)hipEventRecord(ev[0], stream);
   if (gd)
-    hipLaunchKernelGGL(k_regen<true>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, G);
+    hipLaunchKernelGGL(k_regen<true>, dim3(std::min(chunks, cfg.regen_blocks)), dim3(256), 0, stream, pool, jobs_dev,
+                       order_dev, G);
   else
-    hipLaunchKernelGGL(k_regen<false>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, G);
+    hipLaunchKernelGGL(k_regen<false>, dim3(std::min(chunks, cfg.regen_blocks)), dim3(256), 0, stream, pool, jobs_dev,
+                     order_dev, G);
   if (ev) (void)hipEventRecord(ev[1], stream);
   hipLaunchKernelGGL(k_extend, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
                      cfg.gstack, cfg.stack, cfg.refill);
   if (gd)
-    hipLaunchKernelGGL((k_classify<true, false>), dim3(chunks), dim3(256), 0, stream, pool, G);
+    hipLaunchKernelGGL((k_classify<true, false>), dim3(std::min(chunks, 2 * cfg.regen_blocks)), dim3(256), 0, stream, pool, G);
   else
-    hipLaunchKernelGGL((k_classify<false, false>), dim3(chunks), dim3(256), 0, stream, pool, G);
+    hipLaunchKernelGGL((k_classify<false, false>), dim3(std::min(chunks, 2 * cfg.regen_blocks)), dim3(256), 0, stream, pool, G);
   if (ev) (void)hipEventRecord(ev[2], stream);
   if (gd) {
     if (G.sample) launch_guide_infer_pool(G, pool, scenes_dev, cj, g, stream);
@@ -1534,7 +1591,7 @@ void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* j
     hipLaunchKernelGGL(k_shade_miss<false>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
   }
   if (ev) (void)hipEventRecord(ev[7], stream);
-  hipLaunchKernelGGL(k_compact_shadow, dim3(chunks), dim3(256), 0, stream, pool);
+  hipLaunchKernelGGL(k_compact_shadow, dim3(std::min(chunks, 2 * cfg.regen_blocks)), dim3(256), 0, stream, pool);
   hipLaunchKernelGGL(k_shadow, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev, cj,
                      cfg.gstack, cfg.stack, cfg.refill);
   if (ev) (void)hipEventRecord(ev[8], stream);
@@ -1546,11 +1603,12 @@ void launch_iteration_vol(const DevPool& pool, const DevScene* scenes_dev, DevJo
   DevGuide G;
   memset(&G, 0, sizeof(G));
   G.job = -1;
-  hipLaunchKernelGGL(k_regen<false>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, G);
+  hipLaunchKernelGGL(k_regen<false>, dim3(std::min(chunks, cfg.regen_blocks)), dim3(256), 0, stream, pool, jobs_dev,
+                     order_dev, G);
   hipLaunchKernelGGL(k_extend, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev, jobs_dev,
                      cfg.gstack, cfg.stack, cfg.refill);
   hipLaunchKernelGGL(k_vol<false>, dim3(cfg.grid), dim3(256), 0, stream, pool, scenes_dev, jobs_dev);
-  hipLaunchKernelGGL(k_compact_shadow, dim3(chunks), dim3(256), 0, stream, pool);
+  hipLaunchKernelGGL(k_compact_shadow, dim3(std::min(chunks, 2 * cfg.regen_blocks)), dim3(256), 0, stream, pool);
   hipLaunchKernelGGL(k_shadow, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev,
                      jobs_dev, cfg.gstack, cfg.stack, cfg.refill);
 }
@@ -1562,18 +1620,24 @@ void launch_iteration_mixed(const DevPool& pool, const DevScene* scenes_dev, Dev
   DevGuide G;
   memset(&G, 0, sizeof(G));
   G.job = -1;
-  hipLaunchKernelGGL(k_regen<false>, dim3(chunks), dim3(256), 0, stream, pool, jobs_dev, order_dev, G);
+  hipLaunchKernelGGL(k_regen<false>, dim3(std::min(chunks, cfg.regen_blocks)), dim3(256), 0, stream, pool, jobs_dev,
+                     order_dev, G);
   hipLaunchKernelGGL(k_extend, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev, jobs_dev,
                      cfg.gstack, cfg.stack, cfg.refill);
-  hipLaunchKernelGGL((k_classify<false, true>), dim3(chunks), dim3(256), 0, stream, pool, G);
+  hipLaunchKernelGGL((k_classify<false, true>), dim3(std::min(chunks, 2 * cfg.regen_blocks)), dim3(256), 0, stream, pool, G);
   hipLaunchKernelGGL((k_shade<0, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
   hipLaunchKernelGGL((k_shade<1, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
   hipLaunchKernelGGL((k_shade<2, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
   hipLaunchKernelGGL(k_shade_miss<false>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
   hipLaunchKernelGGL(k_vol<true>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev);
-  hipLaunchKernelGGL(k_compact_shadow, dim3(chunks), dim3(256), 0, stream, pool);
+  hipLaunchKernelGGL(k_compact_shadow, dim3(std::min(chunks, 2 * cfg.regen_blocks)), dim3(256), 0, stream, pool);
   hipLaunchKernelGGL(k_shadow, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev,
                      jobs_dev, cfg.gstack, cfg.stack, cfg.refill);
+}
+
+void launch_gather(const DevJob* jobs_dev, int job, const DevPool& pool, int npix, hipStream_t stream) {
+  const int blocks = std::min((npix + 255) / 256, 8192);
+  if (blocks > 0) hipLaunchKernelGGL(k_gather, dim3(blocks), dim3(256), 0, stream, jobs_dev, job, pool, npix);
 }
 
 void launch_reset_pool(const DevPool& pool, hipStream_t stream) {

@@ -41,6 +41,7 @@ void launch_iteration_mixed(const DevPool& P, const DevScene* S, DevJob* jobs, c
 void launch_medium_tap(const DevScene*, int, const float*, int, const float*, int, float*, hipStream_t) {}
 void launch_light_tap(const DevScene*, int, int, const float*, const float*, int, float*, hipStream_t) {}
 void launch_camera_tap(const DevJob&, const float*, int, float*, hipStream_t) {}
+void launch_gather(const DevJob*, int, const DevPool&, int, hipStream_t) {}
 void launch_tonemap(const float4*, int, int, float, unsigned int*, uint8_t*, hipStream_t) {}
 // as k_reset_pool: every slot free, every counter zero
 void launch_reset_pool(const DevPool& P, hipStream_t) {
