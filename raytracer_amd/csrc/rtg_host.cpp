@@ -603,7 +603,15 @@ void release_scene(rtg_ctx* ctx, rtg_scene* s) {
 
 // ---------------------------------------------------------------- pool
 void ensure_pool(rtg_ctx* ctx, int capacity) {
-  if (ctx->pool.capacity == capacity && ctx->pool_owned.size()) return;
+  if (ctx->pool_owned.size() && ctx->pool_alloc >= capacity) {
+    // a smaller pool runs on the first `capacity` slots of the arrays already allocated (every
+    // per-slot array, the all-lights ones included, holds pool_alloc slots)
+    if (ctx->pool.capacity != capacity) {
+      (void)hipStreamSynchronize(ctx->stream);
+      ctx->pool.capacity = capacity;
+    }
+    return;
+  }
   (void)hipStreamSynchronize(ctx->stream);
   graphs_clear(ctx);
   ctx->pool_epoch++;
@@ -636,6 +644,7 @@ void ensure_pool(rtg_ctx* ctx, int capacity) {
   P.counters = (uint32_t*)A(C_COUNT * 4);
   P.chunk_of = (uint32_t*)A((n / kClaimChunk + 1) * 4);
   ctx->pool = P;
+  ctx->pool_alloc = capacity;
 }
 
 // Shadow-ray storage of the all-lights strategy (kAllLights per slot), allocated on first use.
@@ -645,7 +654,7 @@ static void ensure_all_lights(rtg_ctx* ctx) {
   graphs_clear(ctx);
   ctx->pool_epoch++;
   auto A = [&](size_t bytes) { return dev_alloc(ctx, bytes, ctx->pool_owned); };
-  const size_t n = (size_t)ctx->pool.capacity * kAllLights;
+  const size_t n = (size_t)ctx->pool_alloc * kAllLights;
   ctx->pool.sh_d2 = (float4*)A(n * 16);
   ctx->pool.sh_c2 = (float4*)A(n * 16);
   ctx->pool.q_shadow2 = (uint32_t*)A(n * 4);
@@ -1081,7 +1090,10 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   *up_job = J;
   hip_check(hipMemcpyAsync(ctx->order_dev, up_order, 2 * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream),
             "order");
-  ensure_pool(ctx, ctx->pool_slots);
+  // Learned guiding trains between iteration batches and samples plainly until its first steps are
+  // done: a pool that holds most of the render at once would draw nearly every sample before the
+  // network exists, so guided renders use at most kGuidedPoolMax slots.
+  ensure_pool(ctx, guided ? std::min(ctx->pool_slots, kGuidedPoolMax) : ctx->pool_slots);
   if (all_lights) ensure_all_lights(ctx);
   ctx->pool.all_lights = all_lights ? 1 : 0;
   ctx->pool.count_work = (ctx->profiling & RTG_PROFILE_COUNT) ? 1 : 0;

@@ -86,6 +86,7 @@ struct rtg_ctx {
   int profiling = 0;
   std::vector<hipEvent_t> events;
   rtg::DevPool pool{};
+  int pool_alloc = 0;  // slots the pool's arrays hold (pool.capacity <= pool_alloc)
   std::vector<void*> pool_owned;
   std::vector<void*> scratch_owned;
   rtg::DevScene* scenes_dev = nullptr;

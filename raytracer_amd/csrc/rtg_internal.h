@@ -141,6 +141,8 @@ struct DevJob {
 static const int kFilterLut = 32 * 32;
 // Slots per claim chunk of the regeneration stage (one block, 256 threads x 8 slots).
 static const int kClaimChunk = 2048;
+// Largest path pool of a render with learned guiding (its training schedule counts iteration batches).
+static const int kGuidedPoolMax = 1 << 23;
 // lights per scene the all-lights strategy supports (shadow rays per vertex)
 static const int kAllLights = 8;
 static const uint32_t kAllLightDim = 1u << 22;  // 2D draws of lights 1.. at depth d: + d * kAllLights + light
