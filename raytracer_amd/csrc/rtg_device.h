@@ -1158,6 +1158,21 @@ __device__ __forceinline__ int cdf_sample(const float* c, int n /* entries incl.
   if (index > n - 2) index = n - 2;
   return index;
 }
+// cdf_sample with the CDF's cut points (kCdfGuide + 1 entries, rtg_host.cpp cdf_guide): the
+// lower bound of s lies between guide[k] and guide[k + 1] for k = floor(s kCdfGuide) (exact: a
+// power-of-two scale), so the binary search covers that range only and returns the same index.
+__device__ __forceinline__ int cdf_sample_guided(const float* c, int n, float s, const int32_t* guide) {
+  const int k = min(max((int)(s * (float)kCdfGuide), 0), kCdfGuide - 1);
+  int lo = guide[k], hi = guide[k + 1];
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (c[mid] < s) lo = mid + 1; else hi = mid;
+  }
+  int index = lo - 1;
+  if (index < 0) index = 0;
+  if (index > n - 2) index = n - 2;
+  return index;
+}
 __device__ __forceinline__ int cdf_sample_reuse(const float* c, int n, float& s, float& pdf) {
   int i = cdf_sample(c, n, s);
   s = (s - c[i]) / (c[i + 1] - c[i]);
@@ -1290,8 +1305,9 @@ __device__ __forceinline__ float env_pdf(const DevScene& S, f3 d) {
 __device__ __forceinline__ LightSample env_sample(const DevScene& S, f3 pFrom, float ux, float uy) {
   LightSample ls;
   int W = S.env_w, H = S.env_h;
-  int row = cdf_sample(S.env_marg_cdf, H + 1, ux);
-  int col = cdf_sample(S.env_row_cdf + (size_t)row * (W + 1), W + 1, uy);
+  int row = cdf_sample_guided(S.env_marg_cdf, H + 1, ux, S.env_marg_guide);
+  int col = cdf_sample_guided(S.env_row_cdf + (size_t)row * (W + 1), W + 1, uy,
+                              S.env_row_guide + (size_t)row * (kCdfGuide + 1));
   float rcol = (float)col + 0.f * (float)W;
   float rrow = (float)row + 0.f * (float)H;
   check_xy(S, rcol, rrow);

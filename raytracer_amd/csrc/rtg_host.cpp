@@ -49,7 +49,6 @@ T* upload(rtg_ctx* ctx, const std::vector<T>& v, std::vector<void*>& owned, uint
   return p;
 }
 
-// ---------------------------------------------------------------- CDF (core/CDF.h:14-34)
 static void cdf_build(const std::vector<float>& w, std::vector<float>& out, float& sum) {
   out.clear();
   out.push_back(0.f);
@@ -425,6 +424,7 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   std::memset(&D, 0, sizeof(D));
   D.env_light = -1;
   std::vector<float> env_map, row_cdf, marg_cdf, env_w;
+  std::vector<int32_t> marg_guide, row_guide;
   if (desc->env_rgb) {
     int W = desc->env_width, H = desc->env_height;
     if (W < 2 || H < 2) fail(RTG_ERR_INVALID, "environment map must be at least 2x2");
@@ -449,6 +449,8 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
     }
     float msum;
     cdf_build(margW, marg_cdf, msum);
+    cdf_guide(marg_cdf.data(), H + 1, marg_guide);
+    for (int i = 0; i < H; i++) cdf_guide(row_cdf.data() + (size_t)i * (W + 1), W + 1, row_guide);
     D.env_w = W;
     D.env_h = H;
     D.env_factor = scale;
@@ -530,6 +532,8 @@ rtg_scene* build_scene(rtg_ctx* ctx, const rtg_scene_desc* desc) {
   D.env_row_cdf = upload(ctx, row_cdf, own, bytes);
   D.env_marg_cdf = upload(ctx, marg_cdf, own, bytes);
   D.env_weight = upload(ctx, env_w, own, bytes);
+  D.env_marg_guide = upload(ctx, marg_guide, own, bytes);
+  D.env_row_guide = upload(ctx, row_guide, own, bytes);
   // camera medium: GridVolume data as given (GridVolume.cpp:7-64), majorant of
   // Heterogeneous.cpp:40 (GridVolume::getMaxSigmaT x scale)
   if (desc->medium_homogeneous) {

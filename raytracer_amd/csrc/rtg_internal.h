@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 namespace rtg {
 
 // ---------------------------------------------------------------- BVH layout
@@ -90,6 +92,8 @@ struct DevScene {
   const float* env_row_cdf;  // H * (W + 1)
   const float* env_marg_cdf; // H + 1
   const float* env_weight;   // H
+  const int32_t* env_marg_guide;  // kCdfGuide + 1 cut points of the marginal CDF (cdf_guide)
+  const int32_t* env_row_guide;   // H * (kCdfGuide + 1) cut points of the row CDFs
   int32_t env_w, env_h;
   float env_factor, env_marg_sum, env_radius;
   int32_t pad;
@@ -141,6 +145,28 @@ struct DevJob {
 static const int kFilterLut = 32 * 32;
 // Slots per claim chunk of the regeneration stage (one block, 256 threads x 8 slots).
 static const int kClaimChunk = 2048;
+// Cut points per environment CDF: entry k is the first index whose CDF value is >= k / kCdfGuide
+// (the lower bound a full binary search would find for that value), entry kCdfGuide the CDF's size.
+static const int kCdfGuide = 64;
+// Host: cut points of a CDF of n values (core/CDF.h layout: c[0] = 0, non-decreasing): for
+// k < kCdfGuide the lower bound of k / kCdfGuide (first index with c[i] >= k / kCdfGuide, by the
+// same float comparison as the device's search), then n. A draw u in [k, k + 1) / kCdfGuide has its
+// lower bound between entries k and k + 1, so the device searches only that range
+// (cdf_sample_guided, rtg_device.h) and finds the index the full search would.
+inline void cdf_guide(const float* c, int n, std::vector<int32_t>& out) {
+  for (int k = 0; k < kCdfGuide; k++) {
+    const float v = (float)k / (float)kCdfGuide;  // exact: kCdfGuide is a power of two
+    int lo = 0, hi = n;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (c[mid] < v) lo = mid + 1; else hi = mid;
+    }
+    out.push_back(lo);
+  }
+  out.push_back(n);
+}
+
+// ---------------------------------------------------------------- CDF (core/CDF.h:14-34)
 // Largest path pool of a render with learned guiding (its training schedule counts iteration batches).
 static const int kGuidedPoolMax = 1 << 23;
 // lights per scene the all-lights strategy supports (shadow rays per vertex)

@@ -64,3 +64,20 @@ def test_sampler_strata_arithmetic():
     r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
     print(r.stdout)
     assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-2000:]
+
+
+def test_guided_cdf_search_equals_full_search():
+    """The environment light's CDF search with cut points (rtg_device.h cdf_sample_guided, cut
+    points from rtg_internal.h cdf_guide) returns the full binary search's index for random draws,
+    every CDF value and every cut value with their float neighbours, on CDFs with zero runs, one
+    dominant entry and heavy skew; the cut points equal their directly counted definition."""
+    exe = os.path.join(BUILD, f"cdf_check.{os.getpid()}")
+    os.makedirs(BUILD, exist_ok=True)
+    cmd = ["g++", "-std=c++17", "-O1", "-ffp-contract=off", "-include", os.path.join(HERE, "cpu_trav", "device_cpu.h"),
+           "-I" + os.path.join(HERE, "mock_hip"), "-I" + CSRC, os.path.join(HERE, "cpu_trav", "cdf_check.cpp"),
+           "-o", exe]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout[-2000:]
