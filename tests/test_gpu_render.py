@@ -269,3 +269,26 @@ def test_tuning_and_graphs_leave_the_image_unchanged(ctx):
             assert np.allclose(img, ref, rtol=1e-4, atol=1e-6), t
     finally:
         ctx.set_tuning(**base)
+
+
+@pytest.mark.parametrize("which", ["cornell", "vol_cornell"])
+def test_gather_film_equals_atomic_film(ctx, which):
+    """A render whose samples all fit the path pool sums every pixel from the slots after the last
+    iteration (k_gather); the same render on a pool smaller than its sample count splats each
+    finished sample with float atomics. The films agree up to the order of the float additions and
+    the weights (sample counts) exactly."""
+    sd = getattr(scenes, which)(width=64, height=48, spp=16)
+    g = ctx.upload(sd)
+    base = ctx.tuning()
+    print(f"[render] {which} 64x48 16spp: gather (2^20 slots) vs atomics (2^14 slots)", flush=True)
+    try:
+        ctx.set_tuning(**dict(base, pool_slots=1 << 20))
+        a, sa = g.render(sd.camera, dict(sd.render, seed=5))
+        ctx.set_tuning(**dict(base, pool_slots=1 << 14))
+        b, sb = g.render(sd.camera, dict(sd.render, seed=5))
+    finally:
+        ctx.set_tuning(**base)
+    assert sa.samples == sb.samples == 64 * 48 * 16 and sa.nan_samples == sb.nan_samples == 0
+    assert sb.iterations > sa.iterations
+    assert np.array_equal(a[..., 3], b[..., 3])
+    assert np.allclose(a[..., :3], b[..., :3], rtol=1e-5, atol=1e-6)
