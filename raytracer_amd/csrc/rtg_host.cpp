@@ -1109,10 +1109,11 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   std::memset(&G, 0, sizeof(G));
   // Guided renders sample plainly (recording training vertices) for the first kWarmSteps training
   // steps, which run kWarmPerBatch at a time on the newest samples (a sample may serve several
-  // steps); then one-sample MIS with the guide, with kPerBatch steps per iteration batch until
-  // kMaxSteps, after which training and the recording that feeds it stop.
+  // steps); then one-sample MIS with the guide, with one step per kGuideBatch new training samples
+  // of the iteration batch (at least one, at most kMaxStepsPerBatch) until kMaxSteps, after which
+  // training and the recording that feeds it stop.
   const uint64_t kWarmSteps = 64, kMaxSteps = 512;
-  const int kWarmPerBatch = 16, kPerBatch = 1;
+  const int kWarmPerBatch = 16;
   const int kMaxStepsPerBatch = kWarmPerBatch;
   if (guided) {
     gs = guide_state(ctx);
@@ -1181,13 +1182,15 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
     pending_train = 0;
     if (*done >= J.total) break;
     if (guided) {
-      // train on the newest samples: one step per full batch of new samples, at most
-      // kMaxStepsPerBatch per iteration batch
+      // train on the newest samples: one step per full batch of new samples (at least one), at
+      // most kMaxStepsPerBatch per iteration batch
       ring_total = *ring_cnt;
       const uint64_t B = kGuideBatch;
       int steps = 0;
       if (G.record && ring_total >= B && ring_total > trained_upto) {
-        steps = train_steps < kWarmSteps ? kWarmPerBatch : kPerBatch;
+        const uint64_t fresh = std::max<uint64_t>(1, (ring_total - trained_upto) / B);
+        steps = train_steps < kWarmSteps ? kWarmPerBatch
+                                         : (int)std::min<uint64_t>(fresh, (uint64_t)kMaxStepsPerBatch);
         steps = (int)std::min<uint64_t>((uint64_t)steps, kMaxSteps - train_steps);
       }
       // step t trains on the newest samples, shifted back by t half-batches (within the ring)

@@ -167,8 +167,11 @@ inline void cdf_guide(const float* c, int n, std::vector<int32_t>& out) {
 }
 
 // ---------------------------------------------------------------- CDF (core/CDF.h:14-34)
-// Largest path pool of a render with learned guiding (its training schedule counts iteration batches).
-static const int kGuidedPoolMax = 1 << 23;
+// Largest path pool of a render with learned guiding: its training schedule counts iteration
+// batches (64 warm-up steps, then one per batch up to 512), which at 2^21 slots completes within a
+// 512x512 4096-spp render; larger pools cover more samples per batch and leave the network
+// undertrained (about 200 steps at 2^23 slots).
+static const int kGuidedPoolMax = 1 << 21;
 // lights per scene the all-lights strategy supports (shadow rays per vertex)
 static const int kAllLights = 8;
 static const uint32_t kAllLightDim = 1u << 22;  // 2D draws of lights 1.. at depth d: + d * kAllLights + light

@@ -133,8 +133,11 @@ constexpr int kRegenK = 8;
 constexpr int kRegenSlots = 256 * kRegenK;
 static_assert(kRegenSlots == kClaimChunk, "the gather's slot map assumes one claim chunk per regeneration block");
 
+// Vertices the network guides: diffuse surfaces at the first kGuideRec depths. (Rough conductors
+// are left to their own sampler: their GGX lobes are narrow, so the product the network would
+// learn is the BSDF's own shape, and guiding them cost more time than it saved.)
 __device__ __forceinline__ bool guided_vertex(int mt, uint32_t depth) {
-  return depth < (uint32_t)kGuideRec && (mt == 0 || (mt == 1 && depth == 0));
+  return depth < (uint32_t)kGuideRec && mt == 0;
 }
 
 // GUIDED: every guided vertex of a finished path whose radiance snapshot was taken appends one
