@@ -400,12 +400,23 @@ def main():
     def progress(msg):
         print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
+    def leg_context():
+        # a leg run on its own takes the launch configuration of an earlier full run when given
+        # (the full run's legs share the benchmark render's tuned context)
+        c = rtg.Context(0)
+        if a.tuning_from:
+            with open(a.tuning_from) as f:
+                prev = json.loads(f.read().strip().splitlines()[-1])["tuning"]
+            prev.pop("trial_samples_per_s", None)
+            c.set_tuning(**prev)
+        return c
+
     if a.only_volume:
         progress("volumetric leg only")
-        print(json.dumps(volumetric_leg(rtg.Context(0), not a.no_cpu, a.volume_stride)), flush=True)
+        print(json.dumps(volumetric_leg(leg_context(), not a.no_cpu, a.volume_stride)), flush=True)
         return
     if a.only_guiding:
-        gctx = rtg.Context(0)
+        gctx = leg_context()
         progress("guided vs plain only")
         print(json.dumps({"guiding": guiding_leg(gctx, "glossy_cornell", a.guide_size, a.guide_ref_spp)}), flush=True)
         return

@@ -1078,9 +1078,14 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   if (all_lights && guided) fail(RTG_ERR_INVALID, "guiding is not available with the all_lights strategy");
   DevJob J = make_job(ctx, scene, *desc, (float4*)out_dev);
   upload_job_filter(ctx, *desc, J, 0);
+  // Learned guiding trains between iteration batches and samples plainly until its first steps are
+  // done: a pool that holds most of the render at once would draw nearly every sample before the
+  // network exists, so guided renders use at most kGuidedPoolMax slots.
+  const int pool_cap = guided ? std::min(ctx->pool_slots, kGuidedPoolMax) : ctx->pool_slots;
   // A pool with a slot for every sample (in whole claim chunks) holds the whole render at once: with
-  // the box filter, pixels are then summed from the slots after the last iteration (k_gather)
-  J.gather = (J.filter == 0 && (uint64_t)ctx->pool_slots >= J.total && ctx->pool_slots % kClaimChunk == 0) ? 1 : 0;
+  // the box filter, pixels are then summed from the slots after the last iteration (k_gather). The
+  // decision takes the pool this render runs on, not the context's setting.
+  J.gather = (J.filter == 0 && (uint64_t)pool_cap >= J.total && pool_cap % kClaimChunk == 0) ? 1 : 0;
   // pinned page: [0] done count, [16] guide ring and inference counts, [40] guide loss, [48] trace
   // tap counters (rtg_trace), [64] claim order, [256] job
   uint64_t* done = (uint64_t*)ctx->pinned;
@@ -1094,10 +1099,13 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   *up_job = J;
   hip_check(hipMemcpyAsync(ctx->order_dev, up_order, 2 * sizeof(int32_t), hipMemcpyHostToDevice, ctx->stream),
             "order");
-  // Learned guiding trains between iteration batches and samples plainly until its first steps are
-  // done: a pool that holds most of the render at once would draw nearly every sample before the
-  // network exists, so guided renders use at most kGuidedPoolMax slots.
-  ensure_pool(ctx, guided ? std::min(ctx->pool_slots, kGuidedPoolMax) : ctx->pool_slots);
+  ensure_pool(ctx, pool_cap);
+  if (J.gather) {
+    if ((uint64_t)ctx->pool.capacity < J.total) fail(RTG_ERR_HIP, "gather render on a pool smaller than its samples");
+    // every sample chunk unclaimed until this render's first regeneration claims it
+    launch_fill_zero(ctx->pool.chunk_of, (size_t)(J.total + kClaimChunk - 1) / kClaimChunk * sizeof(uint32_t),
+                     ctx->stream);
+  }
   if (all_lights) ensure_all_lights(ctx);
   ctx->pool.all_lights = all_lights ? 1 : 0;
   ctx->pool.count_work = (ctx->profiling & RTG_PROFILE_COUNT) ? 1 : 0;
@@ -1227,6 +1235,7 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   uint32_t C[C_COUNT];
   hip_check(hipMemcpy(C, ctx->pool.counters, sizeof(C), hipMemcpyDeviceToHost), "counters");
   if (C[C_OVF]) fail(RTG_ERR_HIP, "BVH traversal stack overflow (the builder's depth bound was exceeded)");
+  if (C[C_GATHER]) fail(RTG_ERR_HIP, "gather met a sample chunk outside the path pool");
   if (st) {
     auto u64 = [&](int i) {
       uint64_t v;

@@ -213,8 +213,8 @@ struct DevPool {
   uint8_t* cls_q;       // material class per extension-queue entry (0..2 BSDF type, 3 miss)
   uint8_t* shflag;      // per slot: the shade stage produced a shadow ray
   uint32_t* counters;   // C_* below; u64 counters occupy two words
-  uint32_t* chunk_of;   // gather renders: slot chunk (kClaimChunk slots) that claimed each chunk of
-                        // kClaimChunk consecutive samples in the first regeneration
+  uint32_t* chunk_of;   // gather renders: 1 + the slot chunk (kClaimChunk slots) that claimed each
+                        // chunk of kClaimChunk consecutive samples in the first regeneration (0: none)
 };
 enum {
   C_EXT = 0, C_SHADE = 1, C_SHADOW = 5, C_ACTIVE = 6, C_NAN = 7, C_EXTRAYS = 8, C_SHRAYS = 10, C_DONE = 12,
@@ -223,7 +223,8 @@ enum {
   C_VOLQ = 24,  // volumetric queue of a mixed iteration (entries of VolPathTracing jobs)
   C_SPILL = 25, // traversal-stack pushes past the LDS entries (counting mode)
   C_OVF = 26,   // 1 when a traversal stack overflowed (an error)
-  C_COUNT = 27
+  C_GATHER = 27, // 1 when k_gather met a sample chunk outside the pool (an error)
+  C_COUNT = 28
 };
 
 // ---------------------------------------------------------------- path guiding (rtg_guide.hip)

@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
       if (b == ~0ull || b >= J.total) continue;
       // gather renders claim every sample in their first regeneration, each block kClaimChunk of
       // them into its (fully free) chunk of slots: remember which slot chunk holds which samples
-      if (J.gather && threadIdx.x == 0) P.chunk_of[b / kClaimChunk] = (uint32_t)chunk;
+      if (J.gather && threadIdx.x == 0) P.chunk_of[b / kClaimChunk] = (uint32_t)chunk + 1u;  // 0: unclaimed
       uint64_t got = J.total - b;
       if (got > total) got = total;
       uint32_t idx = off;
@@ -1407,7 +1407,7 @@ __global__ void k_medium_tap(const DevScene* scene, int op, const float* in, int
 }
 
 // ---------------------------------------------------------------- gather (box-filter renders in one pass)
-// Sample s of a gather job sits in slot chunk_of[s / 2048] * 2048 + t + 256 k with s % 2048 = 8 t + k
+// Sample s of a gather job sits in slot (chunk_of[s / 2048] - 1) * 2048 + t + 256 k with s % 2048 = 8 t + k
 // (the regeneration's claim order inside a fully free chunk). One thread per pixel, pixels in the
 // claim order's strip walk; a pixel's samples are summed in sample order (as the reference adds them,
 // core/Screen.cpp:44-97) and added to the film, NaN samples dropped with their weight.
@@ -1420,10 +1420,16 @@ __global__ void __launch_bounds__(256) k_gather(const DevJob* jobs, int job, Dev
     const uint32_t within = r - strip * 8u * W;
     const uint32_t x = within / hs, y = strip * 8u + within % hs;
     float sr = 0.f, sg = 0.f, sb = 0.f, w = 0.f;
+    const uint32_t nchunks = (uint32_t)(P.capacity / kClaimChunk);
     for (int si = 0; si < J.spp; si++) {
       const uint64_t s = (uint64_t)si * (uint64_t)npix + r;
       const uint32_t rr = (uint32_t)(s % kClaimChunk);
-      const size_t slot = (size_t)P.chunk_of[s / kClaimChunk] * kClaimChunk + (rr >> 3) + (rr & 7u) * 256u;
+      const uint32_t c = P.chunk_of[s / kClaimChunk] - 1u;  // unclaimed (0) wraps past nchunks
+      if (c >= nchunks) {  // never expected: the host gathers only renders the pool holds whole
+        P.counters[C_GATHER] = 1u;
+        continue;
+      }
+      const size_t slot = (size_t)c * kClaimChunk + (rr >> 3) + (rr & 7u) * 256u;
       const float4 L = P.L[slot];
       if (!(isnan(L.x) || isnan(L.y) || isnan(L.z))) {
         sr += L.x; sg += L.y; sb += L.z; w += 1.f;

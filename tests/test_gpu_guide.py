@@ -269,6 +269,38 @@ def test_guided_render_matches_plain(ctx):
     assert np.abs(z).max() < 6.0 and np.mean(z * z) < 3.0
 
 
+def test_guided_render_on_a_pool_larger_than_its_cap(ctx):
+    """Regression: on a context whose pool holds the whole render (plain renders then gather their
+    pixels from the slots), a guided render runs on its capped pool of 2^21 slots, so it must splat
+    its samples instead of gathering (gathering from a pool that never held them read unclaimed slot
+    chunks). Both renders put every sample in its pixel exactly once (equal weight films) and their
+    16x16-block means agree within a few standard errors."""
+    sd = scenes.cornell(width=128, height=128)
+    rp = dict(sd.render, spp=256, seed=3)  # 4.2 M samples: more than 2^21, fewer than 2^23
+    g = ctx.upload(sd)
+    base = ctx.tuning()
+    print("[guide] cornell 128x128 256spp on a 2^23-slot context: plain (gather) and guided (capped pool)",
+          flush=True)
+    try:
+        ctx.set_tuning(**dict(base, pool_slots=1 << 23))
+        plain, sp = g.render(sd.camera, dict(rp, guiding=0))
+        guided, sg = g.render(sd.camera, dict(rp, guiding=1))
+    finally:
+        ctx.set_tuning(**base)
+    assert sp.samples == sg.samples == 128 * 128 * 256 and sp.nan_samples == sg.nan_samples == 0
+    assert sg.guide_train_steps > 0
+    assert np.array_equal(plain[..., 3], guided[..., 3])
+    a = rtg.resolve(plain).astype(np.float64).reshape(8, 16, 8, 16, 3)
+    b = rtg.resolve(guided).astype(np.float64).reshape(8, 16, 8, 16, 3)
+    ma, mb = a.mean(axis=(1, 3)), b.mean(axis=(1, 3))
+    # per-block standard error from the pixel spread (pixels within a block are independent)
+    se = np.sqrt(a.var(axis=(1, 3)) / 256 + b.var(axis=(1, 3)) / 256) + 1e-4
+    z = (mb - ma) / se
+    print(f"  max|z| {np.abs(z).max():.2f} mean z^2 {np.mean(z * z):.2f} means {mb.mean():.5f} {ma.mean():.5f}",
+          flush=True)
+    assert np.abs(z).max() < 6.0 and np.mean(z * z) < 3.0
+
+
 def relmse(x, r):
     return float(np.mean((x - r) ** 2 / (r * r + 1e-2)))
 
