@@ -99,20 +99,49 @@ __device__ __forceinline__ void store_grad(uint16_t* Gt, const f32x16& c, int co
   }
 }
 
+// Features 8b..8b+7 of one half (b = 3d + a/4: coordinate d, frequencies a..a+3) into e[0..7]:
+// sin and cos of pi * 2^f * v for f = a..a+3, the argument reduced exactly mod 2. (Deriving the
+// odd frequencies by the double-angle identities instead cost rare bf16 rounding flips of single
+// features, which the deeper layers amplified into measurable output differences.)
+__device__ __forceinline__ void feature_block(float vd, int a, float e[8]) {
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    float t = ldexpf(vd, a + q);
+    t = t - 2.f * floorf(t * 0.5f);
+    sincospif(t, &e[2 * q], &e[2 * q + 1]);
+  }
+}
+
 // 72 features of this lane's half (h = 0: position, h = 1: normal) for row r into the X tile.
-// Feature d*24 + 2f (+1) = sin (cos) of pi * 2^f * v_d; the argument is reduced exactly mod 2.
+// Feature 72h + 24d + 2f (+1) = sin (cos) of pi * 2^f * v_d.
 __device__ __forceinline__ void encode_half(uint16_t* X, int r, int h, const float v[3]) {
+  uint16_t* x = X + r * kXS + 72 * h;
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
+  for (int b = 0; b < 9; b++) {
+    float e[8];
+    feature_block(v[b / 3], 4 * (b % 3), e);
 #pragma unroll
-    for (int f = 0; f < 12; f++) {
-      float t = ldexpf(v[d], f);
-      t = t - 2.f * floorf(t * 0.5f);
-      float s, c;
-      sincospif(t, &s, &c);
-      X[r * kXS + 72 * h + 24 * d + 2 * f] = f2bf(s);
-      X[r * kXS + 72 * h + 24 * d + 2 * f + 1] = f2bf(c);
-    }
+    for (int q = 0; q < 4; q++)  // sin and cos of one frequency: one 32-bit LDS store
+      *reinterpret_cast<uint32_t*>(x + 8 * b + 2 * q) = (uint32_t)f2bf(e[2 * q]) | ((uint32_t)f2bf(e[2 * q + 1]) << 16);
+  }
+}
+
+// First layer with the features computed in registers as MFMA A operands: in k-step b, lane
+// (r, h) supplies feature block b of its half (columns 72h + 8b .. +7 of row r) and the B operand
+// the same 8 columns of W1's output rows r and 32 + r, so each k-step sums 16 of the 144 columns.
+__device__ __forceinline__ void layer1_from_features(const float v[3], const uint16_t* W1, f32x16& c0, f32x16& c1) {
+  const int l = __lane_id(), r = l & 31, h = l >> 5;
+  const uint16_t* w0 = W1 + r * kGuideIn + 72 * h;
+  const uint16_t* w1 = W1 + (32 + r) * kGuideIn + 72 * h;
+#pragma unroll
+  for (int b = 0; b < 9; b++) {
+    float e[8];
+    feature_block(v[b / 3], 4 * (b % 3), e);
+    bf16x8 a;
+#pragma unroll
+    for (int j = 0; j < 8; j++) a[j] = (__bf16)e[j];
+    c0 = mfma_bf16(a, ld8(w0 + 8 * b), c0);
+    c1 = mfma_bf16(a, ld8(w1 + 8 * b), c1);
   }
 }
 
@@ -155,41 +184,49 @@ __device__ __forceinline__ void forward_tile(const DevGuide& G, uint16_t* X, uin
   wave_sync();
 }
 
-// Mixture parameters (pi, mu xyz, kappa) x 8 from the 40 raw outputs of one row.
-__device__ __forceinline__ void mixture_from_raw(const float* o, float* mix) {
-  float amax = o[0];
+// Mixture parameters (pi, mu xyz, kappa) x 8 from the 40 raw outputs of one row: the softmax's
+// maximum and normaliser over the 8 logits, then lobe k's 5 parameters into m[0..4].
+__device__ __forceinline__ void mixture_softmax(const float* o, float& amax, float& z) {
+  amax = o[0];
 #pragma unroll
   for (int k = 1; k < kGuideLobes; k++) amax = fmaxf(amax, o[5 * k]);
-  float z = 0.f;
+  z = 0.f;
 #pragma unroll
   for (int k = 0; k < kGuideLobes; k++) z += expf(o[5 * k] - amax);
+}
+__device__ __forceinline__ void mixture_lobe(const float* o, int k, float amax, float z, float* m) {
+  float m0 = o[5 * k + 1], m1 = o[5 * k + 2], m2 = o[5 * k + 3];
+  float n = fmaxf(sqrtf(m0 * m0 + (m1 * m1 + m2 * m2)), 1e-6f);
+  float s = fminf(fmaxf(o[5 * k + 4], kGuideLogKappaMin), kGuideLogKappaMax);
+  m[0] = expf(o[5 * k] - amax) / z;
+  m[1] = m0 / n;
+  m[2] = m1 / n;
+  m[3] = m2 / n;
+  m[4] = expf(s);
+}
+__device__ __forceinline__ void mixture_from_raw(const float* o, float* mix) {
+  float amax, z;
+  mixture_softmax(o, amax, z);
 #pragma unroll
-  for (int k = 0; k < kGuideLobes; k++) {
-    float m0 = o[5 * k + 1], m1 = o[5 * k + 2], m2 = o[5 * k + 3];
-    float n = fmaxf(sqrtf(m0 * m0 + (m1 * m1 + m2 * m2)), 1e-6f);
-    float s = fminf(fmaxf(o[5 * k + 4], kGuideLogKappaMin), kGuideLogKappaMax);
-    mix[5 * k] = expf(o[5 * k] - amax) / z;
-    mix[5 * k + 1] = m0 / n;
-    mix[5 * k + 2] = m1 / n;
-    mix[5 * k + 3] = m2 / n;
-    mix[5 * k + 4] = expf(s);
-  }
+  for (int k = 0; k < kGuideLobes; k++) mixture_lobe(o, k, amax, z, mix + 5 * k);
 }
 
 // ---------------------------------------------------------------- inference
 // MODE 0: rows are the guide queue's pool slots (hit surface -> inputs, mixture -> G.mix[slot]).
 // MODE 1: rows are in6[] inputs (position in [0,1]^3, normal), mixture -> out[row].
-constexpr int kInferWaves = 2;  // 14 KB of LDS per wave: 10 waves per CU
+// The first layer takes its features from registers (layer1_from_features); one LDS tile per wave
+// holds the hidden activations (bf16, stride kHS) and, aliasing them, the FP32 outputs, each layer
+// overwriting its input once the wave has read it.
+constexpr int kInferWaves = 4;
 template <int MODE>
 __global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(DevGuide G, DevPool P, const DevScene* scenes,
                                                                    const DevJob* jobs, const float* in6, int n_arr,
                                                                    float* out) {
-  __shared__ __attribute__((aligned(16))) uint16_t s_x[kInferWaves][32 * kXS];
-  __shared__ __attribute__((aligned(16))) uint16_t s_h[kInferWaves][32 * kHS];
+  __shared__ __attribute__((aligned(16))) float s_t[kInferWaves][32 * kOS];
   const int wave = threadIdx.x >> 6, l = __lane_id(), r = l & 31, h = l >> 5;
-  uint16_t* X = s_x[wave];
-  float* O = reinterpret_cast<float*>(X);
-  static_assert(32 * kOS * 4 <= 32 * kXS * 2 && 32 * kHS <= 32 * kXS, "outputs and H2 alias the feature tile");
+  float* O = s_t[wave];
+  uint16_t* H = reinterpret_cast<uint16_t*>(O);
+  static_assert(32 * kHS * 2 <= 32 * kOS * 4, "hidden tiles alias the output tile");
   const uint32_t n = MODE == 0 ? *G.gq_count : (uint32_t)n_arr;
   if (MODE == 0 && blockIdx.x == 0 && threadIdx.x == 0 && n) atomicAdd(G.infer_count, (unsigned long long)n);
   const uint32_t ntiles = (n + 31) / 32;
@@ -225,18 +262,39 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(DevGuide G, De
         }
       }
     }
-    encode_half(X, r, h, v);
+    f32x16 c0 = {}, c1 = {};
+    layer1_from_features(v, G.wf + kGWf1, c0, c1);
+    store_relu(H, c0, 0, G.bias + kGBias1);
+    store_relu(H, c1, 32, G.bias + kGBias1);
     wave_sync();
-    forward_tile(G, X, s_h[wave], X, O, nullptr);
-    if (valid && h == 0) {
-      float raw[kGuideOut], mix[kGuideOut];
+    const int offs[2] = {kGWf2, kGWf3};
+    const int boffs[2] = {kGBias2, kGBias3};
 #pragma unroll
-      for (int k = 0; k < kGuideOut; k++) raw[k] = O[r * kOS + k];
-      mixture_from_raw(raw, mix);
-      float* dst = MODE == 0 ? G.mix + (size_t)slot * kGuideOut : out + (size_t)e * kGuideOut;
+    for (int k = 0; k < 2; k++) {
+      c0 = f32x16{}; c1 = f32x16{};
+      gemm_tile<4>(H, kHS, G.wf + offs[k], kGuideHid, c0, c1);
+      wave_sync();
+      store_relu(H, c0, 0, G.bias + boffs[k]);
+      store_relu(H, c1, 32, G.bias + boffs[k]);
+      wave_sync();
+    }
+    c0 = f32x16{}; c1 = f32x16{};
+    gemm_tile<4>(H, kHS, G.wf + kGWf4, kGuideHid, c0, c1);
+    wave_sync();
+    store_out(O, c0, 0, G.bias + kGBias4);
+    store_out(O, c1, 32, G.bias + kGBias4);
+    wave_sync();
+    if (valid) {
+      // both halves of the wave: half h writes lobes 4h..4h+3 (floats 20h..20h+19 of the row)
+      const float* o = O + r * kOS;
+      float amax, z, m[kGuideOut / 2];
+      mixture_softmax(o, amax, z);
+#pragma unroll
+      for (int j = 0; j < kGuideLobes / 2; j++) mixture_lobe(o, (kGuideLobes / 2) * h + j, amax, z, m + 5 * j);
+      float* dst = (MODE == 0 ? G.mix + (size_t)slot * kGuideOut : out + (size_t)e * kGuideOut) + (kGuideOut / 2) * h;
       float4* d4 = reinterpret_cast<float4*>(dst);
 #pragma unroll
-      for (int k = 0; k < kGuideOut / 4; k++) d4[k] = make_float4(mix[4 * k], mix[4 * k + 1], mix[4 * k + 2], mix[4 * k + 3]);
+      for (int k = 0; k < kGuideOut / 8; k++) d4[k] = make_float4(m[4 * k], m[4 * k + 1], m[4 * k + 2], m[4 * k + 3]);
     }
     wave_sync();
   }

@@ -79,6 +79,7 @@ GuideState* guide_state(rtg_ctx* ctx) {
 void guide_destroy(rtg_ctx* ctx) {
   if (!ctx->guide) return;
   for (void* p : ctx->guide->pool_owned) (void)hipFree(p);
+  for (void* p : ctx->guide->infer_owned) (void)hipFree(p);
   delete ctx->guide;
   ctx->guide = nullptr;
 }
@@ -159,6 +160,34 @@ void guide_prepare_render(rtg_ctx* ctx, GuideState* g, rtg_scene* scene, uint64_
     G.bmin[c] = lo;
     G.inv_ext[c] = ext > 0.f ? 1.f / ext : 1.f;
   }
+}
+
+DevGuide guide_infer_phase(rtg_ctx* ctx, GuideState* g, const DevGuide& G, int capacity) {
+  if (g->infer_capacity < capacity) {
+    if (g->infer_capacity) {
+      (void)hipStreamSynchronize(ctx->stream);
+      for (void* p : g->infer_owned) (void)hipFree(p);
+      g->infer_owned.clear();
+      g->infer_capacity = 0;
+    }
+    auto A = [&](size_t bytes) {
+      void* p = nullptr;
+      hip_check(hipMalloc(&p, bytes), "hipMalloc guide inference buffers");
+      g->infer_owned.push_back(p);
+      return p;
+    };
+    g->infer_mix = (float*)A((size_t)capacity * kGuideOut * sizeof(float));
+    g->infer_gq = (uint32_t*)A((size_t)capacity * sizeof(uint32_t));
+    g->infer_capacity = capacity;
+  }
+  DevGuide F = G;
+  F.mix = g->infer_mix;
+  F.gq = g->infer_gq;
+  F.rec = nullptr;
+  F.rstate = nullptr;
+  F.record = 0;
+  F.sample = 1;
+  return F;
 }
 
 // One Adam step on n samples of `samples` starting at `base` (indices masked by `mask`).

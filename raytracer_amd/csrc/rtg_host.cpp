@@ -671,6 +671,22 @@ int stack_for_resident(rtg_ctx* ctx) {
   return s;
 }
 
+// Adds the accumulating pool counters of an earlier phase of a render (C0) to C: work and ray
+// counts, NaN samples and stack spills add, error flags combine.
+static void merge_counters(uint32_t* C, const uint32_t* C0) {
+  for (int i : {C_EXTRAYS, C_SHRAYS, C_NODES, C_PRIMS, C_SHNODES, C_SHPRIMS}) {
+    uint64_t a, b;
+    std::memcpy(&a, &C[i], 8);
+    std::memcpy(&b, &C0[i], 8);
+    a += b;
+    std::memcpy(&C[i], &a, 8);
+  }
+  C[C_NAN] += C0[C_NAN];
+  C[C_SPILL] += C0[C_SPILL];
+  C[C_OVF] |= C0[C_OVF];
+  C[C_GATHER] |= C0[C_GATHER];
+}
+
 LaunchCfg launch_cfg(rtg_ctx* ctx) {
   LaunchCfg cfg;
   cfg.grid = ctx->grid;
@@ -1086,9 +1102,12 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   // the box filter, pixels are then summed from the slots after the last iteration (k_gather). The
   // decision takes the pool this render runs on, not the context's setting.
   J.gather = (J.filter == 0 && (uint64_t)pool_cap >= J.total && pool_cap % kClaimChunk == 0) ? 1 : 0;
-  // pinned page: [0] done count, [16] guide ring and inference counts, [40] guide loss, [48] trace
-  // tap counters (rtg_trace), [64] claim order, [256] job
-  uint64_t* done = (uint64_t*)ctx->pinned;
+  // pinned page: [0] claim counter and done count, [16] guide ring and inference counts, [40] guide
+  // loss, [48] trace tap counters (rtg_trace), [56] claim limit upload, [64] claim order, [256] job
+  const uint64_t* next_h = (const uint64_t*)ctx->pinned;
+  uint64_t* done = (uint64_t*)ctx->pinned + 1;
+  uint64_t* up_total = (uint64_t*)((char*)ctx->pinned + 56);
+  static_assert(offsetof(DevJob, done) == offsetof(DevJob, next) + 8, "claim counter and done count adjacent");
   unsigned long long* ring_cnt = (unsigned long long*)((char*)ctx->pinned + 16);
   float* loss_h = (float*)((char*)ctx->pinned + 40);
   int32_t* up_order = (int32_t*)((char*)ctx->pinned + 64);
@@ -1123,6 +1142,13 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   const uint64_t kWarmSteps = 64, kMaxSteps = 512;
   const int kWarmPerBatch = 16;
   const int kMaxStepsPerBatch = kWarmPerBatch;
+  // On a context whose pool is larger than the guided cap, the network trains in a first phase on
+  // the capped pool; once training ends (or a quarter of the samples is claimed) that pool stops
+  // claiming and drains, and the rest of the render runs on the full pool with the trained
+  // network, inference only. Phase: 0 single pool, 1 training, 2 draining, 3 inference.
+  int phase = (guided && !J.gather && ctx->pool_slots > pool_cap) ? 1 : 0;
+  uint64_t cut = J.total;  // samples of the training phase
+  uint32_t C0[C_COUNT] = {};  // pool counters at the end of the training phase
   if (guided) {
     gs = guide_state(ctx);
     guide_prepare_render(ctx, gs, scene, desc->seed, G);
@@ -1160,7 +1186,8 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
     }
     iters += batch;
     hip_check(hipGetLastError(), "kernel launch");
-    hip_check(hipMemcpyAsync(done, &ctx->jobs_dev[0].done, sizeof(uint64_t), hipMemcpyDeviceToHost, ctx->stream),
+    hip_check(hipMemcpyAsync(ctx->pinned, &ctx->jobs_dev[0].next, 2 * sizeof(uint64_t), hipMemcpyDeviceToHost,
+                             ctx->stream),
               "done readback");
     if (guided) {
       hip_check(hipMemcpyAsync(ring_cnt, gs->ring_count, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -1189,6 +1216,22 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
     if (pending_train) last_loss = *loss_h;
     pending_train = 0;
     if (*done >= J.total) break;
+    if (phase == 2 && *done >= cut) {
+      // the capped pool has drained: the rest of the render on the full pool, inference only (the
+      // pool reset clears the counters, so the training phase's are kept on the host)
+      hip_check(hipMemcpy(C0, ctx->pool.counters, sizeof(C0), hipMemcpyDeviceToHost), "counters");
+      ensure_pool(ctx, ctx->pool_slots);
+      ctx->pool.count_work = (ctx->profiling & RTG_PROFILE_COUNT) ? 1 : 0;
+      cfg = launch_cfg(ctx);
+      launch_reset_pool(ctx->pool, ctx->stream);
+      G = guide_infer_phase(ctx, gs, G, ctx->pool.capacity);
+      *up_total = J.total;
+      hip_check(hipMemcpyAsync(&ctx->jobs_dev[0].total, up_total, sizeof(uint64_t), hipMemcpyHostToDevice,
+                               ctx->stream),
+                "claim limit");
+      phase = 3;
+      continue;
+    }
     if (guided) {
       // train on the newest samples: one step per full batch of new samples (at least one), at
       // most kMaxStepsPerBatch per iteration batch
@@ -1217,6 +1260,17 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
         if (train_steps >= kWarmSteps) G.sample = 1;
         if (train_steps >= kMaxSteps) G.record = 0;
       }
+      if (phase == 1 && (!G.record || *next_h >= J.total / 4) && *next_h < J.total) {
+        // end of the training phase: no further claims on the capped pool (no kernel runs now, so
+        // the claim counter read back is current)
+        cut = *next_h;
+        *up_total = cut;
+        hip_check(hipMemcpyAsync(&ctx->jobs_dev[0].total, up_total, sizeof(uint64_t), hipMemcpyHostToDevice,
+                                 ctx->stream),
+                  "claim limit");
+        G.record = 0;
+        phase = 2;
+      }
     }
     double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - wall0).count();
     if (iters > 200000 || wall > 600.0) fail(RTG_ERR_HIP, "render did not finish (iteration or time limit)");
@@ -1234,6 +1288,7 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   (void)hipEventDestroy(e1);
   uint32_t C[C_COUNT];
   hip_check(hipMemcpy(C, ctx->pool.counters, sizeof(C), hipMemcpyDeviceToHost), "counters");
+  if (phase == 3) merge_counters(C, C0);
   if (C[C_OVF]) fail(RTG_ERR_HIP, "BVH traversal stack overflow (the builder's depth bound was exceeded)");
   if (C[C_GATHER]) fail(RTG_ERR_HIP, "gather met a sample chunk outside the path pool");
   if (st) {

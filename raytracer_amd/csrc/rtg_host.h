@@ -42,6 +42,11 @@ struct GuideState {
   float* ring = nullptr;
   uint32_t ring_cap = 0;
   unsigned long long* ring_count = nullptr;  // [0] ring count, [1] inference count
+  // a trained network's inference on a larger pool: mixtures and guide queue per slot (only grow)
+  std::vector<void*> infer_owned;
+  int infer_capacity = 0;
+  float* infer_mix = nullptr;
+  uint32_t* infer_gq = nullptr;
   ~GuideState();
 };
 
@@ -140,6 +145,9 @@ void guide_init_params(rtg_ctx* ctx, GuideState* g, uint64_t seed);
 void guide_set_params(rtg_ctx* ctx, GuideState* g, const float* params);
 DevGuide guide_dev(GuideState* g);
 void guide_prepare_render(rtg_ctx* ctx, GuideState* g, rtg_scene* scene, uint64_t seed, DevGuide& G);
+// The guide of a render's inference phase on a pool of `capacity` slots: the trained network with
+// per-slot buffers of that size, sampling on, recording (and so training) off.
+DevGuide guide_infer_phase(rtg_ctx* ctx, GuideState* g, const DevGuide& G, int capacity);
 void guide_train_step(rtg_ctx* ctx, GuideState* g, const DevGuide& G, const float* samples, uint64_t base,
                       uint32_t mask, int n, float lr);
 }  // namespace rtg

@@ -143,6 +143,8 @@ __device__ __forceinline__ bool guided_vertex(int mt, uint32_t depth) {
 // GUIDED: every guided vertex of a finished path whose radiance snapshot was taken appends one
 // training sample (position, normal, direction, luminance of the radiance gathered after the
 // vertex divided by the throughput before it) to the guide's ring; one 64-bit atomic per chunk.
+// While the guide does not record, the per-slot records are neither read nor written (an inference
+// phase runs without them).
 template <bool GUIDED>
 __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const int32_t* order, DevGuide G) {
   __shared__ uint32_t s_w[4];
@@ -226,7 +228,7 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
           sample2d(rng, 0, sx, sy);
           splat(J, pix, sx, sy, l);
         }
-        if (GUIDED) {
+        if (GUIDED && G.record) {
 #pragma unroll
           for (int r = 0; r < kGuideRec; r++) {
             const uint8_t rs = G.rstate[(size_t)i * kGuideRec + r];
@@ -616,7 +618,7 @@ __device__ __forceinline__ void prev_segment(const DevScene& S, f3 Le, float pdf
 // along the guided direction, weighted by the vertex's sampling weight.
 // (depth: the event's depth; the vertex recorded at depth - 1 is the one awaiting it.)
 __device__ __forceinline__ void guide_snapshot(const DevGuide& G, uint32_t slot, uint32_t depth, const float4& L) {
-  if (depth < 1 || depth > (uint32_t)kGuideRec) return;
+  if (!G.record || depth < 1 || depth > (uint32_t)kGuideRec) return;
   const size_t r = (size_t)slot * kGuideRec + (depth - 1);
   if (G.rstate[r] == 1) {
     float4* q = G.rec + r * 4;

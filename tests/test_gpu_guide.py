@@ -271,10 +271,13 @@ def test_guided_render_matches_plain(ctx):
 
 def test_guided_render_on_a_pool_larger_than_its_cap(ctx):
     """Regression: on a context whose pool holds the whole render (plain renders then gather their
-    pixels from the slots), a guided render runs on its capped pool of 2^21 slots, so it must splat
+    pixels from the slots), a guided render trains on its capped pool of 2^21 slots, so it must splat
     its samples instead of gathering (gathering from a pool that never held them read unclaimed slot
-    chunks). Both renders put every sample in its pixel exactly once (equal weight films) and their
-    16x16-block means agree within a few standard errors."""
+    chunks). The guided render also switches pools: after its training phase (at most a quarter of
+    the samples) the capped pool stops claiming and drains, and the remaining samples run on the
+    context's 2^23 slots with the trained network. Both renders put every sample in its pixel exactly
+    once (equal weight films: no sample lost or repeated at the switch) and their 16x16-block means
+    agree within a few standard errors."""
     sd = scenes.cornell(width=128, height=128)
     rp = dict(sd.render, spp=256, seed=3)  # 4.2 M samples: more than 2^21, fewer than 2^23
     g = ctx.upload(sd)
