@@ -99,16 +99,30 @@ __device__ __forceinline__ void store_grad(uint16_t* Gt, const f32x16& c, int co
   }
 }
 
-// Features 8b..8b+7 of one half (b = 3d + a/4: coordinate d, frequencies a..a+3) into e[0..7]:
-// sin and cos of pi * 2^f * v for f = a..a+3, the argument reduced exactly mod 2. (Deriving the
-// odd frequencies by the double-angle identities instead cost rare bf16 rounding flips of single
-// features, which the deeper layers amplified into measurable output differences.)
-__device__ __forceinline__ void feature_block(float vd, int a, float e[8]) {
+// The 24 features of one coordinate v: sin and cos of pi * 2^f * v for f = 0..11. sincospi at
+// f = 0 in FP64, the higher frequencies by the double-angle identities sin 2x = 2 sin x cos x,
+// cos 2x = (cos x - sin x)(cos x + sin x), also in FP64: each doubling at most doubles the error,
+// about 2^11 ulp of FP64 at f = 11, so the features are FP32 values within a rounding of the exact
+// ones (the bf16 features then differ from correctly rounded ones only at exact rounding ties).
+// freq_block gives the next four frequencies (8 features) and advances the state past them.
+struct FreqState {
+  double s, c;
+};
+__device__ __forceinline__ FreqState freq_start(float v) {
+  FreqState q;
+  sincospi((double)v, &q.s, &q.c);
+  return q;
+}
+__device__ __forceinline__ void freq_block(FreqState& q, bool first, float e[8]) {
 #pragma unroll
-  for (int q = 0; q < 4; q++) {
-    float t = ldexpf(vd, a + q);
-    t = t - 2.f * floorf(t * 0.5f);
-    sincospif(t, &e[2 * q], &e[2 * q + 1]);
+  for (int k = 0; k < 4; k++) {
+    if (!first || k > 0) {
+      const double s2 = 2.0 * (q.s * q.c);
+      q.c = (q.c - q.s) * (q.c + q.s);
+      q.s = s2;
+    }
+    e[2 * k] = (float)q.s;
+    e[2 * k + 1] = (float)q.c;
   }
 }
 
@@ -117,12 +131,17 @@ __device__ __forceinline__ void feature_block(float vd, int a, float e[8]) {
 __device__ __forceinline__ void encode_half(uint16_t* X, int r, int h, const float v[3]) {
   uint16_t* x = X + r * kXS + 72 * h;
 #pragma unroll
-  for (int b = 0; b < 9; b++) {
-    float e[8];
-    feature_block(v[b / 3], 4 * (b % 3), e);
+  for (int d = 0; d < 3; d++) {
+    FreqState q = freq_start(v[d]);
 #pragma unroll
-    for (int q = 0; q < 4; q++)  // sin and cos of one frequency: one 32-bit LDS store
-      *reinterpret_cast<uint32_t*>(x + 8 * b + 2 * q) = (uint32_t)f2bf(e[2 * q]) | ((uint32_t)f2bf(e[2 * q + 1]) << 16);
+    for (int b = 0; b < 3; b++) {
+      float e[8];
+      freq_block(q, b == 0, e);
+#pragma unroll
+      for (int k = 0; k < 4; k++)  // sin and cos of one frequency: one 32-bit LDS store
+        *reinterpret_cast<uint32_t*>(x + 24 * d + 8 * b + 2 * k) =
+            (uint32_t)f2bf(e[2 * k]) | ((uint32_t)f2bf(e[2 * k + 1]) << 16);
+    }
   }
 }
 
@@ -134,14 +153,19 @@ __device__ __forceinline__ void layer1_from_features(const float v[3], const uin
   const uint16_t* w0 = W1 + r * kGuideIn + 72 * h;
   const uint16_t* w1 = W1 + (32 + r) * kGuideIn + 72 * h;
 #pragma unroll
-  for (int b = 0; b < 9; b++) {
-    float e[8];
-    feature_block(v[b / 3], 4 * (b % 3), e);
-    bf16x8 a;
+  for (int d = 0; d < 3; d++) {
+    FreqState q = freq_start(v[d]);
 #pragma unroll
-    for (int j = 0; j < 8; j++) a[j] = (__bf16)e[j];
-    c0 = mfma_bf16(a, ld8(w0 + 8 * b), c0);
-    c1 = mfma_bf16(a, ld8(w1 + 8 * b), c1);
+    for (int bb = 0; bb < 3; bb++) {
+      const int b = 3 * d + bb;
+      float e[8];
+      freq_block(q, bb == 0, e);
+      bf16x8 a;
+#pragma unroll
+      for (int j = 0; j < 8; j++) a[j] = (__bf16)e[j];
+      c0 = mfma_bf16(a, ld8(w0 + 8 * b), c0);
+      c1 = mfma_bf16(a, ld8(w1 + 8 * b), c1);
+    }
   }
 }
 
@@ -588,6 +612,14 @@ __global__ void k_guide_adam(float* master, float* m, float* v, const float* gra
 // ---------------------------------------------------------------- launchers
 void launch_guide_refresh(const float* master, uint16_t* wf, uint16_t* wb, float* bias, hipStream_t stream) {
   hipLaunchKernelGGL(k_guide_refresh, dim3((kGuideParams + 255) / 256), dim3(256), 0, stream, master, wf, wb, bias);
+}
+
+int guide_infer_blocks_per_cu() {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(k_guide_infer<0>),
+                                                   64 * kInferWaves, 0) != hipSuccess)
+    return 0;
+  return nb;
 }
 
 void launch_guide_infer_pool(const DevGuide& G, const DevPool& P, const DevScene* scenes, const DevJob* jobs,

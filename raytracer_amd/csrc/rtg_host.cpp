@@ -696,6 +696,7 @@ LaunchCfg launch_cfg(rtg_ctx* ctx) {
   cfg.gstack = ctx->gstack;
   cfg.refill = ctx->refill;
   cfg.regen_blocks = std::max(64, 4 * ctx->cu_count);
+  kernel_grids(cfg, ctx->cu_count);
   return cfg;
 }
 
@@ -1139,13 +1140,22 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   // steps); then one-sample MIS with the guide, with one step per kGuideBatch new training samples
   // of the iteration batch (at least one, at most kMaxStepsPerBatch) until kMaxSteps, after which
   // training and the recording that feeds it stop.
-  const uint64_t kWarmSteps = 64, kMaxSteps = 512;
-  const int kWarmPerBatch = 16;
+#ifndef RTG_GUIDE_MAX_STEPS
+#define RTG_GUIDE_MAX_STEPS 128
+#endif
+#ifndef RTG_GUIDE_STEPS_PER_BATCH
+#define RTG_GUIDE_STEPS_PER_BATCH 16
+#endif
+#ifndef RTG_GUIDE_PHASE_DIV
+#define RTG_GUIDE_PHASE_DIV 16
+#endif
+  const uint64_t kWarmSteps = 64, kMaxSteps = RTG_GUIDE_MAX_STEPS;
+  const int kWarmPerBatch = RTG_GUIDE_STEPS_PER_BATCH;
   const int kMaxStepsPerBatch = kWarmPerBatch;
   // On a context whose pool is larger than the guided cap, the network trains in a first phase on
-  // the capped pool; once training ends (or a quarter of the samples is claimed) that pool stops
-  // claiming and drains, and the rest of the render runs on the full pool with the trained
-  // network, inference only. Phase: 0 single pool, 1 training, 2 draining, 3 inference.
+  // the capped pool; once training ends (or 1/RTG_GUIDE_PHASE_DIV of the samples is claimed) that
+  // pool stops claiming and drains, and the rest of the render runs on the full pool with the
+  // trained network, inference only. Phase: 0 single pool, 1 training, 2 draining, 3 inference.
   int phase = (guided && !J.gather && ctx->pool_slots > pool_cap) ? 1 : 0;
   uint64_t cut = J.total;  // samples of the training phase
   uint32_t C0[C_COUNT] = {};  // pool counters at the end of the training phase
@@ -1260,7 +1270,7 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
         if (train_steps >= kWarmSteps) G.sample = 1;
         if (train_steps >= kMaxSteps) G.record = 0;
       }
-      if (phase == 1 && (!G.record || *next_h >= J.total / 4) && *next_h < J.total) {
+      if (phase == 1 && (!G.record || *next_h >= J.total / RTG_GUIDE_PHASE_DIV) && *next_h < J.total) {
         // end of the training phase: no further claims on the capped pool (no kernel runs now, so
         // the claim counter read back is current)
         cut = *next_h;

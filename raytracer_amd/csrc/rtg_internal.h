@@ -171,7 +171,10 @@ inline void cdf_guide(const float* c, int n, std::vector<int32_t>& out) {
 // batches (64 warm-up steps, then one per batch up to 512), which at 2^21 slots completes within a
 // 512x512 4096-spp render; larger pools cover more samples per batch and leave the network
 // undertrained (about 200 steps at 2^23 slots).
-static const int kGuidedPoolMax = 1 << 21;
+#ifndef RTG_GUIDED_POOL_LOG2
+#define RTG_GUIDED_POOL_LOG2 21
+#endif
+static const int kGuidedPoolMax = 1 << RTG_GUIDED_POOL_LOG2;
 // lights per scene the all-lights strategy supports (shadow rays per vertex)
 static const int kAllLights = 8;
 static const uint32_t kAllLightDim = 1u << 22;  // 2D draws of lights 1.. at depth d: + d * kAllLights + light
@@ -282,7 +285,16 @@ struct LaunchCfg {
   uint2* gstack; // spill area: (stack - kLdsStack) entries x (trace_blocks * 128) lanes
   int refill = kTraceRefill;  // persistent traversal: refill a wave once this many lanes are idle
   int regen_blocks = 1024;    // regeneration blocks: one resident round (4 per CU), each looping over chunks
+  // blocks of 256 for the grid-stride stages: one resident round each (set by kernel_grids)
+  int shade_grid[3] = {0, 0, 0};  // k_shade per material type
+  int miss_grid = 0;              // k_shade_miss
+  int vol_grid = 0;               // k_vol
+  int infer_grid = 0;             // guide-network inference
 };
+// Fills the grid-stride stages' grids of cfg for a device of `cu_count` compute units: the blocks
+// the device keeps resident at each kernel's register and LDS use (falls back to cfg.grid).
+void kernel_grids(LaunchCfg& cfg, int cu_count);
+int guide_infer_blocks_per_cu();
 static const int kMaxStack = 96;
 // traversal-stack entries of each lane kept in LDS (8 bytes each); the rest spill to global memory
 #ifndef RTG_LDS_STACK

@@ -1553,9 +1553,41 @@ void launch_camera_tap(const DevJob& job, const float* pxy, int n, float* rays, 
 }
 static size_t stack_bytes(int block) { return (size_t)block * kLdsStack * 8u; }
 
+// A grid-stride kernel launched with more blocks than the device keeps resident runs a partial
+// second round of blocks after the first (each block has the same share of the queue), so the
+// stages get exactly one resident round: blocks per CU at the kernel's register and LDS use.
+template <class K>
+static int resident_blocks_per_cu(K kernel) {
+  int nb = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(kernel), 256, 0) !=
+      hipSuccess)
+    return 0;
+  return nb;
+}
+void kernel_grids(LaunchCfg& cfg, int cu_count) {
+  static std::mutex mu;
+  static int per_cu[6] = {-1, -1, -1, -1, -1, -1};  // shade 0..2, miss, vol, infer
+  {
+    std::lock_guard<std::mutex> lk(mu);
+    if (per_cu[0] < 0) {
+      // the guided and plain instantiations of a stage use the same registers to within a few
+      per_cu[0] = std::min(resident_blocks_per_cu(k_shade<0, true>), resident_blocks_per_cu(k_shade<0, false>));
+      per_cu[1] = std::min(resident_blocks_per_cu(k_shade<1, true>), resident_blocks_per_cu(k_shade<1, false>));
+      per_cu[2] = std::min(resident_blocks_per_cu(k_shade<2, true>), resident_blocks_per_cu(k_shade<2, false>));
+      per_cu[3] = std::min(resident_blocks_per_cu(k_shade_miss<true>), resident_blocks_per_cu(k_shade_miss<false>));
+      per_cu[4] = std::min(resident_blocks_per_cu(k_vol<true>), resident_blocks_per_cu(k_vol<false>));
+      per_cu[5] = guide_infer_blocks_per_cu();
+    }
+  }
+  auto grid = [&](int i) { return per_cu[i] > 0 ? per_cu[i] * cu_count : cfg.grid; };
+  for (int m = 0; m < 3; m++) cfg.shade_grid[m] = grid(m);
+  cfg.miss_grid = grid(3);
+  cfg.vol_grid = grid(4);
+  cfg.infer_grid = grid(5);
+}
+
 void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev, const int32_t* order_dev,
                       const LaunchCfg& cfg, hipStream_t stream, hipEvent_t* ev, const DevGuide* guide) {
-  const int g = cfg.grid;
   const DevJob* cj = jobs_dev;
   const int chunks = (pool.capacity + kRegenSlots - 1) / kRegenSlots;
   const bool gd = guide && guide->enabled;
@@ -1582,24 +1614,24 @@ void launch_iteration(const DevPool& pool, const DevScene* scenes_dev, DevJob* j
     hipLaunchKernelGGL((k_classify<false, false>), dim3(std::min(chunks, 2 * cfg.regen_blocks)), dim3(256), 0, stream, pool, G);
   if (ev) (void)hipEventRecord(ev[2], stream);
   if (gd) {
-    if (G.sample) launch_guide_infer_pool(G, pool, scenes_dev, cj, g, stream);
+    if (G.sample) launch_guide_infer_pool(G, pool, scenes_dev, cj, cfg.infer_grid, stream);
     if (ev) (void)hipEventRecord(ev[3], stream);
-    hipLaunchKernelGGL((k_shade<0, true>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+    hipLaunchKernelGGL((k_shade<0, true>), dim3(cfg.shade_grid[0]), dim3(256), 0, stream, pool, scenes_dev, cj, G);
     if (ev) (void)hipEventRecord(ev[4], stream);
-    hipLaunchKernelGGL((k_shade<1, true>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+    hipLaunchKernelGGL((k_shade<1, true>), dim3(cfg.shade_grid[1]), dim3(256), 0, stream, pool, scenes_dev, cj, G);
     if (ev) (void)hipEventRecord(ev[5], stream);
-    hipLaunchKernelGGL((k_shade<2, true>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+    hipLaunchKernelGGL((k_shade<2, true>), dim3(cfg.shade_grid[2]), dim3(256), 0, stream, pool, scenes_dev, cj, G);
     if (ev) (void)hipEventRecord(ev[6], stream);
-    hipLaunchKernelGGL(k_shade_miss<true>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+    hipLaunchKernelGGL(k_shade_miss<true>, dim3(cfg.miss_grid), dim3(256), 0, stream, pool, scenes_dev, cj, G);
   } else {
     if (ev) (void)hipEventRecord(ev[3], stream);
-    hipLaunchKernelGGL((k_shade<0, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+    hipLaunchKernelGGL((k_shade<0, false>), dim3(cfg.shade_grid[0]), dim3(256), 0, stream, pool, scenes_dev, cj, G);
     if (ev) (void)hipEventRecord(ev[4], stream);
-    hipLaunchKernelGGL((k_shade<1, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+    hipLaunchKernelGGL((k_shade<1, false>), dim3(cfg.shade_grid[1]), dim3(256), 0, stream, pool, scenes_dev, cj, G);
     if (ev) (void)hipEventRecord(ev[5], stream);
-    hipLaunchKernelGGL((k_shade<2, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+    hipLaunchKernelGGL((k_shade<2, false>), dim3(cfg.shade_grid[2]), dim3(256), 0, stream, pool, scenes_dev, cj, G);
     if (ev) (void)hipEventRecord(ev[6], stream);
-    hipLaunchKernelGGL(k_shade_miss<false>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, cj, G);
+    hipLaunchKernelGGL(k_shade_miss<false>, dim3(cfg.miss_grid), dim3(256), 0, stream, pool, scenes_dev, cj, G);
   }
   if (ev) (void)hipEventRecord(ev[7], stream);
   hipLaunchKernelGGL(k_compact_shadow, dim3(std::min(chunks, 2 * cfg.regen_blocks)), dim3(256), 0, stream, pool);
@@ -1618,7 +1650,7 @@ void launch_iteration_vol(const DevPool& pool, const DevScene* scenes_dev, DevJo
                      order_dev, G);
   hipLaunchKernelGGL(k_extend, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev, jobs_dev,
                      cfg.gstack, cfg.stack, cfg.refill);
-  hipLaunchKernelGGL(k_vol<false>, dim3(cfg.grid), dim3(256), 0, stream, pool, scenes_dev, jobs_dev);
+  hipLaunchKernelGGL(k_vol<false>, dim3(cfg.vol_grid), dim3(256), 0, stream, pool, scenes_dev, jobs_dev);
   hipLaunchKernelGGL(k_compact_shadow, dim3(std::min(chunks, 2 * cfg.regen_blocks)), dim3(256), 0, stream, pool);
   hipLaunchKernelGGL(k_shadow, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev,
                      jobs_dev, cfg.gstack, cfg.stack, cfg.refill);
@@ -1626,7 +1658,6 @@ void launch_iteration_vol(const DevPool& pool, const DevScene* scenes_dev, DevJo
 
 void launch_iteration_mixed(const DevPool& pool, const DevScene* scenes_dev, DevJob* jobs_dev,
                             const int32_t* order_dev, const LaunchCfg& cfg, hipStream_t stream) {
-  const int g = cfg.grid;
   const int chunks = (pool.capacity + kRegenSlots - 1) / kRegenSlots;
   DevGuide G;
   memset(&G, 0, sizeof(G));
@@ -1636,11 +1667,11 @@ void launch_iteration_mixed(const DevPool& pool, const DevScene* scenes_dev, Dev
   hipLaunchKernelGGL(k_extend, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev, jobs_dev,
                      cfg.gstack, cfg.stack, cfg.refill);
   hipLaunchKernelGGL((k_classify<false, true>), dim3(std::min(chunks, 2 * cfg.regen_blocks)), dim3(256), 0, stream, pool, G);
-  hipLaunchKernelGGL((k_shade<0, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
-  hipLaunchKernelGGL((k_shade<1, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
-  hipLaunchKernelGGL((k_shade<2, false>), dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
-  hipLaunchKernelGGL(k_shade_miss<false>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
-  hipLaunchKernelGGL(k_vol<true>, dim3(g), dim3(256), 0, stream, pool, scenes_dev, jobs_dev);
+  hipLaunchKernelGGL((k_shade<0, false>), dim3(cfg.shade_grid[0]), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
+  hipLaunchKernelGGL((k_shade<1, false>), dim3(cfg.shade_grid[1]), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
+  hipLaunchKernelGGL((k_shade<2, false>), dim3(cfg.shade_grid[2]), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
+  hipLaunchKernelGGL(k_shade_miss<false>, dim3(cfg.miss_grid), dim3(256), 0, stream, pool, scenes_dev, jobs_dev, G);
+  hipLaunchKernelGGL(k_vol<true>, dim3(cfg.vol_grid), dim3(256), 0, stream, pool, scenes_dev, jobs_dev);
   hipLaunchKernelGGL(k_compact_shadow, dim3(std::min(chunks, 2 * cfg.regen_blocks)), dim3(256), 0, stream, pool);
   hipLaunchKernelGGL(k_shadow, dim3(cfg.trace_blocks), dim3(128), stack_bytes(128), stream, pool, scenes_dev,
                      jobs_dev, cfg.gstack, cfg.stack, cfg.refill);

@@ -42,6 +42,11 @@ void launch_medium_tap(const DevScene*, int, const float*, int, const float*, in
 void launch_light_tap(const DevScene*, int, int, const float*, const float*, int, float*, hipStream_t) {}
 void launch_camera_tap(const DevJob&, const float*, int, float*, hipStream_t) {}
 void launch_gather(const DevJob*, int, const DevPool&, int, hipStream_t) {}
+void kernel_grids(LaunchCfg& cfg, int) {
+  for (int m = 0; m < 3; m++) cfg.shade_grid[m] = cfg.grid;
+  cfg.miss_grid = cfg.vol_grid = cfg.infer_grid = cfg.grid;
+}
+int guide_infer_blocks_per_cu() { return 0; }
 void launch_tonemap(const float4*, int, int, float, unsigned int*, uint8_t*, hipStream_t) {}
 // as k_reset_pool: every slot free, every counter zero
 void launch_reset_pool(const DevPool& P, hipStream_t) {
