@@ -255,8 +255,8 @@ int rtg_ctx_get_tuning(rtg_ctx* ctx, rtg_tuning* out);
 /* Fails with RTG_ERR_INVALID while a server exists on the context. */
 int rtg_ctx_set_tuning(rtg_ctx* ctx, const rtg_tuning* tuning);
 /* Autotuner: times renders of `desc` on `scene` (its size, about trial_samples samples each,
- * 0 = 2^24; no guiding) under a coordinate search over traversal waves, refill threshold and pool
- * size (2^20 to 2^26 slots, at most twice the trial's samples), starting from the context's current
+ * 0 = 2^24; no guiding) under a coordinate search over the pool size (2^20 to 2^26 slots, at most
+ * twice the trial's samples), then traversal waves and refill threshold, starting from the context's current
  * configuration; applies the fastest to the context and returns it (and its samples/s). Trials as
  * large as the real render let it find the pool that render wants. Fails with RTG_ERR_INVALID while
  * a server exists. */

@@ -1385,7 +1385,20 @@ int rtg_autotune(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, ui
     try {
       double r_best = rate(cur);
       rtg_tuning t_best = cur;
-      // coordinate search: traversal waves, then the refill threshold, then the pool size
+      // coordinate search: the pool size first (it changes the launch shapes the most: a render
+      // whose samples are all in flight takes its fewest, largest iterations), then the traversal
+      // waves and the refill threshold at that pool (the best waves depend on the pool).
+      // Pools up to 2^26 slots (~10 GB at ~150 B per slot); a pool larger than twice the trial's
+      // sample count could not fill and is not tried.
+      const uint64_t trial_total = npix * (uint64_t)d.spp;
+      for (int p : {1 << 20, 3 << 19, 1 << 21, 3 << 20, 1 << 22, 3 << 21, 1 << 23, 3 << 22, 1 << 24, 3 << 23,
+                    1 << 25, 3 << 24, 1 << 26}) {
+        rtg_tuning t = t_best;
+        t.pool_slots = p;
+        if (p == t_best.pool_slots || (uint64_t)p > 2 * trial_total) continue;
+        const double r = rate(t);
+        if (r > r_best) { r_best = r; t_best = t; }
+      }
       for (int w : {3, 4, 5, 6, 8}) {
         rtg_tuning t = t_best;
         t.trace_waves = w;
@@ -1397,18 +1410,6 @@ int rtg_autotune(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* desc, ui
         rtg_tuning t = t_best;
         t.trace_refill = f;
         if (f == t_best.trace_refill) continue;
-        const double r = rate(t);
-        if (r > r_best) { r_best = r; t_best = t; }
-      }
-      // Pools up to 2^26 slots (~10 GB at ~150 B per slot): a pool as large as the render keeps
-      // every sample in flight at once, so the render takes its fewest iterations; a pool larger
-      // than twice the trial's sample count could not fill and is not tried.
-      const uint64_t trial_total = npix * (uint64_t)d.spp;
-      for (int p : {1 << 20, 3 << 19, 1 << 21, 3 << 20, 1 << 22, 3 << 21, 1 << 23, 3 << 22, 1 << 24, 3 << 23,
-                    1 << 25, 3 << 24, 1 << 26}) {
-        rtg_tuning t = t_best;
-        t.pool_slots = p;
-        if (p == t_best.pool_slots || (uint64_t)p > 2 * trial_total) continue;
         const double r = rate(t);
         if (r > r_best) { r_best = r; t_best = t; }
       }

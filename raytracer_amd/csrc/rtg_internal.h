@@ -246,9 +246,11 @@ static const int kGuideBatch = 65536;        // training samples per step
 static const float kGuideLogKappaMin = -3.f, kGuideLogKappaMax = 8.f;
 static const uint32_t kGuideSelDim = 1u << 20; // 1D sampler dimension of the strategy choice (+ depth)
 // Guided vertices: diffuse ones up to depth kGuideRec-1 (their product f*cos*L_in does not depend
-// on the incoming direction, so a network of position and normal can represent it), rough
-// conductors at depth 0 only (there the incoming direction is fixed by the pixel).
-static const int kGuideRec = 3;
+// on the incoming direction, so a network of position and normal can represent it).
+#ifndef RTG_GUIDE_DEPTHS
+#define RTG_GUIDE_DEPTHS 3
+#endif
+static const int kGuideRec = RTG_GUIDE_DEPTHS;
 // training sample weights are clamped at this multiple of their batch mean
 static const float kGuideWClamp = 16.f;
 
