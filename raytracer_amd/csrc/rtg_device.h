@@ -11,13 +11,17 @@ namespace dev {
 
 // Path-slot flags word: bits 0..11 depth of the pending hit, 12..19 job slot, bit 20 LAST (only the
 // emission of the pending segment remains), bit 21 EMISSION (depth-0 emission enabled), bits 22-23
-// volumetric path state (rtg_kernels.hip), bit 24 VOL (the path belongs to a VolPathTracing job).
+// volumetric path state (rtg_kernels.hip), bit 24 VOL (the path belongs to a VolPathTracing job),
+// bits 25/26 XEDGE/YEDGE (the sample's pixel jitter is exactly 0 in x/y: all the box filter's splat
+// needs of the jitter, recorded when the sample is claimed).
 static const uint32_t DEPTH_MASK = 0xFFFu;
 static const uint32_t JOB_SHIFT = 12;
 static const uint32_t JOB_MASK = 0xFFu;
 static const uint32_t FL_LAST = 1u << 20;
 static const uint32_t FL_EMIT = 1u << 21;
 static const uint32_t FL_VOL = 1u << 24;
+static const uint32_t FL_XEDGE = 1u << 25;
+static const uint32_t FL_YEDGE = 1u << 26;
 __device__ __forceinline__ uint32_t fl_depth(uint32_t f) { return f & DEPTH_MASK; }
 __device__ __forceinline__ uint32_t fl_job(uint32_t f) { return (f >> JOB_SHIFT) & JOB_MASK; }
 

@@ -221,6 +221,9 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
         const bool bad = isnan(l.x) || isnan(l.y) || isnan(l.z);
         if (bad) {
           atomicAdd(&s_nan, 1u);
+        } else if (J.filter == 0) {
+          // box filter: only whether the jitter was 0 matters (flags, set at the claim)
+          splat(J, P.pixel[i], (f & FL_XEDGE) ? 0.f : 0.5f, (f & FL_YEDGE) ? 0.f : 0.5f, l);
         } else {
           uint32_t pix = P.pixel[i], si = P.sample[i];
           PathRng rng = make_rng(J.seed, pix, si, J);
@@ -322,7 +325,8 @@ __global__ void __launch_bounds__(256, 4) k_regen(DevPool P, DevJob* jobs, const
           P.L[i] = make_float4(0.f, 0.f, 0.f, 1.f);
           P.pixel[i] = pix;
           P.sample[i] = si;
-          P.flags[i] = ((uint32_t)j << JOB_SHIFT) | FL_EMIT | (J.integrator == INTEGRATOR_VOLPATH ? FL_VOL : 0u);
+          P.flags[i] = ((uint32_t)j << JOB_SHIFT) | FL_EMIT | (J.integrator == INTEGRATOR_VOLPATH ? FL_VOL : 0u) |
+                       (sx == 0.f ? FL_XEDGE : 0u) | (sy == 0.f ? FL_YEDGE : 0u);
           P.state[i] = ST_ACTIVE;
           need &= ~(1u << k);
           act |= 1u << k;
