@@ -66,6 +66,8 @@ __device__ __forceinline__ float luminance(f3 c) { return 0.2126f * c.x + 0.7152
 __device__ __forceinline__ bool is_zero(f3 c) {
   return fabsf(c.x) <= 1e-5f && fabsf(c.y) <= 1e-5f && fabsf(c.z) <= 1e-5f;
 }
+// every component exactly 0 (a NaN component is not): a contribution that adds nothing
+__device__ __forceinline__ bool exactly_zero(f3 c) { return c.x == 0.f && c.y == 0.f && c.z == 0.f; }
 __device__ __forceinline__ float average(float a) {  // Color(a).average(), Color.h:151-158
   float s = 0.f;
   s += a; s += a; s += a;

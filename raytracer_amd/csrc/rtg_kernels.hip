@@ -699,10 +699,12 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
                 const float cosT = fabsf(wi_l.z);
                 const float w = power_heuristic(ls.pdf, bsdf_pdf<MT>(M, wi_l, wo_l));
                 const f3 c = cmul(cmul(mk(T.x, T.y, T.z), value), bsdf_eval<MT>(M, wi_l, wo_l)) * cosT * w;
-                const size_t e = (size_t)slot * kAllLights + cnt;
-                P.sh_d2[e] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - EPS);
-                P.sh_c2[e] = make_float4(c.x, c.y, c.z, 0.f);
-                cnt++;
+                if (!exactly_zero(c)) {  // a zero contribution adds nothing, visible or not: no shadow ray
+                  const size_t e = (size_t)slot * kAllLights + cnt;
+                  P.sh_d2[e] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - EPS);
+                  P.sh_c2[e] = make_float4(c.x, c.y, c.z, 0.f);
+                  cnt++;
+                }
               }
             }
           }
@@ -726,9 +728,13 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
               float w = power_heuristic(lpdf, bp);
               f3 fe = bsdf_eval<MT>(M, wi_l, wo_l);
               f3 c = cmul(cmul(mk(T.x, T.y, T.z), value), fe) * cosT * w;
-              P.sh_d[slot] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - EPS);
-              P.sh_c[slot] = make_float4(c.x, c.y, c.z, 0.f);
-              nshadow = 1;
+              // a zero contribution (the light below the surface for this BSDF, or a zero MIS
+              // weight) adds nothing whether the light is visible or not: no shadow ray
+              if (!exactly_zero(c)) {
+                P.sh_d[slot] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - EPS);
+                P.sh_c[slot] = make_float4(c.x, c.y, c.z, 0.f);
+                nshadow = 1;
+              }
             }
           }
         }
@@ -1002,9 +1008,11 @@ __global__ void __launch_bounds__(256, 3) k_vol(DevPool P, const DevScene* scene
               const float pf = hg_eval(S.hg_g, ls.dir, -d);
               const float w = power_heuristic(lpdf, pf);
               const f3 c = (cmul(tp, value) * pf) * w;
-              P.sh_d[slot] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - EPS);
-              P.sh_c[slot] = make_float4(c.x, c.y, c.z, 0.f);
-              shadow = true;
+              if (!exactly_zero(c)) {  // a zero contribution needs no shadow ray
+                P.sh_d[slot] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - EPS);
+                P.sh_c[slot] = make_float4(c.x, c.y, c.z, 0.f);
+                shadow = true;
+              }
             }
           }
         }
@@ -1055,9 +1063,11 @@ __global__ void __launch_bounds__(256, 3) k_vol(DevPool P, const DevScene* scene
               const float cosT = fabsf(wi_l.z);
               const float w = power_heuristic(lpdf, bsdf_pdf_any(M, wi_l, wo_l));
               const f3 c = (cmul(cmul(tp, value), bsdf_eval_any(M, wi_l, wo_l)) * cosT) * w;
-              P.sh_d[slot] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - EPS);
-              P.sh_c[slot] = make_float4(c.x, c.y, c.z, 0.f);
-              shadow = true;
+              if (!exactly_zero(c)) {  // a zero contribution needs no shadow ray
+                P.sh_d[slot] = make_float4(ls.dir.x, ls.dir.y, ls.dir.z, ls.dist - EPS);
+                P.sh_c[slot] = make_float4(c.x, c.y, c.z, 0.f);
+                shadow = true;
+              }
             }
           }
         }
