@@ -148,10 +148,11 @@ __device__ __forceinline__ void encode_half(uint16_t* X, int r, int h, const flo
 // First layer with the features computed in registers as MFMA A operands: in k-step b, lane
 // (r, h) supplies feature block b of its half (columns 72h + 8b .. +7 of row r) and the B operand
 // the same 8 columns of W1's output rows r and 32 + r, so each k-step sums 16 of the 144 columns.
-__device__ __forceinline__ void layer1_from_features(const float v[3], const uint16_t* W1, f32x16& c0, f32x16& c1) {
+__device__ __forceinline__ void layer1_from_features(const float v[3], const uint16_t* W1, int sw, f32x16& c0,
+                                                     f32x16& c1) {
   const int l = __lane_id(), r = l & 31, h = l >> 5;
-  const uint16_t* w0 = W1 + r * kGuideIn + 72 * h;
-  const uint16_t* w1 = W1 + (32 + r) * kGuideIn + 72 * h;
+  const uint16_t* w0 = W1 + r * sw + 72 * h;
+  const uint16_t* w1 = W1 + (32 + r) * sw + 72 * h;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
     FreqState q = freq_start(v[d]);
@@ -240,13 +241,31 @@ __device__ __forceinline__ void mixture_from_raw(const float* o, float* mix) {
 // MODE 1: rows are in6[] inputs (position in [0,1]^3, normal), mixture -> out[row].
 // The first layer takes its features from registers (layer1_from_features); one LDS tile per wave
 // holds the hidden activations (bf16, stride kHS) and, aliasing them, the FP32 outputs, each layer
-// overwriting its input once the wave has read it.
+// overwriting its input once the wave has read it. The block copies the bf16 weights into LDS once
+// (rows padded to the activation tiles' strides, which keep the MFMA operand reads conflict-free):
+// read from global memory, every 32-row tile would fetch all 43 KB of them again through the
+// vector memory path.
 constexpr int kInferWaves = 4;
 template <int MODE>
 __global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(DevGuide G, DevPool P, const DevScene* scenes,
                                                                    const DevJob* jobs, const float* in6, int n_arr,
                                                                    float* out) {
   __shared__ __attribute__((aligned(16))) float s_t[kInferWaves][32 * kOS];
+  __shared__ __attribute__((aligned(16))) uint16_t s_w1[64 * kXS];
+  __shared__ __attribute__((aligned(16))) uint16_t s_wh[3][64 * kHS];
+  for (int i = threadIdx.x; i < 64 * (kGuideIn / 8); i += blockDim.x) {
+    const int row = i / (kGuideIn / 8), c = i - row * (kGuideIn / 8);
+    *reinterpret_cast<uint4*>(s_w1 + row * kXS + 8 * c) =
+        *reinterpret_cast<const uint4*>(G.wf + kGWf1 + row * kGuideIn + 8 * c);
+  }
+  for (int i = threadIdx.x; i < 3 * 64 * (kGuideHid / 8); i += blockDim.x) {
+    const int m = i / (64 * (kGuideHid / 8)), j = i - m * (64 * (kGuideHid / 8));
+    const int row = j / (kGuideHid / 8), c = j - row * (kGuideHid / 8);
+    const int off = m == 0 ? kGWf2 : (m == 1 ? kGWf3 : kGWf4);
+    *reinterpret_cast<uint4*>(s_wh[m] + row * kHS + 8 * c) =
+        *reinterpret_cast<const uint4*>(G.wf + off + row * kGuideHid + 8 * c);
+  }
+  __syncthreads();
   const int wave = threadIdx.x >> 6, l = __lane_id(), r = l & 31, h = l >> 5;
   float* O = s_t[wave];
   uint16_t* H = reinterpret_cast<uint16_t*>(O);
@@ -287,23 +306,22 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(DevGuide G, De
       }
     }
     f32x16 c0 = {}, c1 = {};
-    layer1_from_features(v, G.wf + kGWf1, c0, c1);
+    layer1_from_features(v, s_w1, kXS, c0, c1);
     store_relu(H, c0, 0, G.bias + kGBias1);
     store_relu(H, c1, 32, G.bias + kGBias1);
     wave_sync();
-    const int offs[2] = {kGWf2, kGWf3};
     const int boffs[2] = {kGBias2, kGBias3};
 #pragma unroll
     for (int k = 0; k < 2; k++) {
       c0 = f32x16{}; c1 = f32x16{};
-      gemm_tile<4>(H, kHS, G.wf + offs[k], kGuideHid, c0, c1);
+      gemm_tile<4>(H, kHS, s_wh[k], kHS, c0, c1);
       wave_sync();
       store_relu(H, c0, 0, G.bias + boffs[k]);
       store_relu(H, c1, 32, G.bias + boffs[k]);
       wave_sync();
     }
     c0 = f32x16{}; c1 = f32x16{};
-    gemm_tile<4>(H, kHS, G.wf + kGWf4, kGuideHid, c0, c1);
+    gemm_tile<4>(H, kHS, s_wh[2], kHS, c0, c1);
     wave_sync();
     store_out(O, c0, 0, G.bias + kGBias4);
     store_out(O, c1, 32, G.bias + kGBias4);
