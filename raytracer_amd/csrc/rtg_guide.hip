@@ -244,8 +244,9 @@ __device__ __forceinline__ void mixture_from_raw(const float* o, float* mix) {
 // overwriting its input once the wave has read it. The block copies the bf16 weights into LDS once
 // (rows padded to the activation tiles' strides, which keep the MFMA operand reads conflict-free):
 // read from global memory, every 32-row tile would fetch all 43 KB of them again through the
-// vector memory path.
-constexpr int kInferWaves = 4;
+// vector memory path. Blocks of 16 waves share one copy: 46 KB of weights + 16 x 5.1 KB of tiles
+// = 128 KB, one block per CU, 4 waves per SIMD (blocks of 4 waves, two per CU, kept 2 per SIMD).
+constexpr int kInferWaves = 16;
 template <int MODE>
 __global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(DevGuide G, DevPool P, const DevScene* scenes,
                                                                    const DevJob* jobs, const float* in6, int n_arr,
