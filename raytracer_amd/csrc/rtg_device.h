@@ -1131,18 +1131,9 @@ __device__ __forceinline__ float mix_pdf(const float* mx, f3 w) {
   for (int k = 0; k < 8; k++) p += mx[5 * k] * vmf_pdf(mk(mx[5 * k + 1], mx[5 * k + 2], mx[5 * k + 3]), mx[5 * k + 4], w);
   return p;
 }
-// Sample a lobe with ul, then its vMF with (u1, u2) by inverting the CDF of cos(theta) (Wenzel
-// Jakob, "Numerically stable sampling of the von Mises Fisher distribution on S2", 2012).
-__device__ __forceinline__ f3 mix_sample(const float* mx, float ul, float u1, float u2) {
-  int sel = 7;  // first lobe whose cumulative weight exceeds ul (the last one absorbs rounding)
-  float acc = 0.f;
-#pragma unroll
-  for (int i = 0; i < 7; i++) {
-    acc += mx[5 * i];
-    if (sel == 7 && ul < acc) sel = i;
-  }
-  const f3 mu = mk(mx[5 * sel + 1], mx[5 * sel + 2], mx[5 * sel + 3]);
-  const float kappa = mx[5 * sel + 4];
+// A direction from the vMF lobe (mu, kappa) with (u1, u2), by inverting the CDF of cos(theta)
+// (Wenzel Jakob, "Numerically stable sampling of the von Mises Fisher distribution on S2", 2012).
+__device__ __forceinline__ f3 vmf_sample(f3 mu, float kappa, float u1, float u2) {
   float w = 1.f + logf(u1 + (1.f - u1) * expf(-2.f * kappa)) / kappa;
   w = fminf(fmaxf(w, -1.f), 1.f);
   const float sr = sqrtf(fmaxf(0.f, 1.f - w * w));
@@ -1150,6 +1141,56 @@ __device__ __forceinline__ f3 mix_sample(const float* mx, float ul, float u1, fl
   sincosf(2.f * PI * u2, &sp, &cp);
   Frame F = make_frame(mu);
   return to_world(F, mk(sr * cp, sr * sp, w));
+}
+// First lobe whose cumulative weight exceeds ul (the last one absorbs rounding); pi(i) gives weight i.
+template <class Pi>
+__device__ __forceinline__ int mix_select(Pi pi, float ul) {
+  int sel = 7;
+  float acc = 0.f;
+#pragma unroll
+  for (int i = 0; i < 7; i++) {
+    acc += pi(i);
+    if (sel == 7 && ul < acc) sel = i;
+  }
+  return sel;
+}
+// Sample a lobe with ul, then its vMF with (u1, u2).
+__device__ __forceinline__ f3 mix_sample(const float* mx, float ul, float u1, float u2) {
+  const int sel = mix_select([&](int i) { return mx[5 * i]; }, ul);
+  return vmf_sample(mk(mx[5 * sel + 1], mx[5 * sel + 2], mx[5 * sel + 3]), mx[5 * sel + 4], u1, u2);
+}
+
+// The renderer's per-vertex layout of the same mixture (kGuideMixR floats, 16-byte aligned): pi[8],
+// c[8] = pi * kappa / (2 pi (1 - e^{-2 kappa})), mu[8][3], kappa[8]. The normalisers are formed once
+// when the network writes the mixture, so a density costs one exponential per lobe, and the fields
+// load as whole float4s.
+__device__ __forceinline__ float mixr_pdf(const float* mx, f3 w) {
+  const float4* m4 = reinterpret_cast<const float4*>(mx);
+  const float4 c0 = m4[2], c1 = m4[3];
+  const float c[8] = {c0.x, c0.y, c0.z, c0.w, c1.x, c1.y, c1.z, c1.w};
+  float mu[24];
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    const float4 t = m4[4 + q];
+    mu[4 * q] = t.x; mu[4 * q + 1] = t.y; mu[4 * q + 2] = t.z; mu[4 * q + 3] = t.w;
+  }
+  const float4 k0 = m4[10], k1 = m4[11];
+  const float ka[8] = {k0.x, k0.y, k0.z, k0.w, k1.x, k1.y, k1.z, k1.w};
+  float p = 0.f;
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+    const float d = mu[3 * k] * w.x + mu[3 * k + 1] * w.y + mu[3 * k + 2] * w.z;
+    p += c[k] * expf(ka[k] * (d - 1.f));
+  }
+  return p;
+}
+__device__ __forceinline__ f3 mixr_sample(const float* mx, float ul, float u1, float u2) {
+  const float4* m4 = reinterpret_cast<const float4*>(mx);
+  const float4 p0 = m4[0], p1 = m4[1];
+  const float pi[8] = {p0.x, p0.y, p0.z, p0.w, p1.x, p1.y, p1.z, p1.w};
+  const int sel = mix_select([&](int i) { return pi[i]; }, ul);
+  const float* mu = mx + 16 + 3 * sel;
+  return vmf_sample(mk(mu[0], mu[1], mu[2]), mx[40 + sel], u1, u2);
 }
 
 // ---------------------------------------------------------------- CDF (core/CDF.h:44-81)

@@ -328,16 +328,33 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(DevGuide G, De
     store_out(O, c1, 32, G.bias + kGBias4);
     wave_sync();
     if (valid) {
-      // both halves of the wave: half h writes lobes 4h..4h+3 (floats 20h..20h+19 of the row)
+      // both halves of the wave: half h forms lobes 4h..4h+3
       const float* o = O + r * kOS;
       float amax, z, m[kGuideOut / 2];
       mixture_softmax(o, amax, z);
 #pragma unroll
       for (int j = 0; j < kGuideLobes / 2; j++) mixture_lobe(o, (kGuideLobes / 2) * h + j, amax, z, m + 5 * j);
-      float* dst = (MODE == 0 ? G.mix + (size_t)slot * kGuideOut : out + (size_t)e * kGuideOut) + (kGuideOut / 2) * h;
-      float4* d4 = reinterpret_cast<float4*>(dst);
+      if (MODE == 0) {
+        // the renderer's layout (kGuideMixR: pi, pi x normaliser, mu, kappa), this half's lobes
+        float4* d4 = reinterpret_cast<float4*>(G.mix + (size_t)slot * kGuideMixR);
+        float c[4];
 #pragma unroll
-      for (int k = 0; k < kGuideOut / 8; k++) d4[k] = make_float4(m[4 * k], m[4 * k + 1], m[4 * k + 2], m[4 * k + 3]);
+        for (int j = 0; j < 4; j++) {
+          const float ka = m[5 * j + 4];
+          c[j] = m[5 * j] * (ka / (2.f * PI * (-expm1f(-2.f * ka))));
+        }
+        d4[h] = make_float4(m[0], m[5], m[10], m[15]);
+        d4[2 + h] = make_float4(c[0], c[1], c[2], c[3]);
+        d4[4 + 3 * h] = make_float4(m[1], m[2], m[3], m[6]);
+        d4[5 + 3 * h] = make_float4(m[7], m[8], m[11], m[12]);
+        d4[6 + 3 * h] = make_float4(m[13], m[16], m[17], m[18]);
+        d4[10 + h] = make_float4(m[4], m[9], m[14], m[19]);
+      } else {
+        // the parameter layout (pi, mu xyz, kappa) x 8: floats 20h..20h+19 of the row
+        float4* d4 = reinterpret_cast<float4*>(out + (size_t)e * kGuideOut + (kGuideOut / 2) * h);
+#pragma unroll
+        for (int k = 0; k < kGuideOut / 8; k++) d4[k] = make_float4(m[4 * k], m[4 * k + 1], m[4 * k + 2], m[4 * k + 3]);
+      }
     }
     wave_sync();
   }

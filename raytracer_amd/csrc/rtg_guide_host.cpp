@@ -59,7 +59,7 @@ static void guide_alloc_pool(rtg_ctx* ctx, GuideState* g, int capacity) {
     return p;
   };
   const size_t n = (size_t)capacity;
-  g->mix = (float*)A(n * kGuideOut * sizeof(float));
+  g->mix = (float*)A(n * kGuideMixR * sizeof(float));
   g->rec = (float4*)A(n * kGuideRec * 4 * sizeof(float4));
   g->rstate = (uint8_t*)A((n * kGuideRec + 3) & ~(size_t)3);
   g->gq = (uint32_t*)A(n * sizeof(uint32_t));
@@ -176,7 +176,7 @@ DevGuide guide_infer_phase(rtg_ctx* ctx, GuideState* g, const DevGuide& G, int c
       g->infer_owned.push_back(p);
       return p;
     };
-    g->infer_mix = (float*)A((size_t)capacity * kGuideOut * sizeof(float));
+    g->infer_mix = (float*)A((size_t)capacity * kGuideMixR * sizeof(float));
     g->infer_gq = (uint32_t*)A((size_t)capacity * sizeof(uint32_t));
     g->infer_capacity = capacity;
   }

@@ -235,6 +235,9 @@ enum {
 // inputs, three hidden layers of 64 (ReLU), 40 outputs = 8 vMF lobes x (logit, mean xyz, log kappa).
 // FP32 master parameters, row-major W[out][in] then bias, per layer:
 static const int kGuideIn = 144, kGuideHid = 64, kGuideOut = 40, kGuideLobes = 8;
+// The mixture as the renderer keeps it per guided vertex (rtg_device.h mixr_pdf): pi[8], pi times
+// the vMF normaliser [8], mu[8][3], kappa[8].
+static const int kGuideMixR = 48;
 static const int kGOffW1 = 0, kGOffB1 = 9216, kGOffW2 = 9280, kGOffB2 = 13376, kGOffW3 = 13440, kGOffB3 = 17536,
                  kGOffW4 = 17600, kGOffB4 = 20160, kGuideParams = 20200;
 // bf16 copies for the MFMA kernels: forward layout wf = W[out][in] per layer (L4 padded to 64 rows),

@@ -749,14 +749,14 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
           // one-sample MIS of BSDF sampling and the guide, each chosen with probability 1/2:
           // weight = (sampler weight * sampler density) / mixture density, so the estimator keeps
           // the plain tracer's expectation; the MIS weights of the next hit use the BSDF's pdf.
-          const float* mx = G.mix + (size_t)slot * kGuideOut;
+          const float* mx = G.mix + (size_t)slot * kGuideMixR;
           const float us = G.sample ? sample1d(rng, kGuideSelDim + depth) : 0.f;
           f3 wl;
           if (us < 0.5f) {
             ok = bsdf_sample<MT>(M, wo_l, bx, by, bs);
             wl = bs.wo;
           } else {
-            wl = to_local(sf.sh, mix_sample(mx, (us - 0.5f) * 2.f, bx, by));
+            wl = to_local(sf.sh, mixr_sample(mx, (us - 0.5f) * 2.f, bx, by));
             ok = true;
           }
           f3 wref;
@@ -764,7 +764,7 @@ __global__ void __launch_bounds__(256) k_shade(DevPool P, const DevScene* scenes
           if (ok) ok = bsdf_density<MT>(M, wo_l, wl, wref, ps, pref, rel);
           const f3 wd = to_world(sf.sh, wl);
           if (ok) {
-            const float pm = G.sample ? 0.5f * ps + 0.5f * mix_pdf(mx, wd) : ps;
+            const float pm = G.sample ? 0.5f * ps + 0.5f * mixr_pdf(mx, wd) : ps;
             if (us < 0.5f) {
               wref = bs.f;
               pref = bs.pdf;
