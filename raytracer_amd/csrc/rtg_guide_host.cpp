@@ -186,7 +186,6 @@ DevGuide guide_infer_phase(rtg_ctx* ctx, GuideState* g, const DevGuide& G, int c
   F.rec = nullptr;
   F.rstate = nullptr;
   F.record = 0;
-  F.sample = 1;
   return F;
 }
 

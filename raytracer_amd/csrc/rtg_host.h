@@ -146,7 +146,8 @@ void guide_set_params(rtg_ctx* ctx, GuideState* g, const float* params);
 DevGuide guide_dev(GuideState* g);
 void guide_prepare_render(rtg_ctx* ctx, GuideState* g, rtg_scene* scene, uint64_t seed, DevGuide& G);
 // The guide of a render's inference phase on a pool of `capacity` slots: the trained network with
-// per-slot buffers of that size, sampling on, recording (and so training) off.
+// per-slot buffers of that size, recording (and so training) off; guided sampling stays as G has it
+// (off only if the render's training never finished its warm-up steps).
 DevGuide guide_infer_phase(rtg_ctx* ctx, GuideState* g, const DevGuide& G, int capacity);
 void guide_train_step(rtg_ctx* ctx, GuideState* g, const DevGuide& G, const float* samples, uint64_t base,
                       uint32_t mask, int n, float lr);
