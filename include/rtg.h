@@ -415,7 +415,8 @@ int rtg_guide_train(rtg_ctx* ctx, const float* params, const float* samples, int
 /* The network of the context's last guided render (RTG_GUIDE_PARAMS floats, layout as above). */
 int rtg_guide_params(rtg_ctx* ctx, float* params_out);
 /* Directions from mixtures (n x 40, as rtg_guide_eval returns) with u (n x 3: lobe choice, two for
- * the lobe) -> dirs (n x 3) and the mixture density at each (n). */
+ * the lobe) -> dirs (n x 3) and the mixture density at each (n), through the layout and the sampling
+ * and density functions guided renders use. */
 int rtg_guide_sample(rtg_ctx* ctx, const float* mix, const float* u, int32_t n, float* dirs, float* pdf);
 
 /* Continuous-batching serving (no reference counterpart; it replaces the reference's job loop,

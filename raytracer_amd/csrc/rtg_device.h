@@ -1120,17 +1120,6 @@ __device__ __forceinline__ bool bsdf_density(const DevMaterial& m, f3 wi, f3 wo,
   return true;
 }
 
-// von Mises-Fisher lobe on the unit sphere: kappa / (2 pi (1 - e^{-2 kappa})) e^{kappa (mu.w - 1)}
-__device__ __forceinline__ float vmf_pdf(f3 mu, float kappa, f3 w) {
-  return kappa / (2.f * PI * (-expm1f(-2.f * kappa))) * expf(kappa * (dot(mu, w) - 1.f));
-}
-// mixture given as 8 x (pi, mu xyz, kappa)
-__device__ __forceinline__ float mix_pdf(const float* mx, f3 w) {
-  float p = 0.f;
-#pragma unroll
-  for (int k = 0; k < 8; k++) p += mx[5 * k] * vmf_pdf(mk(mx[5 * k + 1], mx[5 * k + 2], mx[5 * k + 3]), mx[5 * k + 4], w);
-  return p;
-}
 // A direction from the vMF lobe (mu, kappa) with (u1, u2), by inverting the CDF of cos(theta)
 // (Wenzel Jakob, "Numerically stable sampling of the von Mises Fisher distribution on S2", 2012).
 __device__ __forceinline__ f3 vmf_sample(f3 mu, float kappa, float u1, float u2) {
@@ -1154,16 +1143,27 @@ __device__ __forceinline__ int mix_select(Pi pi, float ul) {
   }
   return sel;
 }
-// Sample a lobe with ul, then its vMF with (u1, u2).
-__device__ __forceinline__ f3 mix_sample(const float* mx, float ul, float u1, float u2) {
-  const int sel = mix_select([&](int i) { return mx[5 * i]; }, ul);
-  return vmf_sample(mk(mx[5 * sel + 1], mx[5 * sel + 2], mx[5 * sel + 3]), mx[5 * sel + 4], u1, u2);
-}
-
-// The renderer's per-vertex layout of the same mixture (kGuideMixR floats, 16-byte aligned): pi[8],
+// Mixtures of 8 von Mises-Fisher lobes, density sum_k pi_k kappa_k / (2 pi (1 - e^{-2 kappa_k}))
+// e^{kappa_k (mu_k.w - 1)}. The network gives (pi, mu xyz, kappa) per lobe; the renderer keeps each
+// guided vertex's mixture in this layout (kGuideMixR floats, 16-byte aligned): pi[8],
 // c[8] = pi * kappa / (2 pi (1 - e^{-2 kappa})), mu[8][3], kappa[8]. The normalisers are formed once
 // when the network writes the mixture, so a density costs one exponential per lobe, and the fields
-// load as whole float4s.
+// load as whole float4s. mixr_store_half writes lobes 4h..4h+3 (m: their (pi, mu xyz, kappa)).
+__device__ __forceinline__ void mixr_store_half(float* mr, int h, const float m[20]) {
+  float4* d4 = reinterpret_cast<float4*>(mr);
+  float c[4];
+#pragma unroll
+  for (int j = 0; j < 4; j++) {
+    const float ka = m[5 * j + 4];
+    c[j] = m[5 * j] * (ka / (2.f * PI * (-expm1f(-2.f * ka))));
+  }
+  d4[h] = make_float4(m[0], m[5], m[10], m[15]);
+  d4[2 + h] = make_float4(c[0], c[1], c[2], c[3]);
+  d4[4 + 3 * h] = make_float4(m[1], m[2], m[3], m[6]);
+  d4[5 + 3 * h] = make_float4(m[7], m[8], m[11], m[12]);
+  d4[6 + 3 * h] = make_float4(m[13], m[16], m[17], m[18]);
+  d4[10 + h] = make_float4(m[4], m[9], m[14], m[19]);
+}
 __device__ __forceinline__ float mixr_pdf(const float* mx, f3 w) {
   const float4* m4 = reinterpret_cast<const float4*>(mx);
   const float4 c0 = m4[2], c1 = m4[3];
@@ -1184,6 +1184,7 @@ __device__ __forceinline__ float mixr_pdf(const float* mx, f3 w) {
   }
   return p;
 }
+// Sample a lobe with ul, then its vMF with (u1, u2).
 __device__ __forceinline__ f3 mixr_sample(const float* mx, float ul, float u1, float u2) {
   const float4* m4 = reinterpret_cast<const float4*>(mx);
   const float4 p0 = m4[0], p1 = m4[1];

@@ -336,19 +336,7 @@ __global__ void __launch_bounds__(64 * kInferWaves) k_guide_infer(DevGuide G, De
       for (int j = 0; j < kGuideLobes / 2; j++) mixture_lobe(o, (kGuideLobes / 2) * h + j, amax, z, m + 5 * j);
       if (MODE == 0) {
         // the renderer's layout (kGuideMixR: pi, pi x normaliser, mu, kappa), this half's lobes
-        float4* d4 = reinterpret_cast<float4*>(G.mix + (size_t)slot * kGuideMixR);
-        float c[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const float ka = m[5 * j + 4];
-          c[j] = m[5 * j] * (ka / (2.f * PI * (-expm1f(-2.f * ka))));
-        }
-        d4[h] = make_float4(m[0], m[5], m[10], m[15]);
-        d4[2 + h] = make_float4(c[0], c[1], c[2], c[3]);
-        d4[4 + 3 * h] = make_float4(m[1], m[2], m[3], m[6]);
-        d4[5 + 3 * h] = make_float4(m[7], m[8], m[11], m[12]);
-        d4[6 + 3 * h] = make_float4(m[13], m[16], m[17], m[18]);
-        d4[10 + h] = make_float4(m[4], m[9], m[14], m[19]);
+        mixr_store_half(G.mix + (size_t)slot * kGuideMixR, h, m);
       } else {
         // the parameter layout (pi, mu xyz, kappa) x 8: floats 20h..20h+19 of the row
         float4* d4 = reinterpret_cast<float4*>(out + (size_t)e * kGuideOut + (kGuideOut / 2) * h);

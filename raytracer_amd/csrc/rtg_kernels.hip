@@ -1716,15 +1716,20 @@ void launch_bsdf_tap(const DevMaterial* mat, int op, const float* wi, const floa
                      rel_eta);
 }
 
-// Mixture sampling tap: directions from (pi, mu, kappa) x 8 with u = (lobe, u1, u2), and the
-// mixture density at each sampled direction.
+// Mixture sampling tap: the (pi, mu, kappa) x 8 mixture put into the renderer's layout as the
+// network kernel stores it, then directions with u = (lobe, u1, u2) and the mixture density at each
+// sampled direction, by the functions guided shading calls.
 __global__ void k_guide_sample_tap(const float* mix, const float* u, int n, float* dirs, float* pdf) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const float* mx = mix + (size_t)kGuideOut * i;
-  f3 d = mix_sample(mx, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
+  float4 mr4[kGuideMixR / 4];
+  float* mr = reinterpret_cast<float*>(mr4);
+  mixr_store_half(mr, 0, mx);
+  mixr_store_half(mr, 1, mx + kGuideOut / 2);
+  f3 d = mixr_sample(mr, u[3 * i], u[3 * i + 1], u[3 * i + 2]);
   dirs[3 * i] = d.x; dirs[3 * i + 1] = d.y; dirs[3 * i + 2] = d.z;
-  pdf[i] = mix_pdf(mx, d);
+  pdf[i] = mixr_pdf(mr, d);
 }
 
 void launch_guide_sample_tap(const float* mix, const float* u, int n, float* dirs, float* pdf, hipStream_t stream) {
