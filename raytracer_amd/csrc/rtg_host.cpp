@@ -1140,20 +1140,11 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
   // steps); then one-sample MIS with the guide, with one step per kGuideBatch new training samples
   // of the iteration batch (at least one, at most kMaxStepsPerBatch) until kMaxSteps, after which
   // training and the recording that feeds it stop.
-#ifndef RTG_GUIDE_MAX_STEPS
-#define RTG_GUIDE_MAX_STEPS 128
-#endif
-#ifndef RTG_GUIDE_STEPS_PER_BATCH
-#define RTG_GUIDE_STEPS_PER_BATCH 16
-#endif
-#ifndef RTG_GUIDE_PHASE_DIV
-#define RTG_GUIDE_PHASE_DIV 16
-#endif
-  const uint64_t kWarmSteps = 64, kMaxSteps = RTG_GUIDE_MAX_STEPS;
-  const int kWarmPerBatch = RTG_GUIDE_STEPS_PER_BATCH;
+  const uint64_t kWarmSteps = 64, kMaxSteps = kGuideMaxSteps;
+  const int kWarmPerBatch = kGuideStepsPerBatch;
   const int kMaxStepsPerBatch = kWarmPerBatch;
   // On a context whose pool is larger than the guided cap, the network trains in a first phase on
-  // the capped pool; once training ends (or 1/RTG_GUIDE_PHASE_DIV of the samples is claimed) that
+  // the capped pool; once training ends (or 1/kGuidePhaseDiv of the samples is claimed) that
   // pool stops claiming and drains, and the rest of the render runs on the full pool with the
   // trained network, inference only. Phase: 0 single pool, 1 training, 2 draining, 3 inference.
   int phase = (guided && !J.gather && ctx->pool_slots > pool_cap) ? 1 : 0;
@@ -1270,7 +1261,7 @@ static void render_impl(rtg_ctx* ctx, rtg_scene* scene, const rtg_render_desc* d
         if (train_steps >= kWarmSteps) G.sample = 1;
         if (train_steps >= kMaxSteps) G.record = 0;
       }
-      if (phase == 1 && (!G.record || *next_h >= J.total / RTG_GUIDE_PHASE_DIV) && *next_h < J.total) {
+      if (phase == 1 && (!G.record || *next_h >= J.total / kGuidePhaseDiv) && *next_h < J.total) {
         // end of the training phase: no further claims on the capped pool (no kernel runs now, so
         // the claim counter read back is current)
         cut = *next_h;

@@ -175,6 +175,21 @@ inline void cdf_guide(const float* c, int n, std::vector<int32_t>& out) {
 #define RTG_GUIDED_POOL_LOG2 21
 #endif
 static const int kGuidedPoolMax = 1 << RTG_GUIDED_POOL_LOG2;
+// Guided-render training schedule: at most kGuideMaxSteps Adam steps, at most kGuideStepsPerBatch
+// per batch of iterations; on a context with a larger pool the training phase ends after those
+// steps or 1/kGuidePhaseDiv of the render's samples, whichever comes first.
+#ifndef RTG_GUIDE_MAX_STEPS
+#define RTG_GUIDE_MAX_STEPS 128
+#endif
+#ifndef RTG_GUIDE_STEPS_PER_BATCH
+#define RTG_GUIDE_STEPS_PER_BATCH 16
+#endif
+#ifndef RTG_GUIDE_PHASE_DIV
+#define RTG_GUIDE_PHASE_DIV 16
+#endif
+static const uint64_t kGuideMaxSteps = RTG_GUIDE_MAX_STEPS;
+static const int kGuideStepsPerBatch = RTG_GUIDE_STEPS_PER_BATCH;
+static const uint64_t kGuidePhaseDiv = RTG_GUIDE_PHASE_DIV;
 // lights per scene the all-lights strategy supports (shadow rays per vertex)
 static const int kAllLights = 8;
 static const uint32_t kAllLightDim = 1u << 22;  // 2D draws of lights 1.. at depth d: + d * kAllLights + light
