@@ -906,6 +906,8 @@ int rtg_ctx_create(int device, rtg_ctx** out) {
     if (ctx->arch.rfind("gfx950", 0) != 0) fail(RTG_ERR_INVALID, "librtg is built for gfx950, device is " + ctx->arch);
     hip_check(hipMalloc(&ctx->scenes_dev, sizeof(DevScene) * kMaxScenes), "hipMalloc scenes");
     hip_check(hipMalloc(&ctx->jobs_dev, sizeof(DevJob) * kMaxJobs), "hipMalloc jobs");
+    // every job-table entry starts inactive with nothing claimed (a render uploads only its own)
+    launch_fill_zero(ctx->jobs_dev, sizeof(DevJob) * kMaxJobs, ctx->stream);
     hip_check(hipMalloc(&ctx->order_dev, sizeof(int32_t) * (kMaxJobs + 1)), "hipMalloc job order");
     hip_check(hipMalloc(&ctx->lut_dev, sizeof(float) * kFilterLut * kMaxJobs), "hipMalloc filter tables");
     hip_check(hipHostMalloc(&ctx->pinned, 4096, hipHostMallocDefault), "hipHostMalloc");

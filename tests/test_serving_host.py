@@ -60,3 +60,40 @@ def test_serving_host_logic_under_heap_checks():
     print(r.stdout[-2000:])
     assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-3000:])
     assert "ALL OK" in r.stdout
+
+
+GUIDED_RUNNER = r"""
+import sys
+import numpy as np
+sys.path.insert(0, {root!r})
+from raytracer_amd import rtg, scenes
+rtg.LIB_PATH = {lib!r}
+ctx = rtg.Context(0)
+ctx.set_pool(1 << 22)  # larger than the 2^21-slot cap guided renders train on
+sd = scenes.cornell(width=64, height=64)
+g = ctx.upload(sd)
+spp = 1024  # 4.2 M samples: the training phase claims part of them on the capped pool
+out, st = g.render(sd.camera, dict(sd.render, spp=spp, seed=3, guiding=1))
+n = 64 * 64 * spp
+assert st.samples == n, st.samples
+# the stand-in adds (1, sample index, 0.5, 1) to pixel s mod (64*64) for every finished sample s
+assert np.array_equal(out[..., 3].ravel(), np.full(64 * 64, float(spp), np.float32)), "sample counts per pixel"
+assert np.array_equal(out[..., 1].ravel(), np.full(64 * 64, spp * (spp - 1) / 2.0, np.float32)), "sample sets"
+ctx.close()
+print("ALL OK", flush=True)
+"""
+
+
+def test_guided_render_pool_switch_host_logic():
+    """A learned-guiding render on a context whose pool is larger than the guided cap trains on the
+    capped pool, stops its claims there (the job's claim limit set to the claim counter), drains it
+    and renders the rest on the full pool. Through the HIP stand-in, which finishes claimed samples
+    and records each one in its pixel, every sample index must be rendered exactly once across the
+    switch: per pixel exactly spp samples whose indices sum to spp (spp - 1) / 2."""
+    lib = build_mock_lib()
+    code = GUIDED_RUNNER.format(root=ROOT, lib=lib)
+    env = dict(os.environ, MALLOC_CHECK_="3")
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, env=env, timeout=600)
+    print(r.stdout[-2000:])
+    assert r.returncode == 0, (r.returncode, r.stdout[-2000:], r.stderr[-3000:])
+    assert "ALL OK" in r.stdout
