@@ -57,8 +57,8 @@ enum { RTG_TONEMAP_CLAMP = 0, RTG_TONEMAP_GAMMA = 1, RTG_TONEMAP_REINHARD = 2 };
 /* integrator/PathTracingMIS.cpp, integrator/VolPathTracing.cpp */
 enum { RTG_INTEGRATOR_PATH_MIS = 0, RTG_INTEGRATOR_VOLPATH = 1 };
 /* rtg_render_desc.guiding (no reference counterpart): RTG_GUIDING_OFF is the reference's estimator;
- * RTG_GUIDING_LEARNED adds the online-trained vMF-mixture guide at camera vertices and the
- * rough-dielectric exit proposals; RTG_GUIDING_EXIT the exit proposals alone. Both guided modes
+ * RTG_GUIDING_LEARNED adds the online-trained vMF-mixture guide at diffuse vertices of the first
+ * three path depths and the rough-dielectric exit proposals; RTG_GUIDING_EXIT the exit proposals alone. Both guided modes
  * keep the plain estimator's expectation. */
 enum { RTG_GUIDING_OFF = 0, RTG_GUIDING_LEARNED = 1, RTG_GUIDING_EXIT = 2 };
 
@@ -176,9 +176,12 @@ typedef struct rtg_render_desc {
   uint64_t seed;
   int32_t guiding;             /* RTG_GUIDING_* (no reference counterpart). OFF: plain MIS path
                                   tracing. LEARNED: a vMF-mixture network trained online on this
-                                  render's own paths guides diffuse and rough-conductor camera
-                                  vertices (one-sample MIS with BSDF sampling; rtg_render /
-                                  rtg_render_device only), plus EXIT's proposals. EXIT: at
+                                  render's own paths guides diffuse vertices at depths 0-2
+                                  (one-sample MIS with BSDF sampling; rtg_render /
+                                  rtg_render_device only), plus EXIT's proposals. It trains on at
+                                  most 2^21 path slots (128 steps or 1/16 of the samples); on a
+                                  context with a larger pool the rest of the render then runs on
+                                  that pool with the trained network. EXIT: at
                                   rough-dielectric vertices on the denser side, one-sample MIS of
                                   the sampler with two proposals for the normals near the critical
                                   angle and near grazing, where the sampler's weights are unbounded
@@ -226,7 +229,7 @@ typedef struct rtg_stats {
   uint64_t guide_samples;      /* training samples the render produced */
   double guide_loss;           /* loss of the last training step */
   double guide_train_ms;       /* device time of the training steps (RTG_PROFILE_EVENTS) */
-  uint64_t guide_queries;      /* network inferences (guided camera vertices) */
+  uint64_t guide_queries;      /* network inferences (guided vertices) */
   uint64_t stack_spills;       /* traversal-stack pushes past the 12 on-chip (LDS) entries into the
                                   global spill area (RTG_PROFILE_COUNT) */
 } rtg_stats;
